@@ -1,0 +1,875 @@
+// C ABI of the gfx950 fermion-inversion kernels (declared in include/cl2qcd.h).
+// Argument validation happens here, on the host, before any launch: shapes,
+// parities, formats and pointer aliasing that a kernel cannot survive are
+// rejected with CL2_E_ARG.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cl2_internal.hpp"
+
+using namespace cl2;
+
+namespace {
+
+std::string g_create_error;
+std::mutex g_create_mutex;
+
+int fail(cl2_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return code;
+}
+
+#define CL2_HIP(ctx, call)                                                                         \
+    do {                                                                                           \
+        hipError_t e_ = (call);                                                                    \
+        if (e_ != hipSuccess) return fail(ctx, CL2_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CL2_REQUIRE(ctx, cond, msg) \
+    do {                            \
+        if (!(cond)) return fail(ctx, CL2_E_ARG, msg); \
+    } while (0)
+
+size_t spinor_bytes(const cl2_ctx* c, int prec)
+{
+    return (size_t)12 * c->g.veo * (prec == CL2_F64 ? 16 : 8);
+}
+size_t su3vec_bytes(const cl2_ctx* c, int prec)
+{
+    return (size_t)3 * c->g.veo * (prec == CL2_F64 ? 16 : 8);
+}
+size_t gauge_bytes(const cl2_ctx* c, int fmt)
+{
+    const size_t links = (size_t)8 * c->g.veo;  // 4 dirs x 2 parities x Veo
+    switch (fmt) {
+        case CL2_GAUGE_F64_18: return links * 9 * 16;
+        case CL2_GAUGE_F64_12: return links * 6 * 16;
+        case CL2_GAUGE_F32_12: return links * 3 * 16;
+        default: return 0;
+    }
+}
+bool valid_prec(int p)
+{
+    return p == CL2_F64 || p == CL2_F32;
+}
+bool valid_fmt(int f)
+{
+    return f == CL2_GAUGE_F64_18 || f == CL2_GAUGE_F64_12 || f == CL2_GAUGE_F32_12;
+}
+int fmt_prec(int f)
+{
+    return f == CL2_GAUGE_F32_12 ? CL2_F32 : CL2_F64;
+}
+
+inline int site_parity(const cl2_ctx* c, size_t s)
+{
+    const size_t ns = c->g.ns;
+    size_t x = s % ns, r = s / ns;
+    size_t y = r % ns;
+    r /= ns;
+    size_t z = r % ns, t = r / ns;
+    return (int)((x + y + z + t) & 1);
+}
+
+cl2_scalar value(double re, double im = 0.)
+{
+    cl2_scalar s;
+    s.kind  = CL2_SCALAR_VALUE;
+    s.re    = re;
+    s.im    = im;
+    s.dev   = nullptr;
+    s.index = 0;
+    return s;
+}
+
+int check_scalar(cl2_ctx* c, const cl2_scalar* s)
+{
+    CL2_REQUIRE(c, s != nullptr, "scalar argument is null");
+    CL2_REQUIRE(c, s->kind >= CL2_SCALAR_VALUE && s->kind <= CL2_SCALAR_DEV_VECTOR, "unknown scalar kind");
+    CL2_REQUIRE(c, s->kind == CL2_SCALAR_VALUE || s->dev != nullptr, "device scalar pointer is null");
+    CL2_REQUIRE(c, s->kind != CL2_SCALAR_DEV_VECTOR || s->index >= 0, "negative vector index");
+    return CL2_OK;
+}
+
+/* elementwise BLAS dispatch shared by the Wilson and staggered entry points */
+int run_blas(cl2_ctx* c, int mode, size_t n, void* out, const void* x, const void* y, const void* z, int prec,
+             const cl2_scalar& a, const cl2_scalar& b, double* norm_res, size_t tail)
+{
+    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    CL2_REQUIRE(c, out && x, "null field pointer");
+    BlasArgs args;
+    args.n             = n;
+    args.tail          = tail;
+    args.out           = out;
+    args.x             = x;
+    args.y             = y;
+    args.z             = z;
+    args.a             = a;
+    args.b             = b;
+    args.mode          = mode;
+    args.norm_partials = norm_res ? c->partials : nullptr;
+    unsigned nb        = 0;
+    CL2_HIP(c, launch_blas(c, args, prec, &nb));
+    if (norm_res) CL2_HIP(c, launch_reduce_stage2(c, nb, norm_res, 1, nullptr, 0));
+    return CL2_OK;
+}
+
+int run_reduce(cl2_ctx* c, double* res, const void* x, const void* y, size_t n, int prec, int mode, int out_kind)
+{
+    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    CL2_REQUIRE(c, res && x && (mode == R_NORM || y), "null pointer");
+    unsigned nb = 0;
+    CL2_HIP(c, launch_reduce_stage1(c, x, y, n, prec, mode, &nb));
+    CL2_HIP(c, launch_reduce_stage2(c, nb, res, out_kind, nullptr, 0));
+    return CL2_OK;
+}
+
+int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void* gauge, int fmt, int evenodd,
+               double kappa, int epilogue, const double f1[2], const double f2[2], double scale, int gamma5)
+{
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, evenodd == CL2_EVEN || evenodd == CL2_ODD, "evenodd must be CL2_EVEN or CL2_ODD");
+    CL2_REQUIRE(c, out && in && gauge, "null field pointer");
+    CL2_REQUIRE(c, out != in, "dslash cannot run in place: out aliases in");
+    CL2_REQUIRE(c, epilogue != EPI_AEE || x != nullptr, "null diagonal-term field");
+    CL2_REQUIRE(c, epilogue != EPI_AEE || x != in, "diagonal-term field must differ from the hopping input");
+    DslashArgs a;
+    a.in            = in;
+    a.out           = out;
+    a.gauge         = gauge;
+    a.x             = x;
+    a.g             = c->g;
+    a.out_parity    = (evenodd == CL2_ODD) ? 0 : 1;  // ODD -> even output sites
+    a.epilogue      = epilogue;
+    a.gamma5        = gamma5;
+    a.f1[0]         = f1[0];
+    a.f1[1]         = f1[1];
+    a.f2[0]         = f2[0];
+    a.f2[1]         = f2[1];
+    a.scale         = scale;
+    a.norm_partials = nullptr;
+    a.w             = wilson_weights(c, kappa);
+    CL2_HIP(c, launch_dslash(c, a, fmt));
+    return CL2_OK;
+}
+
+/* factor pairs (spins 0,1 ; spins 2,3) of the twisted-mass site-diagonal terms */
+void tm_factors(int variant, double mubar, double f1[2], double f2[2], double* scale)
+{
+    *scale = 1.;
+    switch (variant) {
+        case CL2_MTM_SITEDIAGONAL:
+            f1[0] = 1., f1[1] = mubar, f2[0] = 1., f2[1] = -mubar;
+            break;
+        case CL2_MTM_INVERSE_SITEDIAGONAL:
+            f1[0] = 1., f1[1] = -mubar, f2[0] = 1., f2[1] = mubar;
+            *scale = 1. / (1. + mubar * mubar);
+            break;
+        case CL2_MTM_SITEDIAGONAL_MINUS:
+            f1[0] = 1., f1[1] = -mubar, f2[0] = 1., f2[1] = mubar;
+            break;
+        case CL2_MTM_INVERSE_SITEDIAGONAL_MINUS:
+            f1[0] = 1., f1[1] = mubar, f2[0] = 1., f2[1] = -mubar;
+            *scale = 1. / (1. + mubar * mubar);
+            break;
+        case CL2_MTM_SITEDIAGONAL_AND_GAMMA5:
+            f1[0] = 1., f1[1] = mubar, f2[0] = -1., f2[1] = mubar;
+            break;
+        default:  // CL2_MTM_SITEDIAGONAL_MINUS_AND_GAMMA5
+            f1[0] = 1., f1[1] = -mubar, f2[0] = -1., f2[1] = -mubar;
+            break;
+    }
+}
+
+/* Aee family with two fused kernels: tmp = M^-1 D in (Wilson: D in), then
+ * out = M_site in - D tmp [gamma5]   (fermionmatrix.cpp:224-591). */
+int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge, int fmt, int action, double kappa,
+            double mubar, bool minus, bool gamma5)
+{
+    double f1[2], f2[2], s;
+    int rc;
+    if (action == CL2_ACTION_WILSON) {
+        const double one[2] = {1., 0.};
+        rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_PLAIN, one, one, 1., 0);
+        if (rc) return rc;
+        return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, one, one, 1., gamma5 ? 1 : 0);
+    }
+    tm_factors(minus ? CL2_MTM_INVERSE_SITEDIAGONAL_MINUS : CL2_MTM_INVERSE_SITEDIAGONAL, mubar, f1, f2, &s);
+    rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_MINV, f1, f2, s, 0);
+    if (rc) return rc;
+    tm_factors(minus ? CL2_MTM_SITEDIAGONAL_MINUS : CL2_MTM_SITEDIAGONAL, mubar, f1, f2, &s);
+    return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, f1, f2, 1., gamma5 ? 1 : 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cl2_ctx_create(cl2_ctx** out, int device, const cl2_lattice* lat)
+{
+    auto create_fail = [&](int code, const std::string& m) {
+        std::lock_guard<std::mutex> lk(g_create_mutex);
+        g_create_error = m;
+        return code;
+    };
+    if (!out || !lat) return create_fail(CL2_E_ARG, "null argument");
+    *out = nullptr;
+    if (lat->ns < 2 || lat->nt < 2 || (lat->ns & 1) || (lat->nt & 1))
+        return create_fail(CL2_E_ARG, "lattice extents must be even and >= 2");
+    const double vol = (double)lat->ns * lat->ns * lat->ns * lat->nt;
+    if (vol / 2 * 12 >= 4294967296.0) return create_fail(CL2_E_ARG, "lattice too large for 32-bit site indices");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
+        return create_fail(CL2_E_ARG, "no such HIP device");
+    if (hipSetDevice(device) != hipSuccess) return create_fail(CL2_E_HIP, "hipSetDevice failed");
+    cl2_ctx* c = new cl2_ctx();
+    c->device  = device;
+    c->lat     = *lat;
+    c->g.ns    = (uint32_t)lat->ns;
+    c->g.nt    = (uint32_t)lat->nt;
+    c->g.vs    = c->g.ns * c->g.ns * c->g.ns;
+    c->g.veo   = c->g.vs * c->g.nt / 2;
+    const double PI = 3.14159265358979323846;
+    c->bs[0] = std::cos(lat->theta_s * PI / lat->ns);
+    c->bs[1] = std::sin(lat->theta_s * PI / lat->ns);
+    c->bt[0] = std::cos(lat->theta_t * PI / lat->nt);
+    c->bt[1] = std::sin(lat->theta_t * PI / lat->nt);
+    double er = lat->use_cp_re ? std::exp(lat->cp_re) : 1., mer = lat->use_cp_re ? std::exp(-lat->cp_re) : 1.;
+    double cr = lat->use_cp_im ? std::cos(lat->cp_im) : 1., ci = lat->use_cp_im ? std::sin(lat->cp_im) : 0.;
+    c->cp_fwd[0] = er * cr;
+    c->cp_fwd[1] = er * ci;
+    c->cp_bwd[0] = mer * cr;
+    c->cp_bwd[1] = mer * ci;
+    c->npartials = std::max<size_t>(REDUCE_MAX_BLOCKS, blocks_for(c->g.veo, DSLASH_BLOCK));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->partials, c->npartials * 2 * sizeof(double)) != hipSuccess ||
+        hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
+        cl2_ctx_destroy(c);
+        return create_fail(CL2_E_HIP, "context allocation failed");
+    }
+    *out = c;
+    return CL2_OK;
+}
+
+int cl2_ctx_destroy(cl2_ctx* c)
+{
+    if (!c) return CL2_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->partials) (void)hipFree(c->partials);
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CL2_OK;
+}
+
+const char* cl2_last_error(const cl2_ctx* c)
+{
+    if (!c) return g_create_error.c_str();
+    return c->err.c_str();
+}
+
+int cl2_sync(cl2_ctx* c)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_HIP(c, hipStreamSynchronize(c->stream));
+    return CL2_OK;
+}
+
+void* cl2_get_stream(cl2_ctx* c)
+{
+    return c ? (void*)c->stream : nullptr;
+}
+
+long cl2_eo_sites(const cl2_ctx* c)
+{
+    return c ? (long)c->g.veo : 0;
+}
+
+size_t cl2_field_bytes(const cl2_ctx* c, int kind, int prec_or_format)
+{
+    if (!c) return 0;
+    switch (kind) {
+        case CL2_FIELD_SPINOR_EO: return valid_prec(prec_or_format) ? spinor_bytes(c, prec_or_format) : 0;
+        case CL2_FIELD_SU3VEC_EO: return valid_prec(prec_or_format) ? su3vec_bytes(c, prec_or_format) : 0;
+        case CL2_FIELD_GAUGE: return gauge_bytes(c, prec_or_format);
+        default: return 0;
+    }
+}
+
+int cl2_alloc(cl2_ctx* c, void** dev, size_t bytes)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dev != nullptr, "null output pointer");
+    *dev = nullptr;
+    CL2_REQUIRE(c, bytes > 0, "zero-byte allocation");
+    CL2_HIP(c, hipSetDevice(c->device));
+    CL2_HIP(c, hipMalloc(dev, bytes));
+    CL2_HIP(c, hipMemsetAsync(*dev, 0, bytes, c->stream));
+    CL2_HIP(c, hipStreamSynchronize(c->stream));
+    return CL2_OK;
+}
+
+int cl2_free(cl2_ctx* c, void* dev)
+{
+    if (!c) return CL2_E_ARG;
+    if (!dev) return CL2_OK;
+    CL2_HIP(c, hipStreamSynchronize(c->stream));
+    CL2_HIP(c, hipFree(dev));
+    return CL2_OK;
+}
+
+int cl2_copy(cl2_ctx* c, void* dst, const void* src, size_t bytes)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dst && src, "null pointer");
+    if (dst == src || bytes == 0) return CL2_OK;
+    CL2_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return CL2_OK;
+}
+
+static int complex_field_import(cl2_ctx* c, void* dev, const double* host, int prec, int ncomp)
+{
+    CL2_REQUIRE(c, dev && host, "null pointer");
+    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    const size_t S = c->g.veo;
+    if (prec == CL2_F64) {
+        std::vector<double> stage(2 * ncomp * S);
+        for (size_t i = 0; i < S; ++i)
+            for (int k = 0; k < ncomp; ++k) {
+                stage[2 * (k * S + i)]     = host[2 * ncomp * i + 2 * k];
+                stage[2 * (k * S + i) + 1] = host[2 * ncomp * i + 2 * k + 1];
+            }
+        CL2_HIP(c, hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        std::vector<float> stage(2 * ncomp * S);
+        for (size_t i = 0; i < S; ++i)
+            for (int k = 0; k < ncomp; ++k) {
+                stage[2 * (k * S + i)]     = (float)host[2 * ncomp * i + 2 * k];
+                stage[2 * (k * S + i) + 1] = (float)host[2 * ncomp * i + 2 * k + 1];
+            }
+        CL2_HIP(c, hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    return CL2_OK;
+}
+
+static int complex_field_export(cl2_ctx* c, double* host, const void* dev, int prec, int ncomp)
+{
+    CL2_REQUIRE(c, dev && host, "null pointer");
+    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    const size_t S = c->g.veo;
+    if (prec == CL2_F64) {
+        std::vector<double> stage(2 * ncomp * S);
+        CL2_HIP(c, hipMemcpyAsync(stage.data(), dev, stage.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < S; ++i)
+            for (int k = 0; k < ncomp; ++k) {
+                host[2 * ncomp * i + 2 * k]     = stage[2 * (k * S + i)];
+                host[2 * ncomp * i + 2 * k + 1] = stage[2 * (k * S + i) + 1];
+            }
+    } else {
+        std::vector<float> stage(2 * ncomp * S);
+        CL2_HIP(c, hipMemcpyAsync(stage.data(), dev, stage.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+        for (size_t i = 0; i < S; ++i)
+            for (int k = 0; k < ncomp; ++k) {
+                host[2 * ncomp * i + 2 * k]     = stage[2 * (k * S + i)];
+                host[2 * ncomp * i + 2 * k + 1] = stage[2 * (k * S + i) + 1];
+            }
+    }
+    return CL2_OK;
+}
+
+int cl2_spinor_import(cl2_ctx* c, void* dev, const double* host, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return complex_field_import(c, dev, host, prec, 12);
+}
+int cl2_spinor_export(cl2_ctx* c, double* host, const void* dev, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return complex_field_export(c, host, dev, prec, 12);
+}
+int cl2_su3vec_import(cl2_ctx* c, void* dev, const double* host, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return complex_field_import(c, dev, host, prec, 3);
+}
+int cl2_su3vec_export(cl2_ctx* c, double* host, const void* dev, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return complex_field_export(c, host, dev, prec, 3);
+}
+
+int cl2_gauge_import(cl2_ctx* c, void* dev, const double* host, int fmt)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dev && host, "null pointer");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    const size_t S = c->g.veo, V = 2 * S;
+    if (fmt == CL2_GAUGE_F32_12) {
+        std::vector<float> stage(gauge_bytes(c, fmt) / sizeof(float));
+        for (size_t s = 0; s < V; ++s) {
+            const int par = site_parity(c, s);
+            const size_t eo = s / 2;
+            for (int mu = 0; mu < 4; ++mu) {
+                const double* u = host + 18 * (mu + 4 * s);
+                for (int k = 0; k < 3; ++k) {
+                    float* d = &stage[4 * (((size_t)(mu * 2 + par) * 3 + k) * S + eo)];
+                    d[0]     = (float)u[4 * k];
+                    d[1]     = (float)u[4 * k + 1];
+                    d[2]     = (float)u[4 * k + 2];
+                    d[3]     = (float)u[4 * k + 3];
+                }
+            }
+        }
+        CL2_HIP(c, hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    } else {
+        const int lanes = (fmt == CL2_GAUGE_F64_18) ? 9 : 6;
+        std::vector<double> stage(gauge_bytes(c, fmt) / sizeof(double));
+        for (size_t s = 0; s < V; ++s) {
+            const int par = site_parity(c, s);
+            const size_t eo = s / 2;
+            for (int mu = 0; mu < 4; ++mu) {
+                const double* u = host + 18 * (mu + 4 * s);
+                for (int k = 0; k < lanes; ++k) {
+                    double* d = &stage[2 * (((size_t)(mu * 2 + par) * lanes + k) * S + eo)];
+                    d[0]      = u[2 * k];
+                    d[1]      = u[2 * k + 1];
+                }
+            }
+        }
+        CL2_HIP(c, hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    CL2_HIP(c, hipStreamSynchronize(c->stream));
+    return CL2_OK;
+}
+
+int cl2_scalar_store(cl2_ctx* c, double* dev, const double* host, int n)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dev && host && n > 0, "bad scalar store");
+    CL2_HIP(c, hipMemcpyAsync(dev, host, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CL2_HIP(c, hipStreamSynchronize(c->stream));
+    return CL2_OK;
+}
+
+int cl2_scalar_get(cl2_ctx* c, double* host, const double* dev, int n)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dev && host && n > 0, "bad scalar get");
+    CL2_HIP(c, hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CL2_HIP(c, hipStreamSynchronize(c->stream));
+    return CL2_OK;
+}
+
+/* ---------------- Wilson ---------------- */
+
+int cl2_dslash_eo(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, int evenodd, double kappa)
+{
+    if (!c) return CL2_E_ARG;
+    const double one[2] = {1., 0.};
+    return run_dslash(c, out, in, nullptr, gauge, fmt, evenodd, kappa, EPI_PLAIN, one, one, 1., 0);
+}
+
+int cl2_dslash_AND_M_tm_inverse_sitediagonal_eo(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt,
+                                                int evenodd, double kappa, double mubar, int minus)
+{
+    if (!c) return CL2_E_ARG;
+    double f1[2], f2[2], s;
+    tm_factors(minus ? CL2_MTM_INVERSE_SITEDIAGONAL_MINUS : CL2_MTM_INVERSE_SITEDIAGONAL, mubar, f1, f2, &s);
+    return run_dslash(c, out, in, nullptr, gauge, fmt, evenodd, kappa, EPI_MINV, f1, f2, s, 0);
+}
+
+int cl2_dslash_AND_sitediagonal_saxpy_eo(cl2_ctx* c, void* out, const void* in, const void* x, const void* gauge,
+                                         int fmt, int evenodd, double kappa, double f1_re, double f1_im, double f2_re,
+                                         double f2_im, int gamma5)
+{
+    if (!c) return CL2_E_ARG;
+    const double f1[2] = {f1_re, f1_im}, f2[2] = {f2_re, f2_im};
+    return run_dslash(c, out, in, x, gauge, fmt, evenodd, kappa, EPI_AEE, f1, f2, 1., gamma5 ? 1 : 0);
+}
+
+int cl2_M_tm_sitediagonal_eo(cl2_ctx* c, void* out, const void* in, int prec, double mubar, int variant)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && in, "null field pointer");
+    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    CL2_REQUIRE(c, variant >= CL2_MTM_SITEDIAGONAL && variant <= CL2_MTM_SITEDIAGONAL_MINUS_AND_GAMMA5,
+                "unknown site-diagonal variant");
+    double f1[2], f2[2], s;
+    tm_factors(variant, mubar, f1, f2, &s);
+    CL2_HIP(c, launch_sitediag(c, out, in, prec, f1, f2, s));
+    return CL2_OK;
+}
+
+int cl2_gamma5_eo(cl2_ctx* c, void* inout, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return run_blas(c, B_NEG_TAIL, (size_t)12 * c->g.veo, inout, inout, nullptr, nullptr, prec, value(0.), value(0.),
+                    nullptr, (size_t)6 * c->g.veo);
+}
+
+int cl2_saxpy_AND_gamma5_eo(cl2_ctx* c, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    CL2_REQUIRE(c, y != nullptr, "null field pointer");
+    return run_blas(c, B_G5_Y_MINUS_AX, (size_t)12 * c->g.veo, out, x, y, nullptr, prec, *alpha, value(0.), nullptr,
+                    (size_t)6 * c->g.veo);
+}
+
+int cl2_fermionmatrix_eo(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt, int action,
+                         double kappa, double mubar, void* tmp)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, op >= CL2_OP_AEE && op <= CL2_OP_QPLUSQMINUS_EO, "unknown operator");
+    CL2_REQUIRE(c, action == CL2_ACTION_WILSON || action == CL2_ACTION_TWISTEDMASS,
+                "unknown fermion action: wilson or twistedmass");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, out && in && gauge && tmp, "null field pointer");
+    const size_t fb = spinor_bytes(c, fmt_prec(fmt));
+    void* t0        = tmp;
+    void* t1        = static_cast<char*>(tmp) + fb;
+    CL2_REQUIRE(c, out != t0 && out != t1 && in != t0 && in != t1, "tmp must not alias in or out");
+    switch (op) {
+        case CL2_OP_AEE: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, false);
+        case CL2_OP_AEE_AND_GAMMA5:
+        case CL2_OP_QPLUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, true);
+        case CL2_OP_AEE_MINUS: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, false);
+        case CL2_OP_AEE_MINUS_AND_GAMMA5:
+        case CL2_OP_QMINUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, true);
+        default: {
+            int rc = run_aee(c, t1, in, t0, gauge, fmt, action, kappa, mubar, true, true);
+            if (rc) return rc;
+            return run_aee(c, out, t1, t0, gauge, fmt, action, kappa, mubar, false, true);
+        }
+    }
+}
+
+/* ---------------- Wilson BLAS ---------------- */
+
+int cl2_saxpy_eo(cl2_ctx* c, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    CL2_REQUIRE(c, y != nullptr, "null field pointer");
+    return run_blas(c, B_Y_MINUS_AX, (size_t)12 * c->g.veo, out, x, y, nullptr, prec, *alpha, value(0.), nullptr, 0);
+}
+
+int cl2_sax_eo(cl2_ctx* c, void* out, const void* x, int prec, const cl2_scalar* alpha)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    return run_blas(c, B_AX, (size_t)12 * c->g.veo, out, x, nullptr, nullptr, prec, *alpha, value(0.), nullptr, 0);
+}
+
+int cl2_saxsbypz_eo(cl2_ctx* c, void* out, const void* x, const void* y, const void* z, int prec,
+                    const cl2_scalar* alpha, const cl2_scalar* beta)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    if (int rc = check_scalar(c, beta)) return rc;
+    CL2_REQUIRE(c, y && z, "null field pointer");
+    return run_blas(c, B_AX_BY_Z, (size_t)12 * c->g.veo, out, x, y, z, prec, *alpha, *beta, nullptr, 0);
+}
+
+int cl2_scalar_product_eo(cl2_ctx* c, double* res, const void* x, const void* y, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return run_reduce(c, res, x, y, (size_t)12 * c->g.veo, prec, R_DOT, 1);
+}
+
+int cl2_squarenorm_eo(cl2_ctx* c, double* res, const void* x, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return run_reduce(c, res, x, nullptr, (size_t)12 * c->g.veo, prec, R_NORM, 0);
+}
+
+int cl2_saxpy_AND_squarenorm_eo(cl2_ctx* c, void* out, const void* x, const void* y, int prec,
+                                const cl2_scalar* alpha, double* res)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    CL2_REQUIRE(c, y && res, "null pointer");
+    return run_blas(c, B_Y_MINUS_AX, (size_t)12 * c->g.veo, out, x, y, nullptr, prec, *alpha, value(0.), res, 0);
+}
+
+int cl2_set_zero_eo(cl2_ctx* c, void* out, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
+    CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo, prec, 0., 0.));
+    return CL2_OK;
+}
+
+int cl2_set_cold_eo(cl2_ctx* c, void* out, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
+    const double norm = 1. / std::sqrt((double)(12. * 2. * c->g.veo));
+    CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo, prec, norm, 0.));
+    return CL2_OK;
+}
+
+/* ---------------- staggered ---------------- */
+
+int cl2_D_KS_eo(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, int evenodd)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, evenodd == CL2_EVEN || evenodd == CL2_ODD, "evenodd must be CL2_EVEN or CL2_ODD");
+    CL2_REQUIRE(c, out && in && gauge, "null field pointer");
+    CL2_REQUIRE(c, out != in, "D_KS_eo cannot run in place: out aliases in");
+    CL2_HIP(c, launch_dks(c, out, in, gauge, fmt, evenodd));
+    return CL2_OK;
+}
+
+int cl2_MdagM_eo(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, double mass, int upper_left,
+                 void* tmp)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, tmp && tmp != in && tmp != out, "tmp must be a separate field");
+    CL2_REQUIRE(c, upper_left == CL2_EVEN || upper_left == CL2_ODD, "upper_left must be CL2_EVEN or CL2_ODD");
+    CL2_REQUIRE(c, out != in, "MdagM_eo cannot run in place");
+    const int first = (upper_left == CL2_EVEN) ? CL2_ODD : CL2_EVEN;
+    int rc          = cl2_D_KS_eo(c, tmp, in, gauge, fmt, first);
+    if (rc) return rc;
+    rc = cl2_D_KS_eo(c, out, tmp, gauge, fmt, upper_left);
+    if (rc) return rc;
+    return run_blas(c, B_AX_BY, (size_t)3 * c->g.veo, out, in, out, nullptr, fmt_prec(fmt), value(mass * mass),
+                    value(-1.), nullptr, 0);
+}
+
+int cl2_saxpy_stagg_eo(cl2_ctx* c, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    CL2_REQUIRE(c, y != nullptr, "null field pointer");
+    return run_blas(c, B_AX_PLUS_Y, (size_t)3 * c->g.veo, out, x, y, nullptr, prec, *alpha, value(0.), nullptr, 0);
+}
+
+int cl2_saxpby_stagg_eo(cl2_ctx* c, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha,
+                        const cl2_scalar* beta)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    if (int rc = check_scalar(c, beta)) return rc;
+    CL2_REQUIRE(c, y != nullptr, "null field pointer");
+    return run_blas(c, B_AX_BY, (size_t)3 * c->g.veo, out, x, y, nullptr, prec, *alpha, *beta, nullptr, 0);
+}
+
+int cl2_sax_stagg_eo(cl2_ctx* c, void* out, const void* x, int prec, const cl2_scalar* alpha)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    return run_blas(c, B_AX, (size_t)3 * c->g.veo, out, x, nullptr, nullptr, prec, *alpha, value(0.), nullptr, 0);
+}
+
+int cl2_saxpbypz_stagg_eo(cl2_ctx* c, void* out, const void* x, const void* y, const void* z, int prec,
+                          const cl2_scalar* alpha, const cl2_scalar* beta)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    if (int rc = check_scalar(c, beta)) return rc;
+    CL2_REQUIRE(c, y && z, "null field pointer");
+    return run_blas(c, B_AX_BY_Z, (size_t)3 * c->g.veo, out, x, y, z, prec, *alpha, *beta, nullptr, 0);
+}
+
+int cl2_scalar_product_stagg_eo(cl2_ctx* c, double* res, const void* x, const void* y, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return run_reduce(c, res, x, y, (size_t)3 * c->g.veo, prec, R_DOT, 1);
+}
+
+int cl2_scalar_product_real_part_stagg_eo(cl2_ctx* c, double* res, const void* x, const void* y, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return run_reduce(c, res, x, y, (size_t)3 * c->g.veo, prec, R_DOT_RE, 0);
+}
+
+int cl2_squarenorm_stagg_eo(cl2_ctx* c, double* res, const void* x, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    return run_reduce(c, res, x, nullptr, (size_t)3 * c->g.veo, prec, R_NORM, 0);
+}
+
+int cl2_sax_vec_AND_squarenorm_stagg_eo(cl2_ctx* c, double* res, const double* alpha, int n, const void* x, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, res && alpha && x && n > 0, "bad argument");
+    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    unsigned nb = 0;
+    CL2_HIP(c, launch_reduce_stage1(c, x, nullptr, (size_t)3 * c->g.veo, prec, R_NORM, &nb));
+    CL2_HIP(c, launch_reduce_stage2(c, nb, res, 2, alpha, n));
+    return CL2_OK;
+}
+
+int cl2_set_zero_stagg_eo(cl2_ctx* c, void* out, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
+    CL2_HIP(c, launch_fill(c, out, (size_t)3 * c->g.veo, prec, 0., 0.));
+    return CL2_OK;
+}
+
+int cl2_set_cold_stagg_eo(cl2_ctx* c, void* out, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
+    const double norm = 1. / std::sqrt((double)(3. * 2. * c->g.veo));
+    CL2_HIP(c, launch_fill(c, out, (size_t)3 * c->g.veo, prec, norm, 0.));
+    return CL2_OK;
+}
+
+/* ---------------- device scalars ---------------- */
+
+int cl2_complex_op(cl2_ctx* c, int op, double* out, const double* a, const double* b)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && a && b, "null scalar");
+    CL2_REQUIRE(c, op >= CL2_SCALAR_OP_PRODUCT && op <= CL2_SCALAR_OP_SUBTRACTION, "unknown scalar op");
+    CL2_HIP(c, launch_scalar_op(c, op, 1, out, a, b));
+    return CL2_OK;
+}
+
+int cl2_real_op(cl2_ctx* c, int op, double* out, const double* a, const double* b)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && a && b, "null scalar");
+    CL2_REQUIRE(c, op >= CL2_SCALAR_OP_PRODUCT && op <= CL2_SCALAR_OP_SUBTRACTION, "unknown scalar op");
+    CL2_HIP(c, launch_scalar_op(c, op, 0, out, a, b));
+    return CL2_OK;
+}
+
+int cl2_real_get_vector_element(cl2_ctx* c, double* out, const double* vec, int index)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && vec && index >= 0, "bad argument");
+    CL2_HIP(c, launch_vec_element(c, out, vec, 0, index));
+    return CL2_OK;
+}
+
+int cl2_real_set_vector_element(cl2_ctx* c, double* vec, const double* in, int index)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, in && vec && index >= 0, "bad argument");
+    CL2_HIP(c, launch_vec_element(c, vec, in, index, 0));
+    return CL2_OK;
+}
+
+int cl2_update_zeta_cgm(cl2_ctx* c, double* out, const double* zeta_prev, const double* zeta_prev_prev,
+                        const double* sbeta_prev, const double* sbeta_pres, const double* salpha_prev,
+                        const double* sigma, int numeq)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && zeta_prev && zeta_prev_prev && sbeta_prev && sbeta_pres && salpha_prev && sigma && numeq > 0,
+                "bad argument");
+    CL2_HIP(c, launch_cgm_update(c, 0, out, zeta_prev, zeta_prev_prev, sbeta_prev, sbeta_pres, salpha_prev, sigma,
+                                 numeq));
+    return CL2_OK;
+}
+
+int cl2_update_beta_cgm(cl2_ctx* c, double* out, const double* sbeta_pres, const double* zeta_pres,
+                        const double* zeta_prev, int numeq)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && sbeta_pres && zeta_pres && zeta_prev && numeq > 0, "bad argument");
+    CL2_HIP(c, launch_cgm_update(c, 1, out, sbeta_pres, zeta_pres, zeta_prev, nullptr, nullptr, nullptr, numeq));
+    return CL2_OK;
+}
+
+int cl2_update_alpha_cgm(cl2_ctx* c, double* out, const double* salpha_pres, const double* zeta_pres,
+                         const double* beta_pres, const double* zeta_prev, const double* sbeta_pres, int numeq)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out && salpha_pres && zeta_pres && beta_pres && zeta_prev && sbeta_pres && numeq > 0,
+                "bad argument");
+    CL2_HIP(c, launch_cgm_update(c, 2, out, salpha_pres, zeta_pres, beta_pres, zeta_prev, sbeta_pres, nullptr,
+                                 numeq));
+    return CL2_OK;
+}
+
+/* ---------------- timing ---------------- */
+
+int cl2_timer_start(cl2_ctx* c)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_HIP(c, hipEventRecord(c->ev_start, c->stream));
+    return CL2_OK;
+}
+
+int cl2_timer_stop(cl2_ctx* c, double* ms)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, ms != nullptr, "null output");
+    CL2_HIP(c, hipEventRecord(c->ev_stop, c->stream));
+    CL2_HIP(c, hipEventSynchronize(c->ev_stop));
+    float f = 0.f;
+    CL2_HIP(c, hipEventElapsedTime(&f, c->ev_start, c->ev_stop));
+    *ms = f;
+    return CL2_OK;
+}
+
+/* ---------------- byte / flop models ---------------- */
+
+uint64_t cl2_reference_read_write_size(const cl2_ctx* c, const char* k)
+{
+    if (!c || !k) return 0;
+    const uint64_t Seo = c->g.veo, D = 8, C = 2, R = 9;
+    const std::string n(k);
+    if (n == "dslash_eo" || n == "dslash_AND_M_tm_inverse_sitediagonal_eo" ||
+        n == "dslash_AND_M_tm_inverse_sitediagonal_minus_eo")
+        return (C * 12 * (2 * 4 + 1) + C * 2 * 4 * R) * D * Seo;  // fermions.cpp:860-871
+    if (n == "gamma5_eo" || n.rfind("M_tm_", 0) == 0) return 48 * D * Seo;  // fermions.cpp:834-857,872-877
+    if (n == "saxpy_AND_gamma5_eo") return C * D * Seo * (12 * (2 + 1) + 1);  // fermions.cpp:878-880
+    if (n == "D_KS_eo") return (C * 3 * (2 * 4 + 1) + C * 2 * 4 * R) * D * Seo;  // fermions_staggered.cpp:189-193
+    return 0;
+}
+
+uint64_t cl2_reference_flop_size(const cl2_ctx* c, const char* k)
+{
+    if (!c || !k) return 0;
+    const uint64_t Seo = c->g.veo;
+    const std::string n(k);
+    const uint64_t dslash = 4 * 2 * (66 * 2 + 6 * 3 * 2 + 2 * 3 * 6);  // 1632, fermions.cpp:895-910
+    const uint64_t diag   = 3 * 4 * 6;
+    if (n == "dslash_eo") return Seo * dslash;
+    if (n == "dslash_AND_M_tm_inverse_sitediagonal_eo" || n == "dslash_AND_M_tm_inverse_sitediagonal_minus_eo")
+        return Seo * dslash + Seo * (diag + 3 * 4 * 2);
+    if (n == "M_tm_sitediagonal" || n == "M_tm_sitediagonal_minus") return Seo * diag;
+    if (n == "M_tm_inverse_sitediagonal" || n == "M_tm_inverse_sitediagonal_minus") return Seo * (diag + 24);
+    if (n == "gamma5_eo") return Seo * 12;
+    if (n == "D_KS_eo") return Seo * 570;  // fermions_staggered.cpp:259-263
+    return 0;
+}
+
+uint64_t cl2_algorithmic_bytes(const cl2_ctx* c, const char* k, int fmt)
+{
+    if (!c || !k || !valid_fmt(fmt)) return 0;
+    const uint64_t Seo = c->g.veo;
+    const uint64_t L   = gauge_bytes(c, fmt) / (8 * Seo);
+    const uint64_t S   = (fmt == CL2_GAUGE_F32_12) ? 96 : 192;
+    const uint64_t s   = S / 4;
+    const std::string n(k);
+    if (n == "dslash_eo" || n == "dslash_AND_M_tm_inverse_sitediagonal_eo") return Seo * (8 * L + 2 * S);
+    if (n == "dslash_AND_sitediagonal_saxpy_eo") return Seo * (8 * L + 3 * S);
+    if (n == "D_KS_eo") return Seo * (8 * L + 2 * s);
+    if (n == "M_tm_sitediagonal" || n == "gamma5_eo" || n == "squarenorm") return Seo * 2 * S / (n == "squarenorm" ? 2 : 1);
+    if (n == "saxpy" || n == "saxpy_AND_squarenorm") return Seo * 3 * S;
+    if (n == "scalar_product") return Seo * 2 * S;
+    if (n == "saxsbypz") return Seo * 4 * S;
+    return 0;
+}
+
+}  // extern "C"

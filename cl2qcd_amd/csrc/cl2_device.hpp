@@ -1,0 +1,313 @@
+// Device-side building blocks for the gfx950 kernels: complex and SU(3)
+// arithmetic, eo lattice geometry, gauge-link formats and wavefront/LDS
+// reductions.  All kernels are bandwidth bound; nothing here uses MFMA.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cl2_types.hpp"
+
+namespace cl2 {
+
+template<typename R>
+struct V2;
+template<>
+struct V2<double> {
+    using T = double2;
+};
+template<>
+struct V2<float> {
+    using T = float2;
+};
+template<typename R>
+using vec2 = typename V2<R>::T;
+
+template<typename R>
+struct Cx {
+    R re, im;
+};
+
+template<typename R>
+__device__ __forceinline__ Cx<R> cx(R a, R b)
+{
+    return Cx<R>{a, b};
+}
+template<typename R>
+__device__ __forceinline__ Cx<R> operator+(Cx<R> a, Cx<R> b)
+{
+    return Cx<R>{a.re + b.re, a.im + b.im};
+}
+template<typename R>
+__device__ __forceinline__ Cx<R> operator-(Cx<R> a, Cx<R> b)
+{
+    return Cx<R>{a.re - b.re, a.im - b.im};
+}
+template<typename R>
+__device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b)
+{
+    return Cx<R>{a.re * b.re - a.im * b.im, a.im * b.re + a.re * b.im};
+}
+template<typename R>
+__device__ __forceinline__ Cx<R> cmulconj(Cx<R> a, Cx<R> b)
+{  // a * conj(b)
+    return Cx<R>{a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+template<typename R>
+__device__ __forceinline__ Cx<R> conj(Cx<R> a)
+{
+    return Cx<R>{a.re, -a.im};
+}
+template<typename R>
+__device__ __forceinline__ Cx<R> scale(Cx<R> a, R s)
+{
+    return Cx<R>{a.re * s, a.im * s};
+}
+
+/* Multiplication by one of {+1, -1, +i, -i}: the only factors the Wilson spin
+ * projectors (1 -/+ gamma_mu) of the chiral basis need. */
+enum Unit : int { P1 = 0, M1 = 1, PI = 2, MI = 3 };
+template<int U>
+struct UnitConj;
+template<>
+struct UnitConj<P1> {
+    static constexpr int v = P1;
+};
+template<>
+struct UnitConj<M1> {
+    static constexpr int v = M1;
+};
+template<>
+struct UnitConj<PI> {
+    static constexpr int v = MI;
+};
+template<>
+struct UnitConj<MI> {
+    static constexpr int v = PI;
+};
+template<int U, typename R>
+__device__ __forceinline__ Cx<R> add_unit(Cx<R> a, Cx<R> b)
+{  // a + U*b
+    if constexpr (U == P1) return Cx<R>{a.re + b.re, a.im + b.im};
+    if constexpr (U == M1) return Cx<R>{a.re - b.re, a.im - b.im};
+    if constexpr (U == PI) return Cx<R>{a.re - b.im, a.im + b.re};
+    return Cx<R>{a.re + b.im, a.im - b.re};
+}
+
+template<typename R>
+struct Su3 {
+    Cx<R> e[3][3];
+};
+
+template<typename R>
+__device__ __forceinline__ void su3_times(const Su3<R>& u, const Cx<R> (&v)[3], Cx<R> (&r)[3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        R re = u.e[i][0].re * v[0].re - u.e[i][0].im * v[0].im;
+        R im = u.e[i][0].re * v[0].im + u.e[i][0].im * v[0].re;
+        re += u.e[i][1].re * v[1].re - u.e[i][1].im * v[1].im;
+        im += u.e[i][1].re * v[1].im + u.e[i][1].im * v[1].re;
+        re += u.e[i][2].re * v[2].re - u.e[i][2].im * v[2].im;
+        im += u.e[i][2].re * v[2].im + u.e[i][2].im * v[2].re;
+        r[i] = Cx<R>{re, im};
+    }
+}
+template<typename R>
+__device__ __forceinline__ void su3dag_times(const Su3<R>& u, const Cx<R> (&v)[3], Cx<R> (&r)[3])
+{
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        R re = u.e[0][i].re * v[0].re + u.e[0][i].im * v[0].im;
+        R im = u.e[0][i].re * v[0].im - u.e[0][i].im * v[0].re;
+        re += u.e[1][i].re * v[1].re + u.e[1][i].im * v[1].im;
+        im += u.e[1][i].re * v[1].im - u.e[1][i].im * v[1].re;
+        re += u.e[2][i].re * v[2].re + u.e[2][i].im * v[2].im;
+        im += u.e[2][i].re * v[2].im - u.e[2][i].im * v[2].re;
+        r[i] = Cx<R>{re, im};
+    }
+}
+
+/* Third row of an SU(3) matrix from the first two: conj(row0 x row1).  The
+ * reference's 12-real path (operations_gaugefield.cl:41-46) omits the
+ * conjugate; this is the correct form for det U = 1. */
+template<typename R>
+__device__ __forceinline__ void su3_reconstruct_row2(Su3<R>& u)
+{
+    u.e[2][0] = conj(cmul(u.e[0][1], u.e[1][2]) - cmul(u.e[0][2], u.e[1][1]));
+    u.e[2][1] = conj(cmul(u.e[0][2], u.e[1][0]) - cmul(u.e[0][0], u.e[1][2]));
+    u.e[2][2] = conj(cmul(u.e[0][0], u.e[1][1]) - cmul(u.e[0][1], u.e[1][0]));
+}
+
+/* ---------------- lattice geometry ----------------
+ * Reference site order: site = x + NS*(y + NS*(z + NS*t)), eo index = site/2,
+ * directions T=0,X=1,Y=2,Z=3 (ocl_kernel/operations_geometry.cl:72-300).  A
+ * thread that owns eo index i of parity P sits at the site of parity P among
+ * {2i, 2i+1}; both share y, z, t because NS is even. */
+struct SiteNb {
+    uint32_t up[4];  // eo index of x + mu
+    uint32_t dn[4];  // eo index of x - mu
+};
+
+__device__ __forceinline__ void site_neighbours(const LatticeGeom& g, uint32_t i, int parity, SiteNb& nb)
+{
+    const uint32_t s0 = 2u * i;
+    const uint32_t ns = g.ns;
+    uint32_t r        = s0 / ns;
+    const uint32_t x0 = s0 - r * ns;
+    const uint32_t y  = r % ns;
+    r /= ns;
+    const uint32_t z = r % ns;
+    const uint32_t t = r / ns;
+    const uint32_t x = x0 + (((y + z + t) & 1u) ^ (uint32_t)parity);
+
+    const uint32_t ns2 = ns * ns;
+    const uint32_t vs  = g.vs;
+    const uint32_t base = ns * y + ns2 * z + vs * t;  // site of (0,y,z,t)
+    const uint32_t xu = (x + 1 == ns) ? 0u : x + 1;
+    const uint32_t xd = (x == 0) ? ns - 1 : x - 1;
+    nb.up[1] = (base + xu) >> 1;
+    nb.dn[1] = (base + xd) >> 1;
+    const uint32_t site = base + x;
+    nb.up[2] = (y + 1 == ns ? site - (ns - 1) * ns : site + ns) >> 1;
+    nb.dn[2] = (y == 0 ? site + (ns - 1) * ns : site - ns) >> 1;
+    nb.up[3] = (z + 1 == ns ? site - (ns - 1) * ns2 : site + ns2) >> 1;
+    nb.dn[3] = (z == 0 ? site + (ns - 1) * ns2 : site - ns2) >> 1;
+    nb.up[0] = (t + 1 == g.nt ? site - (g.nt - 1) * vs : site + vs) >> 1;
+    nb.dn[0] = (t == 0 ? site + (g.nt - 1) * vs : site - vs) >> 1;
+}
+
+__device__ __forceinline__ void site_coords(const LatticeGeom& g, uint32_t i, int parity, uint32_t& x, uint32_t& y,
+                                            uint32_t& z, uint32_t& t)
+{
+    const uint32_t s0 = 2u * i;
+    uint32_t r        = s0 / g.ns;
+    const uint32_t x0 = s0 - r * g.ns;
+    y                 = r % g.ns;
+    r /= g.ns;
+    z = r % g.ns;
+    t = r / g.ns;
+    x = x0 + (((y + z + t) & 1u) ^ (uint32_t)parity);
+}
+
+/* Streaming (non-temporal) loads for data read exactly once per launch. */
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ double2 load_stream(const double2* p)
+{
+    nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
+    return make_double2(v.x, v.y);
+}
+__device__ __forceinline__ float4 load_stream(const float4* p)
+{
+    nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
+/* ---------------- gauge-link formats ----------------
+ * Device layout [dir][parity][lane][gstride] where lane = complex entry (18-real)
+ * or pairs of complex entries packed into a 16-byte vector (12-real, FP32). */
+template<int FMT>
+struct GaugeLoad;
+
+template<>
+struct GaugeLoad<GAUGE_F64_18> {
+    using R = double;
+    static constexpr int LANES = 9;
+    __device__ static __forceinline__ void load(const void* g, uint32_t gstride, int dir, int par, uint32_t eo,
+                                                Su3<double>& u)
+    {
+        const double2* p = reinterpret_cast<const double2*>(g) + (size_t)((dir * 2 + par) * 9) * gstride + eo;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            double2 v           = load_stream(p + (size_t)k * gstride);
+            u.e[k / 3][k % 3] = Cx<double>{v.x, v.y};
+        }
+    }
+};
+template<>
+struct GaugeLoad<GAUGE_F64_12> {
+    using R = double;
+    static constexpr int LANES = 6;
+    __device__ static __forceinline__ void load(const void* g, uint32_t gstride, int dir, int par, uint32_t eo,
+                                                Su3<double>& u)
+    {
+        const double2* p = reinterpret_cast<const double2*>(g) + (size_t)((dir * 2 + par) * 6) * gstride + eo;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            double2 v           = load_stream(p + (size_t)k * gstride);
+            u.e[k / 3][k % 3] = Cx<double>{v.x, v.y};
+        }
+        su3_reconstruct_row2(u);
+    }
+};
+template<>
+struct GaugeLoad<GAUGE_F32_12> {
+    using R = float;
+    static constexpr int LANES = 3;  // 3 float4 = 6 complex
+    __device__ static __forceinline__ void load(const void* g, uint32_t gstride, int dir, int par, uint32_t eo,
+                                                Su3<float>& u)
+    {
+        const float4* p = reinterpret_cast<const float4*>(g) + (size_t)((dir * 2 + par) * 3) * gstride + eo;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float4 v                      = load_stream(p + (size_t)k * gstride);
+            u.e[(2 * k) / 3][(2 * k) % 3] = Cx<float>{v.x, v.y};
+            u.e[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<float>{v.z, v.w};
+        }
+        su3_reconstruct_row2(u);
+    }
+};
+
+/* ---------------- reductions ----------------
+ * Block partial sums with a fixed combination order (wavefront butterfly, then
+ * LDS across the block's waves), so results are bit-reproducible for a given
+ * launch shape. */
+template<typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template<int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* lds)
+{
+    v              = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double r = 0.;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) r += lds[w];
+    }
+    __syncthreads();
+    return r;  // valid in thread 0
+}
+
+template<int BLOCK>
+__device__ __forceinline__ double2 block_sum2(double2 v, double* lds)
+{
+    v.x            = wave_sum(v.x);
+    v.y            = wave_sum(v.y);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[2 * wid]     = v.x;
+        lds[2 * wid + 1] = v.y;
+    }
+    __syncthreads();
+    double2 r = make_double2(0., 0.);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int w = 0; w < BLOCK / 64; ++w) {
+            r.x += lds[2 * w];
+            r.y += lds[2 * w + 1];
+        }
+    }
+    __syncthreads();
+    return r;
+}
+
+}  // namespace cl2

@@ -1,0 +1,82 @@
+// Host-side internals shared by the kernel launchers and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "cl2_types.hpp"
+
+struct cl2_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cl2_lattice lat{};
+    cl2::LatticeGeom g{};
+    double bs[2] = {1., 0.}, bt[2] = {1., 0.};      // boundary phases exp(i pi theta / N)
+    double cp_fwd[2] = {1., 0.}, cp_bwd[2] = {1., 0.};  // factors on forward / backward T links
+    double* partials = nullptr;  // reduction scratch: one double2 per block
+    size_t npartials = 0;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    std::string err;
+};
+
+namespace cl2 {
+
+constexpr int DSLASH_BLOCK = 256;
+constexpr int BLAS_BLOCK   = 256;
+constexpr int REDUCE_MAX_BLOCKS = 2048;
+
+inline unsigned blocks_for(size_t n, int block)
+{
+    return (unsigned)((n + block - 1) / block);
+}
+
+/* hopping weights -kappa * b_mu * c_mu (forward) and -kappa * conj(b_mu * c_mu') (backward) */
+HopWeights<double> wilson_weights(const cl2_ctx* c, double kappa);
+
+/* launchers (return hipSuccess or the launch error) */
+hipError_t launch_dslash(cl2_ctx* c, const DslashArgs& a, int format);
+hipError_t launch_sitediag(cl2_ctx* c, void* out, const void* in, int prec, const double f1[2], const double f2[2],
+                           double scale);
+hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd);
+
+/* elementwise complex BLAS over n complex numbers */
+enum BlasMode : int {
+    B_Y_MINUS_AX = 0,  // y - a x          (Wilson saxpy)
+    B_AX         = 1,  // a x
+    B_AX_BY_Z    = 2,  // b y + a x + z     (Wilson saxsbypz, staggered saxpbypz)
+    B_AX_PLUS_Y  = 3,  // a x + y          (staggered saxpy)
+    B_AX_BY      = 4,  // a x + b y        (staggered saxpby)
+    B_G5_Y_MINUS_AX = 5,  // gamma5 (y - a x) on a Wilson spinor
+    B_NEG_TAIL   = 6,  // gamma5 in place: negate complex elements [tail, n)
+};
+struct BlasArgs {
+    size_t n;     // complex elements
+    size_t tail;  // first element with a negative gamma5 sign (Wilson: 6 * Veo)
+    void* out;
+    const void* x;
+    const void* y;
+    const void* z;
+    cl2_scalar a, b;
+    int mode;
+    double* norm_partials;  // if set: per-block sum |out|^2
+};
+hipError_t launch_blas(cl2_ctx* c, const BlasArgs& a, int prec, unsigned* nblocks_out);
+
+enum ReduceMode : int { R_DOT = 0, R_DOT_RE = 1, R_NORM = 2 };
+/* stage 1 of a two-stage reduction; returns the number of partials written */
+hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t n, int prec, int mode,
+                                unsigned* nblocks);
+/* stage 2: sum nparts double2 partials in fixed order.  out_kind 0: res[0] = real part;
+ * 1: res[0..1] = complex sum;  2: res[k] = alpha_vec[k]^2 * real part for k < nvec */
+hipError_t launch_reduce_stage2(cl2_ctx* c, unsigned nparts, double* res, int out_kind, const double* alpha_vec,
+                                int nvec);
+
+hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im);
+
+hipError_t launch_scalar_op(cl2_ctx* c, int op, int is_complex, double* out, const double* a, const double* b);
+hipError_t launch_vec_element(cl2_ctx* c, double* out, const double* in, int out_index, int in_index);
+hipError_t launch_cgm_update(cl2_ctx* c, int which, double* out, const double* a, const double* b, const double* d,
+                             const double* e, const double* f, const double* g, int numeq);
+
+}  // namespace cl2

@@ -1,0 +1,254 @@
+/*
+ * cl2qcd.h -- C ABI of the MI355X (gfx950) fermion-inversion kernels.
+ *
+ * This is the drop-in boundary under the C++ operator/solver layer
+ * (physics::fermionmatrix, physics::algorithms::solvers).  Every entry point
+ * enqueues on the context's HIP stream, takes plain device pointers, and
+ * returns 0 on success or a CL2_E* code; cl2_last_error() gives the message.
+ * Each declaration names the reference interface it replaces (paths relative
+ * to the CL2QCD source tree).
+ *
+ * Device layouts (all fields are dense, stride = number of eo sites Veo):
+ *   Wilson eo spinor  F64: double2[12][Veo]  component c = 3*spin + colour
+ *                     F32: float2 [12][Veo]
+ *   staggered eo vec  F64: double2[3][Veo],  F32: float2[3][Veo]
+ *   gauge field      F64_18: double2[4 dir][2 parity][9][Veo]   (row-major 3x3)
+ *                    F64_12: double2[4][2][6][Veo]   rows 0,1; row 2 = conj(row0 x row1)
+ *                    F32_12: float4 [4][2][3][Veo]   rows 0,1 packed two entries per float4
+ *   device scalars: complex = 2 doubles (re, im); real = 1 double.
+ * Host arrays use the reference's AoS order: site = x + NS*(y + NS*(z + NS*t)),
+ * eo index = site/2, link index = mu + 4*site with mu = T,X,Y,Z.
+ */
+#ifndef CL2QCD_H
+#define CL2QCD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define CL2_OK 0
+#define CL2_E_ARG 1      /* invalid argument (std::invalid_argument / Invalid_Parameters in the reference) */
+#define CL2_E_HIP 2      /* HIP runtime error (OpenclException in the reference) */
+#define CL2_E_STUCK 3    /* solver stuck: NaN or vanishing rho (SolverStuck, solvers/exceptions.hpp:35-50) */
+#define CL2_E_NOT_CONVERGED 4 /* maximum iterations reached (SolverDidNotSolve, solvers/exceptions.hpp:52-64) */
+#define CL2_E_UNSUPPORTED 5
+
+/* ---- parities (common_header_files/globaldefs.hpp:38-39) ---- */
+#define CL2_EVEN 0
+#define CL2_ODD 1
+
+/* ---- precisions and gauge formats ---- */
+#define CL2_F64 0
+#define CL2_F32 1
+#define CL2_GAUGE_F64_18 0
+#define CL2_GAUGE_F64_12 1
+#define CL2_GAUGE_F32_12 2
+
+/* ---- field kinds for cl2_field_bytes ---- */
+#define CL2_FIELD_SPINOR_EO 0
+#define CL2_FIELD_SU3VEC_EO 1
+#define CL2_FIELD_GAUGE 2
+
+/* ---- fermion actions (common::action) and eo operators ---- */
+#define CL2_ACTION_WILSON 0
+#define CL2_ACTION_TWISTEDMASS 1
+#define CL2_OP_AEE 0
+#define CL2_OP_AEE_AND_GAMMA5 1
+#define CL2_OP_AEE_MINUS 2
+#define CL2_OP_AEE_MINUS_AND_GAMMA5 3
+#define CL2_OP_QPLUS_EO 4
+#define CL2_OP_QMINUS_EO 5
+#define CL2_OP_QPLUSQMINUS_EO 6
+
+/* ---- twisted-mass site-diagonal variants ---- */
+#define CL2_MTM_SITEDIAGONAL 0
+#define CL2_MTM_INVERSE_SITEDIAGONAL 1
+#define CL2_MTM_SITEDIAGONAL_MINUS 2
+#define CL2_MTM_INVERSE_SITEDIAGONAL_MINUS 3
+#define CL2_MTM_SITEDIAGONAL_AND_GAMMA5 4
+#define CL2_MTM_SITEDIAGONAL_MINUS_AND_GAMMA5 5
+
+/* ---- scalar arguments: a host value or a device scalar ---- */
+#define CL2_SCALAR_VALUE 0      /* use re, im */
+#define CL2_SCALAR_DEV_COMPLEX 1 /* dev -> (re, im)          Scalar<hmc_complex> */
+#define CL2_SCALAR_DEV_REAL 2    /* dev -> re                 Scalar<hmc_float> */
+#define CL2_SCALAR_DEV_VECTOR 3  /* dev[index] (real)         Vector<hmc_float> element */
+typedef struct {
+    int kind;
+    double re, im;
+    const double* dev;
+    int index;
+} cl2_scalar;
+
+/* ---- lattice: the reference's compile-time kernel macros as runtime values
+ *      (hardware/code/opencl_module.cpp:61-225) ---- */
+typedef struct {
+    int ns;            /* NSPACE, even */
+    int nt;            /* NTIME, even */
+    double theta_s;    /* fermion BC: every spatial link weighted by exp(i pi theta_s / ns) */
+    double theta_t;    /* every temporal link weighted by exp(i pi theta_t / nt) */
+    int use_cp_re;     /* _CP_REAL_: T links times exp(+-cp_re) */
+    double cp_re;
+    int use_cp_im;     /* _CP_IMAG_: T links times exp(i cp_im) */
+    double cp_im;
+} cl2_lattice;
+
+typedef struct cl2_ctx cl2_ctx;
+
+/* ==== context, memory, transfer ==== */
+
+/* hardware::System / hardware::Device construction (hardware/system.hpp:64-107,
+ * hardware/device.cpp:33-143) for one GPU; the context owns a HIP stream
+ * (the in-order queue, device.cpp:79-80). */
+int cl2_ctx_create(cl2_ctx** ctx, int device, const cl2_lattice* lat);
+int cl2_ctx_destroy(cl2_ctx* ctx);
+const char* cl2_last_error(const cl2_ctx* ctx);
+/* Device::synchronize (hardware/device.cpp:147-150) */
+int cl2_sync(cl2_ctx* ctx);
+void* cl2_get_stream(cl2_ctx* ctx);
+long cl2_eo_sites(const cl2_ctx* ctx);
+/* bytes of one field of the given kind and precision / gauge format */
+size_t cl2_field_bytes(const cl2_ctx* ctx, int kind, int prec_or_format);
+/* buffer creation (hardware/buffers/buffer.cpp); memory is zeroed */
+int cl2_alloc(cl2_ctx* ctx, void** dev, size_t bytes);
+int cl2_free(cl2_ctx* ctx, void* dev);
+/* hardware::buffers::copyData (physics/lattices/util.hpp:64-78) */
+int cl2_copy(cl2_ctx* ctx, void* dst, const void* src, size_t bytes);
+/* Spinorfield_eo::import / export with AoS<->SoA conversion
+ * (physics/lattices/spinorfield_eo.cpp:54-120, ocl_kernel/spinorfield_eo_convert.cl:53-67) */
+int cl2_spinor_import(cl2_ctx* ctx, void* dev, const double* host_aos, int prec);
+int cl2_spinor_export(cl2_ctx* ctx, double* host_aos, const void* dev, int prec);
+/* Staggeredfield_eo::import / export (physics/lattices/staggeredfield_eo.cpp) */
+int cl2_su3vec_import(cl2_ctx* ctx, void* dev, const double* host_aos, int prec);
+int cl2_su3vec_export(cl2_ctx* ctx, double* host_aos, const void* dev, int prec);
+/* Gaugefield upload and SoA conversion (hardware/lattices/gaugefield.cpp:73-102,
+ * ocl_kernel/gaugefield_convert.cl:22-48) into the chosen format */
+int cl2_gauge_import(cl2_ctx* ctx, void* dev, const double* host_aos, int format);
+/* Scalar<T>::store / get (hardware/lattices/scalar.hpp:41-110) */
+int cl2_scalar_store(cl2_ctx* ctx, double* dev, const double* host, int n);
+int cl2_scalar_get(cl2_ctx* ctx, double* host, const double* dev, int n);
+
+/* ==== Wilson / twisted-mass kernels (hardware/code/fermions.hpp:47-198) ==== */
+
+/* Fermions::dslash_eo_device (fermions.cpp:430-468): -kappa * hopping term.
+ * evenodd == CL2_ODD writes the even sites from odd neighbours, CL2_EVEN the odd
+ * sites (ocl_kernel/fermionmatrix_eo_dslash.cl:50-58).  The gauge format selects
+ * the arithmetic: F64_* on F64 spinors, F32_12 on F32 spinors. */
+int cl2_dslash_eo(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, int evenodd,
+                  double kappa);
+/* Fermions::dslash_AND_M_tm_inverse_sitediagonal[_minus]_eo_device (fermions.cpp:550-624) */
+int cl2_dslash_AND_M_tm_inverse_sitediagonal_eo(cl2_ctx* ctx, void* out, const void* in, const void* gauge,
+                                                int format, int evenodd, double kappa, double mubar, int minus);
+/* Fused second half of Aee / Aee_minus [/ _AND_gamma5] (physics/fermionmatrix/fermionmatrix.cpp:250-257,
+ * 338, 432-439, 522): out = diag(f1 on spins 0,1; f2 on spins 2,3) * x - dslash(in) [then gamma5].
+ * No single reference kernel; replaces M_tm_sitediagonal + dslash_eo + saxpy[_AND_gamma5]. */
+int cl2_dslash_AND_sitediagonal_saxpy_eo(cl2_ctx* ctx, void* out, const void* in, const void* x, const void* gauge,
+                                         int format, int evenodd, double kappa, double f1_re, double f1_im,
+                                         double f2_re, double f2_im, int gamma5);
+/* Fermions::M_tm_sitediagonal*, M_tm_inverse_sitediagonal*, *_AND_gamma5_eo (fermions.cpp:626-809) */
+int cl2_M_tm_sitediagonal_eo(cl2_ctx* ctx, void* out, const void* in, int prec, double mubar, int variant);
+/* Fermions::gamma5_eo_device (fermions.cpp:384-396) */
+int cl2_gamma5_eo(cl2_ctx* ctx, void* inout, int prec);
+/* Fermions::saxpy_AND_gamma5_eo_device (fermions.cpp:398-428): out = gamma5 (y - alpha x) */
+int cl2_saxpy_AND_gamma5_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha);
+/* Whole eo operator with fused kernels: Aee, Aee_AND_gamma5_eo, Aee_minus, Aee_minus_AND_gamma5_eo,
+ * Qplus_eo, Qminus_eo, QplusQminus_eo (physics/fermionmatrix/fermionmatrix.cpp:224-664).
+ * tmp: caller-provided scratch of 2 spinor fields of the operator's precision. */
+int cl2_fermionmatrix_eo(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge, int format, int action,
+                         double kappa, double mubar, void* tmp);
+
+/* ==== Wilson eo BLAS (hardware/code/spinors.hpp:50-224) ==== */
+
+/* saxpy_eoprec / saxpy_arg_eoprec (spinors.cpp:462-521): out = y - alpha x */
+int cl2_saxpy_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha);
+/* sax_eoprec (spinors.cpp:523-551): out = alpha x */
+int cl2_sax_eo(cl2_ctx* ctx, void* out, const void* x, int prec, const cl2_scalar* alpha);
+/* saxsbypz_eoprec (spinors.cpp:553-623): out = alpha x + beta y + z */
+int cl2_saxsbypz_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, const void* z, int prec,
+                    const cl2_scalar* alpha, const cl2_scalar* beta);
+/* scalar_product_eoprec + reduction (spinors.cpp:670-713): res = sum conj(x) y, complex, on device */
+int cl2_scalar_product_eo(cl2_ctx* ctx, double* res, const void* x, const void* y, int prec);
+/* global_squarenorm_eoprec + reduction (spinors.cpp:715-789): res = sum |x|^2, real, on device */
+int cl2_squarenorm_eo(cl2_ctx* ctx, double* res, const void* x, int prec);
+/* saxpy_AND_squarenorm_eo (spinors.cpp:858-916): out = y - alpha x; res = |out|^2 (complex (norm, 0)) */
+int cl2_saxpy_AND_squarenorm_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec,
+                                const cl2_scalar* alpha, double* res);
+/* set_zero_spinorfield_eoprec / set_eoprec_spinorfield_cold (spinors.cpp:448-460,805-817):
+ * cold = every complex component 1/sqrt(12 V) */
+int cl2_set_zero_eo(cl2_ctx* ctx, void* out, int prec);
+int cl2_set_cold_eo(cl2_ctx* ctx, void* out, int prec);
+
+/* ==== staggered (hardware/code/fermions_staggered.hpp:46-91, spinors_staggered.hpp) ==== */
+
+/* Fermions_staggered::D_KS_eo_device (fermions_staggered.cpp:141-168); output parity == evenodd
+ * (ocl_kernel/fermionmatrix_staggered_eo_DKS.cl:50-69) */
+int cl2_D_KS_eo(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, int evenodd);
+/* MdagM_eo (physics/fermionmatrix/fermionmatrix_stagg.cpp:72-91): out = m^2 in - D D in;
+ * upper_left = CL2_EVEN: D_eo D_oe.  tmp: one staggered field. */
+int cl2_MdagM_eo(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, double mass,
+                 int upper_left, void* tmp);
+/* saxpy_{cplx,real,real_vec}_staggered_eoprec: out = alpha x + y */
+int cl2_saxpy_stagg_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha);
+/* saxpby_{cplx,real,real_vec}_staggered_eoprec: out = alpha x + beta y */
+int cl2_saxpby_stagg_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha,
+                        const cl2_scalar* beta);
+/* sax_{cplx,real,real_vec}_staggered_eoprec: out = alpha x */
+int cl2_sax_stagg_eo(cl2_ctx* ctx, void* out, const void* x, int prec, const cl2_scalar* alpha);
+/* saxpbypz_cplx_staggered_eoprec: out = alpha x + beta y + z */
+int cl2_saxpbypz_stagg_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, const void* z, int prec,
+                          const cl2_scalar* alpha, const cl2_scalar* beta);
+/* scalar_product_staggered_eoprec (complex) and _real_part (real) */
+int cl2_scalar_product_stagg_eo(cl2_ctx* ctx, double* res, const void* x, const void* y, int prec);
+int cl2_scalar_product_real_part_stagg_eo(cl2_ctx* ctx, double* res, const void* x, const void* y, int prec);
+/* global_squarenorm_staggered_eoprec */
+int cl2_squarenorm_stagg_eo(cl2_ctx* ctx, double* res, const void* x, int prec);
+/* sax_vectorized_and_squarenorm_eoprec (spinorfield_staggered_eo_sax_AND_squarenorm.cl:33-154):
+ * res[k] = |alpha[k] x|^2 for k < n, one pass over x */
+int cl2_sax_vec_AND_squarenorm_stagg_eo(cl2_ctx* ctx, double* res, const double* alpha, int n, const void* x,
+                                        int prec);
+/* set_zero / set_cold_spinorfield_stagg_eoprec: cold = 1/sqrt(3 V) per component */
+int cl2_set_zero_stagg_eo(cl2_ctx* ctx, void* out, int prec);
+int cl2_set_cold_stagg_eo(cl2_ctx* ctx, void* out, int prec);
+
+/* ==== device scalars (hardware/code/complex.cpp:123-170, real.cpp:189-400) ==== */
+#define CL2_SCALAR_OP_PRODUCT 0
+#define CL2_SCALAR_OP_RATIO 1
+#define CL2_SCALAR_OP_SUM 2
+#define CL2_SCALAR_OP_SUBTRACTION 3
+/* complex_{product,ratio,sum,subtraction}: out = a (op) b on device complex scalars */
+int cl2_complex_op(cl2_ctx* ctx, int op, double* out, const double* a, const double* b);
+/* real_{product,ratio,sum,subtraction} */
+int cl2_real_op(cl2_ctx* ctx, int op, double* out, const double* a, const double* b);
+/* access_real_vector_element both ways (real_access_vector_element.cl) */
+int cl2_real_get_vector_element(cl2_ctx* ctx, double* out, const double* vec, int index);
+int cl2_real_set_vector_element(cl2_ctx* ctx, double* vec, const double* in, int index);
+/* update_{zeta,beta,alpha}_cgm (real.cpp:285-400, ocl_kernel/operations_real.cl:20-39) */
+int cl2_update_zeta_cgm(cl2_ctx* ctx, double* out, const double* zeta_prev, const double* zeta_prev_prev,
+                        const double* sbeta_prev, const double* sbeta_pres, const double* salpha_prev,
+                        const double* sigma, int numeq);
+int cl2_update_beta_cgm(cl2_ctx* ctx, double* out, const double* sbeta_pres, const double* zeta_pres,
+                        const double* zeta_prev, int numeq);
+int cl2_update_alpha_cgm(cl2_ctx* ctx, double* out, const double* salpha_pres, const double* zeta_pres,
+                         const double* beta_pres, const double* zeta_prev, const double* sbeta_pres, int numeq);
+
+/* ==== timing on the context stream (Device profiling events, hardware/device.cpp:233-242) ==== */
+int cl2_timer_start(cl2_ctx* ctx);
+int cl2_timer_stop(cl2_ctx* ctx, double* ms);
+
+/* ==== byte / flop models (Opencl_Module::get_read_write_size / get_flop_size,
+ *      hardware/code/fermions.cpp:811-975, fermions_staggered.cpp:180-280) ==== */
+/* The reference's own counts for a named kernel at this lattice (FP64, 18-real links). */
+uint64_t cl2_reference_read_write_size(const cl2_ctx* ctx, const char* kernel);
+uint64_t cl2_reference_flop_size(const cl2_ctx* ctx, const char* kernel);
+/* Compulsory HBM bytes of one launch of a named kernel in this build's layout
+ * (each link, input and output moved once); the roofline numerator. */
+uint64_t cl2_algorithmic_bytes(const cl2_ctx* ctx, const char* kernel, int format);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

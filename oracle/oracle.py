@@ -59,6 +59,8 @@ def lib():
         sig = {
             "orc_volume": (l, [L]),
             "orc_veo": (l, [L]),
+            "orc_even_site": (i, [L, l, ctypes.POINTER(l)]),
+            "orc_odd_site": (i, [L, l, ctypes.POINTER(l)]),
             "orc_fill_doubles_drand48": (None, [_P, l, i]),
             "orc_gauge_cold": (None, [L, _P]),
             "orc_ildg_read": (i, [ctypes.c_char_p, L, _P, ctypes.c_char_p, i]),
@@ -104,6 +106,17 @@ def volume(lat):
 
 def veo(lat):
     return volume(lat) // 2
+
+
+def eo_sites(lat, parity):
+    """Lattice site of every eo index for the given parity (0 even, 1 odd)."""
+    f = lib().orc_even_site if parity == 0 else lib().orc_odd_site
+    out = np.empty(veo(lat), dtype=np.int64)
+    s = ctypes.c_long()
+    for i in range(out.size):
+        f(ctypes.byref(lat), i, ctypes.byref(s))
+        out[i] = s.value
+    return out
 
 
 def set_threads(n):
