@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark of the eo hopping kernel on one MI355X.
+
+A step is one FP64 dslash_eo application (18-real links) over a 48^3x96
+lattice, alternating EVEN/ODD as the reference's dslash benchmark does
+(executables/dslashBenchmark.cpp:42-50).  Inputs are synthetic and resident in
+HBM before timing: random SU(3) links from the device generator and a seeded
+Gaussian spinor.  `value` is algorithmic GB/s: compulsory bytes per application
+(8 links + input + output spinor per eo site) times steps over the timed
+region, measured with HIP events on the context stream.
+
+Also reported: the same for 12-real FP64 and FP32 links, the reference's own
+byte convention (2880 B/site), the CPU oracle timed on host cores
+(cpu_baseline), and the HBM traffic per launch measured by rocprofv3 counters
+(read from profiles/ when the counter summary for this workload is there).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "eo-Dslash GB/s & %HBM peak; mixed vs FP64 CG time-to-1e-10 at 48³×96"
+HBM_PEAK_GBPS = 8000.0
+COUNTERS_FILE = os.path.join(ROOT, "profiles", "dslash_eo_f64_18_48x96_counters.txt")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def time_steps(ctx, fn, steps):
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.timer_start()
+    for k in range(steps):
+        fn(k)
+    ms = ctypes.c_double()
+    ctx.check(ctx.lib.cl2_timer_stop(ctx.handle, ctypes.byref(ms)))
+    ctx.sync()
+    return ms.value, (time.perf_counter() - t0) * 1e3
+
+
+def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src):
+    g = ctx.gauge(fmt=fmt, seed=20261202, epsilon=0.35)
+    a, b = ctx.spinor(prec).upload(src), ctx.spinor(prec)
+    kappa = 0.16
+
+    def step(k):
+        if k % 2 == 0:
+            ctx.dslash_eo(b.ptr, a.ptr, g.ptr, fmt, abi.EVEN, kappa)
+        else:
+            ctx.dslash_eo(a.ptr, b.ptr, g.ptr, fmt, abi.ODD, kappa)
+
+    for k in range(warmup):
+        step(k)
+    ev_ms, wall_ms = time_steps(ctx, step, steps)
+    out = b.download() if steps % 2 else a.download()
+    finite = bool(np.isfinite(out).all())
+    a.close()
+    b.close()
+    g.close()
+    alg = ctx.lib.cl2_algorithmic_bytes(ctx.handle, b"dslash_eo", fmt)
+    return dict(ms=ev_ms / steps, wall_ms=wall_ms / steps, alg_bytes=alg, GBps=alg * steps / (ev_ms * 1e6),
+                finite=finite)
+
+
+def read_counters():
+    """HBM bytes per launch of the FP64 dslash from the committed rocprofv3 counter summary."""
+    if not os.path.exists(COUNTERS_FILE):
+        return None
+    vals = {}
+    for line in open(COUNTERS_FILE):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(seconds=12.0):
+    """The oracle's dslash_eo on host cores over a bounded sample (24^3x48, FP64)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    lat = O.lattice(24, 48, theta_t=1.0)
+    rng = np.random.default_rng(7)
+    g = O.gauge_cold(lat)
+    g += 0.01 * rng.standard_normal(g.shape)  # values do not affect the timing
+    src = rng.standard_normal((O.veo(lat), 24))
+    O.dslash_eo(lat, src, g, O.EVEN, 0.16)  # warm caches and the thread pool
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        O.dslash_eo(lat, src, g, O.EVEN if n % 2 == 0 else O.ODD, 0.16)
+        n += 1
+    dt = time.perf_counter() - t0
+    per_app = 1536 * O.veo(lat)
+    return dict(value=round(per_app * n / dt / 1e9, 3), unit="GB/s", cores=O.threads(), kind="port",
+                sample=f"oracle dslash_eo FP64 18-real, 24^3x48, {n} applications in {dt:.1f} s, "
+                       f"{O.threads()} OpenMP threads")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--ns", type=int, default=48)
+    ap.add_argument("--nt", type=int, default=96)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only-f64", action="store_true", help="time only the FP64 18-real leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
+              "described in DESIGN.md but not built)", file=sys.stderr)
+        sys.exit(2)
+
+    from cl2qcd_amd import abi
+
+    ctx = abi.Context(args.ns, args.nt, theta_t=1.0)
+    src = np.random.default_rng(1).standard_normal((ctx.veo, 24))
+    # counter calibration for rocprofv3 runs of this script: one pure read and one
+    # pure write of exactly 12 * Veo * 16 bytes (k_reduce1 and k_fill)
+    cal = ctx.spinor()
+    res = ctx.scalars(1)
+    ctx.squarenorm_eo(res.ptr, cal.ptr, abi.F64)
+    ctx.set_zero_eo(cal.ptr, abi.F64)
+    ctx.sync()
+    cal.close()
+    res.close()
+    log(f"lattice {args.ns}^3x{args.nt}, {ctx.veo} eo sites; warmup {args.warmup}, steps {args.steps}")
+    f64 = dslash_leg(ctx, abi, abi.GAUGE_F64_18, abi.F64, args.steps, args.warmup, src)
+    log(f"dslash F64/18: {f64['ms']:.4f} ms/app, {f64['GBps']:.0f} GB/s")
+    extra = {}
+    if not args.only_f64:
+        for name, fmt, prec in (("f64_12", abi.GAUGE_F64_12, abi.F64), ("f32_12", abi.GAUGE_F32_12, abi.F32)):
+            r = dslash_leg(ctx, abi, fmt, prec, max(args.steps // 4, 2), args.warmup, src)
+            log(f"dslash {name}: {r['ms']:.4f} ms/app, {r['GBps']:.0f} GB/s")
+            extra[f"dslash_{name}"] = dict(ms_per_app=round(r["ms"], 5), GBps=round(r["GBps"], 1),
+                                           pct_hbm_peak=round(100 * r["GBps"] / HBM_PEAK_GBPS, 1))
+    ref_bytes = ctx.lib.cl2_reference_read_write_size(ctx.handle, b"dslash_eo")
+    veo = ctx.veo
+    ctx.close()
+
+    traffic = read_counters()
+    achieved = f64["alg_bytes"] / (f64["ms"] * 1e6)
+    line = {
+        "metric": METRIC,
+        "value": round(f64["GBps"], 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(f64["ms"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: device-generated random SU(3) links (epsilon 0.35, seeded), seeded Gaussian spinor",
+        "config": {"workload": f"dslash_eo FP64 18-real links, {args.ns}^3x{args.nt}, kappa 0.16, antiperiodic t",
+                   "lattice": f"{args.ns}^3x{args.nt}", "eo_sites": veo,
+                   "bytes_per_site": 1536},
+        "pct_hbm_peak": round(100 * f64["GBps"] / HBM_PEAK_GBPS, 1),
+        "wall_ms_per_step": round(f64["wall_ms"], 5),
+        "cl2qcd_convention_GBps": round(ref_bytes / (f64["ms"] * 1e6), 1),
+        "output_finite": f64["finite"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
+        **extra,
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle on host cores) ...")
+        line["cpu_baseline"] = cpu_baseline()
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

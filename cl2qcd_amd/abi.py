@@ -78,6 +78,9 @@ SIGNATURES = {
     "cl2_su3vec_import": (_i, [_vp, _vp, _dp, _i]),
     "cl2_su3vec_export": (_i, [_vp, _dp, _vp, _i]),
     "cl2_gauge_import": (_i, [_vp, _vp, _dp, _i]),
+    "cl2_gauge_set_random": (_i, [_vp, _vp, _i, ctypes.c_uint64, _d]),
+    "cl2_gauge_convert": (_i, [_vp, _vp, _i, _vp]),
+    "cl2_gauge_export": (_i, [_vp, _dp, _vp, _i]),
     "cl2_scalar_store": (_i, [_vp, _dp, _dp, _i]),
     "cl2_scalar_get": (_i, [_vp, _dp, _dp, _i]),
     "cl2_dslash_eo": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d]),
@@ -114,6 +117,8 @@ SIGNATURES = {
     "cl2_update_zeta_cgm": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i]),
     "cl2_update_beta_cgm": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
     "cl2_update_alpha_cgm": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _i]),
+    "cl2_tune_set": (_i, [_vp, ctypes.c_char_p, ctypes.c_long]),
+    "cl2_tune_get": (ctypes.c_long, [_vp, ctypes.c_char_p]),
     "cl2_timer_start": (_i, [_vp]),
     "cl2_timer_stop": (_i, [_vp, ctypes.POINTER(ctypes.c_double)]),
     "cl2_reference_read_write_size": (ctypes.c_uint64, [_vp, ctypes.c_char_p]),
@@ -195,12 +200,31 @@ class Field(DeviceBuffer):
 
 
 class Gauge(DeviceBuffer):
-    def __init__(self, ctx, host_aos, fmt=GAUGE_F64_18):
+    """A device gauge field: from a host AoS array, or a random/cold start on the device."""
+
+    def __init__(self, ctx, host_aos=None, fmt=GAUGE_F64_18, seed=None, epsilon=0.0):
         self.fmt = fmt
         super().__init__(ctx, ctx.lib.cl2_field_bytes(ctx.handle, FIELD_GAUGE, fmt))
-        host = np.ascontiguousarray(host_aos, dtype=np.float64)
-        assert host.size == 4 * 2 * ctx.veo * 18
-        ctx.check(ctx.lib.cl2_gauge_import(ctx.handle, self.ptr, host.ctypes.data, fmt))
+        if host_aos is not None:
+            host = np.ascontiguousarray(host_aos, dtype=np.float64)
+            assert host.size == 4 * 2 * ctx.veo * 18
+            ctx.check(ctx.lib.cl2_gauge_import(ctx.handle, self.ptr, host.ctypes.data, fmt))
+        else:
+            ctx.check(ctx.lib.cl2_gauge_set_random(ctx.handle, self.ptr, fmt, seed or 0, epsilon))
+
+    def download(self):
+        out = np.empty((8 * self.ctx.veo, 18))
+        self.ctx.check(self.ctx.lib.cl2_gauge_export(self.ctx.handle, out.ctypes.data, self.ptr, self.fmt))
+        return out
+
+    def converted(self, fmt):
+        """A copy of this F64_18 field in another device format."""
+        assert self.fmt == GAUGE_F64_18
+        g = Gauge.__new__(Gauge)
+        g.fmt = fmt
+        DeviceBuffer.__init__(g, self.ctx, self.ctx.lib.cl2_field_bytes(self.ctx.handle, FIELD_GAUGE, fmt))
+        self.ctx.check(self.ctx.lib.cl2_gauge_convert(self.ctx.handle, g.ptr, fmt, self.ptr))
+        return g
 
 
 class DeviceScalars(DeviceBuffer):
@@ -262,8 +286,8 @@ class Context:
     def su3vec(self, prec=F64):
         return Field(self, FIELD_SU3VEC_EO, prec)
 
-    def gauge(self, host_aos, fmt=GAUGE_F64_18):
-        return Gauge(self, host_aos, fmt)
+    def gauge(self, host_aos=None, fmt=GAUGE_F64_18, seed=None, epsilon=0.0):
+        return Gauge(self, host_aos, fmt, seed, epsilon)
 
     def scalars(self, n):
         return DeviceScalars(self, n)

@@ -156,6 +156,7 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.scale         = scale;
     a.norm_partials = nullptr;
     a.w             = wilson_weights(c, kappa);
+    a.map           = work_map(c, DSLASH_BLOCK);
     CL2_HIP(c, launch_dslash(c, a, fmt));
     return CL2_OK;
 }
@@ -452,6 +453,75 @@ int cl2_gauge_import(cl2_ctx* c, void* dev, const double* host, int fmt)
         CL2_HIP(c, hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
     CL2_HIP(c, hipStreamSynchronize(c->stream));
+    return CL2_OK;
+}
+
+int cl2_gauge_set_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, double epsilon)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dev != nullptr, "null pointer");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, epsilon >= 0. && std::isfinite(epsilon), "epsilon must be finite and >= 0");
+    CL2_HIP(c, launch_gauge_random(c, dev, fmt, seed, epsilon));
+    return CL2_OK;
+}
+
+int cl2_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dst && src && dst != src, "bad pointers");
+    CL2_REQUIRE(c, valid_fmt(dfmt), "unknown gauge format");
+    CL2_HIP(c, launch_gauge_convert(c, dst, dfmt, src));
+    return CL2_OK;
+}
+
+int cl2_gauge_export(cl2_ctx* c, double* host, const void* dev, int fmt)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dev && host, "null pointer");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    const size_t S = c->g.veo, V = 2 * S;
+    const int lanes = fmt == CL2_GAUGE_F64_18 ? 9 : 6;
+    std::vector<double> stage(2 * lanes * 8 * S);
+    if (fmt == CL2_GAUGE_F32_12) {
+        std::vector<float> f(gauge_bytes(c, fmt) / sizeof(float));
+        CL2_HIP(c, hipMemcpyAsync(f.data(), dev, f.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+        for (size_t dp = 0; dp < 8; ++dp)
+            for (int k = 0; k < 3; ++k)
+                for (size_t eo = 0; eo < S; ++eo)
+                    for (int q = 0; q < 4; ++q)
+                        stage[2 * ((dp * 6 + 2 * k + q / 2) * S + eo) + (q & 1)] = f[4 * ((dp * 3 + k) * S + eo) + q];
+    } else {
+        CL2_HIP(c, hipMemcpyAsync(stage.data(), dev, stage.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+    }
+    for (size_t s = 0; s < V; ++s) {
+        const int par = site_parity(c, s);
+        const size_t eo = s / 2;
+        for (int mu = 0; mu < 4; ++mu) {
+            double* u = host + 18 * (mu + 4 * s);
+            for (int k = 0; k < lanes; ++k) {
+                const double* d = &stage[2 * (((size_t)(mu * 2 + par) * lanes + k) * S + eo)];
+                u[2 * k]        = d[0];
+                u[2 * k + 1]    = d[1];
+            }
+            if (lanes == 6) {  // third row = conj(row0 x row1)
+                auto cm = [](const double* a, const double* b, double* r) {
+                    r[0] = a[0] * b[0] - a[1] * b[1];
+                    r[1] = a[0] * b[1] + a[1] * b[0];
+                };
+                const int idx[3][2] = {{1, 2}, {2, 0}, {0, 1}};
+                for (int col = 0; col < 3; ++col) {
+                    double p1[2], p2[2];
+                    cm(&u[2 * idx[col][0]], &u[2 * (3 + idx[col][1])], p1);
+                    cm(&u[2 * idx[col][1]], &u[2 * (3 + idx[col][0])], p2);
+                    u[2 * (6 + col)]     = p1[0] - p2[0];
+                    u[2 * (6 + col) + 1] = -(p1[1] - p2[1]);
+                }
+            }
+        }
+    }
     return CL2_OK;
 }
 
@@ -798,6 +868,35 @@ int cl2_update_alpha_cgm(cl2_ctx* c, double* out, const double* salpha_pres, con
     CL2_HIP(c, launch_cgm_update(c, 2, out, salpha_pres, zeta_pres, beta_pres, zeta_prev, sbeta_pres, nullptr,
                                  numeq));
     return CL2_OK;
+}
+
+/* ---------------- tuning ---------------- */
+
+int cl2_tune_set(cl2_ctx* c, const char* key, long value)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, key != nullptr, "null key");
+    const std::string k(key);
+    if (k == "hop.zplanes") {
+        CL2_REQUIRE(c, value == -1 || value == 0 || (value > 0 && c->g.ns % (uint32_t)value == 0),
+                    "hop.zplanes must be -1, 0 or a divisor of ns");
+        c->tune_zplanes = (int)value;
+        return CL2_OK;
+    }
+    if (k == "hop.xcd") {
+        c->tune_xcd = value ? 1 : 0;
+        return CL2_OK;
+    }
+    return fail(c, CL2_E_ARG, "unknown tuning key " + k);
+}
+
+long cl2_tune_get(const cl2_ctx* c, const char* key)
+{
+    if (!c || !key) return -2;
+    const std::string k(key);
+    if (k == "hop.zplanes") return c->tune_zplanes;
+    if (k == "hop.xcd") return c->tune_xcd;
+    return -2;
 }
 
 /* ---------------- timing ---------------- */

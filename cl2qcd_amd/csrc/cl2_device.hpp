@@ -177,6 +177,26 @@ __device__ __forceinline__ void site_neighbours(const LatticeGeom& g, uint32_t i
     nb.dn[0] = (t == 0 ? site + (g.nt - 1) * vs : site - vs) >> 1;
 }
 
+/* eo index handled by thread tid of hardware block h under the WorkMap;
+ * returns false for threads past the end of the lattice. */
+template<int BLOCK>
+__device__ __forceinline__ bool map_site(const WorkMap& m, uint32_t veo, uint32_t h, uint32_t tid, uint32_t& i)
+{
+    uint32_t L = h;
+    if (m.xcd_q && h < 8u * m.xcd_q) L = (h & 7u) * m.xcd_q + (h >> 3);
+    const uint32_t l = L * BLOCK + tid;
+    if (l >= veo) return false;
+    if (m.P == 0) {
+        i = l;
+        return true;
+    }
+    const uint32_t zc = l / m.col;
+    const uint32_t r  = l - zc * m.col;
+    const uint32_t t  = r / m.P;
+    i                 = t * m.half_vs + zc * m.P + (r - t * m.P);
+    return true;
+}
+
 __device__ __forceinline__ void site_coords(const LatticeGeom& g, uint32_t i, int parity, uint32_t& x, uint32_t& y,
                                             uint32_t& z, uint32_t& t)
 {
@@ -193,6 +213,7 @@ __device__ __forceinline__ void site_coords(const LatticeGeom& g, uint32_t i, in
 /* Streaming (non-temporal) loads for data read exactly once per launch. */
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
+#ifndef CL2_LINK_LOAD_CACHED
 __device__ __forceinline__ double2 load_stream(const double2* p)
 {
     nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
@@ -203,6 +224,16 @@ __device__ __forceinline__ float4 load_stream(const float4* p)
     nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
     return make_float4(v.x, v.y, v.z, v.w);
 }
+#else
+__device__ __forceinline__ double2 load_stream(const double2* p)
+{
+    return *p;
+}
+__device__ __forceinline__ float4 load_stream(const float4* p)
+{
+    return *p;
+}
+#endif
 
 /* ---------------- gauge-link formats ----------------
  * Device layout [dir][parity][lane][gstride] where lane = complex entry (18-real)

@@ -1,0 +1,102 @@
+// Device-side gauge-field starts: random SU(3) links from a counter-based
+// generator (the Gaugefield hot/cold starts of physics/lattices/gaugefield.cpp:35-72),
+// written straight into any of the device link formats.
+#include "cl2_device.hpp"
+#include "cl2_internal.hpp"
+
+namespace cl2 {
+
+/* splitmix64 finaliser: a stateless hash of (seed, counter) */
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t h)
+{
+    return ((h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+/* One link per thread: A = 1 + eps * G with G complex Gaussian, then rows 0,1
+ * Gram-Schmidt orthonormalised and row 2 = conj(row0 x row1), so det U = 1. */
+__global__ void k_gauge_random(void* dev, int fmt, uint32_t veo, uint64_t seed, double eps)
+{
+    const uint64_t link = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;  // (dir, parity, eo)
+    if (link >= 8ull * veo) return;
+    Su3<double> u;
+    uint64_t ctr = (seed * 0x100000001b3ull) ^ (link * 32ull);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double r1 = u01(mix64(ctr++)), r2 = u01(mix64(ctr++));
+        double rad = sqrt(-2.0 * log(r1)), ph = 6.283185307179586 * r2;
+        const int row = k / 3, col = k % 3;
+        u.e[row][col] = Cx<double>{(row == col ? 1.0 : 0.0) + eps * rad * cos(ph), eps * rad * sin(ph)};
+    }
+    double n0 = 0.;
+    for (int c = 0; c < 3; ++c) n0 += u.e[0][c].re * u.e[0][c].re + u.e[0][c].im * u.e[0][c].im;
+    n0 = 1.0 / sqrt(n0);
+    for (int c = 0; c < 3; ++c) u.e[0][c] = scale(u.e[0][c], n0);
+    Cx<double> p{0., 0.};  // <row0, row1>
+    for (int c = 0; c < 3; ++c) p = p + cmulconj(u.e[1][c], u.e[0][c]);
+    for (int c = 0; c < 3; ++c) u.e[1][c] = u.e[1][c] - cmul(p, u.e[0][c]);
+    double n1 = 0.;
+    for (int c = 0; c < 3; ++c) n1 += u.e[1][c].re * u.e[1][c].re + u.e[1][c].im * u.e[1][c].im;
+    n1 = 1.0 / sqrt(n1);
+    for (int c = 0; c < 3; ++c) u.e[1][c] = scale(u.e[1][c], n1);
+    su3_reconstruct_row2(u);
+
+    const uint32_t dp = (uint32_t)(link / veo), eo = (uint32_t)(link % veo);
+    if (fmt == GAUGE_F32_12) {
+        float4* q = reinterpret_cast<float4*>(dev) + (size_t)(dp * 3) * veo + eo;
+        for (int k = 0; k < 3; ++k) {
+            Cx<double> a = u.e[(2 * k) / 3][(2 * k) % 3], b = u.e[(2 * k + 1) / 3][(2 * k + 1) % 3];
+            q[(size_t)k * veo] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+        }
+    } else {
+        const int lanes = (fmt == GAUGE_F64_18) ? 9 : 6;
+        double2* q      = reinterpret_cast<double2*>(dev) + (size_t)(dp * lanes) * veo + eo;
+        for (int k = 0; k < lanes; ++k) q[(size_t)k * veo] = make_double2(u.e[k / 3][k % 3].re, u.e[k / 3][k % 3].im);
+    }
+}
+
+hipError_t launch_gauge_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, double eps)
+{
+    const size_t n    = 8ull * c->g.veo;
+    const unsigned nb = blocks_for(n, 256);
+    hipLaunchKernelGGL(k_gauge_random, dim3(nb), dim3(256), 0, c->stream, dev, fmt, c->g.veo, seed, eps);
+    return hipGetLastError();
+}
+
+/* Device-layout conversion between formats (F64_18 -> F64_12 / F32_12), so one
+ * resident FP64 field can feed the reduced-precision solvers. */
+__global__ void k_gauge_convert(void* dst, int dfmt, const double2* __restrict__ src, uint32_t veo)
+{
+    const uint64_t link = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (link >= 8ull * veo) return;
+    const uint32_t dp = (uint32_t)(link / veo), eo = (uint32_t)(link % veo);
+    const double2* s  = src + (size_t)(dp * 9) * veo + eo;
+    if (dfmt == GAUGE_F32_12) {
+        float4* q = reinterpret_cast<float4*>(dst) + (size_t)(dp * 3) * veo + eo;
+        for (int k = 0; k < 3; ++k) {
+            double2 a = s[(size_t)(2 * k) * veo], b = s[(size_t)(2 * k + 1) * veo];
+            q[(size_t)k * veo] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+        }
+    } else {
+        const int lanes = (dfmt == GAUGE_F64_18) ? 9 : 6;
+        double2* q      = reinterpret_cast<double2*>(dst) + (size_t)(dp * lanes) * veo + eo;
+        for (int k = 0; k < lanes; ++k) q[(size_t)k * veo] = s[(size_t)k * veo];
+    }
+}
+
+hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src)
+{
+    const size_t n    = 8ull * c->g.veo;
+    const unsigned nb = blocks_for(n, 256);
+    hipLaunchKernelGGL(k_gauge_convert, dim3(nb), dim3(256), 0, c->stream, dst, dfmt,
+                       reinterpret_cast<const double2*>(src), c->g.veo);
+    return hipGetLastError();
+}
+
+}  // namespace cl2

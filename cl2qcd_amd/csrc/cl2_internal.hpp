@@ -17,6 +17,8 @@ struct cl2_ctx {
     double* partials = nullptr;  // reduction scratch: one double2 per block
     size_t npartials = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    int tune_zplanes = -1;  // z-planes per slab of the hopping kernels (-1: heuristic, 0: identity order)
+    int tune_xcd     = 1;   // XCD-contiguous block assignment
     std::string err;
 };
 
@@ -30,6 +32,9 @@ inline unsigned blocks_for(size_t n, int block)
 {
     return (unsigned)((n + block - 1) / block);
 }
+
+/* work order of the hopping kernels for this context's lattice and tuning */
+WorkMap work_map(const cl2_ctx* c, int block);
 
 /* hopping weights -kappa * b_mu * c_mu (forward) and -kappa * conj(b_mu * c_mu') (backward) */
 HopWeights<double> wilson_weights(const cl2_ctx* c, double kappa);
@@ -71,6 +76,9 @@ hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t
  * 1: res[0..1] = complex sum;  2: res[k] = alpha_vec[k]^2 * real part for k < nvec */
 hipError_t launch_reduce_stage2(cl2_ctx* c, unsigned nparts, double* res, int out_kind, const double* alpha_vec,
                                 int nvec);
+
+hipError_t launch_gauge_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, double eps);
+hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src);
 
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im);
 

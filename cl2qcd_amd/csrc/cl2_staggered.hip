@@ -12,6 +12,7 @@ struct DksArgs {
     LatticeGeom g;
     int out_parity;
     double sf[4][2];  // 0.5 * b_mu (* imaginary chemical potential on T)
+    WorkMap map;
 };
 
 /* D_KS_eo_local (ocl_kernel/fermionmatrix_staggered_eo_DKS_local.cl:65-136):
@@ -23,9 +24,9 @@ __global__ __launch_bounds__(BLOCK) void k_dks_eo(DksArgs a)
 {
     using R = typename GaugeLoad<FMT>::R;
     using V = vec2<R>;
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t S = a.g.veo;
-    if (i >= S) return;
+    uint32_t i       = 0;
+    if (!map_site<BLOCK>(a.map, S, blockIdx.x, threadIdx.x, i)) return;
     uint32_t x, y, z, t;
     site_coords(a.g, i, a.out_parity, x, y, z, t);
     SiteNb nb;
@@ -73,6 +74,7 @@ hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, 
     a.gauge      = gauge;
     a.g          = c->g;
     a.out_parity = evenodd;
+    a.map        = work_map(c, DSLASH_BLOCK);
     for (int mu = 0; mu < 4; ++mu) {
         const double* b = (mu == 0) ? c->bt : c->bs;
         double re = 0.5 * b[0], im = 0.5 * b[1];

@@ -20,6 +20,19 @@ struct LatticeGeom {
     uint32_t veo;  // ns^3*nt/2, also the field stride (no padding)
 };
 
+/* Work order of the hopping kernels.  Logical work item l enumerates
+ * (z-chunk, t, site-in-slab) with the site fastest, then t, then the z-chunk;
+ * a slab is zplanes consecutive z-planes of one t-slice (P = zplanes*ns^2/2 eo
+ * sites).  Each of the 8 XCDs gets a contiguous range of logical blocks, so one
+ * XCD sweeps its z-chunk through all t and the t+-1 / z+-1 neighbours of a site
+ * are still in that XCD's L2. */
+struct WorkMap {
+    uint32_t P;        // eo sites per slab (0: identity order)
+    uint32_t col;      // P * nt
+    uint32_t half_vs;  // eo sites per t-slice
+    uint32_t xcd_q;    // logical blocks per XCD (0: no XCD remap)
+};
+
 /* Epilogues fused into the Wilson hopping kernel. */
 enum DslashEpilogue : int {
     EPI_PLAIN = 0,  // out = D in                          (dslash_eo)
@@ -48,6 +61,7 @@ struct DslashArgs {
     double scale;
     double* norm_partials;  // if non-null: per-block sum |out|^2
     HopWeights<double> w;
+    WorkMap map;
 };
 
 }  // namespace cl2

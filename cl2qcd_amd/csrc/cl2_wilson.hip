@@ -67,10 +67,10 @@ __global__ __launch_bounds__(BLOCK) void k_dslash_eo(DslashArgs a)
     using R = typename GaugeLoad<FMT>::R;
     using V = vec2<R>;
     __shared__ double lds[2 * BLOCK / 64];
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     const uint32_t S = a.g.veo;
+    uint32_t i       = 0;
     double nrm       = 0.;
-    if (i < S) {
+    if (map_site<BLOCK>(a.map, S, blockIdx.x, threadIdx.x, i)) {
         SiteNb nb;
         site_neighbours(a.g, i, a.out_parity, nb);
         const V* in    = reinterpret_cast<const V*>(a.in);
@@ -131,6 +131,32 @@ __global__ __launch_bounds__(BLOCK) void k_sitediag(vec2<R>* __restrict__ out, c
         Cx<R> r   = scale(cmul(Cx<R>{v.x, v.y}, k < 6 ? f1 : f2), s);
         out[(size_t)k * S + i] = vec2<R>{r.re, r.im};
     }
+}
+
+WorkMap work_map(const cl2_ctx* c, int block)
+{
+    WorkMap m{0u, 0u, c->g.vs / 2, 0u};
+    const uint32_t ns = c->g.ns, plane = ns * ns / 2;
+    int zp            = c->tune_zplanes;
+    if (zp < 0) {
+        // one z-chunk per XCD when ns divides by 8 (measured best at 48^3x96:
+        // 8 chunks of 6 planes); else the largest divisor of ns whose slab
+        // stays within 8192 eo sites
+        if (ns % 8 == 0) {
+            zp = (int)(ns / 8);
+        } else {
+            zp = 1;
+            for (uint32_t d = 1; d <= ns; ++d)
+                if (ns % d == 0 && d * plane <= 8192u) zp = (int)d;
+        }
+    }
+    if (zp > 0 && ns % (uint32_t)zp == 0) {
+        m.P   = (uint32_t)zp * plane;
+        m.col = m.P * c->g.nt;
+    }
+    const uint32_t nblocks = (c->g.veo + block - 1) / block;
+    if (c->tune_xcd && nblocks >= 64) m.xcd_q = nblocks / 8;
+    return m;
 }
 
 HopWeights<double> wilson_weights(const cl2_ctx* c, double kappa)

@@ -128,6 +128,15 @@ int cl2_su3vec_export(cl2_ctx* ctx, double* host_aos, const void* dev, int prec)
 /* Gaugefield upload and SoA conversion (hardware/lattices/gaugefield.cpp:73-102,
  * ocl_kernel/gaugefield_convert.cl:22-48) into the chosen format */
 int cl2_gauge_import(cl2_ctx* ctx, void* dev, const double* host_aos, int format);
+/* Gaugefield hot / cold start (physics/lattices/gaugefield.cpp:35-72, hardware::lattices::Gaugefield::set_hot /
+ * set_cold): every link U = reunitarised (1 + epsilon * complex Gaussian) from a counter-based generator keyed by
+ * seed; epsilon = 0 gives the cold (unit) field, epsilon >~ 10 a Haar-like hot field. */
+int cl2_gauge_set_random(cl2_ctx* ctx, void* dev, int format, uint64_t seed, double epsilon);
+/* Re-layout a resident F64_18 field into another format (feeds reduced-precision solvers). */
+int cl2_gauge_convert(cl2_ctx* ctx, void* dst, int dst_format, const void* src_f64_18);
+/* Gaugefield export to host AoS (hardware/lattices/gaugefield.cpp:104-124); 12-real formats are
+ * returned with the reconstructed third row. */
+int cl2_gauge_export(cl2_ctx* ctx, double* host_aos, const void* dev, int format);
 /* Scalar<T>::store / get (hardware/lattices/scalar.hpp:41-110) */
 int cl2_scalar_store(cl2_ctx* ctx, double* dev, const double* host, int n);
 int cl2_scalar_get(cl2_ctx* ctx, double* host, const double* dev, int n);
@@ -234,6 +243,14 @@ int cl2_update_beta_cgm(cl2_ctx* ctx, double* out, const double* sbeta_pres, con
                         const double* zeta_prev, int numeq);
 int cl2_update_alpha_cgm(cl2_ctx* ctx, double* out, const double* salpha_pres, const double* zeta_pres,
                          const double* beta_pres, const double* zeta_prev, const double* sbeta_pres, int numeq);
+
+/* ==== launch tuning (no reference counterpart: the reference picks work sizes in
+ *      hardware/device_info.cpp:111-128 and hardware/code/opencl_module.cpp get_work_sizes) ==== */
+/* keys: "hop.zplanes" -- z-planes per slab of the t-fastest traversal of the hopping kernels
+ *       (-1 heuristic, 0 lexicographic order, else a divisor of ns);
+ *       "hop.xcd" -- 1: give each XCD a contiguous range of blocks, 0: hardware round robin. */
+int cl2_tune_set(cl2_ctx* ctx, const char* key, long value);
+long cl2_tune_get(const cl2_ctx* ctx, const char* key);
 
 /* ==== timing on the context stream (Device profiling events, hardware/device.cpp:233-242) ==== */
 int cl2_timer_start(cl2_ctx* ctx);
