@@ -1,0 +1,64 @@
+// Krylov solvers of the fermion-inversion path: cg (solvers/cg.hpp:41-56),
+// bicgstab (solvers/bicgstab.hpp), cg_m (solver_shifted.hpp:48-52) with the
+// reference's semantics (absolute |r|^2 < prec, checked every
+// cgIterationBlockSize iterations, restart every iterRefresh), exceptions
+// (solvers/exceptions.hpp:35-64), plus the reduced-precision and batched
+// variants this build adds on top.
+#pragma once
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../fermionmatrix/fermionmatrix.hpp"
+
+namespace physics {
+namespace algorithms {
+namespace solvers {
+
+class SolverException : public std::runtime_error {
+  public:
+    SolverException(const std::string& what, int iter) : std::runtime_error(what), iter(iter) {}
+    int get_iterations() const noexcept { return iter; }
+
+  private:
+    int iter;
+};
+/* NaN residual or vanishing rho */
+class SolverStuck : public SolverException {
+  public:
+    SolverStuck(int iter, const std::string& where)
+        : SolverException("Solver got stuck after " + std::to_string(iter) + " iterations (" + where + ")", iter)
+    {
+    }
+};
+/* maximum iteration count reached */
+class SolverDidNotSolve : public SolverException {
+  public:
+    SolverDidNotSolve(int iter, const std::string& where)
+        : SolverException("Solver did not solve in " + std::to_string(iter) + " iterations (" + where + ")", iter)
+    {
+    }
+};
+
+/* CG on a hermitian eo operator; returns the iteration index at convergence */
+int cg(const lattices::Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& A, const lattices::Gaugefield& gf,
+       const lattices::Spinorfield_eo& b, const hardware::System& system, physics::InterfacesHandler& interfacesHandler,
+       hmc_float prec, const physics::AdditionalParameters& additionalParameters);
+
+/* BiCGStab (bicgstab_fast, or bicgstab_save when the solver parameter is bicgstab_save) */
+int bicgstab(const lattices::Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& A,
+             const lattices::Gaugefield& gf, const lattices::Spinorfield_eo& b, const hardware::System& system,
+             physics::InterfacesHandler& interfacesHandler, hmc_float prec,
+             const physics::AdditionalParameters& additionalParameters);
+
+/* multi-shift CG for (A + sigma_k) x_k = b */
+int cg_m(std::vector<std::shared_ptr<lattices::Staggeredfield_eo>> x, const fermionmatrix::Fermionmatrix_stagg_eo& A,
+         const lattices::Gaugefield& gf, const std::vector<hmc_float> sigma, const lattices::Staggeredfield_eo& b,
+         const hardware::System& system, physics::InterfacesHandler& interfacesHandler, hmc_float prec,
+         const physics::AdditionalParameters& additionalParameters);
+
+}  // namespace solvers
+}  // namespace algorithms
+}  // namespace physics

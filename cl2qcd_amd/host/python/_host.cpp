@@ -1,0 +1,296 @@
+// Python binding of the C++ drop-in (physics::lattices, physics::fermionmatrix,
+// physics::algorithms::solvers) so Python tests and tools drive the same
+// classes a C++ caller of the reference would.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstring>
+
+#include "../include/physics/algorithms/solvers/solvers.hpp"
+
+namespace py = pybind11;
+using namespace physics;
+using namespace physics::lattices;
+namespace fm = physics::fermionmatrix;
+namespace sv = physics::algorithms::solvers;
+
+namespace {
+
+template<typename T>
+py::array_t<double> to_numpy(const std::vector<T>& v)
+{
+    const size_t w = sizeof(T) / sizeof(double);
+    py::array_t<double> a({v.size(), w});
+    std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(T));
+    return a;
+}
+template<typename T>
+std::vector<T> from_numpy(py::array_t<double, py::array::c_style | py::array::forcecast> a, size_t n)
+{
+    if ((size_t)a.size() != n * sizeof(T) / sizeof(double)) throw std::invalid_argument("array has the wrong size");
+    std::vector<T> v(n);
+    std::memcpy(v.data(), a.data(), n * sizeof(T));
+    return v;
+}
+
+/* owns the parameters and the System so Python keeps them alive together */
+struct Lattice {
+    explicit Lattice(const std::vector<std::string>& args)
+        : params(args), system(params), handler(params), prng_params(params), prng(system, &prng_params)
+    {
+    }
+    meta::Inputparameters params;
+    hardware::System system;
+    InterfacesHandlerImplementation handler;
+    PrngParametersImplementation prng_params;
+    PRNG prng;
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_host, m)
+{
+    m.doc() = "C++ drop-in of the CL2QCD fermion-inversion interface on the gfx950 C ABI";
+    static py::exception<sv::SolverException> exc_solver(m, "SolverException");
+    static py::exception<sv::SolverStuck> exc_stuck(m, "SolverStuck", exc_solver.ptr());
+    static py::exception<sv::SolverDidNotSolve> exc_dns(m, "SolverDidNotSolve", exc_solver.ptr());
+    static py::exception<hardware::HipException> exc_hip(m, "HipException");
+    py::register_exception_translator([](std::exception_ptr p) {
+        try {
+            if (p) std::rethrow_exception(p);
+        } catch (const sv::SolverStuck& e) {
+            PyErr_SetObject(exc_stuck.ptr(), py::make_tuple(e.what(), e.get_iterations()).ptr());
+        } catch (const sv::SolverDidNotSolve& e) {
+            PyErr_SetObject(exc_dns.ptr(), py::make_tuple(e.what(), e.get_iterations()).ptr());
+        } catch (const hardware::HipException& e) {
+            PyErr_SetObject(exc_hip.ptr(), py::make_tuple(e.what(), e.code).ptr());
+        }
+    });
+    m.attr("EVEN") = EVEN;
+    m.attr("ODD")  = ODD;
+    m.attr("F64")  = CL2_F64;
+    m.attr("F32")  = CL2_F32;
+
+    py::class_<Lattice>(m, "Lattice", "run parameters + hardware::System + InterfacesHandler + PRNG")
+        .def(py::init<const std::vector<std::string>&>(), py::arg("args") = std::vector<std::string>{})
+        .def_property_readonly("eo_sites", [](const Lattice& l) { return l.system.eo_sites(); })
+        .def_property_readonly("mubar", [](const Lattice& l) { return meta::get_mubar(l.params); })
+        .def_property_readonly("kappa", [](const Lattice& l) { return l.params.get_kappa(); })
+        .def_property_readonly("mass", [](const Lattice& l) { return l.params.get_mass(); })
+        .def_property_readonly("solver_prec", [](const Lattice& l) { return l.params.get_solver_prec(); })
+        .def("synchronize", [](const Lattice& l) { l.system.synchronize(); })
+        .def("wilson_parameters",
+             [](Lattice& l) -> const AdditionalParameters& {
+                 return l.handler.getAdditionalParameters<Spinorfield_eo>();
+             },
+             py::return_value_policy::reference_internal)
+        .def("staggered_parameters",
+             [](Lattice& l) -> const AdditionalParameters& {
+                 return l.handler.getAdditionalParameters<Staggeredfield_eo>();
+             },
+             py::return_value_policy::reference_internal)
+        .def("tune", [](Lattice& l, const std::string& key, long v) {
+            l.system.check(cl2_tune_set(l.system.ctx(), key.c_str(), v));
+        });
+
+    py::class_<AdditionalParameters>(m, "AdditionalParameters")
+        .def("getKappa", &AdditionalParameters::getKappa)
+        .def("getMubar", &AdditionalParameters::getMubar)
+        .def("getMass", &AdditionalParameters::getMass);
+    py::class_<WilsonAdditionalParameters, AdditionalParameters>(m, "WilsonAdditionalParameters")
+        .def(py::init<hmc_float, hmc_float>());
+    py::class_<StaggeredAdditionalParameters, AdditionalParameters>(m, "StaggeredAdditionalParameters")
+        .def(py::init<hmc_float>());
+
+    py::class_<Gaugefield>(m, "Gaugefield")
+        .def(py::init([](Lattice& l, bool hot) {
+                 return new Gaugefield(l.system, &l.handler.getInterface<Gaugefield>(), l.prng, hot);
+             }),
+             py::keep_alive<1, 2>(), py::arg("lattice"), py::arg("hot") = false)
+        .def(py::init([](Lattice& l, const std::string& file) {
+                 return new Gaugefield(l.system, &l.handler.getInterface<Gaugefield>(), l.prng, file);
+             }),
+             py::keep_alive<1, 2>())
+        .def("get_host", [](const Gaugefield& g) { return to_numpy(g.get_host()); })
+        .def("import_", [](const Gaugefield& g, py::array_t<double> a) {
+            auto v = from_numpy<Matrixsu3>(a, 8 * g.get_system().eo_sites());
+            g.import(v.data());
+        });
+
+    py::class_<Spinorfield>(m, "Spinorfield")
+        .def(py::init([](Lattice& l) { return new Spinorfield(l.system, l.handler.getInterface<Spinorfield>()); }),
+             py::keep_alive<1, 2>())
+        .def("get_host", [](const Spinorfield& f) { return to_numpy(f.get_host()); })
+        .def("import_", [](const Spinorfield& f, py::array_t<double> a) {
+            auto v = from_numpy<spinor>(a, f.get_elements());
+            f.import(v.data());
+        });
+
+    py::class_<Spinorfield_eo>(m, "Spinorfield_eo")
+        .def(py::init([](Lattice& l, int prec) { return new Spinorfield_eo(l.system, prec); }), py::keep_alive<1, 2>(),
+             py::arg("lattice"), py::arg("prec") = CL2_F64)
+        .def("gamma5", &Spinorfield_eo::gamma5)
+        .def("zero", &Spinorfield_eo::zero)
+        .def("cold", &Spinorfield_eo::cold)
+        .def("get_host", [](const Spinorfield_eo& f) { return to_numpy(f.get_host()); })
+        .def("import_", [](const Spinorfield_eo& f, py::array_t<double> a) {
+            auto v = from_numpy<spinor>(a, f.get_elements());
+            f.import(v.data());
+        });
+
+    py::class_<Staggeredfield_eo>(m, "Staggeredfield_eo")
+        .def(py::init([](Lattice& l, int prec) { return new Staggeredfield_eo(l.system, prec); }),
+             py::keep_alive<1, 2>(), py::arg("lattice"), py::arg("prec") = CL2_F64)
+        .def("set_zero", &Staggeredfield_eo::set_zero)
+        .def("set_cold", &Staggeredfield_eo::set_cold)
+        .def("get_host", [](const Staggeredfield_eo& f) { return to_numpy(f.get_host()); })
+        .def("import_", [](const Staggeredfield_eo& f, py::array_t<double> a) {
+            auto v = from_numpy<su3vec>(a, f.get_elements());
+            f.import(v.data());
+        });
+
+    py::class_<Scalar<hmc_complex>>(m, "ComplexScalar")
+        .def(py::init([](Lattice& l) { return new Scalar<hmc_complex>(l.system); }), py::keep_alive<1, 2>())
+        .def("store", [](const Scalar<hmc_complex>& s, std::complex<double> v) { s.store({v.real(), v.imag()}); })
+        .def("get", [](const Scalar<hmc_complex>& s) {
+            hmc_complex c = s.get();
+            return std::complex<double>(c.re, c.im);
+        });
+    py::class_<Scalar<hmc_float>>(m, "RealScalar")
+        .def(py::init([](Lattice& l) { return new Scalar<hmc_float>(l.system); }), py::keep_alive<1, 2>())
+        .def("store", &Scalar<hmc_float>::store)
+        .def("get", &Scalar<hmc_float>::get);
+    py::class_<Vector<hmc_float>>(m, "RealVector")
+        .def(py::init([](Lattice& l, size_t n) { return new Vector<hmc_float>(n, l.system); }), py::keep_alive<1, 2>())
+        .def("store", &Vector<hmc_float>::store)
+        .def("get", &Vector<hmc_float>::get);
+
+    auto cplx = [](std::complex<double> c) { return hmc_complex{c.real(), c.imag()}; };
+    // Wilson BLAS
+    m.def("copyData", [](const Spinorfield_eo& to, const Spinorfield_eo& from) { copyData(&to, from); });
+    m.def("scalar_product", [](const Spinorfield_eo& a, const Spinorfield_eo& b) {
+        hmc_complex c = scalar_product(a, b);
+        return std::complex<double>(c.re, c.im);
+    });
+    m.def("squarenorm", [](const Spinorfield_eo& a) { return squarenorm(a); });
+    m.def("saxpy", [cplx](const Spinorfield_eo& out, std::complex<double> a, const Spinorfield_eo& x,
+                          const Spinorfield_eo& y) { saxpy(&out, cplx(a), x, y); });
+    m.def("sax", [cplx](const Spinorfield_eo& out, std::complex<double> a, const Spinorfield_eo& x) {
+        sax(&out, cplx(a), x);
+    });
+    m.def("saxsbypz", [cplx](const Spinorfield_eo& out, std::complex<double> a, const Spinorfield_eo& x,
+                             std::complex<double> b, const Spinorfield_eo& y,
+                             const Spinorfield_eo& z) { saxsbypz(&out, cplx(a), x, cplx(b), y, z); });
+    m.def("saxpy_AND_gamma5_eo", [cplx](const Spinorfield_eo& out, std::complex<double> a, const Spinorfield_eo& x,
+                                        const Spinorfield_eo& y) { saxpy_AND_gamma5_eo(&out, cplx(a), x, y); });
+    m.def("saxpy_AND_squarenorm", [](const Spinorfield_eo& out, const Scalar<hmc_complex>& a, const Spinorfield_eo& x,
+                                     const Spinorfield_eo& y, const Scalar<hmc_complex>& r) {
+        saxpy_AND_squarenorm(&out, a, x, y, r);
+    });
+    m.def("convert_to_eoprec", [](const Spinorfield_eo& e, const Spinorfield_eo& o, const Spinorfield& in) {
+        convert_to_eoprec(&e, &o, in);
+    });
+    m.def("pseudo_randomize", [](const Spinorfield& f, int seed) { pseudo_randomize(&f, seed); });
+    m.def("pseudo_randomize", [](const Spinorfield_eo& f, int seed) { pseudo_randomize(&f, seed); });
+    m.def("pseudo_randomize", [](const Staggeredfield_eo& f, int seed) { pseudo_randomize(&f, seed); });
+    // staggered BLAS
+    m.def("copyData", [](const Staggeredfield_eo& to, const Staggeredfield_eo& from) { copyData(&to, from); });
+    m.def("squarenorm", [](const Staggeredfield_eo& a) { return squarenorm(a); });
+    m.def("scalar_product", [](const Staggeredfield_eo& a, const Staggeredfield_eo& b) {
+        hmc_complex c = scalar_product(a, b);
+        return std::complex<double>(c.re, c.im);
+    });
+    m.def("scalar_product_real_part",
+          [](const Staggeredfield_eo& a, const Staggeredfield_eo& b) { return scalar_product_real_part(a, b); });
+    m.def("saxpy", [cplx](const Staggeredfield_eo& out, std::complex<double> a, const Staggeredfield_eo& x,
+                          const Staggeredfield_eo& y) { saxpy(&out, cplx(a), x, y); });
+    m.def("saxpby", [cplx](const Staggeredfield_eo& out, std::complex<double> a, const Staggeredfield_eo& x,
+                           std::complex<double> b, const Staggeredfield_eo& y) { saxpby(&out, cplx(a), x, cplx(b), y); });
+    m.def("sax", [cplx](const Staggeredfield_eo& out, std::complex<double> a, const Staggeredfield_eo& x) {
+        sax(&out, cplx(a), x);
+    });
+
+    // explicit kernels
+    m.def("dslash", [](const Spinorfield_eo& out, const Gaugefield& gf, const Spinorfield_eo& in, int eo,
+                       hmc_float kappa) { fm::dslash(&out, gf, in, eo, kappa); });
+    m.def("M_tm_inverse_sitediagonal", [](const Spinorfield_eo& o, const Spinorfield_eo& i, hmc_float mb) {
+        fm::M_tm_inverse_sitediagonal(&o, i, mb);
+    });
+    m.def("M_tm_sitediagonal",
+          [](const Spinorfield_eo& o, const Spinorfield_eo& i, hmc_float mb) { fm::M_tm_sitediagonal(&o, i, mb); });
+    m.def("M_tm_inverse_sitediagonal_minus", [](const Spinorfield_eo& o, const Spinorfield_eo& i, hmc_float mb) {
+        fm::M_tm_inverse_sitediagonal_minus(&o, i, mb);
+    });
+    m.def("M_tm_sitediagonal_minus", [](const Spinorfield_eo& o, const Spinorfield_eo& i, hmc_float mb) {
+        fm::M_tm_sitediagonal_minus(&o, i, mb);
+    });
+    m.def("dslash_AND_M_tm_inverse_sitediagonal",
+          [](const Spinorfield_eo& o, const Gaugefield& gf, const Spinorfield_eo& i, int eo, hmc_float k, hmc_float mb) {
+              fm::dslash_AND_M_tm_inverse_sitediagonal(&o, gf, i, eo, k, mb);
+          });
+    m.def("dslash_AND_M_tm_inverse_sitediagonal_minus",
+          [](const Spinorfield_eo& o, const Gaugefield& gf, const Spinorfield_eo& i, int eo, hmc_float k, hmc_float mb) {
+              fm::dslash_AND_M_tm_inverse_sitediagonal_minus(&o, gf, i, eo, k, mb);
+          });
+    m.def("DKS_eo", [](const Staggeredfield_eo& out, const Gaugefield& gf, const Staggeredfield_eo& in, int eo) {
+        fm::DKS_eo(&out, gf, in, eo);
+    });
+
+    py::class_<fm::Fermionmatrix_eo>(m, "Fermionmatrix_eo")
+        .def("__call__", [](const fm::Fermionmatrix_eo& A, const Spinorfield_eo& out, const Gaugefield& gf,
+                            const Spinorfield_eo& in, const AdditionalParameters& ap) { A(&out, gf, in, ap); })
+        .def("isHermitian", &fm::Fermionmatrix_eo::isHermitian)
+        .def("get_flops", &fm::Fermionmatrix_eo::get_flops)
+        .def("get_read_write_size", &fm::Fermionmatrix_eo::get_read_write_size);
+#define CL2_BIND_EO(NAME)                                                                                  \
+    py::class_<fm::NAME, fm::Fermionmatrix_eo>(m, #NAME)                                                   \
+        .def(py::init([](Lattice& l) { return new fm::NAME(l.system, l.handler.getInterface<fm::NAME>()); }), \
+             py::keep_alive<1, 2>());
+    CL2_BIND_EO(Aee)
+    CL2_BIND_EO(Aee_AND_gamma5_eo)
+    CL2_BIND_EO(Aee_minus)
+    CL2_BIND_EO(Aee_minus_AND_gamma5_eo)
+    CL2_BIND_EO(Qplus_eo)
+    CL2_BIND_EO(Qminus_eo)
+    CL2_BIND_EO(QplusQminus_eo)
+#undef CL2_BIND_EO
+
+    py::class_<fm::Fermionmatrix_stagg_eo>(m, "Fermionmatrix_stagg_eo")
+        .def("__call__",
+             [](const fm::Fermionmatrix_stagg_eo& A, const Staggeredfield_eo& out, const Gaugefield& gf,
+                const Staggeredfield_eo& in, const AdditionalParameters* ap) { A(&out, gf, in, ap); },
+             py::arg("out"), py::arg("gf"), py::arg("in"), py::arg("ap") = nullptr)
+        .def("get_flops", &fm::Fermionmatrix_stagg_eo::get_flops)
+        .def("get_read_write_size", &fm::Fermionmatrix_stagg_eo::get_read_write_size);
+    py::class_<fm::D_KS_eo, fm::Fermionmatrix_stagg_eo>(m, "D_KS_eo")
+        .def(py::init([](Lattice& l, int eo) {
+                 return new fm::D_KS_eo(l.system, l.handler.getInterface<fm::D_KS_eo>(), eo != 0);
+             }),
+             py::keep_alive<1, 2>());
+    py::class_<fm::MdagM_eo, fm::Fermionmatrix_stagg_eo>(m, "MdagM_eo")
+        .def(py::init([](Lattice& l, int ul) {
+                 return new fm::MdagM_eo(l.system, l.handler.getInterface<fm::MdagM_eo>(), ul != 0);
+             }),
+             py::keep_alive<1, 2>(), py::arg("lattice"), py::arg("upper_left") = EVEN);
+
+    m.def("cg", [](const Spinorfield_eo& x, const fm::Fermionmatrix_eo& A, const Gaugefield& gf,
+                   const Spinorfield_eo& b, Lattice& l, hmc_float prec, const AdditionalParameters& ap) {
+        py::gil_scoped_release rel;
+        return sv::cg(&x, A, gf, b, l.system, l.handler, prec, ap);
+    });
+    m.def("bicgstab", [](const Spinorfield_eo& x, const fm::Fermionmatrix_eo& A, const Gaugefield& gf,
+                         const Spinorfield_eo& b, Lattice& l, hmc_float prec, const AdditionalParameters& ap) {
+        py::gil_scoped_release rel;
+        return sv::bicgstab(&x, A, gf, b, l.system, l.handler, prec, ap);
+    });
+    m.def("cg_m", [](std::vector<Staggeredfield_eo*> xs, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf,
+                     std::vector<hmc_float> sigma, const Staggeredfield_eo& b, Lattice& l, hmc_float prec,
+                     const AdditionalParameters& ap) {
+        std::vector<std::shared_ptr<Staggeredfield_eo>> x;
+        for (auto* p : xs) x.emplace_back(p, [](Staggeredfield_eo*) {});  // Python owns the fields
+        py::gil_scoped_release rel;
+        return sv::cg_m(x, A, gf, sigma, b, l.system, l.handler, prec, ap);
+    });
+}

@@ -1,0 +1,237 @@
+// Fermion matrices of the drop-in (physics/fermionmatrix/fermionmatrix.cpp:224-676,
+// explicit.cpp:26-222, fermionmatrix_stagg.cpp:50-114, explicit_stagg.cpp:26-45).
+#include <stdexcept>
+
+#include "../include/physics/fermionmatrix/fermionmatrix.hpp"
+
+namespace physics {
+namespace fermionmatrix {
+
+using lattices::Gaugefield;
+using lattices::Spinorfield_eo;
+using lattices::Staggeredfield_eo;
+
+namespace {
+
+/* gauge pointer and format matching the spinor precision of an application */
+struct Links {
+    const void* ptr;
+    int fmt;
+};
+Links links_for(const Gaugefield& gf, int prec)
+{
+    if (prec == CL2_F32) return Links{gf.dev_f32(), CL2_GAUGE_F32_12};
+    return Links{gf.dev(), gf.format()};
+}
+
+void check_parity(int evenodd)
+{
+    if (evenodd != EVEN && evenodd != ODD) throw std::invalid_argument("evenodd must be EVEN or ODD");
+}
+
+void sitediag(const Spinorfield_eo* out, const Spinorfield_eo& in, hmc_float mubar, int variant)
+{
+    if (out->precision() != in.precision()) throw std::invalid_argument("fields of different precision");
+    in.get_system().check(
+        cl2_M_tm_sitediagonal_eo(in.get_system().ctx(), out->dev(), in.dev(), in.precision(), mubar, variant));
+}
+
+/* per-application models of the reference (hardware/code/fermions.cpp:811-975,
+ * spinors.cpp, fermions_staggered.cpp:175-278), FP64 with 18-real links */
+constexpr cl_ulong DSLASH_FLOPS = 1632, DSLASH_BYTES = 2880, DIAG_FLOPS = 72, DIAG_INV_FLOPS = 96,
+                   DIAG_BYTES = 384, SAXPY_FLOPS = 96, SAXPY_BYTES = 608, G5SAXPY_FLOPS = 108,
+                   G5SAXPY_BYTES = 592, G5_FLOPS = 12, G5_BYTES = 384, DKS_FLOPS = 570, DKS_BYTES = 1584,
+                   SAXPBY_STAGG_FLOPS = 42, SAXPBY_STAGG_BYTES = 144;
+
+}  // namespace
+
+/* ---------------- explicit kernels ---------------- */
+
+void dslash(const Spinorfield_eo* out, const Gaugefield& gf, const Spinorfield_eo& in, int evenodd, hmc_float kappa)
+{
+    check_parity(evenodd);
+    if (out->precision() != in.precision()) throw std::invalid_argument("fields of different precision");
+    const Links l = links_for(gf, in.precision());
+    in.get_system().check(cl2_dslash_eo(in.get_system().ctx(), out->dev(), in.dev(), l.ptr, l.fmt, evenodd, kappa));
+}
+void M_tm_inverse_sitediagonal(const Spinorfield_eo* out, const Spinorfield_eo& in, hmc_float mubar)
+{
+    sitediag(out, in, mubar, CL2_MTM_INVERSE_SITEDIAGONAL);
+}
+void M_tm_sitediagonal(const Spinorfield_eo* out, const Spinorfield_eo& in, hmc_float mubar)
+{
+    sitediag(out, in, mubar, CL2_MTM_SITEDIAGONAL);
+}
+void M_tm_inverse_sitediagonal_minus(const Spinorfield_eo* out, const Spinorfield_eo& in, hmc_float mubar)
+{
+    sitediag(out, in, mubar, CL2_MTM_INVERSE_SITEDIAGONAL_MINUS);
+}
+void M_tm_sitediagonal_minus(const Spinorfield_eo* out, const Spinorfield_eo& in, hmc_float mubar)
+{
+    sitediag(out, in, mubar, CL2_MTM_SITEDIAGONAL_MINUS);
+}
+void M_tm_sitediagonal_AND_gamma5_eo(const Spinorfield_eo* out, const Spinorfield_eo& in, hmc_float mubar)
+{
+    sitediag(out, in, mubar, CL2_MTM_SITEDIAGONAL_AND_GAMMA5);
+}
+void M_tm_sitediagonal_minus_AND_gamma5_eo(const Spinorfield_eo* out, const Spinorfield_eo& in, hmc_float mubar)
+{
+    sitediag(out, in, mubar, CL2_MTM_SITEDIAGONAL_MINUS_AND_GAMMA5);
+}
+static void dslash_minv(const Spinorfield_eo* out, const Gaugefield& gf, const Spinorfield_eo& in, int evenodd,
+                        hmc_float kappa, hmc_float mubar, int minus)
+{
+    check_parity(evenodd);
+    if (out->precision() != in.precision()) throw std::invalid_argument("fields of different precision");
+    const Links l = links_for(gf, in.precision());
+    in.get_system().check(cl2_dslash_AND_M_tm_inverse_sitediagonal_eo(in.get_system().ctx(), out->dev(), in.dev(),
+                                                                      l.ptr, l.fmt, evenodd, kappa, mubar, minus));
+}
+void dslash_AND_M_tm_inverse_sitediagonal(const Spinorfield_eo* out, const Gaugefield& gf, const Spinorfield_eo& in,
+                                          int evenodd, hmc_float kappa, hmc_float mubar)
+{
+    dslash_minv(out, gf, in, evenodd, kappa, mubar, 0);
+}
+void dslash_AND_M_tm_inverse_sitediagonal_minus(const Spinorfield_eo* out, const Gaugefield& gf,
+                                                const Spinorfield_eo& in, int evenodd, hmc_float kappa,
+                                                hmc_float mubar)
+{
+    dslash_minv(out, gf, in, evenodd, kappa, mubar, 1);
+}
+void DKS_eo(const Staggeredfield_eo* out, const Gaugefield& gf, const Staggeredfield_eo& in, int evenodd)
+{
+    check_parity(evenodd);
+    if (out->precision() != in.precision()) throw std::invalid_argument("fields of different precision");
+    const Links l = links_for(gf, in.precision());
+    in.get_system().check(cl2_D_KS_eo(in.get_system().ctx(), out->dev(), in.dev(), l.ptr, l.fmt, evenodd));
+}
+
+/* ---------------- Wilson / tm eo operators ---------------- */
+
+BuiltinEoOperator::BuiltinEoOperator(int op, bool herm, const hardware::System& system,
+                                     const FermionEoParametersInterface& p)
+    : Fermionmatrix_eo(herm, system, p), op(op),
+      tmp(system, 2 * cl2_field_bytes(system.ctx(), CL2_FIELD_SPINOR_EO, CL2_F64))
+{
+}
+
+void BuiltinEoOperator::operator()(const Spinorfield_eo* out, const Gaugefield& gf, const Spinorfield_eo& in,
+                                   const physics::AdditionalParameters& ap) const
+{
+    const common::action a = fermionmatrixParametersInterface.getFermionicActionType();
+    int abi_action;
+    if (a == common::wilson)
+        abi_action = CL2_ACTION_WILSON;
+    else if (a == common::twistedmass)
+        abi_action = CL2_ACTION_TWISTEDMASS;
+    else
+        throw std::invalid_argument("Unkown fermion action! wilson or twistedmass");
+    if (out->precision() != in.precision()) throw std::invalid_argument("fields of different precision");
+    const hmc_float kappa = ap.getKappa();
+    const hmc_float mubar = abi_action == CL2_ACTION_TWISTEDMASS ? ap.getMubar() : 0.;
+    const Links l         = links_for(gf, in.precision());
+    get_system().check(cl2_fermionmatrix_eo(get_system().ctx(), op, out->dev(), in.dev(), l.ptr, l.fmt, abi_action,
+                                            kappa, mubar, tmp.ptr()));
+}
+
+cl_ulong BuiltinEoOperator::get_flops() const
+{
+    const cl_ulong S = get_system().eo_sites();
+    const bool tm    = fermionmatrixParametersInterface.getFermionicActionType() == common::twistedmass;
+    const bool merged = fermionmatrixParametersInterface.useMergedFermionicKernels();
+    auto aee = [&](bool g5) {
+        cl_ulong f = 2 * DSLASH_FLOPS + (g5 ? G5SAXPY_FLOPS : SAXPY_FLOPS);
+        if (tm) f += DIAG_FLOPS + DIAG_INV_FLOPS;
+        return f;
+    };
+    auto q = [&]() { return merged ? aee(true) : aee(false) + G5_FLOPS; };
+    switch (op) {
+        case CL2_OP_AEE:
+        case CL2_OP_AEE_MINUS: return S * aee(false);
+        case CL2_OP_AEE_AND_GAMMA5:
+        case CL2_OP_AEE_MINUS_AND_GAMMA5: return S * aee(true);
+        case CL2_OP_QPLUS_EO:
+        case CL2_OP_QMINUS_EO: return S * q();
+        default: return 2 * S * q();
+    }
+}
+
+cl_ulong BuiltinEoOperator::get_read_write_size() const
+{
+    const cl_ulong S = get_system().eo_sites();
+    const bool tm    = fermionmatrixParametersInterface.getFermionicActionType() == common::twistedmass;
+    const bool merged = fermionmatrixParametersInterface.useMergedFermionicKernels();
+    auto aee = [&](bool g5) {
+        cl_ulong b = 2 * DSLASH_BYTES + (g5 ? G5SAXPY_BYTES : SAXPY_BYTES);
+        if (tm) b += 2 * DIAG_BYTES;
+        return b;
+    };
+    auto q = [&]() { return merged ? aee(true) : aee(false) + G5_BYTES; };
+    switch (op) {
+        case CL2_OP_AEE:
+        case CL2_OP_AEE_MINUS: return S * aee(false);
+        case CL2_OP_AEE_AND_GAMMA5:
+        case CL2_OP_AEE_MINUS_AND_GAMMA5: return S * aee(true);
+        case CL2_OP_QPLUS_EO:
+        case CL2_OP_QMINUS_EO: return S * q();
+        default: return 2 * S * q();
+    }
+}
+
+/* ---------------- staggered ---------------- */
+
+hmc_float Fermionmatrix_stagg_eo::getThresholdForMinimumEigenvalue(hmc_float) const
+{
+    throw std::logic_error("Threshold for minimum eigenvalue not existing or not implemented!");
+}
+
+void D_KS_eo::operator()(const Staggeredfield_eo* out, const Gaugefield& gf, const Staggeredfield_eo& in,
+                         const physics::AdditionalParameters* ap) const
+{
+    if (ap != nullptr)
+        throw std::invalid_argument(
+            "D_KS_eo operator applied passing to it some additional parameters! This should not happen!");
+    DKS_eo(out, gf, in, evenodd);
+}
+cl_ulong D_KS_eo::get_flops() const
+{
+    return get_system().eo_sites() * DKS_FLOPS;
+}
+cl_ulong D_KS_eo::get_read_write_size() const
+{
+    return get_system().eo_sites() * DKS_BYTES;
+}
+
+MdagM_eo::MdagM_eo(const hardware::System& system, const FermionStaggeredEoParametersInterface& p, bool upper_left)
+    : Fermionmatrix_stagg_eo(true, system, p), upper_left(upper_left), tmp(system)
+{
+}
+
+void MdagM_eo::operator()(const Staggeredfield_eo* out, const Gaugefield& gf, const Staggeredfield_eo& in,
+                          const physics::AdditionalParameters* ap) const
+{
+    if (ap == nullptr)
+        throw std::invalid_argument("MdagM_eo operator applied without the MANDATORY additional parameters!");
+    if (out->precision() != in.precision()) throw std::invalid_argument("fields of different precision");
+    const hmc_float mass = ap->getMass();
+    const Links l        = links_for(gf, in.precision());
+    if (in.precision() == tmp.precision()) {
+        get_system().check(
+            cl2_MdagM_eo(get_system().ctx(), out->dev(), in.dev(), l.ptr, l.fmt, mass, upper_left, tmp.dev()));
+    } else {
+        Staggeredfield_eo t(get_system(), in.precision());
+        get_system().check(
+            cl2_MdagM_eo(get_system().ctx(), out->dev(), in.dev(), l.ptr, l.fmt, mass, upper_left, t.dev()));
+    }
+}
+cl_ulong MdagM_eo::get_flops() const
+{
+    return get_system().eo_sites() * (2 * DKS_FLOPS + SAXPBY_STAGG_FLOPS);
+}
+cl_ulong MdagM_eo::get_read_write_size() const
+{
+    return get_system().eo_sites() * (2 * DKS_BYTES + SAXPBY_STAGG_BYTES);
+}
+
+}  // namespace fermionmatrix
+}  // namespace physics

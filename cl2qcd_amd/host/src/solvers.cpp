@@ -1,0 +1,306 @@
+// cg, bicgstab and cg_m of the drop-in: the reference's algorithms and stopping
+// rules (physics/algorithms/solvers/cg.cpp:277-400, bicgstab.cpp:401-733,
+// physics/algorithms/solver_shifted.cpp:114-475) on device-resident fields and
+// device scalars; the host reads a scalar only where the reference does.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "../include/physics/algorithms/solvers/solvers.hpp"
+
+namespace physics {
+namespace algorithms {
+namespace solvers {
+
+using namespace physics::lattices;
+
+namespace {
+
+bool is_nan(hmc_float v)
+{
+    return v != v;
+}
+
+void debug(const char* fmt, int iter, double v)
+{
+    if (std::getenv("CL2QCD_DEBUG")) std::fprintf(stderr, fmt, iter, v);
+}
+
+}  // namespace
+
+int cg(const Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& f, const Gaugefield& gf,
+       const Spinorfield_eo& b, const hardware::System& system, physics::InterfacesHandler& interfacesHandler,
+       hmc_float prec, const physics::AdditionalParameters& additionalParameters)
+{
+    const SolversParametersInterface& par = interfacesHandler.getSolversParametersInterface();
+    const int cgmax = par.getCgMax(), refresh = par.getIterRefresh(), check = par.getCgIterationBlockSize();
+    const int min_iter = par.getCgMinimumIterationCount();
+    const bool merged  = par.getUseMergeKernelsSpinor();
+    const int prec_f   = b.precision();
+
+    const Spinorfield_eo p(system, prec_f), rn(system, prec_f), v(system, prec_f);
+    const Scalar<hmc_complex> alpha(system), beta(system), omega(system), rho(system), rho_next(system),
+        tmp1(system), tmp2(system), one(system), minus_one(system);
+    one.store(hmc_complex_one);
+    minus_one.store(hmc_complex_minusone);
+
+    int iter = 0;
+    hmc_float resid = 0.;
+    for (iter = 0; iter < cgmax || iter < min_iter; iter++) {
+        if (iter % refresh == 0) {
+            f(&rn, gf, *x, additionalParameters);  // rn = A x
+            saxpy(&rn, one, rn, b);                // rn = b - A x
+            copyData(&p, rn);
+            scalar_product(&omega, rn, rn);
+        } else {
+            copyData(&omega, rho_next);
+        }
+        f(&v, gf, p, additionalParameters);  // v = A p
+        scalar_product(&rho, p, v);
+        divide(&alpha, omega, rho);  // alpha = (r,r)/(p,Ap)
+        multiply(&tmp1, minus_one, alpha);
+        saxpy(x, tmp1, p, *x);  // x = x + alpha p
+        if (merged) {
+            saxpy_AND_squarenorm(&rn, alpha, v, rn, rho_next);
+        } else {
+            saxpy(&rn, alpha, v, rn);  // rn = rn - alpha v
+            scalar_product(&rho_next, rn, rn);
+        }
+        if (iter % check == 0) {
+            resid = rho_next.get().re;
+            debug("CG [%d] resid %.16g\n", iter, resid);
+            if (is_nan(resid)) throw SolverStuck(iter, "cg: NAN occured");
+            if (resid < prec && iter >= min_iter) return iter;
+        }
+        divide(&beta, rho_next, omega);
+        multiply(&tmp2, minus_one, beta);
+        saxpy(&p, tmp2, p, rn);  // p = rn + beta p
+    }
+    throw SolverDidNotSolve(iter, "cg");
+}
+
+namespace {
+
+int bicgstab_save(const Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& f, const Gaugefield& gf,
+                  const Spinorfield_eo& b, const hardware::System& system, const SolversParametersInterface& par,
+                  hmc_float prec, const physics::AdditionalParameters& ap)
+{
+    const int pf = b.precision();
+    const Spinorfield_eo s(system, pf), t(system, pf), v(system, pf), p(system, pf), rn(system, pf),
+        rhat(system, pf), aux(system, pf);
+    const Scalar<hmc_complex> alpha(system), beta(system), omega(system), rho(system), rho_next(system),
+        tmp1(system), tmp2(system), one(system), minus_one(system);
+    one.store(hmc_complex_one);
+    minus_one.store(hmc_complex_minusone);
+    const int cgmax = par.getCgMax();
+    hmc_float resid = 1.;
+    int iter        = 0;
+    for (iter = 0; iter < cgmax; iter++) {
+        if (iter % par.getIterRefresh() == 0) {
+            v.zero();
+            p.zero();
+            f(&rn, gf, *x, ap);
+            saxpy(&rn, one, rn, b);
+            copyData(&rhat, rn);
+            copyData(&alpha, one);
+            copyData(&omega, one);
+            copyData(&rho, one);
+        }
+        scalar_product(&rho_next, rhat, rn);
+        const hmc_complex test = rho_next.get();
+        if (std::fabs(test.re) < 1e-25 && std::fabs(test.im) < 1e-25) throw SolverStuck(iter, "bicgstab_save");
+        divide(&tmp1, rho_next, rho);
+        copyData(&rho, rho_next);
+        divide(&tmp2, alpha, omega);
+        multiply(&beta, tmp1, tmp2);
+        multiply(&tmp1, beta, omega);
+        multiply(&tmp2, minus_one, tmp1);
+        saxsbypz(&p, beta, p, tmp2, v, rn);  // p = beta p - beta omega v + rn
+        f(&v, gf, p, ap);
+        scalar_product(&tmp1, rhat, v);
+        divide(&alpha, rho, tmp1);
+        saxpy(&s, alpha, v, rn);  // s = rn - alpha v
+        f(&t, gf, s, ap);
+        scalar_product(&tmp1, t, s);
+        scalar_product(&tmp2, t, t);
+        divide(&omega, tmp1, tmp2);
+        saxpy(&rn, omega, t, s);               // rn = s - omega t
+        saxsbypz(x, alpha, p, omega, s, *x);  // x = alpha p + omega s + x
+        resid = squarenorm(rn);
+        debug("BiCGstab_save [%d] resid %.16g\n", iter, resid);
+        if (is_nan(resid)) throw SolverStuck(iter, "bicgstab_save: NAN occured");
+        if (resid < prec) {
+            f(&aux, gf, *x, ap);
+            saxpy(&aux, one, aux, b);
+            const hmc_float trueresid = squarenorm(aux);
+            if (trueresid < prec) return iter;
+        }
+    }
+    throw SolverDidNotSolve(iter, "bicgstab_save");
+}
+
+int bicgstab_fast(const Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& f, const Gaugefield& gf,
+                  const Spinorfield_eo& b, const hardware::System& system, const SolversParametersInterface& par,
+                  hmc_float prec, const physics::AdditionalParameters& ap)
+{
+    const int pf = b.precision();
+    const Spinorfield_eo p(system, pf), rn(system, pf), rhat(system, pf), s(system, pf), t(system, pf),
+        v(system, pf);
+    const Scalar<hmc_complex> alpha(system), beta(system), omega(system), rho(system), rho_next(system),
+        tmp1(system), tmp2(system), one(system), minus_one(system);
+    one.store(hmc_complex_one);
+    minus_one.store(hmc_complex_minusone);
+    hmc_float resid = NAN;
+    int iter        = 0;
+    for (iter = 0; iter < par.getCgMax(); iter++) {
+        if (iter % par.getIterRefresh() == 0) {
+            f(&rn, gf, *x, ap);
+            saxpy(&p, one, rn, b);
+            copyData(&rhat, p);
+            copyData(&rn, p);
+            scalar_product(&rho, rhat, rn);
+        }
+        resid = squarenorm(rn);
+        debug("BiCGstab [%d] resid %.16g\n", iter, resid);
+        if (is_nan(resid)) throw SolverStuck(iter, "bicgstab: NAN occured");
+        if (resid < prec) return iter;
+        f(&v, gf, p, ap);
+        scalar_product(&tmp1, rhat, v);
+        divide(&alpha, rho, tmp1);
+        saxpy(&s, alpha, v, rn);
+        f(&t, gf, s, ap);
+        scalar_product(&tmp1, t, s);
+        scalar_product(&tmp2, t, t);
+        divide(&omega, tmp1, tmp2);
+        saxsbypz(x, alpha, p, omega, s, *x);
+        saxpy(&rn, omega, t, s);
+        scalar_product(&rho_next, rhat, rn);
+        const hmc_complex test = rho_next.get();
+        if (std::fabs(test.re) < 1e-25 && std::fabs(test.im) < 1e-25) throw SolverStuck(iter, "bicgstab");
+        divide(&tmp1, rho_next, rho);
+        divide(&tmp2, alpha, omega);
+        multiply(&beta, tmp1, tmp2);
+        multiply(&tmp1, beta, omega);
+        multiply(&tmp2, minus_one, tmp1);
+        saxsbypz(&p, beta, p, tmp2, v, rn);
+        copyData(&rho, rho_next);
+    }
+    throw SolverDidNotSolve(iter, "bicgstab");
+}
+
+}  // namespace
+
+int bicgstab(const Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& A, const Gaugefield& gf,
+             const Spinorfield_eo& b, const hardware::System& system, physics::InterfacesHandler& interfacesHandler,
+             hmc_float prec, const physics::AdditionalParameters& additionalParameters)
+{
+    const SolversParametersInterface& par = interfacesHandler.getSolversParametersInterface();
+    if (par.getSolver() == common::bicgstab_save)
+        return bicgstab_save(x, A, gf, b, system, par, prec, additionalParameters);
+    return bicgstab_fast(x, A, gf, b, system, par, prec, additionalParameters);
+}
+
+/* SolverShifted::solve (solver_shifted.cpp:428-475) */
+int cg_m(std::vector<std::shared_ptr<Staggeredfield_eo>> x, const fermionmatrix::Fermionmatrix_stagg_eo& A,
+         const Gaugefield& gf, const std::vector<hmc_float> sigma, const Staggeredfield_eo& b,
+         const hardware::System& system, physics::InterfacesHandler& interfacesHandler, hmc_float prec,
+         const physics::AdditionalParameters& additionalParameters)
+{
+    if (sigma.size() != x.size()) throw std::invalid_argument("Wrong size of multi-shifted inverter parameters!");
+    const SolversParametersInterface& par = interfacesHandler.getSolversParametersInterface();
+    const int n      = (int)sigma.size();
+    const int pf     = b.precision();
+    const bool merged = par.getUseMergeKernelsSpinor();
+    const int min_iter = par.getCgMinimumIterationCount();
+    if (squarenorm(b) == 0) {
+        for (auto& xi : x) xi->set_zero();
+        return 0;
+    }
+    const Staggeredfield_eo r(system, pf), p(system, pf), v(system, pf);
+    std::vector<std::unique_ptr<Staggeredfield_eo>> ps;
+    for (int k = 0; k < n; ++k) ps.emplace_back(new Staggeredfield_eo(system, pf));
+    const Vector<hmc_float> zeta_prev(n, system), zeta(n, system), zeta_foll(n, system), alpha_vec(n, system),
+        beta_vec(n, system), shift(n, system), single_eq_resid(n, system);
+    const Scalar<hmc_float> alpha_scalar_prev(system), alpha_scalar(system), beta_scalar_prev(system),
+        beta_scalar(system), zero(system), tmp1(system), tmp2(system), tmp3(system);
+    std::vector<bool> converged(n, false);
+    int nconv = 0;
+
+    // setInitialConditions (solver_shifted.cpp:114-134)
+    zero.store(0.0);
+    zeta_prev.store(std::vector<hmc_float>(n, 1.));
+    zeta.store(std::vector<hmc_float>(n, 1.));
+    alpha_vec.store(std::vector<hmc_float>(n, 0.));
+    shift.store(sigma);
+    for (int k = 0; k < n; ++k) {
+        x[k]->set_zero();
+        copyData(ps[k].get(), b);
+    }
+    copyData(&r, b);
+    copyData(&p, b);
+    scalar_product_real_part(&tmp1, r, r);
+    beta_scalar.store(1.0);
+    alpha_scalar.store(0.0);
+
+    int iter = 0;
+    while (iter < par.getCgMax() || iter < min_iter) {
+        // updateBetaScalar
+        copyData(&beta_scalar_prev, beta_scalar);
+        A(&v, gf, p, &additionalParameters);
+        scalar_product_real_part(&tmp3, p, v);
+        divide(&beta_scalar, tmp1, tmp3);
+        subtract(&beta_scalar, zero, beta_scalar);
+        // updateAuxiliaryFieldR: r = beta v + r
+        saxpy(&r, beta_scalar, v, r);
+        // updateAlphaScalar
+        scalar_product_real_part(&tmp2, r, r);
+        copyData(&alpha_scalar_prev, alpha_scalar);
+        divide(&alpha_scalar, tmp2, tmp1);
+        // updateAuxiliaryFieldP: p = alpha p + r
+        saxpy(&p, alpha_scalar, p, r);
+        // updateVectorQuantities
+        update_zeta_cgm(&zeta_foll, zeta, zeta_prev, beta_scalar_prev, beta_scalar, alpha_scalar_prev, shift, n);
+        update_beta_cgm(&beta_vec, beta_scalar, zeta_foll, zeta, n);
+        update_alpha_cgm(&alpha_vec, alpha_scalar, zeta_foll, beta_vec, zeta, beta_scalar, n);
+        const bool check = iter % par.getCgIterationBlockSize() == 0;
+        std::vector<hmc_float> resid_host;
+        if (merged) {
+            sax_vec_and_squarenorm(&single_eq_resid, zeta_foll, r);
+            resid_host = single_eq_resid.get();
+        }
+        for (int k = 0; k < n; ++k) {
+            if (converged[k]) continue;
+            saxpy(x[k].get(), beta_vec, k, *ps[k], *x[k]);               // x_k = beta_k ps_k + x_k
+            saxpby(ps[k].get(), zeta_foll, k, r, alpha_vec, k, *ps[k]);  // ps_k = zeta_k r + alpha_k ps_k
+            if (check) {
+                hmc_float res_k;
+                if (merged) {
+                    res_k = resid_host[k];
+                } else {
+                    sax(&v, zeta_foll, k, r);
+                    res_k = squarenorm(v);
+                }
+                if (res_k < prec) {
+                    converged[k] = true;
+                    ++nconv;
+                }
+            }
+        }
+        // updateQuantitiesForFollowingIteration
+        copyData(&zeta_prev, zeta);
+        copyData(&zeta, zeta_foll);
+        copyData(&tmp1, tmp2);
+        if (nconv == n) {
+            const hmc_float resid = tmp2.get();
+            debug("CG-M [%d] resid %.16g\n", iter, resid);
+            if (is_nan(resid)) throw SolverStuck(iter, "cg_m: NAN occured");
+            if (resid < prec && iter >= min_iter) return iter;
+        }
+        ++iter;
+    }
+    throw SolverDidNotSolve(iter, "cg_m");
+}
+
+}  // namespace solvers
+}  // namespace algorithms
+}  // namespace physics
