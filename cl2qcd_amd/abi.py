@@ -79,6 +79,9 @@ SIGNATURES = {
     "cl2_su3vec_export": (_i, [_vp, _dp, _vp, _i]),
     "cl2_gauge_import": (_i, [_vp, _vp, _dp, _i]),
     "cl2_gauge_set_random": (_i, [_vp, _vp, _i, ctypes.c_uint64, _d]),
+    "cl2_gauge_heatbath": (_i, [_vp, _vp, _d, _i, _i, ctypes.c_uint64]),
+    "cl2_gauge_overrelax": (_i, [_vp, _vp, _i]),
+    "cl2_gauge_plaquette": (_i, [_vp, _vp, ctypes.POINTER(ctypes.c_double)]),
     "cl2_gauge_convert": (_i, [_vp, _vp, _i, _vp]),
     "cl2_gauge_export": (_i, [_vp, _dp, _vp, _i]),
     "cl2_scalar_store": (_i, [_vp, _dp, _dp, _i]),
@@ -98,6 +101,7 @@ SIGNATURES = {
     "cl2_saxpy_AND_squarenorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
     "cl2_set_zero_eo": (_i, [_vp, _vp, _i]),
     "cl2_set_cold_eo": (_i, [_vp, _vp, _i]),
+    "cl2_convert_precision_eo": (_i, [_vp, _i, _vp, _i, _vp, _i]),
     "cl2_D_KS_eo": (_i, [_vp, _vp, _vp, _vp, _i, _i]),
     "cl2_MdagM_eo": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, _vp]),
     "cl2_saxpy_stagg_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP]),
@@ -216,6 +220,15 @@ class Gauge(DeviceBuffer):
         out = np.empty((8 * self.ctx.veo, 18))
         self.ctx.check(self.ctx.lib.cl2_gauge_export(self.ctx.handle, out.ctypes.data, self.ptr, self.fmt))
         return out
+
+    def heatbath(self, beta, sweeps, overrelax=0, seed=0):
+        assert self.fmt == GAUGE_F64_18
+        self.ctx.check(self.ctx.lib.cl2_gauge_heatbath(self.ctx.handle, self.ptr, beta, sweeps, overrelax, seed))
+
+    def plaquette(self):
+        p = ctypes.c_double()
+        self.ctx.check(self.ctx.lib.cl2_gauge_plaquette(self.ctx.handle, self.ptr, ctypes.byref(p)))
+        return p.value
 
     def converted(self, fmt):
         """A copy of this F64_18 field in another device format."""

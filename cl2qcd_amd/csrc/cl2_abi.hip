@@ -251,6 +251,7 @@ int cl2_ctx_create(cl2_ctx** out, int device, const cl2_lattice* lat)
     c->npartials = std::max<size_t>(REDUCE_MAX_BLOCKS, blocks_for(c->g.veo, DSLASH_BLOCK));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, c->npartials * 2 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
         hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
         cl2_ctx_destroy(c);
         return create_fail(CL2_E_HIP, "context allocation failed");
@@ -265,6 +266,7 @@ int cl2_ctx_destroy(cl2_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->partials) (void)hipFree(c->partials);
+    if (c->dscratch) (void)hipFree(c->dscratch);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -463,6 +465,41 @@ int cl2_gauge_set_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, double e
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
     CL2_REQUIRE(c, epsilon >= 0. && std::isfinite(epsilon), "epsilon must be finite and >= 0");
     CL2_HIP(c, launch_gauge_random(c, dev, fmt, seed, epsilon));
+    return CL2_OK;
+}
+
+int cl2_gauge_heatbath(cl2_ctx* c, void* gauge, double beta, int n_sweeps, int n_overrelax, uint64_t seed)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, gauge != nullptr, "null pointer");
+    CL2_REQUIRE(c, beta > 0. && std::isfinite(beta), "beta must be positive");
+    CL2_REQUIRE(c, n_sweeps >= 0 && n_overrelax >= 0, "negative sweep count");
+    for (int s = 0; s < n_sweeps; ++s) {
+        const uint64_t key = seed * 1000003ull + (uint64_t)s * (n_overrelax + 1);
+        CL2_HIP(c, launch_gauge_update(c, gauge, beta, 0, key));
+        for (int o = 0; o < n_overrelax; ++o) CL2_HIP(c, launch_gauge_update(c, gauge, beta, 1, key + 1 + o));
+    }
+    return CL2_OK;
+}
+
+int cl2_gauge_overrelax(cl2_ctx* c, void* gauge, int n_steps)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, gauge != nullptr, "null pointer");
+    CL2_REQUIRE(c, n_steps >= 0, "negative sweep count");
+    for (int o = 0; o < n_steps; ++o) CL2_HIP(c, launch_gauge_update(c, gauge, 1.0, 1, 0));
+    return CL2_OK;
+}
+
+int cl2_gauge_plaquette(cl2_ctx* c, const void* gauge, double* plaq)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, gauge && plaq, "null pointer");
+    CL2_HIP(c, launch_plaquette(c, gauge, c->dscratch));
+    double s = 0.;
+    CL2_HIP(c, hipMemcpyAsync(&s, c->dscratch, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CL2_HIP(c, hipStreamSynchronize(c->stream));
+    *plaq = s / (18. * 2. * c->g.veo);
     return CL2_OK;
 }
 
@@ -689,6 +726,20 @@ int cl2_set_cold_eo(cl2_ctx* c, void* out, int prec)
     CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
     const double norm = 1. / std::sqrt((double)(12. * 2. * c->g.veo));
     CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo, prec, norm, 0.));
+    return CL2_OK;
+}
+
+int cl2_convert_precision_eo(cl2_ctx* c, int kind, void* dst, int dst_prec, const void* src, int src_prec)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, kind == CL2_FIELD_SPINOR_EO || kind == CL2_FIELD_SU3VEC_EO, "kind must be a spinor or su3vec field");
+    CL2_REQUIRE(c, dst && src && dst != src, "bad pointers");
+    CL2_REQUIRE(c, valid_prec(dst_prec) && valid_prec(src_prec), "unknown precision");
+    const size_t n = (size_t)(kind == CL2_FIELD_SPINOR_EO ? 12 : 3) * c->g.veo;
+    if (dst_prec == src_prec)
+        CL2_HIP(c, hipMemcpyAsync(dst, src, n * (dst_prec == CL2_F64 ? 16 : 8), hipMemcpyDeviceToDevice, c->stream));
+    else
+        CL2_HIP(c, launch_convert(c, dst, dst_prec, src, src_prec, n));
     return CL2_OK;
 }
 

@@ -196,6 +196,32 @@ hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, dou
     return hipGetLastError();
 }
 
+/* precision conversion between complex fields (FP64 <-> FP32) */
+template<typename RD, typename RS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_convert(vec2<RD>* __restrict__ out, const vec2<RS>* __restrict__ in,
+                                                   size_t n)
+{
+    const size_t step = (size_t)gridDim.x * BLOCK;
+    for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < n; k += step) {
+        vec2<RS> v = in[k];
+        out[k]     = vec2<RD>{RD(v.x), RD(v.y)};
+    }
+}
+
+hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n)
+{
+    const unsigned nb = reduce_blocks(n);
+    if (out_prec == CL2_F32 && in_prec == CL2_F64)
+        hipLaunchKernelGGL((k_convert<float, double, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream,
+                           reinterpret_cast<float2*>(out), reinterpret_cast<const double2*>(in), n);
+    else if (out_prec == CL2_F64 && in_prec == CL2_F32)
+        hipLaunchKernelGGL((k_convert<double, float, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream,
+                           reinterpret_cast<double2*>(out), reinterpret_cast<const float2*>(in), n);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 /* Device-resident scalar arithmetic (complex_{product,ratio,sum,subtraction}.cl,
  * real_{product,ratio,sum,subtraction}.cl); ratio follows complexdivide
  * (common_header_files/operations_complex.hpp:59-66). */

@@ -16,6 +16,7 @@ struct cl2_ctx {
     double cp_fwd[2] = {1., 0.}, cp_bwd[2] = {1., 0.};  // factors on forward / backward T links
     double* partials = nullptr;  // reduction scratch: one double2 per block
     size_t npartials = 0;
+    double* dscratch = nullptr;  // a few device doubles for host-read results
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int tune_zplanes = -1;  // z-planes per slab of the hopping kernels (-1: heuristic, 0: identity order)
     int tune_xcd     = 1;   // XCD-contiguous block assignment
@@ -77,10 +78,13 @@ hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t
 hipError_t launch_reduce_stage2(cl2_ctx* c, unsigned nparts, double* res, int out_kind, const double* alpha_vec,
                                 int nvec);
 
+hipError_t launch_gauge_update(cl2_ctx* c, void* gauge, double beta, int mode, uint64_t key);
+hipError_t launch_plaquette(cl2_ctx* c, const void* gauge, double* res);
 hipError_t launch_gauge_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, double eps);
 hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src);
 
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im);
+hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n);
 
 hipError_t launch_scalar_op(cl2_ctx* c, int op, int is_complex, double* out, const double* a, const double* b);
 hipError_t launch_vec_element(cl2_ctx* c, double* out, const double* in, int out_index, int in_index);

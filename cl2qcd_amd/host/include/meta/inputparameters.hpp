@@ -25,6 +25,7 @@ class Inputparameters {
     uint32_t get_host_seed() const noexcept { return host_seed; }
 
     common::action get_fermact() const noexcept { return fermact; }
+    double get_beta() const noexcept { return beta; }
     double get_kappa() const noexcept { return kappa; }
     double get_mu() const noexcept { return mu; }
     double get_mass() const noexcept { return mass; }
@@ -55,6 +56,7 @@ class Inputparameters {
     std::string initial_conf;
     uint32_t host_seed = 4815;
     common::action fermact = common::wilson;
+    double beta = 4.0;
     double kappa = 0.125, mu = 0.006, mass = 0.1, theta_s = 0., theta_t = 0.;
     bool use_cp_re = false, use_cp_im = false;
     double cp_re = 0., cp_im = 0.;

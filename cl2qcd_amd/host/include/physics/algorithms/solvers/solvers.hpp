@@ -59,6 +59,30 @@ int cg_m(std::vector<std::shared_ptr<lattices::Staggeredfield_eo>> x, const ferm
          const hardware::System& system, physics::InterfacesHandler& interfacesHandler, hmc_float prec,
          const physics::AdditionalParameters& additionalParameters);
 
+/* ---- reduced precision (no reference counterpart: the reference is FP64 only) ---- */
+
+struct ReliableUpdateOptions {
+    int inner_precision = CL2_F32;  // precision of the Dslash and Krylov vectors of the inner loop
+    double delta        = 0.1;      // FP64 reliable update when |r| drops by this factor
+    int max_iterations  = 100000;
+};
+
+struct SolveStats {
+    int iterations       = 0;
+    int reliable_updates = 0;
+    double true_relative_residual = 0.;  // |b - A x| / |b| recomputed in FP64 at exit
+};
+
+/* CG with an FP64 reliable-update outer loop (solution accumulation, true residual,
+ * reductions and scalar recurrences in FP64) around an inner CG whose operator and
+ * vectors are in inner_precision.  Converged when the FP64 true residual satisfies
+ * |b - A x| <= rel_tol |b|.  Returns the total number of inner iterations. */
+int cg_reliable_update(const lattices::Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& A,
+                       const lattices::Gaugefield& gf, const lattices::Spinorfield_eo& b,
+                       const hardware::System& system, hmc_float rel_tol,
+                       const physics::AdditionalParameters& additionalParameters, const ReliableUpdateOptions& options,
+                       SolveStats* stats = nullptr);
+
 }  // namespace solvers
 }  // namespace algorithms
 }  // namespace physics

@@ -75,6 +75,12 @@ class Gaugefield {
     std::vector<Matrixsu3> get_host() const;
     /* device pointer and link format of the resident FP64 field */
     const void* dev() const noexcept { return buf->ptr(); }
+    /* device pointer for in-place updates; drops the reduced-precision copy */
+    void* dev_mutable()
+    {
+        buf_f32.reset();
+        return buf->ptr();
+    }
     int format() const noexcept { return fmt; }
     /* FP32 12-real copy for reduced-precision operators, made on first use */
     const void* dev_f32() const;
@@ -169,6 +175,9 @@ void saxpy_AND_gamma5_eo(const Spinorfield_eo* out, const hmc_complex alpha, con
 void convert_to_eoprec(const Spinorfield_eo* even, const Spinorfield_eo* odd, const Spinorfield& in);
 void convert_from_eoprec(const Spinorfield* merged, const Spinorfield_eo& even, const Spinorfield_eo& odd);
 void log_squarenorm(const std::string& msg, const Spinorfield_eo& x);
+/* copy between precisions (or a plain copy when they agree) */
+void convert_precision(const Spinorfield_eo* to, const Spinorfield_eo& from);
+void convert_precision(const Staggeredfield_eo* to, const Staggeredfield_eo& from);
 
 /* ---- staggered eo BLAS (physics/lattices/staggeredfield_eo.hpp:120-258) ---- */
 void copyData(const Staggeredfield_eo* to, const Staggeredfield_eo& from);

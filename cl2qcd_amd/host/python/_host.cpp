@@ -9,6 +9,8 @@
 #include <cstring>
 
 #include "../include/physics/algorithms/solvers/solvers.hpp"
+#include "../include/physics/algorithms/su3heatbath.hpp"
+#include "../include/physics/observables/gaugeObservables.hpp"
 
 namespace py = pybind11;
 using namespace physics;
@@ -118,6 +120,16 @@ PYBIND11_MODULE(_host, m)
             auto v = from_numpy<Matrixsu3>(a, 8 * g.get_system().eo_sites());
             g.import(v.data());
         });
+
+    m.def(
+        "su3heatbath", [](Gaugefield& gf, Lattice& l, int overrelax) { physics::algorithms::su3heatbath(gf, l.prng, overrelax); },
+        py::arg("gf"), py::arg("lattice"), py::arg("overrelax") = 0);
+    m.def(
+        "overrelax", [](Gaugefield& gf, Lattice& l, int steps) { physics::algorithms::overrelax(gf, l.prng, steps); },
+        py::arg("gf"), py::arg("lattice"), py::arg("steps") = 1);
+    m.def("measurePlaquette", [](const Gaugefield& gf) { return physics::observables::measurePlaquette(&gf); });
+    m.def("measurePlaquetteWithoutNormalization",
+          [](const Gaugefield& gf) { return physics::observables::measurePlaquetteWithoutNormalization(&gf); });
 
     py::class_<Spinorfield>(m, "Spinorfield")
         .def(py::init([](Lattice& l) { return new Spinorfield(l.system, l.handler.getInterface<Spinorfield>()); }),
@@ -285,6 +297,29 @@ PYBIND11_MODULE(_host, m)
         py::gil_scoped_release rel;
         return sv::bicgstab(&x, A, gf, b, l.system, l.handler, prec, ap);
     });
+    m.def(
+        "cg_reliable_update",
+        [](const Spinorfield_eo& x, const fm::Fermionmatrix_eo& A, const Gaugefield& gf, const Spinorfield_eo& b,
+           Lattice& l, hmc_float rel_tol, const AdditionalParameters& ap, int inner_prec, double delta, int maxit) {
+            sv::ReliableUpdateOptions o;
+            o.inner_precision = inner_prec;
+            o.delta           = delta;
+            o.max_iterations  = maxit;
+            sv::SolveStats st;
+            {
+                py::gil_scoped_release rel;
+                sv::cg_reliable_update(&x, A, gf, b, l.system, rel_tol, ap, o, &st);
+            }
+            py::dict d;
+            d["iterations"]             = st.iterations;
+            d["reliable_updates"]       = st.reliable_updates;
+            d["true_relative_residual"] = st.true_relative_residual;
+            return d;
+        },
+        py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("b"), py::arg("lattice"), py::arg("rel_tol"), py::arg("ap"),
+        py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000);
+    m.def("convert_precision",
+          [](const Spinorfield_eo& to, const Spinorfield_eo& from) { convert_precision(&to, from); });
     m.def("cg_m", [](std::vector<Staggeredfield_eo*> xs, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf,
                      std::vector<hmc_float> sigma, const Staggeredfield_eo& b, Lattice& l, hmc_float prec,
                      const AdditionalParameters& ap) {

@@ -436,6 +436,18 @@ void convert_from_eoprec(const Spinorfield* merged, const Spinorfield_eo& even, 
         merged->host[eo_site(ns, i, 1)] = o[i];
     }
 }
+void convert_precision(const Spinorfield_eo* to, const Spinorfield_eo& from)
+{
+    const auto& s = from.get_system();
+    s.check(cl2_convert_precision_eo(ctx(s), CL2_FIELD_SPINOR_EO, to->dev(), to->precision(), from.dev(),
+                                     from.precision()));
+}
+void convert_precision(const Staggeredfield_eo* to, const Staggeredfield_eo& from)
+{
+    const auto& s = from.get_system();
+    s.check(cl2_convert_precision_eo(ctx(s), CL2_FIELD_SU3VEC_EO, to->dev(), to->precision(), from.dev(),
+                                     from.precision()));
+}
 void log_squarenorm(const std::string& msg, const Spinorfield_eo& x)
 {
     if (std::getenv("CL2QCD_DEBUG")) std::fprintf(stderr, "%s%.16g\n", msg.c_str(), squarenorm(x));

@@ -81,6 +81,7 @@ void Inputparameters::parse(const std::vector<std::string>& args)
              else
                  throw std::invalid_argument("Unkown fermion action! wilson, twistedmass, rooted_stagg: " + v);
          }},
+        {"beta", [&](const std::string& v) { beta = parse_double(v); }},
         {"kappa", [&](const std::string& v) { kappa = parse_double(v); }},
         {"mu", [&](const std::string& v) { mu = parse_double(v); }},
         {"mass", [&](const std::string& v) { mass = parse_double(v); }},

@@ -2,6 +2,7 @@
 // rules (physics/algorithms/solvers/cg.cpp:277-400, bicgstab.cpp:401-733,
 // physics/algorithms/solver_shifted.cpp:114-475) on device-resident fields and
 // device scalars; the host reads a scalar only where the reference does.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -299,6 +300,92 @@ int cg_m(std::vector<std::shared_ptr<Staggeredfield_eo>> x, const fermionmatrix:
         ++iter;
     }
     throw SolverDidNotSolve(iter, "cg_m");
+}
+
+}  // namespace solvers
+}  // namespace algorithms
+}  // namespace physics
+
+namespace physics {
+namespace algorithms {
+namespace solvers {
+
+/* Reliable-update CG: the inner recurrence runs on inner_precision fields with
+ * the operator applied in that precision; whenever the iterated residual has
+ * dropped by delta since the last update (or claims convergence), the partial
+ * solution is folded into the FP64 solution and the residual is recomputed in
+ * FP64 from b - A x, which also decides convergence.  Scalars (alpha, beta,
+ * |r|^2) and all reductions are FP64. */
+int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& A, const Gaugefield& gf,
+                       const Spinorfield_eo& b, const hardware::System& system, hmc_float rel_tol,
+                       const physics::AdditionalParameters& ap, const ReliableUpdateOptions& opt, SolveStats* stats)
+{
+    if (x->precision() != CL2_F64 || b.precision() != CL2_F64)
+        throw std::invalid_argument("cg_reliable_update: x and b are FP64 fields");
+    if (opt.delta <= 0. || opt.delta >= 1.) throw std::invalid_argument("cg_reliable_update: delta must be in (0,1)");
+    const int lo = opt.inner_precision;
+    const Spinorfield_eo r(system, CL2_F64), t64(system, CL2_F64);
+    const Spinorfield_eo r_lo(system, lo), p(system, lo), v(system, lo), x_lo(system, lo);
+    const Scalar<hmc_complex> alpha(system), malpha(system), beta(system), mbeta(system), pAp(system), r2c(system),
+        r2new(system), minus_one(system);
+    minus_one.store(hmc_complex_minusone);
+    const hmc_complex one{1., 0.};
+
+    const hmc_float b2     = squarenorm(b);
+    const hmc_float target = rel_tol * rel_tol * b2;
+    auto true_residual = [&]() {
+        A(&t64, gf, *x, ap);
+        saxpy(&r, one, t64, b);  // r = b - A x
+        return squarenorm(r);
+    };
+    hmc_float r2 = true_residual();
+    SolveStats st;
+    if (b2 == 0. || r2 <= target) {
+        st.true_relative_residual = b2 > 0. ? std::sqrt(r2 / b2) : 0.;
+        if (stats) *stats = st;
+        return 0;
+    }
+    convert_precision(&r_lo, r);
+    copyData(&p, r_lo);
+    x_lo.zero();
+    r2c.store(hmc_complex{r2, 0.});
+    hmc_float rmax = std::sqrt(r2);
+    int k          = 0;
+    for (k = 0; k < opt.max_iterations; ++k) {
+        A(&v, gf, p, ap);
+        scalar_product(&pAp, p, v);
+        divide(&alpha, r2c, pAp);
+        multiply(&malpha, minus_one, alpha);
+        saxpy(&x_lo, malpha, p, x_lo);                  // x_lo += alpha p
+        saxpy_AND_squarenorm(&r_lo, alpha, v, r_lo, r2new);  // r_lo -= alpha v, |r_lo|^2
+        hmc_float rn = r2new.get().re;
+        if (rn != rn) throw SolverStuck(k, "cg_reliable_update: NAN occured");
+        if (std::sqrt(rn) < opt.delta * rmax || rn <= target) {
+            convert_precision(&t64, x_lo);
+            saxpy(x, hmc_complex{-1., 0.}, t64, *x);  // x += x_lo
+            x_lo.zero();
+            rn = true_residual();
+            ++st.reliable_updates;
+            if (rn <= target) {
+                ++k;
+                break;
+            }
+            convert_precision(&r_lo, r);
+            r2new.store(hmc_complex{rn, 0.});
+            rmax = std::sqrt(rn);
+        }
+        rmax = std::max(rmax, std::sqrt(rn));
+        divide(&beta, r2new, r2c);
+        multiply(&mbeta, minus_one, beta);
+        saxpy(&p, mbeta, p, r_lo);  // p = r_lo + beta p
+        copyData(&r2c, r2new);
+        r2 = rn;
+    }
+    st.iterations = k;
+    st.true_relative_residual = std::sqrt(squarenorm(r) / b2);
+    if (stats) *stats = st;
+    if (st.true_relative_residual > rel_tol) throw SolverDidNotSolve(k, "cg_reliable_update");
+    return k;
 }
 
 }  // namespace solvers

@@ -132,6 +132,17 @@ int cl2_gauge_import(cl2_ctx* ctx, void* dev, const double* host_aos, int format
  * set_cold): every link U = reunitarised (1 + epsilon * complex Gaussian) from a counter-based generator keyed by
  * seed; epsilon = 0 gives the cold (unit) field, epsilon >~ 10 a Haar-like hot field. */
 int cl2_gauge_set_random(cl2_ctx* ctx, void* dev, int format, uint64_t seed, double epsilon);
+/* su3heatbath + overrelax (physics/algorithms/su3heatbath.cpp:33-73, hardware/code/heatbath.cpp) for the
+ * Wilson gauge action at coupling beta on an F64_18 field: n_sweeps times, one heatbath sweep over all links
+ * followed by n_overrelax overrelaxation sweeps.  Cabibbo-Marinari SU(2) subgroups, Kennedy-Pendleton
+ * sampling, random numbers from a counter hash keyed by seed (the reference's ranluxcl streams depend on
+ * the device's thread count and are not reproduced). */
+int cl2_gauge_heatbath(cl2_ctx* ctx, void* gauge, double beta, int n_sweeps, int n_overrelax, uint64_t seed);
+/* n_steps overrelaxation sweeps (physics/algorithms/su3heatbath.cpp:61-73); deterministic */
+int cl2_gauge_overrelax(cl2_ctx* ctx, void* gauge, int n_steps);
+/* average plaquette (1/(18 V)) sum_x sum_{mu<nu} Re tr P_mu,nu(x) of an F64_18 field
+ * (physics/observables/gaugeObservables.cpp, ocl_kernel/gaugeobservables_plaquette.cl) */
+int cl2_gauge_plaquette(cl2_ctx* ctx, const void* gauge, double* plaquette);
 /* Re-layout a resident F64_18 field into another format (feeds reduced-precision solvers). */
 int cl2_gauge_convert(cl2_ctx* ctx, void* dst, int dst_format, const void* src_f64_18);
 /* Gaugefield export to host AoS (hardware/lattices/gaugefield.cpp:104-124); 12-real formats are
@@ -190,6 +201,11 @@ int cl2_saxpy_AND_squarenorm_eo(cl2_ctx* ctx, void* out, const void* x, const vo
  * cold = every complex component 1/sqrt(12 V) */
 int cl2_set_zero_eo(cl2_ctx* ctx, void* out, int prec);
 int cl2_set_cold_eo(cl2_ctx* ctx, void* out, int prec);
+
+/* Copy a spinor / staggered field into another precision (F64 <-> F32, or a plain copy);
+ * no reference counterpart: the reference runs in one precision throughout
+ * (CMakeLists.txt:127-132).  Feeds the reduced-precision inner solvers. */
+int cl2_convert_precision_eo(cl2_ctx* ctx, int kind, void* dst, int dst_prec, const void* src, int src_prec);
 
 /* ==== staggered (hardware/code/fermions_staggered.hpp:46-91, spinors_staggered.hpp) ==== */
 

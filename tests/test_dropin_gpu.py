@@ -249,3 +249,47 @@ def test_blas_dropin():
     assert rel(H.squarenorm(x), O.norm2(X)) < 1e-13
     out.cold()
     assert rel(H.squarenorm(out), 0.5) < 1e-14  # |phi|^2 of the whole lattice is 1, eo half
+
+
+def test_convert_precision():
+    l = lattice("--nTime=4")
+    x, back = H.Spinorfield_eo(l), H.Spinorfield_eo(l)
+    lo = H.Spinorfield_eo(l, H.F32)
+    H.pseudo_randomize(x, 41)
+    H.convert_precision(lo, x)
+    H.convert_precision(back, lo)
+    X = x.get_host()
+    assert np.array_equal(back.get_host(), X.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("action,op", [("wilson", "QplusQminus_eo"), ("twistedmass", "QplusQminus_eo"),
+                                       ("twistedmass", "Aee")])
+def test_cg_reliable_update(action, op):
+    """FP32 inner CG with FP64 reliable updates reaches |b - A x| / |b| <= 1e-10 in FP64 (checked with
+    the oracle's FP64 operator) and agrees with the FP64 CG solution.  Aee is not hermitian; CG on it is
+    not expected to converge and must raise a solver exception instead of returning garbage."""
+    l = lattice("--nSpace=8", "--nTime=8", f"--fermionAction={action}", "--kappa=0.15", "--mu=0.02",
+                "--beta=5.7")
+    gf = H.Gaugefield(l, False)
+    for _ in range(3):
+        H.su3heatbath(gf, l, 1)
+    b, _ = eo_pair(l, 51)
+    x = H.Spinorfield_eo(l)
+    A = getattr(H, op)(l)
+    if op == "Aee":
+        with pytest.raises((H.SolverDidNotSolve, H.SolverStuck)):
+            H.cg_reliable_update(x, A, gf, b, l, 1e-10, l.wilson_parameters(), max_iterations=300)
+        return
+    st = H.cg_reliable_update(x, A, gf, b, l, 1e-10, l.wilson_parameters())
+    assert st["true_relative_residual"] <= 1e-10 and st["reliable_updates"] >= 2
+    lat = O.lattice(8, 8)
+    g = gf.get_host()
+    act = O.WILSON if action == "wilson" else O.TWISTEDMASS
+    bh, xh = b.get_host(), x.get_host()
+    r = O.saxpy(O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, xh, g, act, 0.15, l.mubar), bh, 1.0)
+    assert np.sqrt(O.norm2(r) / O.norm2(bh)) <= 1.01e-10
+    x64 = H.Spinorfield_eo(l)
+    x64.zero()
+    H.cg(x64, A, gf, b, l, 1e-24 * H.squarenorm(b), l.wilson_parameters())
+    x64h = x64.get_host()
+    assert np.sqrt(((xh - x64h) ** 2).sum() / (x64h ** 2).sum()) < 1e-6
