@@ -13,6 +13,12 @@ Also reported: the same for 12-real FP64 and FP32 links, the reference's own
 byte convention (2880 B/site), the CPU oracle timed on host cores
 (cpu_baseline), and the HBM traffic per launch measured by rocprofv3 counters
 (read from profiles/ when the counter summary for this workload is there).
+
+The "cg" object is the solver half of the metric: on a heatbath beta 5.7
+configuration generated on the device, the reference's FP64 eo-CG on
+QplusQminus_eo (twisted mass, kappa 0.16, mu 0.005) against the reliable-update
+CG with a reduced-precision inner loop, both to |r|/|b| <= 1e-10, with the true
+residual recomputed in FP64 and the distance between the two solutions.
 """
 import argparse
 import ctypes
@@ -71,6 +77,69 @@ def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src):
                 finite=finite)
 
 
+def cg_leg(ns, nt, sweeps, overrelax, seed):
+    """FP64 eo-CG (reference algorithm) vs FP32-inner reliable-update CG on QplusQminus_eo, both to
+    |r|/|b| <= 1e-10, through the C++ drop-in.  Twisted mass kappa 0.16, mu 0.005, antiperiodic in t, on a
+    heatbath beta 5.7 configuration made on the device from a cold start (seeded), Gaussian source."""
+    from cl2qcd_amd import _host as H
+
+    l = H.Lattice([f"--nSpace={ns}", f"--nTime={nt}", "--beta=5.7", "--fermionAction=twistedmass",
+                   "--kappa=0.16", "--mu=0.005", "--thetaFermionTemporal=1", "--solverMaxIterations=200000",
+                   f"--hostSeed={seed}"])
+    gf = H.Gaugefield(l, False)
+    t0 = time.perf_counter()
+    for _ in range(sweeps):
+        H.su3heatbath(gf, l, overrelax)
+    plaq = H.measurePlaquette(gf)
+    hb_s = time.perf_counter() - t0
+    log(f"heatbath: {sweeps} x (1 + {overrelax} OR) sweeps in {hb_s:.1f} s, plaquette {plaq:.6f}")
+    b = H.Spinorfield_eo(l)
+    b.import_(np.random.default_rng(seed).standard_normal((l.eo_sites, 24)))
+    b2 = H.squarenorm(b)
+    qq, ap = H.QplusQminus_eo(l), l.wilson_parameters()
+    r = H.Spinorfield_eo(l)
+
+    def true_rel_residual(x):
+        qq(r, gf, x, ap)
+        H.saxpy(r, 1.0, r, b)  # r = b - A x
+        return float(np.sqrt(H.squarenorm(r) / b2))
+
+    out = {"lattice": f"{ns}^3x{nt}", "heatbath": f"beta 5.7, cold start, {sweeps} sweeps x (heatbath + "
+                                                  f"{overrelax} overrelaxation), seed {seed}",
+           "plaquette": round(plaq, 6), "operator": "QplusQminus_eo twisted mass kappa 0.16 mu 0.005",
+           "target_rel_residual": 1e-10}
+    x64 = H.Spinorfield_eo(l)
+    x64.zero()
+    l.synchronize()
+    t0 = time.perf_counter()
+    it = H.cg(x64, qq, gf, b, l, 1e-20 * b2, ap)
+    l.synchronize()
+    dt = time.perf_counter() - t0
+    out["fp64"] = dict(iterations=it, seconds=round(dt, 3), ms_per_iteration=round(1e3 * dt / max(it, 1), 3),
+                       true_rel_residual=true_rel_residual(x64))
+    log(f"FP64 CG: {it} iterations, {dt:.2f} s, true |r|/|b| {out['fp64']['true_rel_residual']:.3e}")
+    x64n = H.squarenorm(x64)
+    d = H.Spinorfield_eo(l)
+    for name, prec in (("fp32", H.F32),):
+        x = H.Spinorfield_eo(l)
+        x.zero()
+        l.synchronize()
+        t0 = time.perf_counter()
+        st = H.cg_reliable_update(x, qq, gf, b, l, 1e-10, ap, prec, 0.1)
+        l.synchronize()
+        dt = time.perf_counter() - t0
+        H.saxpy(d, 1.0, x64, x)
+        out[name] = dict(iterations=st["iterations"], reliable_updates=st["reliable_updates"],
+                         seconds=round(dt, 3), ms_per_iteration=round(1e3 * dt / max(st["iterations"], 1), 3),
+                         true_rel_residual=true_rel_residual(x),
+                         rel_diff_to_fp64_solution=float(np.sqrt(H.squarenorm(d) / x64n)),
+                         speedup_vs_fp64=round(out["fp64"]["seconds"] / dt, 3))
+        log(f"{name}-inner CG: {st['iterations']} iterations ({st['reliable_updates']} reliable updates), "
+            f"{dt:.2f} s, true |r|/|b| {out[name]['true_rel_residual']:.3e}, "
+            f"speedup {out[name]['speedup_vs_fp64']:.2f}x")
+    return out
+
+
 def read_counters():
     """HBM bytes per launch of the FP64 dslash from the committed rocprofv3 counter summary."""
     if not os.path.exists(COUNTERS_FILE):
@@ -117,6 +186,9 @@ def main():
     ap.add_argument("--nt", type=int, default=96)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only-f64", action="store_true", help="time only the FP64 18-real leg")
+    ap.add_argument("--no-cg", action="store_true", help="skip the FP64 vs mixed-precision CG comparison")
+    ap.add_argument("--hb-sweeps", type=int, default=50, help="heatbath sweeps for the CG configuration")
+    ap.add_argument("--hb-overrelax", type=int, default=1, help="overrelaxation sweeps per heatbath sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
@@ -150,6 +222,10 @@ def main():
     veo = ctx.veo
     ctx.close()
 
+    cg = None
+    if not args.no_cg:
+        cg = cg_leg(args.ns, args.nt, args.hb_sweeps, args.hb_overrelax, 20261202)
+
     traffic = read_counters()
     achieved = f64["alg_bytes"] / (f64["ms"] * 1e6)
     line = {
@@ -175,6 +251,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         **extra,
+        "cg": cg,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores) ...")
