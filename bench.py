@@ -120,7 +120,7 @@ def cg_leg(ns, nt, sweeps, overrelax, seed):
     log(f"FP64 CG: {it} iterations, {dt:.2f} s, true |r|/|b| {out['fp64']['true_rel_residual']:.3e}")
     x64n = H.squarenorm(x64)
     d = H.Spinorfield_eo(l)
-    for name, prec in (("fp32", H.F32),):
+    for name, prec in (("fp32", H.F32), ("fp16", H.F16)):
         x = H.Spinorfield_eo(l)
         x.zero()
         l.synchronize()

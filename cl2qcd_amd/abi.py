@@ -16,8 +16,8 @@ HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "cl2qcd.h")
 
 OK, E_ARG, E_HIP, E_STUCK, E_NOT_CONVERGED, E_UNSUPPORTED = 0, 1, 2, 3, 4, 5
 EVEN, ODD = 0, 1
-F64, F32 = 0, 1
-GAUGE_F64_18, GAUGE_F64_12, GAUGE_F32_12 = 0, 1, 2
+F64, F32, F16 = 0, 1, 2
+GAUGE_F64_18, GAUGE_F64_12, GAUGE_F32_12, GAUGE_F16_12 = 0, 1, 2, 3
 FIELD_SPINOR_EO, FIELD_SU3VEC_EO, FIELD_GAUGE = 0, 1, 2
 ACTION_WILSON, ACTION_TWISTEDMASS = 0, 1
 OP_AEE, OP_AEE_AND_GAMMA5, OP_AEE_MINUS, OP_AEE_MINUS_AND_GAMMA5 = 0, 1, 2, 3
@@ -99,6 +99,7 @@ SIGNATURES = {
     "cl2_scalar_product_eo": (_i, [_vp, _dp, _vp, _vp, _i]),
     "cl2_squarenorm_eo": (_i, [_vp, _dp, _vp, _i]),
     "cl2_saxpy_AND_squarenorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
+    "cl2_saxpy_AND_cgnorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
     "cl2_set_zero_eo": (_i, [_vp, _vp, _i]),
     "cl2_set_cold_eo": (_i, [_vp, _vp, _i]),
     "cl2_convert_precision_eo": (_i, [_vp, _i, _vp, _i, _vp, _i]),

@@ -39,6 +39,7 @@ int fail(cl2_ctx* c, int code, const std::string& msg)
 
 size_t spinor_bytes(const cl2_ctx* c, int prec)
 {
+    if (prec == CL2_F16) return (size_t)52 * c->g.veo;  // 12 complex halves + one float norm per site
     return (size_t)12 * c->g.veo * (prec == CL2_F64 ? 16 : 8);
 }
 size_t su3vec_bytes(const cl2_ctx* c, int prec)
@@ -52,20 +53,27 @@ size_t gauge_bytes(const cl2_ctx* c, int fmt)
         case CL2_GAUGE_F64_18: return links * 9 * 16;
         case CL2_GAUGE_F64_12: return links * 6 * 16;
         case CL2_GAUGE_F32_12: return links * 3 * 16;
+        case CL2_GAUGE_F16_12: return links * 3 * 8;
         default: return 0;
     }
 }
+/* precisions of every field kind */
 bool valid_prec(int p)
 {
     return p == CL2_F64 || p == CL2_F32;
 }
+/* precisions of Wilson eo spinors (FP16 with per-site scale included) */
+bool valid_wprec(int p)
+{
+    return valid_prec(p) || p == CL2_F16;
+}
 bool valid_fmt(int f)
 {
-    return f == CL2_GAUGE_F64_18 || f == CL2_GAUGE_F64_12 || f == CL2_GAUGE_F32_12;
+    return f == CL2_GAUGE_F64_18 || f == CL2_GAUGE_F64_12 || f == CL2_GAUGE_F32_12 || f == CL2_GAUGE_F16_12;
 }
 int fmt_prec(int f)
 {
-    return f == CL2_GAUGE_F32_12 ? CL2_F32 : CL2_F64;
+    return f == CL2_GAUGE_F16_12 ? CL2_F16 : (f == CL2_GAUGE_F32_12 ? CL2_F32 : CL2_F64);
 }
 
 inline int site_parity(const cl2_ctx* c, size_t s)
@@ -77,6 +85,15 @@ inline int site_parity(const cl2_ctx* c, size_t s)
     size_t z = r % ns, t = r / ns;
     return (int)((x + y + z + t) & 1);
 }
+
+/* a temporary device buffer freed on scope exit (import / export staging only) */
+struct TempDev {
+    void* p = nullptr;
+    ~TempDev()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
 
 cl2_scalar value(double re, double im = 0.)
 {
@@ -100,9 +117,11 @@ int check_scalar(cl2_ctx* c, const cl2_scalar* s)
 
 /* elementwise BLAS dispatch shared by the Wilson and staggered entry points */
 int run_blas(cl2_ctx* c, int mode, size_t n, void* out, const void* x, const void* y, const void* z, int prec,
-             const cl2_scalar& a, const cl2_scalar& b, double* norm_res, size_t tail)
+             const cl2_scalar& a, const cl2_scalar& b, double* norm_res, size_t tail, int cg_norm = 0)
 {
-    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    const bool half = prec == CL2_F16;
+    CL2_REQUIRE(c, valid_prec(prec) || (half && n == (size_t)12 * c->g.veo),
+                half ? "FP16 storage is for Wilson eo spinors only" : "unknown precision");
     CL2_REQUIRE(c, out && x, "null field pointer");
     BlasArgs args;
     args.n             = n;
@@ -115,18 +134,29 @@ int run_blas(cl2_ctx* c, int mode, size_t n, void* out, const void* x, const voi
     args.b             = b;
     args.mode          = mode;
     args.norm_partials = norm_res ? c->partials : nullptr;
+    args.cg_norm       = cg_norm;
     unsigned nb        = 0;
-    CL2_HIP(c, launch_blas(c, args, prec, &nb));
-    if (norm_res) CL2_HIP(c, launch_reduce_stage2(c, nb, norm_res, 1, nullptr, 0));
+    if (half) {
+        args.n = c->g.veo;
+        CL2_HIP(c, launch_blas_h(c, args, &nb));
+    } else {
+        CL2_HIP(c, launch_blas(c, args, prec, &nb));
+    }
+    if (norm_res) CL2_HIP(c, launch_reduce_stage2(c, nb, norm_res, cg_norm ? 3 : 1, nullptr, 0));
     return CL2_OK;
 }
 
 int run_reduce(cl2_ctx* c, double* res, const void* x, const void* y, size_t n, int prec, int mode, int out_kind)
 {
-    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    const bool half = prec == CL2_F16;
+    CL2_REQUIRE(c, valid_prec(prec) || (half && n == (size_t)12 * c->g.veo),
+                half ? "FP16 storage is for Wilson eo spinors only" : "unknown precision");
     CL2_REQUIRE(c, res && x && (mode == R_NORM || y), "null pointer");
     unsigned nb = 0;
-    CL2_HIP(c, launch_reduce_stage1(c, x, y, n, prec, mode, &nb));
+    if (half)
+        CL2_HIP(c, launch_reduce_stage1_h(c, x, y, mode, &nb));
+    else
+        CL2_HIP(c, launch_reduce_stage1(c, x, y, n, prec, mode, &nb));
     CL2_HIP(c, launch_reduce_stage2(c, nb, res, out_kind, nullptr, 0));
     return CL2_OK;
 }
@@ -301,7 +331,7 @@ size_t cl2_field_bytes(const cl2_ctx* c, int kind, int prec_or_format)
 {
     if (!c) return 0;
     switch (kind) {
-        case CL2_FIELD_SPINOR_EO: return valid_prec(prec_or_format) ? spinor_bytes(c, prec_or_format) : 0;
+        case CL2_FIELD_SPINOR_EO: return valid_wprec(prec_or_format) ? spinor_bytes(c, prec_or_format) : 0;
         case CL2_FIELD_SU3VEC_EO: return valid_prec(prec_or_format) ? su3vec_bytes(c, prec_or_format) : 0;
         case CL2_FIELD_GAUGE: return gauge_bytes(c, prec_or_format);
         default: return 0;
@@ -396,12 +426,24 @@ static int complex_field_export(cl2_ctx* c, double* host, const void* dev, int p
 int cl2_spinor_import(cl2_ctx* c, void* dev, const double* host, int prec)
 {
     if (!c) return CL2_E_ARG;
-    return complex_field_import(c, dev, host, prec, 12);
+    if (prec != CL2_F16) return complex_field_import(c, dev, host, prec, 12);
+    CL2_REQUIRE(c, dev && host, "null pointer");
+    TempDev t;
+    CL2_HIP(c, hipMalloc(&t.p, spinor_bytes(c, CL2_F64)));
+    if (int rc = complex_field_import(c, t.p, host, CL2_F64, 12)) return rc;
+    CL2_HIP(c, launch_convert_h(c, dev, CL2_F16, t.p, CL2_F64));
+    CL2_HIP(c, hipStreamSynchronize(c->stream));
+    return CL2_OK;
 }
 int cl2_spinor_export(cl2_ctx* c, double* host, const void* dev, int prec)
 {
     if (!c) return CL2_E_ARG;
-    return complex_field_export(c, host, dev, prec, 12);
+    if (prec != CL2_F16) return complex_field_export(c, host, dev, prec, 12);
+    CL2_REQUIRE(c, dev && host, "null pointer");
+    TempDev t;
+    CL2_HIP(c, hipMalloc(&t.p, spinor_bytes(c, CL2_F64)));
+    CL2_HIP(c, launch_convert_h(c, t.p, CL2_F64, dev, CL2_F16));
+    return complex_field_export(c, host, t.p, CL2_F64, 12);
 }
 int cl2_su3vec_import(cl2_ctx* c, void* dev, const double* host, int prec)
 {
@@ -419,6 +461,14 @@ int cl2_gauge_import(cl2_ctx* c, void* dev, const double* host, int fmt)
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, dev && host, "null pointer");
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    if (fmt == CL2_GAUGE_F16_12) {
+        TempDev t;
+        CL2_HIP(c, hipMalloc(&t.p, gauge_bytes(c, CL2_GAUGE_F64_18)));
+        if (int rc = cl2_gauge_import(c, t.p, host, CL2_GAUGE_F64_18)) return rc;
+        CL2_HIP(c, launch_gauge_convert(c, dev, CL2_GAUGE_F16_12, t.p));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+        return CL2_OK;
+    }
     const size_t S = c->g.veo, V = 2 * S;
     if (fmt == CL2_GAUGE_F32_12) {
         std::vector<float> stage(gauge_bytes(c, fmt) / sizeof(float));
@@ -517,6 +567,7 @@ int cl2_gauge_export(cl2_ctx* c, double* host, const void* dev, int fmt)
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, dev && host, "null pointer");
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    if (fmt == CL2_GAUGE_F16_12) return fail(c, CL2_E_UNSUPPORTED, "export of F16_12 links is not supported");
     const size_t S = c->g.veo, V = 2 * S;
     const int lanes = fmt == CL2_GAUGE_F64_18 ? 9 : 6;
     std::vector<double> stage(2 * lanes * 8 * S);
@@ -611,12 +662,15 @@ int cl2_M_tm_sitediagonal_eo(cl2_ctx* c, void* out, const void* in, int prec, do
 {
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, out && in, "null field pointer");
-    CL2_REQUIRE(c, valid_prec(prec), "unknown precision");
+    CL2_REQUIRE(c, valid_wprec(prec), "unknown precision");
     CL2_REQUIRE(c, variant >= CL2_MTM_SITEDIAGONAL && variant <= CL2_MTM_SITEDIAGONAL_MINUS_AND_GAMMA5,
                 "unknown site-diagonal variant");
     double f1[2], f2[2], s;
     tm_factors(variant, mubar, f1, f2, &s);
-    CL2_HIP(c, launch_sitediag(c, out, in, prec, f1, f2, s));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_sitediag_h(c, out, in, f1, f2, s));
+    else
+        CL2_HIP(c, launch_sitediag(c, out, in, prec, f1, f2, s));
     return CL2_OK;
 }
 
@@ -712,20 +766,35 @@ int cl2_saxpy_AND_squarenorm_eo(cl2_ctx* c, void* out, const void* x, const void
     return run_blas(c, B_Y_MINUS_AX, (size_t)12 * c->g.veo, out, x, y, nullptr, prec, *alpha, value(0.), res, 0);
 }
 
+int cl2_saxpy_AND_cgnorm_eo(cl2_ctx* c, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha,
+                            double* res)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    CL2_REQUIRE(c, y && res, "null pointer");
+    return run_blas(c, B_Y_MINUS_AX, (size_t)12 * c->g.veo, out, x, y, nullptr, prec, *alpha, value(0.), res, 0, 1);
+}
+
 int cl2_set_zero_eo(cl2_ctx* c, void* out, int prec)
 {
     if (!c) return CL2_E_ARG;
-    CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
-    CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo, prec, 0., 0.));
+    CL2_REQUIRE(c, out && valid_wprec(prec), "bad argument");
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_fill_h(c, out, 0., 0.));
+    else
+        CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo, prec, 0., 0.));
     return CL2_OK;
 }
 
 int cl2_set_cold_eo(cl2_ctx* c, void* out, int prec)
 {
     if (!c) return CL2_E_ARG;
-    CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
+    CL2_REQUIRE(c, out && valid_wprec(prec), "bad argument");
     const double norm = 1. / std::sqrt((double)(12. * 2. * c->g.veo));
-    CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo, prec, norm, 0.));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_fill_h(c, out, norm, 0.));
+    else
+        CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo, prec, norm, 0.));
     return CL2_OK;
 }
 
@@ -734,10 +803,16 @@ int cl2_convert_precision_eo(cl2_ctx* c, int kind, void* dst, int dst_prec, cons
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, kind == CL2_FIELD_SPINOR_EO || kind == CL2_FIELD_SU3VEC_EO, "kind must be a spinor or su3vec field");
     CL2_REQUIRE(c, dst && src && dst != src, "bad pointers");
-    CL2_REQUIRE(c, valid_prec(dst_prec) && valid_prec(src_prec), "unknown precision");
+    if (kind == CL2_FIELD_SPINOR_EO)
+        CL2_REQUIRE(c, valid_wprec(dst_prec) && valid_wprec(src_prec), "unknown precision");
+    else
+        CL2_REQUIRE(c, valid_prec(dst_prec) && valid_prec(src_prec), "unknown precision (FP16 is for Wilson spinors)");
     const size_t n = (size_t)(kind == CL2_FIELD_SPINOR_EO ? 12 : 3) * c->g.veo;
     if (dst_prec == src_prec)
-        CL2_HIP(c, hipMemcpyAsync(dst, src, n * (dst_prec == CL2_F64 ? 16 : 8), hipMemcpyDeviceToDevice, c->stream));
+        CL2_HIP(c, hipMemcpyAsync(dst, src, kind == CL2_FIELD_SPINOR_EO ? spinor_bytes(c, dst_prec) : su3vec_bytes(c, dst_prec),
+                                  hipMemcpyDeviceToDevice, c->stream));
+    else if (dst_prec == CL2_F16 || src_prec == CL2_F16)
+        CL2_HIP(c, launch_convert_h(c, dst, dst_prec, src, src_prec));
     else
         CL2_HIP(c, launch_convert(c, dst, dst_prec, src, src_prec, n));
     return CL2_OK;
@@ -748,7 +823,7 @@ int cl2_convert_precision_eo(cl2_ctx* c, int kind, void* dst, int dst_prec, cons
 int cl2_D_KS_eo(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, int evenodd)
 {
     if (!c) return CL2_E_ARG;
-    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, valid_fmt(fmt) && fmt != CL2_GAUGE_F16_12, "unknown gauge format (F16_12 is for Wilson operators)");
     CL2_REQUIRE(c, evenodd == CL2_EVEN || evenodd == CL2_ODD, "evenodd must be CL2_EVEN or CL2_ODD");
     CL2_REQUIRE(c, out && in && gauge, "null field pointer");
     CL2_REQUIRE(c, out != in, "D_KS_eo cannot run in place: out aliases in");
@@ -1009,12 +1084,12 @@ uint64_t cl2_algorithmic_bytes(const cl2_ctx* c, const char* k, int fmt)
     if (!c || !k || !valid_fmt(fmt)) return 0;
     const uint64_t Seo = c->g.veo;
     const uint64_t L   = gauge_bytes(c, fmt) / (8 * Seo);
-    const uint64_t S   = (fmt == CL2_GAUGE_F32_12) ? 96 : 192;
+    const uint64_t S   = (fmt == CL2_GAUGE_F16_12) ? 52 : (fmt == CL2_GAUGE_F32_12) ? 96 : 192;
     const uint64_t s   = S / 4;
     const std::string n(k);
     if (n == "dslash_eo" || n == "dslash_AND_M_tm_inverse_sitediagonal_eo") return Seo * (8 * L + 2 * S);
     if (n == "dslash_AND_sitediagonal_saxpy_eo") return Seo * (8 * L + 3 * S);
-    if (n == "D_KS_eo") return Seo * (8 * L + 2 * s);
+    if (n == "D_KS_eo") return fmt == CL2_GAUGE_F16_12 ? 0 : Seo * (8 * L + 2 * s);
     if (n == "M_tm_sitediagonal" || n == "gamma5_eo" || n == "squarenorm") return Seo * 2 * S / (n == "squarenorm" ? 2 : 1);
     if (n == "saxpy" || n == "saxpy_AND_squarenorm") return Seo * 3 * S;
     if (n == "scalar_product") return Seo * 2 * S;

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas(BlasArgs a)
     const V* y = reinterpret_cast<const V*>(a.y);
     const V* z = reinterpret_cast<const V*>(a.z);
     V* out     = reinterpret_cast<V*>(a.out);
-    double nrm = 0.;
+    double nrm = 0., ry = 0.;
     const size_t step = (size_t)gridDim.x * BLOCK;
     for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < a.n; k += step) {
         Cx<R> r;
@@ -40,6 +40,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas(BlasArgs a)
             case B_Y_MINUS_AX: {
                 V xv = x[k], yv = y[k];
                 r    = Cx<R>{yv.x, yv.y} - cmul(Cx<R>{xv.x, xv.y}, al);
+                ry += (double)r.re * (double)yv.x + (double)r.im * (double)yv.y;
                 break;
             }
             case B_AX: {
@@ -78,8 +79,8 @@ __global__ __launch_bounds__(BLOCK) void k_blas(BlasArgs a)
         nrm += (double)r.re * (double)r.re + (double)r.im * (double)r.im;
     }
     if (a.norm_partials) {
-        double s = block_sum<BLOCK>(nrm, lds);
-        if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s, a.norm_partials[2 * blockIdx.x + 1] = 0.;
+        double2 s = block_sum2<BLOCK>(make_double2(nrm, a.cg_norm ? ry : 0.), lds);
+        if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s.x, a.norm_partials[2 * blockIdx.x + 1] = s.y;
     }
 }
 
@@ -144,7 +145,8 @@ hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t
 }
 
 /* stage 2: one block sums the partials in a fixed order.
- * out_kind 0: res[0] = re;  1: res = (re, im);  2: res[k] = alpha[k]^2 * re for k < nvec */
+ * out_kind 0: res[0] = re;  1: res = (re, im);  2: res[k] = alpha[k]^2 * re for k < nvec;
+ * 3: res = (re, im, re, 0) */
 template<int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_reduce2(const double* __restrict__ partials, unsigned nparts,
                                                    double* __restrict__ res, int out_kind,
@@ -163,6 +165,11 @@ __global__ __launch_bounds__(BLOCK) void k_reduce2(const double* __restrict__ pa
         } else if (out_kind == 1) {
             res[0] = s.x;
             res[1] = s.y;
+        } else if (out_kind == 3) {
+            res[0] = s.x;
+            res[1] = s.y;
+            res[2] = s.x;
+            res[3] = 0.;
         } else {
             for (int k = 0; k < nvec; ++k) res[k] = alpha[k] * alpha[k] * s.x;
         }

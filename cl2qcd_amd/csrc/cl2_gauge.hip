@@ -6,6 +6,19 @@
 
 namespace cl2 {
 
+typedef _Float16 gh4 __attribute__((ext_vector_type(4)));
+
+/* rows 0,1 of u into 3 lanes of 4 halves (entries 2k, 2k+1 of the 6) */
+template<typename R>
+__device__ __forceinline__ void store_link_h(void* dev, uint32_t dp, uint32_t veo, uint32_t eo, const Su3<R>& u)
+{
+    gh4* q = reinterpret_cast<gh4*>(dev) + (size_t)(dp * 3) * veo + eo;
+    for (int k = 0; k < 3; ++k) {
+        const Cx<R> a = u.e[(2 * k) / 3][(2 * k) % 3], b = u.e[(2 * k + 1) / 3][(2 * k + 1) % 3];
+        q[(size_t)k * veo] = gh4{(_Float16)a.re, (_Float16)a.im, (_Float16)b.re, (_Float16)b.im};
+    }
+}
+
 /* splitmix64 finaliser: a stateless hash of (seed, counter) */
 __device__ __forceinline__ uint64_t mix64(uint64_t z)
 {
@@ -48,7 +61,9 @@ __global__ void k_gauge_random(void* dev, int fmt, uint32_t veo, uint64_t seed, 
     su3_reconstruct_row2(u);
 
     const uint32_t dp = (uint32_t)(link / veo), eo = (uint32_t)(link % veo);
-    if (fmt == GAUGE_F32_12) {
+    if (fmt == GAUGE_F16_12) {
+        store_link_h(dev, dp, veo, eo, u);
+    } else if (fmt == GAUGE_F32_12) {
         float4* q = reinterpret_cast<float4*>(dev) + (size_t)(dp * 3) * veo + eo;
         for (int k = 0; k < 3; ++k) {
             Cx<double> a = u.e[(2 * k) / 3][(2 * k) % 3], b = u.e[(2 * k + 1) / 3][(2 * k + 1) % 3];
@@ -69,7 +84,7 @@ hipError_t launch_gauge_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, do
     return hipGetLastError();
 }
 
-/* Device-layout conversion between formats (F64_18 -> F64_12 / F32_12), so one
+/* Device-layout conversion between formats (F64_18 -> F64_12 / F32_12 / F16_12), so one
  * resident FP64 field can feed the reduced-precision solvers. */
 __global__ void k_gauge_convert(void* dst, int dfmt, const double2* __restrict__ src, uint32_t veo)
 {
@@ -77,7 +92,11 @@ __global__ void k_gauge_convert(void* dst, int dfmt, const double2* __restrict__
     if (link >= 8ull * veo) return;
     const uint32_t dp = (uint32_t)(link / veo), eo = (uint32_t)(link % veo);
     const double2* s  = src + (size_t)(dp * 9) * veo + eo;
-    if (dfmt == GAUGE_F32_12) {
+    if (dfmt == GAUGE_F16_12) {
+        Su3<double> u;
+        for (int k = 0; k < 6; ++k) u.e[k / 3][k % 3] = Cx<double>{s[(size_t)k * veo].x, s[(size_t)k * veo].y};
+        store_link_h(dst, dp, veo, eo, u);
+    } else if (dfmt == GAUGE_F32_12) {
         float4* q = reinterpret_cast<float4*>(dst) + (size_t)(dp * 3) * veo + eo;
         for (int k = 0; k < 3; ++k) {
             double2 a = s[(size_t)(2 * k) * veo], b = s[(size_t)(2 * k + 1) * veo];

@@ -1,0 +1,363 @@
+// FP16 storage for the reduced-precision inner solves: Wilson eo spinors as
+// half precision with one FP32 scale per site, and 12-real links in half
+// precision.  All arithmetic is FP32; reductions accumulate in FP64.
+//
+// Spinor layout (52 bytes per eo site): 6 lanes of 4 halves, lane j holding
+// complex components 2j and 2j+1 (component = 3*spin + colour) as (re, im, re,
+// im), stored [6][Veo]; then float norm[Veo].  The value of a component is
+// half * norm, with norm the largest |re| or |im| on the site, so the stored
+// halves lie in [-1, 1] and keep 11 significant bits relative to the site's
+// largest entry.
+//
+// Link layout (24 bytes per link): [dir][parity][3][Veo] of 4 halves, lane k
+// holding entries 2k, 2k+1 of rows 0 and 1; row 2 = conj(row0 x row1).
+#include "cl2_device.hpp"
+#include "cl2_internal.hpp"
+
+namespace cl2 {
+
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ const float* half_norms(const void* p, uint32_t S)
+{
+    return reinterpret_cast<const float*>(static_cast<const char*>(p) + (size_t)48 * S);
+}
+__device__ __forceinline__ float* half_norms(void* p, uint32_t S)
+{
+    return reinterpret_cast<float*>(static_cast<char*>(p) + (size_t)48 * S);
+}
+
+__device__ __forceinline__ void load_site_h(const void* p, uint32_t S, uint32_t i, Cx<float> (&v)[12])
+{
+    const h4* l   = reinterpret_cast<const h4*>(p);
+    const float n = half_norms(p, S)[i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        const f4v f  = __builtin_convertvector(l[(size_t)j * S + i], f4v);
+        v[2 * j]     = Cx<float>{f.x * n, f.y * n};
+        v[2 * j + 1] = Cx<float>{f.z * n, f.w * n};
+    }
+}
+
+/* quantise one site; NaN and Inf stay visible as NaN after the round trip */
+__device__ __forceinline__ void store_site_h(void* p, uint32_t S, uint32_t i, const Cx<float> (&v)[12])
+{
+    float m = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
+    const float inv = m > 0.f ? 1.f / m : 0.f;
+    h4* l           = reinterpret_cast<h4*>(p);
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+        f4v f{v[2 * j].re * inv, v[2 * j].im * inv, v[2 * j + 1].re * inv, v[2 * j + 1].im * inv};
+        l[(size_t)j * S + i] = __builtin_convertvector(f, h4);
+    }
+    half_norms(p, S)[i] = m;
+}
+
+__device__ __forceinline__ void load_link_h(const void* g, uint32_t S, int dir, int par, uint32_t eo, Su3<float>& u)
+{
+    const h4* p = reinterpret_cast<const h4*>(g) + (size_t)((dir * 2 + par) * 3) * S + eo;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const f4v f                           = __builtin_convertvector(__builtin_nontemporal_load(p + (size_t)k * S), f4v);
+        u.e[(2 * k) / 3][(2 * k) % 3]         = Cx<float>{f.x, f.y};
+        u.e[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<float>{f.z, f.w};
+    }
+    su3_reconstruct_row2(u);
+}
+
+/* one projector row pair of a hop on a neighbour spinor already in registers
+ * (same algebra as the FP64/FP32 kernel: h = psi_A + C psi_B, r = w U h) */
+template<int A, int B, int CU, bool FWD>
+__device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const Cx<float> (&psi)[12], const Su3<float>& u,
+                                           Cx<float> w)
+{
+    Cx<float> h[3], r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) h[c] = add_unit<CU>(psi[3 * A + c], psi[3 * B + c]);
+    if constexpr (FWD)
+        su3_times(u, h, r);
+    else
+        su3dag_times(u, h, r);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        Cx<float> t    = cmul(r[c], w);
+        acc[3 * A + c] = acc[3 * A + c] + t;
+        acc[3 * B + c] = add_unit<UnitConj<CU>::v>(acc[3 * B + c], t);
+    }
+}
+
+template<int DIR, bool FWD>
+__device__ __forceinline__ void hop_h(Cx<float> (&acc)[12], const void* in, uint32_t S, uint32_t nb, const void* gauge,
+                                      int linkpar, uint32_t linkeo, Cx<float> w)
+{
+    Su3<float> u;
+    load_link_h(gauge, S, DIR, linkpar, linkeo, u);
+    Cx<float> psi[12];
+    load_site_h(in, S, nb, psi);
+    if constexpr (DIR == 0) {
+        half_hop_r<0, 2, FWD ? P1 : M1, FWD>(acc, psi, u, w);
+        half_hop_r<1, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w);
+    } else if constexpr (DIR == 1) {
+        half_hop_r<0, 3, FWD ? PI : MI, FWD>(acc, psi, u, w);
+        half_hop_r<1, 2, FWD ? PI : MI, FWD>(acc, psi, u, w);
+    } else if constexpr (DIR == 2) {
+        half_hop_r<0, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w);
+        half_hop_r<1, 2, FWD ? M1 : P1, FWD>(acc, psi, u, w);
+    } else {
+        half_hop_r<0, 2, FWD ? PI : MI, FWD>(acc, psi, u, w);
+        half_hop_r<1, 3, FWD ? MI : PI, FWD>(acc, psi, u, w);
+    }
+}
+
+template<int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_dslash_eo_h(DslashArgs a)
+{
+    __shared__ double lds[2 * BLOCK / 64];
+    const uint32_t S = a.g.veo;
+    uint32_t i       = 0;
+    double nrm       = 0.;
+    if (map_site<BLOCK>(a.map, S, blockIdx.x, threadIdx.x, i)) {
+        SiteNb nb;
+        site_neighbours(a.g, i, a.out_parity, nb);
+        const int P  = a.out_parity;
+        const int Pn = P ^ 1;
+        Cx<float> acc[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[k] = Cx<float>{0.f, 0.f};
+        hop_h<0, true>(acc, a.in, S, nb.up[0], a.gauge, P, i, Cx<float>{float(a.w.wf[0][0]), float(a.w.wf[0][1])});
+        hop_h<0, false>(acc, a.in, S, nb.dn[0], a.gauge, Pn, nb.dn[0], Cx<float>{float(a.w.wb[0][0]), float(a.w.wb[0][1])});
+        hop_h<1, true>(acc, a.in, S, nb.up[1], a.gauge, P, i, Cx<float>{float(a.w.wf[1][0]), float(a.w.wf[1][1])});
+        hop_h<1, false>(acc, a.in, S, nb.dn[1], a.gauge, Pn, nb.dn[1], Cx<float>{float(a.w.wb[1][0]), float(a.w.wb[1][1])});
+        hop_h<2, true>(acc, a.in, S, nb.up[2], a.gauge, P, i, Cx<float>{float(a.w.wf[2][0]), float(a.w.wf[2][1])});
+        hop_h<2, false>(acc, a.in, S, nb.dn[2], a.gauge, Pn, nb.dn[2], Cx<float>{float(a.w.wb[2][0]), float(a.w.wb[2][1])});
+        hop_h<3, true>(acc, a.in, S, nb.up[3], a.gauge, P, i, Cx<float>{float(a.w.wf[3][0]), float(a.w.wf[3][1])});
+        hop_h<3, false>(acc, a.in, S, nb.dn[3], a.gauge, Pn, nb.dn[3], Cx<float>{float(a.w.wb[3][0]), float(a.w.wb[3][1])});
+
+        const Cx<float> f1{float(a.f1[0]), float(a.f1[1])}, f2{float(a.f2[0]), float(a.f2[1])};
+        if (a.epilogue == EPI_MINV) {
+            const float s = float(a.scale);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) acc[k] = scale(cmul(acc[k], k < 6 ? f1 : f2), s);
+        } else if (a.epilogue == EPI_AEE) {
+            Cx<float> xv[12];
+            load_site_h(a.x, S, i, xv);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                acc[k] = cmul(xv[k], k < 6 ? f1 : f2) - acc[k];
+                if (a.gamma5 && k >= 6) acc[k] = Cx<float>{-acc[k].re, -acc[k].im};
+            }
+        }
+        store_site_h(a.out, S, i, acc);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) nrm += (double)acc[k].re * acc[k].re + (double)acc[k].im * acc[k].im;
+    }
+    if (a.norm_partials) {
+        double s = block_sum<BLOCK>(nrm, lds);
+        if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s, a.norm_partials[2 * blockIdx.x + 1] = 0.;
+    }
+}
+
+hipError_t launch_dslash_h(cl2_ctx* c, const DslashArgs& a)
+{
+    const unsigned nb = blocks_for(a.g.veo, DSLASH_BLOCK);
+    hipLaunchKernelGGL((k_dslash_eo_h<DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
+    return hipGetLastError();
+}
+
+/* ---------------- per-site BLAS on FP16 spinors ---------------- */
+
+__device__ __forceinline__ Cx<float> resolve_f(const cl2_scalar& s)
+{
+    switch (s.kind) {
+        case CL2_SCALAR_DEV_COMPLEX: return Cx<float>{float(s.dev[0]), float(s.dev[1])};
+        case CL2_SCALAR_DEV_REAL: return Cx<float>{float(s.dev[0]), 0.f};
+        case CL2_SCALAR_DEV_VECTOR: return Cx<float>{float(s.dev[s.index]), 0.f};
+        default: return Cx<float>{float(s.re), float(s.im)};
+    }
+}
+
+/* the Wilson modes of k_blas, one thread per site; a.n is the site count */
+template<int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_blas_h(BlasArgs a)
+{
+    __shared__ double lds[2 * BLOCK / 64];
+    const Cx<float> al = resolve_f(a.a), be = resolve_f(a.b);
+    const uint32_t S   = (uint32_t)a.n;
+    double nrm = 0., ry = 0.;
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < S; i += gridDim.x * BLOCK) {
+        Cx<float> x[12], r[12];
+        load_site_h(a.x, S, i, x);
+        if (a.mode == B_AX) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) r[k] = cmul(x[k], al);
+        } else if (a.mode == B_NEG_TAIL) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) r[k] = k < 6 ? x[k] : Cx<float>{-x[k].re, -x[k].im};
+        } else {
+            Cx<float> y[12];
+            load_site_h(a.y, S, i, y);
+            if (a.mode == B_AX_BY_Z) {
+                Cx<float> z[12];
+                load_site_h(a.z, S, i, z);
+#pragma unroll
+                for (int k = 0; k < 12; ++k) r[k] = cmul(y[k], be) + cmul(x[k], al) + z[k];
+            } else {  // B_Y_MINUS_AX, B_G5_Y_MINUS_AX
+                const bool g5 = a.mode == B_G5_Y_MINUS_AX;
+#pragma unroll
+                for (int k = 0; k < 12; ++k) {
+                    r[k] = y[k] - cmul(x[k], al);
+                    if (g5 && k >= 6) r[k] = Cx<float>{-r[k].re, -r[k].im};
+                    ry += (double)r[k].re * y[k].re + (double)r[k].im * y[k].im;
+                }
+            }
+        }
+        store_site_h(a.out, S, i, r);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) nrm += (double)r[k].re * r[k].re + (double)r[k].im * r[k].im;
+    }
+    if (a.norm_partials) {
+        double2 s = block_sum2<BLOCK>(make_double2(nrm, a.cg_norm ? ry : 0.), lds);
+        if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s.x, a.norm_partials[2 * blockIdx.x + 1] = s.y;
+    }
+}
+
+static unsigned site_blocks(uint32_t S)
+{
+    unsigned nb = blocks_for(S, BLAS_BLOCK);
+    return nb > (unsigned)REDUCE_MAX_BLOCKS ? (unsigned)REDUCE_MAX_BLOCKS : (nb == 0 ? 1u : nb);
+}
+
+hipError_t launch_blas_h(cl2_ctx* c, const BlasArgs& a, unsigned* nblocks_out)
+{
+    if (a.mode == B_AX_PLUS_Y || a.mode == B_AX_BY) return hipErrorInvalidValue;  // staggered modes
+    const unsigned nb = site_blocks((uint32_t)a.n);
+    if (nblocks_out) *nblocks_out = nb;
+    hipLaunchKernelGGL((k_blas_h<BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, a);
+    return hipGetLastError();
+}
+
+template<int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_reduce1_h(const void* x, const void* y, uint32_t S, int mode,
+                                                     double* __restrict__ partials)
+{
+    __shared__ double lds[2 * BLOCK / 64];
+    double2 acc = make_double2(0., 0.);
+    for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < S; i += gridDim.x * BLOCK) {
+        Cx<float> u[12];
+        load_site_h(x, S, i, u);
+        if (mode == R_NORM) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) acc.x += (double)u[k].re * u[k].re + (double)u[k].im * u[k].im;
+        } else {
+            Cx<float> v[12];
+            load_site_h(y, S, i, v);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                acc.x += (double)u[k].re * v[k].re + (double)u[k].im * v[k].im;
+                acc.y += (double)u[k].re * v[k].im - (double)v[k].re * u[k].im;
+            }
+        }
+    }
+    double2 s = block_sum2<BLOCK>(acc, lds);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x]     = s.x;
+        partials[2 * blockIdx.x + 1] = (mode == R_DOT) ? s.y : 0.;
+    }
+}
+
+hipError_t launch_reduce_stage1_h(cl2_ctx* c, const void* x, const void* y, int mode, unsigned* nblocks)
+{
+    const unsigned nb = site_blocks(c->g.veo);
+    *nblocks          = nb;
+    hipLaunchKernelGGL((k_reduce1_h<BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, x, y, c->g.veo, mode,
+                       c->partials);
+    return hipGetLastError();
+}
+
+/* site-diagonal twisted-mass factors: spins 0,1 times f1, spins 2,3 times f2, times s */
+__global__ __launch_bounds__(BLAS_BLOCK) void k_sitediag_h(void* out, const void* in, uint32_t S, Cx<float> f1,
+                                                           Cx<float> f2, float s)
+{
+    const uint32_t i = blockIdx.x * BLAS_BLOCK + threadIdx.x;
+    if (i >= S) return;
+    Cx<float> v[12];
+    load_site_h(in, S, i, v);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = scale(cmul(v[k], k < 6 ? f1 : f2), s);
+    store_site_h(out, S, i, v);
+}
+
+hipError_t launch_sitediag_h(cl2_ctx* c, void* out, const void* in, const double f1[2], const double f2[2], double s)
+{
+    hipLaunchKernelGGL(k_sitediag_h, dim3(blocks_for(c->g.veo, BLAS_BLOCK)), dim3(BLAS_BLOCK), 0, c->stream, out, in,
+                       c->g.veo, Cx<float>{float(f1[0]), float(f1[1])}, Cx<float>{float(f2[0]), float(f2[1])}, float(s));
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(BLAS_BLOCK) void k_fill_h(void* out, uint32_t S, float re, float im)
+{
+    const uint32_t i = blockIdx.x * BLAS_BLOCK + threadIdx.x;
+    if (i >= S) return;
+    Cx<float> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) v[k] = Cx<float>{re, im};
+    store_site_h(out, S, i, v);
+}
+
+hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im)
+{
+    hipLaunchKernelGGL(k_fill_h, dim3(blocks_for(c->g.veo, BLAS_BLOCK)), dim3(BLAS_BLOCK), 0, c->stream, out,
+                       c->g.veo, float(re), float(im));
+    return hipGetLastError();
+}
+
+/* FP64 / FP32 component-major spinor <-> FP16 with per-site norm */
+template<typename R>
+__global__ __launch_bounds__(BLAS_BLOCK) void k_to_half(void* out, const vec2<R>* __restrict__ in, uint32_t S)
+{
+    const uint32_t i = blockIdx.x * BLAS_BLOCK + threadIdx.x;
+    if (i >= S) return;
+    Cx<float> v[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+        vec2<R> q = in[(size_t)k * S + i];
+        v[k]      = Cx<float>{float(q.x), float(q.y)};
+    }
+    store_site_h(out, S, i, v);
+}
+template<typename R>
+__global__ __launch_bounds__(BLAS_BLOCK) void k_from_half(vec2<R>* __restrict__ out, const void* in, uint32_t S)
+{
+    const uint32_t i = blockIdx.x * BLAS_BLOCK + threadIdx.x;
+    if (i >= S) return;
+    Cx<float> v[12];
+    load_site_h(in, S, i, v);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) out[(size_t)k * S + i] = vec2<R>{R(v[k].re), R(v[k].im)};
+}
+
+hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec)
+{
+    const uint32_t S  = c->g.veo;
+    const unsigned nb = blocks_for(S, BLAS_BLOCK);
+    if (out_prec == CL2_F16 && in_prec == CL2_F64)
+        hipLaunchKernelGGL(k_to_half<double>, dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, out,
+                           reinterpret_cast<const double2*>(in), S);
+    else if (out_prec == CL2_F16 && in_prec == CL2_F32)
+        hipLaunchKernelGGL(k_to_half<float>, dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, out,
+                           reinterpret_cast<const float2*>(in), S);
+    else if (out_prec == CL2_F64 && in_prec == CL2_F16)
+        hipLaunchKernelGGL(k_from_half<double>, dim3(nb), dim3(BLAS_BLOCK), 0, c->stream,
+                           reinterpret_cast<double2*>(out), in, S);
+    else if (out_prec == CL2_F32 && in_prec == CL2_F16)
+        hipLaunchKernelGGL(k_from_half<float>, dim3(nb), dim3(BLAS_BLOCK), 0, c->stream,
+                           reinterpret_cast<float2*>(out), in, S);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace cl2

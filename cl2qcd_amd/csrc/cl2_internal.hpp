@@ -66,6 +66,7 @@ struct BlasArgs {
     cl2_scalar a, b;
     int mode;
     double* norm_partials;  // if set: per-block sum |out|^2
+    int cg_norm;            // with norm_partials: second partial = sum Re conj(out) y (y read before out is written)
 };
 hipError_t launch_blas(cl2_ctx* c, const BlasArgs& a, int prec, unsigned* nblocks_out);
 
@@ -74,7 +75,8 @@ enum ReduceMode : int { R_DOT = 0, R_DOT_RE = 1, R_NORM = 2 };
 hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t n, int prec, int mode,
                                 unsigned* nblocks);
 /* stage 2: sum nparts double2 partials in fixed order.  out_kind 0: res[0] = real part;
- * 1: res[0..1] = complex sum;  2: res[k] = alpha_vec[k]^2 * real part for k < nvec */
+ * 1: res[0..1] = complex sum;  2: res[k] = alpha_vec[k]^2 * real part for k < nvec;
+ * 3: res[0..3] = (re, im, re, 0) */
 hipError_t launch_reduce_stage2(cl2_ctx* c, unsigned nparts, double* res, int out_kind, const double* alpha_vec,
                                 int nvec);
 
@@ -84,6 +86,14 @@ hipError_t launch_gauge_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, do
 hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src);
 
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im);
+
+/* FP16 Wilson spinors (per-site scale) and F16_12 links */
+hipError_t launch_dslash_h(cl2_ctx* c, const DslashArgs& a);
+hipError_t launch_blas_h(cl2_ctx* c, const BlasArgs& a, unsigned* nblocks_out);  // a.n = sites
+hipError_t launch_reduce_stage1_h(cl2_ctx* c, const void* x, const void* y, int mode, unsigned* nblocks);
+hipError_t launch_sitediag_h(cl2_ctx* c, void* out, const void* in, const double f1[2], const double f2[2], double s);
+hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im);
+hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec);
 hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n);
 
 hipError_t launch_scalar_op(cl2_ctx* c, int op, int is_complex, double* out, const double* a, const double* b);

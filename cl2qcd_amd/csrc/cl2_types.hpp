@@ -12,6 +12,7 @@ enum GaugeFmt : int {
     GAUGE_F64_18 = CL2_GAUGE_F64_18,
     GAUGE_F64_12 = CL2_GAUGE_F64_12,
     GAUGE_F32_12 = CL2_GAUGE_F32_12,
+    GAUGE_F16_12 = CL2_GAUGE_F16_12,
 };
 
 struct LatticeGeom {

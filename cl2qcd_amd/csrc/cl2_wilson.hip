@@ -199,6 +199,7 @@ hipError_t launch_dslash(cl2_ctx* c, const DslashArgs& a, int format)
             hipLaunchKernelGGL((k_dslash_eo<GAUGE_F32_12, DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream,
                                a);
             break;
+        case GAUGE_F16_12: return launch_dslash_h(c, a);
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

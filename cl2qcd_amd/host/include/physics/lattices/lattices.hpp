@@ -75,15 +75,17 @@ class Gaugefield {
     std::vector<Matrixsu3> get_host() const;
     /* device pointer and link format of the resident FP64 field */
     const void* dev() const noexcept { return buf->ptr(); }
-    /* device pointer for in-place updates; drops the reduced-precision copy */
+    /* device pointer for in-place updates; drops the reduced-precision copies */
     void* dev_mutable()
     {
         buf_f32.reset();
+        buf_f16.reset();
         return buf->ptr();
     }
     int format() const noexcept { return fmt; }
-    /* FP32 12-real copy for reduced-precision operators, made on first use */
+    /* FP32 / FP16 12-real copies for reduced-precision operators, made on first use */
     const void* dev_f32() const;
+    const void* dev_f16() const;
     const hardware::System& get_system() const noexcept { return system; }
 
   private:
@@ -92,7 +94,8 @@ class Gaugefield {
     const hardware::System& system;
     int fmt;
     std::unique_ptr<DeviceBuffer> buf;
-    mutable std::unique_ptr<DeviceBuffer> buf_f32;
+    mutable std::unique_ptr<DeviceBuffer> buf_f32, buf_f16;
+    const void* reduced_copy(std::unique_ptr<DeviceBuffer>& cache, int fmt) const;
 };
 
 /* full-lattice spinor field, host resident: source of convert_to_eoprec */
@@ -170,6 +173,10 @@ void saxsbypz(const Spinorfield_eo* out, const Scalar<hmc_complex>& alpha, const
               const Scalar<hmc_complex>& beta, const Spinorfield_eo& y, const Spinorfield_eo& z);
 void saxpy_AND_squarenorm(const Spinorfield_eo* out, const Scalar<hmc_complex>& alpha, const Spinorfield_eo& x,
                           const Spinorfield_eo& y, const Scalar<hmc_complex>& squarenorm);
+/* out = y - alpha x; res[0..3] = (|out|^2, Re <out, y_old>, |out|^2, 0): the fused update of the
+ * reduced-precision CG */
+void saxpy_AND_cgnorm(const Spinorfield_eo* out, const Scalar<hmc_complex>& alpha, const Spinorfield_eo& x,
+                      const Spinorfield_eo& y, const Vector<hmc_float>& res);
 void saxpy_AND_gamma5_eo(const Spinorfield_eo* out, const hmc_complex alpha, const Spinorfield_eo& x,
                          const Spinorfield_eo& y);
 void convert_to_eoprec(const Spinorfield_eo* even, const Spinorfield_eo* odd, const Spinorfield& in);

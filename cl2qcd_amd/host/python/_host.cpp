@@ -74,6 +74,7 @@ PYBIND11_MODULE(_host, m)
     m.attr("ODD")  = ODD;
     m.attr("F64")  = CL2_F64;
     m.attr("F32")  = CL2_F32;
+    m.attr("F16")  = CL2_F16;
 
     py::class_<Lattice>(m, "Lattice", "run parameters + hardware::System + InterfacesHandler + PRNG")
         .def(py::init<const std::vector<std::string>&>(), py::arg("args") = std::vector<std::string>{})

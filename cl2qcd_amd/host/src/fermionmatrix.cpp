@@ -21,6 +21,7 @@ struct Links {
 Links links_for(const Gaugefield& gf, int prec)
 {
     if (prec == CL2_F32) return Links{gf.dev_f32(), CL2_GAUGE_F32_12};
+    if (prec == CL2_F16) return Links{gf.dev_f16(), CL2_GAUGE_F16_12};
     return Links{gf.dev(), gf.format()};
 }
 

@@ -161,6 +161,7 @@ void Gaugefield::import(const Matrixsu3* host) const
 {
     system.check(cl2_gauge_import(system.ctx(), buf->ptr(), reinterpret_cast<const double*>(host), fmt));
     buf_f32.reset();
+    buf_f16.reset();
 }
 
 std::vector<Matrixsu3> Gaugefield::get_host() const
@@ -170,21 +171,27 @@ std::vector<Matrixsu3> Gaugefield::get_host() const
     return h;
 }
 
-const void* Gaugefield::dev_f32() const
+const void* Gaugefield::reduced_copy(std::unique_ptr<DeviceBuffer>& cache, int rfmt) const
 {
-    if (!buf_f32) {
-        std::unique_ptr<DeviceBuffer> b(
-            new DeviceBuffer(system, cl2_field_bytes(system.ctx(), CL2_FIELD_GAUGE, CL2_GAUGE_F32_12)));
+    if (!cache) {
+        std::unique_ptr<DeviceBuffer> b(new DeviceBuffer(system, cl2_field_bytes(system.ctx(), CL2_FIELD_GAUGE, rfmt)));
         if (fmt == CL2_GAUGE_F64_18) {
-            system.check(cl2_gauge_convert(system.ctx(), b->ptr(), CL2_GAUGE_F32_12, buf->ptr()));
+            system.check(cl2_gauge_convert(system.ctx(), b->ptr(), rfmt, buf->ptr()));
         } else {
             std::vector<Matrixsu3> h = get_host();
-            system.check(cl2_gauge_import(system.ctx(), b->ptr(), reinterpret_cast<const double*>(h.data()),
-                                          CL2_GAUGE_F32_12));
+            system.check(cl2_gauge_import(system.ctx(), b->ptr(), reinterpret_cast<const double*>(h.data()), rfmt));
         }
-        buf_f32 = std::move(b);
+        cache = std::move(b);
     }
-    return buf_f32->ptr();
+    return cache->ptr();
+}
+const void* Gaugefield::dev_f32() const
+{
+    return reduced_copy(buf_f32, CL2_GAUGE_F32_12);
+}
+const void* Gaugefield::dev_f16() const
+{
+    return reduced_copy(buf_f16, CL2_GAUGE_F16_12);
 }
 
 /* ILDG/LIME reader (ildg_io/ildgIo_gaugefield.cpp:142-221, ildg_io/lime/limeFileReader.cpp:56-130):
@@ -406,6 +413,14 @@ void saxpy_AND_squarenorm(const Spinorfield_eo* out, const Scalar<hmc_complex>& 
     cl2_scalar a = dev(alpha);
     x.get_system().check(
         cl2_saxpy_AND_squarenorm_eo(ctx(x.get_system()), out->dev(), x.dev(), y.dev(), x.precision(), &a, sq.dev()));
+}
+void saxpy_AND_cgnorm(const Spinorfield_eo* out, const Scalar<hmc_complex>& alpha, const Spinorfield_eo& x,
+                      const Spinorfield_eo& y, const Vector<hmc_float>& res)
+{
+    if (res.get_vector_size() < 4) throw std::invalid_argument("saxpy_AND_cgnorm needs a result vector of 4 elements");
+    cl2_scalar a = dev(alpha);
+    x.get_system().check(
+        cl2_saxpy_AND_cgnorm_eo(ctx(x.get_system()), out->dev(), x.dev(), y.dev(), x.precision(), &a, res.dev()));
 }
 void saxpy_AND_gamma5_eo(const Spinorfield_eo* out, const hmc_complex alpha, const Spinorfield_eo& x,
                          const Spinorfield_eo& y)

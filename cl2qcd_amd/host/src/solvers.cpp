@@ -315,7 +315,13 @@ namespace solvers {
  * dropped by delta since the last update (or claims convergence), the partial
  * solution is folded into the FP64 solution and the residual is recomputed in
  * FP64 from b - A x, which also decides convergence.  Scalars (alpha, beta,
- * |r|^2) and all reductions are FP64. */
+ * |r|^2) and all reductions are FP64.
+ *
+ * Two standard measures keep the low-precision recurrence close to exact CG:
+ * beta is the Polak-Ribiere value <r_new, r_new - r_old> / |r_old|^2 (from the
+ * fused residual update; the Fletcher-Reeves |r_new|^2 / |r_old|^2 when that is
+ * negative), and after every reliable update the search direction is made
+ * orthogonal to the new residual again, p -= (<r, p> / |r|^2) r. */
 int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& A, const Gaugefield& gf,
                        const Spinorfield_eo& b, const hardware::System& system, hmc_float rel_tol,
                        const physics::AdditionalParameters& ap, const ReliableUpdateOptions& opt, SolveStats* stats)
@@ -326,8 +332,8 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
     const int lo = opt.inner_precision;
     const Spinorfield_eo r(system, CL2_F64), t64(system, CL2_F64);
     const Spinorfield_eo r_lo(system, lo), p(system, lo), v(system, lo), x_lo(system, lo);
-    const Scalar<hmc_complex> alpha(system), malpha(system), beta(system), mbeta(system), pAp(system), r2c(system),
-        r2new(system), minus_one(system);
+    const Scalar<hmc_complex> alpha(system), malpha(system), pAp(system), r2true(system), minus_one(system);
+    const Vector<hmc_float> norms(4, system);  // (|r|^2, Re <r_new, r_old>, |r|^2, 0) of the last update
     minus_one.store(hmc_complex_minusone);
     const hmc_complex one{1., 0.};
 
@@ -348,18 +354,23 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
     convert_precision(&r_lo, r);
     copyData(&p, r_lo);
     x_lo.zero();
-    r2c.store(hmc_complex{r2, 0.});
-    hmc_float rmax = std::sqrt(r2);
-    int k          = 0;
+    r2true.store(hmc_complex{r2, 0.});
+    const double* r2c = r2true.dev();  // |r|^2 of the current step as a complex device scalar
+    hmc_float rmax    = std::sqrt(r2);
+    int k             = 0;
     for (k = 0; k < opt.max_iterations; ++k) {
         A(&v, gf, p, ap);
         scalar_product(&pAp, p, v);
-        divide(&alpha, r2c, pAp);
+        system.check(cl2_complex_op(system.ctx(), CL2_SCALAR_OP_RATIO, alpha.dev(), r2c, pAp.dev()));
         multiply(&malpha, minus_one, alpha);
-        saxpy(&x_lo, malpha, p, x_lo);                  // x_lo += alpha p
-        saxpy_AND_squarenorm(&r_lo, alpha, v, r_lo, r2new);  // r_lo -= alpha v, |r_lo|^2
-        hmc_float rn = r2new.get().re;
+        saxpy(&x_lo, malpha, p, x_lo);                   // x_lo += alpha p
+        saxpy_AND_cgnorm(&r_lo, alpha, v, r_lo, norms);  // r_lo -= alpha v and its norms
+        const std::vector<hmc_float> nr = norms.get();
+        hmc_float rn                    = nr[0];
         if (rn != rn) throw SolverStuck(k, "cg_reliable_update: NAN occured");
+        r2c            = norms.dev() + 2;
+        hmc_float beta = (rn - nr[1]) / r2;  // Polak-Ribiere
+        if (!(beta >= 0.)) beta = rn / r2;
         if (std::sqrt(rn) < opt.delta * rmax || rn <= target) {
             convert_precision(&t64, x_lo);
             saxpy(x, hmc_complex{-1., 0.}, t64, *x);  // x += x_lo
@@ -371,14 +382,16 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
                 break;
             }
             convert_precision(&r_lo, r);
-            r2new.store(hmc_complex{rn, 0.});
+            // restore <r, p> = 0 for the new residual
+            const hmc_complex rp = scalar_product(r_lo, p);
+            saxpy(&p, hmc_complex{rp.re / rn, rp.im / rn}, r_lo, p);
+            beta = rn / r2;
             rmax = std::sqrt(rn);
+            r2true.store(hmc_complex{rn, 0.});
+            r2c = r2true.dev();
         }
         rmax = std::max(rmax, std::sqrt(rn));
-        divide(&beta, r2new, r2c);
-        multiply(&mbeta, minus_one, beta);
-        saxpy(&p, mbeta, p, r_lo);  // p = r_lo + beta p
-        copyData(&r2c, r2new);
+        saxpy(&p, hmc_complex{-beta, 0.}, p, r_lo);  // p = r_lo + beta p
         r2 = rn;
     }
     st.iterations = k;

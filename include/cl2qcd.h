@@ -11,10 +11,13 @@
  * Device layouts (all fields are dense, stride = number of eo sites Veo):
  *   Wilson eo spinor  F64: double2[12][Veo]  component c = 3*spin + colour
  *                     F32: float2 [12][Veo]
+ *                     F16: half[6][Veo][4] (components 2j, 2j+1 as re,im,re,im) then float norm[Veo];
+ *                          component = half * norm (norm = largest |re|,|im| on the site)
  *   staggered eo vec  F64: double2[3][Veo],  F32: float2[3][Veo]
  *   gauge field      F64_18: double2[4 dir][2 parity][9][Veo]   (row-major 3x3)
  *                    F64_12: double2[4][2][6][Veo]   rows 0,1; row 2 = conj(row0 x row1)
  *                    F32_12: float4 [4][2][3][Veo]   rows 0,1 packed two entries per float4
+ *                    F16_12: half[4][2][3][Veo][4]   like F32_12 in half precision
  *   device scalars: complex = 2 doubles (re, im); real = 1 double.
  * Host arrays use the reference's AoS order: site = x + NS*(y + NS*(z + NS*t)),
  * eo index = site/2, link index = mu + 4*site with mu = T,X,Y,Z.
@@ -44,9 +47,11 @@ extern "C" {
 /* ---- precisions and gauge formats ---- */
 #define CL2_F64 0
 #define CL2_F32 1
+#define CL2_F16 2 /* Wilson eo spinors only: half storage with one float scale per site, FP32 arithmetic */
 #define CL2_GAUGE_F64_18 0
 #define CL2_GAUGE_F64_12 1
 #define CL2_GAUGE_F32_12 2
+#define CL2_GAUGE_F16_12 3 /* Wilson operators on F16 spinors */
 
 /* ---- field kinds for cl2_field_bytes ---- */
 #define CL2_FIELD_SPINOR_EO 0
@@ -197,6 +202,12 @@ int cl2_squarenorm_eo(cl2_ctx* ctx, double* res, const void* x, int prec);
 /* saxpy_AND_squarenorm_eo (spinors.cpp:858-916): out = y - alpha x; res = |out|^2 (complex (norm, 0)) */
 int cl2_saxpy_AND_squarenorm_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec,
                                 const cl2_scalar* alpha, double* res);
+/* No counterpart in the reference: the fused update of the reduced-precision CG.  out = y - alpha x (out may
+ * alias y); res (4 device doubles) = (|out|^2, Re <out, y>, |out|^2, 0) with y read before it is overwritten:
+ * res[0] - res[1] = <r_new, r_new - r_old> gives the Polak-Ribiere beta of a CG step, and res + 2 is the new
+ * |r|^2 as a complex device scalar for the next step's alpha. */
+int cl2_saxpy_AND_cgnorm_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha,
+                            double* res);
 /* set_zero_spinorfield_eoprec / set_eoprec_spinorfield_cold (spinors.cpp:448-460,805-817):
  * cold = every complex component 1/sqrt(12 V) */
 int cl2_set_zero_eo(cl2_ctx* ctx, void* out, int prec);

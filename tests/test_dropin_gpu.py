@@ -293,3 +293,20 @@ def test_cg_reliable_update(action, op):
     H.cg(x64, A, gf, b, l, 1e-24 * H.squarenorm(b), l.wilson_parameters())
     x64h = x64.get_host()
     assert np.sqrt(((xh - x64h) ** 2).sum() / (x64h ** 2).sum()) < 1e-6
+
+
+def test_cg_reliable_update_half():
+    """FP16 (per-site scale) inner CG with FP64 reliable updates reaches |b - A x| / |b| <= 1e-10 in FP64."""
+    l = lattice("--nSpace=8", "--nTime=8", "--fermionAction=twistedmass", "--kappa=0.15", "--mu=0.02", "--beta=5.7")
+    gf = H.Gaugefield(l, False)
+    for _ in range(3):
+        H.su3heatbath(gf, l, 1)
+    b, _ = eo_pair(l, 52)
+    x = H.Spinorfield_eo(l)
+    A = H.QplusQminus_eo(l)
+    st = H.cg_reliable_update(x, A, gf, b, l, 1e-10, l.wilson_parameters(), H.F16)
+    assert st["true_relative_residual"] <= 1e-10 and st["reliable_updates"] >= 3
+    lat = O.lattice(8, 8)
+    bh, xh = b.get_host(), x.get_host()
+    r = O.saxpy(O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, xh, gf.get_host(), O.TWISTEDMASS, 0.15, l.mubar), bh, 1.0)
+    assert np.sqrt(O.norm2(r) / O.norm2(bh)) <= 1.01e-10
