@@ -140,6 +140,87 @@ def cg_leg(ns, nt, sweeps, overrelax, seed):
     return out
 
 
+def batch_leg(ns, nt, sweeps, seed, nrhs=12):
+    """12 point sources (every spin x colour at the origin) on QplusQminus_eo, twisted mass kappa 0.16,
+    mu 0.005, heatbath beta 5.7: one lockstep block solve with the fused multi-RHS operator against 12
+    single-source solves, both FP32 inner / FP64 reliable updates to |r|/|b| <= 1e-10.  Also the per-RHS
+    cost of one FP32 hopping application with 12 right-hand sides against one with a single field."""
+    from cl2qcd_amd import _host as H, abi
+
+    l = H.Lattice([f"--nSpace={ns}", f"--nTime={nt}", "--beta=5.7", "--fermionAction=twistedmass",
+                   "--kappa=0.16", "--mu=0.005", "--thetaFermionTemporal=1", "--solverMaxIterations=200000",
+                   f"--hostSeed={seed}"])
+    gf = H.Gaugefield(l, False)
+    for _ in range(sweeps):
+        H.su3heatbath(gf, l, 1)
+    qq, ap = H.QplusQminus_eo(l), l.wilson_parameters()
+    veo = l.eo_sites
+    bs, xs = [], []
+    for r in range(nrhs):
+        src = np.zeros((veo, 24))
+        src[0, 2 * r] = 1.0  # eo site 0 is the origin (even); component r = 3 * spin + colour
+        b = H.Spinorfield_eo(l)
+        b.import_(src)
+        bs.append(b)
+        x = H.Spinorfield_eo(l)
+        x.zero()
+        xs.append(x)
+    l.synchronize()
+    t0 = time.perf_counter()
+    seq_it = []
+    for r in range(nrhs):
+        st = H.cg_reliable_update(xs[r], qq, gf, bs[r], l, 1e-10, ap, H.F32, 0.1)
+        seq_it.append(st["iterations"])
+    l.synchronize()
+    t_seq = time.perf_counter() - t0
+    for x in xs:
+        x.zero()
+    l.synchronize()
+    t0 = time.perf_counter()
+    it, per = H.cg_reliable_update_block(xs, qq, gf, bs, l, 1e-10, ap, H.F32, 0.1)
+    l.synchronize()
+    t_blk = time.perf_counter() - t0
+    worst = max(p["true_relative_residual"] for p in per)
+    log(f"12 point sources {ns}^3x{nt}: sequential {t_seq:.2f} s ({sum(seq_it)} iterations), block {t_blk:.2f} s "
+        f"({it} lockstep iterations), worst true |r|/|b| {worst:.2e}")
+    # hopping term alone, FP32 12-real links: 12 right-hand sides in one launch vs one field
+    ctx = abi.Context(ns, nt, theta_t=1.0)
+    g = ctx.gauge(fmt=abi.GAUGE_F32_12, seed=3, epsilon=0.35)
+    one_b = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F32)
+    a1, b1 = ctx.buffer(one_b), ctx.buffer(one_b)
+    an, bn = ctx.buffer(nrhs * one_b), ctx.buffer(nrhs * one_b)
+    ctx.set_cold_eo(a1.ptr, abi.F32)
+    ctx.block_set_zero_eo(an.ptr, abi.F32, nrhs)
+
+    def rate(fn, reps=50):
+        fn()
+        ctx.sync()
+        ctx.timer_start()
+        for _ in range(reps):
+            fn()
+        ms = ctypes.c_double()
+        ctx.check(ctx.lib.cl2_timer_stop(ctx.handle, ctypes.byref(ms)))
+        return ms.value / reps
+
+    t1 = rate(lambda: ctx.dslash_eo(b1.ptr, a1.ptr, g.ptr, abi.GAUGE_F32_12, abi.EVEN, 0.16))
+    tn = rate(lambda: ctx.dslash_eo_mrhs(bn.ptr, an.ptr, g.ptr, abi.GAUGE_F32_12, abi.EVEN, 0.16, nrhs))
+    alg_n = veo * (8 * 48 + 2 * nrhs * 96)
+    ctx.close()
+    log(f"FP32 hopping term: 1 field {t1:.4f} ms, {nrhs} fields {tn:.4f} ms ({tn / nrhs:.4f} ms per field, "
+        f"{alg_n / (tn * 1e6):.0f} GB/s algorithmic)")
+    return {"lattice": f"{ns}^3x{nt}", "sources": f"{nrhs} point sources (spin x colour) at the origin",
+            "operator": "QplusQminus_eo twisted mass kappa 0.16 mu 0.005, FP32 inner, FP64 reliable updates",
+            "sequential": {"seconds": round(t_seq, 3), "per_rhs_seconds": round(t_seq / nrhs, 4),
+                           "iterations": seq_it},
+            "block": {"seconds": round(t_blk, 3), "per_rhs_seconds": round(t_blk / nrhs, 4),
+                      "lockstep_iterations": it, "iterations": [p["iterations"] for p in per],
+                      "worst_true_rel_residual": worst},
+            "speedup_per_rhs": round(t_seq / t_blk, 3),
+            "dslash_f32_ms": {"single": round(t1, 5), f"mrhs{nrhs}": round(tn, 5),
+                              f"mrhs{nrhs}_per_rhs": round(tn / nrhs, 5),
+                              f"mrhs{nrhs}_alg_GBps": round(alg_n / (tn * 1e6), 1)}}
+
+
 def read_counters():
     """HBM bytes per launch of the FP64 dslash from the committed rocprofv3 counter summary."""
     if not os.path.exists(COUNTERS_FILE):
@@ -188,6 +269,7 @@ def main():
     ap.add_argument("--only-f64", action="store_true", help="time only the FP64 18-real leg")
     ap.add_argument("--no-cg", action="store_true", help="skip the FP64 vs mixed-precision CG comparison")
     ap.add_argument("--hb-sweeps", type=int, default=50, help="heatbath sweeps for the CG configuration")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 12-source block solve at 32^3x64")
     ap.add_argument("--hb-overrelax", type=int, default=1, help="overrelaxation sweeps per heatbath sweep")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -226,6 +308,10 @@ def main():
     if not args.no_cg:
         cg = cg_leg(args.ns, args.nt, args.hb_sweeps, args.hb_overrelax, 20261202)
 
+    batch = None
+    if not args.no_batch:
+        batch = batch_leg(32, 64, args.hb_sweeps, 20261203)
+
     traffic = read_counters()
     achieved = f64["alg_bytes"] / (f64["ms"] * 1e6)
     line = {
@@ -252,6 +338,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         **extra,
         "cg": cg,
+        "batch": batch,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores) ...")

@@ -100,6 +100,17 @@ SIGNATURES = {
     "cl2_squarenorm_eo": (_i, [_vp, _dp, _vp, _i]),
     "cl2_saxpy_AND_squarenorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
     "cl2_saxpy_AND_cgnorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
+    "cl2_fermionmatrix_eo_mrhs": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i]),
+    "cl2_dslash_eo_mrhs": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _i]),
+    "cl2_block_insert_eo": (_i, [_vp, _vp, _vp, _i, _i, _i]),
+    "cl2_block_extract_eo": (_i, [_vp, _vp, _vp, _i, _i, _i]),
+    "cl2_block_axpy_eo": (_i, [_vp, _vp, _vp, _vp, _i, _i, _dp, _d, _dp]),
+    "cl2_block_dot_eo": (_i, [_vp, _dp, _vp, _vp, _i, _i]),
+    "cl2_block_squarenorm_eo": (_i, [_vp, _dp, _vp, _i, _i]),
+    "cl2_block_convert_eo": (_i, [_vp, _vp, _i, _vp, _i, _i]),
+    "cl2_block_set_zero_eo": (_i, [_vp, _vp, _i, _i]),
+    "cl2_block_cg_alpha": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
+    "cl2_block_cg_beta": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
     "cl2_set_zero_eo": (_i, [_vp, _vp, _i]),
     "cl2_set_cold_eo": (_i, [_vp, _vp, _i]),
     "cl2_convert_precision_eo": (_i, [_vp, _i, _vp, _i, _vp, _i]),

@@ -162,9 +162,12 @@ int run_reduce(cl2_ctx* c, double* res, const void* x, const void* y, size_t n, 
 }
 
 int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void* gauge, int fmt, int evenodd,
-               double kappa, int epilogue, const double f1[2], const double f2[2], double scale, int gamma5)
+               double kappa, int epilogue, const double f1[2], const double f2[2], double scale, int gamma5,
+               int nrhs = 1)
 {
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && (nrhs == 1 || fmt != CL2_GAUGE_F16_12),
+                "unsupported number of right-hand sides (1, 2, 4, 8, 12 or 16; FP16 fields: 1)");
     CL2_REQUIRE(c, evenodd == CL2_EVEN || evenodd == CL2_ODD, "evenodd must be CL2_EVEN or CL2_ODD");
     CL2_REQUIRE(c, out && in && gauge, "null field pointer");
     CL2_REQUIRE(c, out != in, "dslash cannot run in place: out aliases in");
@@ -186,7 +189,8 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.scale         = scale;
     a.norm_partials = nullptr;
     a.w             = wilson_weights(c, kappa);
-    a.map           = work_map(c, DSLASH_BLOCK);
+    a.map           = work_map(c, DSLASH_BLOCK, nrhs);
+    a.nrhs          = nrhs;
     CL2_HIP(c, launch_dslash(c, a, fmt));
     return CL2_OK;
 }
@@ -222,21 +226,49 @@ void tm_factors(int variant, double mubar, double f1[2], double f2[2], double* s
 /* Aee family with two fused kernels: tmp = M^-1 D in (Wilson: D in), then
  * out = M_site in - D tmp [gamma5]   (fermionmatrix.cpp:224-591). */
 int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge, int fmt, int action, double kappa,
-            double mubar, bool minus, bool gamma5)
+            double mubar, bool minus, bool gamma5, int nrhs = 1)
 {
     double f1[2], f2[2], s;
     int rc;
     if (action == CL2_ACTION_WILSON) {
         const double one[2] = {1., 0.};
-        rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_PLAIN, one, one, 1., 0);
+        rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_PLAIN, one, one, 1., 0, nrhs);
         if (rc) return rc;
-        return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, one, one, 1., gamma5 ? 1 : 0);
+        return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, one, one, 1., gamma5 ? 1 : 0, nrhs);
     }
     tm_factors(minus ? CL2_MTM_INVERSE_SITEDIAGONAL_MINUS : CL2_MTM_INVERSE_SITEDIAGONAL, mubar, f1, f2, &s);
-    rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_MINV, f1, f2, s, 0);
+    rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_MINV, f1, f2, s, 0, nrhs);
     if (rc) return rc;
     tm_factors(minus ? CL2_MTM_SITEDIAGONAL_MINUS : CL2_MTM_SITEDIAGONAL, mubar, f1, f2, &s);
-    return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, f1, f2, 1., gamma5 ? 1 : 0);
+    return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, f1, f2, 1., gamma5 ? 1 : 0, nrhs);
+}
+
+/* the eo operators on nrhs right-hand sides ([12][Veo][nrhs] fields, tmp = 2 such fields) */
+int run_fermionmatrix(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt, int action,
+                      double kappa, double mubar, void* tmp, int nrhs)
+{
+    CL2_REQUIRE(c, op >= CL2_OP_AEE && op <= CL2_OP_QPLUSQMINUS_EO, "unknown operator");
+    CL2_REQUIRE(c, action == CL2_ACTION_WILSON || action == CL2_ACTION_TWISTEDMASS,
+                "unknown fermion action: wilson or twistedmass");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, out && in && gauge && tmp, "null field pointer");
+    const size_t fb = spinor_bytes(c, fmt_prec(fmt)) * (size_t)nrhs;
+    void* t0        = tmp;
+    void* t1        = static_cast<char*>(tmp) + fb;
+    CL2_REQUIRE(c, out != t0 && out != t1 && in != t0 && in != t1, "tmp must not alias in or out");
+    switch (op) {
+        case CL2_OP_AEE: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, false, nrhs);
+        case CL2_OP_AEE_AND_GAMMA5:
+        case CL2_OP_QPLUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, true, nrhs);
+        case CL2_OP_AEE_MINUS: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, false, nrhs);
+        case CL2_OP_AEE_MINUS_AND_GAMMA5:
+        case CL2_OP_QMINUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs);
+        default: {
+            int rc = run_aee(c, t1, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs);
+            if (rc) return rc;
+            return run_aee(c, out, t1, t0, gauge, fmt, action, kappa, mubar, false, true, nrhs);
+        }
+    }
 }
 
 }  // namespace
@@ -278,7 +310,9 @@ int cl2_ctx_create(cl2_ctx** out, int device, const cl2_lattice* lat)
     c->cp_fwd[1] = er * ci;
     c->cp_bwd[0] = mer * cr;
     c->cp_bwd[1] = mer * ci;
-    c->npartials = std::max<size_t>(REDUCE_MAX_BLOCKS, blocks_for(c->g.veo, DSLASH_BLOCK));
+    // one double2 per block: REDUCE_MAX_BLOCKS blocks for each of up to 16 right-hand sides, or one per
+    // hopping-kernel block
+    c->npartials = std::max<size_t>((size_t)REDUCE_MAX_BLOCKS * 16, blocks_for(c->g.veo, DSLASH_BLOCK));
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, c->npartials * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
@@ -694,28 +728,24 @@ int cl2_fermionmatrix_eo(cl2_ctx* c, int op, void* out, const void* in, const vo
                          double kappa, double mubar, void* tmp)
 {
     if (!c) return CL2_E_ARG;
-    CL2_REQUIRE(c, op >= CL2_OP_AEE && op <= CL2_OP_QPLUSQMINUS_EO, "unknown operator");
-    CL2_REQUIRE(c, action == CL2_ACTION_WILSON || action == CL2_ACTION_TWISTEDMASS,
-                "unknown fermion action: wilson or twistedmass");
-    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
-    CL2_REQUIRE(c, out && in && gauge && tmp, "null field pointer");
-    const size_t fb = spinor_bytes(c, fmt_prec(fmt));
-    void* t0        = tmp;
-    void* t1        = static_cast<char*>(tmp) + fb;
-    CL2_REQUIRE(c, out != t0 && out != t1 && in != t0 && in != t1, "tmp must not alias in or out");
-    switch (op) {
-        case CL2_OP_AEE: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, false);
-        case CL2_OP_AEE_AND_GAMMA5:
-        case CL2_OP_QPLUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, true);
-        case CL2_OP_AEE_MINUS: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, false);
-        case CL2_OP_AEE_MINUS_AND_GAMMA5:
-        case CL2_OP_QMINUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, true);
-        default: {
-            int rc = run_aee(c, t1, in, t0, gauge, fmt, action, kappa, mubar, true, true);
-            if (rc) return rc;
-            return run_aee(c, out, t1, t0, gauge, fmt, action, kappa, mubar, false, true);
-        }
-    }
+    return run_fermionmatrix(c, op, out, in, gauge, fmt, action, kappa, mubar, tmp, 1);
+}
+
+int cl2_fermionmatrix_eo_mrhs(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt, int action,
+                              double kappa, double mubar, void* tmp, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && fmt != CL2_GAUGE_F16_12,
+                "unsupported number of right-hand sides (1, 2, 4, 8, 12 or 16; FP64/FP32 fields)");
+    return run_fermionmatrix(c, op, out, in, gauge, fmt, action, kappa, mubar, tmp, nrhs);
+}
+
+int cl2_dslash_eo_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, int evenodd, double kappa,
+                       int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    const double one[2] = {1., 0.};
+    return run_dslash(c, out, in, nullptr, gauge, fmt, evenodd, kappa, EPI_PLAIN, one, one, 1., 0, nrhs);
 }
 
 /* ---------------- Wilson BLAS ---------------- */
@@ -815,6 +845,98 @@ int cl2_convert_precision_eo(cl2_ctx* c, int kind, void* dst, int dst_prec, cons
         CL2_HIP(c, launch_convert_h(c, dst, dst_prec, src, src_prec));
     else
         CL2_HIP(c, launch_convert(c, dst, dst_prec, src, src_prec, n));
+    return CL2_OK;
+}
+
+/* ---------------- blocks of right-hand sides ---------------- */
+
+#define CL2_BLOCK_ARGS(prec, nrhs)                                                                        \
+    CL2_REQUIRE(c, valid_prec(prec), "block fields are FP64 or FP32");                                    \
+    CL2_REQUIRE(c, (nrhs) >= 1 && (nrhs) <= 64, "number of right-hand sides must be in [1, 64]")
+
+int cl2_block_insert_eo(cl2_ctx* c, void* block, const void* single, int prec, int nrhs, int r)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, block && single && r >= 0 && r < nrhs, "bad argument");
+    CL2_HIP(c, launch_block_slot(c, block, single, nullptr, prec, nrhs, r));
+    return CL2_OK;
+}
+
+int cl2_block_extract_eo(cl2_ctx* c, void* single, const void* block, int prec, int nrhs, int r)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, block && single && r >= 0 && r < nrhs, "bad argument");
+    CL2_HIP(c, launch_block_slot(c, const_cast<void*>(block), nullptr, single, prec, nrhs, r));
+    return CL2_OK;
+}
+
+int cl2_block_axpy_eo(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs, const double* coef,
+                      double sign, double* norms)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, out && x && y && coef, "null pointer");
+    CL2_HIP(c, launch_block_axpy(c, out, x, y, prec, nrhs, coef, sign, norms));
+    return CL2_OK;
+}
+
+int cl2_block_dot_eo(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, res && x && y, "null pointer");
+    CL2_HIP(c, launch_block_dot(c, res, x, y, prec, nrhs, 0));
+    return CL2_OK;
+}
+
+int cl2_block_squarenorm_eo(cl2_ctx* c, double* res, const void* x, int prec, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, res && x, "null pointer");
+    CL2_HIP(c, launch_block_dot(c, res, x, x, prec, nrhs, 1));
+    return CL2_OK;
+}
+
+int cl2_block_convert_eo(cl2_ctx* c, void* dst, int dst_prec, const void* src, int src_prec, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(dst_prec, nrhs);
+    CL2_REQUIRE(c, valid_prec(src_prec), "block fields are FP64 or FP32");
+    CL2_REQUIRE(c, dst && src && dst != src, "bad pointers");
+    const size_t n = (size_t)12 * c->g.veo * nrhs;
+    if (dst_prec == src_prec)
+        CL2_HIP(c, hipMemcpyAsync(dst, src, n * (dst_prec == CL2_F64 ? 16 : 8), hipMemcpyDeviceToDevice, c->stream));
+    else
+        CL2_HIP(c, launch_convert(c, dst, dst_prec, src, src_prec, n));
+    return CL2_OK;
+}
+
+int cl2_block_set_zero_eo(cl2_ctx* c, void* out, int prec, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, out != nullptr, "null pointer");
+    CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo * nrhs, prec, 0., 0.));
+    return CL2_OK;
+}
+
+int cl2_block_cg_alpha(cl2_ctx* c, double* alpha, const double* r2, const double* pAp, const double* active,
+                       int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, alpha && r2 && pAp && active && nrhs >= 1 && nrhs <= 64, "bad argument");
+    CL2_HIP(c, launch_block_cg_scalars(c, 0, alpha, const_cast<double*>(r2), pAp, active, nrhs));
+    return CL2_OK;
+}
+
+int cl2_block_cg_beta(cl2_ctx* c, double* beta, double* r2, const double* norms, const double* active, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, beta && r2 && norms && active && nrhs >= 1 && nrhs <= 64, "bad argument");
+    CL2_HIP(c, launch_block_cg_scalars(c, 1, beta, r2, norms, active, nrhs));
     return CL2_OK;
 }
 
@@ -1013,6 +1135,11 @@ int cl2_tune_set(cl2_ctx* c, const char* key, long value)
         c->tune_xcd = value ? 1 : 0;
         return CL2_OK;
     }
+    if (k == "mrhs.rb") {
+        CL2_REQUIRE(c, value == 0 || value == 1 || value == 2 || value == 4, "mrhs.rb must be 0, 1, 2 or 4");
+        c->tune_mrhs_rb = (int)value;
+        return CL2_OK;
+    }
     return fail(c, CL2_E_ARG, "unknown tuning key " + k);
 }
 
@@ -1022,6 +1149,7 @@ long cl2_tune_get(const cl2_ctx* c, const char* key)
     const std::string k(key);
     if (k == "hop.zplanes") return c->tune_zplanes;
     if (k == "hop.xcd") return c->tune_xcd;
+    if (k == "mrhs.rb") return c->tune_mrhs_rb;
     return -2;
 }
 

@@ -197,6 +197,34 @@ __device__ __forceinline__ bool map_site(const WorkMap& m, uint32_t veo, uint32_
     return true;
 }
 
+/* the same for NRHS right-hand sides per site: logical item l covers site l / NRHS, right-hand side
+ * l % NRHS, and the slab order applies to the site part */
+template<int BLOCK, int NRHS>
+__device__ __forceinline__ bool map_site_rhs(const WorkMap& m, uint32_t veo, uint32_t h, uint32_t tid, uint32_t& i,
+                                             uint32_t& rhs)
+{
+    if constexpr (NRHS == 1) {
+        rhs = 0;
+        return map_site<BLOCK>(m, veo, h, tid, i);
+    } else {
+        uint32_t L = h;
+        if (m.xcd_q && h < 8u * m.xcd_q) L = (h & 7u) * m.xcd_q + (h >> 3);
+        const uint32_t l = L * BLOCK + tid;
+        if (l >= veo * NRHS) return false;
+        const uint32_t ls = l / NRHS;
+        rhs               = l - ls * NRHS;
+        if (m.P == 0) {
+            i = ls;
+            return true;
+        }
+        const uint32_t zc = ls / m.col;
+        const uint32_t r  = ls - zc * m.col;
+        const uint32_t t  = r / m.P;
+        i                 = t * m.half_vs + zc * m.P + (r - t * m.P);
+        return true;
+    }
+}
+
 __device__ __forceinline__ void site_coords(const LatticeGeom& g, uint32_t i, int parity, uint32_t& x, uint32_t& y,
                                             uint32_t& z, uint32_t& t)
 {

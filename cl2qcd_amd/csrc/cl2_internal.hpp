@@ -20,6 +20,7 @@ struct cl2_ctx {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int tune_zplanes = -1;  // z-planes per slab of the hopping kernels (-1: heuristic, 0: identity order)
     int tune_xcd     = 1;   // XCD-contiguous block assignment
+    int tune_mrhs_rb = 0;   // right-hand sides per thread of the multi-RHS hopping kernel (0: by format)
     std::string err;
 };
 
@@ -35,7 +36,9 @@ inline unsigned blocks_for(size_t n, int block)
 }
 
 /* work order of the hopping kernels for this context's lattice and tuning */
-WorkMap work_map(const cl2_ctx* c, int block);
+WorkMap work_map(const cl2_ctx* c, int block, int nrhs = 1);
+/* right-hand-side counts with a compiled multi-RHS hopping kernel */
+bool supported_nrhs(int n);
 
 /* hopping weights -kappa * b_mu * c_mu (forward) and -kappa * conj(b_mu * c_mu') (backward) */
 HopWeights<double> wilson_weights(const cl2_ctx* c, double kappa);
@@ -95,6 +98,15 @@ hipError_t launch_sitediag_h(cl2_ctx* c, void* out, const void* in, const double
 hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im);
 hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec);
 hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n);
+
+/* blocks of nrhs Wilson spinors [12][Veo][nrhs] */
+hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,
+                             const double* coef, double sign, double* norms);
+hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode);
+hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, void* single_out, int prec, int nrhs,
+                             int r);
+hipError_t launch_block_cg_scalars(cl2_ctx* c, int which, double* out, double* r2, const double* in,
+                                   const double* active, int nrhs);
 
 hipError_t launch_scalar_op(cl2_ctx* c, int op, int is_complex, double* out, const double* a, const double* b);
 hipError_t launch_vec_element(cl2_ctx* c, double* out, const double* in, int out_index, int in_index);

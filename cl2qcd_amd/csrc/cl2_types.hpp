@@ -63,6 +63,7 @@ struct DslashArgs {
     double* norm_partials;  // if non-null: per-block sum |out|^2
     HopWeights<double> w;
     WorkMap map;
+    int nrhs;  // right-hand sides per launch (fields [12][Veo][nrhs]); 1 for single fields
 };
 
 }  // namespace cl2

@@ -6,6 +6,8 @@
 // site's own eo index and the backward link shares the neighbour's, so link
 // loads are as coalesced as the spinor gathers.  The kernel is HBM bound
 // (8 links + 2 spinors per site), so no MFMA.
+#include <type_traits>
+
 #include "cl2_device.hpp"
 #include "cl2_internal.hpp"
 
@@ -17,7 +19,7 @@ namespace cl2 {
  * reproduce dslash_eoprec_unified_local (ocl_kernel/fermionmatrix_eo.cl:30-253);
  * w already carries -kappa and the boundary phase. */
 template<int A, int B, int CU, bool FWD, typename R>
-__device__ __forceinline__ void half_hop(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, uint32_t S, uint32_t nb,
+__device__ __forceinline__ void half_hop(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, size_t S, uint32_t nb,
                                          const Su3<R>& u, Cx<R> w)
 {
     Cx<R> h[3], r[3];
@@ -39,38 +41,44 @@ __device__ __forceinline__ void half_hop(Cx<R> (&acc)[12], const vec2<R>* __rest
     }
 }
 
+/* SS: component stride of the spinor field (Veo * nrhs), nb: its element index; the link is
+ * element linkeo of the link field of stride Veo */
 template<int DIR, bool FWD, int FMT, typename R>
-__device__ __forceinline__ void hop(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, uint32_t S, uint32_t nb,
-                                    const void* gauge, int linkpar, uint32_t linkeo, Cx<R> w)
+__device__ __forceinline__ void hop(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, size_t SS, uint32_t nb,
+                                    const void* gauge, uint32_t S, int linkpar, uint32_t linkeo, Cx<R> w)
 {
     Su3<R> u;
     GaugeLoad<FMT>::load(gauge, S, DIR, linkpar, linkeo, u);
     // projector rows of (1 - gamma_mu) forward and (1 + gamma_mu) backward, chiral basis
     if constexpr (DIR == 0) {
-        half_hop<0, 2, FWD ? P1 : M1, FWD>(acc, in, S, nb, u, w);
-        half_hop<1, 3, FWD ? P1 : M1, FWD>(acc, in, S, nb, u, w);
+        half_hop<0, 2, FWD ? P1 : M1, FWD>(acc, in, SS, nb, u, w);
+        half_hop<1, 3, FWD ? P1 : M1, FWD>(acc, in, SS, nb, u, w);
     } else if constexpr (DIR == 1) {
-        half_hop<0, 3, FWD ? PI : MI, FWD>(acc, in, S, nb, u, w);
-        half_hop<1, 2, FWD ? PI : MI, FWD>(acc, in, S, nb, u, w);
+        half_hop<0, 3, FWD ? PI : MI, FWD>(acc, in, SS, nb, u, w);
+        half_hop<1, 2, FWD ? PI : MI, FWD>(acc, in, SS, nb, u, w);
     } else if constexpr (DIR == 2) {
-        half_hop<0, 3, FWD ? P1 : M1, FWD>(acc, in, S, nb, u, w);
-        half_hop<1, 2, FWD ? M1 : P1, FWD>(acc, in, S, nb, u, w);
+        half_hop<0, 3, FWD ? P1 : M1, FWD>(acc, in, SS, nb, u, w);
+        half_hop<1, 2, FWD ? M1 : P1, FWD>(acc, in, SS, nb, u, w);
     } else {
-        half_hop<0, 2, FWD ? PI : MI, FWD>(acc, in, S, nb, u, w);
-        half_hop<1, 3, FWD ? MI : PI, FWD>(acc, in, S, nb, u, w);
+        half_hop<0, 2, FWD ? PI : MI, FWD>(acc, in, SS, nb, u, w);
+        half_hop<1, 3, FWD ? MI : PI, FWD>(acc, in, SS, nb, u, w);
     }
 }
 
-template<int FMT, int BLOCK>
+/* NRHS right-hand sides in one launch: the fields are [12][Veo][NRHS] (right-hand side fastest) and
+ * thread l works on site l / NRHS, right-hand side l % NRHS, so the NRHS lanes of one site issue the same
+ * link addresses and each link leaves HBM once per launch.  NRHS = 1 is the single-field kernel. */
+template<int FMT, int BLOCK, int NRHS>
 __global__ __launch_bounds__(BLOCK) void k_dslash_eo(DslashArgs a)
 {
     using R = typename GaugeLoad<FMT>::R;
     using V = vec2<R>;
     __shared__ double lds[2 * BLOCK / 64];
     const uint32_t S = a.g.veo;
-    uint32_t i       = 0;
-    double nrm       = 0.;
-    if (map_site<BLOCK>(a.map, S, blockIdx.x, threadIdx.x, i)) {
+    const size_t SS  = (size_t)S * NRHS;
+    uint32_t i = 0, rhs = 0;
+    double nrm = 0.;
+    if (map_site_rhs<BLOCK, NRHS>(a.map, S, blockIdx.x, threadIdx.x, i, rhs)) {
         SiteNb nb;
         site_neighbours(a.g, i, a.out_parity, nb);
         const V* in    = reinterpret_cast<const V*>(a.in);
@@ -80,14 +88,19 @@ __global__ __launch_bounds__(BLOCK) void k_dslash_eo(DslashArgs a)
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] = Cx<R>{R(0), R(0)};
         // order T, X, Y, Z as fermionmatrix_eo_dslash.cl:38-45
-        hop<0, true, FMT>(acc, in, S, nb.up[0], a.gauge, P, i, Cx<R>{R(a.w.wf[0][0]), R(a.w.wf[0][1])});
-        hop<0, false, FMT>(acc, in, S, nb.dn[0], a.gauge, Pn, nb.dn[0], Cx<R>{R(a.w.wb[0][0]), R(a.w.wb[0][1])});
-        hop<1, true, FMT>(acc, in, S, nb.up[1], a.gauge, P, i, Cx<R>{R(a.w.wf[1][0]), R(a.w.wf[1][1])});
-        hop<1, false, FMT>(acc, in, S, nb.dn[1], a.gauge, Pn, nb.dn[1], Cx<R>{R(a.w.wb[1][0]), R(a.w.wb[1][1])});
-        hop<2, true, FMT>(acc, in, S, nb.up[2], a.gauge, P, i, Cx<R>{R(a.w.wf[2][0]), R(a.w.wf[2][1])});
-        hop<2, false, FMT>(acc, in, S, nb.dn[2], a.gauge, Pn, nb.dn[2], Cx<R>{R(a.w.wb[2][0]), R(a.w.wb[2][1])});
-        hop<3, true, FMT>(acc, in, S, nb.up[3], a.gauge, P, i, Cx<R>{R(a.w.wf[3][0]), R(a.w.wf[3][1])});
-        hop<3, false, FMT>(acc, in, S, nb.dn[3], a.gauge, Pn, nb.dn[3], Cx<R>{R(a.w.wb[3][0]), R(a.w.wb[3][1])});
+        hop<0, true, FMT>(acc, in, SS, nb.up[0] * NRHS + rhs, a.gauge, S, P, i, Cx<R>{R(a.w.wf[0][0]), R(a.w.wf[0][1])});
+        hop<0, false, FMT>(acc, in, SS, nb.dn[0] * NRHS + rhs, a.gauge, S, Pn, nb.dn[0],
+                           Cx<R>{R(a.w.wb[0][0]), R(a.w.wb[0][1])});
+        hop<1, true, FMT>(acc, in, SS, nb.up[1] * NRHS + rhs, a.gauge, S, P, i, Cx<R>{R(a.w.wf[1][0]), R(a.w.wf[1][1])});
+        hop<1, false, FMT>(acc, in, SS, nb.dn[1] * NRHS + rhs, a.gauge, S, Pn, nb.dn[1],
+                           Cx<R>{R(a.w.wb[1][0]), R(a.w.wb[1][1])});
+        hop<2, true, FMT>(acc, in, SS, nb.up[2] * NRHS + rhs, a.gauge, S, P, i, Cx<R>{R(a.w.wf[2][0]), R(a.w.wf[2][1])});
+        hop<2, false, FMT>(acc, in, SS, nb.dn[2] * NRHS + rhs, a.gauge, S, Pn, nb.dn[2],
+                           Cx<R>{R(a.w.wb[2][0]), R(a.w.wb[2][1])});
+        hop<3, true, FMT>(acc, in, SS, nb.up[3] * NRHS + rhs, a.gauge, S, P, i, Cx<R>{R(a.w.wf[3][0]), R(a.w.wf[3][1])});
+        hop<3, false, FMT>(acc, in, SS, nb.dn[3] * NRHS + rhs, a.gauge, S, Pn, nb.dn[3],
+                           Cx<R>{R(a.w.wb[3][0]), R(a.w.wb[3][1])});
+        const uint32_t o = i * NRHS + rhs;
 
         const Cx<R> f1{R(a.f1[0]), R(a.f1[1])}, f2{R(a.f2[0]), R(a.f2[1])};
         if (a.epilogue == EPI_MINV) {
@@ -98,7 +111,7 @@ __global__ __launch_bounds__(BLOCK) void k_dslash_eo(DslashArgs a)
             const V* x = reinterpret_cast<const V*>(a.x);
 #pragma unroll
             for (int k = 0; k < 12; ++k) {
-                V xv   = x[(size_t)k * S + i];
+                V xv   = x[(size_t)k * SS + o];
                 acc[k] = cmul(Cx<R>{xv.x, xv.y}, k < 6 ? f1 : f2) - acc[k];
                 if (a.gamma5 && k >= 6) acc[k] = Cx<R>{-acc[k].re, -acc[k].im};
             }
@@ -106,13 +119,120 @@ __global__ __launch_bounds__(BLOCK) void k_dslash_eo(DslashArgs a)
         V* out = reinterpret_cast<V*>(a.out);
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            out[(size_t)k * S + i] = V{acc[k].re, acc[k].im};
+            out[(size_t)k * SS + o] = V{acc[k].re, acc[k].im};
             nrm += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;
         }
     }
     if (a.norm_partials) {
         double s = block_sum<BLOCK>(nrm, lds);
         if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s, a.norm_partials[2 * blockIdx.x + 1] = 0.;
+    }
+}
+
+/* Register-blocked multi-RHS variant: one thread per (site, group of RB right-hand sides); each link is
+ * loaded and reconstructed once per thread and applied to RB neighbour spinors, so link loads and SU(3)
+ * reconstruction are amortised over RB right-hand sides as well as shared across the NRHS/RB lanes of a
+ * site.  Arithmetic per right-hand side is the single-field kernel's, in the same order. */
+template<int A, int B, int CU, bool FWD, int RB, typename R>
+__device__ __forceinline__ void half_hop_rb(Cx<R> (&acc)[RB][12], const vec2<R>* __restrict__ in, size_t SS,
+                                            uint32_t nb, const Su3<R>& u, Cx<R> w)
+{
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        Cx<R> h[3], r[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            vec2<R> a = in[(size_t)(3 * A + c) * SS + nb + q];
+            vec2<R> b = in[(size_t)(3 * B + c) * SS + nb + q];
+            h[c]      = add_unit<CU>(Cx<R>{a.x, a.y}, Cx<R>{b.x, b.y});
+        }
+        if constexpr (FWD)
+            su3_times(u, h, r);
+        else
+            su3dag_times(u, h, r);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            Cx<R> t           = cmul(r[c], w);
+            acc[q][3 * A + c] = acc[q][3 * A + c] + t;
+            acc[q][3 * B + c] = add_unit<UnitConj<CU>::v>(acc[q][3 * B + c], t);
+        }
+    }
+}
+
+template<int DIR, bool FWD, int FMT, int RB, typename R>
+__device__ __forceinline__ void hop_rb(Cx<R> (&acc)[RB][12], const vec2<R>* __restrict__ in, size_t SS, uint32_t nb,
+                                       const void* gauge, uint32_t S, int linkpar, uint32_t linkeo, Cx<R> w)
+{
+    Su3<R> u;
+    GaugeLoad<FMT>::load(gauge, S, DIR, linkpar, linkeo, u);
+    if constexpr (DIR == 0) {
+        half_hop_rb<0, 2, FWD ? P1 : M1, FWD, RB>(acc, in, SS, nb, u, w);
+        half_hop_rb<1, 3, FWD ? P1 : M1, FWD, RB>(acc, in, SS, nb, u, w);
+    } else if constexpr (DIR == 1) {
+        half_hop_rb<0, 3, FWD ? PI : MI, FWD, RB>(acc, in, SS, nb, u, w);
+        half_hop_rb<1, 2, FWD ? PI : MI, FWD, RB>(acc, in, SS, nb, u, w);
+    } else if constexpr (DIR == 2) {
+        half_hop_rb<0, 3, FWD ? P1 : M1, FWD, RB>(acc, in, SS, nb, u, w);
+        half_hop_rb<1, 2, FWD ? M1 : P1, FWD, RB>(acc, in, SS, nb, u, w);
+    } else {
+        half_hop_rb<0, 2, FWD ? PI : MI, FWD, RB>(acc, in, SS, nb, u, w);
+        half_hop_rb<1, 3, FWD ? MI : PI, FWD, RB>(acc, in, SS, nb, u, w);
+    }
+}
+
+template<int FMT, int BLOCK, int NRHS, int RB>
+__global__ __launch_bounds__(BLOCK) void k_dslash_eo_rb(DslashArgs a)
+{
+    using R = typename GaugeLoad<FMT>::R;
+    using V = vec2<R>;
+    constexpr int G = NRHS / RB;  // thread groups per site
+    const uint32_t S = a.g.veo;
+    const size_t SS  = (size_t)S * NRHS;
+    uint32_t i = 0, grp = 0;
+    if (!map_site_rhs<BLOCK, G>(a.map, S, blockIdx.x, threadIdx.x, i, grp)) return;
+    const uint32_t r0 = grp * RB;
+    SiteNb nb;
+    site_neighbours(a.g, i, a.out_parity, nb);
+    const V* in  = reinterpret_cast<const V*>(a.in);
+    const int P  = a.out_parity;
+    const int Pn = P ^ 1;
+    Cx<R> acc[RB][12];
+#pragma unroll
+    for (int q = 0; q < RB; ++q)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[q][k] = Cx<R>{R(0), R(0)};
+    hop_rb<0, true, FMT, RB>(acc, in, SS, nb.up[0] * NRHS + r0, a.gauge, S, P, i, Cx<R>{R(a.w.wf[0][0]), R(a.w.wf[0][1])});
+    hop_rb<0, false, FMT, RB>(acc, in, SS, nb.dn[0] * NRHS + r0, a.gauge, S, Pn, nb.dn[0],
+                              Cx<R>{R(a.w.wb[0][0]), R(a.w.wb[0][1])});
+    hop_rb<1, true, FMT, RB>(acc, in, SS, nb.up[1] * NRHS + r0, a.gauge, S, P, i, Cx<R>{R(a.w.wf[1][0]), R(a.w.wf[1][1])});
+    hop_rb<1, false, FMT, RB>(acc, in, SS, nb.dn[1] * NRHS + r0, a.gauge, S, Pn, nb.dn[1],
+                              Cx<R>{R(a.w.wb[1][0]), R(a.w.wb[1][1])});
+    hop_rb<2, true, FMT, RB>(acc, in, SS, nb.up[2] * NRHS + r0, a.gauge, S, P, i, Cx<R>{R(a.w.wf[2][0]), R(a.w.wf[2][1])});
+    hop_rb<2, false, FMT, RB>(acc, in, SS, nb.dn[2] * NRHS + r0, a.gauge, S, Pn, nb.dn[2],
+                              Cx<R>{R(a.w.wb[2][0]), R(a.w.wb[2][1])});
+    hop_rb<3, true, FMT, RB>(acc, in, SS, nb.up[3] * NRHS + r0, a.gauge, S, P, i, Cx<R>{R(a.w.wf[3][0]), R(a.w.wf[3][1])});
+    hop_rb<3, false, FMT, RB>(acc, in, SS, nb.dn[3] * NRHS + r0, a.gauge, S, Pn, nb.dn[3],
+                              Cx<R>{R(a.w.wb[3][0]), R(a.w.wb[3][1])});
+    const Cx<R> f1{R(a.f1[0]), R(a.f1[1])}, f2{R(a.f2[0]), R(a.f2[1])};
+    const uint32_t o = i * NRHS + r0;
+    V* out           = reinterpret_cast<V*>(a.out);
+    const V* x       = reinterpret_cast<const V*>(a.x);
+#pragma unroll
+    for (int q = 0; q < RB; ++q) {
+        if (a.epilogue == EPI_MINV) {
+            const R s = R(a.scale);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) acc[q][k] = scale(cmul(acc[q][k], k < 6 ? f1 : f2), s);
+        } else if (a.epilogue == EPI_AEE) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                V xv      = x[(size_t)k * SS + o + q];
+                acc[q][k] = cmul(Cx<R>{xv.x, xv.y}, k < 6 ? f1 : f2) - acc[q][k];
+                if (a.gamma5 && k >= 6) acc[q][k] = Cx<R>{-acc[q][k].re, -acc[q][k].im};
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 12; ++k) out[(size_t)k * SS + o + q] = V{acc[q][k].re, acc[q][k].im};
     }
 }
 
@@ -133,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void k_sitediag(vec2<R>* __restrict__ out, c
     }
 }
 
-WorkMap work_map(const cl2_ctx* c, int block)
+WorkMap work_map(const cl2_ctx* c, int block, int nrhs)
 {
     WorkMap m{0u, 0u, c->g.vs / 2, 0u};
     const uint32_t ns = c->g.ns, plane = ns * ns / 2;
@@ -154,7 +274,7 @@ WorkMap work_map(const cl2_ctx* c, int block)
         m.P   = (uint32_t)zp * plane;
         m.col = m.P * c->g.nt;
     }
-    const uint32_t nblocks = (c->g.veo + block - 1) / block;
+    const uint32_t nblocks = (uint32_t)(((size_t)c->g.veo * nrhs + block - 1) / block);
     if (c->tune_xcd && nblocks >= 64) m.xcd_q = nblocks / 8;
     return m;
 }
@@ -183,26 +303,86 @@ HopWeights<double> wilson_weights(const cl2_ctx* c, double kappa)
     return w;
 }
 
+template<int FMT, int NRHS, int RB>
+static hipError_t launch_rb(cl2_ctx* c, const DslashArgs& a)
+{
+    const unsigned nb = blocks_for((size_t)a.g.veo * (NRHS / RB), DSLASH_BLOCK);
+    hipLaunchKernelGGL((k_dslash_eo_rb<FMT, DSLASH_BLOCK, NRHS, RB>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
+    return hipGetLastError();
+}
+
+/* right-hand sides per thread of the multi-RHS kernel: the tuned value if it divides NRHS, else 1 (one
+ * thread per site and right-hand side, links shared across the lanes of a site), which measured fastest on
+ * gfx950 for every format: RB > 1 keeps RB x 12 accumulators live and drops to one wave per SIMD */
+template<int FMT, int NRHS>
+static int rhs_per_thread(const cl2_ctx* c)
+{
+    const int want = c->tune_mrhs_rb > 0 ? c->tune_mrhs_rb : 1;
+    for (int rb : {want, 4, 2, 1})
+        if (rb <= 4 && rb != 3 && NRHS % rb == 0) return rb;
+    return 1;
+}
+
+template<int NRHS>
+static hipError_t launch_dslash_n(cl2_ctx* c, const DslashArgs& a, int format)
+{
+    if constexpr (NRHS == 1) {
+        const unsigned nb = blocks_for((size_t)a.g.veo, DSLASH_BLOCK);
+        switch (format) {
+            case GAUGE_F64_18:
+                hipLaunchKernelGGL((k_dslash_eo<GAUGE_F64_18, DSLASH_BLOCK, 1>), dim3(nb), dim3(DSLASH_BLOCK), 0,
+                                   c->stream, a);
+                break;
+            case GAUGE_F64_12:
+                hipLaunchKernelGGL((k_dslash_eo<GAUGE_F64_12, DSLASH_BLOCK, 1>), dim3(nb), dim3(DSLASH_BLOCK), 0,
+                                   c->stream, a);
+                break;
+            case GAUGE_F32_12:
+                hipLaunchKernelGGL((k_dslash_eo<GAUGE_F32_12, DSLASH_BLOCK, 1>), dim3(nb), dim3(DSLASH_BLOCK), 0,
+                                   c->stream, a);
+                break;
+            case GAUGE_F16_12: return launch_dslash_h(c, a);
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    } else {
+        auto by_rb = [&](auto fmt_tag) -> hipError_t {
+            constexpr int F = decltype(fmt_tag)::value;
+            switch (rhs_per_thread<F, NRHS>(c)) {
+                case 4:
+                    if constexpr (NRHS % 4 == 0) return launch_rb<F, NRHS, 4>(c, a);
+                    return hipErrorInvalidValue;
+                case 2:
+                    if constexpr (NRHS % 2 == 0) return launch_rb<F, NRHS, 2>(c, a);
+                    return hipErrorInvalidValue;
+                default: return launch_rb<F, NRHS, 1>(c, a);
+            }
+        };
+        switch (format) {
+            case GAUGE_F64_18: return by_rb(std::integral_constant<int, GAUGE_F64_18>{});
+            case GAUGE_F64_12: return by_rb(std::integral_constant<int, GAUGE_F64_12>{});
+            case GAUGE_F32_12: return by_rb(std::integral_constant<int, GAUGE_F32_12>{});
+            default: return hipErrorInvalidValue;
+        }
+    }
+}
+
+bool supported_nrhs(int n)
+{
+    return n == 1 || n == 2 || n == 4 || n == 8 || n == 12 || n == 16;
+}
+
 hipError_t launch_dslash(cl2_ctx* c, const DslashArgs& a, int format)
 {
-    const unsigned nb = blocks_for(a.g.veo, DSLASH_BLOCK);
-    switch (format) {
-        case GAUGE_F64_18:
-            hipLaunchKernelGGL((k_dslash_eo<GAUGE_F64_18, DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream,
-                               a);
-            break;
-        case GAUGE_F64_12:
-            hipLaunchKernelGGL((k_dslash_eo<GAUGE_F64_12, DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream,
-                               a);
-            break;
-        case GAUGE_F32_12:
-            hipLaunchKernelGGL((k_dslash_eo<GAUGE_F32_12, DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream,
-                               a);
-            break;
-        case GAUGE_F16_12: return launch_dslash_h(c, a);
+    switch (a.nrhs) {
+        case 1: return launch_dslash_n<1>(c, a, format);
+        case 2: return launch_dslash_n<2>(c, a, format);
+        case 4: return launch_dslash_n<4>(c, a, format);
+        case 8: return launch_dslash_n<8>(c, a, format);
+        case 12: return launch_dslash_n<12>(c, a, format);
+        case 16: return launch_dslash_n<16>(c, a, format);
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 hipError_t launch_sitediag(cl2_ctx* c, void* out, const void* in, int prec, const double f1[2], const double f2[2],

@@ -83,6 +83,18 @@ int cg_reliable_update(const lattices::Spinorfield_eo* x, const fermionmatrix::F
                        const physics::AdditionalParameters& additionalParameters, const ReliableUpdateOptions& options,
                        SolveStats* stats = nullptr);
 
+/* The reliable-update CG of cg_reliable_update on a block of right-hand sides b[r] (FP64 fields, nrhs = 1, 2,
+ * 4, 8, 12 or 16), solved in lockstep: every inner iteration applies A once to the whole block with the
+ * multi-right-hand-side operator, each right-hand side keeps its own alpha and beta (device arrays), a
+ * reliable update is made for all right-hand sides when one of them has dropped by delta, and a
+ * right-hand side whose FP64 true residual meets rel_tol is frozen.  Inner precision FP32 or FP64.
+ * Returns the number of lockstep iterations; stats (if given) receives one entry per right-hand side. */
+int cg_reliable_update_block(const std::vector<const lattices::Spinorfield_eo*>& x,
+                             const fermionmatrix::Fermionmatrix_eo& A, const lattices::Gaugefield& gf,
+                             const std::vector<const lattices::Spinorfield_eo*>& b, const hardware::System& system,
+                             hmc_float rel_tol, const physics::AdditionalParameters& additionalParameters,
+                             const ReliableUpdateOptions& options, std::vector<SolveStats>* stats = nullptr);
+
 }  // namespace solvers
 }  // namespace algorithms
 }  // namespace physics

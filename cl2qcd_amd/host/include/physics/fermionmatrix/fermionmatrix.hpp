@@ -107,6 +107,12 @@ CL2_EO_OPERATOR(Qminus_eo, CL2_OP_QMINUS_EO, false)
 CL2_EO_OPERATOR(QplusQminus_eo, CL2_OP_QPLUSQMINUS_EO, true)
 #undef CL2_EO_OPERATOR
 
+/* A applied to every right-hand side of a block in one pass over the links (A must be one of the built-in
+ * classes; tmp holds two block fields of in's shape).  No counterpart in the reference. */
+void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* out, const lattices::Gaugefield& gf,
+                 const lattices::Spinorfield_eo_block& in, const physics::AdditionalParameters& ap,
+                 const lattices::DeviceBuffer& tmp);
+
 /* ---- staggered (fermionmatrix_stagg.hpp:49-152) ---- */
 class Fermionmatrix_stagg_eo : public Fermionmatrix_basic {
   public:

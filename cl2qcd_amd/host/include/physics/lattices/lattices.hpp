@@ -153,6 +153,26 @@ class Staggeredfield_eo {
     DeviceBuffer buf;
 };
 
+/* nrhs Wilson eo spinors in one device array [12][Veo][nrhs] (right-hand side fastest), the operand of
+ * the multi-right-hand-side operators; no counterpart in the reference, which holds one field per source */
+class Spinorfield_eo_block {
+  public:
+    Spinorfield_eo_block(const hardware::System& system, int nrhs, int prec = CL2_F64);
+    void insert(int r, const Spinorfield_eo& f) const;
+    void extract(int r, const Spinorfield_eo* f) const;
+    void zero() const;
+    void* dev() const noexcept { return buf.ptr(); }
+    int precision() const noexcept { return prec; }
+    int nrhs() const noexcept { return n; }
+    size_t bytes() const noexcept { return buf.bytes(); }
+    const hardware::System& get_system() const noexcept { return system; }
+
+  private:
+    const hardware::System& system;
+    int n, prec;
+    DeviceBuffer buf;
+};
+
 /* ---- Wilson eo BLAS (physics/lattices/spinorfield_eo.hpp:161-313) ---- */
 void copyData(const Spinorfield_eo* to, const Spinorfield_eo& from);
 void copyData(const Spinorfield_eo* to, const Spinorfield_eo* from);

@@ -319,6 +319,34 @@ PYBIND11_MODULE(_host, m)
         },
         py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("b"), py::arg("lattice"), py::arg("rel_tol"), py::arg("ap"),
         py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000);
+    m.def(
+        "cg_reliable_update_block",
+        [](std::vector<Spinorfield_eo*> xs, const fm::Fermionmatrix_eo& A, const Gaugefield& gf,
+           std::vector<Spinorfield_eo*> bs, Lattice& l, hmc_float rel_tol, const AdditionalParameters& ap,
+           int inner_prec, double delta, int maxit) {
+            sv::ReliableUpdateOptions o;
+            o.inner_precision = inner_prec;
+            o.delta           = delta;
+            o.max_iterations  = maxit;
+            std::vector<const Spinorfield_eo*> x(xs.begin(), xs.end()), b(bs.begin(), bs.end());
+            std::vector<sv::SolveStats> st;
+            int it;
+            {
+                py::gil_scoped_release rel;
+                it = sv::cg_reliable_update_block(x, A, gf, b, l.system, rel_tol, ap, o, &st);
+            }
+            py::list per;
+            for (const auto& s : st) {
+                py::dict d;
+                d["iterations"]             = s.iterations;
+                d["reliable_updates"]       = s.reliable_updates;
+                d["true_relative_residual"] = s.true_relative_residual;
+                per.append(d);
+            }
+            return py::make_tuple(it, per);
+        },
+        py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("b"), py::arg("lattice"), py::arg("rel_tol"), py::arg("ap"),
+        py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000);
     m.def("convert_precision",
           [](const Spinorfield_eo& to, const Spinorfield_eo& from) { convert_precision(&to, from); });
     m.def("cg_m", [](std::vector<Staggeredfield_eo*> xs, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf,

@@ -179,6 +179,25 @@ cl_ulong BuiltinEoOperator::get_read_write_size() const
     }
 }
 
+void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* out, const Gaugefield& gf,
+                 const lattices::Spinorfield_eo_block& in, const physics::AdditionalParameters& ap,
+                 const lattices::DeviceBuffer& tmp)
+{
+    const int op = A.abi_operator();
+    if (op < 0) throw std::invalid_argument("apply_block: not a built-in eo operator");
+    if (out->precision() != in.precision() || out->nrhs() != in.nrhs())
+        throw std::invalid_argument("apply_block: blocks of different shape");
+    if (tmp.bytes() < 2 * in.bytes()) throw std::invalid_argument("apply_block: tmp must hold two blocks");
+    const common::action a = A.action();
+    if (a != common::wilson && a != common::twistedmass)
+        throw std::invalid_argument("Unkown fermion action! wilson or twistedmass");
+    const int abi_action  = a == common::twistedmass ? CL2_ACTION_TWISTEDMASS : CL2_ACTION_WILSON;
+    const hmc_float mubar = a == common::twistedmass ? ap.getMubar() : 0.;
+    const Links l         = links_for(gf, in.precision());
+    in.get_system().check(cl2_fermionmatrix_eo_mrhs(in.get_system().ctx(), op, out->dev(), in.dev(), l.ptr, l.fmt,
+                                                    abi_action, ap.getKappa(), mubar, tmp.ptr(), in.nrhs()));
+}
+
 /* ---------------- staggered ---------------- */
 
 hmc_float Fermionmatrix_stagg_eo::getThresholdForMinimumEigenvalue(hmc_float) const

@@ -404,3 +404,151 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
 }  // namespace solvers
 }  // namespace algorithms
 }  // namespace physics
+
+namespace physics {
+namespace algorithms {
+namespace solvers {
+
+int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const fermionmatrix::Fermionmatrix_eo& A,
+                             const Gaugefield& gf, const std::vector<const Spinorfield_eo*>& b,
+                             const hardware::System& system, hmc_float rel_tol, const physics::AdditionalParameters& ap,
+                             const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
+{
+    const int n = (int)b.size();
+    if (n < 1 || x.size() != b.size()) throw std::invalid_argument("cg_reliable_update_block: x and b differ in size");
+    if (!(n == 1 || n == 2 || n == 4 || n == 8 || n == 12 || n == 16))
+        throw std::invalid_argument("cg_reliable_update_block: 1, 2, 4, 8, 12 or 16 right-hand sides");
+    for (int r = 0; r < n; ++r)
+        if (x[r]->precision() != CL2_F64 || b[r]->precision() != CL2_F64)
+            throw std::invalid_argument("cg_reliable_update_block: x and b are FP64 fields");
+    if (opt.delta <= 0. || opt.delta >= 1.) throw std::invalid_argument("cg_reliable_update_block: delta must be in (0,1)");
+    const int lo = opt.inner_precision;
+    if (lo != CL2_F32 && lo != CL2_F64) throw std::invalid_argument("cg_reliable_update_block: inner precision FP32 or FP64");
+    cl2_ctx* c = system.ctx();
+
+    const Spinorfield_eo_block B(system, n, CL2_F64), X(system, n, CL2_F64), R(system, n, CL2_F64),
+        T(system, n, CL2_F64);
+    const Spinorfield_eo_block Rlo(system, n, lo), P(system, n, lo), V(system, n, lo), Xlo(system, n, lo);
+    const DeviceBuffer tmp64(system, 2 * T.bytes()), tmplo(system, 2 * P.bytes());
+    // per-right-hand-side device scalars
+    const Vector<hmc_float> ones(2 * n, system), alpha(2 * n, system), beta(2 * n, system), pAp(2 * n, system),
+        norms(2 * n, system), r2(n, system), active(n, system), coef(2 * n, system), res(2 * n, system);
+    std::vector<hmc_float> one_v(2 * n, 0.);
+    for (int r = 0; r < n; ++r) one_v[2 * r] = 1.;
+    ones.store(one_v);
+    for (int r = 0; r < n; ++r) {
+        B.insert(r, *b[r]);
+        X.insert(r, *x[r]);
+    }
+    auto block_norms = [&](const Spinorfield_eo_block& f) {
+        system.check(cl2_block_squarenorm_eo(c, res.dev(), f.dev(), f.precision(), n));
+        std::vector<hmc_float> v = res.get();
+        v.resize(n);
+        return v;
+    };
+    auto true_residual = [&]() {
+        fermionmatrix::apply_block(A, &T, gf, X, ap, tmp64);
+        system.check(cl2_block_axpy_eo(c, R.dev(), T.dev(), B.dev(), CL2_F64, n, ones.dev(), -1., nullptr));
+        return block_norms(R);  // |b - A x|^2 per right-hand side
+    };
+    const std::vector<hmc_float> b2 = block_norms(B);
+    std::vector<hmc_float> target(n), rn = true_residual(), act(n), r2h(n), rmax(n);
+    std::vector<SolveStats> st(n);
+    int n_active = 0;
+    for (int r = 0; r < n; ++r) {
+        target[r] = rel_tol * rel_tol * b2[r];
+        act[r]    = (b2[r] > 0. && rn[r] > target[r]) ? 1. : 0.;
+        n_active += act[r] != 0.;
+        r2h[r]  = rn[r];
+        rmax[r] = std::sqrt(rn[r]);
+    }
+    auto finish = [&](int k) {
+        for (int r = 0; r < n; ++r) {
+            X.extract(r, x[r]);
+            st[r].true_relative_residual = b2[r] > 0. ? std::sqrt(rn[r] / b2[r]) : 0.;
+            if (act[r] != 0.) st[r].iterations = k;
+        }
+        if (stats) *stats = st;
+    };
+    if (n_active == 0) {
+        finish(0);
+        return 0;
+    }
+    system.check(cl2_block_convert_eo(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
+    system.check(cl2_copy(c, P.dev(), Rlo.dev(), P.bytes()));
+    Xlo.zero();
+    r2.store(r2h);
+    active.store(act);
+    int k           = 0;
+    int reliable    = 0;
+    for (k = 0; k < opt.max_iterations && n_active > 0; ++k) {
+        fermionmatrix::apply_block(A, &V, gf, P, ap, tmplo);
+        system.check(cl2_block_dot_eo(c, pAp.dev(), P.dev(), V.dev(), lo, n));
+        system.check(cl2_block_cg_alpha(c, alpha.dev(), r2.dev(), pAp.dev(), active.dev(), n));
+        system.check(cl2_block_axpy_eo(c, Xlo.dev(), P.dev(), Xlo.dev(), lo, n, alpha.dev(), 1., nullptr));
+        system.check(cl2_block_axpy_eo(c, Rlo.dev(), V.dev(), Rlo.dev(), lo, n, alpha.dev(), -1., norms.dev()));
+        const std::vector<hmc_float> nr = norms.get();
+        bool update                     = false;
+        for (int r = 0; r < n; ++r) {
+            if (act[r] == 0.) continue;
+            const hmc_float q = nr[2 * r];
+            if (q != q) throw SolverStuck(k, "cg_reliable_update_block: NAN occured");
+            if (std::sqrt(q) < opt.delta * rmax[r] || q <= target[r]) update = true;
+        }
+        if (update) {
+            system.check(cl2_block_convert_eo(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
+            system.check(cl2_block_axpy_eo(c, X.dev(), T.dev(), X.dev(), CL2_F64, n, ones.dev(), 1., nullptr));
+            Xlo.zero();
+            rn = true_residual();
+            ++reliable;
+            std::vector<hmc_float> bt(2 * n, 0.), oc(2 * n, 0.);
+            for (int r = 0; r < n; ++r) {
+                if (act[r] == 0.) continue;
+                if (rn[r] <= target[r]) {  // converged: freeze this right-hand side
+                    act[r]           = 0.;
+                    st[r].iterations = k + 1;
+                    --n_active;
+                }
+            }
+            if (n_active == 0) {
+                ++k;
+                break;
+            }
+            system.check(cl2_block_convert_eo(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
+            // restore <r, p> = 0 for the new residuals
+            system.check(cl2_block_dot_eo(c, res.dev(), Rlo.dev(), P.dev(), lo, n));
+            const std::vector<hmc_float> rp = res.get();
+            for (int r = 0; r < n; ++r) {
+                if (act[r] == 0.) continue;
+                oc[2 * r]     = rp[2 * r] / rn[r];
+                oc[2 * r + 1] = rp[2 * r + 1] / rn[r];
+                bt[2 * r]     = rn[r] / r2h[r];
+                r2h[r]        = rn[r];
+                rmax[r]       = std::sqrt(rn[r]);
+            }
+            coef.store(oc);
+            system.check(cl2_block_axpy_eo(c, P.dev(), Rlo.dev(), P.dev(), lo, n, coef.dev(), -1., nullptr));
+            beta.store(bt);
+            r2.store(r2h);
+            active.store(act);
+        } else {
+            system.check(cl2_block_cg_beta(c, beta.dev(), r2.dev(), norms.dev(), active.dev(), n));
+            for (int r = 0; r < n; ++r) {
+                if (act[r] == 0.) continue;
+                r2h[r]  = nr[2 * r];
+                rmax[r] = std::max(rmax[r], std::sqrt(nr[2 * r]));
+            }
+        }
+        // p = r_lo + beta p
+        system.check(cl2_block_axpy_eo(c, P.dev(), P.dev(), Rlo.dev(), lo, n, beta.dev(), 1., nullptr));
+    }
+    for (int r = 0; r < n; ++r) st[r].reliable_updates = reliable;
+    const bool done = n_active == 0;
+    finish(k);
+    if (!done) throw SolverDidNotSolve(k, "cg_reliable_update_block");
+    return k;
+}
+
+}  // namespace solvers
+}  // namespace algorithms
+}  // namespace physics

@@ -218,6 +218,37 @@ int cl2_set_cold_eo(cl2_ctx* ctx, void* out, int prec);
  * (CMakeLists.txt:127-132).  Feeds the reduced-precision inner solvers. */
 int cl2_convert_precision_eo(cl2_ctx* ctx, int kind, void* dst, int dst_prec, const void* src, int src_prec);
 
+/* ==== blocks of right-hand sides ====
+ * No counterpart in the reference, which inverts the 12 spin-colour sources of a propagator one after
+ * the other (physics/algorithms/inversion.cpp:78-242).  A block holds nrhs Wilson eo spinors as one
+ * [12][Veo][nrhs] complex array (right-hand side fastest), so one launch of an operator reuses every
+ * loaded link for all right-hand sides.  Block fields are FP64 or FP32; per-right-hand-side scalars are
+ * device arrays (complex: 2 doubles each). */
+/* the eo operators of cl2_fermionmatrix_eo / cl2_dslash_eo on nrhs = 1, 2, 4, 8, 12 or 16 right-hand
+ * sides; tmp = 2 block fields */
+int cl2_fermionmatrix_eo_mrhs(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge, int format,
+                              int action, double kappa, double mubar, void* tmp, int nrhs);
+int cl2_dslash_eo_mrhs(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, int evenodd,
+                       double kappa, int nrhs);
+/* copy a single field into / out of slot r of a block (nrhs in [1, 64]) */
+int cl2_block_insert_eo(cl2_ctx* ctx, void* block, const void* single, int prec, int nrhs, int r);
+int cl2_block_extract_eo(cl2_ctx* ctx, void* single, const void* block, int prec, int nrhs, int r);
+/* out_r = y_r + sign * coef_r x_r (coef: nrhs complex); out may alias y.  If norms is not null it receives
+ * per right-hand side (|out_r|^2, Re <out_r, y_r>) with y read before it is overwritten. */
+int cl2_block_axpy_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, int nrhs, const double* coef,
+                      double sign, double* norms);
+/* res[r] = <x_r, y_r> (complex, 2 doubles each) / |x_r|^2 (one double each) */
+int cl2_block_dot_eo(cl2_ctx* ctx, double* res, const void* x, const void* y, int prec, int nrhs);
+int cl2_block_squarenorm_eo(cl2_ctx* ctx, double* res, const void* x, int prec, int nrhs);
+int cl2_block_convert_eo(cl2_ctx* ctx, void* dst, int dst_prec, const void* src, int src_prec, int nrhs);
+int cl2_block_set_zero_eo(cl2_ctx* ctx, void* out, int prec, int nrhs);
+/* lockstep CG scalars: alpha_r = r2_r / pAp_r (complex) where active_r != 0, else 0;
+ * beta_r = Polak-Ribiere (norms_r[0] - norms_r[1]) / r2_r (Fletcher-Reeves norms_r[0] / r2_r if negative)
+ * where active_r != 0, else 0, and then r2_r = norms_r[0] */
+int cl2_block_cg_alpha(cl2_ctx* ctx, double* alpha, const double* r2, const double* pAp, const double* active,
+                       int nrhs);
+int cl2_block_cg_beta(cl2_ctx* ctx, double* beta, double* r2, const double* norms, const double* active, int nrhs);
+
 /* ==== staggered (hardware/code/fermions_staggered.hpp:46-91, spinors_staggered.hpp) ==== */
 
 /* Fermions_staggered::D_KS_eo_device (fermions_staggered.cpp:141-168); output parity == evenodd
