@@ -221,6 +221,62 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
                               f"mrhs{nrhs}_alg_GBps": round(alg_n / (tn * 1e6), 1)}}
 
 
+def multishift_leg(ns, nt, sweeps, seed, nshift=16):
+    """Staggered MdagM_eo (mass 0.01) with 16 shifts sigma_k = 10^(-4 + 4k/15) on a heatbath beta 5.9
+    configuration, Gaussian source, every shift to |b - (A + sigma_k) x_k| <= 1e-10 |b| in FP64:
+    the fused multi-shift solver (FP32 CG-M, then FP64 reliable-update refinement per shift) against
+    16 independent FP32-inner reliable-update solves and against the reference's FP64 CG-M."""
+    from cl2qcd_amd import _host as H
+
+    l = H.Lattice([f"--nSpace={ns}", f"--nTime={nt}", "--beta=5.9", "--fermionAction=rooted_stagg",
+                   "--mass=0.01", "--solverMaxIterations=200000", f"--hostSeed={seed}"])
+    gf = H.Gaugefield(l, False)
+    for _ in range(sweeps):
+        H.su3heatbath(gf, l, 1)
+    plaq = H.measurePlaquette(gf)
+    A, ap = H.MdagM_eo(l), l.staggered_parameters()
+    sigma = [10.0 ** (-4 + 4 * k / (nshift - 1)) for k in range(nshift)]
+    b = H.Staggeredfield_eo(l)
+    b.import_(np.random.default_rng(seed).standard_normal((l.eo_sites, 6)))
+    b2 = H.squarenorm(b)
+    xs = [H.Staggeredfield_eo(l) for _ in sigma]
+    l.synchronize()
+    t0 = time.perf_counter()
+    total, ms_it, per = H.cg_m_mixed(xs, A, gf, sigma, b, l, 1e-10, ap, H.F32, 1e-7)
+    l.synchronize()
+    t_ms = time.perf_counter() - t0
+    worst_ms = max(p["true_relative_residual"] for p in per)
+    seq = [H.Staggeredfield_eo(l) for _ in sigma]
+    l.synchronize()
+    t0 = time.perf_counter()
+    seq_it = []
+    for s_, x in zip(sigma, seq):
+        x.set_zero()
+        seq_it.append(H.cg_reliable_update_stagg(x, A, gf, s_, b, l, 1e-10, ap, H.F32)["iterations"])
+    l.synchronize()
+    t_seq = time.perf_counter() - t0
+    ref = [H.Staggeredfield_eo(l) for _ in sigma]
+    l.synchronize()
+    t0 = time.perf_counter()
+    ref_it = H.cg_m(ref, A, gf, sigma, b, l, 1e-20 * b2, ap)
+    l.synchronize()
+    t_ref = time.perf_counter() - t0
+    log(f"16-shift CG-M {ns}^3x{nt} (plaquette {plaq:.5f}): fused mixed {t_ms:.2f} s ({ms_it} multi-shift + "
+        f"{total - ms_it} refinement iterations, worst |r|/|b| {worst_ms:.2e}); 16 sequential mixed solves "
+        f"{t_seq:.2f} s ({sum(seq_it)} iterations); reference FP64 CG-M {t_ref:.2f} s ({ref_it} iterations)")
+    return {"lattice": f"{ns}^3x{nt}", "plaquette": round(plaq, 6),
+            "heatbath": f"beta 5.9, cold start, {sweeps} sweeps x (heatbath + 1 overrelaxation), seed {seed}",
+            "operator": "staggered MdagM_eo mass 0.01", "shifts": "16, log-spaced in [1e-4, 1]",
+            "fused_mixed": {"seconds": round(t_ms, 3), "per_shift_seconds": round(t_ms / nshift, 4),
+                            "multishift_iterations": ms_it, "refinement_iterations": total - ms_it,
+                            "worst_true_rel_residual": worst_ms},
+            "sequential_mixed": {"seconds": round(t_seq, 3), "per_shift_seconds": round(t_seq / nshift, 4),
+                                 "iterations": sum(seq_it)},
+            "reference_fp64_cgm": {"seconds": round(t_ref, 3), "iterations": ref_it,
+                                   "stop": "|zeta_k r|^2 < 1e-20 |b|^2 for every shift"},
+            "speedup_vs_sequential": round(t_seq / t_ms, 3), "speedup_vs_reference_cgm": round(t_ref / t_ms, 3)}
+
+
 def read_counters():
     """HBM bytes per launch of the FP64 dslash from the committed rocprofv3 counter summary."""
     if not os.path.exists(COUNTERS_FILE):
@@ -270,6 +326,7 @@ def main():
     ap.add_argument("--no-cg", action="store_true", help="skip the FP64 vs mixed-precision CG comparison")
     ap.add_argument("--hb-sweeps", type=int, default=50, help="heatbath sweeps for the CG configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-source block solve at 32^3x64")
+    ap.add_argument("--no-multishift", action="store_true", help="skip the 16-shift staggered solve at 48^3x12")
     ap.add_argument("--hb-overrelax", type=int, default=1, help="overrelaxation sweeps per heatbath sweep")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -312,6 +369,10 @@ def main():
     if not args.no_batch:
         batch = batch_leg(32, 64, args.hb_sweeps, 20261203)
 
+    multishift = None
+    if not args.no_multishift:
+        multishift = multishift_leg(48, 12, args.hb_sweeps, 20261204)
+
     traffic = read_counters()
     achieved = f64["alg_bytes"] / (f64["ms"] * 1e6)
     line = {
@@ -339,6 +400,7 @@ def main():
         **extra,
         "cg": cg,
         "batch": batch,
+        "multishift": multishift,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores) ...")

@@ -316,6 +316,7 @@ int cl2_ctx_create(cl2_ctx** out, int device, const cl2_lattice* lat)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, c->npartials * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c->ptr_table, 128 * sizeof(void*)) != hipSuccess ||
         hipEventCreate(&c->ev_start) != hipSuccess || hipEventCreate(&c->ev_stop) != hipSuccess) {
         cl2_ctx_destroy(c);
         return create_fail(CL2_E_HIP, "context allocation failed");
@@ -331,6 +332,7 @@ int cl2_ctx_destroy(cl2_ctx* c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->partials) (void)hipFree(c->partials);
     if (c->dscratch) (void)hipFree(c->dscratch);
+    if (c->ptr_table) (void)hipFree(c->ptr_table);
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->ev_stop) (void)hipEventDestroy(c->ev_stop);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -937,6 +939,33 @@ int cl2_block_cg_beta(cl2_ctx* c, double* beta, double* r2, const double* norms,
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, beta && r2 && norms && active && nrhs >= 1 && nrhs <= 64, "bad argument");
     CL2_HIP(c, launch_block_cg_scalars(c, 1, beta, r2, norms, active, nrhs));
+    return CL2_OK;
+}
+
+/* ---------------- multi-shift ---------------- */
+
+int cl2_multishift_update_stagg_eo(cl2_ctx* c, void* const* x, int x_prec, void* const* p, const void* r, int prec,
+                                   int nshift, const double* beta, const double* zeta, const double* alpha,
+                                   const double* active)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, valid_prec(prec) && valid_prec(x_prec), "unknown precision");
+    CL2_REQUIRE(c, x_prec == CL2_F64 || prec == CL2_F32, "x must be at least as precise as r and p");
+    CL2_REQUIRE(c, nshift >= 1 && nshift <= 64, "number of shifts must be in [1, 64]");
+    CL2_REQUIRE(c, x && p && r && beta && zeta && alpha && active, "null pointer");
+    std::vector<void*> tab(128, nullptr);
+    for (int k = 0; k < nshift; ++k) {
+        CL2_REQUIRE(c, x[k] && p[k] && x[k] != p[k], "null or aliased shift field");
+        tab[k]      = x[k];
+        tab[64 + k] = p[k];
+    }
+    if (tab != c->ptr_table_host) {  // upload only when the fields change
+        CL2_HIP(c, hipMemcpyAsync(c->ptr_table, tab.data(), 128 * sizeof(void*), hipMemcpyHostToDevice, c->stream));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+        c->ptr_table_host = tab;
+    }
+    CL2_HIP(c, launch_multishift(c, c->ptr_table, x_prec, c->ptr_table + 64, r, prec, (size_t)3 * c->g.veo, nshift, beta,
+                                 zeta, alpha, active));
     return CL2_OK;
 }
 

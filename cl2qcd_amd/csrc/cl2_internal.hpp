@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "cl2_types.hpp"
 
@@ -17,6 +18,8 @@ struct cl2_ctx {
     double* partials = nullptr;  // reduction scratch: one double2 per block
     size_t npartials = 0;
     double* dscratch = nullptr;  // a few device doubles for host-read results
+    void** ptr_table = nullptr;  // device copy of the field-pointer table of the multi-shift update (2 x 64)
+    std::vector<void*> ptr_table_host;  // what ptr_table holds
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     int tune_zplanes = -1;  // z-planes per slab of the hopping kernels (-1: heuristic, 0: identity order)
     int tune_xcd     = 1;   // XCD-contiguous block assignment
@@ -107,6 +110,11 @@ hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, voi
                              int r);
 hipError_t launch_block_cg_scalars(cl2_ctx* c, int which, double* out, double* r2, const double* in,
                                    const double* active, int nrhs);
+
+/* fused multi-shift CG vector update over n complex elements */
+hipError_t launch_multishift(cl2_ctx* c, void* const* xs_dev, int x_prec, void* const* ps_dev, const void* r, int prec,
+                             size_t n, int nshift, const double* beta, const double* zeta, const double* alpha,
+                             const double* active);
 
 hipError_t launch_scalar_op(cl2_ctx* c, int op, int is_complex, double* out, const double* a, const double* b);
 hipError_t launch_vec_element(cl2_ctx* c, double* out, const double* in, int out_index, int in_index);

@@ -83,6 +83,34 @@ int cg_reliable_update(const lattices::Spinorfield_eo* x, const fermionmatrix::F
                        const physics::AdditionalParameters& additionalParameters, const ReliableUpdateOptions& options,
                        SolveStats* stats = nullptr);
 
+/* Reliable-update CG for one shifted staggered system (A + sigma) x = b with A hermitian positive (MdagM_eo):
+ * inner recurrence in options.inner_precision (FP32 or FP64) with the operator in that precision, FP64
+ * solution accumulation and true residual; x is the initial guess and the result (FP64).  Converged when
+ * |b - (A + sigma) x| <= rel_tol |b| in FP64. */
+int cg_reliable_update_stagg(const lattices::Staggeredfield_eo* x, const fermionmatrix::Fermionmatrix_stagg_eo& A,
+                             const lattices::Gaugefield& gf, hmc_float sigma, const lattices::Staggeredfield_eo& b,
+                             const hardware::System& system, hmc_float rel_tol,
+                             const physics::AdditionalParameters& additionalParameters,
+                             const ReliableUpdateOptions& options, SolveStats* stats = nullptr);
+
+struct MultishiftOptions {
+    int inner_precision = CL2_F32;  // precision of the multi-shift phase and of the refinement's inner loop
+    double sloppy_tol   = 1e-7;     // relative residual of each shift at which the multi-shift phase stops
+    double delta        = 0.1;      // reliable-update factor of the refinement
+    int max_iterations  = 100000;
+};
+
+/* Multi-shift CG for (A + sigma_k) x_k = b in two phases: the reference's CG-M recurrences
+ * (solver_shifted.cpp:114-475) on inner-precision fields with all shifts' vector updates fused into one
+ * pass per iteration, until every shift's residual zeta_k r is below sloppy_tol |b|; then each x_k is
+ * refined with cg_reliable_update_stagg to |b - (A + sigma_k) x_k| <= rel_tol |b| in FP64.  Returns the
+ * multi-shift iterations plus all refinement iterations; stats[k] holds shift k's refinement. */
+int cg_m_mixed(const std::vector<std::shared_ptr<lattices::Staggeredfield_eo>>& x,
+               const fermionmatrix::Fermionmatrix_stagg_eo& A, const lattices::Gaugefield& gf,
+               const std::vector<hmc_float>& sigma, const lattices::Staggeredfield_eo& b, const hardware::System& system,
+               hmc_float rel_tol, const physics::AdditionalParameters& additionalParameters,
+               const MultishiftOptions& options, std::vector<SolveStats>* stats = nullptr, int* multishift_iterations = nullptr);
+
 /* The reliable-update CG of cg_reliable_update on a block of right-hand sides b[r] (FP64 fields, nrhs = 1, 2,
  * 4, 8, 12 or 16), solved in lockstep: every inner iteration applies A once to the whole block with the
  * multi-right-hand-side operator, each right-hand side keeps its own alpha and beta (device arrays), a

@@ -162,6 +162,7 @@ class MdagM_eo final : public Fermionmatrix_stagg_eo {
   private:
     const bool upper_left;
     lattices::Staggeredfield_eo tmp;
+    mutable std::unique_ptr<lattices::Staggeredfield_eo> tmp_lo;  // for reduced-precision applications
 };
 
 }  // namespace fermionmatrix

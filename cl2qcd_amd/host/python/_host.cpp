@@ -79,6 +79,8 @@ PYBIND11_MODULE(_host, m)
     py::class_<Lattice>(m, "Lattice", "run parameters + hardware::System + InterfacesHandler + PRNG")
         .def(py::init<const std::vector<std::string>&>(), py::arg("args") = std::vector<std::string>{})
         .def_property_readonly("eo_sites", [](const Lattice& l) { return l.system.eo_sites(); })
+        .def_property_readonly("nspace", [](const Lattice& l) { return l.params.get_nspace(); })
+        .def_property_readonly("ntime", [](const Lattice& l) { return l.params.get_ntime(); })
         .def_property_readonly("mubar", [](const Lattice& l) { return meta::get_mubar(l.params); })
         .def_property_readonly("kappa", [](const Lattice& l) { return l.params.get_kappa(); })
         .def_property_readonly("mass", [](const Lattice& l) { return l.params.get_mass(); })
@@ -349,6 +351,60 @@ PYBIND11_MODULE(_host, m)
         py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000);
     m.def("convert_precision",
           [](const Spinorfield_eo& to, const Spinorfield_eo& from) { convert_precision(&to, from); });
+    m.def(
+        "cg_reliable_update_stagg",
+        [](const Staggeredfield_eo& x, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf, hmc_float sigma,
+           const Staggeredfield_eo& b, Lattice& l, hmc_float rel_tol, const AdditionalParameters& ap, int inner_prec,
+           double delta, int maxit) {
+            sv::ReliableUpdateOptions o;
+            o.inner_precision = inner_prec;
+            o.delta           = delta;
+            o.max_iterations  = maxit;
+            sv::SolveStats st;
+            {
+                py::gil_scoped_release rel;
+                sv::cg_reliable_update_stagg(&x, A, gf, sigma, b, l.system, rel_tol, ap, o, &st);
+            }
+            py::dict d;
+            d["iterations"]             = st.iterations;
+            d["reliable_updates"]       = st.reliable_updates;
+            d["true_relative_residual"] = st.true_relative_residual;
+            return d;
+        },
+        py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("sigma"), py::arg("b"), py::arg("lattice"),
+        py::arg("rel_tol"), py::arg("ap"), py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1,
+        py::arg("max_iterations") = 100000);
+    m.def(
+        "cg_m_mixed",
+        [](std::vector<Staggeredfield_eo*> xs, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf,
+           std::vector<hmc_float> sigma, const Staggeredfield_eo& b, Lattice& l, hmc_float rel_tol,
+           const AdditionalParameters& ap, int inner_prec, double sloppy_tol, double delta, int maxit) {
+            std::vector<std::shared_ptr<Staggeredfield_eo>> x;
+            for (auto* p : xs) x.emplace_back(p, [](Staggeredfield_eo*) {});  // Python owns the fields
+            sv::MultishiftOptions o;
+            o.inner_precision = inner_prec;
+            o.sloppy_tol      = sloppy_tol;
+            o.delta           = delta;
+            o.max_iterations  = maxit;
+            std::vector<sv::SolveStats> st;
+            int total = 0, ms = 0;
+            {
+                py::gil_scoped_release rel;
+                total = sv::cg_m_mixed(x, A, gf, sigma, b, l.system, rel_tol, ap, o, &st, &ms);
+            }
+            py::list per;
+            for (const auto& s : st) {
+                py::dict d;
+                d["iterations"]             = s.iterations;
+                d["reliable_updates"]       = s.reliable_updates;
+                d["true_relative_residual"] = s.true_relative_residual;
+                per.append(d);
+            }
+            return py::make_tuple(total, ms, per);
+        },
+        py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("sigma"), py::arg("b"), py::arg("lattice"),
+        py::arg("rel_tol"), py::arg("ap"), py::arg("inner_prec") = CL2_F32, py::arg("sloppy_tol") = 1e-7,
+        py::arg("delta") = 0.1, py::arg("max_iterations") = 100000);
     m.def("cg_m", [](std::vector<Staggeredfield_eo*> xs, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf,
                      std::vector<hmc_float> sigma, const Staggeredfield_eo& b, Lattice& l, hmc_float prec,
                      const AdditionalParameters& ap) {

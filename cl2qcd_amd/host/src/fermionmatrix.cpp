@@ -239,9 +239,10 @@ void MdagM_eo::operator()(const Staggeredfield_eo* out, const Gaugefield& gf, co
         get_system().check(
             cl2_MdagM_eo(get_system().ctx(), out->dev(), in.dev(), l.ptr, l.fmt, mass, upper_left, tmp.dev()));
     } else {
-        Staggeredfield_eo t(get_system(), in.precision());
+        if (!tmp_lo || tmp_lo->precision() != in.precision())
+            tmp_lo.reset(new Staggeredfield_eo(get_system(), in.precision()));
         get_system().check(
-            cl2_MdagM_eo(get_system().ctx(), out->dev(), in.dev(), l.ptr, l.fmt, mass, upper_left, t.dev()));
+            cl2_MdagM_eo(get_system().ctx(), out->dev(), in.dev(), l.ptr, l.fmt, mass, upper_left, tmp_lo->dev()));
     }
 }
 cl_ulong MdagM_eo::get_flops() const

@@ -552,3 +552,185 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
 }  // namespace solvers
 }  // namespace algorithms
 }  // namespace physics
+
+namespace physics {
+namespace algorithms {
+namespace solvers {
+
+int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fermionmatrix_stagg_eo& A,
+                             const Gaugefield& gf, hmc_float sigma, const Staggeredfield_eo& b,
+                             const hardware::System& system, hmc_float rel_tol, const physics::AdditionalParameters& ap,
+                             const ReliableUpdateOptions& opt, SolveStats* stats)
+{
+    if (x->precision() != CL2_F64 || b.precision() != CL2_F64)
+        throw std::invalid_argument("cg_reliable_update_stagg: x and b are FP64 fields");
+    if (opt.delta <= 0. || opt.delta >= 1.) throw std::invalid_argument("cg_reliable_update_stagg: delta must be in (0,1)");
+    const int lo = opt.inner_precision;
+    if (lo != CL2_F32 && lo != CL2_F64) throw std::invalid_argument("cg_reliable_update_stagg: inner precision FP32 or FP64");
+    const Staggeredfield_eo r(system, CL2_F64), t64(system, CL2_F64);
+    const Staggeredfield_eo r_lo(system, lo), p(system, lo), v(system, lo), x_lo(system, lo);
+    const Scalar<hmc_float> alpha(system), malpha(system), pAp(system), r2(system), zero(system);
+    zero.store(0.);
+    auto apply = [&](const Staggeredfield_eo* out, const Staggeredfield_eo& in) {  // out = (A + sigma) in
+        A(out, gf, in, &ap);
+        if (sigma != 0.) saxpy(out, sigma, in, *out);
+    };
+    const hmc_float b2     = squarenorm(b);
+    const hmc_float target = rel_tol * rel_tol * b2;
+    auto true_residual = [&]() {
+        apply(&t64, *x);
+        saxpby(&r, 1., b, -1., t64);  // r = b - (A + sigma) x
+        return squarenorm(r);
+    };
+    hmc_float rn = true_residual();
+    SolveStats st;
+    if (b2 == 0. || rn <= target) {
+        st.true_relative_residual = b2 > 0. ? std::sqrt(rn / b2) : 0.;
+        if (stats) *stats = st;
+        return 0;
+    }
+    convert_precision(&r_lo, r);
+    copyData(&p, r_lo);
+    x_lo.set_zero();
+    r2.store(rn);
+    hmc_float r2h = rn, rmax = std::sqrt(rn);
+    int k         = 0;
+    for (k = 0; k < opt.max_iterations; ++k) {
+        apply(&v, p);
+        scalar_product_real_part(&pAp, p, v);
+        divide(&alpha, r2, pAp);
+        subtract(&malpha, zero, alpha);
+        saxpy(&x_lo, alpha, p, x_lo);   // x_lo += alpha p
+        saxpy(&r_lo, malpha, v, r_lo);  // r_lo -= alpha v
+        squarenorm(&r2, r_lo);
+        rn = r2.get();
+        if (rn != rn) throw SolverStuck(k, "cg_reliable_update_stagg: NAN occured");
+        hmc_float beta = rn / r2h;
+        if (std::sqrt(rn) < opt.delta * rmax || rn <= target) {
+            convert_precision(&t64, x_lo);
+            saxpy(x, 1., t64, *x);  // x += x_lo
+            x_lo.set_zero();
+            rn = true_residual();
+            ++st.reliable_updates;
+            if (rn <= target) {
+                ++k;
+                break;
+            }
+            convert_precision(&r_lo, r);
+            const hmc_complex rp = scalar_product(r_lo, p);  // restore <r, p> = 0
+            saxpy(&p, hmc_complex{-rp.re / rn, -rp.im / rn}, r_lo, p);
+            beta = rn / r2h;
+            rmax = std::sqrt(rn);
+            r2.store(rn);
+        }
+        rmax = std::max(rmax, std::sqrt(rn));
+        saxpby(&p, 1., r_lo, beta, p);  // p = r_lo + beta p
+        r2h = rn;
+    }
+    st.iterations             = k;
+    st.true_relative_residual = std::sqrt(squarenorm(r) / b2);
+    if (stats) *stats = st;
+    if (st.true_relative_residual > rel_tol) throw SolverDidNotSolve(k, "cg_reliable_update_stagg");
+    return k;
+}
+
+int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const fermionmatrix::Fermionmatrix_stagg_eo& A,
+               const Gaugefield& gf, const std::vector<hmc_float>& sigma, const Staggeredfield_eo& b,
+               const hardware::System& system, hmc_float rel_tol, const physics::AdditionalParameters& ap,
+               const MultishiftOptions& opt, std::vector<SolveStats>* stats, int* multishift_iterations)
+{
+    const int n = (int)sigma.size();
+    if (n < 1 || n > 64 || (int)x.size() != n) throw std::invalid_argument("cg_m_mixed: 1 to 64 shifts, one x per shift");
+    if (b.precision() != CL2_F64) throw std::invalid_argument("cg_m_mixed: b is an FP64 field");
+    for (const auto& xi : x)
+        if (xi->precision() != CL2_F64) throw std::invalid_argument("cg_m_mixed: x are FP64 fields");
+    const int lo = opt.inner_precision;
+    const hmc_float b2 = squarenorm(b);
+    std::vector<SolveStats> st(n);
+    if (b2 == 0.) {
+        for (auto& xi : x) xi->set_zero();
+        if (stats) *stats = st;
+        if (multishift_iterations) *multishift_iterations = 0;
+        return 0;
+    }
+    // ---- phase 1: CG-M on inner-precision r and p_k, solutions accumulated in FP64, fused vector updates ----
+    const Staggeredfield_eo b_lo(system, lo), r(system, lo), p(system, lo), v(system, lo);
+    std::vector<std::unique_ptr<Staggeredfield_eo>> ps;
+    std::vector<void*> xp(n), pp(n);
+    for (int k = 0; k < n; ++k) {
+        ps.emplace_back(new Staggeredfield_eo(system, lo));
+        xp[k] = x[k]->dev();
+        pp[k] = ps[k]->dev();
+    }
+    const Vector<hmc_float> zeta_prev(n, system), zeta(n, system), zeta_foll(n, system), alpha_vec(n, system),
+        beta_vec(n, system), shift(n, system), active(n, system);
+    const Scalar<hmc_float> alpha_scalar_prev(system), alpha_scalar(system), beta_scalar_prev(system),
+        beta_scalar(system), zero(system), tmp1(system), tmp2(system), tmp3(system);
+    zero.store(0.0);
+    zeta_prev.store(std::vector<hmc_float>(n, 1.));
+    zeta.store(std::vector<hmc_float>(n, 1.));
+    alpha_vec.store(std::vector<hmc_float>(n, 0.));
+    shift.store(sigma);
+    std::vector<hmc_float> act(n, 1.);
+    active.store(act);
+    convert_precision(&b_lo, b);
+    for (int k = 0; k < n; ++k) {
+        x[k]->set_zero();
+        copyData(ps[k].get(), b_lo);
+    }
+    copyData(&r, b_lo);
+    copyData(&p, b_lo);
+    scalar_product_real_part(&tmp1, r, r);
+    beta_scalar.store(1.0);
+    alpha_scalar.store(0.0);
+    const hmc_float target1 = opt.sloppy_tol * opt.sloppy_tol * b2;
+    int n_active = n, iter = 0;
+    for (iter = 0; iter < opt.max_iterations && n_active > 0; ++iter) {
+        copyData(&beta_scalar_prev, beta_scalar);
+        A(&v, gf, p, &ap);
+        scalar_product_real_part(&tmp3, p, v);
+        divide(&beta_scalar, tmp1, tmp3);
+        subtract(&beta_scalar, zero, beta_scalar);
+        saxpy(&r, beta_scalar, v, r);  // r = beta v + r
+        scalar_product_real_part(&tmp2, r, r);
+        copyData(&alpha_scalar_prev, alpha_scalar);
+        divide(&alpha_scalar, tmp2, tmp1);
+        saxpy(&p, alpha_scalar, p, r);  // p = alpha p + r
+        update_zeta_cgm(&zeta_foll, zeta, zeta_prev, beta_scalar_prev, beta_scalar, alpha_scalar_prev, shift, n);
+        update_beta_cgm(&beta_vec, beta_scalar, zeta_foll, zeta, n);
+        update_alpha_cgm(&alpha_vec, alpha_scalar, zeta_foll, beta_vec, zeta, beta_scalar, n);
+        system.check(cl2_multishift_update_stagg_eo(system.ctx(), xp.data(), CL2_F64, pp.data(), r.dev(), lo, n,
+                                                    beta_vec.dev(), zeta_foll.dev(), alpha_vec.dev(), active.dev()));
+        copyData(&zeta_prev, zeta);
+        copyData(&zeta, zeta_foll);
+        copyData(&tmp1, tmp2);
+        if (iter % 10 == 9) {  // shift k's residual is zeta_k r
+            const hmc_float rr = tmp2.get();
+            if (rr != rr) throw SolverStuck(iter, "cg_m_mixed: NAN occured");
+            const std::vector<hmc_float> z = zeta.get();
+            bool changed                   = false;
+            for (int k = 0; k < n; ++k)
+                if (act[k] != 0. && z[k] * z[k] * rr < target1) {
+                    act[k] = 0.;
+                    --n_active;
+                    changed = true;
+                }
+            if (changed) active.store(act);
+        }
+    }
+    if (multishift_iterations) *multishift_iterations = iter;
+    // ---- phase 2: refine every shift in FP64 ----
+    ReliableUpdateOptions ro;
+    ro.inner_precision = lo;
+    ro.delta           = opt.delta;
+    ro.max_iterations  = opt.max_iterations;
+    int total          = iter;
+    for (int k = 0; k < n; ++k)
+        total += cg_reliable_update_stagg(x[k].get(), A, gf, sigma[k], b, system, rel_tol, ap, ro, &st[k]);
+    if (stats) *stats = st;
+    return total;
+}
+
+}  // namespace solvers
+}  // namespace algorithms
+}  // namespace physics

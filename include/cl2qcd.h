@@ -249,6 +249,17 @@ int cl2_block_cg_alpha(cl2_ctx* ctx, double* alpha, const double* r2, const doub
                        int nrhs);
 int cl2_block_cg_beta(cl2_ctx* ctx, double* beta, double* r2, const double* norms, const double* active, int nrhs);
 
+/* ==== fused multi-shift update ====
+ * SolverShifted::updateVectorQuantities (physics/algorithms/solver_shifted.cpp:174-198), which launches
+ * saxpy_stagg_eo and saxpby_stagg_eo once per unconverged shift, as one pass: for every k < nshift with
+ * active[k] != 0,  x[k] = beta[k] p[k] + x[k]  and then  p[k] = zeta[k] r + alpha[k] p[k].  x and p are
+ * host arrays of nshift device pointers to staggered eo fields; beta, zeta, alpha, active are device arrays
+ * (the reference's beta_vec, zeta_foll, alpha_vec and a convergence mask).  r and p are in precision prec,
+ * the x[k] in x_prec (FP64 x with FP32 r, p accumulates a reduced-precision recurrence's solutions in FP64). */
+int cl2_multishift_update_stagg_eo(cl2_ctx* ctx, void* const* x, int x_prec, void* const* p, const void* r, int prec,
+                                   int nshift, const double* beta, const double* zeta, const double* alpha,
+                                   const double* active);
+
 /* ==== staggered (hardware/code/fermions_staggered.hpp:46-91, spinors_staggered.hpp) ==== */
 
 /* Fermions_staggered::D_KS_eo_device (fermions_staggered.cpp:141-168); output parity == evenodd
