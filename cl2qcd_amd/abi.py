@@ -111,6 +111,7 @@ SIGNATURES = {
     "cl2_block_set_zero_eo": (_i, [_vp, _vp, _i, _i]),
     "cl2_block_cg_alpha": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
     "cl2_block_cg_beta": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
+    "cl2_gaussian_eo": (_i, [_vp, _i, _vp, _i, ctypes.c_uint64]),
     "cl2_multishift_update_stagg_eo": (_i, [_vp, ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp), _vp, _i, _i, _dp, _dp,
                                             _dp, _dp]),
     "cl2_set_zero_eo": (_i, [_vp, _vp, _i]),

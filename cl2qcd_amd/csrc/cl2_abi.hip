@@ -807,6 +807,15 @@ int cl2_saxpy_AND_cgnorm_eo(cl2_ctx* c, void* out, const void* x, const void* y,
     return run_blas(c, B_Y_MINUS_AX, (size_t)12 * c->g.veo, out, x, y, nullptr, prec, *alpha, value(0.), res, 0, 1);
 }
 
+int cl2_gaussian_eo(cl2_ctx* c, int kind, void* out, int prec, uint64_t seed)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, kind == CL2_FIELD_SPINOR_EO || kind == CL2_FIELD_SU3VEC_EO, "kind must be a spinor or su3vec field");
+    CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
+    CL2_HIP(c, launch_gaussian(c, out, prec, (size_t)(kind == CL2_FIELD_SPINOR_EO ? 12 : 3) * c->g.veo, seed));
+    return CL2_OK;
+}
+
 int cl2_set_zero_eo(cl2_ctx* c, void* out, int prec)
 {
     if (!c) return CL2_E_ARG;

@@ -109,6 +109,29 @@ __global__ void k_gauge_convert(void* dst, int dfmt, const double2* __restrict__
     }
 }
 
+/* complex Gaussian field, N(0, 1) per real component, from the counter hash of (seed, element) (Box-Muller) */
+template<typename R>
+__global__ void k_gaussian(vec2<R>* out, size_t n, uint64_t seed)
+{
+    const size_t step = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
+        const uint64_t ctr = (seed * 0x100000001b3ull) ^ (e * 2ull);
+        const double r1 = u01(mix64(ctr)), r2 = u01(mix64(ctr + 1));
+        const double rad = sqrt(-2.0 * log(r1)), ph = 6.283185307179586 * r2;
+        out[e]           = vec2<R>{R(rad * cos(ph)), R(rad * sin(ph))};
+    }
+}
+
+hipError_t launch_gaussian(cl2_ctx* c, void* dev, int prec, size_t n, uint64_t seed)
+{
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
+    if (prec == CL2_F64)
+        hipLaunchKernelGGL(k_gaussian<double>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<double2*>(dev), n, seed);
+    else
+        hipLaunchKernelGGL(k_gaussian<float>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<float2*>(dev), n, seed);
+    return hipGetLastError();
+}
+
 hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src)
 {
     const size_t n    = 8ull * c->g.veo;

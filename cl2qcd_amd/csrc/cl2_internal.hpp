@@ -90,6 +90,7 @@ hipError_t launch_gauge_update(cl2_ctx* c, void* gauge, double beta, int mode, u
 hipError_t launch_plaquette(cl2_ctx* c, const void* gauge, double* res);
 hipError_t launch_gauge_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, double eps);
 hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src);
+hipError_t launch_gaussian(cl2_ctx* c, void* dev, int prec, size_t n, uint64_t seed);
 
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im);
 

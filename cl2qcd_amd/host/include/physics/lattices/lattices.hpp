@@ -120,6 +120,9 @@ class Spinorfield_eo {
     void gamma5() const;
     void zero() const;
     void cold() const;
+    /* Gaussian field, N(0, 1) per real component, seeded from the PRNG (spinorfield_eo.hpp:93) */
+    void gaussian(const PRNG& prng) const;
+    void gaussian_from_seed(uint64_t seed) const;
     unsigned get_elements() const noexcept;
     void import(const spinor* host) const;
     std::vector<spinor> get_host() const;
@@ -140,6 +143,9 @@ class Staggeredfield_eo {
     Staggeredfield_eo(const hardware::System& system, int prec = CL2_F64);
     void set_zero() const;
     void set_cold() const;
+    /* Gaussian field, N(0, 1) per real component, seeded from the PRNG (staggeredfield_eo.hpp:87) */
+    void set_gaussian(const PRNG& prng) const;
+    void gaussian_from_seed(uint64_t seed) const;
     unsigned get_elements() const noexcept;
     void import(const su3vec* host) const;
     std::vector<su3vec> get_host() const;

@@ -1,0 +1,116 @@
+// Inversion serving on one MI355X: gauge configurations stay resident in HBM, a
+// FIFO queue of inversion requests is admitted against an HBM budget, requests
+// that share an operator (configuration, action and mass parameters) are merged
+// into one multi-right-hand-side solve, and a pool of workers -- each with its own
+// HIP stream -- runs admitted batches concurrently, so requests arriving while
+// others run are started as soon as a worker and memory are free (continuous
+// batching).  No counterpart in the reference, whose inverter executable solves
+// one source after the other (physics/algorithms/inversion.cpp:78-242).
+#pragma once
+
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../physics/algorithms/solvers/solvers.hpp"
+
+namespace serving {
+
+enum class Action { twistedmass = 0, staggered = 1 };
+
+struct Request {
+    int config = 0;  // id returned by Engine::load_config
+    Action action = Action::twistedmass;
+    double kappa = 0.16, mu = 0.005;  // twisted mass: QplusQminus_eo with mubar = 2 kappa mu
+    double mass  = 0.01;              // staggered: MdagM_eo
+    // right-hand sides as host eo fields in the reference's AoS order: Veo x 24 doubles (Wilson) or
+    // Veo x 6 doubles (staggered)
+    std::vector<std::vector<double>> sources;
+    // or, when sources is empty, one Gaussian right-hand side per seed, generated on the device
+    std::vector<uint64_t> gaussian_seeds;
+    double rel_tol        = 1e-10;
+    bool return_solutions = true;  // false: only residuals and timings come back
+    int nrhs() const { return sources.empty() ? (int)gaussian_seeds.size() : (int)sources.size(); }
+};
+
+struct Result {
+    long id = -1;
+    bool ok = false;
+    std::string error;
+    std::vector<std::vector<double>> solutions;  // same layout as the sources
+    int width = 0;                               // doubles per eo site: 24 (Wilson) or 6 (staggered)
+    std::vector<double> true_rel_residual;
+    int iterations = 0;           // inner iterations of the solve that carried this request
+    int batch_requests = 0;       // requests merged into that solve
+    int batch_rhs      = 0;       // right-hand sides in that solve
+    double arrival = 0., admitted = 0., finished = 0.;  // seconds on the engine clock
+};
+
+struct EngineOptions {
+    size_t hbm_budget_bytes = (size_t)64 << 30;  // device memory for solver work fields
+    int workers             = 4;                 // concurrent solves, one HIP stream each
+    int inner_precision     = CL2_F32;           // reduced-precision inner loop of the mixed solvers
+    int max_batch_rhs       = 12;                // right-hand sides merged into one Wilson solve
+    bool mixed              = true;              // false: the reference's FP64 solvers, one request at a time
+};
+
+class Engine {
+  public:
+    explicit Engine(const EngineOptions& options);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    /* heatbath configuration of ns^3 x nt at beta (cold start, sweeps x (heatbath + overrelaxation),
+     * antiperiodic fermion boundary in time), resident until the engine is destroyed; returns its id */
+    int load_config(int ns, int nt, double beta, int sweeps, uint64_t seed);
+    double plaquette(int config) const;
+    long eo_sites(int config) const;
+    size_t resident_bytes() const;
+
+    /* queue a request (arrival time = now); returns its id */
+    long submit(Request r);
+    /* block until request id has finished */
+    Result wait(long id);
+    /* block until every submitted request has finished */
+    void drain();
+    double now() const;
+    /* create every worker's device context and operators for every loaded lattice size now, so the
+     * first requests do not pay for them */
+    void warmup();
+    /* largest HBM use of work fields seen so far */
+    size_t peak_work_bytes() const;
+
+  private:
+    struct Config;
+    struct Pending;
+    struct Batch;
+    struct Worker;
+    size_t work_bytes(const Pending& p) const;
+    bool compatible(const Pending& a, const Pending& b) const;
+    void schedule_locked();
+    void worker_loop(int w);
+    void run_batch(Worker& wk, Batch& b);
+
+    EngineOptions opt;
+    std::vector<std::unique_ptr<Config>> configs;
+    mutable std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<std::unique_ptr<Pending>> queue;
+    std::map<long, Result> done;
+    std::vector<std::unique_ptr<Worker>> workers;
+    std::vector<std::thread> threads;
+    long next_id       = 0;
+    long outstanding   = 0;
+    size_t used_bytes  = 0, peak_bytes = 0;
+    bool stopping      = false;
+    double t0          = 0.;
+};
+
+}  // namespace serving

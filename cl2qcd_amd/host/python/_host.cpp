@@ -11,6 +11,7 @@
 #include "../include/physics/algorithms/solvers/solvers.hpp"
 #include "../include/physics/algorithms/su3heatbath.hpp"
 #include "../include/physics/observables/gaugeObservables.hpp"
+#include "../include/serving/engine.hpp"
 
 namespace py = pybind11;
 using namespace physics;
@@ -147,6 +148,7 @@ PYBIND11_MODULE(_host, m)
         .def(py::init([](Lattice& l, int prec) { return new Spinorfield_eo(l.system, prec); }), py::keep_alive<1, 2>(),
              py::arg("lattice"), py::arg("prec") = CL2_F64)
         .def("gamma5", &Spinorfield_eo::gamma5)
+        .def("gaussian_from_seed", &Spinorfield_eo::gaussian_from_seed)
         .def("zero", &Spinorfield_eo::zero)
         .def("cold", &Spinorfield_eo::cold)
         .def("get_host", [](const Spinorfield_eo& f) { return to_numpy(f.get_host()); })
@@ -160,6 +162,7 @@ PYBIND11_MODULE(_host, m)
              py::keep_alive<1, 2>(), py::arg("lattice"), py::arg("prec") = CL2_F64)
         .def("set_zero", &Staggeredfield_eo::set_zero)
         .def("set_cold", &Staggeredfield_eo::set_cold)
+        .def("gaussian_from_seed", &Staggeredfield_eo::gaussian_from_seed)
         .def("get_host", [](const Staggeredfield_eo& f) { return to_numpy(f.get_host()); })
         .def("import_", [](const Staggeredfield_eo& f, py::array_t<double> a) {
             auto v = from_numpy<su3vec>(a, f.get_elements());
@@ -413,4 +416,89 @@ PYBIND11_MODULE(_host, m)
         py::gil_scoped_release rel;
         return sv::cg_m(x, A, gf, sigma, b, l.system, l.handler, prec, ap);
     });
+
+    py::class_<serving::Engine>(m, "Engine")
+        .def(py::init([](size_t hbm_budget_bytes, int workers, int inner_precision, int max_batch_rhs, bool mixed) {
+                 serving::EngineOptions o;
+                 o.hbm_budget_bytes = hbm_budget_bytes;
+                 o.workers          = workers;
+                 o.inner_precision  = inner_precision;
+                 o.max_batch_rhs    = max_batch_rhs;
+                 o.mixed            = mixed;
+                 return new serving::Engine(o);
+             }),
+             py::arg("hbm_budget_bytes") = (size_t)64 << 30, py::arg("workers") = 4,
+             py::arg("inner_precision") = CL2_F32, py::arg("max_batch_rhs") = 12, py::arg("mixed") = true)
+        .def(
+            "load_config",
+            [](serving::Engine& e, int ns, int nt, double beta, int sweeps, uint64_t seed) {
+                py::gil_scoped_release rel;
+                return e.load_config(ns, nt, beta, sweeps, seed);
+            },
+            py::arg("ns"), py::arg("nt"), py::arg("beta"), py::arg("sweeps"), py::arg("seed"))
+        .def("plaquette", &serving::Engine::plaquette)
+        .def("eo_sites", &serving::Engine::eo_sites)
+        .def("resident_bytes", &serving::Engine::resident_bytes)
+        .def("peak_work_bytes", &serving::Engine::peak_work_bytes)
+        .def("now", &serving::Engine::now)
+        .def(
+            "submit",
+            [](serving::Engine& e, int config, const std::string& action, double kappa, double mu, double mass,
+               std::vector<py::array_t<double, py::array::c_style | py::array::forcecast>> sources,
+               std::vector<uint64_t> gaussian_seeds, double rel_tol, bool return_solutions) {
+                serving::Request r;
+                r.config = config;
+                if (action == "twistedmass")
+                    r.action = serving::Action::twistedmass;
+                else if (action == "staggered")
+                    r.action = serving::Action::staggered;
+                else
+                    throw std::invalid_argument("action must be twistedmass or staggered");
+                r.kappa   = kappa;
+                r.mu      = mu;
+                r.mass    = mass;
+                r.rel_tol = rel_tol;
+                for (auto& a : sources) r.sources.emplace_back(a.data(), a.data() + a.size());
+                r.gaussian_seeds   = gaussian_seeds;
+                r.return_solutions = return_solutions;
+                py::gil_scoped_release rel;
+                return e.submit(std::move(r));
+            },
+            py::arg("config"), py::arg("action"), py::arg("kappa") = 0.16, py::arg("mu") = 0.005,
+            py::arg("mass") = 0.01, py::arg("sources") = std::vector<py::array_t<double>>(),
+            py::arg("gaussian_seeds") = std::vector<uint64_t>(), py::arg("rel_tol") = 1e-10,
+            py::arg("return_solutions") = true)
+        .def("warmup", &serving::Engine::warmup)
+        .def("wait",
+             [](serving::Engine& e, long id) {
+                 serving::Result r;
+                 {
+                     py::gil_scoped_release rel;
+                     r = e.wait(id);
+                 }
+                 py::dict d;
+                 d["id"]      = r.id;
+                 d["ok"]      = r.ok;
+                 d["error"]   = r.error;
+                 py::list sols;
+                 for (const auto& s : r.solutions) {
+                     const size_t w = (size_t)r.width;
+                     py::array_t<double> a({s.size() / w, w});
+                     std::memcpy(a.mutable_data(), s.data(), s.size() * sizeof(double));
+                     sols.append(a);
+                 }
+                 d["solutions"]         = sols;
+                 d["true_rel_residual"] = r.true_rel_residual;
+                 d["iterations"]        = r.iterations;
+                 d["batch_requests"]    = r.batch_requests;
+                 d["batch_rhs"]         = r.batch_rhs;
+                 d["arrival"]           = r.arrival;
+                 d["admitted"]          = r.admitted;
+                 d["finished"]          = r.finished;
+                 return d;
+             })
+        .def("drain", [](serving::Engine& e) {
+            py::gil_scoped_release rel;
+            e.drain();
+        });
 }

@@ -274,6 +274,15 @@ Spinorfield_eo::Spinorfield_eo(const hardware::System& system, int prec)
     : system(system), prec(prec), buf(system, cl2_field_bytes(system.ctx(), CL2_FIELD_SPINOR_EO, prec))
 {
 }
+void Spinorfield_eo::gaussian(const PRNG& prng) const
+{
+    gaussian_from_seed(prng.next_seed());
+}
+void Spinorfield_eo::gaussian_from_seed(uint64_t seed) const
+{
+    if (prec == CL2_F16) throw std::invalid_argument("gaussian: FP64 or FP32 field");
+    system.check(cl2_gaussian_eo(system.ctx(), CL2_FIELD_SPINOR_EO, buf.ptr(), prec, seed));
+}
 unsigned Spinorfield_eo::get_elements() const noexcept
 {
     return (unsigned)system.eo_sites();
@@ -331,6 +340,14 @@ Staggeredfield_eo::Staggeredfield_eo(const hardware::System& system, const Stagg
 Staggeredfield_eo::Staggeredfield_eo(const hardware::System& system, int prec)
     : system(system), prec(prec), buf(system, cl2_field_bytes(system.ctx(), CL2_FIELD_SU3VEC_EO, prec))
 {
+}
+void Staggeredfield_eo::set_gaussian(const PRNG& prng) const
+{
+    gaussian_from_seed(prng.next_seed());
+}
+void Staggeredfield_eo::gaussian_from_seed(uint64_t seed) const
+{
+    system.check(cl2_gaussian_eo(system.ctx(), CL2_FIELD_SU3VEC_EO, buf.ptr(), prec, seed));
 }
 unsigned Staggeredfield_eo::get_elements() const noexcept
 {

@@ -1,0 +1,435 @@
+// Inversion serving engine: resident configurations, FIFO admission against an
+// HBM budget, merging of requests that share an operator into one multi-RHS solve,
+// and a worker pool with one HIP stream per worker so admitted solves overlap.
+#include "../include/serving/engine.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+
+#include "../include/physics/algorithms/su3heatbath.hpp"
+#include "../include/physics/observables/gaugeObservables.hpp"
+
+namespace serving {
+
+using namespace physics;
+using namespace physics::lattices;
+namespace fm = physics::fermionmatrix;
+namespace sv = physics::algorithms::solvers;
+
+namespace {
+
+std::vector<std::string> lattice_args(int ns, int nt, const char* action)
+{
+    return {"--nSpace=" + std::to_string(ns), "--nTime=" + std::to_string(nt), std::string("--fermionAction=") + action,
+            "--thetaFermionTemporal=1", "--solverMaxIterations=1000000"};
+}
+
+/* right-hand-side counts with a compiled multi-RHS operator */
+int padded_rhs(int n)
+{
+    for (int s : {1, 2, 4, 8, 12, 16})
+        if (n <= s) return s;
+    return -1;
+}
+
+}  // namespace
+
+/* one resident configuration and the objects that own it */
+struct Engine::Config {
+    int ns, nt;
+    std::unique_ptr<meta::Inputparameters> params;
+    std::unique_ptr<hardware::System> system;
+    std::unique_ptr<PrngParametersImplementation> prng_params;
+    std::unique_ptr<PRNG> prng;
+    std::unique_ptr<InterfacesHandlerImplementation> handler;
+    std::unique_ptr<Gaugefield> gf;
+    double plaquette = 0.;
+    size_t bytes     = 0;
+};
+
+struct Engine::Pending {
+    long id;
+    Request req;
+    double arrival;
+    int nrhs() const { return req.nrhs(); }
+};
+
+struct Engine::Batch {
+    std::vector<std::unique_ptr<Pending>> items;
+    size_t bytes   = 0;
+    double admitted = 0.;
+};
+
+/* a worker's device context and operators for one lattice size */
+struct WorkerLattice {
+    std::unique_ptr<meta::Inputparameters> p_tm, p_st;
+    std::unique_ptr<hardware::System> system;
+    std::unique_ptr<InterfacesHandlerImplementation> h_tm, h_st;
+    std::unique_ptr<fm::QplusQminus_eo> qq;
+    std::unique_ptr<fm::MdagM_eo> mdagm;
+};
+
+struct Engine::Worker {
+    std::map<std::pair<int, int>, std::unique_ptr<WorkerLattice>> lattices;
+    std::unique_ptr<Batch> batch;  // assigned, not yet finished
+    bool busy = false;
+    WorkerLattice& lattice(int ns, int nt)
+    {
+        auto& w = lattices[{ns, nt}];
+        if (!w) {
+            w.reset(new WorkerLattice);
+            w->p_tm.reset(new meta::Inputparameters(lattice_args(ns, nt, "twistedmass")));
+            w->p_st.reset(new meta::Inputparameters(lattice_args(ns, nt, "rooted_stagg")));
+            w->system.reset(new hardware::System(*w->p_tm));
+            w->h_tm.reset(new InterfacesHandlerImplementation(*w->p_tm));
+            w->h_st.reset(new InterfacesHandlerImplementation(*w->p_st));
+            w->qq.reset(new fm::QplusQminus_eo(*w->system, w->h_tm->getInterface<fm::QplusQminus_eo>()));
+            w->mdagm.reset(new fm::MdagM_eo(*w->system, w->h_st->getInterface<fm::MdagM_eo>()));
+        }
+        return *w;
+    }
+};
+
+Engine::Engine(const EngineOptions& options) : opt(options)
+{
+    if (opt.workers < 1 || opt.workers > 16) throw std::invalid_argument("serving: 1 to 16 workers");
+    if (opt.max_batch_rhs < 1 || opt.max_batch_rhs > 16) throw std::invalid_argument("serving: max_batch_rhs in [1, 16]");
+    if (opt.inner_precision != CL2_F32 && opt.inner_precision != CL2_F16)
+        throw std::invalid_argument("serving: inner precision FP32 or FP16");
+    t0 = 0.;
+    t0 = now();
+    for (int w = 0; w < opt.workers; ++w) workers.emplace_back(new Worker);
+    for (int w = 0; w < opt.workers; ++w) threads.emplace_back(&Engine::worker_loop, this, w);
+}
+
+Engine::~Engine()
+{
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        stopping = true;
+    }
+    cv_work.notify_all();
+    for (auto& t : threads) t.join();
+}
+
+double Engine::now() const
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count() - t0;
+}
+
+int Engine::load_config(int ns, int nt, double beta, int sweeps, uint64_t seed)
+{
+    std::unique_ptr<Config> c(new Config);
+    c->ns = ns;
+    c->nt = nt;
+    std::vector<std::string> a = lattice_args(ns, nt, "twistedmass");
+    a.push_back("--beta=" + std::to_string(beta));
+    a.push_back("--hostSeed=" + std::to_string(seed));
+    c->params.reset(new meta::Inputparameters(a));
+    c->system.reset(new hardware::System(*c->params));
+    c->prng_params.reset(new PrngParametersImplementation(*c->params));
+    c->prng.reset(new PRNG(*c->system, c->prng_params.get()));
+    c->handler.reset(new InterfacesHandlerImplementation(*c->params));
+    c->gf.reset(new Gaugefield(*c->system, &c->handler->getInterface<Gaugefield>(), *c->prng, false));
+    for (int s = 0; s < sweeps; ++s) physics::algorithms::su3heatbath(*c->gf, *c->prng, 1);
+    c->plaquette = physics::observables::measurePlaquette(c->gf.get());
+    // reduced-precision link copies are made now, so workers only read them
+    c->gf->dev_f32();
+    if (opt.inner_precision == CL2_F16) c->gf->dev_f16();
+    c->system->synchronize();
+    const size_t veo = (size_t)c->system->eo_sites();
+    c->bytes = veo * 8 * (144 + 48 + (opt.inner_precision == CL2_F16 ? 24 : 0));
+    std::lock_guard<std::mutex> lk(mu);
+    configs.push_back(std::move(c));
+    return (int)configs.size() - 1;
+}
+
+double Engine::plaquette(int config) const
+{
+    return configs.at(config)->plaquette;
+}
+long Engine::eo_sites(int config) const
+{
+    return configs.at(config)->system->eo_sites();
+}
+size_t Engine::resident_bytes() const
+{
+    size_t s = 0;
+    for (const auto& c : configs) s += c->bytes;
+    return s;
+}
+size_t Engine::peak_work_bytes() const
+{
+    std::lock_guard<std::mutex> lk(mu);
+    return peak_bytes;
+}
+
+/* device bytes a request's solve needs: the block solver's FP64 and inner-precision fields and
+ * temporaries per (padded) right-hand side for Wilson, the reliable-update CG's fields for staggered */
+size_t Engine::work_bytes(const Pending& p) const
+{
+    const size_t veo = (size_t)configs.at(p.req.config)->system->eo_sites();
+    if (p.req.action == Action::twistedmass) {
+        const int n = std::max(1, padded_rhs(p.nrhs()));
+        return veo * (size_t)n * (6 * 192 + 6 * 96 + 2 * 192);
+    }
+    return veo * (4 * 48 + 4 * 24 + 72 + 2 * 48 * (size_t)p.nrhs());
+}
+
+bool Engine::compatible(const Pending& a, const Pending& b) const
+{
+    return opt.mixed && a.req.action == Action::twistedmass && b.req.action == Action::twistedmass &&
+           a.req.config == b.req.config && a.req.kappa == b.req.kappa && a.req.mu == b.req.mu &&
+           a.req.rel_tol == b.req.rel_tol;
+}
+
+long Engine::submit(Request r)
+{
+    if (r.config < 0 || r.config >= (int)configs.size()) throw std::invalid_argument("serving: unknown configuration");
+    if (r.nrhs() < 1 || r.nrhs() > 16) throw std::invalid_argument("serving: 1 to 16 right-hand sides");
+    if (!r.sources.empty() && !r.gaussian_seeds.empty())
+        throw std::invalid_argument("serving: give sources or Gaussian seeds, not both");
+    const size_t veo = (size_t)configs[r.config]->system->eo_sites();
+    const size_t per = veo * (r.action == Action::twistedmass ? 24 : 6);
+    for (const auto& s : r.sources)
+        if (s.size() != per) throw std::invalid_argument("serving: source of the wrong size for its lattice");
+    std::unique_ptr<Pending> p(new Pending);
+    p->req     = std::move(r);
+    p->arrival = now();
+    std::lock_guard<std::mutex> lk(mu);
+    p->id = next_id++;
+    const long id = p->id;
+    queue.push_back(std::move(p));
+    ++outstanding;
+    schedule_locked();
+    return id;
+}
+
+/* FIFO admission: the head of the queue starts when a worker is idle and its work fields fit the HBM
+ * budget (or nothing is running, so an oversized request cannot wait forever); queued requests that
+ * share its operator are merged into the same solve while right-hand sides and memory allow. */
+void Engine::schedule_locked()
+{
+    while (!queue.empty()) {
+        int free_w = -1;
+        bool any_busy = false;
+        for (int w = 0; w < (int)workers.size(); ++w) {
+            if (workers[w]->busy)
+                any_busy = true;
+            else if (free_w < 0)
+                free_w = w;
+        }
+        if (free_w < 0) return;
+        if (!opt.mixed && any_busy) return;  // the FP64 baseline runs one request at a time
+        const size_t need = work_bytes(*queue.front());
+        if (used_bytes + need > opt.hbm_budget_bytes && any_busy) return;
+        std::unique_ptr<Batch> b(new Batch);
+        b->items.push_back(std::move(queue.front()));
+        queue.pop_front();
+        int rhs = b->items[0]->nrhs();
+        for (auto it = queue.begin(); it != queue.end();) {
+            if (compatible(*b->items[0], **it) && rhs + (*it)->nrhs() <= opt.max_batch_rhs) {
+                rhs += (*it)->nrhs();
+                b->items.push_back(std::move(*it));
+                it = queue.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        // memory of the merged solve (its padded right-hand-side count)
+        Pending merged;
+        merged.req.config = b->items[0]->req.config;
+        merged.req.action = b->items[0]->req.action;
+        merged.req.gaussian_seeds.resize(rhs);
+        b->bytes    = work_bytes(merged);
+        b->admitted = now();
+        used_bytes += b->bytes;
+        peak_bytes = std::max(peak_bytes, used_bytes);
+        workers[free_w]->busy  = true;
+        workers[free_w]->batch = std::move(b);
+        cv_work.notify_all();
+    }
+}
+
+void Engine::worker_loop(int w)
+{
+    Worker& wk = *workers[w];
+    for (;;) {
+        std::unique_ptr<Batch> b;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stopping || wk.batch != nullptr; });
+            if (!wk.batch) return;  // stopping with nothing assigned
+            b = std::move(wk.batch);
+        }
+        run_batch(wk, *b);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            used_bytes -= b->bytes;
+            wk.busy = false;
+            outstanding -= (long)b->items.size();
+            schedule_locked();
+        }
+        cv_done.notify_all();
+    }
+}
+
+void Engine::run_batch(Worker& wk, Batch& b)
+{
+    const Request& head = b.items[0]->req;
+    Config& cfg         = *configs[head.config];
+    std::vector<Result> res(b.items.size());
+    int total_rhs = 0;
+    for (const auto& it : b.items) total_rhs += it->nrhs();
+    try {
+        WorkerLattice& L              = wk.lattice(cfg.ns, cfg.nt);
+        const hardware::System& sys   = *L.system;
+        const Gaugefield& gf          = *cfg.gf;
+        int iterations                = 0;
+        std::vector<std::vector<double>> sol;
+        std::vector<double> resid;
+        std::vector<bool> keep;  // right-hand sides whose solutions go back to the host
+        for (const auto& it : b.items)
+            for (int k = 0; k < it->nrhs(); ++k) keep.push_back(it->req.return_solutions);
+        if (head.action == Action::twistedmass) {
+            const WilsonAdditionalParameters ap(head.kappa, 2. * head.kappa * head.mu);
+            const int n = opt.mixed ? padded_rhs(total_rhs) : total_rhs;
+            std::vector<std::unique_ptr<Spinorfield_eo>> bs, xs;
+            for (int r = 0; r < n; ++r) {
+                bs.emplace_back(new Spinorfield_eo(sys));
+                xs.emplace_back(new Spinorfield_eo(sys));
+                xs.back()->zero();
+            }
+            int r = 0;
+            for (const auto& it : b.items) {
+                for (const auto& s : it->req.sources) bs[r++]->import(reinterpret_cast<const spinor*>(s.data()));
+                for (uint64_t seed : it->req.gaussian_seeds) bs[r++]->gaussian_from_seed(seed);
+            }
+            for (; r < n; ++r) bs[r]->zero();  // padding right-hand sides are inactive from the start
+            if (!opt.mixed) {
+                for (int k = 0; k < n; ++k)
+                    iterations += sv::cg(xs[k].get(), *L.qq, gf, *bs[k], sys, *L.h_tm,
+                                         head.rel_tol * head.rel_tol * squarenorm(*bs[k]), ap);
+            } else if (n == 1) {
+                sv::ReliableUpdateOptions o;
+                o.inner_precision = opt.inner_precision;
+                iterations        = sv::cg_reliable_update(xs[0].get(), *L.qq, gf, *bs[0], sys, head.rel_tol, ap, o);
+            } else {
+                sv::ReliableUpdateOptions o;
+                o.inner_precision = CL2_F32;
+                std::vector<const Spinorfield_eo*> xv, bv;
+                for (int k = 0; k < n; ++k) {
+                    xv.push_back(xs[k].get());
+                    bv.push_back(bs[k].get());
+                }
+                iterations = sv::cg_reliable_update_block(xv, *L.qq, gf, bv, sys, head.rel_tol, ap, o);
+            }
+            // FP64 true residual of every answer
+            const Spinorfield_eo t(sys);
+            for (int k = 0; k < total_rhs; ++k) {
+                (*L.qq)(&t, gf, *xs[k], ap);
+                saxpy(&t, hmc_complex{1., 0.}, t, *bs[k]);  // b - A x
+                resid.push_back(std::sqrt(squarenorm(t) / squarenorm(*bs[k])));
+                if (keep[k]) {
+                    const std::vector<spinor> h = xs[k]->get_host();
+                    const double* d             = reinterpret_cast<const double*>(h.data());
+                    sol.emplace_back(d, d + h.size() * 24);
+                } else {
+                    sol.emplace_back();
+                }
+            }
+        } else {
+            const StaggeredAdditionalParameters ap(head.mass);
+            const Staggeredfield_eo bf(sys), xf(sys), t(sys);
+            int k = 0;
+            for (const auto& it : b.items)
+                for (int j = 0; j < it->nrhs(); ++j, ++k) {
+                    if (it->req.sources.empty())
+                        bf.gaussian_from_seed(it->req.gaussian_seeds[j]);
+                    else
+                        bf.import(reinterpret_cast<const su3vec*>(it->req.sources[j].data()));
+                    xf.set_zero();
+                    if (opt.mixed) {
+                        sv::ReliableUpdateOptions o;
+                        o.inner_precision = CL2_F32;
+                        iterations += sv::cg_reliable_update_stagg(&xf, *L.mdagm, gf, 0., bf, sys, head.rel_tol, ap, o);
+                    } else {
+                        std::vector<std::shared_ptr<Staggeredfield_eo>> xv{
+                            std::shared_ptr<Staggeredfield_eo>(const_cast<Staggeredfield_eo*>(&xf),
+                                                               [](Staggeredfield_eo*) {})};
+                        iterations += sv::cg_m(xv, *L.mdagm, gf, {0.}, bf, sys, *L.h_st,
+                                               head.rel_tol * head.rel_tol * squarenorm(bf), ap);
+                    }
+                    (*L.mdagm)(&t, gf, xf, &ap);
+                    saxpby(&t, 1., bf, -1., t);
+                    resid.push_back(std::sqrt(squarenorm(t) / squarenorm(bf)));
+                    if (keep[k]) {
+                        const std::vector<su3vec> h = xf.get_host();
+                        const double* d             = reinterpret_cast<const double*>(h.data());
+                        sol.emplace_back(d, d + h.size() * 6);
+                    } else {
+                        sol.emplace_back();
+                    }
+                }
+        }
+        const double fin = now();
+        int r            = 0;
+        for (size_t i = 0; i < b.items.size(); ++i) {
+            Result& o        = res[i];
+            o.ok             = true;
+            o.iterations     = iterations;
+            o.batch_requests = (int)b.items.size();
+            o.batch_rhs      = total_rhs;
+            o.width          = head.action == Action::twistedmass ? 24 : 6;
+            for (int k = 0; k < b.items[i]->nrhs(); ++k, ++r) {
+                if (b.items[i]->req.return_solutions) o.solutions.push_back(std::move(sol[r]));
+                o.true_rel_residual.push_back(resid[r]);
+            }
+            o.finished = fin;
+        }
+    } catch (const std::exception& e) {
+        const double fin = now();
+        for (auto& o : res) {
+            o.ok       = false;
+            o.error    = e.what();
+            o.finished = fin;
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    for (size_t i = 0; i < b.items.size(); ++i) {
+        res[i].id       = b.items[i]->id;
+        res[i].arrival  = b.items[i]->arrival;
+        res[i].admitted = b.admitted;
+        done[res[i].id] = std::move(res[i]);
+    }
+}
+
+void Engine::warmup()
+{
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& w : workers) {
+        if (w->busy) throw std::logic_error("serving: warmup while requests run");
+        for (const auto& c : configs) w->lattice(c->ns, c->nt);
+    }
+}
+
+Result Engine::wait(long id)
+{
+    std::unique_lock<std::mutex> lk(mu);
+    if (id < 0 || id >= next_id) throw std::invalid_argument("serving: unknown request id");
+    cv_done.wait(lk, [&] { return done.count(id) != 0; });
+    Result r = std::move(done[id]);
+    done.erase(id);
+    return r;
+}
+
+void Engine::drain()
+{
+    std::unique_lock<std::mutex> lk(mu);
+    cv_done.wait(lk, [&] { return outstanding == 0; });
+}
+
+}  // namespace serving

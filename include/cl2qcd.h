@@ -210,6 +210,12 @@ int cl2_saxpy_AND_cgnorm_eo(cl2_ctx* ctx, void* out, const void* x, const void* 
                             double* res);
 /* set_zero_spinorfield_eoprec / set_eoprec_spinorfield_cold (spinors.cpp:448-460,805-817):
  * cold = every complex component 1/sqrt(12 V) */
+/* Spinorfield_eo::gaussian / Staggeredfield_eo::set_gaussian (physics/lattices/spinorfield_eo.hpp:93,
+ * staggeredfield_eo.hpp:87; ocl_kernel/spinorfield_eo_gaussian.cl):
+ * every real component N(0, 1) from a counter hash keyed by seed (the reference draws from ranluxcl with
+ * variance 1/2; fields here are reproducible from the seed alone).  kind: CL2_FIELD_SPINOR_EO or
+ * CL2_FIELD_SU3VEC_EO, FP64 or FP32. */
+int cl2_gaussian_eo(cl2_ctx* ctx, int kind, void* out, int prec, uint64_t seed);
 int cl2_set_zero_eo(cl2_ctx* ctx, void* out, int prec);
 int cl2_set_cold_eo(cl2_ctx* ctx, void* out, int prec);
 
