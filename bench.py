@@ -277,6 +277,92 @@ def multishift_leg(ns, nt, sweeps, seed, nshift=16):
             "speedup_vs_sequential": round(t_seq / t_ms, 3), "speedup_vs_reference_cgm": round(t_ref / t_ms, 3)}
 
 
+SERVE_SIZES = ((16, 32), (24, 48), (32, 64))
+SERVE_CONFIGS = {(16, 32): 3, (24, 48): 3, (32, 64): 2}  # 8 resident configurations
+
+
+def serve_stream(seconds, rate, seed, cfgs):
+    """Poisson arrivals (rate per second, seeded) of inversion requests: lattice size uniform over
+    16^3x32, 24^3x48, 32^3x64, one of that size's resident configurations, twisted mass (kappa, mu) or
+    staggered (mass) with equal probability, parameters uniform on 5-point grids over kappa in
+    [0.15, 0.16], mu in [0.003, 0.02], mass in [0.005, 0.05] (so that requests can share an operator),
+    1-12 Gaussian right-hand sides generated on the device, |r|/|b| <= 1e-10."""
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    kappas, mus, masses = np.linspace(0.15, 0.16, 5), np.linspace(0.003, 0.02, 5), np.linspace(0.005, 0.05, 5)
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            return out
+        size = SERVE_SIZES[rng.integers(3)]
+        cfg = cfgs[size][rng.integers(len(cfgs[size]))]
+        n = int(rng.integers(1, 13))
+        req = dict(config=cfg, gaussian_seeds=[int(v) for v in rng.integers(0, 2 ** 62, size=n)], rel_tol=1e-10,
+                   return_solutions=False)
+        if rng.random() < 0.5:
+            req.update(action="twistedmass", kappa=float(rng.choice(kappas)), mu=float(rng.choice(mus)))
+        else:
+            req.update(action="staggered", mass=float(rng.choice(masses)))
+        out.append((t, req))
+
+
+def serve_run(engine, stream):
+    start = engine.now()
+    ids = []
+    for t, req in stream:
+        delay = start + t - engine.now()
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(engine.submit(**req))
+    res = [engine.wait(i) for i in ids]
+    end = max(r["finished"] for r in res)
+    lat = np.array([r["finished"] - r["arrival"] for r in res])
+    ok = all(r["ok"] for r in res)
+    worst = max(max(r["true_rel_residual"]) for r in res) if ok else None
+    rhs = sum(len(r["true_rel_residual"]) for r in res)
+    return {"requests": len(res), "right_hand_sides": rhs, "all_ok": ok, "worst_true_rel_residual": worst,
+            "throughput_req_per_s": round(len(res) / (end - start), 3),
+            "throughput_rhs_per_s": round(rhs / (end - start), 3),
+            "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
+            "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
+            "merged_solves": sum(1 for r in res if r["batch_requests"] > 1),
+            "makespan_s": round(end - start, 3)}
+
+
+def serving_leg(seconds, rate, seed, n_baseline):
+    """Continuous-batching engine (4 workers / HIP streams, FP32-inner mixed solvers, HBM budget 160 GB)
+    on a Poisson request stream, against the plain FP64 solvers one request at a time on the stream's
+    first n_baseline requests (and the engine on those same requests)."""
+    from cl2qcd_amd import _host as H
+
+    def engine(**kw):
+        e = H.Engine(hbm_budget_bytes=160 << 30, **kw)
+        cfgs, k = {}, 0
+        for size in SERVE_SIZES:
+            cfgs[size] = []
+            for _ in range(SERVE_CONFIGS[size]):
+                cfgs[size].append(e.load_config(size[0], size[1], 5.7, 20, seed + k))
+                k += 1
+        e.warmup()
+        return e, cfgs
+
+    e, cfgs = engine(workers=4, mixed=True)
+    stream = serve_stream(seconds, rate, seed, cfgs)
+    log(f"serving: {len(stream)} requests over {seconds} s at {rate}/s ...")
+    full = serve_run(e, stream)
+    log(f"  engine: {full}")
+    first = serve_run(e, stream[:n_baseline])
+    del e
+    f, _ = engine(workers=1, mixed=False)
+    base = serve_run(f, stream[:n_baseline])
+    del f
+    log(f"  first {n_baseline}: engine {first}; FP64 one at a time {base}")
+    return {"stream": f"Poisson {rate}/s for {seconds} s, seed {seed}; sizes 16^3x32 / 24^3x48 / 32^3x64, "
+                      "twisted mass or staggered, 1-12 Gaussian RHS, |r|/|b| <= 1e-10; 8 resident heatbath "
+                      "beta 5.7 configurations",
+            "engine": full, "first_requests": {"n": n_baseline, "engine": first, "fp64_one_at_a_time": base}}
+
+
 def read_counters():
     """HBM bytes per launch of the FP64 dslash from the committed rocprofv3 counter summary."""
     if not os.path.exists(COUNTERS_FILE):
@@ -327,6 +413,10 @@ def main():
     ap.add_argument("--hb-sweeps", type=int, default=50, help="heatbath sweeps for the CG configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-source block solve at 32^3x64")
     ap.add_argument("--no-multishift", action="store_true", help="skip the 16-shift staggered solve at 48^3x12")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
+    ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the Poisson request stream")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="mean requests per second")
+    ap.add_argument("--serve-baseline", type=int, default=10, help="requests also run by the FP64 baseline")
     ap.add_argument("--hb-overrelax", type=int, default=1, help="overrelaxation sweeps per heatbath sweep")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -373,6 +463,10 @@ def main():
     if not args.no_multishift:
         multishift = multishift_leg(48, 12, args.hb_sweeps, 20261204)
 
+    serving = None
+    if not args.no_serving:
+        serving = serving_leg(args.serve_seconds, args.serve_rate, 20261205, args.serve_baseline)
+
     traffic = read_counters()
     achieved = f64["alg_bytes"] / (f64["ms"] * 1e6)
     line = {
@@ -401,6 +495,7 @@ def main():
         "cg": cg,
         "batch": batch,
         "multishift": multishift,
+        "serving": serving,
     }
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores) ...")
