@@ -9,7 +9,7 @@ Gaussian spinor.  `value` is algorithmic GB/s: compulsory bytes per application
 (8 links + input + output spinor per eo site) times steps over the timed
 region, measured with HIP events on the context stream.
 
-Also reported: the same for 12-real FP64 and FP32 links, the reference's own
+Also reported: the same for 12-real FP64, FP32 and FP16 (per-site scale) links, the reference's own
 byte convention (2880 B/site), the CPU oracle timed on host cores
 (cpu_baseline), and the HBM traffic per launch measured by rocprofv3 counters
 (read from profiles/ when the counter summary for this workload is there).
@@ -57,6 +57,9 @@ def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src):
     g = ctx.gauge(fmt=fmt, seed=20261202, epsilon=0.35)
     a, b = ctx.spinor(prec).upload(src), ctx.spinor(prec)
     kappa = 0.16
+    # launch shape from the autotuner (z-planes per slab, XCD mapping), timed on this field
+    tuned = [ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()]
+    ctx.check(ctx.lib.cl2_autotune_dslash(ctx.handle, g.ptr, fmt, 1, *[ctypes.byref(t) for t in tuned]))
 
     def step(k):
         if k % 2 == 0:
@@ -74,7 +77,8 @@ def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src):
     g.close()
     alg = ctx.lib.cl2_algorithmic_bytes(ctx.handle, b"dslash_eo", fmt)
     return dict(ms=ev_ms / steps, wall_ms=wall_ms / steps, alg_bytes=alg, GBps=alg * steps / (ev_ms * 1e6),
-                finite=finite)
+                finite=finite, tuned={"zplanes": tuned[1].value, "xcd": tuned[2].value, "block": abs(tuned[4].value),
+                                      "min_waves_per_simd": 4 if tuned[4].value < 0 else None})
 
 
 def cg_leg(ns, nt, sweeps, overrelax, seed):
@@ -442,11 +446,13 @@ def main():
     log(f"dslash F64/18: {f64['ms']:.4f} ms/app, {f64['GBps']:.0f} GB/s")
     extra = {}
     if not args.only_f64:
-        for name, fmt, prec in (("f64_12", abi.GAUGE_F64_12, abi.F64), ("f32_12", abi.GAUGE_F32_12, abi.F32)):
+        for name, fmt, prec in (("f64_12", abi.GAUGE_F64_12, abi.F64), ("f32_12", abi.GAUGE_F32_12, abi.F32),
+                                ("f16_12", abi.GAUGE_F16_12, abi.F16)):
             r = dslash_leg(ctx, abi, fmt, prec, max(args.steps // 4, 2), args.warmup, src)
             log(f"dslash {name}: {r['ms']:.4f} ms/app, {r['GBps']:.0f} GB/s")
             extra[f"dslash_{name}"] = dict(ms_per_app=round(r["ms"], 5), GBps=round(r["GBps"], 1),
-                                           pct_hbm_peak=round(100 * r["GBps"] / HBM_PEAK_GBPS, 1))
+                                           pct_hbm_peak=round(100 * r["GBps"] / HBM_PEAK_GBPS, 1),
+                                           launch_shape=r["tuned"])
     ref_bytes = ctx.lib.cl2_reference_read_write_size(ctx.handle, b"dslash_eo")
     veo = ctx.veo
     ctx.close()
@@ -489,6 +495,7 @@ def main():
         "wall_ms_per_step": round(f64["wall_ms"], 5),
         "cl2qcd_convention_GBps": round(ref_bytes / (f64["ms"] * 1e6), 1),
         "output_finite": f64["finite"],
+        "launch_shape": f64["tuned"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         **extra,

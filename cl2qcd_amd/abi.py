@@ -111,6 +111,15 @@ SIGNATURES = {
     "cl2_block_set_zero_eo": (_i, [_vp, _vp, _i, _i]),
     "cl2_block_cg_alpha": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
     "cl2_block_cg_beta": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
+    "cl2_autotune_dslash": (_i, [_vp, _vp, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                                 ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "cl2_graph_begin": (_i, [_vp]),
+    "cl2_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "cl2_graph_launch": (_i, [_vp, _vp, _i]),
+    "cl2_graph_destroy": (_i, [_vp]),
+    "cl2_cg_beta": (_i, [_vp, _dp, _dp, _dp]),
+    "cl2_scalar_copy": (_i, [_vp, _dp, _dp, _i]),
+    "cl2_slice_squarenorm_eo": (_i, [_vp, _dp, _vp, _i, _i, _i]),
     "cl2_gaussian_eo": (_i, [_vp, _i, _vp, _i, ctypes.c_uint64]),
     "cl2_multishift_update_stagg_eo": (_i, [_vp, ctypes.POINTER(_vp), _i, ctypes.POINTER(_vp), _vp, _i, _i, _dp, _dp,
                                             _dp, _dp]),

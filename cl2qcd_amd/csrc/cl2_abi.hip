@@ -189,8 +189,12 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.scale         = scale;
     a.norm_partials = nullptr;
     a.w             = wilson_weights(c, kappa);
-    a.map           = work_map(c, DSLASH_BLOCK, nrhs);
+    const cl2_hop_tuning t = hop_tuning(c, fmt, nrhs);
+    a.block         = (nrhs == 1 && fmt != CL2_GAUGE_F16_12) ? t.block : DSLASH_BLOCK;
+    a.occ           = t.occ;
+    a.map           = work_map(c, a.block, nrhs, t);
     a.nrhs          = nrhs;
+    a.rb            = t.rb;
     CL2_HIP(c, launch_dslash(c, a, fmt));
     return CL2_OK;
 }
@@ -816,6 +820,15 @@ int cl2_gaussian_eo(cl2_ctx* c, int kind, void* out, int prec, uint64_t seed)
     return CL2_OK;
 }
 
+int cl2_slice_squarenorm_eo(cl2_ctx* c, double* res, const void* x, int prec, int dir, int accumulate)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, res && x && valid_prec(prec), "bad argument");
+    CL2_REQUIRE(c, dir == 0 || dir == 3, "correlator direction must be 0 (t) or 3 (z)");
+    CL2_HIP(c, launch_slice_norm(c, res, x, prec, dir, accumulate ? 1 : 0));
+    return CL2_OK;
+}
+
 int cl2_set_zero_eo(cl2_ctx* c, void* out, int prec)
 {
     if (!c) return CL2_E_ARG;
@@ -1156,6 +1169,76 @@ int cl2_update_alpha_cgm(cl2_ctx* c, double* out, const double* salpha_pres, con
     return CL2_OK;
 }
 
+/* ---------------- captured launch sequences ---------------- */
+
+}  // extern "C"
+
+struct cl2_graph {
+    hipGraph_t graph    = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int cl2_graph_begin(cl2_ctx* c)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    return CL2_OK;
+}
+
+int cl2_graph_end(cl2_ctx* c, cl2_graph** out)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, out != nullptr, "null output pointer");
+    *out = nullptr;
+    hipGraph_t g = nullptr;
+    CL2_HIP(c, hipStreamEndCapture(c->stream, &g));
+    cl2_graph* h = new cl2_graph;
+    h->graph     = g;
+    if (hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        delete h;
+        return fail(c, CL2_E_HIP, "hipGraphInstantiate failed");
+    }
+    *out = h;
+    return CL2_OK;
+}
+
+int cl2_graph_launch(cl2_ctx* c, cl2_graph* g, int times)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, g != nullptr && times >= 0, "bad argument");
+    for (int k = 0; k < times; ++k) CL2_HIP(c, hipGraphLaunch(g->exec, c->stream));
+    return CL2_OK;
+}
+
+int cl2_graph_destroy(cl2_graph* g)
+{
+    if (!g) return CL2_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return CL2_OK;
+}
+
+int cl2_scalar_copy(cl2_ctx* c, double* dst, const double* src, int n)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, dst && src && n > 0 && n <= (1 << 20), "bad scalar copy");
+    if (dst == src) return CL2_OK;
+    CL2_HIP(c, launch_scalar_copy(c, dst, src, n));
+    return CL2_OK;
+}
+
+int cl2_cg_beta(cl2_ctx* c, double* mbeta, double* r2, const double* norms)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, mbeta && r2 && norms, "null pointer");
+    CL2_HIP(c, launch_cg_beta(c, mbeta, r2, norms));
+    return CL2_OK;
+}
+
 /* ---------------- tuning ---------------- */
 
 int cl2_tune_set(cl2_ctx* c, const char* key, long value)
@@ -1167,15 +1250,18 @@ int cl2_tune_set(cl2_ctx* c, const char* key, long value)
         CL2_REQUIRE(c, value == -1 || value == 0 || (value > 0 && c->g.ns % (uint32_t)value == 0),
                     "hop.zplanes must be -1, 0 or a divisor of ns");
         c->tune_zplanes = (int)value;
+        c->tune_manual  = true;
         return CL2_OK;
     }
     if (k == "hop.xcd") {
-        c->tune_xcd = value ? 1 : 0;
+        c->tune_xcd    = value ? 1 : 0;
+        c->tune_manual = true;
         return CL2_OK;
     }
     if (k == "mrhs.rb") {
         CL2_REQUIRE(c, value == 0 || value == 1 || value == 2 || value == 4, "mrhs.rb must be 0, 1, 2 or 4");
         c->tune_mrhs_rb = (int)value;
+        c->tune_manual  = true;
         return CL2_OK;
     }
     return fail(c, CL2_E_ARG, "unknown tuning key " + k);
@@ -1189,6 +1275,78 @@ long cl2_tune_get(const cl2_ctx* c, const char* key)
     if (k == "hop.xcd") return c->tune_xcd;
     if (k == "mrhs.rb") return c->tune_mrhs_rb;
     return -2;
+}
+
+/* Autotuner of the hopping kernel's launch shape: every candidate (z-planes per slab, XCD mapping,
+ * right-hand sides per thread) is timed with HIP events on scratch fields and the caller's gauge field;
+ * the fastest is stored for (format, nrhs) and used by every later launch on this context unless
+ * cl2_tune_set has fixed the shape by hand. */
+int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double* best_ms, int* zplanes, int* xcd,
+                        int* rb, int* block)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, gauge && valid_fmt(fmt), "bad gauge argument");
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && (nrhs == 1 || fmt != CL2_GAUGE_F16_12), "unsupported right-hand sides");
+    const int prec    = fmt_prec(fmt);
+    const size_t fb   = spinor_bytes(c, prec) * (size_t)nrhs;
+    TempDev in, out;
+    CL2_HIP(c, hipMalloc(&in.p, fb));
+    CL2_HIP(c, hipMalloc(&out.p, fb));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_fill_h(c, in.p, 0.1, 0.));
+    else
+        CL2_HIP(c, launch_fill(c, in.p, (size_t)12 * c->g.veo * nrhs, prec, 0.1, 0.));
+    std::vector<int> zps = {-1, 0};
+    for (uint32_t d = 1; d <= c->g.ns; ++d)
+        if (c->g.ns % d == 0 && (size_t)d * c->g.ns * c->g.ns / 2 * nrhs <= 262144) zps.push_back((int)d);
+    std::vector<int> rbs = {1};
+    if (nrhs > 1)
+        for (int r : {2, 4})
+            if (nrhs % r == 0) rbs.push_back(r);
+    // (block, occupancy hint) shapes of the single-field kernel
+    std::vector<std::pair<int, int>> shapes = {{256, 0}};
+    if (nrhs == 1 && fmt != CL2_GAUGE_F16_12) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 4}, {128, 4}};
+    const bool manual = c->tune_manual;
+    c->tune_manual    = false;
+    const double one[2] = {1., 0.};
+    double best         = 1e30;
+    cl2_hop_tuning bt{-1, 1, 1, DSLASH_BLOCK, 0};
+    int rc = CL2_OK;
+    for (int zp : zps)
+        for (int x : {1, 0})
+            for (int r : rbs)
+                for (const auto& sh : shapes) {
+                c->tuned[{fmt, nrhs}] = cl2_hop_tuning{zp, x, r, sh.first, sh.second};
+                for (int k = 0; k < 2 && rc == CL2_OK; ++k)
+                    rc = run_dslash(c, out.p, in.p, nullptr, gauge, fmt, CL2_EVEN, 0.125, EPI_PLAIN, one, one, 1., 0,
+                                    nrhs);
+                if (rc) break;
+                CL2_HIP(c, hipEventRecord(c->ev_start, c->stream));
+                for (int k = 0; k < 5 && rc == CL2_OK; ++k)
+                    rc = run_dslash(c, out.p, in.p, nullptr, gauge, fmt, k & 1 ? CL2_ODD : CL2_EVEN, 0.125, EPI_PLAIN,
+                                    one, one, 1., 0, nrhs);
+                if (rc) break;
+                CL2_HIP(c, hipEventRecord(c->ev_stop, c->stream));
+                CL2_HIP(c, hipEventSynchronize(c->ev_stop));
+                float ms = 0.f;
+                CL2_HIP(c, hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
+                if (ms / 5. < best) {
+                    best = ms / 5.;
+                    bt   = cl2_hop_tuning{zp, x, r, sh.first, sh.second};
+                }
+            }
+    c->tune_manual = manual;
+    if (rc) {
+        c->tuned.erase({fmt, nrhs});
+        return rc;
+    }
+    c->tuned[{fmt, nrhs}] = bt;
+    if (best_ms) *best_ms = best;
+    if (zplanes) *zplanes = bt.zplanes;
+    if (xcd) *xcd = bt.xcd;
+    if (rb) *rb = bt.rb;
+    if (block) *block = bt.occ ? -bt.block : bt.block;
+    return CL2_OK;
 }
 
 /* ---------------- timing ---------------- */

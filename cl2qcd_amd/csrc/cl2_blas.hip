@@ -191,6 +191,49 @@ __global__ __launch_bounds__(BLOCK) void k_fill(vec2<R>* __restrict__ out, size_
     for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < n; k += step) out[k] = vec2<R>{re, im};
 }
 
+/* per-slice |x|^2 of an eo spinor field.  The eo sites of slice s are nchunk contiguous runs of
+ * chunk sites, run j starting at s * slice_off + j * chunk_stride: the timeslice t is the one run
+ * [t * vs/2, (t+1) * vs/2); the z-slice z is, for every t, the run of the NS^2/2 eo sites of the
+ * (x, y) planes with that z (NS is even, so both sites of an eo pair share z).  Block s reduces
+ * its runs in a fixed order. */
+template<typename R>
+__global__ __launch_bounds__(256) void k_slice_norm(double* res, const vec2<R>* __restrict__ x, uint32_t S,
+                                                    uint32_t slice_off, uint32_t chunk, uint32_t chunk_stride,
+                                                    uint32_t nchunk, int accumulate)
+{
+    __shared__ double lds[8];
+    const uint32_t s = blockIdx.x;
+    double acc       = 0.;
+    for (uint32_t i = threadIdx.x; i < nchunk * chunk; i += 256) {
+        const uint32_t e = s * slice_off + (i / chunk) * chunk_stride + i % chunk;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const vec2<R> v = x[(size_t)k * S + e];
+            acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+    }
+    const double sum = block_sum<256>(acc, lds);
+    if (threadIdx.x == 0) res[s] = accumulate ? res[s] + sum : sum;
+}
+
+hipError_t launch_slice_norm(cl2_ctx* c, double* res, const void* x, int prec, int dir, int accumulate)
+{
+    const uint32_t half_vs = c->g.vs / 2, half_plane = c->g.ns * c->g.ns / 2;
+    const uint32_t nslice       = dir == 0 ? c->g.nt : c->g.ns;
+    const uint32_t slice_off    = dir == 0 ? half_vs : half_plane;
+    const uint32_t chunk        = dir == 0 ? half_vs : half_plane;
+    const uint32_t nchunk       = dir == 0 ? 1 : c->g.nt;
+    if (prec == CL2_F64)
+        hipLaunchKernelGGL(k_slice_norm<double>, dim3(nslice), dim3(256), 0, c->stream, res,
+                           reinterpret_cast<const double2*>(x), c->g.veo, slice_off, chunk, half_vs, nchunk,
+                           accumulate);
+    else
+        hipLaunchKernelGGL(k_slice_norm<float>, dim3(nslice), dim3(256), 0, c->stream, res,
+                           reinterpret_cast<const float2*>(x), c->g.veo, slice_off, chunk, half_vs, nchunk,
+                           accumulate);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im)
 {
     const unsigned nb = reduce_blocks(n);

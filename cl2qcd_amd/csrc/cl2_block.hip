@@ -150,6 +150,39 @@ __global__ void k_block_cg_beta(double* beta, double* r2, const double* norms, c
 
 /* grid of a block kernel: with per-right-hand-side partials at most REDUCE_MAX_BLOCKS blocks (and what the
  * partials buffer holds), else enough blocks to stream without a grid-stride tail */
+/* single-RHS CG: mbeta = -(Polak-Ribiere beta) as a complex scalar, then r2 = (|r_new|^2, 0);
+ * norms = (|r_new|^2, Re <r_new, r_old>) */
+__global__ void k_cg_beta(double* mbeta, double* r2, const double* norms)
+{
+    if (threadIdx.x != 0) return;
+    const double rn = norms[0], ry = norms[1];
+    double b        = (rn - ry) / r2[0];
+    if (!(b >= 0.)) b = rn / r2[0];
+    mbeta[0] = -b;
+    mbeta[1] = 0.;
+    r2[0]    = rn;
+    r2[1]    = 0.;
+}
+
+hipError_t launch_cg_beta(cl2_ctx* c, double* mbeta, double* r2, const double* norms)
+{
+    hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(64), 0, c->stream, mbeta, r2, norms);
+    return hipGetLastError();
+}
+
+/* copy of a few device doubles as a kernel (so it can sit in a recorded iteration) */
+__global__ void k_scalar_copy(double* dst, const double* src, int n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
+hipError_t launch_scalar_copy(cl2_ctx* c, double* dst, const double* src, int n)
+{
+    hipLaunchKernelGGL(k_scalar_copy, dim3((n + 63) / 64), dim3(64), 0, c->stream, dst, src, n);
+    return hipGetLastError();
+}
+
 static unsigned grid_for(cl2_ctx* c, size_t n, int nrhs, bool partials)
 {
     const unsigned bt = block_threads(nrhs);

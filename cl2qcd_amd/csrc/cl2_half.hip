@@ -2,22 +2,25 @@
 // half precision with one FP32 scale per site, and 12-real links in half
 // precision.  All arithmetic is FP32; reductions accumulate in FP64.
 //
-// Spinor layout (52 bytes per eo site): 6 lanes of 4 halves, lane j holding
-// complex components 2j and 2j+1 (component = 3*spin + colour) as (re, im, re,
-// im), stored [6][Veo]; then float norm[Veo].  The value of a component is
+// Spinor layout (52 bytes per eo site): 3 lanes of 8 halves (16 bytes), lane j
+// holding complex components 4j..4j+3 (component = 3*spin + colour) as re, im
+// pairs, stored [3][Veo]; then float norm[Veo].  The value of a component is
 // half * norm, with norm the largest |re| or |im| on the site, so the stored
 // halves lie in [-1, 1] and keep 11 significant bits relative to the site's
-// largest entry.
+// largest entry.  16-byte lanes keep every wavefront load a 1 KiB segment.
 //
-// Link layout (24 bytes per link): [dir][parity][3][Veo] of 4 halves, lane k
-// holding entries 2k, 2k+1 of rows 0 and 1; row 2 = conj(row0 x row1).
+// Link layout (24 bytes per link): for each (dir, parity) a [Veo] array of
+// 16-byte lanes holding entries 0..3 of rows 0,1 (row-major) followed by a [Veo]
+// array of 8-byte lanes holding entries 4, 5; row 2 = conj(row0 x row1).
 #include "cl2_device.hpp"
 #include "cl2_internal.hpp"
 
 namespace cl2 {
 
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
+typedef float f8v __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ const float* half_norms(const void* p, uint32_t S)
 {
@@ -30,13 +33,13 @@ __device__ __forceinline__ float* half_norms(void* p, uint32_t S)
 
 __device__ __forceinline__ void load_site_h(const void* p, uint32_t S, uint32_t i, Cx<float> (&v)[12])
 {
-    const h4* l   = reinterpret_cast<const h4*>(p);
+    const h8* l   = reinterpret_cast<const h8*>(p);
     const float n = half_norms(p, S)[i];
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        const f4v f  = __builtin_convertvector(l[(size_t)j * S + i], f4v);
-        v[2 * j]     = Cx<float>{f.x * n, f.y * n};
-        v[2 * j + 1] = Cx<float>{f.z * n, f.w * n};
+    for (int j = 0; j < 3; ++j) {
+        const f8v f = __builtin_convertvector(l[(size_t)j * S + i], f8v);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[4 * j + q] = Cx<float>{f[2 * q] * n, f[2 * q + 1] * n};
     }
 }
 
@@ -47,24 +50,32 @@ __device__ __forceinline__ void store_site_h(void* p, uint32_t S, uint32_t i, co
 #pragma unroll
     for (int k = 0; k < 12; ++k) m = fmaxf(m, fmaxf(fabsf(v[k].re), fabsf(v[k].im)));
     const float inv = m > 0.f ? 1.f / m : 0.f;
-    h4* l           = reinterpret_cast<h4*>(p);
+    h8* l           = reinterpret_cast<h8*>(p);
 #pragma unroll
-    for (int j = 0; j < 6; ++j) {
-        f4v f{v[2 * j].re * inv, v[2 * j].im * inv, v[2 * j + 1].re * inv, v[2 * j + 1].im * inv};
-        l[(size_t)j * S + i] = __builtin_convertvector(f, h4);
+    for (int j = 0; j < 3; ++j) {
+        f8v f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f[2 * q]     = v[4 * j + q].re * inv;
+            f[2 * q + 1] = v[4 * j + q].im * inv;
+        }
+        l[(size_t)j * S + i] = __builtin_convertvector(f, h8);
     }
     half_norms(p, S)[i] = m;
 }
 
 __device__ __forceinline__ void load_link_h(const void* g, uint32_t S, int dir, int par, uint32_t eo, Su3<float>& u)
 {
-    const h4* p = reinterpret_cast<const h4*>(g) + (size_t)((dir * 2 + par) * 3) * S + eo;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const f4v f                           = __builtin_convertvector(__builtin_nontemporal_load(p + (size_t)k * S), f4v);
-        u.e[(2 * k) / 3][(2 * k) % 3]         = Cx<float>{f.x, f.y};
-        u.e[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<float>{f.z, f.w};
-    }
+    const char* base = static_cast<const char*>(g) + (size_t)(dir * 2 + par) * S * 24;
+    const f8v a      = __builtin_convertvector(__builtin_nontemporal_load(reinterpret_cast<const h8*>(base) + eo), f8v);
+    const f4v b      = __builtin_convertvector(
+        __builtin_nontemporal_load(reinterpret_cast<const h4*>(base + (size_t)S * 16) + eo), f4v);
+    u.e[0][0] = Cx<float>{a[0], a[1]};
+    u.e[0][1] = Cx<float>{a[2], a[3]};
+    u.e[0][2] = Cx<float>{a[4], a[5]};
+    u.e[1][0] = Cx<float>{a[6], a[7]};
+    u.e[1][1] = Cx<float>{b.x, b.y};
+    u.e[1][2] = Cx<float>{b.z, b.w};
     su3_reconstruct_row2(u);
 }
 

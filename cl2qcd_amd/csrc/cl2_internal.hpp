@@ -3,10 +3,21 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "cl2_types.hpp"
+
+/* launch shape of the hopping kernels */
+struct cl2_hop_tuning {
+    int zplanes;  // z-planes per slab (-1: heuristic, 0: identity order)
+    int xcd;      // XCD-contiguous block assignment
+    int rb;       // right-hand sides per thread of the multi-RHS kernel
+    int block;    // threads per block of the single-field kernel
+    int occ;      // occupancy hint of the single-field kernel (0 or 4 waves per SIMD)
+};
 
 struct cl2_ctx {
     int device = 0;
@@ -24,6 +35,8 @@ struct cl2_ctx {
     int tune_zplanes = -1;  // z-planes per slab of the hopping kernels (-1: heuristic, 0: identity order)
     int tune_xcd     = 1;   // XCD-contiguous block assignment
     int tune_mrhs_rb = 0;   // right-hand sides per thread of the multi-RHS hopping kernel (0: by format)
+    bool tune_manual = false;  // set by cl2_tune_set: the values above override autotuned ones
+    std::map<std::pair<int, int>, cl2_hop_tuning> tuned;  // (link format, right-hand sides) -> autotuned shape
     std::string err;
 };
 
@@ -38,8 +51,11 @@ inline unsigned blocks_for(size_t n, int block)
     return (unsigned)((n + block - 1) / block);
 }
 
-/* work order of the hopping kernels for this context's lattice and tuning */
-WorkMap work_map(const cl2_ctx* c, int block, int nrhs = 1);
+/* launch shape for (format, right-hand sides): manual settings, else the autotuned entry, else the
+ * measured defaults */
+cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs);
+/* work order of the hopping kernels for this context's lattice and a launch shape */
+WorkMap work_map(const cl2_ctx* c, int block, int nrhs, const cl2_hop_tuning& t);
 /* right-hand-side counts with a compiled multi-RHS hopping kernel */
 bool supported_nrhs(int n);
 
@@ -93,6 +109,7 @@ hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src
 hipError_t launch_gaussian(cl2_ctx* c, void* dev, int prec, size_t n, uint64_t seed);
 
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im);
+hipError_t launch_slice_norm(cl2_ctx* c, double* res, const void* x, int prec, int dir, int accumulate);
 
 /* FP16 Wilson spinors (per-site scale) and F16_12 links */
 hipError_t launch_dslash_h(cl2_ctx* c, const DslashArgs& a);
@@ -109,6 +126,8 @@ hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y
 hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode);
 hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, void* single_out, int prec, int nrhs,
                              int r);
+hipError_t launch_scalar_copy(cl2_ctx* c, double* dst, const double* src, int n);
+hipError_t launch_cg_beta(cl2_ctx* c, double* mbeta, double* r2, const double* norms);
 hipError_t launch_block_cg_scalars(cl2_ctx* c, int which, double* out, double* r2, const double* in,
                                    const double* active, int nrhs);
 

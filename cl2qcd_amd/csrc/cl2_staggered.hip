@@ -74,7 +74,7 @@ hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, 
     a.gauge      = gauge;
     a.g          = c->g;
     a.out_parity = evenodd;
-    a.map        = work_map(c, DSLASH_BLOCK);
+    a.map        = work_map(c, DSLASH_BLOCK, 1, hop_tuning(c, format | 0x100, 1));
     for (int mu = 0; mu < 4; ++mu) {
         const double* b = (mu == 0) ? c->bt : c->bs;
         double re = 0.5 * b[0], im = 0.5 * b[1];

@@ -64,6 +64,9 @@ struct DslashArgs {
     HopWeights<double> w;
     WorkMap map;
     int nrhs;  // right-hand sides per launch (fields [12][Veo][nrhs]); 1 for single fields
+    int rb;    // right-hand sides per thread of the multi-RHS kernel
+    int block; // threads per block of the single-field kernel (128, 256 or 512)
+    int occ;   // 4: single-field kernel compiled for at least 4 waves per SIMD (<= 128 VGPRs), else 0
 };
 
 }  // namespace cl2

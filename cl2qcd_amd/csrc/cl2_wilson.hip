@@ -68,8 +68,8 @@ __device__ __forceinline__ void hop(Cx<R> (&acc)[12], const vec2<R>* __restrict_
 /* NRHS right-hand sides in one launch: the fields are [12][Veo][NRHS] (right-hand side fastest) and
  * thread l works on site l / NRHS, right-hand side l % NRHS, so the NRHS lanes of one site issue the same
  * link addresses and each link leaves HBM once per launch.  NRHS = 1 is the single-field kernel. */
-template<int FMT, int BLOCK, int NRHS>
-__global__ __launch_bounds__(BLOCK) void k_dslash_eo(DslashArgs a)
+template<int FMT, int BLOCK, int NRHS, int MINW = 1>
+__global__ __launch_bounds__(BLOCK, MINW) void k_dslash_eo(DslashArgs a)
 {
     using R = typename GaugeLoad<FMT>::R;
     using V = vec2<R>;
@@ -253,11 +253,20 @@ __global__ __launch_bounds__(BLOCK) void k_sitediag(vec2<R>* __restrict__ out, c
     }
 }
 
-WorkMap work_map(const cl2_ctx* c, int block, int nrhs)
+cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs)
+{
+    if (!c->tune_manual) {
+        auto it = c->tuned.find({fmt, nrhs});
+        if (it != c->tuned.end()) return it->second;
+    }
+    return cl2_hop_tuning{c->tune_zplanes, c->tune_xcd, c->tune_mrhs_rb, DSLASH_BLOCK, 0};
+}
+
+WorkMap work_map(const cl2_ctx* c, int block, int nrhs, const cl2_hop_tuning& t)
 {
     WorkMap m{0u, 0u, c->g.vs / 2, 0u};
     const uint32_t ns = c->g.ns, plane = ns * ns / 2;
-    int zp            = c->tune_zplanes;
+    int zp            = t.zplanes;
     if (zp < 0) {
         // one z-chunk per XCD when ns divides by 8 (measured best at 48^3x96:
         // 8 chunks of 6 planes); else the largest divisor of ns whose slab
@@ -275,7 +284,7 @@ WorkMap work_map(const cl2_ctx* c, int block, int nrhs)
         m.col = m.P * c->g.nt;
     }
     const uint32_t nblocks = (uint32_t)(((size_t)c->g.veo * nrhs + block - 1) / block);
-    if (c->tune_xcd && nblocks >= 64) m.xcd_q = nblocks / 8;
+    if (t.xcd && nblocks >= 64) m.xcd_q = nblocks / 8;
     return m;
 }
 
@@ -315,9 +324,9 @@ static hipError_t launch_rb(cl2_ctx* c, const DslashArgs& a)
  * thread per site and right-hand side, links shared across the lanes of a site), which measured fastest on
  * gfx950 for every format: RB > 1 keeps RB x 12 accumulators live and drops to one wave per SIMD */
 template<int FMT, int NRHS>
-static int rhs_per_thread(const cl2_ctx* c)
+static int rhs_per_thread(const DslashArgs& a)
 {
-    const int want = c->tune_mrhs_rb > 0 ? c->tune_mrhs_rb : 1;
+    const int want = a.rb > 0 ? a.rb : 1;
     for (int rb : {want, 4, 2, 1})
         if (rb <= 4 && rb != 3 && NRHS % rb == 0) return rb;
     return 1;
@@ -327,28 +336,33 @@ template<int NRHS>
 static hipError_t launch_dslash_n(cl2_ctx* c, const DslashArgs& a, int format)
 {
     if constexpr (NRHS == 1) {
-        const unsigned nb = blocks_for((size_t)a.g.veo, DSLASH_BLOCK);
+        if (format == GAUGE_F16_12) return launch_dslash_h(c, a);
+        auto by_shape = [&](auto fmt_tag) -> hipError_t {
+            constexpr int F = decltype(fmt_tag)::value;
+            const unsigned bs = (unsigned)a.block;
+            const unsigned nb = blocks_for((size_t)a.g.veo, (int)bs);
+            if (bs == 128 && a.occ == 4)
+                hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 4>), dim3(nb), dim3(128), 0, c->stream, a);
+            else if (bs == 128)
+                hipLaunchKernelGGL((k_dslash_eo<F, 128, 1>), dim3(nb), dim3(128), 0, c->stream, a);
+            else if (bs == 512)
+                hipLaunchKernelGGL((k_dslash_eo<F, 512, 1>), dim3(nb), dim3(512), 0, c->stream, a);
+            else if (a.occ == 4)
+                hipLaunchKernelGGL((k_dslash_eo<F, 256, 1, 4>), dim3(nb), dim3(256), 0, c->stream, a);
+            else
+                hipLaunchKernelGGL((k_dslash_eo<F, 256, 1>), dim3(nb), dim3(256), 0, c->stream, a);
+            return hipGetLastError();
+        };
         switch (format) {
-            case GAUGE_F64_18:
-                hipLaunchKernelGGL((k_dslash_eo<GAUGE_F64_18, DSLASH_BLOCK, 1>), dim3(nb), dim3(DSLASH_BLOCK), 0,
-                                   c->stream, a);
-                break;
-            case GAUGE_F64_12:
-                hipLaunchKernelGGL((k_dslash_eo<GAUGE_F64_12, DSLASH_BLOCK, 1>), dim3(nb), dim3(DSLASH_BLOCK), 0,
-                                   c->stream, a);
-                break;
-            case GAUGE_F32_12:
-                hipLaunchKernelGGL((k_dslash_eo<GAUGE_F32_12, DSLASH_BLOCK, 1>), dim3(nb), dim3(DSLASH_BLOCK), 0,
-                                   c->stream, a);
-                break;
-            case GAUGE_F16_12: return launch_dslash_h(c, a);
+            case GAUGE_F64_18: return by_shape(std::integral_constant<int, GAUGE_F64_18>{});
+            case GAUGE_F64_12: return by_shape(std::integral_constant<int, GAUGE_F64_12>{});
+            case GAUGE_F32_12: return by_shape(std::integral_constant<int, GAUGE_F32_12>{});
             default: return hipErrorInvalidValue;
         }
-        return hipGetLastError();
     } else {
         auto by_rb = [&](auto fmt_tag) -> hipError_t {
             constexpr int F = decltype(fmt_tag)::value;
-            switch (rhs_per_thread<F, NRHS>(c)) {
+            switch (rhs_per_thread<F, NRHS>(a)) {
                 case 4:
                     if constexpr (NRHS % 4 == 0) return launch_rb<F, NRHS, 4>(c, a);
                     return hipErrorInvalidValue;

@@ -22,6 +22,12 @@ class Inputparameters {
     bool get_use_rec12() const noexcept { return use_rec12; }
     common::startcondition get_startcondition() const noexcept { return startcondition; }
     const std::string& get_sourcefile() const noexcept { return initial_conf; }
+    int get_num_sources() const noexcept { return num_sources; }
+    int get_source_x() const noexcept { return source_x; }
+    int get_source_y() const noexcept { return source_y; }
+    int get_source_z() const noexcept { return source_z; }
+    int get_source_t() const noexcept { return source_t; }
+    int get_corr_dir() const noexcept { return corr_dir; }
     uint32_t get_host_seed() const noexcept { return host_seed; }
 
     common::action get_fermact() const noexcept { return fermact; }
@@ -54,6 +60,8 @@ class Inputparameters {
     bool use_rec12 = false;
     common::startcondition startcondition = common::cold_start;
     std::string initial_conf;
+    int num_sources = 12, source_x = 0, source_y = 0, source_z = 0, source_t = 0;
+    int corr_dir = 3;  // meta/parametersObs.cpp:72: z by default
     uint32_t host_seed = 4815;
     common::action fermact = common::wilson;
     double beta = 4.0;

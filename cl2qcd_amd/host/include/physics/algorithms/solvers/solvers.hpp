@@ -65,6 +65,7 @@ struct ReliableUpdateOptions {
     int inner_precision = CL2_F32;  // precision of the Dslash and Krylov vectors of the inner loop
     double delta        = 0.1;      // FP64 reliable update when |r| drops by this factor
     int max_iterations  = 100000;
+    bool use_graph      = true;     // record one inner iteration and replay it (same arithmetic either way)
 };
 
 struct SolveStats {
@@ -98,6 +99,7 @@ struct MultishiftOptions {
     double sloppy_tol   = 1e-7;     // relative residual of each shift at which the multi-shift phase stops
     double delta        = 0.1;      // reliable-update factor of the refinement
     int max_iterations  = 100000;
+    bool use_graph      = true;     // record one CG-M iteration and replay it (same arithmetic either way)
 };
 
 /* Multi-shift CG for (A + sigma_k) x_k = b in two phases: the reference's CG-M recurrences

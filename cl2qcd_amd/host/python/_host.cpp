@@ -8,6 +8,7 @@
 
 #include <cstring>
 
+#include "../include/physics/algorithms/inversion.hpp"
 #include "../include/physics/algorithms/solvers/solvers.hpp"
 #include "../include/physics/algorithms/su3heatbath.hpp"
 #include "../include/physics/observables/gaugeObservables.hpp"
@@ -207,6 +208,9 @@ PYBIND11_MODULE(_host, m)
                                      const Spinorfield_eo& y, const Scalar<hmc_complex>& r) {
         saxpy_AND_squarenorm(&out, a, x, y, r);
     });
+    m.def("convert_from_eoprec", [](const Spinorfield& out, const Spinorfield_eo& e, const Spinorfield_eo& o) {
+        convert_from_eoprec(&out, e, o);
+    });
     m.def("convert_to_eoprec", [](const Spinorfield_eo& e, const Spinorfield_eo& o, const Spinorfield& in) {
         convert_to_eoprec(&e, &o, in);
     });
@@ -306,11 +310,13 @@ PYBIND11_MODULE(_host, m)
     m.def(
         "cg_reliable_update",
         [](const Spinorfield_eo& x, const fm::Fermionmatrix_eo& A, const Gaugefield& gf, const Spinorfield_eo& b,
-           Lattice& l, hmc_float rel_tol, const AdditionalParameters& ap, int inner_prec, double delta, int maxit) {
+           Lattice& l, hmc_float rel_tol, const AdditionalParameters& ap, int inner_prec, double delta, int maxit,
+           bool use_graph) {
             sv::ReliableUpdateOptions o;
             o.inner_precision = inner_prec;
             o.delta           = delta;
             o.max_iterations  = maxit;
+            o.use_graph       = use_graph;
             sv::SolveStats st;
             {
                 py::gil_scoped_release rel;
@@ -323,16 +329,18 @@ PYBIND11_MODULE(_host, m)
             return d;
         },
         py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("b"), py::arg("lattice"), py::arg("rel_tol"), py::arg("ap"),
-        py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000);
+        py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000,
+        py::arg("use_graph") = true);
     m.def(
         "cg_reliable_update_block",
         [](std::vector<Spinorfield_eo*> xs, const fm::Fermionmatrix_eo& A, const Gaugefield& gf,
            std::vector<Spinorfield_eo*> bs, Lattice& l, hmc_float rel_tol, const AdditionalParameters& ap,
-           int inner_prec, double delta, int maxit) {
+           int inner_prec, double delta, int maxit, bool use_graph) {
             sv::ReliableUpdateOptions o;
             o.inner_precision = inner_prec;
             o.delta           = delta;
             o.max_iterations  = maxit;
+            o.use_graph       = use_graph;
             std::vector<const Spinorfield_eo*> x(xs.begin(), xs.end()), b(bs.begin(), bs.end());
             std::vector<sv::SolveStats> st;
             int it;
@@ -351,18 +359,39 @@ PYBIND11_MODULE(_host, m)
             return py::make_tuple(it, per);
         },
         py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("b"), py::arg("lattice"), py::arg("rel_tol"), py::arg("ap"),
-        py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000);
+        py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000,
+        py::arg("use_graph") = true);
+    m.def("point_source_ps_correlator", [](const Gaugefield& gf, Lattice& l) {
+        py::gil_scoped_release rel;
+        return physics::algorithms::point_source_ps_correlator(gf, l.system, l.handler, l.handler.getAdditionalParameters<Spinorfield_eo>());
+    });
+    m.def(
+        "invert_eo",
+        [](const Spinorfield_eo& pe, const Spinorfield_eo& po, const Gaugefield& gf, const Spinorfield_eo& ee,
+           const Spinorfield_eo& eo, Lattice& l, hmc_float prec) {
+            py::gil_scoped_release rel;
+            return physics::algorithms::invert_eo(&pe, &po, gf, ee, eo, l.system, l.handler, prec,
+                                                  l.handler.getAdditionalParameters<Spinorfield_eo>());
+        });
+    m.def(
+        "autotune_dslash",
+        [](const Gaugefield& gf, Lattice& l, int prec, int nrhs) {
+            py::gil_scoped_release rel;
+            return fm::autotune_dslash(gf, l.system, prec, nrhs);
+        },
+        py::arg("gf"), py::arg("lattice"), py::arg("prec") = CL2_F64, py::arg("nrhs") = 1);
     m.def("convert_precision",
           [](const Spinorfield_eo& to, const Spinorfield_eo& from) { convert_precision(&to, from); });
     m.def(
         "cg_reliable_update_stagg",
         [](const Staggeredfield_eo& x, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf, hmc_float sigma,
            const Staggeredfield_eo& b, Lattice& l, hmc_float rel_tol, const AdditionalParameters& ap, int inner_prec,
-           double delta, int maxit) {
+           double delta, int maxit, bool use_graph) {
             sv::ReliableUpdateOptions o;
             o.inner_precision = inner_prec;
             o.delta           = delta;
             o.max_iterations  = maxit;
+            o.use_graph       = use_graph;
             sv::SolveStats st;
             {
                 py::gil_scoped_release rel;
@@ -376,12 +405,13 @@ PYBIND11_MODULE(_host, m)
         },
         py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("sigma"), py::arg("b"), py::arg("lattice"),
         py::arg("rel_tol"), py::arg("ap"), py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1,
-        py::arg("max_iterations") = 100000);
+        py::arg("max_iterations") = 100000, py::arg("use_graph") = true);
     m.def(
         "cg_m_mixed",
         [](std::vector<Staggeredfield_eo*> xs, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf,
            std::vector<hmc_float> sigma, const Staggeredfield_eo& b, Lattice& l, hmc_float rel_tol,
-           const AdditionalParameters& ap, int inner_prec, double sloppy_tol, double delta, int maxit) {
+           const AdditionalParameters& ap, int inner_prec, double sloppy_tol, double delta, int maxit,
+           bool use_graph) {
             std::vector<std::shared_ptr<Staggeredfield_eo>> x;
             for (auto* p : xs) x.emplace_back(p, [](Staggeredfield_eo*) {});  // Python owns the fields
             sv::MultishiftOptions o;
@@ -389,6 +419,7 @@ PYBIND11_MODULE(_host, m)
             o.sloppy_tol      = sloppy_tol;
             o.delta           = delta;
             o.max_iterations  = maxit;
+            o.use_graph       = use_graph;
             std::vector<sv::SolveStats> st;
             int total = 0, ms = 0;
             {
@@ -407,7 +438,7 @@ PYBIND11_MODULE(_host, m)
         },
         py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("sigma"), py::arg("b"), py::arg("lattice"),
         py::arg("rel_tol"), py::arg("ap"), py::arg("inner_prec") = CL2_F32, py::arg("sloppy_tol") = 1e-7,
-        py::arg("delta") = 0.1, py::arg("max_iterations") = 100000);
+        py::arg("delta") = 0.1, py::arg("max_iterations") = 100000, py::arg("use_graph") = true);
     m.def("cg_m", [](std::vector<Staggeredfield_eo*> xs, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf,
                      std::vector<hmc_float> sigma, const Staggeredfield_eo& b, Lattice& l, hmc_float prec,
                      const AdditionalParameters& ap) {

@@ -179,6 +179,14 @@ cl_ulong BuiltinEoOperator::get_read_write_size() const
     }
 }
 
+double autotune_dslash(const Gaugefield& gf, const hardware::System& system, int prec, int nrhs)
+{
+    const Links l = links_for(gf, prec);
+    double ms     = 0.;
+    system.check(cl2_autotune_dslash(system.ctx(), l.ptr, l.fmt, nrhs, &ms, nullptr, nullptr, nullptr, nullptr));
+    return ms;
+}
+
 void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* out, const Gaugefield& gf,
                  const lattices::Spinorfield_eo_block& in, const physics::AdditionalParameters& ap,
                  const lattices::DeviceBuffer& tmp)

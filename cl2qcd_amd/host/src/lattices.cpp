@@ -677,17 +677,17 @@ CL2_SCALAR_OP(divide, CL2_SCALAR_OP_RATIO, hmc_float, cl2_real_op)
 
 void copyData(const Scalar<hmc_complex>* to, const Scalar<hmc_complex>& from)
 {
-    to->get_system().check(cl2_copy(ctx(to->get_system()), to->dev(), from.dev(), sizeof(hmc_complex)));
+    to->get_system().check(cl2_scalar_copy(ctx(to->get_system()), to->dev(), from.dev(), 2));
 }
 void copyData(const Scalar<hmc_float>* to, const Scalar<hmc_float>& from)
 {
-    to->get_system().check(cl2_copy(ctx(to->get_system()), to->dev(), from.dev(), sizeof(hmc_float)));
+    to->get_system().check(cl2_scalar_copy(ctx(to->get_system()), to->dev(), from.dev(), 1));
 }
 void copyData(const Vector<hmc_float>* to, const Vector<hmc_float>& from)
 {
     if (to->get_vector_size() != from.get_vector_size()) throw std::invalid_argument("vector sizes differ");
     to->get_system().check(
-        cl2_copy(ctx(to->get_system()), to->dev(), from.dev(), from.get_vector_size() * sizeof(hmc_float)));
+        cl2_scalar_copy(ctx(to->get_system()), to->dev(), from.dev(), (int)from.get_vector_size()));
 }
 void access_real_vector_element(const Scalar<hmc_float>* out, const Vector<hmc_float>& in, const int index)
 {

@@ -82,6 +82,23 @@ void Inputparameters::parse(const std::vector<std::string>& args)
                  throw std::invalid_argument("Unkown fermion action! wilson, twistedmass, rooted_stagg: " + v);
          }},
         {"beta", [&](const std::string& v) { beta = parse_double(v); }},
+        {"nSources", [&](const std::string& v) { num_sources = parse_int(v); }},
+        {"sourceX", [&](const std::string& v) { source_x = parse_int(v); }},
+        {"sourceY", [&](const std::string& v) { source_y = parse_int(v); }},
+        {"sourceZ", [&](const std::string& v) { source_z = parse_int(v); }},
+        {"sourceT", [&](const std::string& v) { source_t = parse_int(v); }},
+        {"sourceType", [&](const std::string& v) {
+             if (v != "point") throw std::invalid_argument("only point sources are supported: " + v);
+         }},
+        {"sourceContent", [&](const std::string& v) {
+             if (v != "one") throw std::invalid_argument("only the source content 'one' is supported: " + v);
+         }},
+        {"measureCorrelators", [&](const std::string& v) { parse_bool(v); }},
+        {"correlatorDirection", [&](const std::string& v) {
+             corr_dir = parse_int(v);
+             if (corr_dir != 0 && corr_dir != 3)
+                 throw std::invalid_argument("correlatorDirection must be 0 (t) or 3 (z): " + v);
+         }},
         {"kappa", [&](const std::string& v) { kappa = parse_double(v); }},
         {"mu", [&](const std::string& v) { mu = parse_double(v); }},
         {"mass", [&](const std::string& v) { mass = parse_double(v); }},

@@ -320,8 +320,11 @@ namespace solvers {
  * Two standard measures keep the low-precision recurrence close to exact CG:
  * beta is the Polak-Ribiere value <r_new, r_new - r_old> / |r_old|^2 (from the
  * fused residual update; the Fletcher-Reeves |r_new|^2 / |r_old|^2 when that is
- * negative), and after every reliable update the search direction is made
- * orthogonal to the new residual again, p -= (<r, p> / |r|^2) r. */
+ * negative), and at every reliable update the search direction is rebuilt on the
+ * true residual: p = r_true + d with d = beta p_old minus its component along
+ * r_true, so that <r_true, beta p_old> = 0 holds as in exact CG.  One inner
+ * iteration (operator, alpha, x and r updates, beta, p update) is a fixed kernel
+ * sequence with every scalar on the device; it is recorded once and replayed. */
 int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatrix_eo& A, const Gaugefield& gf,
                        const Spinorfield_eo& b, const hardware::System& system, hmc_float rel_tol,
                        const physics::AdditionalParameters& ap, const ReliableUpdateOptions& opt, SolveStats* stats)
@@ -332,10 +335,12 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
     const int lo = opt.inner_precision;
     const Spinorfield_eo r(system, CL2_F64), t64(system, CL2_F64);
     const Spinorfield_eo r_lo(system, lo), p(system, lo), v(system, lo), x_lo(system, lo);
-    const Scalar<hmc_complex> alpha(system), malpha(system), pAp(system), r2true(system), minus_one(system);
+    const Scalar<hmc_complex> alpha(system), malpha(system), pAp(system), r2c(system), mbeta(system),
+        minus_one(system);
     const Vector<hmc_float> norms(4, system);  // (|r|^2, Re <r_new, r_old>, |r|^2, 0) of the last update
     minus_one.store(hmc_complex_minusone);
     const hmc_complex one{1., 0.};
+    cl2_ctx* c = system.ctx();
 
     const hmc_float b2     = squarenorm(b);
     const hmc_float target = rel_tol * rel_tol * b2;
@@ -354,23 +359,46 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
     convert_precision(&r_lo, r);
     copyData(&p, r_lo);
     x_lo.zero();
-    r2true.store(hmc_complex{r2, 0.});
-    const double* r2c = r2true.dev();  // |r|^2 of the current step as a complex device scalar
-    hmc_float rmax    = std::sqrt(r2);
-    int k             = 0;
-    for (k = 0; k < opt.max_iterations; ++k) {
+    r2c.store(hmc_complex{r2, 0.});  // |r|^2 of the current step as a complex device scalar
+    A(&v, gf, p, ap);                // makes the reduced-precision link copy before anything is recorded
+    // one inner iteration, every scalar on the device: alpha = r2 / <p, A p>, x_lo += alpha p,
+    // r_lo -= alpha v with its norms, beta (Polak-Ribiere) and r2 <- |r_lo|^2, p = r_lo + beta p
+    auto body = [&]() {
         A(&v, gf, p, ap);
         scalar_product(&pAp, p, v);
-        system.check(cl2_complex_op(system.ctx(), CL2_SCALAR_OP_RATIO, alpha.dev(), r2c, pAp.dev()));
+        divide(&alpha, r2c, pAp);
         multiply(&malpha, minus_one, alpha);
-        saxpy(&x_lo, malpha, p, x_lo);                   // x_lo += alpha p
-        saxpy_AND_cgnorm(&r_lo, alpha, v, r_lo, norms);  // r_lo -= alpha v and its norms
+        saxpy(&x_lo, malpha, p, x_lo);
+        saxpy_AND_cgnorm(&r_lo, alpha, v, r_lo, norms);
+        system.check(cl2_cg_beta(c, mbeta.dev(), r2c.dev(), norms.dev()));
+        saxpy(&p, mbeta, p, r_lo);
+    };
+    struct Graph {
+        cl2_graph* g = nullptr;
+        ~Graph() { cl2_graph_destroy(g); }
+    } graph;
+    if (opt.use_graph) {
+        system.check(cl2_graph_begin(c));
+        try {
+            body();
+        } catch (...) {
+            cl2_graph* g = nullptr;
+            (void)cl2_graph_end(c, &g);
+            cl2_graph_destroy(g);
+            throw;
+        }
+        system.check(cl2_graph_end(c, &graph.g));
+    }
+    hmc_float rmax = std::sqrt(r2);
+    int k          = 0;
+    for (k = 0; k < opt.max_iterations; ++k) {
+        if (graph.g)
+            system.check(cl2_graph_launch(c, graph.g, 1));
+        else
+            body();
         const std::vector<hmc_float> nr = norms.get();
         hmc_float rn                    = nr[0];
         if (rn != rn) throw SolverStuck(k, "cg_reliable_update: NAN occured");
-        r2c            = norms.dev() + 2;
-        hmc_float beta = (rn - nr[1]) / r2;  // Polak-Ribiere
-        if (!(beta >= 0.)) beta = rn / r2;
         if (std::sqrt(rn) < opt.delta * rmax || rn <= target) {
             convert_precision(&t64, x_lo);
             saxpy(x, hmc_complex{-1., 0.}, t64, *x);  // x += x_lo
@@ -381,18 +409,17 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
                 ++k;
                 break;
             }
+            // p = r_iter + beta p_old was formed with the iterated residual: rebuild it on the true one,
+            // p = r_true + (beta p_old with its component along r_true removed)
+            saxpy(&p, one, r_lo, p);  // beta p_old = p - r_iter
             convert_precision(&r_lo, r);
-            // restore <r, p> = 0 for the new residual
             const hmc_complex rp = scalar_product(r_lo, p);
             saxpy(&p, hmc_complex{rp.re / rn, rp.im / rn}, r_lo, p);
-            beta = rn / r2;
+            saxpy(&p, hmc_complex{-1., 0.}, r_lo, p);  // + r_true
+            r2c.store(hmc_complex{rn, 0.});
             rmax = std::sqrt(rn);
-            r2true.store(hmc_complex{rn, 0.});
-            r2c = r2true.dev();
         }
         rmax = std::max(rmax, std::sqrt(rn));
-        saxpy(&p, hmc_complex{-beta, 0.}, p, r_lo);  // p = r_lo + beta p
-        r2 = rn;
     }
     st.iterations = k;
     st.true_relative_residual = std::sqrt(squarenorm(r) / b2);
@@ -481,12 +508,38 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
     active.store(act);
     int k           = 0;
     int reliable    = 0;
-    for (k = 0; k < opt.max_iterations && n_active > 0; ++k) {
+    fermionmatrix::apply_block(A, &V, gf, P, ap, tmplo);  // reduced-precision links made before recording
+    // one lockstep iteration, every scalar on the device
+    auto body = [&]() {
         fermionmatrix::apply_block(A, &V, gf, P, ap, tmplo);
         system.check(cl2_block_dot_eo(c, pAp.dev(), P.dev(), V.dev(), lo, n));
         system.check(cl2_block_cg_alpha(c, alpha.dev(), r2.dev(), pAp.dev(), active.dev(), n));
         system.check(cl2_block_axpy_eo(c, Xlo.dev(), P.dev(), Xlo.dev(), lo, n, alpha.dev(), 1., nullptr));
         system.check(cl2_block_axpy_eo(c, Rlo.dev(), V.dev(), Rlo.dev(), lo, n, alpha.dev(), -1., norms.dev()));
+        system.check(cl2_block_cg_beta(c, beta.dev(), r2.dev(), norms.dev(), active.dev(), n));
+        system.check(cl2_block_axpy_eo(c, P.dev(), P.dev(), Rlo.dev(), lo, n, beta.dev(), 1., nullptr));  // p = r + beta p
+    };
+    struct Graph {
+        cl2_graph* g = nullptr;
+        ~Graph() { cl2_graph_destroy(g); }
+    } graph;
+    if (opt.use_graph) {
+        system.check(cl2_graph_begin(c));
+        try {
+            body();
+        } catch (...) {
+            cl2_graph* g = nullptr;
+            (void)cl2_graph_end(c, &g);
+            cl2_graph_destroy(g);
+            throw;
+        }
+        system.check(cl2_graph_end(c, &graph.g));
+    }
+    for (k = 0; k < opt.max_iterations && n_active > 0; ++k) {
+        if (graph.g)
+            system.check(cl2_graph_launch(c, graph.g, 1));
+        else
+            body();
         const std::vector<hmc_float> nr = norms.get();
         bool update                     = false;
         for (int r = 0; r < n; ++r) {
@@ -494,53 +547,45 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
             const hmc_float q = nr[2 * r];
             if (q != q) throw SolverStuck(k, "cg_reliable_update_block: NAN occured");
             if (std::sqrt(q) < opt.delta * rmax[r] || q <= target[r]) update = true;
+            rmax[r] = std::max(rmax[r], std::sqrt(q));
         }
-        if (update) {
-            system.check(cl2_block_convert_eo(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
-            system.check(cl2_block_axpy_eo(c, X.dev(), T.dev(), X.dev(), CL2_F64, n, ones.dev(), 1., nullptr));
-            Xlo.zero();
-            rn = true_residual();
-            ++reliable;
-            std::vector<hmc_float> bt(2 * n, 0.), oc(2 * n, 0.);
-            for (int r = 0; r < n; ++r) {
-                if (act[r] == 0.) continue;
-                if (rn[r] <= target[r]) {  // converged: freeze this right-hand side
-                    act[r]           = 0.;
-                    st[r].iterations = k + 1;
-                    --n_active;
-                }
-            }
-            if (n_active == 0) {
-                ++k;
-                break;
-            }
-            system.check(cl2_block_convert_eo(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
-            // restore <r, p> = 0 for the new residuals
-            system.check(cl2_block_dot_eo(c, res.dev(), Rlo.dev(), P.dev(), lo, n));
-            const std::vector<hmc_float> rp = res.get();
-            for (int r = 0; r < n; ++r) {
-                if (act[r] == 0.) continue;
-                oc[2 * r]     = rp[2 * r] / rn[r];
-                oc[2 * r + 1] = rp[2 * r + 1] / rn[r];
-                bt[2 * r]     = rn[r] / r2h[r];
-                r2h[r]        = rn[r];
-                rmax[r]       = std::sqrt(rn[r]);
-            }
-            coef.store(oc);
-            system.check(cl2_block_axpy_eo(c, P.dev(), Rlo.dev(), P.dev(), lo, n, coef.dev(), -1., nullptr));
-            beta.store(bt);
-            r2.store(r2h);
-            active.store(act);
-        } else {
-            system.check(cl2_block_cg_beta(c, beta.dev(), r2.dev(), norms.dev(), active.dev(), n));
-            for (int r = 0; r < n; ++r) {
-                if (act[r] == 0.) continue;
-                r2h[r]  = nr[2 * r];
-                rmax[r] = std::max(rmax[r], std::sqrt(nr[2 * r]));
+        if (!update) continue;
+        system.check(cl2_block_convert_eo(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
+        system.check(cl2_block_axpy_eo(c, X.dev(), T.dev(), X.dev(), CL2_F64, n, ones.dev(), 1., nullptr));
+        Xlo.zero();
+        rn = true_residual();
+        ++reliable;
+        std::vector<hmc_float> oc(2 * n, 0.);
+        for (int r = 0; r < n; ++r) {
+            if (act[r] == 0.) continue;
+            if (rn[r] <= target[r]) {  // converged: freeze this right-hand side
+                act[r]           = 0.;
+                st[r].iterations = k + 1;
+                --n_active;
             }
         }
-        // p = r_lo + beta p
-        system.check(cl2_block_axpy_eo(c, P.dev(), P.dev(), Rlo.dev(), lo, n, beta.dev(), 1., nullptr));
+        if (n_active == 0) {
+            ++k;
+            break;
+        }
+        // p = r_iter + beta p_old was formed with the iterated residuals: rebuild it on the true ones,
+        // p = r_true + (beta p_old with its component along r_true removed)
+        system.check(cl2_block_axpy_eo(c, P.dev(), Rlo.dev(), P.dev(), lo, n, ones.dev(), -1., nullptr));
+        system.check(cl2_block_convert_eo(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
+        system.check(cl2_block_dot_eo(c, res.dev(), Rlo.dev(), P.dev(), lo, n));
+        const std::vector<hmc_float> rp = res.get();
+        for (int r = 0; r < n; ++r) {
+            if (act[r] == 0.) continue;
+            oc[2 * r]     = rp[2 * r] / rn[r];
+            oc[2 * r + 1] = rp[2 * r + 1] / rn[r];
+            r2h[r]        = rn[r];
+            rmax[r]       = std::sqrt(rn[r]);
+        }
+        coef.store(oc);
+        system.check(cl2_block_axpy_eo(c, P.dev(), Rlo.dev(), P.dev(), lo, n, coef.dev(), -1., nullptr));
+        system.check(cl2_block_axpy_eo(c, P.dev(), Rlo.dev(), P.dev(), lo, n, ones.dev(), 1., nullptr));
+        r2.store(r2h);
+        active.store(act);
     }
     for (int r = 0; r < n; ++r) st[r].reliable_updates = reliable;
     const bool done = n_active == 0;
@@ -569,8 +614,11 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
     if (lo != CL2_F32 && lo != CL2_F64) throw std::invalid_argument("cg_reliable_update_stagg: inner precision FP32 or FP64");
     const Staggeredfield_eo r(system, CL2_F64), t64(system, CL2_F64);
     const Staggeredfield_eo r_lo(system, lo), p(system, lo), v(system, lo), x_lo(system, lo);
-    const Scalar<hmc_float> alpha(system), malpha(system), pAp(system), r2(system), zero(system);
+    const Scalar<hmc_float> alpha(system), malpha(system), pAp(system), r2(system), rn_s(system), beta(system),
+        zero(system), one_s(system);
     zero.store(0.);
+    one_s.store(1.);
+    cl2_ctx* c  = system.ctx();
     auto apply = [&](const Staggeredfield_eo* out, const Staggeredfield_eo& in) {  // out = (A + sigma) in
         A(out, gf, in, &ap);
         if (sigma != 0.) saxpy(out, sigma, in, *out);
@@ -593,19 +641,45 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
     copyData(&p, r_lo);
     x_lo.set_zero();
     r2.store(rn);
-    hmc_float r2h = rn, rmax = std::sqrt(rn);
-    int k         = 0;
-    for (k = 0; k < opt.max_iterations; ++k) {
+    apply(&v, p);  // makes the reduced-precision link copy and MdagM temporary before anything is recorded
+    // one inner iteration with every scalar on the device (beta = |r_new|^2 / |r_old|^2)
+    auto body = [&]() {
         apply(&v, p);
         scalar_product_real_part(&pAp, p, v);
         divide(&alpha, r2, pAp);
         subtract(&malpha, zero, alpha);
         saxpy(&x_lo, alpha, p, x_lo);   // x_lo += alpha p
         saxpy(&r_lo, malpha, v, r_lo);  // r_lo -= alpha v
-        squarenorm(&r2, r_lo);
-        rn = r2.get();
+        squarenorm(&rn_s, r_lo);
+        divide(&beta, rn_s, r2);
+        add(&r2, rn_s, zero);           // r2 = |r_new|^2
+        saxpby(&p, one_s, r_lo, beta, p);  // p = r_lo + beta p
+    };
+    struct Graph {
+        cl2_graph* g = nullptr;
+        ~Graph() { cl2_graph_destroy(g); }
+    } graph;
+    if (opt.use_graph) {
+        system.check(cl2_graph_begin(c));
+        try {
+            body();
+        } catch (...) {
+            cl2_graph* g = nullptr;
+            (void)cl2_graph_end(c, &g);
+            cl2_graph_destroy(g);
+            throw;
+        }
+        system.check(cl2_graph_end(c, &graph.g));
+    }
+    hmc_float rmax = std::sqrt(rn);
+    int k          = 0;
+    for (k = 0; k < opt.max_iterations; ++k) {
+        if (graph.g)
+            system.check(cl2_graph_launch(c, graph.g, 1));
+        else
+            body();
+        rn = rn_s.get();
         if (rn != rn) throw SolverStuck(k, "cg_reliable_update_stagg: NAN occured");
-        hmc_float beta = rn / r2h;
         if (std::sqrt(rn) < opt.delta * rmax || rn <= target) {
             convert_precision(&t64, x_lo);
             saxpy(x, 1., t64, *x);  // x += x_lo
@@ -616,16 +690,17 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
                 ++k;
                 break;
             }
+            // p = r_iter + beta p_old was formed with the iterated residual: rebuild it on the true one,
+            // p = r_true + (beta p_old with its component along r_true removed)
+            saxpy(&p, -1., r_lo, p);  // beta p_old = p - r_iter
             convert_precision(&r_lo, r);
-            const hmc_complex rp = scalar_product(r_lo, p);  // restore <r, p> = 0
+            const hmc_complex rp = scalar_product(r_lo, p);
             saxpy(&p, hmc_complex{-rp.re / rn, -rp.im / rn}, r_lo, p);
-            beta = rn / r2h;
-            rmax = std::sqrt(rn);
+            saxpy(&p, 1., r_lo, p);  // + r_true
             r2.store(rn);
+            rmax = std::sqrt(rn);
         }
         rmax = std::max(rmax, std::sqrt(rn));
-        saxpby(&p, 1., r_lo, beta, p);  // p = r_lo + beta p
-        r2h = rn;
     }
     st.iterations             = k;
     st.true_relative_residual = std::sqrt(squarenorm(r) / b2);
@@ -684,8 +759,17 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
     beta_scalar.store(1.0);
     alpha_scalar.store(0.0);
     const hmc_float target1 = opt.sloppy_tol * opt.sloppy_tol * b2;
-    int n_active = n, iter = 0;
-    for (iter = 0; iter < opt.max_iterations && n_active > 0; ++iter) {
+    cl2_ctx* c              = system.ctx();
+    {  // upload the multi-shift pointer table (an update with every shift masked off) before recording
+        const Vector<hmc_float> none(n, system);
+        none.store(std::vector<hmc_float>(n, 0.));
+        system.check(cl2_multishift_update_stagg_eo(c, xp.data(), CL2_F64, pp.data(), r.dev(), lo, n, beta_vec.dev(),
+                                                    zeta_foll.dev(), alpha_vec.dev(), none.dev()));
+        A(&v, gf, p, &ap);  // reduced-precision links and MdagM temporary
+        system.synchronize();
+    }
+    // one CG-M iteration (solver_shifted.cpp:136-198), every scalar on the device
+    auto body = [&]() {
         copyData(&beta_scalar_prev, beta_scalar);
         A(&v, gf, p, &ap);
         scalar_product_real_part(&tmp3, p, v);
@@ -699,24 +783,48 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
         update_zeta_cgm(&zeta_foll, zeta, zeta_prev, beta_scalar_prev, beta_scalar, alpha_scalar_prev, shift, n);
         update_beta_cgm(&beta_vec, beta_scalar, zeta_foll, zeta, n);
         update_alpha_cgm(&alpha_vec, alpha_scalar, zeta_foll, beta_vec, zeta, beta_scalar, n);
-        system.check(cl2_multishift_update_stagg_eo(system.ctx(), xp.data(), CL2_F64, pp.data(), r.dev(), lo, n,
-                                                    beta_vec.dev(), zeta_foll.dev(), alpha_vec.dev(), active.dev()));
+        system.check(cl2_multishift_update_stagg_eo(c, xp.data(), CL2_F64, pp.data(), r.dev(), lo, n, beta_vec.dev(),
+                                                    zeta_foll.dev(), alpha_vec.dev(), active.dev()));
         copyData(&zeta_prev, zeta);
         copyData(&zeta, zeta_foll);
         copyData(&tmp1, tmp2);
-        if (iter % 10 == 9) {  // shift k's residual is zeta_k r
-            const hmc_float rr = tmp2.get();
-            if (rr != rr) throw SolverStuck(iter, "cg_m_mixed: NAN occured");
-            const std::vector<hmc_float> z = zeta.get();
-            bool changed                   = false;
-            for (int k = 0; k < n; ++k)
-                if (act[k] != 0. && z[k] * z[k] * rr < target1) {
-                    act[k] = 0.;
-                    --n_active;
-                    changed = true;
-                }
-            if (changed) active.store(act);
+    };
+    struct Graph {
+        cl2_graph* g = nullptr;
+        ~Graph() { cl2_graph_destroy(g); }
+    } graph;
+    if (opt.use_graph) {
+        system.check(cl2_graph_begin(c));
+        try {
+            body();
+        } catch (...) {
+            cl2_graph* g = nullptr;
+            (void)cl2_graph_end(c, &g);
+            cl2_graph_destroy(g);
+            throw;
         }
+        system.check(cl2_graph_end(c, &graph.g));
+    }
+    int n_active = n, iter = 0;
+    while (iter < opt.max_iterations && n_active > 0) {
+        // ten iterations between convergence checks, as cgIterationBlockSize
+        const int steps = std::min(10, opt.max_iterations - iter);
+        if (graph.g)
+            system.check(cl2_graph_launch(c, graph.g, steps));
+        else
+            for (int s = 0; s < steps; ++s) body();
+        iter += steps;
+        const hmc_float rr = tmp2.get();  // shift k's residual is zeta_k r
+        if (rr != rr) throw SolverStuck(iter, "cg_m_mixed: NAN occured");
+        const std::vector<hmc_float> z = zeta.get();
+        bool changed                   = false;
+        for (int k = 0; k < n; ++k)
+            if (act[k] != 0. && z[k] * z[k] * rr < target1) {
+                act[k] = 0.;
+                --n_active;
+                changed = true;
+            }
+        if (changed) active.store(act);
     }
     if (multishift_iterations) *multishift_iterations = iter;
     // ---- phase 2: refine every shift in FP64 ----
@@ -724,6 +832,7 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
     ro.inner_precision = lo;
     ro.delta           = opt.delta;
     ro.max_iterations  = opt.max_iterations;
+    ro.use_graph       = opt.use_graph;
     int total          = iter;
     for (int k = 0; k < n; ++k)
         total += cg_reliable_update_stagg(x[k].get(), A, gf, sigma[k], b, system, rel_tol, ap, ro, &st[k]);

@@ -11,13 +11,14 @@
  * Device layouts (all fields are dense, stride = number of eo sites Veo):
  *   Wilson eo spinor  F64: double2[12][Veo]  component c = 3*spin + colour
  *                     F32: float2 [12][Veo]
- *                     F16: half[6][Veo][4] (components 2j, 2j+1 as re,im,re,im) then float norm[Veo];
+ *                     F16: half[3][Veo][8] (components 4j..4j+3 as re,im pairs) then float norm[Veo];
  *                          component = half * norm (norm = largest |re|,|im| on the site)
  *   staggered eo vec  F64: double2[3][Veo],  F32: float2[3][Veo]
  *   gauge field      F64_18: double2[4 dir][2 parity][9][Veo]   (row-major 3x3)
  *                    F64_12: double2[4][2][6][Veo]   rows 0,1; row 2 = conj(row0 x row1)
  *                    F32_12: float4 [4][2][3][Veo]   rows 0,1 packed two entries per float4
- *                    F16_12: half[4][2][3][Veo][4]   like F32_12 in half precision
+ *                    F16_12: per (dir, parity): half[Veo][8] (entries 0..3 of rows 0,1), then half[Veo][4]
+ *                            (entries 4, 5); 24 B per link
  *   device scalars: complex = 2 doubles (re, im); real = 1 double.
  * Host arrays use the reference's AoS order: site = x + NS*(y + NS*(z + NS*t)),
  * eo index = site/2, link index = mu + 4*site with mu = T,X,Y,Z.
@@ -216,6 +217,11 @@ int cl2_saxpy_AND_cgnorm_eo(cl2_ctx* ctx, void* out, const void* x, const void* 
  * variance 1/2; fields here are reproducible from the seed alone).  kind: CL2_FIELD_SPINOR_EO or
  * CL2_FIELD_SU3VEC_EO, FP64 or FP32. */
 int cl2_gaussian_eo(cl2_ctx* ctx, int kind, void* out, int prec, uint64_t seed);
+/* the summand of the pseudoscalar point-source correlator (ocl_kernel/fermionobservables/correlator_ps_point.cl
+ * correlator_ps_t, correlator_ps_z; direction dispatch physics/observables/wilsonTwoFlavourCorrelators.cpp:357):
+ * res[s] (= or +=, device doubles) of sum over the eo sites of slice s of |x|^2, slices along dir = 0 (t, nt
+ * entries) or dir = 3 (z, ns entries); FP64 or FP32 Wilson eo field */
+int cl2_slice_squarenorm_eo(cl2_ctx* ctx, double* res, const void* x, int prec, int dir, int accumulate);
 int cl2_set_zero_eo(cl2_ctx* ctx, void* out, int prec);
 int cl2_set_cold_eo(cl2_ctx* ctx, void* out, int prec);
 
@@ -319,6 +325,26 @@ int cl2_update_beta_cgm(cl2_ctx* ctx, double* out, const double* sbeta_pres, con
 int cl2_update_alpha_cgm(cl2_ctx* ctx, double* out, const double* salpha_pres, const double* zeta_pres,
                          const double* beta_pres, const double* zeta_prev, const double* sbeta_pres, int numeq);
 
+/* ==== captured launch sequences ====
+ * No counterpart in the reference.  Everything enqueued on the context's stream between cl2_graph_begin and
+ * cl2_graph_end is recorded instead of run (HIP stream capture) and replayed by cl2_graph_launch with one
+ * submission: the fixed kernel sequence of a solver iteration.  Only kernel-launching entry points may be
+ * recorded (no transfers, synchronisations or allocations); device pointers and device scalars keep the
+ * values they have at replay time, host-value scalars the value they had when recorded. */
+typedef struct cl2_graph cl2_graph;
+int cl2_graph_begin(cl2_ctx* ctx);
+int cl2_graph_end(cl2_ctx* ctx, cl2_graph** graph);
+int cl2_graph_launch(cl2_ctx* ctx, cl2_graph* graph, int times);
+int cl2_graph_destroy(cl2_graph* graph);
+/* copyData between device scalars / vectors (physics/lattices/scalar.hpp, vector.hpp): n doubles, done by a
+ * kernel so that it can be part of a recorded sequence */
+int cl2_scalar_copy(cl2_ctx* ctx, double* dst, const double* src, int n);
+/* No counterpart in the reference: the beta step of the reduced-precision CG on the device, so that a whole
+ * iteration can be recorded.  norms = (|r_new|^2, Re <r_new, r_old>) from cl2_saxpy_AND_cgnorm_eo; mbeta
+ * (complex) = -beta with the Polak-Ribiere beta (Fletcher-Reeves when negative), r2 (complex) = (|r_new|^2, 0)
+ * after reading the old value. */
+int cl2_cg_beta(cl2_ctx* ctx, double* mbeta, double* r2, const double* norms);
+
 /* ==== launch tuning (no reference counterpart: the reference picks work sizes in
  *      hardware/device_info.cpp:111-128 and hardware/code/opencl_module.cpp get_work_sizes) ==== */
 /* keys: "hop.zplanes" -- z-planes per slab of the t-fastest traversal of the hopping kernels
@@ -326,6 +352,15 @@ int cl2_update_alpha_cgm(cl2_ctx* ctx, double* out, const double* salpha_pres, c
  *       "hop.xcd" -- 1: give each XCD a contiguous range of blocks, 0: hardware round robin. */
 int cl2_tune_set(cl2_ctx* ctx, const char* key, long value);
 long cl2_tune_get(const cl2_ctx* ctx, const char* key);
+
+/* No counterpart in the reference (which fixes work-group sizes per device in hardware/device.cpp:286-334):
+ * autotune the hopping kernel's launch shape for (format, nrhs) on this context -- z-planes per slab, XCD
+ * mapping, right-hand sides per thread, and for single fields the block size (128, 256, 512) with or
+ * without a 4-waves-per-SIMD register budget -- by timing every candidate on scratch fields with the given
+ * gauge field; the fastest is kept for later launches (unless cl2_tune_set fixed the shape) and returned
+ * (block < 0: the register-budget variant of block size -block).  Output pointers may be null. */
+int cl2_autotune_dslash(cl2_ctx* ctx, const void* gauge, int format, int nrhs, double* best_ms, int* zplanes,
+                        int* xcd, int* rhs_per_thread, int* block);
 
 /* ==== timing on the context stream (Device profiling events, hardware/device.cpp:233-242) ==== */
 int cl2_timer_start(cl2_ctx* ctx);

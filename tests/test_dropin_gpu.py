@@ -310,3 +310,62 @@ def test_cg_reliable_update_half():
     bh, xh = b.get_host(), x.get_host()
     r = O.saxpy(O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, xh, gf.get_host(), O.TWISTEDMASS, 0.15, l.mubar), bh, 1.0)
     assert np.sqrt(O.norm2(r) / O.norm2(bh)) <= 1.01e-10
+
+
+@pytest.mark.parametrize("prec", ["F32", "F16"])
+def test_reliable_update_graph_replays_match_eager(prec):
+    """The recorded inner iteration, replayed once per iteration with eager reliable updates and host reads
+    in between, gives bit-identical solutions and iteration counts to the same iteration run eagerly."""
+    l = lattice("--nSpace=8", "--nTime=8", "--fermionAction=twistedmass", "--kappa=0.15", "--mu=0.02", "--beta=5.7")
+    gf = H.Gaugefield(l, False)
+    for _ in range(3):
+        H.su3heatbath(gf, l, 1)
+    b, _ = eo_pair(l, 61)
+    A = H.QplusQminus_eo(l)
+    out = []
+    for use_graph in (True, False, True):
+        x = H.Spinorfield_eo(l)
+        x.zero()
+        st = H.cg_reliable_update(x, A, gf, b, l, 1e-10, l.wilson_parameters(), getattr(H, prec),
+                                  use_graph=use_graph)
+        assert st["true_relative_residual"] <= 1e-10
+        out.append((st["iterations"], x.get_host()))
+    for it, xh in out[1:]:
+        assert it == out[0][0] and np.array_equal(xh, out[0][1])
+
+
+def test_block_and_staggered_graph_replays_match_eager():
+    l = lattice("--nSpace=8", "--nTime=8", "--fermionAction=twistedmass", "--kappa=0.15", "--mu=0.02", "--beta=5.7")
+    gf = H.Gaugefield(l, False)
+    for _ in range(3):
+        H.su3heatbath(gf, l, 1)
+    rng = np.random.default_rng(62)
+    bs = []
+    for _ in range(4):
+        b = H.Spinorfield_eo(l)
+        b.import_(rng.standard_normal((l.eo_sites, 24)))
+        bs.append(b)
+    A = H.QplusQminus_eo(l)
+    res = []
+    for use_graph in (True, False):
+        xs = [H.Spinorfield_eo(l) for _ in bs]
+        for x in xs:
+            x.zero()
+        it, per = H.cg_reliable_update_block(xs, A, gf, bs, l, 1e-10, l.wilson_parameters(), use_graph=use_graph)
+        assert max(p["true_relative_residual"] for p in per) <= 1e-10
+        res.append((it, [x.get_host() for x in xs]))
+    assert res[0][0] == res[1][0] and all(np.array_equal(a, b) for a, b in zip(res[0][1], res[1][1]))
+    ls = lattice("--nTime=4", "--fermionAction=rooted_stagg", "--mass=0.05")
+    g2 = conf(ls)
+    M = H.MdagM_eo(ls)
+    bst = H.Staggeredfield_eo(ls)
+    H.pseudo_randomize(bst, 63)
+    sres = []
+    for use_graph in (True, False):
+        x = H.Staggeredfield_eo(ls)
+        x.set_zero()
+        st = H.cg_reliable_update_stagg(x, M, g2, 0.01, bst, ls, 1e-10, ls.staggered_parameters(),
+                                        use_graph=use_graph)
+        assert st["true_relative_residual"] <= 1e-10
+        sres.append((st["iterations"], x.get_host()))
+    assert sres[0][0] == sres[1][0] and np.array_equal(sres[0][1], sres[1][1])

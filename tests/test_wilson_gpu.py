@@ -213,3 +213,47 @@ def test_rejects_bad_arguments():
             ctx.dslash_eo(a.ptr, gf.ptr, gf.ptr, 9, abi.EVEN, 0.1)
     with pytest.raises(abi.Cl2Error):
         abi.Context(5, 4)
+
+
+@pytest.mark.parametrize("fmt,prec,nrhs", [(abi.GAUGE_F64_18, abi.F64, 1), (abi.GAUGE_F32_12, abi.F32, 1),
+                                           (abi.GAUGE_F16_12, abi.F16, 1), (abi.GAUGE_F32_12, abi.F32, 4)])
+def test_autotune_keeps_results(fmt, prec, nrhs):
+    """The autotuned launch shape only changes which thread runs which site, so results stay bit identical."""
+    import ctypes
+    lat, ctx = make(8, 8, theta_t=1.0)
+    with ctx:
+        g = random_su3_field(4 * O.volume(lat), 77)
+        gf = ctx.gauge(g, fmt)
+        fb = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, prec)
+        src = np.random.default_rng(78).standard_normal((ctx.veo, 24))
+        single = ctx.spinor(prec).upload(src)
+        a, b = ctx.buffer(nrhs * fb), ctx.buffer(nrhs * fb)
+        for r in range(nrhs):
+            if nrhs == 1:
+                ctx.copy(a.ptr, single.ptr, fb)
+            else:
+                ctx.block_insert_eo(a.ptr, single.ptr, prec, nrhs, r)
+
+        outf = ctx.spinor(prec)
+
+        def run():
+            if nrhs == 1:
+                ctx.dslash_eo(b.ptr, a.ptr, gf.ptr, fmt, abi.EVEN, 0.16)
+            else:
+                ctx.dslash_eo_mrhs(b.ptr, a.ptr, gf.ptr, fmt, abi.EVEN, 0.16, nrhs)
+            res = []
+            for r in range(nrhs):
+                if nrhs == 1:
+                    ctx.copy(outf.ptr, b.ptr, fb)
+                else:
+                    ctx.block_extract_eo(outf.ptr, b.ptr, prec, nrhs, r)
+                res.append(outf.download())
+            return np.stack(res)
+
+        before = run()
+        ms, zp, xcd, rb, blk = ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ctx.autotune_dslash(gf.ptr, fmt, nrhs, ctypes.byref(ms), ctypes.byref(zp), ctypes.byref(xcd),
+                            ctypes.byref(rb), ctypes.byref(blk))
+        assert ms.value > 0 and zp.value >= -1 and xcd.value in (0, 1) and rb.value in (1, 2, 4)
+        assert abs(blk.value) in (128, 256, 512)
+        assert np.array_equal(run(), before)
