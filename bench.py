@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 METRIC = "eo-Dslash GB/s & %HBM peak; mixed vs FP64 CG time-to-1e-10 at 48³×96"
 HBM_PEAK_GBPS = 8000.0
 COUNTERS_FILE = os.path.join(ROOT, "profiles", "dslash_eo_f64_18_48x96_counters.txt")
+USE_GRAPH = True  # replay the captured inner iteration of the mixed-precision solvers
 
 
 def log(*a):
@@ -53,13 +54,15 @@ def time_steps(ctx, fn, steps):
     return ms.value, (time.perf_counter() - t0) * 1e3
 
 
-def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src):
+def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src, autotune=True):
     g = ctx.gauge(fmt=fmt, seed=20261202, epsilon=0.35)
     a, b = ctx.spinor(prec).upload(src), ctx.spinor(prec)
     kappa = 0.16
-    # launch shape from the autotuner (z-planes per slab, XCD mapping), timed on this field
-    tuned = [ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()]
-    ctx.check(ctx.lib.cl2_autotune_dslash(ctx.handle, g.ptr, fmt, 1, *[ctypes.byref(t) for t in tuned]))
+    # launch shape from the autotuner (z-planes per slab, XCD mapping, block size), timed on this field;
+    # without it the library's default shape
+    tuned = [ctypes.c_double(), ctypes.c_int(-1), ctypes.c_int(1), ctypes.c_int(1), ctypes.c_int(256)]
+    if autotune:
+        ctx.check(ctx.lib.cl2_autotune_dslash(ctx.handle, g.ptr, fmt, 1, *[ctypes.byref(t) for t in tuned]))
 
     def step(k):
         if k % 2 == 0:
@@ -78,7 +81,8 @@ def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src):
     alg = ctx.lib.cl2_algorithmic_bytes(ctx.handle, b"dslash_eo", fmt)
     return dict(ms=ev_ms / steps, wall_ms=wall_ms / steps, alg_bytes=alg, GBps=alg * steps / (ev_ms * 1e6),
                 finite=finite, tuned={"zplanes": tuned[1].value, "xcd": tuned[2].value, "block": abs(tuned[4].value),
-                                      "min_waves_per_simd": 4 if tuned[4].value < 0 else None})
+                                      "min_waves_per_simd": 4 if tuned[4].value < 0 else None,
+                                      "autotuned": autotune})
 
 
 def cg_leg(ns, nt, sweeps, overrelax, seed):
@@ -129,7 +133,7 @@ def cg_leg(ns, nt, sweeps, overrelax, seed):
         x.zero()
         l.synchronize()
         t0 = time.perf_counter()
-        st = H.cg_reliable_update(x, qq, gf, b, l, 1e-10, ap, prec, 0.1)
+        st = H.cg_reliable_update(x, qq, gf, b, l, 1e-10, ap, prec, 0.1, use_graph=USE_GRAPH)
         l.synchronize()
         dt = time.perf_counter() - t0
         H.saxpy(d, 1.0, x64, x)
@@ -173,7 +177,7 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
     t0 = time.perf_counter()
     seq_it = []
     for r in range(nrhs):
-        st = H.cg_reliable_update(xs[r], qq, gf, bs[r], l, 1e-10, ap, H.F32, 0.1)
+        st = H.cg_reliable_update(xs[r], qq, gf, bs[r], l, 1e-10, ap, H.F32, 0.1, use_graph=USE_GRAPH)
         seq_it.append(st["iterations"])
     l.synchronize()
     t_seq = time.perf_counter() - t0
@@ -181,7 +185,7 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
         x.zero()
     l.synchronize()
     t0 = time.perf_counter()
-    it, per = H.cg_reliable_update_block(xs, qq, gf, bs, l, 1e-10, ap, H.F32, 0.1)
+    it, per = H.cg_reliable_update_block(xs, qq, gf, bs, l, 1e-10, ap, H.F32, 0.1, use_graph=USE_GRAPH)
     l.synchronize()
     t_blk = time.perf_counter() - t0
     worst = max(p["true_relative_residual"] for p in per)
@@ -246,7 +250,7 @@ def multishift_leg(ns, nt, sweeps, seed, nshift=16):
     xs = [H.Staggeredfield_eo(l) for _ in sigma]
     l.synchronize()
     t0 = time.perf_counter()
-    total, ms_it, per = H.cg_m_mixed(xs, A, gf, sigma, b, l, 1e-10, ap, H.F32, 1e-7)
+    total, ms_it, per = H.cg_m_mixed(xs, A, gf, sigma, b, l, 1e-10, ap, H.F32, 1e-7, use_graph=USE_GRAPH)
     l.synchronize()
     t_ms = time.perf_counter() - t0
     worst_ms = max(p["true_relative_residual"] for p in per)
@@ -256,7 +260,7 @@ def multishift_leg(ns, nt, sweeps, seed, nshift=16):
     seq_it = []
     for s_, x in zip(sigma, seq):
         x.set_zero()
-        seq_it.append(H.cg_reliable_update_stagg(x, A, gf, s_, b, l, 1e-10, ap, H.F32)["iterations"])
+        seq_it.append(H.cg_reliable_update_stagg(x, A, gf, s_, b, l, 1e-10, ap, H.F32, use_graph=USE_GRAPH)["iterations"])
     l.synchronize()
     t_seq = time.perf_counter() - t0
     ref = [H.Staggeredfield_eo(l) for _ in sigma]
@@ -422,7 +426,14 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=20.0, help="mean requests per second")
     ap.add_argument("--serve-baseline", type=int, default=10, help="requests also run by the FP64 baseline")
     ap.add_argument("--hb-overrelax", type=int, default=1, help="overrelaxation sweeps per heatbath sweep")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="time the hopping kernels in the library's default launch shape (for profiles whose "
+                         "per-kernel statistics should hold the timed launches only)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch the solvers' inner iterations kernel by kernel instead of replaying a hipGraph")
     args = ap.parse_args()
+    global USE_GRAPH
+    USE_GRAPH = not args.no_graph
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
               "described in DESIGN.md but not built)", file=sys.stderr)
@@ -439,16 +450,25 @@ def main():
     ctx.squarenorm_eo(res.ptr, cal.ptr, abi.F64)
     ctx.set_zero_eo(cal.ptr, abi.F64)
     ctx.sync()
+    # achievable streaming rate on this GPU: the library's copy kernel on one FP64 spinor field
+    dst = ctx.spinor()
+    nbytes = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F64)
+    for _ in range(2):
+        ctx.check(ctx.lib.cl2_copy(ctx.handle, dst.ptr, cal.ptr, nbytes))
+    copy_ms, _ = time_steps(ctx, lambda k: ctx.check(ctx.lib.cl2_copy(ctx.handle, dst.ptr, cal.ptr, nbytes)), 20)
+    copy_GBps = 2 * nbytes * 20 / (copy_ms * 1e6)
+    dst.close()
     cal.close()
     res.close()
+    log(f"copy kernel: {copy_GBps:.0f} GB/s (read + write of one FP64 field)")
     log(f"lattice {args.ns}^3x{args.nt}, {ctx.veo} eo sites; warmup {args.warmup}, steps {args.steps}")
-    f64 = dslash_leg(ctx, abi, abi.GAUGE_F64_18, abi.F64, args.steps, args.warmup, src)
+    f64 = dslash_leg(ctx, abi, abi.GAUGE_F64_18, abi.F64, args.steps, args.warmup, src, not args.no_autotune)
     log(f"dslash F64/18: {f64['ms']:.4f} ms/app, {f64['GBps']:.0f} GB/s")
     extra = {}
     if not args.only_f64:
         for name, fmt, prec in (("f64_12", abi.GAUGE_F64_12, abi.F64), ("f32_12", abi.GAUGE_F32_12, abi.F32),
                                 ("f16_12", abi.GAUGE_F16_12, abi.F16)):
-            r = dslash_leg(ctx, abi, fmt, prec, max(args.steps // 4, 2), args.warmup, src)
+            r = dslash_leg(ctx, abi, fmt, prec, max(args.steps // 4, 2), args.warmup, src, not args.no_autotune)
             log(f"dslash {name}: {r['ms']:.4f} ms/app, {r['GBps']:.0f} GB/s")
             extra[f"dslash_{name}"] = dict(ms_per_app=round(r["ms"], 5), GBps=round(r["GBps"], 1),
                                            pct_hbm_peak=round(100 * r["GBps"] / HBM_PEAK_GBPS, 1),
@@ -492,6 +512,8 @@ def main():
                    "lattice": f"{args.ns}^3x{args.nt}", "eo_sites": veo,
                    "bytes_per_site": 1536},
         "pct_hbm_peak": round(100 * f64["GBps"] / HBM_PEAK_GBPS, 1),
+        "copy_GBps": round(copy_GBps, 1),
+        "frac_of_copy": round(f64["GBps"] / copy_GBps, 4),
         "wall_ms_per_step": round(f64["wall_ms"], 5),
         "cl2qcd_convention_GBps": round(ref_bytes / (f64["ms"] * 1e6), 1),
         "output_finite": f64["finite"],

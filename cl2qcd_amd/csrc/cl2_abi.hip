@@ -405,7 +405,10 @@ int cl2_copy(cl2_ctx* c, void* dst, const void* src, size_t bytes)
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, dst && src, "null pointer");
     if (dst == src || bytes == 0) return CL2_OK;
-    CL2_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (bytes % 16 == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0 && reinterpret_cast<uintptr_t>(src) % 16 == 0)
+        CL2_HIP(c, launch_copy16(c, dst, src, bytes / 16));
+    else
+        CL2_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
     return CL2_OK;
 }
 
@@ -1305,7 +1308,8 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
             if (nrhs % r == 0) rbs.push_back(r);
     // (block, occupancy hint) shapes of the single-field kernel
     std::vector<std::pair<int, int>> shapes = {{256, 0}};
-    if (nrhs == 1 && fmt != CL2_GAUGE_F16_12) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 4}, {128, 4}};
+    if (nrhs == 1 && fmt == CL2_GAUGE_F32_12) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 4}, {128, 4}};
+    else if (nrhs == 1 && fmt != CL2_GAUGE_F16_12) shapes = {{256, 0}, {128, 0}, {512, 0}};
     const bool manual = c->tune_manual;
     c->tune_manual    = false;
     const double one[2] = {1., 0.};

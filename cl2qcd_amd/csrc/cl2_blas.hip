@@ -6,6 +6,8 @@
 // conventions are separate modes.  Reductions accumulate in FP64 and combine in
 // a fixed order (per-block butterfly + LDS, then one block over the partials),
 // so results are reproducible run to run.
+#include <algorithm>
+
 #include "cl2_device.hpp"
 #include "cl2_internal.hpp"
 
@@ -231,6 +233,22 @@ hipError_t launch_slice_norm(cl2_ctx* c, double* res, const void* x, int prec, i
         hipLaunchKernelGGL(k_slice_norm<float>, dim3(nslice), dim3(256), 0, c->stream, res,
                            reinterpret_cast<const float2*>(x), c->g.veo, slice_off, chunk, half_vs, nchunk,
                            accumulate);
+    return hipGetLastError();
+}
+
+/* device-to-device copy of n 16-byte elements with streaming loads and stores, one element per
+ * thread (the launch shape that measured fastest for plain copies on gfx950) */
+__global__ __launch_bounds__(256) void k_copy16(double2* __restrict__ dst, const double2* __restrict__ src, size_t n)
+{
+    const size_t step = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += step) store_stream(dst + i, load_stream(src + i));
+}
+
+hipError_t launch_copy16(cl2_ctx* c, void* dst, const void* src, size_t n)
+{
+    const size_t nb = std::min<size_t>((n + 255) / 256, (size_t)1 << 22);
+    hipLaunchKernelGGL(k_copy16, dim3((unsigned)nb), dim3(256), 0, c->stream, reinterpret_cast<double2*>(dst),
+                       reinterpret_cast<const double2*>(src), n);
     return hipGetLastError();
 }
 

@@ -238,10 +238,12 @@ __device__ __forceinline__ void site_coords(const LatticeGeom& g, uint32_t i, in
     x = x0 + (((y + z + t) & 1u) ^ (uint32_t)parity);
 }
 
-/* Streaming (non-temporal) loads for data read exactly once per launch. */
+/* Streaming (non-temporal) loads for data read exactly once per launch, and streaming stores for
+ * results no later work of the same launch reads: both keep the L2 for data that is reused (the
+ * neighbour spinors of the hopping kernels). */
 typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
 typedef float nt_f4 __attribute__((ext_vector_type(4)));
-#ifndef CL2_LINK_LOAD_CACHED
 __device__ __forceinline__ double2 load_stream(const double2* p)
 {
     nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
@@ -252,16 +254,16 @@ __device__ __forceinline__ float4 load_stream(const float4* p)
     nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
     return make_float4(v.x, v.y, v.z, v.w);
 }
-#else
-__device__ __forceinline__ double2 load_stream(const double2* p)
+__device__ __forceinline__ void store_stream(double2* p, double2 v)
 {
-    return *p;
+    nt_d2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<nt_d2*>(p));
 }
-__device__ __forceinline__ float4 load_stream(const float4* p)
+__device__ __forceinline__ void store_stream(float2* p, float2 v)
 {
-    return *p;
+    nt_f2 w = {v.x, v.y};
+    __builtin_nontemporal_store(w, reinterpret_cast<nt_f2*>(p));
 }
-#endif
 
 /* ---------------- gauge-link formats ----------------
  * Device layout [dir][parity][lane][gstride] where lane = complex entry (18-real)

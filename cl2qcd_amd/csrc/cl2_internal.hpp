@@ -109,6 +109,7 @@ hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src
 hipError_t launch_gaussian(cl2_ctx* c, void* dev, int prec, size_t n, uint64_t seed);
 
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im);
+hipError_t launch_copy16(cl2_ctx* c, void* dst, const void* src, size_t n);
 hipError_t launch_slice_norm(cl2_ctx* c, double* res, const void* x, int prec, int dir, int accumulate);
 
 /* FP16 Wilson spinors (per-site scale) and F16_12 links */

@@ -6,6 +6,15 @@
 // site's own eo index and the backward link shares the neighbour's, so link
 // loads are as coalesced as the spinor gathers.  The kernel is HBM bound
 // (8 links + 2 spinors per site), so no MFMA.
+//
+// What bounds it on gfx950 is the reuse of the neighbour spinors in each
+// XCD's 4 MB L2: every input site is read by 8 output sites, and the fewer
+// sites an XCD has in flight the more of those reads hit.  The single-field
+// kernel therefore keeps few sites in flight (about 4 waves per CU) and gets
+// its memory parallelism from each thread instead: both hops of a direction
+// load their links and full neighbour spinors before either is applied, with
+// a register budget of up to 256 VGPRs, and results leave by streaming
+// stores so they do not evict neighbour spinors from the L2.
 #include <type_traits>
 
 #include "cl2_device.hpp"
@@ -13,22 +22,19 @@
 
 namespace cl2 {
 
-/* One hop direction: the two half-spinor rows (A,B,C) of the projector
- *   h = psi_A + C * psi_B,   r = w * (U or U^dagger) h,
+/* One hop, on a neighbour spinor s already in registers: the two half-spinor rows (A,B,C) of the
+ * projector
+ *   h = s_A + C * s_B,   r = w * (U or U^dagger) h,
  *   acc_A += r,  acc_B += conj(C) * r
  * reproduce dslash_eoprec_unified_local (ocl_kernel/fermionmatrix_eo.cl:30-253);
  * w already carries -kappa and the boundary phase. */
 template<int A, int B, int CU, bool FWD, typename R>
-__device__ __forceinline__ void half_hop(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, size_t S, uint32_t nb,
-                                         const Su3<R>& u, Cx<R> w)
+__device__ __forceinline__ void half_hop_reg(Cx<R> (&acc)[12], const vec2<R> (&s)[12], const Su3<R>& u, Cx<R> w)
 {
     Cx<R> h[3], r[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        vec2<R> a = in[(size_t)(3 * A + c) * S + nb];
-        vec2<R> b = in[(size_t)(3 * B + c) * S + nb];
-        h[c]      = add_unit<CU>(Cx<R>{a.x, a.y}, Cx<R>{b.x, b.y});
-    }
+    for (int c = 0; c < 3; ++c)
+        h[c] = add_unit<CU>(Cx<R>{s[3 * A + c].x, s[3 * A + c].y}, Cx<R>{s[3 * B + c].x, s[3 * B + c].y});
     if constexpr (FWD)
         su3_times(u, h, r);
     else
@@ -41,35 +47,78 @@ __device__ __forceinline__ void half_hop(Cx<R> (&acc)[12], const vec2<R>* __rest
     }
 }
 
-/* SS: component stride of the spinor field (Veo * nrhs), nb: its element index; the link is
- * element linkeo of the link field of stride Veo */
-template<int DIR, bool FWD, int FMT, typename R>
-__device__ __forceinline__ void hop(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, size_t SS, uint32_t nb,
-                                    const void* gauge, uint32_t S, int linkpar, uint32_t linkeo, Cx<R> w)
+/* projector rows of (1 - gamma_mu) forward and (1 + gamma_mu) backward, chiral basis */
+template<int DIR, bool FWD, typename R>
+__device__ __forceinline__ void apply_hop(Cx<R> (&acc)[12], const vec2<R> (&s)[12], const Su3<R>& u, Cx<R> w)
 {
-    Su3<R> u;
-    GaugeLoad<FMT>::load(gauge, S, DIR, linkpar, linkeo, u);
-    // projector rows of (1 - gamma_mu) forward and (1 + gamma_mu) backward, chiral basis
     if constexpr (DIR == 0) {
-        half_hop<0, 2, FWD ? P1 : M1, FWD>(acc, in, SS, nb, u, w);
-        half_hop<1, 3, FWD ? P1 : M1, FWD>(acc, in, SS, nb, u, w);
+        half_hop_reg<0, 2, FWD ? P1 : M1, FWD>(acc, s, u, w);
+        half_hop_reg<1, 3, FWD ? P1 : M1, FWD>(acc, s, u, w);
     } else if constexpr (DIR == 1) {
-        half_hop<0, 3, FWD ? PI : MI, FWD>(acc, in, SS, nb, u, w);
-        half_hop<1, 2, FWD ? PI : MI, FWD>(acc, in, SS, nb, u, w);
+        half_hop_reg<0, 3, FWD ? PI : MI, FWD>(acc, s, u, w);
+        half_hop_reg<1, 2, FWD ? PI : MI, FWD>(acc, s, u, w);
     } else if constexpr (DIR == 2) {
-        half_hop<0, 3, FWD ? P1 : M1, FWD>(acc, in, SS, nb, u, w);
-        half_hop<1, 2, FWD ? M1 : P1, FWD>(acc, in, SS, nb, u, w);
+        half_hop_reg<0, 3, FWD ? P1 : M1, FWD>(acc, s, u, w);
+        half_hop_reg<1, 2, FWD ? M1 : P1, FWD>(acc, s, u, w);
     } else {
-        half_hop<0, 2, FWD ? PI : MI, FWD>(acc, in, SS, nb, u, w);
-        half_hop<1, 3, FWD ? MI : PI, FWD>(acc, in, SS, nb, u, w);
+        half_hop_reg<0, 2, FWD ? PI : MI, FWD>(acc, s, u, w);
+        half_hop_reg<1, 3, FWD ? MI : PI, FWD>(acc, s, u, w);
     }
+}
+
+/* forward and backward hop of direction DIR: both links and both neighbour spinors (elements nf, nbk
+ * of the spinor field) are loaded before either hop is applied.  The forward link is the output
+ * site's own (eo index i, parity P), the backward one the neighbour's (eo index linkbk, parity P^1). */
+/* the same two hops one after the other, for the 128-VGPR variants: each loads only its own link and
+ * neighbour spinor */
+template<int DIR, int FMT, typename R>
+__device__ __forceinline__ void hop_seq(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, size_t SS, uint32_t nf,
+                                        uint32_t nbk, const void* gauge, uint32_t S, int P, uint32_t i,
+                                        uint32_t linkbk, Cx<R> wf, Cx<R> wb)
+{
+    {
+        Su3<R> u;
+        vec2<R> sp[12];
+        GaugeLoad<FMT>::load(gauge, S, DIR, P, i, u);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) sp[k] = in[(size_t)k * SS + nf];
+        apply_hop<DIR, true>(acc, sp, u, wf);
+    }
+    {
+        Su3<R> u;
+        vec2<R> sp[12];
+        GaugeLoad<FMT>::load(gauge, S, DIR, P ^ 1, linkbk, u);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) sp[k] = in[(size_t)k * SS + nbk];
+        apply_hop<DIR, false>(acc, sp, u, wb);
+    }
+}
+
+template<int DIR, int FMT, typename R>
+__device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, size_t SS, uint32_t nf,
+                                         uint32_t nbk, const void* gauge, uint32_t S, int P, uint32_t i,
+                                         uint32_t linkbk, Cx<R> wf, Cx<R> wb)
+{
+    Su3<R> uf, ub;
+    vec2<R> sf[12], sb[12];
+    GaugeLoad<FMT>::load(gauge, S, DIR, P, i, uf);
+    GaugeLoad<FMT>::load(gauge, S, DIR, P ^ 1, linkbk, ub);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) sf[k] = in[(size_t)k * SS + nf];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) sb[k] = in[(size_t)k * SS + nbk];
+    apply_hop<DIR, true>(acc, sf, uf, wf);
+    apply_hop<DIR, false>(acc, sb, ub, wb);
 }
 
 /* NRHS right-hand sides in one launch: the fields are [12][Veo][NRHS] (right-hand side fastest) and
  * thread l works on site l / NRHS, right-hand side l % NRHS, so the NRHS lanes of one site issue the same
  * link addresses and each link leaves HBM once per launch.  NRHS = 1 is the single-field kernel. */
-template<int FMT, int BLOCK, int NRHS, int MINW = 1>
-__global__ __launch_bounds__(BLOCK, MINW) void k_dslash_eo(DslashArgs a)
+/* OCC = 4: compiled for at least 4 waves per SIMD (at most 128 VGPRs); otherwise at most 2 waves per
+ * SIMD, which leaves up to 256 VGPRs for the loads of a hop pair in flight */
+template<int FMT, int BLOCK, int NRHS, int OCC = 0>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC == 4 ? 4 : 1, OCC == 4 ? 8 : 2)))
+void k_dslash_eo(DslashArgs a)
 {
     using R = typename GaugeLoad<FMT>::R;
     using V = vec2<R>;
@@ -83,23 +132,24 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_dslash_eo(DslashArgs a)
         site_neighbours(a.g, i, a.out_parity, nb);
         const V* in    = reinterpret_cast<const V*>(a.in);
         const int P    = a.out_parity;
-        const int Pn   = P ^ 1;
         Cx<R> acc[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] = Cx<R>{R(0), R(0)};
         // order T, X, Y, Z as fermionmatrix_eo_dslash.cl:38-45
-        hop<0, true, FMT>(acc, in, SS, nb.up[0] * NRHS + rhs, a.gauge, S, P, i, Cx<R>{R(a.w.wf[0][0]), R(a.w.wf[0][1])});
-        hop<0, false, FMT>(acc, in, SS, nb.dn[0] * NRHS + rhs, a.gauge, S, Pn, nb.dn[0],
-                           Cx<R>{R(a.w.wb[0][0]), R(a.w.wb[0][1])});
-        hop<1, true, FMT>(acc, in, SS, nb.up[1] * NRHS + rhs, a.gauge, S, P, i, Cx<R>{R(a.w.wf[1][0]), R(a.w.wf[1][1])});
-        hop<1, false, FMT>(acc, in, SS, nb.dn[1] * NRHS + rhs, a.gauge, S, Pn, nb.dn[1],
-                           Cx<R>{R(a.w.wb[1][0]), R(a.w.wb[1][1])});
-        hop<2, true, FMT>(acc, in, SS, nb.up[2] * NRHS + rhs, a.gauge, S, P, i, Cx<R>{R(a.w.wf[2][0]), R(a.w.wf[2][1])});
-        hop<2, false, FMT>(acc, in, SS, nb.dn[2] * NRHS + rhs, a.gauge, S, Pn, nb.dn[2],
-                           Cx<R>{R(a.w.wb[2][0]), R(a.w.wb[2][1])});
-        hop<3, true, FMT>(acc, in, SS, nb.up[3] * NRHS + rhs, a.gauge, S, P, i, Cx<R>{R(a.w.wf[3][0]), R(a.w.wf[3][1])});
-        hop<3, false, FMT>(acc, in, SS, nb.dn[3] * NRHS + rhs, a.gauge, S, Pn, nb.dn[3],
-                           Cx<R>{R(a.w.wb[3][0]), R(a.w.wb[3][1])});
+#define CL2_W(t, m) Cx<R>{R(a.w.t[m][0]), R(a.w.t[m][1])}
+#define CL2_DIR(m)                                                                                                  \
+    if constexpr (OCC == 4)                                                                                         \
+        hop_seq<m, FMT>(acc, in, SS, nb.up[m] * NRHS + rhs, nb.dn[m] * NRHS + rhs, a.gauge, S, P, i, nb.dn[m],      \
+                        CL2_W(wf, m), CL2_W(wb, m));                                                                \
+    else                                                                                                            \
+        hop_pair<m, FMT>(acc, in, SS, nb.up[m] * NRHS + rhs, nb.dn[m] * NRHS + rhs, a.gauge, S, P, i, nb.dn[m],     \
+                         CL2_W(wf, m), CL2_W(wb, m));
+        CL2_DIR(0)
+        CL2_DIR(1)
+        CL2_DIR(2)
+        CL2_DIR(3)
+#undef CL2_DIR
+#undef CL2_W
         const uint32_t o = i * NRHS + rhs;
 
         const Cx<R> f1{R(a.f1[0]), R(a.f1[1])}, f2{R(a.f2[0]), R(a.f2[1])};
@@ -119,7 +169,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_dslash_eo(DslashArgs a)
         V* out = reinterpret_cast<V*>(a.out);
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            out[(size_t)k * SS + o] = V{acc[k].re, acc[k].im};
+            store_stream(out + (size_t)k * SS + o, V{acc[k].re, acc[k].im});
             nrm += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;
         }
     }
@@ -132,7 +182,7 @@ __global__ __launch_bounds__(BLOCK, MINW) void k_dslash_eo(DslashArgs a)
 /* Register-blocked multi-RHS variant: one thread per (site, group of RB right-hand sides); each link is
  * loaded and reconstructed once per thread and applied to RB neighbour spinors, so link loads and SU(3)
  * reconstruction are amortised over RB right-hand sides as well as shared across the NRHS/RB lanes of a
- * site.  Arithmetic per right-hand side is the single-field kernel's, in the same order. */
+ * site.  Per right-hand side it performs the single-field kernel's operations in the same order. */
 template<int A, int B, int CU, bool FWD, int RB, typename R>
 __device__ __forceinline__ void half_hop_rb(Cx<R> (&acc)[RB][12], const vec2<R>* __restrict__ in, size_t SS,
                                             uint32_t nb, const Su3<R>& u, Cx<R> w)
@@ -341,14 +391,21 @@ static hipError_t launch_dslash_n(cl2_ctx* c, const DslashArgs& a, int format)
             constexpr int F = decltype(fmt_tag)::value;
             const unsigned bs = (unsigned)a.block;
             const unsigned nb = blocks_for((size_t)a.g.veo, (int)bs);
-            if (bs == 128 && a.occ == 4)
-                hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 4>), dim3(nb), dim3(128), 0, c->stream, a);
-            else if (bs == 128)
+            // the 128-VGPR shapes exist for FP32 only: the FP64 kernels spill at that budget
+            if constexpr (F == GAUGE_F32_12) {
+                if (a.occ == 4 && bs == 128) {
+                    hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 4>), dim3(nb), dim3(128), 0, c->stream, a);
+                    return hipGetLastError();
+                }
+                if (a.occ == 4 && bs != 512) {
+                    hipLaunchKernelGGL((k_dslash_eo<F, 256, 1, 4>), dim3(nb), dim3(256), 0, c->stream, a);
+                    return hipGetLastError();
+                }
+            }
+            if (bs == 128)
                 hipLaunchKernelGGL((k_dslash_eo<F, 128, 1>), dim3(nb), dim3(128), 0, c->stream, a);
             else if (bs == 512)
                 hipLaunchKernelGGL((k_dslash_eo<F, 512, 1>), dim3(nb), dim3(512), 0, c->stream, a);
-            else if (a.occ == 4)
-                hipLaunchKernelGGL((k_dslash_eo<F, 256, 1, 4>), dim3(nb), dim3(256), 0, c->stream, a);
             else
                 hipLaunchKernelGGL((k_dslash_eo<F, 256, 1>), dim3(nb), dim3(256), 0, c->stream, a);
             return hipGetLastError();

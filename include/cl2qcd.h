@@ -355,8 +355,8 @@ long cl2_tune_get(const cl2_ctx* ctx, const char* key);
 
 /* No counterpart in the reference (which fixes work-group sizes per device in hardware/device.cpp:286-334):
  * autotune the hopping kernel's launch shape for (format, nrhs) on this context -- z-planes per slab, XCD
- * mapping, right-hand sides per thread, and for single fields the block size (128, 256, 512) with or
- * without a 4-waves-per-SIMD register budget -- by timing every candidate on scratch fields with the given
+ * mapping, right-hand sides per thread, and for single fields the block size (128, 256, 512), for FP32
+ * links also with a 4-waves-per-SIMD register budget -- by timing every candidate on scratch fields with the given
  * gauge field; the fastest is kept for later launches (unless cl2_tune_set fixed the shape) and returned
  * (block < 0: the register-budget variant of block size -block).  Output pointers may be null. */
 int cl2_autotune_dslash(cl2_ctx* ctx, const void* gauge, int format, int nrhs, double* best_ms, int* zplanes,
