@@ -60,7 +60,7 @@ def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src, autotune=True):
     kappa = 0.16
     # launch shape from the autotuner (z-planes per slab, XCD mapping, block size), timed on this field;
     # without it the library's default shape
-    tuned = [ctypes.c_double(), ctypes.c_int(-1), ctypes.c_int(1), ctypes.c_int(1), ctypes.c_int(256)]
+    tuned = [ctypes.c_double(), ctypes.c_int(-1), ctypes.c_int(1), ctypes.c_int(1), ctypes.c_int(256), ctypes.c_int(0)]
     if autotune:
         ctx.check(ctx.lib.cl2_autotune_dslash(ctx.handle, g.ptr, fmt, 1, *[ctypes.byref(t) for t in tuned]))
 
@@ -80,9 +80,8 @@ def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src, autotune=True):
     g.close()
     alg = ctx.lib.cl2_algorithmic_bytes(ctx.handle, b"dslash_eo", fmt)
     return dict(ms=ev_ms / steps, wall_ms=wall_ms / steps, alg_bytes=alg, GBps=alg * steps / (ev_ms * 1e6),
-                finite=finite, tuned={"zplanes": tuned[1].value, "xcd": tuned[2].value, "block": abs(tuned[4].value),
-                                      "min_waves_per_simd": 4 if tuned[4].value < 0 else None,
-                                      "autotuned": autotune})
+                finite=finite, tuned={"zplanes": tuned[1].value, "xcd": tuned[2].value, "block": tuned[4].value,
+                                      "occupancy": tuned[5].value, "autotuned": autotune})
 
 
 def cg_leg(ns, nt, sweeps, overrelax, seed):

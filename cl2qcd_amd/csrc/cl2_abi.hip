@@ -48,11 +48,12 @@ size_t su3vec_bytes(const cl2_ctx* c, int prec)
 }
 size_t gauge_bytes(const cl2_ctx* c, int fmt)
 {
-    const size_t links = (size_t)8 * c->g.veo;  // 4 dirs x 2 parities x Veo
+    // 4 dirs x 2 parities x Veo links; the FP64 and FP32 formats pad Veo to whole tiles of 64 sites
+    const size_t links = (size_t)8 * c->g.veo, tiled = (size_t)8 * link_sites(c->g.veo);
     switch (fmt) {
-        case CL2_GAUGE_F64_18: return links * 9 * 16;
-        case CL2_GAUGE_F64_12: return links * 6 * 16;
-        case CL2_GAUGE_F32_12: return links * 3 * 16;
+        case CL2_GAUGE_F64_18: return tiled * 9 * 16;
+        case CL2_GAUGE_F64_12: return tiled * 6 * 16;
+        case CL2_GAUGE_F32_12: return tiled * 3 * 16;
         case CL2_GAUGE_F16_12: return links * 3 * 8;
         default: return 0;
     }
@@ -521,7 +522,7 @@ int cl2_gauge_import(cl2_ctx* c, void* dev, const double* host, int fmt)
             for (int mu = 0; mu < 4; ++mu) {
                 const double* u = host + 18 * (mu + 4 * s);
                 for (int k = 0; k < 3; ++k) {
-                    float* d = &stage[4 * (((size_t)(mu * 2 + par) * 3 + k) * S + eo)];
+                    float* d = &stage[4 * link_elem(mu * 2 + par, 3, k, (uint32_t)eo, (uint32_t)S)];
                     d[0]     = (float)u[4 * k];
                     d[1]     = (float)u[4 * k + 1];
                     d[2]     = (float)u[4 * k + 2];
@@ -539,7 +540,7 @@ int cl2_gauge_import(cl2_ctx* c, void* dev, const double* host, int fmt)
             for (int mu = 0; mu < 4; ++mu) {
                 const double* u = host + 18 * (mu + 4 * s);
                 for (int k = 0; k < lanes; ++k) {
-                    double* d = &stage[2 * (((size_t)(mu * 2 + par) * lanes + k) * S + eo)];
+                    double* d = &stage[2 * link_elem(mu * 2 + par, lanes, k, (uint32_t)eo, (uint32_t)S)];
                     d[0]      = u[2 * k];
                     d[1]      = u[2 * k + 1];
                 }
@@ -613,16 +614,18 @@ int cl2_gauge_export(cl2_ctx* c, double* host, const void* dev, int fmt)
     if (fmt == CL2_GAUGE_F16_12) return fail(c, CL2_E_UNSUPPORTED, "export of F16_12 links is not supported");
     const size_t S = c->g.veo, V = 2 * S;
     const int lanes = fmt == CL2_GAUGE_F64_18 ? 9 : 6;
-    std::vector<double> stage(2 * lanes * 8 * S);
+    // staging in the device layout of the FP64 format with the same rows (FP32 12-real -> FP64 12-real)
+    std::vector<double> stage(gauge_bytes(c, lanes == 9 ? CL2_GAUGE_F64_18 : CL2_GAUGE_F64_12) / sizeof(double));
     if (fmt == CL2_GAUGE_F32_12) {
         std::vector<float> f(gauge_bytes(c, fmt) / sizeof(float));
         CL2_HIP(c, hipMemcpyAsync(f.data(), dev, f.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
         CL2_HIP(c, hipStreamSynchronize(c->stream));
-        for (size_t dp = 0; dp < 8; ++dp)
-            for (int k = 0; k < 3; ++k)
-                for (size_t eo = 0; eo < S; ++eo)
+        for (uint32_t dp = 0; dp < 8; ++dp)
+            for (uint32_t k = 0; k < 3; ++k)
+                for (uint32_t eo = 0; eo < (uint32_t)S; ++eo)
                     for (int q = 0; q < 4; ++q)
-                        stage[2 * ((dp * 6 + 2 * k + q / 2) * S + eo) + (q & 1)] = f[4 * ((dp * 3 + k) * S + eo) + q];
+                        stage[2 * link_elem(dp, 6, 2 * k + q / 2, eo, (uint32_t)S) + (q & 1)] =
+                            f[4 * link_elem(dp, 3, k, eo, (uint32_t)S) + q];
     } else {
         CL2_HIP(c, hipMemcpyAsync(stage.data(), dev, stage.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         CL2_HIP(c, hipStreamSynchronize(c->stream));
@@ -633,7 +636,7 @@ int cl2_gauge_export(cl2_ctx* c, double* host, const void* dev, int fmt)
         for (int mu = 0; mu < 4; ++mu) {
             double* u = host + 18 * (mu + 4 * s);
             for (int k = 0; k < lanes; ++k) {
-                const double* d = &stage[2 * (((size_t)(mu * 2 + par) * lanes + k) * S + eo)];
+                const double* d = &stage[2 * link_elem(mu * 2 + par, lanes, k, (uint32_t)eo, (uint32_t)S)];
                 u[2 * k]        = d[0];
                 u[2 * k + 1]    = d[1];
             }
@@ -1267,6 +1270,18 @@ int cl2_tune_set(cl2_ctx* c, const char* key, long value)
         c->tune_manual  = true;
         return CL2_OK;
     }
+    if (k == "hop.block") {
+        CL2_REQUIRE(c, value == 128 || value == 256 || value == 512, "hop.block must be 128, 256 or 512");
+        c->tune_block  = (int)value;
+        c->tune_manual = true;
+        return CL2_OK;
+    }
+    if (k == "hop.occupancy") {
+        CL2_REQUIRE(c, value == -1 || value == 0 || value == 1 || value == 4, "hop.occupancy must be -1, 0, 1 or 4");
+        c->tune_occ    = (int)value;
+        c->tune_manual = true;
+        return CL2_OK;
+    }
     return fail(c, CL2_E_ARG, "unknown tuning key " + k);
 }
 
@@ -1277,6 +1292,8 @@ long cl2_tune_get(const cl2_ctx* c, const char* key)
     if (k == "hop.zplanes") return c->tune_zplanes;
     if (k == "hop.xcd") return c->tune_xcd;
     if (k == "mrhs.rb") return c->tune_mrhs_rb;
+    if (k == "hop.block") return c->tune_block;
+    if (k == "hop.occupancy") return c->tune_occ;
     return -2;
 }
 
@@ -1285,7 +1302,7 @@ long cl2_tune_get(const cl2_ctx* c, const char* key)
  * the fastest is stored for (format, nrhs) and used by every later launch on this context unless
  * cl2_tune_set has fixed the shape by hand. */
 int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double* best_ms, int* zplanes, int* xcd,
-                        int* rb, int* block)
+                        int* rb, int* block, int* occupancy)
 {
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, gauge && valid_fmt(fmt), "bad gauge argument");
@@ -1309,12 +1326,12 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     // (block, occupancy hint) shapes of the single-field kernel
     std::vector<std::pair<int, int>> shapes = {{256, 0}};
     if (nrhs == 1 && fmt == CL2_GAUGE_F32_12) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 4}, {128, 4}};
-    else if (nrhs == 1 && fmt != CL2_GAUGE_F16_12) shapes = {{256, 0}, {128, 0}, {512, 0}};
+    else if (nrhs == 1 && fmt != CL2_GAUGE_F16_12) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 1}, {128, 1}};
     const bool manual = c->tune_manual;
     c->tune_manual    = false;
     const double one[2] = {1., 0.};
     double best         = 1e30;
-    cl2_hop_tuning bt{-1, 1, 1, DSLASH_BLOCK, 0};
+    cl2_hop_tuning bt{-1, 1, 1, DSLASH_BLOCK, -1};
     int rc = CL2_OK;
     for (int zp : zps)
         for (int x : {1, 0})
@@ -1349,7 +1366,8 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     if (zplanes) *zplanes = bt.zplanes;
     if (xcd) *xcd = bt.xcd;
     if (rb) *rb = bt.rb;
-    if (block) *block = bt.occ ? -bt.block : bt.block;
+    if (block) *block = bt.block;
+    if (occupancy) *occupancy = bt.occ;
     return CL2_OK;
 }
 

@@ -33,25 +33,32 @@ __device__ __forceinline__ Cx<R> cx(R a, R b)
 {
     return Cx<R>{a, b};
 }
+/* The arithmetic helpers below fix their own rounding: no implicit contraction (the pragma) and fused
+ * multiply-adds only where written, so every kernel built from them computes the same operations in the
+ * same order whatever the surrounding code -- results do not depend on launch shape or kernel variant. */
 template<typename R>
 __device__ __forceinline__ Cx<R> operator+(Cx<R> a, Cx<R> b)
 {
+#pragma clang fp contract(off)
     return Cx<R>{a.re + b.re, a.im + b.im};
 }
 template<typename R>
 __device__ __forceinline__ Cx<R> operator-(Cx<R> a, Cx<R> b)
 {
+#pragma clang fp contract(off)
     return Cx<R>{a.re - b.re, a.im - b.im};
 }
 template<typename R>
 __device__ __forceinline__ Cx<R> cmul(Cx<R> a, Cx<R> b)
 {
-    return Cx<R>{a.re * b.re - a.im * b.im, a.im * b.re + a.re * b.im};
+#pragma clang fp contract(off)
+    return Cx<R>{fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re)};
 }
 template<typename R>
 __device__ __forceinline__ Cx<R> cmulconj(Cx<R> a, Cx<R> b)
 {  // a * conj(b)
-    return Cx<R>{a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+#pragma clang fp contract(off)
+    return Cx<R>{fma(a.re, b.re, a.im * b.im), fma(a.im, b.re, -(a.re * b.im))};
 }
 template<typename R>
 __device__ __forceinline__ Cx<R> conj(Cx<R> a)
@@ -61,6 +68,7 @@ __device__ __forceinline__ Cx<R> conj(Cx<R> a)
 template<typename R>
 __device__ __forceinline__ Cx<R> scale(Cx<R> a, R s)
 {
+#pragma clang fp contract(off)
     return Cx<R>{a.re * s, a.im * s};
 }
 
@@ -88,6 +96,7 @@ struct UnitConj<MI> {
 template<int U, typename R>
 __device__ __forceinline__ Cx<R> add_unit(Cx<R> a, Cx<R> b)
 {  // a + U*b
+#pragma clang fp contract(off)
     if constexpr (U == P1) return Cx<R>{a.re + b.re, a.im + b.im};
     if constexpr (U == M1) return Cx<R>{a.re - b.re, a.im - b.im};
     if constexpr (U == PI) return Cx<R>{a.re - b.im, a.im + b.re};
@@ -99,31 +108,44 @@ struct Su3 {
     Cx<R> e[3][3];
 };
 
+/* r = U v and r = U^dagger v, each row an FMA chain over the three columns */
 template<typename R>
 __device__ __forceinline__ void su3_times(const Su3<R>& u, const Cx<R> (&v)[3], Cx<R> (&r)[3])
 {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        R re = u.e[i][0].re * v[0].re - u.e[i][0].im * v[0].im;
-        R im = u.e[i][0].re * v[0].im + u.e[i][0].im * v[0].re;
-        re += u.e[i][1].re * v[1].re - u.e[i][1].im * v[1].im;
-        im += u.e[i][1].re * v[1].im + u.e[i][1].im * v[1].re;
-        re += u.e[i][2].re * v[2].re - u.e[i][2].im * v[2].im;
-        im += u.e[i][2].re * v[2].im + u.e[i][2].im * v[2].re;
+        R re = u.e[i][0].re * v[0].re;
+        R im = u.e[i][0].re * v[0].im;
+        re   = fma(-u.e[i][0].im, v[0].im, re);
+        im   = fma(u.e[i][0].im, v[0].re, im);
+#pragma unroll
+        for (int j = 1; j < 3; ++j) {
+            re = fma(u.e[i][j].re, v[j].re, re);
+            im = fma(u.e[i][j].re, v[j].im, im);
+            re = fma(-u.e[i][j].im, v[j].im, re);
+            im = fma(u.e[i][j].im, v[j].re, im);
+        }
         r[i] = Cx<R>{re, im};
     }
 }
 template<typename R>
 __device__ __forceinline__ void su3dag_times(const Su3<R>& u, const Cx<R> (&v)[3], Cx<R> (&r)[3])
 {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        R re = u.e[0][i].re * v[0].re + u.e[0][i].im * v[0].im;
-        R im = u.e[0][i].re * v[0].im - u.e[0][i].im * v[0].re;
-        re += u.e[1][i].re * v[1].re + u.e[1][i].im * v[1].im;
-        im += u.e[1][i].re * v[1].im - u.e[1][i].im * v[1].re;
-        re += u.e[2][i].re * v[2].re + u.e[2][i].im * v[2].im;
-        im += u.e[2][i].re * v[2].im - u.e[2][i].im * v[2].re;
+        R re = u.e[0][i].re * v[0].re;
+        R im = u.e[0][i].re * v[0].im;
+        re   = fma(u.e[0][i].im, v[0].im, re);
+        im   = fma(-u.e[0][i].im, v[0].re, im);
+#pragma unroll
+        for (int j = 1; j < 3; ++j) {
+            re = fma(u.e[j][i].re, v[j].re, re);
+            im = fma(u.e[j][i].re, v[j].im, im);
+            re = fma(u.e[j][i].im, v[j].im, re);
+            im = fma(-u.e[j][i].im, v[j].re, im);
+        }
         r[i] = Cx<R>{re, im};
     }
 }
@@ -266,8 +288,9 @@ __device__ __forceinline__ void store_stream(float2* p, float2 v)
 }
 
 /* ---------------- gauge-link formats ----------------
- * Device layout [dir][parity][lane][gstride] where lane = complex entry (18-real)
- * or pairs of complex entries packed into a 16-byte vector (12-real, FP32). */
+ * Device layout: link_elem (tiles of 64 eo sites per (dir, parity), lane-major inside a tile), where a
+ * lane is one complex entry (FP64) or a pair of complex entries packed into a 16-byte vector (FP32);
+ * gstride is Veo. */
 template<int FMT>
 struct GaugeLoad;
 
@@ -278,10 +301,10 @@ struct GaugeLoad<GAUGE_F64_18> {
     __device__ static __forceinline__ void load(const void* g, uint32_t gstride, int dir, int par, uint32_t eo,
                                                 Su3<double>& u)
     {
-        const double2* p = reinterpret_cast<const double2*>(g) + (size_t)((dir * 2 + par) * 9) * gstride + eo;
+        const double2* p = reinterpret_cast<const double2*>(g) + link_elem(dir * 2 + par, 9, 0, eo, gstride);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
-            double2 v           = load_stream(p + (size_t)k * gstride);
+            double2 v           = load_stream(p + k * LINK_TILE);
             u.e[k / 3][k % 3] = Cx<double>{v.x, v.y};
         }
     }
@@ -293,10 +316,10 @@ struct GaugeLoad<GAUGE_F64_12> {
     __device__ static __forceinline__ void load(const void* g, uint32_t gstride, int dir, int par, uint32_t eo,
                                                 Su3<double>& u)
     {
-        const double2* p = reinterpret_cast<const double2*>(g) + (size_t)((dir * 2 + par) * 6) * gstride + eo;
+        const double2* p = reinterpret_cast<const double2*>(g) + link_elem(dir * 2 + par, 6, 0, eo, gstride);
 #pragma unroll
         for (int k = 0; k < 6; ++k) {
-            double2 v           = load_stream(p + (size_t)k * gstride);
+            double2 v           = load_stream(p + k * LINK_TILE);
             u.e[k / 3][k % 3] = Cx<double>{v.x, v.y};
         }
         su3_reconstruct_row2(u);
@@ -309,10 +332,10 @@ struct GaugeLoad<GAUGE_F32_12> {
     __device__ static __forceinline__ void load(const void* g, uint32_t gstride, int dir, int par, uint32_t eo,
                                                 Su3<float>& u)
     {
-        const float4* p = reinterpret_cast<const float4*>(g) + (size_t)((dir * 2 + par) * 3) * gstride + eo;
+        const float4* p = reinterpret_cast<const float4*>(g) + link_elem(dir * 2 + par, 3, 0, eo, gstride);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            float4 v                      = load_stream(p + (size_t)k * gstride);
+            float4 v                      = load_stream(p + k * LINK_TILE);
             u.e[(2 * k) / 3][(2 * k) % 3] = Cx<float>{v.x, v.y};
             u.e[(2 * k + 1) / 3][(2 * k + 1) % 3] = Cx<float>{v.z, v.w};
         }

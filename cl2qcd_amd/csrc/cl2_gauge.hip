@@ -67,15 +67,15 @@ __global__ void k_gauge_random(void* dev, int fmt, uint32_t veo, uint64_t seed, 
     if (fmt == GAUGE_F16_12) {
         store_link_h(dev, dp, veo, eo, u);
     } else if (fmt == GAUGE_F32_12) {
-        float4* q = reinterpret_cast<float4*>(dev) + (size_t)(dp * 3) * veo + eo;
+        float4* q = reinterpret_cast<float4*>(dev) + link_elem(dp, 3, 0, eo, veo);
         for (int k = 0; k < 3; ++k) {
             Cx<double> a = u.e[(2 * k) / 3][(2 * k) % 3], b = u.e[(2 * k + 1) / 3][(2 * k + 1) % 3];
-            q[(size_t)k * veo] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
+            q[k * LINK_TILE] = make_float4((float)a.re, (float)a.im, (float)b.re, (float)b.im);
         }
     } else {
         const int lanes = (fmt == GAUGE_F64_18) ? 9 : 6;
-        double2* q      = reinterpret_cast<double2*>(dev) + (size_t)(dp * lanes) * veo + eo;
-        for (int k = 0; k < lanes; ++k) q[(size_t)k * veo] = make_double2(u.e[k / 3][k % 3].re, u.e[k / 3][k % 3].im);
+        double2* q      = reinterpret_cast<double2*>(dev) + link_elem(dp, lanes, 0, eo, veo);
+        for (int k = 0; k < lanes; ++k) q[k * LINK_TILE] = make_double2(u.e[k / 3][k % 3].re, u.e[k / 3][k % 3].im);
     }
 }
 
@@ -94,21 +94,21 @@ __global__ void k_gauge_convert(void* dst, int dfmt, const double2* __restrict__
     const uint64_t link = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (link >= 8ull * veo) return;
     const uint32_t dp = (uint32_t)(link / veo), eo = (uint32_t)(link % veo);
-    const double2* s  = src + (size_t)(dp * 9) * veo + eo;
+    const double2* s  = src + link_elem(dp, 9, 0, eo, veo);
     if (dfmt == GAUGE_F16_12) {
         Su3<double> u;
-        for (int k = 0; k < 6; ++k) u.e[k / 3][k % 3] = Cx<double>{s[(size_t)k * veo].x, s[(size_t)k * veo].y};
+        for (int k = 0; k < 6; ++k) u.e[k / 3][k % 3] = Cx<double>{s[k * LINK_TILE].x, s[k * LINK_TILE].y};
         store_link_h(dst, dp, veo, eo, u);
     } else if (dfmt == GAUGE_F32_12) {
-        float4* q = reinterpret_cast<float4*>(dst) + (size_t)(dp * 3) * veo + eo;
+        float4* q = reinterpret_cast<float4*>(dst) + link_elem(dp, 3, 0, eo, veo);
         for (int k = 0; k < 3; ++k) {
-            double2 a = s[(size_t)(2 * k) * veo], b = s[(size_t)(2 * k + 1) * veo];
-            q[(size_t)k * veo] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+            double2 a = s[(2 * k) * LINK_TILE], b = s[(2 * k + 1) * LINK_TILE];
+            q[k * LINK_TILE] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
         }
     } else {
         const int lanes = (dfmt == GAUGE_F64_18) ? 9 : 6;
-        double2* q      = reinterpret_cast<double2*>(dst) + (size_t)(dp * lanes) * veo + eo;
-        for (int k = 0; k < lanes; ++k) q[(size_t)k * veo] = s[(size_t)k * veo];
+        double2* q      = reinterpret_cast<double2*>(dst) + link_elem(dp, lanes, 0, eo, veo);
+        for (int k = 0; k < lanes; ++k) q[k * LINK_TILE] = s[k * LINK_TILE];
     }
 }
 

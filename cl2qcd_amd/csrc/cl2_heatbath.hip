@@ -46,14 +46,15 @@ __device__ __forceinline__ const double2* link_ptr(const double2* U, const Latti
 {
     const uint32_t site = a.c[1] + g.ns * (a.c[2] + g.ns * (a.c[3] + g.ns * a.c[0]));
     const uint32_t par  = (a.c[0] + a.c[1] + a.c[2] + a.c[3]) & 1u;
-    return U + (size_t)((dir * 2 + par) * 9) * g.veo + (site >> 1);
+    return U + link_elem(dir * 2 + par, 9, 0, site >> 1, g.veo);
 }
 
-__device__ __forceinline__ void load(const double2* p, uint32_t veo, Su3<double>& u)
+/* the 9 lanes of the link at p (tile stride LINK_TILE) */
+__device__ __forceinline__ void load(const double2* p, uint32_t, Su3<double>& u)
 {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        double2 v         = p[(size_t)k * veo];
+        double2 v         = p[k * LINK_TILE];
         u.e[k / 3][k % 3] = Cx<double>{v.x, v.y};
     }
 }
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(128) void k_gauge_update(double2* U, LatticeGeom g,
     }
     reunitarise(u);
 #pragma unroll
-    for (int q = 0; q < 9; ++q) up[(size_t)q * g.veo] = make_double2(u.e[q / 3][q % 3].re, u.e[q / 3][q % 3].im);
+    for (int q = 0; q < 9; ++q) up[q * LINK_TILE] = make_double2(u.e[q / 3][q % 3].re, u.e[q / 3][q % 3].im);
 }
 
 /* per-block sum over sites of sum_{mu<nu} Re tr P_mu,nu(x); grid-stride over both parities */

@@ -16,7 +16,8 @@ struct cl2_hop_tuning {
     int xcd;      // XCD-contiguous block assignment
     int rb;       // right-hand sides per thread of the multi-RHS kernel
     int block;    // threads per block of the single-field kernel
-    int occ;      // occupancy hint of the single-field kernel (0 or 4 waves per SIMD)
+    int occ;      // register budget of the single-field kernel: 0 (<= 2 waves per SIMD, hop pairs), 1 (1 wave,
+                  // four hops in flight), 4 (>= 4 waves, one hop at a time); -1: the format's default
 };
 
 struct cl2_ctx {
@@ -35,6 +36,8 @@ struct cl2_ctx {
     int tune_zplanes = -1;  // z-planes per slab of the hopping kernels (-1: heuristic, 0: identity order)
     int tune_xcd     = 1;   // XCD-contiguous block assignment
     int tune_mrhs_rb = 0;   // right-hand sides per thread of the multi-RHS hopping kernel (0: by format)
+    int tune_block   = 256; // threads per block of the single-field hopping kernel
+    int tune_occ     = -1;  // register budget of the single-field hopping kernel (-1: the format's default)
     bool tune_manual = false;  // set by cl2_tune_set: the values above override autotuned ones
     std::map<std::pair<int, int>, cl2_hop_tuning> tuned;  // (link format, right-hand sides) -> autotuned shape
     std::string err;

@@ -21,6 +21,24 @@ struct LatticeGeom {
     uint32_t veo;  // ns^3*nt/2, also the field stride (no padding)
 };
 
+/* Layout of the FP64 and FP32 gauge-link fields.  For each (direction, parity) pair dp = 2 dir + parity
+ * the LANES 16-byte lanes of a link (9 complex entries for 18-real FP64; 6 for 12-real FP64; 3 float4
+ * holding 6 complex entries for 12-real FP32) are stored in tiles of 64 eo sites, lane-major inside a
+ * tile: element (dp, lane k, eo site e) sits at
+ *   dp * LANES * Vp + (e / 64) * LANES * 64 + k * 64 + e % 64,    Vp = Veo rounded up to 64,
+ * so one wavefront reading the link of 64 consecutive sites reads one contiguous LANES x 1 KiB block
+ * instead of LANES blocks a field apart. */
+constexpr uint32_t LINK_TILE = 64;
+__host__ __device__ inline uint32_t link_sites(uint32_t veo)
+{
+    return (veo + LINK_TILE - 1) / LINK_TILE * LINK_TILE;
+}
+__host__ __device__ inline size_t link_elem(uint32_t dp, uint32_t lanes, uint32_t k, uint32_t eo, uint32_t veo)
+{
+    return (size_t)dp * lanes * link_sites(veo) + (size_t)(eo / LINK_TILE) * lanes * LINK_TILE +
+           (size_t)k * LINK_TILE + eo % LINK_TILE;
+}
+
 /* Work order of the hopping kernels.  Logical work item l enumerates
  * (z-chunk, t, site-in-slab) with the site fastest, then t, then the z-chunk;
  * a slab is zplanes consecutive z-planes of one t-slice (P = zplanes*ns^2/2 eo
@@ -66,7 +84,7 @@ struct DslashArgs {
     int nrhs;  // right-hand sides per launch (fields [12][Veo][nrhs]); 1 for single fields
     int rb;    // right-hand sides per thread of the multi-RHS kernel
     int block; // threads per block of the single-field kernel (128, 256 or 512)
-    int occ;   // 4: single-field kernel compiled for at least 4 waves per SIMD (<= 128 VGPRs), else 0
+    int occ;   // register budget of the single-field kernel: 0 (<= 2 waves per SIMD), 1 (1 wave), 4 (>= 4 waves)
 };
 
 }  // namespace cl2

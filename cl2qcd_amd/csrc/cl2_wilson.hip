@@ -69,6 +69,33 @@ __device__ __forceinline__ void apply_hop(Cx<R> (&acc)[12], const vec2<R> (&s)[1
 /* forward and backward hop of direction DIR: both links and both neighbour spinors (elements nf, nbk
  * of the spinor field) are loaded before either hop is applied.  The forward link is the output
  * site's own (eo index i, parity P), the backward one the neighbour's (eo index linkbk, parity P^1). */
+/* forward and backward hops of directions D0 and D1: four links and four neighbour spinors in flight
+ * before any hop is applied (the 1-wave-per-SIMD variants, which may also stage in AGPRs) */
+template<int D0, int D1, int FMT, int NRHS, typename R>
+__device__ __forceinline__ void hop_quad(Cx<R> (&acc)[12], const vec2<R>* __restrict__ in, size_t SS, const SiteNb& nb,
+                                         uint32_t rhs, const void* gauge, uint32_t S, int P, uint32_t i,
+                                         const HopWeights<double>& w)
+{
+    Su3<R> u0f, u0b, u1f, u1b;
+    vec2<R> s0f[12], s0b[12], s1f[12], s1b[12];
+    GaugeLoad<FMT>::load(gauge, S, D0, P, i, u0f);
+    GaugeLoad<FMT>::load(gauge, S, D0, P ^ 1, nb.dn[D0], u0b);
+    GaugeLoad<FMT>::load(gauge, S, D1, P, i, u1f);
+    GaugeLoad<FMT>::load(gauge, S, D1, P ^ 1, nb.dn[D1], u1b);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s0f[k] = in[(size_t)k * SS + nb.up[D0] * NRHS + rhs];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s0b[k] = in[(size_t)k * SS + nb.dn[D0] * NRHS + rhs];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s1f[k] = in[(size_t)k * SS + nb.up[D1] * NRHS + rhs];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s1b[k] = in[(size_t)k * SS + nb.dn[D1] * NRHS + rhs];
+    apply_hop<D0, true>(acc, s0f, u0f, Cx<R>{R(w.wf[D0][0]), R(w.wf[D0][1])});
+    apply_hop<D0, false>(acc, s0b, u0b, Cx<R>{R(w.wb[D0][0]), R(w.wb[D0][1])});
+    apply_hop<D1, true>(acc, s1f, u1f, Cx<R>{R(w.wf[D1][0]), R(w.wf[D1][1])});
+    apply_hop<D1, false>(acc, s1b, u1b, Cx<R>{R(w.wb[D1][0]), R(w.wb[D1][1])});
+}
+
 /* the same two hops one after the other, for the 128-VGPR variants: each loads only its own link and
  * neighbour spinor */
 template<int DIR, int FMT, typename R>
@@ -114,10 +141,13 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
 /* NRHS right-hand sides in one launch: the fields are [12][Veo][NRHS] (right-hand side fastest) and
  * thread l works on site l / NRHS, right-hand side l % NRHS, so the NRHS lanes of one site issue the same
  * link addresses and each link leaves HBM once per launch.  NRHS = 1 is the single-field kernel. */
-/* OCC = 4: compiled for at least 4 waves per SIMD (at most 128 VGPRs); otherwise at most 2 waves per
- * SIMD, which leaves up to 256 VGPRs for the loads of a hop pair in flight */
+/* Register budget and hops in flight per thread:
+ *   OCC = 0: at most 2 waves per SIMD (up to 256 VGPRs), the two hops of one direction in flight;
+ *   OCC = 1: 1 wave per SIMD, the four hops of two directions in flight (blocks of at most 256);
+ *   OCC = 4: at least 4 waves per SIMD (at most 128 VGPRs), one hop at a time. */
 template<int FMT, int BLOCK, int NRHS, int OCC = 0>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(OCC == 4 ? 4 : 1, OCC == 4 ? 8 : 2)))
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(OCC == 4 ? 4 : 1, OCC == 4 ? 8 : (OCC == 1 ? 1 : 2))))
 void k_dslash_eo(DslashArgs a)
 {
     using R = typename GaugeLoad<FMT>::R;
@@ -144,10 +174,15 @@ void k_dslash_eo(DslashArgs a)
     else                                                                                                            \
         hop_pair<m, FMT>(acc, in, SS, nb.up[m] * NRHS + rhs, nb.dn[m] * NRHS + rhs, a.gauge, S, P, i, nb.dn[m],     \
                          CL2_W(wf, m), CL2_W(wb, m));
-        CL2_DIR(0)
-        CL2_DIR(1)
-        CL2_DIR(2)
-        CL2_DIR(3)
+        if constexpr (OCC == 1) {
+            hop_quad<0, 1, FMT, NRHS>(acc, in, SS, nb, rhs, a.gauge, S, P, i, a.w);
+            hop_quad<2, 3, FMT, NRHS>(acc, in, SS, nb, rhs, a.gauge, S, P, i, a.w);
+        } else {
+            CL2_DIR(0)
+            CL2_DIR(1)
+            CL2_DIR(2)
+            CL2_DIR(3)
+        }
 #undef CL2_DIR
 #undef CL2_W
         const uint32_t o = i * NRHS + rhs;
@@ -309,7 +344,9 @@ cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs)
         auto it = c->tuned.find({fmt, nrhs});
         if (it != c->tuned.end()) return it->second;
     }
-    return cl2_hop_tuning{c->tune_zplanes, c->tune_xcd, c->tune_mrhs_rb, DSLASH_BLOCK, 0};
+    // default register budget: four hops in flight for the FP64 links, hop pairs for FP32 (measured at 48^3x96)
+    const int occ = c->tune_occ >= 0 ? c->tune_occ : ((fmt == GAUGE_F64_18 || fmt == GAUGE_F64_12) ? 1 : 0);
+    return cl2_hop_tuning{c->tune_zplanes, c->tune_xcd, c->tune_mrhs_rb, c->tune_block, occ};
 }
 
 WorkMap work_map(const cl2_ctx* c, int block, int nrhs, const cl2_hop_tuning& t)
@@ -391,7 +428,18 @@ static hipError_t launch_dslash_n(cl2_ctx* c, const DslashArgs& a, int format)
             constexpr int F = decltype(fmt_tag)::value;
             const unsigned bs = (unsigned)a.block;
             const unsigned nb = blocks_for((size_t)a.g.veo, (int)bs);
-            // the 128-VGPR shapes exist for FP32 only: the FP64 kernels spill at that budget
+            // the 1-wave-per-SIMD shapes exist for the FP64 formats (blocks of 128 or 256), the 128-VGPR
+            // shapes for FP32 only (the FP64 kernels spill at that budget)
+            if constexpr (F != GAUGE_F32_12) {
+                if (a.occ == 1 && bs == 128) {
+                    hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 1>), dim3(nb), dim3(128), 0, c->stream, a);
+                    return hipGetLastError();
+                }
+                if (a.occ == 1 && bs != 512) {
+                    hipLaunchKernelGGL((k_dslash_eo<F, 256, 1, 1>), dim3(nb), dim3(256), 0, c->stream, a);
+                    return hipGetLastError();
+                }
+            }
             if constexpr (F == GAUGE_F32_12) {
                 if (a.occ == 4 && bs == 128) {
                     hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 4>), dim3(nb), dim3(128), 0, c->stream, a);

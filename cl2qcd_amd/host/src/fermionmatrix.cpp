@@ -183,7 +183,7 @@ double autotune_dslash(const Gaugefield& gf, const hardware::System& system, int
 {
     const Links l = links_for(gf, prec);
     double ms     = 0.;
-    system.check(cl2_autotune_dslash(system.ctx(), l.ptr, l.fmt, nrhs, &ms, nullptr, nullptr, nullptr, nullptr));
+    system.check(cl2_autotune_dslash(system.ctx(), l.ptr, l.fmt, nrhs, &ms, nullptr, nullptr, nullptr, nullptr, nullptr));
     return ms;
 }
 

@@ -8,15 +8,16 @@
  * Each declaration names the reference interface it replaces (paths relative
  * to the CL2QCD source tree).
  *
- * Device layouts (all fields are dense, stride = number of eo sites Veo):
+ * Device layouts (spinor fields are dense, stride = number of eo sites Veo):
  *   Wilson eo spinor  F64: double2[12][Veo]  component c = 3*spin + colour
  *                     F32: float2 [12][Veo]
  *                     F16: half[3][Veo][8] (components 4j..4j+3 as re,im pairs) then float norm[Veo];
  *                          component = half * norm (norm = largest |re|,|im| on the site)
  *   staggered eo vec  F64: double2[3][Veo],  F32: float2[3][Veo]
- *   gauge field      F64_18: double2[4 dir][2 parity][9][Veo]   (row-major 3x3)
- *                    F64_12: double2[4][2][6][Veo]   rows 0,1; row 2 = conj(row0 x row1)
- *                    F32_12: float4 [4][2][3][Veo]   rows 0,1 packed two entries per float4
+ *   gauge field      F64_18: double2[4 dir][2 parity][Vp/64][9][64]   (9 lanes = row-major 3x3)
+ *                    F64_12: double2[4][2][Vp/64][6][64]   rows 0,1; row 2 = conj(row0 x row1)
+ *                    F32_12: float4 [4][2][Vp/64][3][64]   rows 0,1 packed two entries per float4
+ *                            (tiles of 64 eo sites, lane-major inside a tile; Vp = Veo rounded up to 64)
  *                    F16_12: per (dir, parity): half[Veo][8] (entries 0..3 of rows 0,1), then half[Veo][4]
  *                            (entries 4, 5); 24 B per link
  *   device scalars: complex = 2 doubles (re, im); real = 1 double.
@@ -349,18 +350,25 @@ int cl2_cg_beta(cl2_ctx* ctx, double* mbeta, double* r2, const double* norms);
  *      hardware/device_info.cpp:111-128 and hardware/code/opencl_module.cpp get_work_sizes) ==== */
 /* keys: "hop.zplanes" -- z-planes per slab of the t-fastest traversal of the hopping kernels
  *       (-1 heuristic, 0 lexicographic order, else a divisor of ns);
- *       "hop.xcd" -- 1: give each XCD a contiguous range of blocks, 0: hardware round robin. */
+ *       "hop.xcd" -- 1: give each XCD a contiguous range of blocks, 0: hardware round robin;
+ *       "hop.block" -- threads per block of the single-field hopping kernel (128, 256, 512);
+ *       "hop.occupancy" -- its register budget: 0 (<= 2 waves per SIMD, the two hops of a direction in
+ *       flight), 1 (one wave per SIMD, four hops in flight; FP64 links), 4 (>= 4 waves per SIMD, one hop at a
+ *       time; FP32 links), -1 the format's default (1 for FP64 links, 0 for FP32);
+ *       "mrhs.rb" -- right-hand sides per thread of the multi-right-hand-side kernel (0: by format, 1, 2, 4).
+ * Every setting changes only which thread computes which site and when: results are bit identical. */
 int cl2_tune_set(cl2_ctx* ctx, const char* key, long value);
 long cl2_tune_get(const cl2_ctx* ctx, const char* key);
 
 /* No counterpart in the reference (which fixes work-group sizes per device in hardware/device.cpp:286-334):
  * autotune the hopping kernel's launch shape for (format, nrhs) on this context -- z-planes per slab, XCD
- * mapping, right-hand sides per thread, and for single fields the block size (128, 256, 512), for FP32
- * links also with a 4-waves-per-SIMD register budget -- by timing every candidate on scratch fields with the given
- * gauge field; the fastest is kept for later launches (unless cl2_tune_set fixed the shape) and returned
- * (block < 0: the register-budget variant of block size -block).  Output pointers may be null. */
+ * mapping, right-hand sides per thread, and for single fields the block size (128, 256, 512) and register
+ * budget -- by timing every candidate on scratch fields with the given gauge field; the fastest is kept for
+ * later launches (unless cl2_tune_set fixed the shape) and returned.  occupancy: 0 = up to 2 waves per SIMD
+ * with the two hops of a direction in flight; 1 = one wave per SIMD with four hops in flight (FP64 links);
+ * 4 = at least 4 waves per SIMD, one hop at a time (FP32 links).  Output pointers may be null. */
 int cl2_autotune_dslash(cl2_ctx* ctx, const void* gauge, int format, int nrhs, double* best_ms, int* zplanes,
-                        int* xcd, int* rhs_per_thread, int* block);
+                        int* xcd, int* rhs_per_thread, int* block, int* occupancy);
 
 /* ==== timing on the context stream (Device profiling events, hardware/device.cpp:233-242) ==== */
 int cl2_timer_start(cl2_ctx* ctx);

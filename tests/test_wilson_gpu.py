@@ -218,7 +218,8 @@ def test_rejects_bad_arguments():
 @pytest.mark.parametrize("fmt,prec,nrhs", [(abi.GAUGE_F64_18, abi.F64, 1), (abi.GAUGE_F32_12, abi.F32, 1),
                                            (abi.GAUGE_F16_12, abi.F16, 1), (abi.GAUGE_F32_12, abi.F32, 4)])
 def test_autotune_keeps_results(fmt, prec, nrhs):
-    """The autotuned launch shape only changes which thread runs which site, so results stay bit identical."""
+    """The autotuned launch shape changes which thread runs which site and how many hops a thread has in
+    flight, not the arithmetic, so results stay bit identical."""
     import ctypes
     lat, ctx = make(8, 8, theta_t=1.0)
     with ctx:
@@ -251,9 +252,40 @@ def test_autotune_keeps_results(fmt, prec, nrhs):
             return np.stack(res)
 
         before = run()
-        ms, zp, xcd, rb, blk = ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms, zp, xcd, rb, blk, occ = (ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(),
+                                     ctypes.c_int())
         ctx.autotune_dslash(gf.ptr, fmt, nrhs, ctypes.byref(ms), ctypes.byref(zp), ctypes.byref(xcd),
-                            ctypes.byref(rb), ctypes.byref(blk))
+                            ctypes.byref(rb), ctypes.byref(blk), ctypes.byref(occ))
         assert ms.value > 0 and zp.value >= -1 and xcd.value in (0, 1) and rb.value in (1, 2, 4)
-        assert abs(blk.value) in (128, 256, 512)
+        assert blk.value in (128, 256, 512) and occ.value in (0, 1, 4)
         assert np.array_equal(run(), before)
+
+
+@pytest.mark.parametrize("fmt,prec", [(abi.GAUGE_F64_18, abi.F64), (abi.GAUGE_F64_12, abi.F64),
+                                      (abi.GAUGE_F32_12, abi.F32)])
+def test_every_launch_variant_bit_identical(fmt, prec):
+    """Every block size x register budget x work order of the single-field hopping kernel gives the same
+    bits (the arithmetic helpers fix their rounding), with the Aee epilogue as well as the plain hop; and the
+    FP64 result agrees with the oracle."""
+    lat, ctx = make(8, 8, theta_t=1.0)
+    with ctx:
+        g = random_su3_field(4 * O.volume(lat), 91)
+        gf = ctx.gauge(g, fmt)
+        src = np.random.default_rng(92).standard_normal((ctx.veo, 24))
+        a, b, c = ctx.spinor(prec).upload(src), ctx.spinor(prec), ctx.spinor(prec)
+        tmp = ctx.buffer(2 * ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F64))
+        results = {}
+        for block in (128, 256, 512):
+            for occ in (0, 1, 4):
+                for zp, xcd in ((-1, 1), (0, 0), (2, 1)):
+                    for key, v in (("hop.block", block), ("hop.occupancy", occ), ("hop.zplanes", zp), ("hop.xcd", xcd)):
+                        ctx.check(ctx.lib.cl2_tune_set(ctx.handle, key.encode(), v))
+                    ctx.dslash_eo(b.ptr, a.ptr, gf.ptr, fmt, abi.ODD, 0.16)
+                    ctx.fermionmatrix_eo(abi.OP_AEE, c.ptr, a.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS, 0.16, 0.0016,
+                                         tmp.ptr)
+                    results[(block, occ, zp, xcd)] = (b.download(), c.download())
+        ref = results[(256, 0, -1, 1)]
+        for k, v in results.items():
+            assert np.array_equal(v[0], ref[0]) and np.array_equal(v[1], ref[1]), k
+        if prec == abi.F64:
+            assert site_rel_err(ref[0], O.dslash_eo(lat, src, g, O.ODD, 0.16)) < 1e-13
