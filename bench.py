@@ -19,6 +19,13 @@ configuration generated on the device, the reference's FP64 eo-CG on
 QplusQminus_eo (twisted mass, kappa 0.16, mu 0.005) against the reliable-update
 CG with a reduced-precision inner loop, both to |r|/|b| <= 1e-10, with the true
 residual recomputed in FP64 and the distance between the two solutions.
+
+The other objects: "batch" (12 point sources at 32^3x64, one lockstep block solve
+against 12 sequential solves), "multishift" (16-shift staggered CG-M at 48^3x12,
+fused mixed solver against sequential solves and the reference's FP64 CG-M) and
+"serving" (the inversion engine's capacity with a batch of requests queued at once,
+its p50/p99 latency under a Poisson stream at 70 % of that capacity, and its
+throughput against the reference's FP64 solvers one request at a time).
 """
 import argparse
 import ctypes
@@ -288,38 +295,37 @@ SERVE_SIZES = ((16, 32), (24, 48), (32, 64))
 SERVE_CONFIGS = {(16, 32): 3, (24, 48): 3, (32, 64): 2}  # 8 resident configurations
 
 
-def serve_stream(seconds, rate, seed, cfgs):
-    """Poisson arrivals (rate per second, seeded) of inversion requests: lattice size uniform over
-    16^3x32, 24^3x48, 32^3x64, one of that size's resident configurations, twisted mass (kappa, mu) or
-    staggered (mass) with equal probability, parameters uniform on 5-point grids over kappa in
-    [0.15, 0.16], mu in [0.003, 0.02], mass in [0.005, 0.05] (so that requests can share an operator),
-    1-12 Gaussian right-hand sides generated on the device, |r|/|b| <= 1e-10."""
+def serve_requests(n, seed, cfgs):
+    """n inversion requests: lattice size uniform over 16^3x32, 24^3x48, 32^3x64, one of that size's resident
+    configurations, twisted mass (kappa, mu) or staggered (mass) with equal probability, parameters uniform
+    on 5-point grids over kappa in [0.15, 0.16], mu in [0.003, 0.02], mass in [0.005, 0.05] (so that requests
+    can share an operator), 1-12 Gaussian right-hand sides generated on the device, |r|/|b| <= 1e-10."""
     rng = np.random.default_rng(seed)
-    t, out = 0.0, []
     kappas, mus, masses = np.linspace(0.15, 0.16, 5), np.linspace(0.003, 0.02, 5), np.linspace(0.005, 0.05, 5)
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t > seconds:
-            return out
+    out = []
+    for _ in range(n):
         size = SERVE_SIZES[rng.integers(3)]
         cfg = cfgs[size][rng.integers(len(cfgs[size]))]
-        n = int(rng.integers(1, 13))
-        req = dict(config=cfg, gaussian_seeds=[int(v) for v in rng.integers(0, 2 ** 62, size=n)], rel_tol=1e-10,
+        k = int(rng.integers(1, 13))
+        req = dict(config=cfg, gaussian_seeds=[int(v) for v in rng.integers(0, 2 ** 62, size=k)], rel_tol=1e-10,
                    return_solutions=False)
         if rng.random() < 0.5:
             req.update(action="twistedmass", kappa=float(rng.choice(kappas)), mu=float(rng.choice(mus)))
         else:
             req.update(action="staggered", mass=float(rng.choice(masses)))
-        out.append((t, req))
+        out.append(req)
+    return out
 
 
-def serve_run(engine, stream):
+def serve_run(engine, reqs, times=None):
+    """Submit reqs at the given arrival times (seconds from now; None: all at once) and wait for all."""
     start = engine.now()
     ids = []
-    for t, req in stream:
-        delay = start + t - engine.now()
-        if delay > 0:
-            time.sleep(delay)
+    for k, req in enumerate(reqs):
+        if times is not None:
+            delay = start + times[k] - engine.now()
+            if delay > 0:
+                time.sleep(delay)
         ids.append(engine.submit(**req))
     res = [engine.wait(i) for i in ids]
     end = max(r["finished"] for r in res)
@@ -332,14 +338,15 @@ def serve_run(engine, stream):
             "throughput_rhs_per_s": round(rhs / (end - start), 3),
             "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
             "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
-            "merged_solves": sum(1 for r in res if r["batch_requests"] > 1),
+            "merged_solves": len({(r["admitted"], r["finished"]) for r in res if r["batch_requests"] > 1}),
             "makespan_s": round(end - start, 3)}
 
 
-def serving_leg(seconds, rate, seed, n_baseline):
-    """Continuous-batching engine (4 workers / HIP streams, FP32-inner mixed solvers, HBM budget 160 GB)
-    on a Poisson request stream, against the plain FP64 solvers one request at a time on the stream's
-    first n_baseline requests (and the engine on those same requests)."""
+def serving_leg(n, load, seed, n_baseline, rate=None):
+    """Continuous-batching engine (4 workers / HIP streams, FP32-inner mixed solvers, HBM budget 160 GB):
+    its capacity with all n requests queued at once, then latency under a Poisson stream of the same
+    requests offered at `load` times that capacity (or at `rate`); against the plain FP64 solvers one
+    request at a time on the first n_baseline requests (and the engine on those same requests)."""
     from cl2qcd_amd import _host as H
 
     def engine(**kw):
@@ -354,20 +361,30 @@ def serving_leg(seconds, rate, seed, n_baseline):
         return e, cfgs
 
     e, cfgs = engine(workers=4, mixed=True)
-    stream = serve_stream(seconds, rate, seed, cfgs)
-    log(f"serving: {len(stream)} requests over {seconds} s at {rate}/s ...")
-    full = serve_run(e, stream)
-    log(f"  engine: {full}")
-    first = serve_run(e, stream[:n_baseline])
+    reqs = serve_requests(n, seed, cfgs)
+    log(f"serving: {n} requests queued at once ...")
+    closed = serve_run(e, reqs)
+    log(f"  capacity: {closed}")
+    offered = rate if rate else load * closed["throughput_req_per_s"]
+    times = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / offered, size=n))
+    log(f"serving: the same {n} requests as a Poisson stream at {offered:.3f}/s ...")
+    stream = serve_run(e, reqs, times)
+    log(f"  stream: {stream}")
+    first = serve_run(e, reqs[:n_baseline])
     del e
     f, _ = engine(workers=1, mixed=False)
-    base = serve_run(f, stream[:n_baseline])
+    base = serve_run(f, reqs[:n_baseline])
     del f
-    log(f"  first {n_baseline}: engine {first}; FP64 one at a time {base}")
-    return {"stream": f"Poisson {rate}/s for {seconds} s, seed {seed}; sizes 16^3x32 / 24^3x48 / 32^3x64, "
-                      "twisted mass or staggered, 1-12 Gaussian RHS, |r|/|b| <= 1e-10; 8 resident heatbath "
-                      "beta 5.7 configurations",
-            "engine": full, "first_requests": {"n": n_baseline, "engine": first, "fp64_one_at_a_time": base}}
+    log(f"  first {n_baseline}, queued at once: engine {first}; FP64 one at a time {base}")
+    return {"requests": f"{n} requests, seed {seed}; sizes 16^3x32 / 24^3x48 / 32^3x64, twisted mass or "
+                        "staggered, 1-12 Gaussian RHS, |r|/|b| <= 1e-10; 8 resident heatbath beta 5.7 "
+                        "configurations",
+            "capacity_all_queued": closed,
+            "poisson_stream": {"offered_req_per_s": round(offered, 3),
+                               "load": round(offered / closed["throughput_req_per_s"], 3), **stream},
+            "first_requests_all_queued": {"n": n_baseline, "engine": first, "fp64_one_at_a_time": base,
+                                          "throughput_ratio": round(first["throughput_req_per_s"] /
+                                                                    base["throughput_req_per_s"], 3)}}
 
 
 def read_counters():
@@ -421,8 +438,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-source block solve at 32^3x64")
     ap.add_argument("--no-multishift", action="store_true", help="skip the 16-shift staggered solve at 48^3x12")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
-    ap.add_argument("--serve-seconds", type=float, default=3.0, help="length of the Poisson request stream")
-    ap.add_argument("--serve-rate", type=float, default=20.0, help="mean requests per second")
+    ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="Poisson arrival rate as a fraction of the engine's measured capacity")
+    ap.add_argument("--serve-rate", type=float, default=None, help="fixed arrival rate (requests per second)")
     ap.add_argument("--serve-baseline", type=int, default=10, help="requests also run by the FP64 baseline")
     ap.add_argument("--hb-overrelax", type=int, default=1, help="overrelaxation sweeps per heatbath sweep")
     ap.add_argument("--no-autotune", action="store_true",
@@ -490,7 +509,7 @@ def main():
 
     serving = None
     if not args.no_serving:
-        serving = serving_leg(args.serve_seconds, args.serve_rate, 20261205, args.serve_baseline)
+        serving = serving_leg(args.serve_requests, args.serve_load, 20261205, args.serve_baseline, args.serve_rate)
 
     traffic = read_counters()
     achieved = f64["alg_bytes"] / (f64["ms"] * 1e6)

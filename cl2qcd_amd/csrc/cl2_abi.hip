@@ -48,13 +48,13 @@ size_t su3vec_bytes(const cl2_ctx* c, int prec)
 }
 size_t gauge_bytes(const cl2_ctx* c, int fmt)
 {
-    // 4 dirs x 2 parities x Veo links; the FP64 and FP32 formats pad Veo to whole tiles of 64 sites
-    const size_t links = (size_t)8 * c->g.veo, tiled = (size_t)8 * link_sites(c->g.veo);
+    // 4 dirs x 2 parities x Veo links, Veo padded to whole tiles of 64 sites
+    const size_t tiled = (size_t)8 * link_sites(c->g.veo);
     switch (fmt) {
         case CL2_GAUGE_F64_18: return tiled * 9 * 16;
         case CL2_GAUGE_F64_12: return tiled * 6 * 16;
         case CL2_GAUGE_F32_12: return tiled * 3 * 16;
-        case CL2_GAUGE_F16_12: return links * 3 * 8;
+        case CL2_GAUGE_F16_12: return tiled * 24;
         default: return 0;
     }
 }
@@ -191,7 +191,7 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.norm_partials = nullptr;
     a.w             = wilson_weights(c, kappa);
     const cl2_hop_tuning t = hop_tuning(c, fmt, nrhs);
-    a.block         = (nrhs == 1 && fmt != CL2_GAUGE_F16_12) ? t.block : DSLASH_BLOCK;
+    a.block         = nrhs == 1 ? t.block : DSLASH_BLOCK;
     a.occ           = t.occ;
     a.map           = work_map(c, a.block, nrhs, t);
     a.nrhs          = nrhs;
@@ -1326,7 +1326,9 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     // (block, occupancy hint) shapes of the single-field kernel
     std::vector<std::pair<int, int>> shapes = {{256, 0}};
     if (nrhs == 1 && fmt == CL2_GAUGE_F32_12) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 4}, {128, 4}};
-    else if (nrhs == 1 && fmt != CL2_GAUGE_F16_12) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 1}, {128, 1}};
+    else if (nrhs == 1 && fmt == CL2_GAUGE_F16_12)
+        shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 1}, {128, 1}, {256, 4}, {128, 4}};
+    else if (nrhs == 1) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 1}, {128, 1}};
     const bool manual = c->tune_manual;
     c->tune_manual    = false;
     const double one[2] = {1., 0.};

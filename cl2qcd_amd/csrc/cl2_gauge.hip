@@ -9,17 +9,17 @@ namespace cl2 {
 typedef _Float16 gh4 __attribute__((ext_vector_type(4)));
 typedef _Float16 gh8 __attribute__((ext_vector_type(8)));
 
-/* rows 0,1 of u as F16_12: per (dir, parity) a [Veo] array of 16-byte lanes (entries 0..3) followed by a
- * [Veo] array of 8-byte lanes (entries 4, 5) */
+/* rows 0,1 of u as F16_12 (layout: link_h16_offset / link_h8_offset) */
 template<typename R>
 __device__ __forceinline__ void store_link_h(void* dev, uint32_t dp, uint32_t veo, uint32_t eo, const Su3<R>& u)
 {
-    char* base = static_cast<char*>(dev) + (size_t)dp * veo * 24;
+    char* base     = static_cast<char*>(dev);
     const Cx<R> e0 = u.e[0][0], e1 = u.e[0][1], e2 = u.e[0][2], e3 = u.e[1][0], e4 = u.e[1][1], e5 = u.e[1][2];
-    reinterpret_cast<gh8*>(base)[eo] = gh8{(_Float16)e0.re, (_Float16)e0.im, (_Float16)e1.re, (_Float16)e1.im,
-                                           (_Float16)e2.re, (_Float16)e2.im, (_Float16)e3.re, (_Float16)e3.im};
-    reinterpret_cast<gh4*>(base + (size_t)veo * 16)[eo] = gh4{(_Float16)e4.re, (_Float16)e4.im, (_Float16)e5.re,
-                                                              (_Float16)e5.im};
+    *reinterpret_cast<gh8*>(base + link_h16_offset(dp, eo, veo)) =
+        gh8{(_Float16)e0.re, (_Float16)e0.im, (_Float16)e1.re, (_Float16)e1.im,
+            (_Float16)e2.re, (_Float16)e2.im, (_Float16)e3.re, (_Float16)e3.im};
+    *reinterpret_cast<gh4*>(base + link_h8_offset(dp, eo, veo)) =
+        gh4{(_Float16)e4.re, (_Float16)e4.im, (_Float16)e5.re, (_Float16)e5.im};
 }
 
 /* splitmix64 finaliser: a stateless hash of (seed, counter) */

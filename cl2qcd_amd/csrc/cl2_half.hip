@@ -9,9 +9,16 @@
 // halves lie in [-1, 1] and keep 11 significant bits relative to the site's
 // largest entry.  16-byte lanes keep every wavefront load a 1 KiB segment.
 //
-// Link layout (24 bytes per link): for each (dir, parity) a [Veo] array of
-// 16-byte lanes holding entries 0..3 of rows 0,1 (row-major) followed by a [Veo]
-// array of 8-byte lanes holding entries 4, 5; row 2 = conj(row0 x row1).
+// Link layout (24 bytes per link): for each (dir, parity) tiles of 64 eo sites,
+// each tile 64 16-byte lanes holding entries 0..3 of rows 0,1 (row-major)
+// followed by 64 8-byte lanes holding entries 4, 5; row 2 = conj(row0 x row1).
+//
+// The hopping kernel has the FP64/FP32 kernels' variants: H = 1, 2 or 4 hops
+// with their links and neighbour sites loaded before any is applied, under a
+// register budget that leaves room for them; every variant computes the same
+// bits.
+#include <type_traits>
+
 #include "cl2_device.hpp"
 #include "cl2_internal.hpp"
 
@@ -43,7 +50,9 @@ __device__ __forceinline__ void load_site_h(const void* p, uint32_t S, uint32_t 
     }
 }
 
-/* quantise one site; NaN and Inf stay visible as NaN after the round trip */
+/* quantise one site; NaN and Inf stay visible as NaN after the round trip.  STREAM: non-temporal stores
+ * (results of a hopping kernel, which no later work of the launch reads) */
+template<bool STREAM = false>
 __device__ __forceinline__ void store_site_h(void* p, uint32_t S, uint32_t i, const Cx<float> (&v)[12])
 {
     float m = 0.f;
@@ -59,17 +68,25 @@ __device__ __forceinline__ void store_site_h(void* p, uint32_t S, uint32_t i, co
             f[2 * q]     = v[4 * j + q].re * inv;
             f[2 * q + 1] = v[4 * j + q].im * inv;
         }
-        l[(size_t)j * S + i] = __builtin_convertvector(f, h8);
+        if constexpr (STREAM)
+            __builtin_nontemporal_store(__builtin_convertvector(f, h8), l + (size_t)j * S + i);
+        else
+            l[(size_t)j * S + i] = __builtin_convertvector(f, h8);
     }
-    half_norms(p, S)[i] = m;
+    if constexpr (STREAM)
+        __builtin_nontemporal_store(m, half_norms(p, S) + i);
+    else
+        half_norms(p, S)[i] = m;
 }
 
 __device__ __forceinline__ void load_link_h(const void* g, uint32_t S, int dir, int par, uint32_t eo, Su3<float>& u)
 {
-    const char* base = static_cast<const char*>(g) + (size_t)(dir * 2 + par) * S * 24;
-    const f8v a      = __builtin_convertvector(__builtin_nontemporal_load(reinterpret_cast<const h8*>(base) + eo), f8v);
-    const f4v b      = __builtin_convertvector(
-        __builtin_nontemporal_load(reinterpret_cast<const h4*>(base + (size_t)S * 16) + eo), f4v);
+    const char* base = static_cast<const char*>(g);
+    const uint32_t dp = (uint32_t)(dir * 2 + par);
+    const f8v a       = __builtin_convertvector(
+        __builtin_nontemporal_load(reinterpret_cast<const h8*>(base + link_h16_offset(dp, eo, S))), f8v);
+    const f4v b = __builtin_convertvector(
+        __builtin_nontemporal_load(reinterpret_cast<const h4*>(base + link_h8_offset(dp, eo, S))), f4v);
     u.e[0][0] = Cx<float>{a[0], a[1]};
     u.e[0][1] = Cx<float>{a[2], a[3]};
     u.e[0][2] = Cx<float>{a[4], a[5]};
@@ -101,13 +118,9 @@ __device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const Cx<float>
 }
 
 template<int DIR, bool FWD>
-__device__ __forceinline__ void hop_h(Cx<float> (&acc)[12], const void* in, uint32_t S, uint32_t nb, const void* gauge,
-                                      int linkpar, uint32_t linkeo, Cx<float> w)
+__device__ __forceinline__ void apply_hop_h(Cx<float> (&acc)[12], const Cx<float> (&psi)[12], const Su3<float>& u,
+                                            Cx<float> w)
 {
-    Su3<float> u;
-    load_link_h(gauge, S, DIR, linkpar, linkeo, u);
-    Cx<float> psi[12];
-    load_site_h(in, S, nb, psi);
     if constexpr (DIR == 0) {
         half_hop_r<0, 2, FWD ? P1 : M1, FWD>(acc, psi, u, w);
         half_hop_r<1, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w);
@@ -123,8 +136,51 @@ __device__ __forceinline__ void hop_h(Cx<float> (&acc)[12], const void* in, uint
     }
 }
 
-template<int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_dslash_eo_h(DslashArgs a)
+/* hops of directions D0 .. D0 + ND - 1 (forward then backward each), with the links and neighbour sites of
+ * H of them loaded before they are applied (H = 1, 2 or 4, ND * 2 a multiple of H) */
+template<int D0, int ND, int H>
+__device__ __forceinline__ void hops_h(Cx<float> (&acc)[12], const void* in, uint32_t S, const SiteNb& nb,
+                                       const void* gauge, int P, uint32_t i, const HopWeights<double>& w)
+{
+    if constexpr (ND > 0) {
+        constexpr int G = H < 2 ? 1 : H / 2;  // directions per group
+        Su3<float> u[2 * G];
+        Cx<float> psi[2 * G][12];
+        if constexpr (H == 1) {
+            load_link_h(gauge, S, D0, P, i, u[0]);
+            load_site_h(in, S, nb.up[D0], psi[0]);
+            apply_hop_h<D0, true>(acc, psi[0], u[0], Cx<float>{float(w.wf[D0][0]), float(w.wf[D0][1])});
+            load_link_h(gauge, S, D0, P ^ 1, nb.dn[D0], u[0]);
+            load_site_h(in, S, nb.dn[D0], psi[0]);
+            apply_hop_h<D0, false>(acc, psi[0], u[0], Cx<float>{float(w.wb[D0][0]), float(w.wb[D0][1])});
+            hops_h<D0 + 1, ND - 1, H>(acc, in, S, nb, gauge, P, i, w);
+        } else {
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                load_link_h(gauge, S, D0 + g, P, i, u[2 * g]);
+                load_link_h(gauge, S, D0 + g, P ^ 1, nb.dn[D0 + g], u[2 * g + 1]);
+            }
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                load_site_h(in, S, nb.up[D0 + g], psi[2 * g]);
+                load_site_h(in, S, nb.dn[D0 + g], psi[2 * g + 1]);
+            }
+            apply_hop_h<D0, true>(acc, psi[0], u[0], Cx<float>{float(w.wf[D0][0]), float(w.wf[D0][1])});
+            apply_hop_h<D0, false>(acc, psi[1], u[1], Cx<float>{float(w.wb[D0][0]), float(w.wb[D0][1])});
+            if constexpr (G == 2) {
+                apply_hop_h<D0 + 1, true>(acc, psi[2], u[2], Cx<float>{float(w.wf[D0 + 1][0]), float(w.wf[D0 + 1][1])});
+                apply_hop_h<D0 + 1, false>(acc, psi[3], u[3], Cx<float>{float(w.wb[D0 + 1][0]), float(w.wb[D0 + 1][1])});
+            }
+            hops_h<D0 + G, ND - G, H>(acc, in, S, nb, gauge, P, i, w);
+        }
+    }
+}
+
+/* H hops in flight per thread; the register budget grows with H (H = 4: up to 2 waves per SIMD, H = 2: 4,
+ * H = 1: the compiler's choice) */
+template<int BLOCK, int H>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, H == 4 ? 2 : (H == 2 ? 4 : 8))))
+void k_dslash_eo_h(DslashArgs a)
 {
     __shared__ double lds[2 * BLOCK / 64];
     const uint32_t S = a.g.veo;
@@ -133,19 +189,11 @@ __global__ __launch_bounds__(BLOCK) void k_dslash_eo_h(DslashArgs a)
     if (map_site<BLOCK>(a.map, S, blockIdx.x, threadIdx.x, i)) {
         SiteNb nb;
         site_neighbours(a.g, i, a.out_parity, nb);
-        const int P  = a.out_parity;
-        const int Pn = P ^ 1;
         Cx<float> acc[12];
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] = Cx<float>{0.f, 0.f};
-        hop_h<0, true>(acc, a.in, S, nb.up[0], a.gauge, P, i, Cx<float>{float(a.w.wf[0][0]), float(a.w.wf[0][1])});
-        hop_h<0, false>(acc, a.in, S, nb.dn[0], a.gauge, Pn, nb.dn[0], Cx<float>{float(a.w.wb[0][0]), float(a.w.wb[0][1])});
-        hop_h<1, true>(acc, a.in, S, nb.up[1], a.gauge, P, i, Cx<float>{float(a.w.wf[1][0]), float(a.w.wf[1][1])});
-        hop_h<1, false>(acc, a.in, S, nb.dn[1], a.gauge, Pn, nb.dn[1], Cx<float>{float(a.w.wb[1][0]), float(a.w.wb[1][1])});
-        hop_h<2, true>(acc, a.in, S, nb.up[2], a.gauge, P, i, Cx<float>{float(a.w.wf[2][0]), float(a.w.wf[2][1])});
-        hop_h<2, false>(acc, a.in, S, nb.dn[2], a.gauge, Pn, nb.dn[2], Cx<float>{float(a.w.wb[2][0]), float(a.w.wb[2][1])});
-        hop_h<3, true>(acc, a.in, S, nb.up[3], a.gauge, P, i, Cx<float>{float(a.w.wf[3][0]), float(a.w.wf[3][1])});
-        hop_h<3, false>(acc, a.in, S, nb.dn[3], a.gauge, Pn, nb.dn[3], Cx<float>{float(a.w.wb[3][0]), float(a.w.wb[3][1])});
+        // order T, X, Y, Z, forward then backward, as fermionmatrix_eo_dslash.cl:38-45
+        hops_h<0, 4, H>(acc, a.in, S, nb, a.gauge, a.out_parity, i, a.w);
 
         const Cx<float> f1{float(a.f1[0]), float(a.f1[1])}, f2{float(a.f2[0]), float(a.f2[1])};
         if (a.epilogue == EPI_MINV) {
@@ -161,7 +209,7 @@ __global__ __launch_bounds__(BLOCK) void k_dslash_eo_h(DslashArgs a)
                 if (a.gamma5 && k >= 6) acc[k] = Cx<float>{-acc[k].re, -acc[k].im};
             }
         }
-        store_site_h(a.out, S, i, acc);
+        store_site_h<true>(a.out, S, i, acc);
 #pragma unroll
         for (int k = 0; k < 12; ++k) nrm += (double)acc[k].re * acc[k].re + (double)acc[k].im * acc[k].im;
     }
@@ -171,10 +219,27 @@ __global__ __launch_bounds__(BLOCK) void k_dslash_eo_h(DslashArgs a)
     }
 }
 
+/* launch shape: block size a.block (128, 256, 512), hops in flight by the register budget a.occ (4: one
+ * hop at a time, 0: two, 1: four) */
 hipError_t launch_dslash_h(cl2_ctx* c, const DslashArgs& a)
 {
-    const unsigned nb = blocks_for(a.g.veo, DSLASH_BLOCK);
-    hipLaunchKernelGGL((k_dslash_eo_h<DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
+    const int bs      = (a.block == 128 || a.block == 512) ? a.block : 256;
+    const unsigned nb = blocks_for(a.g.veo, bs);
+    auto go = [&](auto block_tag) {
+        constexpr int B = decltype(block_tag)::value;
+        if (a.occ == 4)
+            hipLaunchKernelGGL((k_dslash_eo_h<B, 1>), dim3(nb), dim3(B), 0, c->stream, a);
+        else if (a.occ == 1 && B != 512)
+            hipLaunchKernelGGL((k_dslash_eo_h<B, 4>), dim3(nb), dim3(B), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_dslash_eo_h<B, 2>), dim3(nb), dim3(B), 0, c->stream, a);
+    };
+    if (bs == 128)
+        go(std::integral_constant<int, 128>{});
+    else if (bs == 512)
+        go(std::integral_constant<int, 512>{});
+    else
+        go(std::integral_constant<int, 256>{});
     return hipGetLastError();
 }
 

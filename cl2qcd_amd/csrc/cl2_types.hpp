@@ -39,6 +39,19 @@ __host__ __device__ inline size_t link_elem(uint32_t dp, uint32_t lanes, uint32_
            (size_t)k * LINK_TILE + eo % LINK_TILE;
 }
 
+/* FP16 12-real links (24 bytes): per (dir, parity) dp, tiles of 64 eo sites, each tile 64 16-byte lanes
+ * (entries 0..3 of rows 0,1) followed by 64 8-byte lanes (entries 4, 5).  Byte offsets of site e's two
+ * lanes: */
+__host__ __device__ inline size_t link_h16_offset(uint32_t dp, uint32_t eo, uint32_t veo)
+{
+    return ((size_t)dp * link_sites(veo) + (size_t)(eo / LINK_TILE) * LINK_TILE) * 24 + (size_t)(eo % LINK_TILE) * 16;
+}
+__host__ __device__ inline size_t link_h8_offset(uint32_t dp, uint32_t eo, uint32_t veo)
+{
+    return ((size_t)dp * link_sites(veo) + (size_t)(eo / LINK_TILE) * LINK_TILE) * 24 + (size_t)LINK_TILE * 16 +
+           (size_t)(eo % LINK_TILE) * 8;
+}
+
 /* Work order of the hopping kernels.  Logical work item l enumerates
  * (z-chunk, t, site-in-slab) with the site fastest, then t, then the z-chunk;
  * a slab is zplanes consecutive z-planes of one t-slice (P = zplanes*ns^2/2 eo

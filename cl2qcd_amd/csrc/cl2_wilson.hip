@@ -344,8 +344,11 @@ cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs)
         auto it = c->tuned.find({fmt, nrhs});
         if (it != c->tuned.end()) return it->second;
     }
-    // default register budget: four hops in flight for the FP64 links, hop pairs for FP32 (measured at 48^3x96)
-    const int occ = c->tune_occ >= 0 ? c->tune_occ : ((fmt == GAUGE_F64_18 || fmt == GAUGE_F64_12) ? 1 : 0);
+    // default register budget (the autotuner's choices at 48^3x96): four hops in flight for the FP64 links,
+    // hop pairs for FP32, one hop at a time for FP16
+    const int occ = c->tune_occ >= 0
+                        ? c->tune_occ
+                        : ((fmt == GAUGE_F64_18 || fmt == GAUGE_F64_12) ? 1 : (fmt == GAUGE_F16_12 ? 4 : 0));
     return cl2_hop_tuning{c->tune_zplanes, c->tune_xcd, c->tune_mrhs_rb, c->tune_block, occ};
 }
 
@@ -403,7 +406,13 @@ template<int FMT, int NRHS, int RB>
 static hipError_t launch_rb(cl2_ctx* c, const DslashArgs& a)
 {
     const unsigned nb = blocks_for((size_t)a.g.veo * (NRHS / RB), DSLASH_BLOCK);
-    hipLaunchKernelGGL((k_dslash_eo_rb<FMT, DSLASH_BLOCK, NRHS, RB>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
+    // one right-hand side per thread: the single-field kernel over the [12][Veo][NRHS] block, with the
+    // hops of a direction in flight together
+    if constexpr (RB == 1)
+        hipLaunchKernelGGL((k_dslash_eo<FMT, DSLASH_BLOCK, NRHS, 0>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
+    else
+        hipLaunchKernelGGL((k_dslash_eo_rb<FMT, DSLASH_BLOCK, NRHS, RB>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream,
+                           a);
     return hipGetLastError();
 }
 

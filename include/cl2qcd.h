@@ -18,8 +18,8 @@
  *                    F64_12: double2[4][2][Vp/64][6][64]   rows 0,1; row 2 = conj(row0 x row1)
  *                    F32_12: float4 [4][2][Vp/64][3][64]   rows 0,1 packed two entries per float4
  *                            (tiles of 64 eo sites, lane-major inside a tile; Vp = Veo rounded up to 64)
- *                    F16_12: per (dir, parity): half[Veo][8] (entries 0..3 of rows 0,1), then half[Veo][4]
- *                            (entries 4, 5); 24 B per link
+ *                    F16_12: per (dir, parity) tiles of 64 eo sites: half[64][8] (entries 0..3 of rows 0,1)
+ *                            then half[64][4] (entries 4, 5); 24 B per link
  *   device scalars: complex = 2 doubles (re, im); real = 1 double.
  * Host arrays use the reference's AoS order: site = x + NS*(y + NS*(z + NS*t)),
  * eo index = site/2, link index = mu + 4*site with mu = T,X,Y,Z.
@@ -353,8 +353,9 @@ int cl2_cg_beta(cl2_ctx* ctx, double* mbeta, double* r2, const double* norms);
  *       "hop.xcd" -- 1: give each XCD a contiguous range of blocks, 0: hardware round robin;
  *       "hop.block" -- threads per block of the single-field hopping kernel (128, 256, 512);
  *       "hop.occupancy" -- its register budget: 0 (<= 2 waves per SIMD, the two hops of a direction in
- *       flight), 1 (one wave per SIMD, four hops in flight; FP64 links), 4 (>= 4 waves per SIMD, one hop at a
- *       time; FP32 links), -1 the format's default (1 for FP64 links, 0 for FP32);
+ *       flight), 1 (one wave per SIMD, four hops in flight; FP64 and FP16 links), 4 (one hop at a time: at
+ *       least 4 waves per SIMD for FP32 links, the compiler's choice for FP16), -1 the format's default (1 for
+ *       FP64 links, 0 for FP32, 4 for FP16);
  *       "mrhs.rb" -- right-hand sides per thread of the multi-right-hand-side kernel (0: by format, 1, 2, 4).
  * Every setting changes only which thread computes which site and when: results are bit identical. */
 int cl2_tune_set(cl2_ctx* ctx, const char* key, long value);
@@ -364,9 +365,8 @@ long cl2_tune_get(const cl2_ctx* ctx, const char* key);
  * autotune the hopping kernel's launch shape for (format, nrhs) on this context -- z-planes per slab, XCD
  * mapping, right-hand sides per thread, and for single fields the block size (128, 256, 512) and register
  * budget -- by timing every candidate on scratch fields with the given gauge field; the fastest is kept for
- * later launches (unless cl2_tune_set fixed the shape) and returned.  occupancy: 0 = up to 2 waves per SIMD
- * with the two hops of a direction in flight; 1 = one wave per SIMD with four hops in flight (FP64 links);
- * 4 = at least 4 waves per SIMD, one hop at a time (FP32 links).  Output pointers may be null. */
+ * later launches (unless cl2_tune_set fixed the shape) and returned.  occupancy: the register budget, as
+ * the "hop.occupancy" key of cl2_tune_set.  Output pointers may be null. */
 int cl2_autotune_dslash(cl2_ctx* ctx, const void* gauge, int format, int nrhs, double* best_ms, int* zplanes,
                         int* xcd, int* rhs_per_thread, int* block, int* occupancy);
 

@@ -262,7 +262,7 @@ def test_autotune_keeps_results(fmt, prec, nrhs):
 
 
 @pytest.mark.parametrize("fmt,prec", [(abi.GAUGE_F64_18, abi.F64), (abi.GAUGE_F64_12, abi.F64),
-                                      (abi.GAUGE_F32_12, abi.F32)])
+                                      (abi.GAUGE_F32_12, abi.F32), (abi.GAUGE_F16_12, abi.F16)])
 def test_every_launch_variant_bit_identical(fmt, prec):
     """Every block size x register budget x work order of the single-field hopping kernel gives the same
     bits (the arithmetic helpers fix their rounding), with the Aee epilogue as well as the plain hop; and the
