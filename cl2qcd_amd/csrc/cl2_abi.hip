@@ -196,6 +196,8 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.map           = work_map(c, a.block, nrhs, t);
     a.nrhs          = nrhs;
     a.rb            = t.rb;
+    a.lds           = t.lds;
+    if (nrhs > 1 && t.lds && t.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
     CL2_HIP(c, launch_dslash(c, a, fmt));
     return CL2_OK;
 }
@@ -1270,6 +1272,11 @@ int cl2_tune_set(cl2_ctx* c, const char* key, long value)
         c->tune_manual  = true;
         return CL2_OK;
     }
+    if (k == "mrhs.lds") {
+        c->tune_mrhs_lds = value ? 1 : 0;
+        c->tune_manual   = true;
+        return CL2_OK;
+    }
     if (k == "hop.block") {
         CL2_REQUIRE(c, value == 128 || value == 256 || value == 512, "hop.block must be 128, 256 or 512");
         c->tune_block  = (int)value;
@@ -1293,6 +1300,7 @@ long cl2_tune_get(const cl2_ctx* c, const char* key)
     if (k == "hop.xcd") return c->tune_xcd;
     if (k == "mrhs.rb") return c->tune_mrhs_rb;
     if (k == "hop.block") return c->tune_block;
+    if (k == "mrhs.lds") return c->tune_mrhs_lds;
     if (k == "hop.occupancy") return c->tune_occ;
     return -2;
 }
@@ -1333,13 +1341,20 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     c->tune_manual    = false;
     const double one[2] = {1., 0.};
     double best         = 1e30;
-    cl2_hop_tuning bt{-1, 1, 1, DSLASH_BLOCK, -1};
+    cl2_hop_tuning bt{-1, 1, 1, 1, DSLASH_BLOCK, -1};
     int rc = CL2_OK;
+    // multi-RHS with one right-hand side per thread: with and without the per-site LDS staging
+    std::vector<std::pair<int, int>> rls;
+    for (int r : rbs) {
+        rls.push_back({r, 0});
+        if (nrhs > 1 && r == 1) rls.push_back({r, 1});
+    }
     for (int zp : zps)
         for (int x : {1, 0})
-            for (int r : rbs)
+            for (const auto& rl : rls)
                 for (const auto& sh : shapes) {
-                c->tuned[{fmt, nrhs}] = cl2_hop_tuning{zp, x, r, sh.first, sh.second};
+                const int r = rl.first;
+                c->tuned[{fmt, nrhs}] = cl2_hop_tuning{zp, x, r, rl.second, sh.first, sh.second};
                 for (int k = 0; k < 2 && rc == CL2_OK; ++k)
                     rc = run_dslash(c, out.p, in.p, nullptr, gauge, fmt, CL2_EVEN, 0.125, EPI_PLAIN, one, one, 1., 0,
                                     nrhs);
@@ -1355,7 +1370,7 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
                 CL2_HIP(c, hipEventElapsedTime(&ms, c->ev_start, c->ev_stop));
                 if (ms / 5. < best) {
                     best = ms / 5.;
-                    bt   = cl2_hop_tuning{zp, x, r, sh.first, sh.second};
+                    bt   = cl2_hop_tuning{zp, x, r, rl.second, sh.first, sh.second};
                 }
             }
     c->tune_manual = manual;
@@ -1367,7 +1382,7 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     if (best_ms) *best_ms = best;
     if (zplanes) *zplanes = bt.zplanes;
     if (xcd) *xcd = bt.xcd;
-    if (rb) *rb = bt.rb;
+    if (rb) *rb = (nrhs > 1 && bt.rb <= 1 && bt.lds) ? -1 : bt.rb;
     if (block) *block = bt.block;
     if (occupancy) *occupancy = bt.occ;
     return CL2_OK;

@@ -219,6 +219,22 @@ __device__ __forceinline__ bool map_site(const WorkMap& m, uint32_t veo, uint32_
     return true;
 }
 
+/* eo index of logical site l of the work order (l < veo) */
+__device__ __forceinline__ uint32_t logical_site(const WorkMap& m, uint32_t l)
+{
+    if (m.P == 0) return l;
+    const uint32_t zc = l / m.col;
+    const uint32_t r  = l - zc * m.col;
+    const uint32_t t  = r / m.P;
+    return t * m.half_vs + zc * m.P + (r - t * m.P);
+}
+
+/* logical block of hardware block h under the XCD remap */
+__device__ __forceinline__ uint32_t logical_block(const WorkMap& m, uint32_t h)
+{
+    return (m.xcd_q && h < 8u * m.xcd_q) ? (h & 7u) * m.xcd_q + (h >> 3) : h;
+}
+
 /* the same for NRHS right-hand sides per site: logical item l covers site l / NRHS, right-hand side
  * l % NRHS, and the slab order applies to the site part */
 template<int BLOCK, int NRHS>

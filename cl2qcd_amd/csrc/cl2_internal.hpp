@@ -15,6 +15,7 @@ struct cl2_hop_tuning {
     int zplanes;  // z-planes per slab (-1: heuristic, 0: identity order)
     int xcd;      // XCD-contiguous block assignment
     int rb;       // right-hand sides per thread of the multi-RHS kernel
+    int lds;      // 1: the multi-RHS kernel stages each site's links and neighbour indices in LDS once
     int block;    // threads per block of the single-field kernel
     int occ;      // register budget of the single-field kernel: 0 (<= 2 waves per SIMD, hop pairs), 1 (1 wave,
                   // four hops in flight), 4 (>= 4 waves, one hop at a time); -1: the format's default
@@ -38,6 +39,7 @@ struct cl2_ctx {
     int tune_mrhs_rb = 0;   // right-hand sides per thread of the multi-RHS hopping kernel (0: by format)
     int tune_block   = 256; // threads per block of the single-field hopping kernel
     int tune_occ     = -1;  // register budget of the single-field hopping kernel (-1: the format's default)
+    int tune_mrhs_lds = 1;  // multi-RHS hopping kernel: links and neighbour indices staged in LDS per site
     bool tune_manual = false;  // set by cl2_tune_set: the values above override autotuned ones
     std::map<std::pair<int, int>, cl2_hop_tuning> tuned;  // (link format, right-hand sides) -> autotuned shape
     std::string err;
@@ -59,6 +61,8 @@ inline unsigned blocks_for(size_t n, int block)
 cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs);
 /* work order of the hopping kernels for this context's lattice and a launch shape */
 WorkMap work_map(const cl2_ctx* c, int block, int nrhs, const cl2_hop_tuning& t);
+/* sites per block of the multi-RHS kernel that stages links in LDS (blocks of sites x nrhs threads) */
+int mrhs_sites_per_block(int nrhs);
 /* right-hand-side counts with a compiled multi-RHS hopping kernel */
 bool supported_nrhs(int n);
 

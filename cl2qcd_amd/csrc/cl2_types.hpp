@@ -96,6 +96,7 @@ struct DslashArgs {
     WorkMap map;
     int nrhs;  // right-hand sides per launch (fields [12][Veo][nrhs]); 1 for single fields
     int rb;    // right-hand sides per thread of the multi-RHS kernel
+    int lds;   // multi-RHS kernel with one right-hand side per thread: 1 stages links and neighbour indices in LDS
     int block; // threads per block of the single-field kernel (128, 256 or 512)
     int occ;   // register budget of the single-field kernel: 0 (<= 2 waves per SIMD), 1 (1 wave), 4 (>= 4 waves)
 };

@@ -141,6 +141,31 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
 /* NRHS right-hand sides in one launch: the fields are [12][Veo][NRHS] (right-hand side fastest) and
  * thread l works on site l / NRHS, right-hand side l % NRHS, so the NRHS lanes of one site issue the same
  * link addresses and each link leaves HBM once per launch.  NRHS = 1 is the single-field kernel. */
+/* Epilogue (site-diagonal factors, Aee saxpy, gamma5) and streaming store of output element O of a field of
+ * component stride SS, adding |out|^2 of the site to NRM.  A macro rather than a function: passing the
+ * accumulator array to a helper, even a force-inlined one, made the compiler keep it in scratch memory. */
+#define CL2_FINISH_SITE(O, NRM)                                                                                   \
+    {                                                                                                             \
+        const size_t o_ = (O);                                                                                    \
+        const Cx<R> f1{R(a.f1[0]), R(a.f1[1])}, f2{R(a.f2[0]), R(a.f2[1])};                                       \
+        if (a.epilogue == EPI_MINV) {                                                                             \
+            const R s_ = R(a.scale);                                                                              \
+            _Pragma("unroll") for (int k = 0; k < 12; ++k) acc[k] = scale(cmul(acc[k], k < 6 ? f1 : f2), s_);    \
+        } else if (a.epilogue == EPI_AEE) {                                                                       \
+            const V* x_ = reinterpret_cast<const V*>(a.x);                                                        \
+            _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                      \
+                V xv   = x_[(size_t)k * SS + o_];                                                                 \
+                acc[k] = cmul(Cx<R>{xv.x, xv.y}, k < 6 ? f1 : f2) - acc[k];                                       \
+                if (a.gamma5 && k >= 6) acc[k] = Cx<R>{-acc[k].re, -acc[k].im};                                   \
+            }                                                                                                     \
+        }                                                                                                         \
+        V* out_ = reinterpret_cast<V*>(a.out);                                                                    \
+        _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                          \
+            store_stream(out_ + (size_t)k * SS + o_, V{acc[k].re, acc[k].im});                                    \
+            NRM += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;                 \
+        }                                                                                                         \
+    }
+
 /* Register budget and hops in flight per thread:
  *   OCC = 0: at most 2 waves per SIMD (up to 256 VGPRs), the two hops of one direction in flight;
  *   OCC = 1: 1 wave per SIMD, the four hops of two directions in flight (blocks of at most 256);
@@ -185,33 +210,101 @@ void k_dslash_eo(DslashArgs a)
         }
 #undef CL2_DIR
 #undef CL2_W
-        const uint32_t o = i * NRHS + rhs;
-
-        const Cx<R> f1{R(a.f1[0]), R(a.f1[1])}, f2{R(a.f2[0]), R(a.f2[1])};
-        if (a.epilogue == EPI_MINV) {
-            const R s = R(a.scale);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) acc[k] = scale(cmul(acc[k], k < 6 ? f1 : f2), s);
-        } else if (a.epilogue == EPI_AEE) {
-            const V* x = reinterpret_cast<const V*>(a.x);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                V xv   = x[(size_t)k * SS + o];
-                acc[k] = cmul(Cx<R>{xv.x, xv.y}, k < 6 ? f1 : f2) - acc[k];
-                if (a.gamma5 && k >= 6) acc[k] = Cx<R>{-acc[k].re, -acc[k].im};
-            }
-        }
-        V* out = reinterpret_cast<V*>(a.out);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            store_stream(out + (size_t)k * SS + o, V{acc[k].re, acc[k].im});
-            nrm += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;
-        }
+        CL2_FINISH_SITE((size_t)i * NRHS + rhs, nrm)
     }
     if (a.norm_partials) {
         double s = block_sum<BLOCK>(nrm, lds);
         if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s, a.norm_partials[2 * blockIdx.x + 1] = 0.;
     }
+}
+
+int mrhs_sites_per_block(int nrhs)
+{
+    return nrhs >= 8 ? 16 : 32;
+}
+
+template<int NRHS>
+struct MrhsLds {
+    static constexpr int SB    = NRHS >= 8 ? 16 : 32;  // sites per block (mrhs_sites_per_block)
+    static constexpr int BLOCK = SB * NRHS;
+};
+
+/* Multi-RHS hopping kernel with per-site staging.  A block owns SB consecutive sites of the work order and
+ * NRHS threads per site (right-hand side fastest).  Each site's neighbour indices are computed once and its 8
+ * links loaded and reconstructed once, by SB x 8 threads, into LDS; every right-hand-side thread then takes
+ * them from LDS and gathers only its own neighbour spinors, both hops of a direction in flight together.
+ * This removes the per-lane index arithmetic, link loads and 12-real reconstruction that otherwise repeat
+ * NRHS times per site.  Per right-hand side the arithmetic is the single-field kernel's, in the same order. */
+template<int FMT, int NRHS>
+__global__ __launch_bounds__(MrhsLds<NRHS>::BLOCK) void k_dslash_eo_mrhs_lds(DslashArgs a)
+{
+    using R = typename GaugeLoad<FMT>::R;
+    using V = vec2<R>;
+    constexpr int SB    = MrhsLds<NRHS>::SB;
+    constexpr int BLOCK = MrhsLds<NRHS>::BLOCK;
+    constexpr uint32_t NONE = 0xffffffffu;
+    __shared__ Su3<R> lu[SB * 8];     // [site][hop], hop = 2 dir + backward
+    __shared__ uint32_t lnb[SB * 8];  // neighbour eo index of each hop
+    __shared__ uint32_t lsite[SB];    // eo index of the site (NONE past the end of the lattice)
+    const uint32_t S  = a.g.veo;
+    const size_t SS   = (size_t)S * NRHS;
+    const uint32_t Lb = logical_block(a.map, blockIdx.x);
+    const int P       = a.out_parity;
+    if (threadIdx.x < SB) {
+        const uint32_t l = Lb * SB + threadIdx.x;
+        uint32_t i       = NONE;
+        if (l < S) {
+            i = logical_site(a.map, l);
+            SiteNb nb;
+            site_neighbours(a.g, i, P, nb);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                lnb[threadIdx.x * 8 + 2 * d]     = nb.up[d];
+                lnb[threadIdx.x * 8 + 2 * d + 1] = nb.dn[d];
+            }
+        }
+        lsite[threadIdx.x] = i;
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < SB * 8; w += BLOCK) {
+        const uint32_t i = lsite[w >> 3];
+        if (i != NONE) {
+            const int h = w & 7;
+            Su3<R> u;
+            // forward link: the site's own; backward link: the neighbour's, of the other parity
+            GaugeLoad<FMT>::load(a.gauge, S, h >> 1, (h & 1) ? P ^ 1 : P, (h & 1) ? lnb[w] : i, u);
+            lu[w] = u;
+        }
+    }
+    __syncthreads();
+    const int s      = threadIdx.x / NRHS;
+    const int rhs    = threadIdx.x - s * NRHS;
+    const uint32_t i = lsite[s];
+    if (i == NONE) return;
+    const V* in = reinterpret_cast<const V*>(a.in);
+    Cx<R> acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = Cx<R>{R(0), R(0)};
+    // order T, X, Y, Z, forward then backward, as fermionmatrix_eo_dslash.cl:38-45
+#define CL2_DIR(m)                                                                                              \
+    {                                                                                                           \
+        vec2<R> sf[12], sb[12];                                                                                 \
+        const size_t ef = (size_t)lnb[s * 8 + 2 * (m)] * NRHS + rhs;                                            \
+        const size_t eb = (size_t)lnb[s * 8 + 2 * (m) + 1] * NRHS + rhs;                                        \
+        _Pragma("unroll") for (int k = 0; k < 12; ++k) sf[k] = in[(size_t)k * SS + ef];                         \
+        _Pragma("unroll") for (int k = 0; k < 12; ++k) sb[k] = in[(size_t)k * SS + eb];                         \
+        const Su3<R> uf = lu[s * 8 + 2 * (m)], ub = lu[s * 8 + 2 * (m) + 1];                                    \
+        apply_hop<m, true>(acc, sf, uf, Cx<R>{R(a.w.wf[m][0]), R(a.w.wf[m][1])});                             \
+        apply_hop<m, false>(acc, sb, ub, Cx<R>{R(a.w.wb[m][0]), R(a.w.wb[m][1])});                            \
+    }
+    CL2_DIR(0)
+    CL2_DIR(1)
+    CL2_DIR(2)
+    CL2_DIR(3)
+#undef CL2_DIR
+    double nrm = 0.;  // no per-block norms from this kernel
+    CL2_FINISH_SITE((size_t)i * NRHS + rhs, nrm)
+    (void)nrm;
 }
 
 /* Register-blocked multi-RHS variant: one thread per (site, group of RB right-hand sides); each link is
@@ -349,7 +442,7 @@ cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs)
     const int occ = c->tune_occ >= 0
                         ? c->tune_occ
                         : ((fmt == GAUGE_F64_18 || fmt == GAUGE_F64_12) ? 1 : (fmt == GAUGE_F16_12 ? 4 : 0));
-    return cl2_hop_tuning{c->tune_zplanes, c->tune_xcd, c->tune_mrhs_rb, c->tune_block, occ};
+    return cl2_hop_tuning{c->tune_zplanes, c->tune_xcd, c->tune_mrhs_rb, c->tune_mrhs_lds, c->tune_block, occ};
 }
 
 WorkMap work_map(const cl2_ctx* c, int block, int nrhs, const cl2_hop_tuning& t)
@@ -406,11 +499,18 @@ template<int FMT, int NRHS, int RB>
 static hipError_t launch_rb(cl2_ctx* c, const DslashArgs& a)
 {
     const unsigned nb = blocks_for((size_t)a.g.veo * (NRHS / RB), DSLASH_BLOCK);
-    // one right-hand side per thread: the single-field kernel over the [12][Veo][NRHS] block, with the
-    // hops of a direction in flight together
-    if constexpr (RB == 1)
-        hipLaunchKernelGGL((k_dslash_eo<FMT, DSLASH_BLOCK, NRHS, 0>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
-    else
+    // one right-hand side per thread: per-site LDS staging of links and neighbour indices, or the single-field
+    // kernel over the [12][Veo][NRHS] block; both keep the hops of a direction in flight together
+    if constexpr (RB == 1) {
+        if (a.lds) {
+            constexpr int SB = MrhsLds<NRHS>::SB;
+            hipLaunchKernelGGL((k_dslash_eo_mrhs_lds<FMT, NRHS>), dim3((a.g.veo + SB - 1) / SB),
+                               dim3(MrhsLds<NRHS>::BLOCK), 0, c->stream, a);
+        } else {
+            hipLaunchKernelGGL((k_dslash_eo<FMT, DSLASH_BLOCK, NRHS, 0>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream,
+                               a);
+        }
+    } else
         hipLaunchKernelGGL((k_dslash_eo_rb<FMT, DSLASH_BLOCK, NRHS, RB>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream,
                            a);
     return hipGetLastError();

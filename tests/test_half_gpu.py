@@ -62,7 +62,8 @@ def test_storage_roundtrip():
 
 @pytest.mark.parametrize("evenodd", [abi.EVEN, abi.ODD])
 @pytest.mark.parametrize("case", [dict(ns=8, nt=8, theta_t=1.0), dict(ns=4, nt=6, theta_s=0.3, theta_t=1.0, cp_re=0.11),
-                                  dict(ns=16, nt=32, theta_t=1.0)], ids=["8x8", "4x6cp", "16x32"])
+                                  dict(ns=16, nt=32, theta_t=1.0), dict(ns=6, nt=10, theta_t=1.0)],
+                         ids=["8x8", "4x6cp", "16x32", "6x10padded"])
 def test_dslash_half(case, evenodd):
     lat = O.lattice(case["ns"], case["nt"], **{k: v for k, v in case.items() if k not in ("ns", "nt")})
     with abi.Context(**case) as ctx:
