@@ -297,12 +297,11 @@ SERVE_CONFIGS = {(16, 32): 3, (24, 48): 3, (32, 64): 2}  # 8 resident configurat
 
 def serve_requests(n, seed, cfgs):
     """n inversion requests: lattice size uniform over 16^3x32, 24^3x48, 32^3x64, one of that size's resident
-    configurations, twisted mass (kappa, mu) or staggered (mass) with equal probability, parameters from two
-    values each (kappa 0.15 / 0.16, mu 0.005 / 0.02, mass 0.01 / 0.05: propagator requests on an ensemble
-    share their physics parameters, so queued requests can share an operator and be merged), 1-12 Gaussian
-    right-hand sides generated on the device, |r|/|b| <= 1e-10."""
+    configurations, twisted mass or staggered with equal probability, kappa uniform in [0.15, 0.16], mu in
+    [0.003, 0.02], mass in [0.005, 0.05], 1-12 Gaussian right-hand sides generated on the device,
+    |r|/|b| <= 1e-10.  Queued twisted-mass requests on one configuration are merged into one multi-RHS solve
+    whatever their kappa and mu."""
     rng = np.random.default_rng(seed)
-    kappas, mus, masses = (0.15, 0.16), (0.005, 0.02), (0.01, 0.05)
     out = []
     for _ in range(n):
         size = SERVE_SIZES[rng.integers(3)]
@@ -311,9 +310,9 @@ def serve_requests(n, seed, cfgs):
         req = dict(config=cfg, gaussian_seeds=[int(v) for v in rng.integers(0, 2 ** 62, size=k)], rel_tol=1e-10,
                    return_solutions=False)
         if rng.random() < 0.5:
-            req.update(action="twistedmass", kappa=float(rng.choice(kappas)), mu=float(rng.choice(mus)))
+            req.update(action="twistedmass", kappa=float(rng.uniform(0.15, 0.16)), mu=float(rng.uniform(0.003, 0.02)))
         else:
-            req.update(action="staggered", mass=float(rng.choice(masses)))
+            req.update(action="staggered", mass=float(rng.uniform(0.005, 0.05)))
         out.append(req)
     return out
 

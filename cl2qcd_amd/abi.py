@@ -101,6 +101,8 @@ SIGNATURES = {
     "cl2_saxpy_AND_squarenorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
     "cl2_saxpy_AND_cgnorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
     "cl2_fermionmatrix_eo_mrhs": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i]),
+    "cl2_fermionmatrix_eo_mrhs_params": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_d),
+                                              _vp, _i]),
     "cl2_dslash_eo_mrhs": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _i]),
     "cl2_block_insert_eo": (_i, [_vp, _vp, _vp, _i, _i, _i]),
     "cl2_block_extract_eo": (_i, [_vp, _vp, _vp, _i, _i, _i]),

@@ -162,9 +162,15 @@ int run_reduce(cl2_ctx* c, double* res, const void* x, const void* y, size_t n, 
     return CL2_OK;
 }
 
+/* per-right-hand-side operator parameters of a multi-RHS launch */
+struct RhsParams {
+    double kappa[16];
+    double f1[16][2], f2[16][2], scale[16];
+};
+
 int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void* gauge, int fmt, int evenodd,
                double kappa, int epilogue, const double f1[2], const double f2[2], double scale, int gamma5,
-               int nrhs = 1)
+               int nrhs = 1, const RhsParams* pr = nullptr)
 {
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
     CL2_REQUIRE(c, supported_nrhs(nrhs) && (nrhs == 1 || fmt != CL2_GAUGE_F16_12),
@@ -189,15 +195,25 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.f2[1]         = f2[1];
     a.scale         = scale;
     a.norm_partials = nullptr;
-    a.w             = wilson_weights(c, kappa);
+    a.per_rhs       = (pr != nullptr && nrhs > 1) ? 1 : 0;
+    a.w             = wilson_weights(c, a.per_rhs ? 1.0 : kappa);
+    for (int r = 0; r < 16; ++r) {
+        const bool use   = a.per_rhs && r < nrhs;
+        a.rhs_kappa[r]   = use ? pr->kappa[r] : 0.;
+        a.rhs_f1[r][0]   = use ? pr->f1[r][0] : 0.;
+        a.rhs_f1[r][1]   = use ? pr->f1[r][1] : 0.;
+        a.rhs_f2[r][0]   = use ? pr->f2[r][0] : 0.;
+        a.rhs_f2[r][1]   = use ? pr->f2[r][1] : 0.;
+        a.rhs_scale[r]   = use ? pr->scale[r] : 0.;
+    }
     const cl2_hop_tuning t = hop_tuning(c, fmt, nrhs);
     a.block         = nrhs == 1 ? t.block : DSLASH_BLOCK;
     a.occ           = t.occ;
     a.map           = work_map(c, a.block, nrhs, t);
     a.nrhs          = nrhs;
-    a.rb            = t.rb;
+    a.rb            = a.per_rhs ? 1 : t.rb;  // per-right-hand-side operators: one right-hand side per thread
     a.lds           = t.lds;
-    if (nrhs > 1 && t.lds && t.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
+    if (nrhs > 1 && t.lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
     CL2_HIP(c, launch_dslash(c, a, fmt));
     return CL2_OK;
 }
@@ -233,10 +249,35 @@ void tm_factors(int variant, double mubar, double f1[2], double f2[2], double* s
 /* Aee family with two fused kernels: tmp = M^-1 D in (Wilson: D in), then
  * out = M_site in - D tmp [gamma5]   (fermionmatrix.cpp:224-591). */
 int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge, int fmt, int action, double kappa,
-            double mubar, bool minus, bool gamma5, int nrhs = 1)
+            double mubar, bool minus, bool gamma5, int nrhs = 1, const double* kappas = nullptr,
+            const double* mubars = nullptr)
 {
     double f1[2], f2[2], s;
     int rc;
+    if (kappas) {  // kappa and mubar per right-hand side (kappas, mubars: nrhs values each)
+        RhsParams p1, p2;
+        const double one[2] = {1., 0.};
+        for (int r = 0; r < nrhs; ++r) {
+            p1.kappa[r] = p2.kappa[r] = kappas[r];
+            double g1[2], g2[2], gs;
+            if (action == CL2_ACTION_WILSON) {
+                g1[0] = g2[0] = 1., g1[1] = g2[1] = 0., gs = 1.;
+            } else {
+                tm_factors(minus ? CL2_MTM_INVERSE_SITEDIAGONAL_MINUS : CL2_MTM_INVERSE_SITEDIAGONAL, mubars[r], g1,
+                           g2, &gs);
+            }
+            p1.f1[r][0] = g1[0], p1.f1[r][1] = g1[1], p1.f2[r][0] = g2[0], p1.f2[r][1] = g2[1], p1.scale[r] = gs;
+            if (action == CL2_ACTION_WILSON) {
+                g1[0] = g2[0] = 1., g1[1] = g2[1] = 0.;
+            } else {
+                tm_factors(minus ? CL2_MTM_SITEDIAGONAL_MINUS : CL2_MTM_SITEDIAGONAL, mubars[r], g1, g2, &gs);
+            }
+            p2.f1[r][0] = g1[0], p2.f1[r][1] = g1[1], p2.f2[r][0] = g2[0], p2.f2[r][1] = g2[1], p2.scale[r] = 1.;
+        }
+        rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, 1., EPI_MINV, one, one, 1., 0, nrhs, &p1);
+        if (rc) return rc;
+        return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, 1., EPI_AEE, one, one, 1., gamma5 ? 1 : 0, nrhs, &p2);
+    }
     if (action == CL2_ACTION_WILSON) {
         const double one[2] = {1., 0.};
         rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_PLAIN, one, one, 1., 0, nrhs);
@@ -252,7 +293,8 @@ int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge,
 
 /* the eo operators on nrhs right-hand sides ([12][Veo][nrhs] fields, tmp = 2 such fields) */
 int run_fermionmatrix(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt, int action,
-                      double kappa, double mubar, void* tmp, int nrhs)
+                      double kappa, double mubar, void* tmp, int nrhs, const double* kappas = nullptr,
+                      const double* mubars = nullptr)
 {
     CL2_REQUIRE(c, op >= CL2_OP_AEE && op <= CL2_OP_QPLUSQMINUS_EO, "unknown operator");
     CL2_REQUIRE(c, action == CL2_ACTION_WILSON || action == CL2_ACTION_TWISTEDMASS,
@@ -263,17 +305,20 @@ int run_fermionmatrix(cl2_ctx* c, int op, void* out, const void* in, const void*
     void* t0        = tmp;
     void* t1        = static_cast<char*>(tmp) + fb;
     CL2_REQUIRE(c, out != t0 && out != t1 && in != t0 && in != t1, "tmp must not alias in or out");
+    const double *ks = kappas, *ms = mubars;
     switch (op) {
-        case CL2_OP_AEE: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, false, nrhs);
+        case CL2_OP_AEE: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, false, nrhs, ks, ms);
         case CL2_OP_AEE_AND_GAMMA5:
-        case CL2_OP_QPLUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, true, nrhs);
-        case CL2_OP_AEE_MINUS: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, false, nrhs);
+        case CL2_OP_QPLUS_EO:
+            return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, false, true, nrhs, ks, ms);
+        case CL2_OP_AEE_MINUS:
+            return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, false, nrhs, ks, ms);
         case CL2_OP_AEE_MINUS_AND_GAMMA5:
-        case CL2_OP_QMINUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs);
+        case CL2_OP_QMINUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs, ks, ms);
         default: {
-            int rc = run_aee(c, t1, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs);
+            int rc = run_aee(c, t1, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs, ks, ms);
             if (rc) return rc;
-            return run_aee(c, out, t1, t0, gauge, fmt, action, kappa, mubar, false, true, nrhs);
+            return run_aee(c, out, t1, t0, gauge, fmt, action, kappa, mubar, false, true, nrhs, ks, ms);
         }
     }
 }
@@ -752,6 +797,16 @@ int cl2_fermionmatrix_eo_mrhs(cl2_ctx* c, int op, void* out, const void* in, con
     CL2_REQUIRE(c, supported_nrhs(nrhs) && fmt != CL2_GAUGE_F16_12,
                 "unsupported number of right-hand sides (1, 2, 4, 8, 12 or 16; FP64/FP32 fields)");
     return run_fermionmatrix(c, op, out, in, gauge, fmt, action, kappa, mubar, tmp, nrhs);
+}
+
+int cl2_fermionmatrix_eo_mrhs_params(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt,
+                                     int action, const double* kappa, const double* mubar, void* tmp, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1 && fmt != CL2_GAUGE_F16_12,
+                "unsupported number of right-hand sides (2, 4, 8, 12 or 16; FP64/FP32 fields)");
+    CL2_REQUIRE(c, kappa != nullptr && (action == CL2_ACTION_WILSON || mubar != nullptr), "null parameter array");
+    return run_fermionmatrix(c, op, out, in, gauge, fmt, action, 0., 0., tmp, nrhs, kappa, mubar);
 }
 
 int cl2_dslash_eo_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, int evenodd, double kappa,

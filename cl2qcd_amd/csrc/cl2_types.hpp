@@ -99,6 +99,12 @@ struct DslashArgs {
     int lds;   // multi-RHS kernel with one right-hand side per thread: 1 stages links and neighbour indices in LDS
     int block; // threads per block of the single-field kernel (128, 256 or 512)
     int occ;   // register budget of the single-field kernel: 0 (<= 2 waves per SIMD), 1 (1 wave), 4 (>= 4 waves)
+    // Operators that differ per right-hand side (multi-RHS launches, one right-hand side per thread): the hop
+    // weights w are those of kappa = 1, the hop sum of right-hand side r is scaled by rhs_kappa[r], and
+    // rhs_f1/rhs_f2/rhs_scale replace f1/f2/scale.
+    int per_rhs;
+    double rhs_kappa[16];
+    double rhs_f1[16][2], rhs_f2[16][2], rhs_scale[16];
 };
 
 }  // namespace cl2

@@ -141,15 +141,23 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
 /* NRHS right-hand sides in one launch: the fields are [12][Veo][NRHS] (right-hand side fastest) and
  * thread l works on site l / NRHS, right-hand side l % NRHS, so the NRHS lanes of one site issue the same
  * link addresses and each link leaves HBM once per launch.  NRHS = 1 is the single-field kernel. */
-/* Epilogue (site-diagonal factors, Aee saxpy, gamma5) and streaming store of output element O of a field of
- * component stride SS, adding |out|^2 of the site to NRM.  A macro rather than a function: passing the
+/* Epilogue (per-right-hand-side kappa when the operator differs per right-hand side, site-diagonal factors,
+ * Aee saxpy, gamma5) and streaming store of output element O (right-hand side rhs) of a field of component
+ * stride SS, adding |out|^2 of the site to NRM.  A macro rather than a function: passing the
  * accumulator array to a helper, even a force-inlined one, made the compiler keep it in scratch memory. */
 #define CL2_FINISH_SITE(O, NRM)                                                                                   \
     {                                                                                                             \
         const size_t o_ = (O);                                                                                    \
-        const Cx<R> f1{R(a.f1[0]), R(a.f1[1])}, f2{R(a.f2[0]), R(a.f2[1])};                                       \
+        Cx<R> f1{R(a.f1[0]), R(a.f1[1])}, f2{R(a.f2[0]), R(a.f2[1])};                                             \
+        R s_ = R(a.scale);                                                                                        \
+        if (a.per_rhs) {                                                                                          \
+            const R kr = R(a.rhs_kappa[rhs]);                                                                     \
+            _Pragma("unroll") for (int k = 0; k < 12; ++k) acc[k] = scale(acc[k], kr);                            \
+            f1 = Cx<R>{R(a.rhs_f1[rhs][0]), R(a.rhs_f1[rhs][1])};                                                 \
+            f2 = Cx<R>{R(a.rhs_f2[rhs][0]), R(a.rhs_f2[rhs][1])};                                                 \
+            s_ = R(a.rhs_scale[rhs]);                                                                             \
+        }                                                                                                         \
         if (a.epilogue == EPI_MINV) {                                                                             \
-            const R s_ = R(a.scale);                                                                              \
             _Pragma("unroll") for (int k = 0; k < 12; ++k) acc[k] = scale(cmul(acc[k], k < 6 ? f1 : f2), s_);    \
         } else if (a.epilogue == EPI_AEE) {                                                                       \
             const V* x_ = reinterpret_cast<const V*>(a.x);                                                        \

@@ -124,6 +124,14 @@ int cg_reliable_update_block(const std::vector<const lattices::Spinorfield_eo*>&
                              const std::vector<const lattices::Spinorfield_eo*>& b, const hardware::System& system,
                              hmc_float rel_tol, const physics::AdditionalParameters& additionalParameters,
                              const ReliableUpdateOptions& options, std::vector<SolveStats>* stats = nullptr);
+/* the same with the operator's parameters per right-hand side (kappa and mubar of a twisted-mass or Wilson
+ * operator; parameters[r] for b[r]): right-hand sides with different quark masses on one gauge field are
+ * solved in one block, sharing every link load */
+int cg_reliable_update_block(const std::vector<const lattices::Spinorfield_eo*>& x,
+                             const fermionmatrix::Fermionmatrix_eo& A, const lattices::Gaugefield& gf,
+                             const std::vector<const lattices::Spinorfield_eo*>& b, const hardware::System& system,
+                             hmc_float rel_tol, const std::vector<const physics::AdditionalParameters*>& parameters,
+                             const ReliableUpdateOptions& options, std::vector<SolveStats>* stats = nullptr);
 
 }  // namespace solvers
 }  // namespace algorithms

@@ -116,6 +116,10 @@ double autotune_dslash(const lattices::Gaugefield& gf, const hardware::System& s
 void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* out, const lattices::Gaugefield& gf,
                  const lattices::Spinorfield_eo_block& in, const physics::AdditionalParameters& ap,
                  const lattices::DeviceBuffer& tmp);
+/* the same with A's parameters per right-hand side: kappa[r] and mubar[r] (mubar unused for Wilson) */
+void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* out, const lattices::Gaugefield& gf,
+                 const lattices::Spinorfield_eo_block& in, const std::vector<hmc_float>& kappa,
+                 const std::vector<hmc_float>& mubar, const lattices::DeviceBuffer& tmp);
 
 /* ---- staggered (fermionmatrix_stagg.hpp:49-152) ---- */
 class Fermionmatrix_stagg_eo : public Fermionmatrix_basic {

@@ -361,6 +361,44 @@ PYBIND11_MODULE(_host, m)
         py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("b"), py::arg("lattice"), py::arg("rel_tol"), py::arg("ap"),
         py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1, py::arg("max_iterations") = 100000,
         py::arg("use_graph") = true);
+    m.def(
+        "cg_reliable_update_block_params",
+        [](std::vector<Spinorfield_eo*> xs, const fm::Fermionmatrix_eo& A, const Gaugefield& gf,
+           std::vector<Spinorfield_eo*> bs, Lattice& l, hmc_float rel_tol, std::vector<hmc_float> kappa,
+           std::vector<hmc_float> mubar, int inner_prec, double delta, int maxit, bool use_graph) {
+            if (kappa.size() != bs.size() || mubar.size() != bs.size())
+                throw std::invalid_argument("one kappa and one mubar per right-hand side");
+            sv::ReliableUpdateOptions o;
+            o.inner_precision = inner_prec;
+            o.delta           = delta;
+            o.max_iterations  = maxit;
+            o.use_graph       = use_graph;
+            std::vector<const Spinorfield_eo*> x(xs.begin(), xs.end()), b(bs.begin(), bs.end());
+            std::vector<std::unique_ptr<physics::WilsonAdditionalParameters>> own;
+            std::vector<const AdditionalParameters*> aps;
+            for (size_t r = 0; r < kappa.size(); ++r) {
+                own.emplace_back(new physics::WilsonAdditionalParameters(kappa[r], mubar[r]));
+                aps.push_back(own.back().get());
+            }
+            std::vector<sv::SolveStats> st;
+            int it;
+            {
+                py::gil_scoped_release rel;
+                it = sv::cg_reliable_update_block(x, A, gf, b, l.system, rel_tol, aps, o, &st);
+            }
+            py::list per;
+            for (const auto& s : st) {
+                py::dict d;
+                d["iterations"]             = s.iterations;
+                d["reliable_updates"]       = s.reliable_updates;
+                d["true_relative_residual"] = s.true_relative_residual;
+                per.append(d);
+            }
+            return py::make_tuple(it, per);
+        },
+        py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("b"), py::arg("lattice"), py::arg("rel_tol"),
+        py::arg("kappa"), py::arg("mubar"), py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1,
+        py::arg("max_iterations") = 100000, py::arg("use_graph") = true);
     m.def("point_source_ps_correlator", [](const Gaugefield& gf, Lattice& l) {
         py::gil_scoped_release rel;
         return physics::algorithms::point_source_ps_correlator(gf, l.system, l.handler, l.handler.getAdditionalParameters<Spinorfield_eo>());

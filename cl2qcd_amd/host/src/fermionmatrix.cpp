@@ -206,6 +206,27 @@ void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block
                                                     abi_action, ap.getKappa(), mubar, tmp.ptr(), in.nrhs()));
 }
 
+void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* out, const Gaugefield& gf,
+                 const lattices::Spinorfield_eo_block& in, const std::vector<hmc_float>& kappa,
+                 const std::vector<hmc_float>& mubar, const lattices::DeviceBuffer& tmp)
+{
+    const int op = A.abi_operator();
+    if (op < 0) throw std::invalid_argument("apply_block: not a built-in eo operator");
+    if (out->precision() != in.precision() || out->nrhs() != in.nrhs())
+        throw std::invalid_argument("apply_block: blocks of different shape");
+    if (tmp.bytes() < 2 * in.bytes()) throw std::invalid_argument("apply_block: tmp must hold two blocks");
+    if ((int)kappa.size() != in.nrhs() || (int)mubar.size() != in.nrhs())
+        throw std::invalid_argument("apply_block: one kappa and one mubar per right-hand side");
+    const common::action a = A.action();
+    if (a != common::wilson && a != common::twistedmass)
+        throw std::invalid_argument("Unkown fermion action! wilson or twistedmass");
+    const int abi_action = a == common::twistedmass ? CL2_ACTION_TWISTEDMASS : CL2_ACTION_WILSON;
+    const Links l        = links_for(gf, in.precision());
+    in.get_system().check(cl2_fermionmatrix_eo_mrhs_params(in.get_system().ctx(), op, out->dev(), in.dev(), l.ptr,
+                                                           l.fmt, abi_action, kappa.data(), mubar.data(), tmp.ptr(),
+                                                           in.nrhs()));
+}
+
 /* ---------------- staggered ---------------- */
 
 hmc_float Fermionmatrix_stagg_eo::getThresholdForMinimumEigenvalue(hmc_float) const

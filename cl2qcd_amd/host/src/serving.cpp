@@ -181,9 +181,9 @@ size_t Engine::work_bytes(const Pending& p) const
 
 bool Engine::compatible(const Pending& a, const Pending& b) const
 {
+    // any twisted-mass requests on one configuration: the block operator takes kappa and mu per right-hand side
     return opt.mixed && a.req.action == Action::twistedmass && b.req.action == Action::twistedmass &&
-           a.req.config == b.req.config && a.req.kappa == b.req.kappa && a.req.mu == b.req.mu &&
-           a.req.rel_tol == b.req.rel_tol;
+           a.req.config == b.req.config && a.req.rel_tol == b.req.rel_tol;
 }
 
 long Engine::submit(Request r)
@@ -295,8 +295,16 @@ void Engine::run_batch(Worker& wk, Batch& b)
         for (const auto& it : b.items)
             for (int k = 0; k < it->nrhs(); ++k) keep.push_back(it->req.return_solutions);
         if (head.action == Action::twistedmass) {
-            const WilsonAdditionalParameters ap(head.kappa, 2. * head.kappa * head.mu);
             const int n = opt.mixed ? padded_rhs(total_rhs) : total_rhs;
+            // each right-hand side with its own request's kappa and mu (padding right-hand sides: the last
+            // request's)
+            std::vector<std::unique_ptr<WilsonAdditionalParameters>> own;
+            std::vector<const physics::AdditionalParameters*> aps;
+            for (const auto& it : b.items) {
+                own.emplace_back(new WilsonAdditionalParameters(it->req.kappa, 2. * it->req.kappa * it->req.mu));
+                for (int j = 0; j < it->nrhs(); ++j) aps.push_back(own.back().get());
+            }
+            while ((int)aps.size() < n) aps.push_back(own.back().get());
             std::vector<std::unique_ptr<Spinorfield_eo>> bs, xs;
             for (int r = 0; r < n; ++r) {
                 bs.emplace_back(new Spinorfield_eo(sys));
@@ -317,8 +325,8 @@ void Engine::run_batch(Worker& wk, Batch& b)
                     const hmc_float b2 = squarenorm(*bs[k]);
                     for (int round = 0; round < 3; ++round) {
                         iterations += sv::cg(xs[k].get(), *L.qq, gf, *bs[k], sys, *L.h_tm,
-                                             head.rel_tol * head.rel_tol * b2, ap);
-                        (*L.qq)(&t, gf, *xs[k], ap);
+                                             head.rel_tol * head.rel_tol * b2, *aps[k]);
+                        (*L.qq)(&t, gf, *xs[k], *aps[k]);
                         saxpy(&t, hmc_complex{1., 0.}, t, *bs[k]);
                         if (squarenorm(t) <= head.rel_tol * head.rel_tol * b2) break;
                     }
@@ -326,7 +334,7 @@ void Engine::run_batch(Worker& wk, Batch& b)
             } else if (n == 1) {
                 sv::ReliableUpdateOptions o;
                 o.inner_precision = opt.inner_precision;
-                iterations        = sv::cg_reliable_update(xs[0].get(), *L.qq, gf, *bs[0], sys, head.rel_tol, ap, o);
+                iterations = sv::cg_reliable_update(xs[0].get(), *L.qq, gf, *bs[0], sys, head.rel_tol, *aps[0], o);
             } else {
                 sv::ReliableUpdateOptions o;
                 o.inner_precision = CL2_F32;
@@ -335,12 +343,12 @@ void Engine::run_batch(Worker& wk, Batch& b)
                     xv.push_back(xs[k].get());
                     bv.push_back(bs[k].get());
                 }
-                iterations = sv::cg_reliable_update_block(xv, *L.qq, gf, bv, sys, head.rel_tol, ap, o);
+                iterations = sv::cg_reliable_update_block(xv, *L.qq, gf, bv, sys, head.rel_tol, aps, o);
             }
             // FP64 true residual of every answer
             const Spinorfield_eo t(sys);
             for (int k = 0; k < total_rhs; ++k) {
-                (*L.qq)(&t, gf, *xs[k], ap);
+                (*L.qq)(&t, gf, *xs[k], *aps[k]);
                 saxpy(&t, hmc_complex{1., 0.}, t, *bs[k]);  // b - A x
                 resid.push_back(std::sqrt(squarenorm(t) / squarenorm(*bs[k])));
                 if (keep[k]) {

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 
 #include "../include/physics/algorithms/solvers/solvers.hpp"
 
@@ -436,10 +437,13 @@ namespace physics {
 namespace algorithms {
 namespace solvers {
 
-int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const fermionmatrix::Fermionmatrix_eo& A,
-                             const Gaugefield& gf, const std::vector<const Spinorfield_eo*>& b,
-                             const hardware::System& system, hmc_float rel_tol, const physics::AdditionalParameters& ap,
-                             const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
+namespace {
+/* out = A in for a block (tmp: two blocks of in's shape) */
+using BlockApply = std::function<void(const Spinorfield_eo_block*, const Spinorfield_eo_block&, const DeviceBuffer&)>;
+
+int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<const Spinorfield_eo*>& b,
+                const hardware::System& system, hmc_float rel_tol, const BlockApply& apply,
+                const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
 {
     const int n = (int)b.size();
     if (n < 1 || x.size() != b.size()) throw std::invalid_argument("cg_reliable_update_block: x and b differ in size");
@@ -474,7 +478,7 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
         return v;
     };
     auto true_residual = [&]() {
-        fermionmatrix::apply_block(A, &T, gf, X, ap, tmp64);
+        apply(&T, X, tmp64);
         system.check(cl2_block_axpy_eo(c, R.dev(), T.dev(), B.dev(), CL2_F64, n, ones.dev(), -1., nullptr));
         return block_norms(R);  // |b - A x|^2 per right-hand side
     };
@@ -508,10 +512,10 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
     active.store(act);
     int k           = 0;
     int reliable    = 0;
-    fermionmatrix::apply_block(A, &V, gf, P, ap, tmplo);  // reduced-precision links made before recording
+    apply(&V, P, tmplo);  // reduced-precision links made before recording
     // one lockstep iteration, every scalar on the device
     auto body = [&]() {
-        fermionmatrix::apply_block(A, &V, gf, P, ap, tmplo);
+        apply(&V, P, tmplo);
         system.check(cl2_block_dot_eo(c, pAp.dev(), P.dev(), V.dev(), lo, n));
         system.check(cl2_block_cg_alpha(c, alpha.dev(), r2.dev(), pAp.dev(), active.dev(), n));
         system.check(cl2_block_axpy_eo(c, Xlo.dev(), P.dev(), Xlo.dev(), lo, n, alpha.dev(), 1., nullptr));
@@ -592,6 +596,46 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
     finish(k);
     if (!done) throw SolverDidNotSolve(k, "cg_reliable_update_block");
     return k;
+}
+
+}  // namespace
+
+int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const fermionmatrix::Fermionmatrix_eo& A,
+                             const Gaugefield& gf, const std::vector<const Spinorfield_eo*>& b,
+                             const hardware::System& system, hmc_float rel_tol, const physics::AdditionalParameters& ap,
+                             const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
+{
+    return block_solve(
+        x, b, system, rel_tol,
+        [&](const Spinorfield_eo_block* out, const Spinorfield_eo_block& in, const DeviceBuffer& tmp) {
+            fermionmatrix::apply_block(A, out, gf, in, ap, tmp);
+        },
+        opt, stats);
+}
+
+int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const fermionmatrix::Fermionmatrix_eo& A,
+                             const Gaugefield& gf, const std::vector<const Spinorfield_eo*>& b,
+                             const hardware::System& system, hmc_float rel_tol,
+                             const std::vector<const physics::AdditionalParameters*>& aps,
+                             const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
+{
+    if (aps.size() != b.size()) throw std::invalid_argument("cg_reliable_update_block: one parameter set per right-hand side");
+    const bool tm = A.action() == common::twistedmass;
+    std::vector<hmc_float> kappa, mubar;
+    bool uniform = true;
+    for (const auto* p : aps) {
+        if (!p) throw std::invalid_argument("cg_reliable_update_block: null parameters");
+        kappa.push_back(p->getKappa());
+        mubar.push_back(tm ? p->getMubar() : 0.);
+        uniform = uniform && kappa.back() == kappa[0] && mubar.back() == mubar[0];
+    }
+    if (uniform) return cg_reliable_update_block(x, A, gf, b, system, rel_tol, *aps[0], opt, stats);
+    return block_solve(
+        x, b, system, rel_tol,
+        [&](const Spinorfield_eo_block* out, const Spinorfield_eo_block& in, const DeviceBuffer& tmp) {
+            fermionmatrix::apply_block(A, out, gf, in, kappa, mubar, tmp);
+        },
+        opt, stats);
 }
 
 }  // namespace solvers

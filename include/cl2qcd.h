@@ -241,6 +241,12 @@ int cl2_convert_precision_eo(cl2_ctx* ctx, int kind, void* dst, int dst_prec, co
  * sides; tmp = 2 block fields */
 int cl2_fermionmatrix_eo_mrhs(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge, int format,
                               int action, double kappa, double mubar, void* tmp, int nrhs);
+/* the same with an operator per right-hand side on one gauge field: kappa[r] and mubar[r] (host arrays of
+ * nrhs values; mubar = 2 kappa mu, unused for Wilson) for right-hand side r, nrhs = 2 .. 16.  Lets requests
+ * with different quark masses on one configuration share every link load (serving). */
+int cl2_fermionmatrix_eo_mrhs_params(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge,
+                                     int format, int action, const double* kappa, const double* mubar, void* tmp,
+                                     int nrhs);
 int cl2_dslash_eo_mrhs(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, int evenodd,
                        double kappa, int nrhs);
 /* copy a single field into / out of slot r of a block (nrhs in [1, 64]) */

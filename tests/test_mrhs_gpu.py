@@ -151,3 +151,74 @@ def test_block_cg_matches_single_solves():
         x1 = xs1.get_host()
         assert np.sqrt(((xh - x1) ** 2).sum() / (x1 ** 2).sum()) < 1e-6
     assert it <= 1.5 * max(p["iterations"] for p in per)
+
+
+@pytest.mark.parametrize("op", [abi.OP_AEE, abi.OP_QPLUSQMINUS_EO])
+@pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12])
+def test_fermionmatrix_mrhs_per_rhs_params(op, fmt):
+    """One block launch with kappa and mubar per right-hand side: slot r equals the single-field operator with
+    that right-hand side's parameters (the hop sum is scaled by kappa after summing instead of inside each hop
+    weight, so to rounding) and, in FP64, the oracle."""
+    import ctypes
+    lat = O.lattice(8, 8, theta_t=1.0)
+    prec = abi.F32 if fmt == abi.GAUGE_F32_12 else abi.F64
+    kappas = [0.15, 0.155, 0.16, 0.145]
+    mubars = [2 * k * m for k, m in zip(kappas, [0.003, 0.02, 0.01, 0.0])]
+    nrhs = len(kappas)
+    with abi.Context(8, 8, theta_t=1.0) as ctx:
+        g = random_su3_field(4 * O.volume(lat), 12)
+        gf = ctx.gauge(g, fmt)
+        rng = np.random.default_rng(13)
+        srcs = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
+        blk = make_block(ctx, srcs, prec)
+        out = ctx.buffer(blk.nbytes)
+        tmp = ctx.buffer(2 * blk.nbytes)
+        ka, ma = (ctypes.c_double * nrhs)(*kappas), (ctypes.c_double * nrhs)(*mubars)
+        ctx.fermionmatrix_eo_mrhs_params(op, out.ptr, blk.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS, ka, ma, tmp.ptr,
+                                         nrhs)
+        a, b = ctx.spinor(prec), ctx.spinor(prec)
+        stmp = ctx.buffer(2 * ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F64))
+        for r in range(nrhs):
+            a.upload(srcs[r])
+            ctx.fermionmatrix_eo(op, b.ptr, a.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS, kappas[r], mubars[r], stmp.ptr)
+            got = block_slot(ctx, out, prec, nrhs, r)
+            assert site_rel_err(got, b.download()) < (1e-14 if prec == abi.F64 else 1e-6)
+            if prec == abi.F64:
+                ref = O.apply_eo(lat, op, srcs[r], g, O.TWISTEDMASS, kappas[r], mubars[r])
+                assert site_rel_err(got, ref) < 1e-12
+
+
+def test_block_cg_per_rhs_params():
+    """The lockstep block solver with a different (kappa, mu) per right-hand side: every solution meets
+    |b - A_r x| / |b| <= 1e-10 with its own operator in the FP64 oracle and agrees with a single solve."""
+    from cl2qcd_amd import _host as H
+    l = H.Lattice(["--nSpace=8", "--nTime=8", "--fermionAction=twistedmass", "--kappa=0.15", "--mu=0.02",
+                   "--beta=5.7", "--thetaFermionTemporal=1"])
+    gf = H.Gaugefield(l, False)
+    for _ in range(3):
+        H.su3heatbath(gf, l, 1)
+    kappas = [0.15, 0.155, 0.145, 0.152]
+    mus = [0.02, 0.005, 0.01, 0.015]
+    mubars = [2 * k * m for k, m in zip(kappas, mus)]
+    rng = np.random.default_rng(14)
+    bs, xs = [], []
+    for r in range(4):
+        b = H.Spinorfield_eo(l)
+        b.import_(rng.standard_normal((l.eo_sites, 24)))
+        bs.append(b)
+        x = H.Spinorfield_eo(l)
+        x.zero()
+        xs.append(x)
+    A = H.QplusQminus_eo(l)
+    it, per = H.cg_reliable_update_block_params(xs, A, gf, bs, l, 1e-10, kappas, mubars)
+    lat = O.lattice(8, 8, theta_t=1.0)
+    g = gf.get_host()
+    for r in range(4):
+        assert per[r]["true_relative_residual"] <= 1e-10
+        bh, xh = bs[r].get_host(), xs[r].get_host()
+        res = O.saxpy(O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, xh, g, O.TWISTEDMASS, kappas[r], mubars[r]), bh, 1.0)
+        assert np.sqrt(O.norm2(res) / O.norm2(bh)) <= 1.01e-10
+    x1 = H.Spinorfield_eo(l)
+    x1.zero()
+    H.cg_reliable_update(x1, A, gf, bs[1], l, 1e-10, H.WilsonAdditionalParameters(kappas[1], mubars[1]))
+    assert np.sqrt(((xs[1].get_host() - x1.get_host()) ** 2).sum() / (x1.get_host() ** 2).sum()) < 1e-6

@@ -48,11 +48,11 @@ def make_engine(H, **kw):
 def requests(e, c0, c1):
     rng = np.random.default_rng(5)
     out = []
-    # with two workers the first two run at once and the rest queue, so the three compatible
-    # twisted-mass requests on config c0 are merged when a worker frees up
+    # with two workers the first two run at once and the rest queue, so the three twisted-mass requests on
+    # config c0 are merged when a worker frees up, one of them with its own kappa and mu
     for cfg, action, kappa, mu, mass, n in ((c1, "twistedmass", 0.155, 0.01, 0, 1), (c0, "staggered", 0, 0, 0.05, 2),
                                              (c0, "twistedmass", 0.15, 0.02, 0, 2), (c1, "staggered", 0, 0, 0.1, 1),
-                                             (c0, "twistedmass", 0.15, 0.02, 0, 3), (c0, "twistedmass", 0.15, 0.02, 0, 1)):
+                                             (c0, "twistedmass", 0.155, 0.005, 0, 3), (c0, "twistedmass", 0.15, 0.02, 0, 1)):
         w = 24 if action == "twistedmass" else 6
         srcs = [rng.standard_normal((e.eo_sites(cfg), w)) for _ in range(n)]
         out.append(dict(config=cfg, action=action, kappa=kappa, mu=mu, mass=mass, sources=srcs))
@@ -76,7 +76,7 @@ def test_same_alone_and_in_mix():
         assert max(m["true_rel_residual"]) <= 1e-10
         for xa, xm in zip(a["solutions"], m["solutions"]):
             assert rel(xm, xa) < 1e-6
-    # the three compatible twisted-mass requests on config 0 were merged into one solve
+    # the twisted-mass requests on config 0 were merged into one solve despite their different masses
     merged = [m["batch_requests"] for r, m in zip(reqs, mix) if r["config"] == c0 and r["action"] == "twistedmass"]
     assert max(merged) >= 2
     # solves overlapped in time
