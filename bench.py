@@ -342,7 +342,7 @@ def serve_run(engine, reqs, times=None):
             "makespan_s": round(end - start, 3)}
 
 
-def serving_leg(n, load, seed, n_baseline, rate=None):
+def serving_leg(n, load, seed, n_baseline, rate=None, inner="f32"):
     """Continuous-batching engine (4 workers / HIP streams, FP32-inner mixed solvers, HBM budget 160 GB):
     its capacity with all n requests queued at once, then latency under a Poisson stream of the same
     requests offered at `load` times that capacity (or at `rate`); against the plain FP64 solvers one
@@ -360,7 +360,7 @@ def serving_leg(n, load, seed, n_baseline, rate=None):
         e.warmup()
         return e, cfgs
 
-    e, cfgs = engine(workers=4, mixed=True)
+    e, cfgs = engine(workers=4, mixed=True, inner_precision=H.F16 if inner == "f16" else H.F32)
     reqs = serve_requests(n, seed, cfgs)
     log(f"serving: {n} requests queued at once ...")
     closed = serve_run(e, reqs)
@@ -443,6 +443,8 @@ def main():
                     help="Poisson arrival rate as a fraction of the engine's measured capacity")
     ap.add_argument("--serve-rate", type=float, default=None, help="fixed arrival rate (requests per second)")
     ap.add_argument("--serve-baseline", type=int, default=10, help="requests also run by the FP64 baseline")
+    ap.add_argument("--serve-inner", choices=("f32", "f16"), default="f32",
+                    help="inner precision of the engine's single-right-hand-side twisted-mass solves")
     ap.add_argument("--hb-overrelax", type=int, default=1, help="overrelaxation sweeps per heatbath sweep")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the hopping kernels in the library's default launch shape (for profiles whose "
@@ -509,7 +511,8 @@ def main():
 
     serving = None
     if not args.no_serving:
-        serving = serving_leg(args.serve_requests, args.serve_load, 20261205, args.serve_baseline, args.serve_rate)
+        serving = serving_leg(args.serve_requests, args.serve_load, 20261205, args.serve_baseline, args.serve_rate,
+                              args.serve_inner)
 
     traffic = read_counters()
     achieved = f64["alg_bytes"] / (f64["ms"] * 1e6)
