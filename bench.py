@@ -168,6 +168,9 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
     for _ in range(sweeps):
         H.su3heatbath(gf, l, 1)
     qq, ap = H.QplusQminus_eo(l), l.wilson_parameters()
+    # launch shapes of the FP32 hopping kernel for one and for nrhs right-hand sides, autotuned on this lattice
+    H.autotune_dslash(gf, l, H.F32, 1)
+    H.autotune_dslash(gf, l, H.F32, nrhs)
     veo = l.eo_sites
     bs, xs = [], []
     for r in range(nrhs):
@@ -249,6 +252,9 @@ def multishift_leg(ns, nt, sweeps, seed, nshift=16):
         H.su3heatbath(gf, l, 1)
     plaq = H.measurePlaquette(gf)
     A, ap = H.MdagM_eo(l), l.staggered_parameters()
+    # launch shapes of the staggered hopping kernel for the FP32 inner loop and the FP64 operator, autotuned here
+    H.autotune_dks(gf, l, H.F32)
+    H.autotune_dks(gf, l, H.F64)
     sigma = [10.0 ** (-4 + 4 * k / (nshift - 1)) for k in range(nshift)]
     b = H.Staggeredfield_eo(l)
     b.import_(np.random.default_rng(seed).standard_normal((l.eo_sites, 6)))

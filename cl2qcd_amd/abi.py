@@ -115,6 +115,8 @@ SIGNATURES = {
     "cl2_block_cg_beta": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
     "cl2_autotune_dslash": (_i, [_vp, _vp, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_i), ctypes.POINTER(_i),
                                  ctypes.POINTER(_i), ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "cl2_autotune_dks": (_i, [_vp, _vp, _i, ctypes.POINTER(_d), ctypes.POINTER(_i), ctypes.POINTER(_i),
+                              ctypes.POINTER(_i), ctypes.POINTER(_i)]),
     "cl2_graph_begin": (_i, [_vp]),
     "cl2_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
     "cl2_graph_launch": (_i, [_vp, _vp, _i]),

@@ -1443,6 +1443,62 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     return CL2_OK;
 }
 
+/* The same for the staggered hopping kernel: z-planes per slab x XCD mapping x block size x register
+ * budget, timed on scratch staggered fields with the caller's gauge field. */
+int cl2_autotune_dks(cl2_ctx* c, const void* gauge, int fmt, double* best_ms, int* zplanes, int* xcd, int* block,
+                     int* occupancy)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, gauge && valid_fmt(fmt) && fmt != CL2_GAUGE_F16_12, "bad gauge argument");
+    const int prec  = fmt_prec(fmt);
+    const size_t fb = su3vec_bytes(c, prec);
+    TempDev in, out;
+    CL2_HIP(c, hipMalloc(&in.p, fb));
+    CL2_HIP(c, hipMalloc(&out.p, fb));
+    CL2_HIP(c, launch_fill(c, in.p, (size_t)3 * c->g.veo, prec, 0.1, 0.));
+    std::vector<int> zps = {-1, 0};
+    for (uint32_t d = 1; d <= c->g.ns; ++d)
+        if (c->g.ns % d == 0 && (size_t)d * c->g.ns * c->g.ns / 2 <= 262144) zps.push_back((int)d);
+    const std::vector<std::pair<int, int>> shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 1}, {128, 1}, {256, 4},
+                                                     {128, 4}};
+    const std::pair<int, int> key{fmt | DKS_TUNING_KEY, 1};
+    const bool manual = c->tune_manual;
+    c->tune_manual    = false;
+    double best       = 1e30;
+    cl2_hop_tuning bt{-1, 1, 1, 1, DSLASH_BLOCK, 0};
+    hipError_t e = hipSuccess;
+    for (int zp : zps)
+        for (int x : {1, 0})
+            for (const auto& sh : shapes) {
+                c->tuned[key] = cl2_hop_tuning{zp, x, 1, 1, sh.first, sh.second};
+                for (int k = 0; k < 2 && e == hipSuccess; ++k) e = launch_dks(c, out.p, in.p, gauge, fmt, CL2_EVEN);
+                if (e == hipSuccess) e = hipEventRecord(c->ev_start, c->stream);
+                for (int k = 0; k < 5 && e == hipSuccess; ++k)
+                    e = launch_dks(c, out.p, in.p, gauge, fmt, k & 1 ? CL2_ODD : CL2_EVEN);
+                if (e == hipSuccess) e = hipEventRecord(c->ev_stop, c->stream);
+                if (e == hipSuccess) e = hipEventSynchronize(c->ev_stop);
+                float ms = 0.f;
+                if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev_start, c->ev_stop);
+                if (e != hipSuccess) break;
+                if (ms / 5. < best) {
+                    best = ms / 5.;
+                    bt   = cl2_hop_tuning{zp, x, 1, 1, sh.first, sh.second};
+                }
+            }
+    c->tune_manual = manual;
+    if (e != hipSuccess) {
+        c->tuned.erase(key);
+        CL2_HIP(c, e);
+    }
+    c->tuned[key] = bt;
+    if (best_ms) *best_ms = best;
+    if (zplanes) *zplanes = bt.zplanes;
+    if (xcd) *xcd = bt.xcd;
+    if (block) *block = bt.block;
+    if (occupancy) *occupancy = bt.occ;
+    return CL2_OK;
+}
+
 /* ---------------- timing ---------------- */
 
 int cl2_timer_start(cl2_ctx* c)

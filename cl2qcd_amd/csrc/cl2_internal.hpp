@@ -61,6 +61,8 @@ inline unsigned blocks_for(size_t n, int block)
 cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs);
 /* work order of the hopping kernels for this context's lattice and a launch shape */
 WorkMap work_map(const cl2_ctx* c, int block, int nrhs, const cl2_hop_tuning& t);
+/* tuning-table key of the staggered hopping kernel: its link format | DKS_TUNING_KEY */
+constexpr int DKS_TUNING_KEY = 0x100;
 /* sites per block of the multi-RHS kernel that stages links in LDS (blocks of sites x nrhs threads) */
 int mrhs_sites_per_block(int nrhs);
 /* right-hand-side counts with a compiled multi-RHS hopping kernel */

@@ -1,5 +1,8 @@
 // Staggered even-odd hopping kernel D_KS_eo for gfx950.  Same site-per-thread
-// SoA scheme as the Wilson kernel; links are 92 % of the traffic.
+// SoA scheme, tiled links, work order and launch variants as the Wilson kernel;
+// links are 92 % of the traffic.
+#include <type_traits>
+
 #include "cl2_device.hpp"
 #include "cl2_internal.hpp"
 
@@ -15,12 +18,36 @@ struct DksArgs {
     WorkMap map;
 };
 
+/* forward and backward hop of one direction on neighbour vectors already in registers:
+ *   acc += w U_mu(x) chi(x+mu) - conj(w) U_mu(x-mu)^dagger chi(x-mu) */
+template<typename R>
+__device__ __forceinline__ void dks_hops(Cx<R> (&acc)[3], const Su3<R>& uf, const vec2<R> (&vf)[3], const Su3<R>& ub,
+                                         const vec2<R> (&vb)[3], Cx<R> w)
+{
+    Cx<R> v[3], r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = Cx<R>{vf[c].x, vf[c].y};
+    su3_times(uf, v, r);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] = acc[c] + cmul(r[c], w);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = Cx<R>{vb[c].x, vb[c].y};
+    su3dag_times(ub, v, r);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] = acc[c] - cmulconj(r[c], w);
+}
+
 /* D_KS_eo_local (ocl_kernel/fermionmatrix_staggered_eo_DKS_local.cl:65-136):
  *   sum_mu eta_mu(x) [ sf_mu U_mu(x) chi(x+mu) - conj(sf_mu) U_mu(x-mu)^dagger chi(x-mu) ]
  * with eta_X = 1, eta_Y = (-1)^x, eta_Z = (-1)^(x+y), eta_T = (-1)^(x+y+z)
- * (ocl_kernel/operations_staggered.cl:56-104). */
-template<int FMT, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_dks_eo(DksArgs a)
+ * (ocl_kernel/operations_staggered.cl:56-104).  Directions T, X, Y, Z, forward then backward.  Register
+ * budget and loads in flight as the Wilson kernel: OCC = 0 both hops of a direction (<= 2 waves per SIMD),
+ * 1 the four hops of two directions (1 wave), 4 one hop at a time (>= 4 waves); every variant computes the
+ * same bits.  Results leave by streaming stores. */
+template<int FMT, int BLOCK, int OCC>
+__global__ __launch_bounds__(BLOCK)
+__attribute__((amdgpu_waves_per_eu(OCC == 4 ? 4 : 1, OCC == 4 ? 8 : (OCC == 1 ? 1 : 2))))
+void k_dks_eo(DksArgs a)
 {
     using R = typename GaugeLoad<FMT>::R;
     using V = vec2<R>;
@@ -37,33 +64,62 @@ __global__ __launch_bounds__(BLOCK) void k_dks_eo(DksArgs a)
 #pragma unroll
     for (int k = 0; k < 3; ++k) acc[k] = Cx<R>{R(0), R(0)};
     const R eta[4] = {R(((x + y + z) & 1u) ? -1 : 1), R(1), R((x & 1u) ? -1 : 1), R(((x + y) & 1u) ? -1 : 1)};
+    auto wt = [&](int mu) { return Cx<R>{R(a.sf[mu][0]) * eta[mu], R(a.sf[mu][1]) * eta[mu]}; };
+    if constexpr (OCC == 1) {
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-        Su3<R> u;
-        Cx<R> v[3], r[3];
-        const Cx<R> w{R(a.sf[mu][0]) * eta[mu], R(a.sf[mu][1]) * eta[mu]};
-        GaugeLoad<FMT>::load(a.gauge, S, mu, P, i, u);
+        for (int m0 = 0; m0 < 4; m0 += 2) {
+            Su3<R> u[4];
+            V v[4][3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            V q  = in[(size_t)c * S + nb.up[mu]];
-            v[c] = Cx<R>{q.x, q.y};
+            for (int d = 0; d < 2; ++d) {
+                GaugeLoad<FMT>::load(a.gauge, S, m0 + d, P, i, u[2 * d]);
+                GaugeLoad<FMT>::load(a.gauge, S, m0 + d, P ^ 1, nb.dn[m0 + d], u[2 * d + 1]);
+            }
+#pragma unroll
+            for (int d = 0; d < 2; ++d)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    v[2 * d][c]     = in[(size_t)c * S + nb.up[m0 + d]];
+                    v[2 * d + 1][c] = in[(size_t)c * S + nb.dn[m0 + d]];
+                }
+            dks_hops(acc, u[0], v[0], u[1], v[1], wt(m0));
+            dks_hops(acc, u[2], v[2], u[3], v[3], wt(m0 + 1));
         }
-        su3_times(u, v, r);
+    } else {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] = acc[c] + cmul(r[c], w);
-        GaugeLoad<FMT>::load(a.gauge, S, mu, P ^ 1, nb.dn[mu], u);
+        for (int mu = 0; mu < 4; ++mu) {
+            Su3<R> uf, ub;
+            V vf[3], vb[3];
+            GaugeLoad<FMT>::load(a.gauge, S, mu, P, i, uf);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            V q  = in[(size_t)c * S + nb.dn[mu]];
-            v[c] = Cx<R>{q.x, q.y};
+            for (int c = 0; c < 3; ++c) vf[c] = in[(size_t)c * S + nb.up[mu]];
+            if constexpr (OCC == 4) {  // one hop at a time: the forward hop is applied before the backward loads
+                Cx<R> vv[3], r[3];
+                const Cx<R> w = wt(mu);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) vv[c] = Cx<R>{vf[c].x, vf[c].y};
+                su3_times(uf, vv, r);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[c] = acc[c] + cmul(r[c], w);
+                GaugeLoad<FMT>::load(a.gauge, S, mu, P ^ 1, nb.dn[mu], ub);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) vb[c] = in[(size_t)c * S + nb.dn[mu]];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) vv[c] = Cx<R>{vb[c].x, vb[c].y};
+                su3dag_times(ub, vv, r);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc[c] = acc[c] - cmulconj(r[c], w);
+            } else {
+                GaugeLoad<FMT>::load(a.gauge, S, mu, P ^ 1, nb.dn[mu], ub);
+#pragma unroll
+                for (int c = 0; c < 3; ++c) vb[c] = in[(size_t)c * S + nb.dn[mu]];
+                dks_hops(acc, uf, vf, ub, vb, wt(mu));
+            }
         }
-        su3dag_times(u, v, r);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[c] = acc[c] - cmulconj(r[c], w);
     }
     V* out = reinterpret_cast<V*>(a.out);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) out[(size_t)c * S + i] = V{acc[c].re, acc[c].im};
+    for (int c = 0; c < 3; ++c) store_stream(out + (size_t)c * S + i, V{acc[c].re, acc[c].im});
 }
 
 hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd)
@@ -74,7 +130,9 @@ hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, 
     a.gauge      = gauge;
     a.g          = c->g;
     a.out_parity = evenodd;
-    a.map        = work_map(c, DSLASH_BLOCK, 1, hop_tuning(c, format | 0x100, 1));
+    const cl2_hop_tuning t = hop_tuning(c, format | DKS_TUNING_KEY, 1);
+    const int bs           = (t.block == 128 || t.block == 512) ? t.block : 256;
+    a.map                  = work_map(c, bs, 1, t);
     for (int mu = 0; mu < 4; ++mu) {
         const double* b = (mu == 0) ? c->bt : c->bs;
         double re = 0.5 * b[0], im = 0.5 * b[1];
@@ -87,17 +145,28 @@ hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, 
         a.sf[mu][0] = re;
         a.sf[mu][1] = im;
     }
-    const unsigned nb = blocks_for(c->g.veo, DSLASH_BLOCK);
+    const unsigned nb = blocks_for(c->g.veo, bs);
+    auto go = [&](auto fmt_tag, auto block_tag) {
+        constexpr int F = decltype(fmt_tag)::value, B = decltype(block_tag)::value;
+        if (t.occ == 4)
+            hipLaunchKernelGGL((k_dks_eo<F, B, 4>), dim3(nb), dim3(B), 0, c->stream, a);
+        else if (t.occ == 1 && B != 512)
+            hipLaunchKernelGGL((k_dks_eo<F, B, 1>), dim3(nb), dim3(B), 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_dks_eo<F, B, 0>), dim3(nb), dim3(B), 0, c->stream, a);
+    };
+    auto by_block = [&](auto fmt_tag) {
+        if (bs == 128)
+            go(fmt_tag, std::integral_constant<int, 128>{});
+        else if (bs == 512)
+            go(fmt_tag, std::integral_constant<int, 512>{});
+        else
+            go(fmt_tag, std::integral_constant<int, 256>{});
+    };
     switch (format) {
-        case GAUGE_F64_18:
-            hipLaunchKernelGGL((k_dks_eo<GAUGE_F64_18, DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
-            break;
-        case GAUGE_F64_12:
-            hipLaunchKernelGGL((k_dks_eo<GAUGE_F64_12, DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
-            break;
-        case GAUGE_F32_12:
-            hipLaunchKernelGGL((k_dks_eo<GAUGE_F32_12, DSLASH_BLOCK>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream, a);
-            break;
+        case GAUGE_F64_18: by_block(std::integral_constant<int, GAUGE_F64_18>{}); break;
+        case GAUGE_F64_12: by_block(std::integral_constant<int, GAUGE_F64_12>{}); break;
+        case GAUGE_F32_12: by_block(std::integral_constant<int, GAUGE_F32_12>{}); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

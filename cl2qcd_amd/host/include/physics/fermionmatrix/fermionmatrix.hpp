@@ -110,6 +110,8 @@ CL2_EO_OPERATOR(QplusQminus_eo, CL2_OP_QPLUSQMINUS_EO, true)
 /* autotune the hopping kernel's launch shape for links matching prec and nrhs right-hand sides on gf's
  * device (no counterpart in the reference); returns the best time per application in ms */
 double autotune_dslash(const lattices::Gaugefield& gf, const hardware::System& system, int prec, int nrhs = 1);
+/* the same for the staggered hopping kernel with links matching prec (FP64 or FP32) */
+double autotune_dks(const lattices::Gaugefield& gf, const hardware::System& system, int prec);
 
 /* A applied to every right-hand side of a block in one pass over the links (A must be one of the built-in
  * classes; tmp holds two block fields of in's shape).  No counterpart in the reference. */

@@ -418,6 +418,13 @@ PYBIND11_MODULE(_host, m)
             return fm::autotune_dslash(gf, l.system, prec, nrhs);
         },
         py::arg("gf"), py::arg("lattice"), py::arg("prec") = CL2_F64, py::arg("nrhs") = 1);
+    m.def(
+        "autotune_dks",
+        [](const Gaugefield& gf, Lattice& l, int prec) {
+            py::gil_scoped_release rel;
+            return fm::autotune_dks(gf, l.system, prec);
+        },
+        py::arg("gf"), py::arg("lattice"), py::arg("prec") = CL2_F64);
     m.def("convert_precision",
           [](const Spinorfield_eo& to, const Spinorfield_eo& from) { convert_precision(&to, from); });
     m.def(

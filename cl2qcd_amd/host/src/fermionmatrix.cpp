@@ -187,6 +187,15 @@ double autotune_dslash(const Gaugefield& gf, const hardware::System& system, int
     return ms;
 }
 
+double autotune_dks(const Gaugefield& gf, const hardware::System& system, int prec)
+{
+    if (prec != CL2_F64 && prec != CL2_F32) throw std::invalid_argument("autotune_dks: FP64 or FP32 links");
+    const Links l = links_for(gf, prec);
+    double ms     = 0.;
+    system.check(cl2_autotune_dks(system.ctx(), l.ptr, l.fmt, &ms, nullptr, nullptr, nullptr, nullptr));
+    return ms;
+}
+
 void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* out, const Gaugefield& gf,
                  const lattices::Spinorfield_eo_block& in, const physics::AdditionalParameters& ap,
                  const lattices::DeviceBuffer& tmp)

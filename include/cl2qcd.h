@@ -379,6 +379,12 @@ long cl2_tune_get(const cl2_ctx* ctx, const char* key);
 int cl2_autotune_dslash(cl2_ctx* ctx, const void* gauge, int format, int nrhs, double* best_ms, int* zplanes,
                         int* xcd, int* rhs_per_thread, int* block, int* occupancy);
 
+/* No counterpart in the reference: the same autotuning for the staggered hopping kernel (cl2_D_KS_eo and the
+ * staggered operators built on it) with links of the given format (FP64 or FP32): z-planes per slab, XCD
+ * mapping, block size and register budget. */
+int cl2_autotune_dks(cl2_ctx* ctx, const void* gauge, int format, double* best_ms, int* zplanes, int* xcd,
+                     int* block, int* occupancy);
+
 /* ==== timing on the context stream (Device profiling events, hardware/device.cpp:233-242) ==== */
 int cl2_timer_start(cl2_ctx* ctx);
 int cl2_timer_stop(cl2_ctx* ctx, double* ms);

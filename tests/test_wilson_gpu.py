@@ -326,3 +326,36 @@ def test_padded_link_tiles():
         assert abs(gf.plaquette() - O.plaquette(lat, g)) < 1e-12
         gf.heatbath(5.7, 2, overrelax=1, seed=3)
         assert abs(gf.plaquette() - O.plaquette(lat, gf.download())) < 1e-12
+
+
+@pytest.mark.parametrize("fmt,prec", [(abi.GAUGE_F64_18, abi.F64), (abi.GAUGE_F32_12, abi.F32)])
+def test_staggered_launch_variants_bit_identical(fmt, prec):
+    """D_KS_eo gives the same bits for every block size x register budget x work order, the autotuner keeps
+    them, and the FP64 result agrees with the oracle."""
+    import ctypes
+    lat, ctx = make(8, 8, theta_t=1.0)
+    with ctx:
+        g = random_su3_field(4 * O.volume(lat), 95)
+        gf = ctx.gauge(g, fmt)
+        src = np.random.default_rng(96).standard_normal((ctx.veo, 6))
+        a, b = ctx.su3vec(prec).upload(src), ctx.su3vec(prec)
+        results = {}
+        for block in (128, 256, 512):
+            for occ in (0, 1, 4):
+                for zp, xcd in ((-1, 1), (0, 0), (2, 1)):
+                    for key, v in (("hop.block", block), ("hop.occupancy", occ), ("hop.zplanes", zp), ("hop.xcd", xcd)):
+                        ctx.check(ctx.lib.cl2_tune_set(ctx.handle, key.encode(), v))
+                    ctx.D_KS_eo(b.ptr, a.ptr, gf.ptr, fmt, abi.ODD)
+                    results[(block, occ, zp, xcd)] = b.download()
+        ref = results[(256, 0, -1, 1)]
+        for k, v in results.items():
+            assert np.array_equal(v, ref), k
+        ctx.check(ctx.lib.cl2_tune_set(ctx.handle, b"hop.occupancy", -1))
+        ms, zp, xcd, blk, occ = ctypes.c_double(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ctx.autotune_dks(gf.ptr, fmt, ctypes.byref(ms), ctypes.byref(zp), ctypes.byref(xcd), ctypes.byref(blk),
+                         ctypes.byref(occ))
+        assert ms.value > 0 and blk.value in (128, 256, 512) and occ.value in (0, 1, 4)
+        ctx.D_KS_eo(b.ptr, a.ptr, gf.ptr, fmt, abi.ODD)
+        assert np.array_equal(b.download(), ref)
+        if prec == abi.F64:
+            assert site_rel_err(ref, O.dks_eo(lat, src, g, O.ODD)) < 1e-13
