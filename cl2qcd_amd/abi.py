@@ -93,6 +93,7 @@ SIGNATURES = {
     "cl2_gamma5_eo": (_i, [_vp, _vp, _i]),
     "cl2_saxpy_AND_gamma5_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP]),
     "cl2_fermionmatrix_eo": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _vp]),
+    "cl2_fermionmatrix_eo_dot": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _dp]),
     "cl2_saxpy_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP]),
     "cl2_sax_eo": (_i, [_vp, _vp, _vp, _i, _SP]),
     "cl2_saxsbypz_eo": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _SP, _SP]),
@@ -100,6 +101,7 @@ SIGNATURES = {
     "cl2_squarenorm_eo": (_i, [_vp, _dp, _vp, _i]),
     "cl2_saxpy_AND_squarenorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
     "cl2_saxpy_AND_cgnorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
+    "cl2_cg_update_xp": (_i, [_vp, _i, _vp, _vp, _vp, _i, _SP, _SP]),
     "cl2_fermionmatrix_eo_mrhs": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _i]),
     "cl2_fermionmatrix_eo_mrhs_params": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, ctypes.POINTER(_d), ctypes.POINTER(_d),
                                               _vp, _i]),

@@ -118,7 +118,8 @@ int check_scalar(cl2_ctx* c, const cl2_scalar* s)
 
 /* elementwise BLAS dispatch shared by the Wilson and staggered entry points */
 int run_blas(cl2_ctx* c, int mode, size_t n, void* out, const void* x, const void* y, const void* z, int prec,
-             const cl2_scalar& a, const cl2_scalar& b, double* norm_res, size_t tail, int cg_norm = 0)
+             const cl2_scalar& a, const cl2_scalar& b, double* norm_res, size_t tail, int cg_norm = 0,
+             void* out2 = nullptr)
 {
     const bool half = prec == CL2_F16;
     CL2_REQUIRE(c, valid_prec(prec) || (half && n == (size_t)12 * c->g.veo),
@@ -128,6 +129,7 @@ int run_blas(cl2_ctx* c, int mode, size_t n, void* out, const void* x, const voi
     args.n             = n;
     args.tail          = tail;
     args.out           = out;
+    args.out2          = out2;
     args.x             = x;
     args.y             = y;
     args.z             = z;
@@ -170,7 +172,7 @@ struct RhsParams {
 
 int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void* gauge, int fmt, int evenodd,
                double kappa, int epilogue, const double f1[2], const double f2[2], double scale, int gamma5,
-               int nrhs = 1, const RhsParams* pr = nullptr)
+               int nrhs = 1, const RhsParams* pr = nullptr, double* norm_res = nullptr)
 {
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
     CL2_REQUIRE(c, supported_nrhs(nrhs) && (nrhs == 1 || fmt != CL2_GAUGE_F16_12),
@@ -194,7 +196,7 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.f2[0]         = f2[0];
     a.f2[1]         = f2[1];
     a.scale         = scale;
-    a.norm_partials = nullptr;
+    a.norm_partials = norm_res ? c->partials : nullptr;
     a.per_rhs       = (pr != nullptr && nrhs > 1) ? 1 : 0;
     a.w             = wilson_weights(c, a.per_rhs ? 1.0 : kappa);
     for (int r = 0; r < 16; ++r) {
@@ -214,7 +216,10 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.rb            = a.per_rhs ? 1 : t.rb;  // per-right-hand-side operators: one right-hand side per thread
     a.lds           = t.lds;
     if (nrhs > 1 && t.lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
+    CL2_REQUIRE(c, norm_res == nullptr || nrhs == 1, "output norms are for single fields");
     CL2_HIP(c, launch_dslash(c, a, fmt));
+    // every single-field variant launches blocks_for(Veo, block) blocks, each writing one partial
+    if (norm_res) CL2_HIP(c, launch_reduce_stage2(c, blocks_for(a.g.veo, a.block), norm_res, 1, nullptr, 0));
     return CL2_OK;
 }
 
@@ -250,7 +255,7 @@ void tm_factors(int variant, double mubar, double f1[2], double f2[2], double* s
  * out = M_site in - D tmp [gamma5]   (fermionmatrix.cpp:224-591). */
 int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge, int fmt, int action, double kappa,
             double mubar, bool minus, bool gamma5, int nrhs = 1, const double* kappas = nullptr,
-            const double* mubars = nullptr)
+            const double* mubars = nullptr, double* norm_res = nullptr)
 {
     double f1[2], f2[2], s;
     int rc;
@@ -282,19 +287,21 @@ int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge,
         const double one[2] = {1., 0.};
         rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_PLAIN, one, one, 1., 0, nrhs);
         if (rc) return rc;
-        return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, one, one, 1., gamma5 ? 1 : 0, nrhs);
+        return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, one, one, 1., gamma5 ? 1 : 0, nrhs,
+                          nullptr, norm_res);
     }
     tm_factors(minus ? CL2_MTM_INVERSE_SITEDIAGONAL_MINUS : CL2_MTM_INVERSE_SITEDIAGONAL, mubar, f1, f2, &s);
     rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_MINV, f1, f2, s, 0, nrhs);
     if (rc) return rc;
     tm_factors(minus ? CL2_MTM_SITEDIAGONAL_MINUS : CL2_MTM_SITEDIAGONAL, mubar, f1, f2, &s);
-    return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, f1, f2, 1., gamma5 ? 1 : 0, nrhs);
+    return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, f1, f2, 1., gamma5 ? 1 : 0, nrhs, nullptr,
+                      norm_res);
 }
 
 /* the eo operators on nrhs right-hand sides ([12][Veo][nrhs] fields, tmp = 2 such fields) */
 int run_fermionmatrix(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt, int action,
                       double kappa, double mubar, void* tmp, int nrhs, const double* kappas = nullptr,
-                      const double* mubars = nullptr)
+                      const double* mubars = nullptr, double* dot_res = nullptr)
 {
     CL2_REQUIRE(c, op >= CL2_OP_AEE && op <= CL2_OP_QPLUSQMINUS_EO, "unknown operator");
     CL2_REQUIRE(c, action == CL2_ACTION_WILSON || action == CL2_ACTION_TWISTEDMASS,
@@ -316,7 +323,10 @@ int run_fermionmatrix(cl2_ctx* c, int op, void* out, const void* in, const void*
         case CL2_OP_AEE_MINUS_AND_GAMMA5:
         case CL2_OP_QMINUS_EO: return run_aee(c, out, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs, ks, ms);
         default: {
-            int rc = run_aee(c, t1, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs, ks, ms);
+            // QplusQminus = Qplus Qminus with Qplus = Qminus^dagger: <in, out> = |Qminus in|^2, the norm of the
+            // intermediate field, summed by its producing kernel
+            CL2_REQUIRE(c, dot_res == nullptr || (nrhs == 1 && ks == nullptr), "<in, out> is for single fields");
+            int rc = run_aee(c, t1, in, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs, ks, ms, dot_res);
             if (rc) return rc;
             return run_aee(c, out, t1, t0, gauge, fmt, action, kappa, mubar, false, true, nrhs, ks, ms);
         }
@@ -362,9 +372,9 @@ int cl2_ctx_create(cl2_ctx** out, int device, const cl2_lattice* lat)
     c->cp_fwd[1] = er * ci;
     c->cp_bwd[0] = mer * cr;
     c->cp_bwd[1] = mer * ci;
-    // one double2 per block: REDUCE_MAX_BLOCKS blocks for each of up to 16 right-hand sides, or one per
-    // hopping-kernel block
-    c->npartials = std::max<size_t>((size_t)REDUCE_MAX_BLOCKS * 16, blocks_for(c->g.veo, DSLASH_BLOCK));
+    // one double2 per block: REDUCE_MAX_BLOCKS blocks for each of up to 16 right-hand sides, BLAS_MAX_BLOCKS
+    // for an elementwise kernel, or one per block of the smallest hopping-kernel block (128 threads)
+    c->npartials = std::max<size_t>({(size_t)REDUCE_MAX_BLOCKS * 16, (size_t)BLAS_MAX_BLOCKS, blocks_for(c->g.veo, 128)});
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, c->npartials * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
@@ -790,6 +800,15 @@ int cl2_fermionmatrix_eo(cl2_ctx* c, int op, void* out, const void* in, const vo
     return run_fermionmatrix(c, op, out, in, gauge, fmt, action, kappa, mubar, tmp, 1);
 }
 
+int cl2_fermionmatrix_eo_dot(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt, int action,
+                             double kappa, double mubar, void* tmp, double* dot)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, op == CL2_OP_QPLUSQMINUS_EO, "<in, A in> with the operator is for QplusQminus_eo");
+    CL2_REQUIRE(c, dot != nullptr, "null result pointer");
+    return run_fermionmatrix(c, op, out, in, gauge, fmt, action, kappa, mubar, tmp, 1, nullptr, nullptr, dot);
+}
+
 int cl2_fermionmatrix_eo_mrhs(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt, int action,
                               double kappa, double mubar, void* tmp, int nrhs)
 {
@@ -872,6 +891,20 @@ int cl2_saxpy_AND_cgnorm_eo(cl2_ctx* c, void* out, const void* x, const void* y,
     if (int rc = check_scalar(c, alpha)) return rc;
     CL2_REQUIRE(c, y && res, "null pointer");
     return run_blas(c, B_Y_MINUS_AX, (size_t)12 * c->g.veo, out, x, y, nullptr, prec, *alpha, value(0.), res, 0, 1);
+}
+
+int cl2_cg_update_xp(cl2_ctx* c, int kind, void* x, void* p, const void* r, int prec, const cl2_scalar* alpha,
+                     const cl2_scalar* beta)
+{
+    if (!c) return CL2_E_ARG;
+    if (int rc = check_scalar(c, alpha)) return rc;
+    if (int rc = check_scalar(c, beta)) return rc;
+    CL2_REQUIRE(c, kind == CL2_FIELD_SPINOR_EO || kind == CL2_FIELD_SU3VEC_EO, "kind must be a spinor or su3vec field");
+    CL2_REQUIRE(c, x && p && r, "null field pointer");
+    CL2_REQUIRE(c, x != p && x != r && p != r, "x, p and r must be different fields");
+    const size_t n = (size_t)(kind == CL2_FIELD_SPINOR_EO ? 12 : 3) * c->g.veo;
+    // out = z - a x on (x <- p, y <- r, z <- x), out2 = y - b x
+    return run_blas(c, B_CG_XP, n, x, p, r, x, prec, *alpha, *beta, nullptr, 0, 0, p);
 }
 
 int cl2_gaussian_eo(cl2_ctx* c, int kind, void* out, int prec, uint64_t seed)

@@ -23,62 +23,111 @@ __device__ __forceinline__ Cx<double> resolve(const cl2_scalar& s)
     }
 }
 
-template<typename R, int BLOCK>
+/* 16-byte chunks of a complex field: one FP64 complex number, or two FP32 numbers when every operand is
+ * 16-byte aligned and the length even (else one FP32 number per 8-byte chunk).  Loads and stores are
+ * streaming: a BLAS pass touches each element once. */
+template<typename R, int CPC>
+struct Chunk;
+template<>
+struct Chunk<double, 1> {
+    __device__ static void load(const void* p, size_t c, Cx<double> (&v)[1])
+    {
+        const double2 w = load_stream(reinterpret_cast<const double2*>(p) + c);
+        v[0]            = Cx<double>{w.x, w.y};
+    }
+    __device__ static void store(void* p, size_t c, const Cx<double> (&v)[1])
+    {
+        store_stream(reinterpret_cast<double2*>(p) + c, make_double2(v[0].re, v[0].im));
+    }
+};
+template<>
+struct Chunk<float, 2> {
+    __device__ static void load(const void* p, size_t c, Cx<float> (&v)[2])
+    {
+        const float4 w = load_stream(reinterpret_cast<const float4*>(p) + c);
+        v[0]           = Cx<float>{w.x, w.y};
+        v[1]           = Cx<float>{w.z, w.w};
+    }
+    __device__ static void store(void* p, size_t c, const Cx<float> (&v)[2])
+    {
+        store_stream(reinterpret_cast<float4*>(p) + c, make_float4(v[0].re, v[0].im, v[1].re, v[1].im));
+    }
+};
+template<>
+struct Chunk<float, 1> {
+    __device__ static void load(const void* p, size_t c, Cx<float> (&v)[1])
+    {
+        const float2 w = load_stream(reinterpret_cast<const float2*>(p) + c);
+        v[0]           = Cx<float>{w.x, w.y};
+    }
+    __device__ static void store(void* p, size_t c, const Cx<float> (&v)[1])
+    {
+        store_stream(reinterpret_cast<float2*>(p) + c, make_float2(v[0].re, v[0].im));
+    }
+};
+
+/* Elementwise BLAS: each thread takes U chunks BLOCK apart, issues all their loads, then computes and
+ * stores; the grid covers the field (up to BLAS_MAX_BLOCKS blocks, beyond that it loops).  Operands of
+ * one element are read before the element is written, so out may alias any input (in-place updates). */
+template<typename R, int CPC, int U, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_blas(BlasArgs a)
 {
-    using V = vec2<R>;
+    using IO = Chunk<R, CPC>;
     __shared__ double lds[2 * BLOCK / 64];
     const Cx<double> A = resolve(a.a), B = resolve(a.b);
     const Cx<R> al{R(A.re), R(A.im)}, be{R(B.re), R(B.im)};
-    const V* x = reinterpret_cast<const V*>(a.x);
-    const V* y = reinterpret_cast<const V*>(a.y);
-    const V* z = reinterpret_cast<const V*>(a.z);
-    V* out     = reinterpret_cast<V*>(a.out);
+    const int mode      = a.mode;
+    const bool has_y    = !(mode == B_AX || mode == B_NEG_TAIL);
+    const bool has_z    = mode == B_AX_BY_Z || mode == B_CG_XP;
+    const size_t nchunk = a.n / CPC;
     double nrm = 0., ry = 0.;
-    const size_t step = (size_t)gridDim.x * BLOCK;
-    for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < a.n; k += step) {
-        Cx<R> r;
-        switch (a.mode) {
-            case B_Y_MINUS_AX: {
-                V xv = x[k], yv = y[k];
-                r    = Cx<R>{yv.x, yv.y} - cmul(Cx<R>{xv.x, xv.y}, al);
-                ry += (double)r.re * (double)yv.x + (double)r.im * (double)yv.y;
-                break;
-            }
-            case B_AX: {
-                V xv = x[k];
-                r    = cmul(Cx<R>{xv.x, xv.y}, al);
-                break;
-            }
-            case B_AX_BY_Z: {
-                V xv = x[k], yv = y[k], zv = z[k];
-                r    = cmul(Cx<R>{yv.x, yv.y}, be) + cmul(Cx<R>{xv.x, xv.y}, al) + Cx<R>{zv.x, zv.y};
-                break;
-            }
-            case B_AX_PLUS_Y: {
-                V xv = x[k], yv = y[k];
-                r    = cmul(Cx<R>{xv.x, xv.y}, al) + Cx<R>{yv.x, yv.y};
-                break;
-            }
-            case B_AX_BY: {
-                V xv = x[k], yv = y[k];
-                r    = cmul(Cx<R>{xv.x, xv.y}, al) + cmul(Cx<R>{yv.x, yv.y}, be);
-                break;
-            }
-            case B_G5_Y_MINUS_AX: {
-                V xv = x[k], yv = y[k];
-                r    = Cx<R>{yv.x, yv.y} - cmul(Cx<R>{xv.x, xv.y}, al);
-                if (k >= a.tail) r = Cx<R>{-r.re, -r.im};
-                break;
-            }
-            default: {  // B_NEG_TAIL: in-place gamma5
-                V xv = x[k];
-                r    = (k >= a.tail) ? Cx<R>{-xv.x, -xv.y} : Cx<R>{xv.x, xv.y};
-                break;
+    const size_t step = (size_t)gridDim.x * BLOCK * U;
+    for (size_t base = (size_t)blockIdx.x * BLOCK * U + threadIdx.x; base < nchunk; base += step) {
+        Cx<R> xv[U][CPC], yv[U][CPC], zv[U][CPC];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t c = base + (size_t)u * BLOCK;
+            if (c < nchunk) {
+                IO::load(a.x, c, xv[u]);
+                if (has_y) IO::load(a.y, c, yv[u]);
+                if (has_z) IO::load(a.z, c, zv[u]);
             }
         }
-        out[k] = V{r.re, r.im};
-        nrm += (double)r.re * (double)r.re + (double)r.im * (double)r.im;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t c = base + (size_t)u * BLOCK;
+            if (c >= nchunk) continue;
+            Cx<R> r[CPC], r2[CPC];
+#pragma unroll
+            for (int j = 0; j < CPC; ++j) {
+                const size_t k = c * CPC + j;
+                const Cx<R> x = xv[u][j], y = yv[u][j], z = zv[u][j];
+                switch (mode) {
+                    case B_Y_MINUS_AX:
+                        r[j] = y - cmul(x, al);
+                        ry += (double)r[j].re * (double)y.re + (double)r[j].im * (double)y.im;
+                        break;
+                    case B_AX: r[j] = cmul(x, al); break;
+                    case B_AX_BY_Z: r[j] = cmul(y, be) + cmul(x, al) + z; break;
+                    case B_AX_PLUS_Y: r[j] = cmul(x, al) + y; break;
+                    case B_AX_BY: r[j] = cmul(x, al) + cmul(y, be); break;
+                    case B_G5_Y_MINUS_AX:
+                        r[j] = y - cmul(x, al);
+                        if (k >= a.tail) r[j] = Cx<R>{-r[j].re, -r[j].im};
+                        break;
+                    case B_CG_XP:
+                        r[j]  = z - cmul(x, al);
+                        r2[j] = y - cmul(x, be);
+                        break;
+                    default:  // B_NEG_TAIL: in-place gamma5
+                        r[j] = (k >= a.tail) ? Cx<R>{-x.re, -x.im} : x;
+                        break;
+                }
+                nrm += (double)r[j].re * (double)r[j].re + (double)r[j].im * (double)r[j].im;
+            }
+            IO::store(a.out, c, r);
+            if (mode == B_CG_XP) IO::store(a.out2, c, r2);
+        }
     }
     if (a.norm_partials) {
         double2 s = block_sum2<BLOCK>(make_double2(nrm, a.cg_norm ? ry : 0.), lds);
@@ -86,41 +135,80 @@ __global__ __launch_bounds__(BLOCK) void k_blas(BlasArgs a)
     }
 }
 
-static unsigned reduce_blocks(size_t n)
+/* chunks per thread of the elementwise kernels: one when nothing is reduced (the shape that streams
+ * fastest), four with per-block partials (a quarter of the partials for stage 2 to sum) */
+constexpr int BLAS_U_PLAIN = 1, BLAS_U_REDUCE = 4;
+
+static unsigned chunk_blocks(size_t nchunk, int u)
 {
-    unsigned nb = blocks_for(n, BLAS_BLOCK);
-    return nb > (unsigned)REDUCE_MAX_BLOCKS ? (unsigned)REDUCE_MAX_BLOCKS : (nb == 0 ? 1u : nb);
+    const size_t nb = (nchunk + (size_t)BLAS_BLOCK * u - 1) / ((size_t)BLAS_BLOCK * u);
+    return nb == 0 ? 1u : (unsigned)std::min<size_t>(nb, BLAS_MAX_BLOCKS);
+}
+
+static bool aligned16(const void* p)
+{
+    return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+template<typename R, int CPC>
+static hipError_t launch_blas_t(cl2_ctx* c, const BlasArgs& a, unsigned* nblocks_out)
+{
+    const size_t nchunk = a.n / CPC;
+    if (a.norm_partials) {
+        const unsigned nb = chunk_blocks(nchunk, BLAS_U_REDUCE);
+        if (nblocks_out) *nblocks_out = nb;
+        hipLaunchKernelGGL((k_blas<R, CPC, BLAS_U_REDUCE, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, a);
+    } else {
+        const unsigned nb = chunk_blocks(nchunk, BLAS_U_PLAIN);
+        if (nblocks_out) *nblocks_out = nb;
+        hipLaunchKernelGGL((k_blas<R, CPC, BLAS_U_PLAIN, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, a);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_blas(cl2_ctx* c, const BlasArgs& a, int prec, unsigned* nblocks_out)
 {
-    const unsigned nb = reduce_blocks(a.n);
-    if (nblocks_out) *nblocks_out = nb;
-    if (prec == CL2_F64)
-        hipLaunchKernelGGL((k_blas<double, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, a);
-    else
-        hipLaunchKernelGGL((k_blas<float, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, a);
-    return hipGetLastError();
+    if (prec == CL2_F64) return launch_blas_t<double, 1>(c, a, nblocks_out);
+    const bool pairs = (a.n % 2) == 0 && aligned16(a.out) && aligned16(a.x) && aligned16(a.y) &&
+                       aligned16(a.z) && (a.mode != B_CG_XP || aligned16(a.out2));
+    return pairs ? launch_blas_t<float, 2>(c, a, nblocks_out) : launch_blas_t<float, 1>(c, a, nblocks_out);
 }
 
-/* stage 1: per-block sums of conj(x) y (R_DOT), its real part (R_DOT_RE) or |x|^2 (R_NORM) */
-template<typename R, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_reduce1(const vec2<R>* __restrict__ x, const vec2<R>* __restrict__ y,
-                                                   size_t n, int mode, double* __restrict__ partials)
+/* stage 1: per-block sums of conj(x) y (R_DOT), its real part (R_DOT_RE) or |x|^2 (R_NORM), chunked as the
+ * elementwise kernel */
+template<typename R, int CPC, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_reduce1(const void* x, const void* y, size_t n, int mode,
+                                                   double* __restrict__ partials)
 {
+    using IO = Chunk<R, CPC>;
+    constexpr int U = BLAS_U_REDUCE;
     __shared__ double lds[2 * BLOCK / 64];
-    double2 acc       = make_double2(0., 0.);
-    const size_t step = (size_t)gridDim.x * BLOCK;
-    if (mode == R_NORM) {
-        for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < n; k += step) {
-            vec2<R> v = x[k];
-            acc.x += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    double2 acc         = make_double2(0., 0.);
+    const size_t nchunk = n / CPC;
+    const size_t step   = (size_t)gridDim.x * BLOCK * U;
+    const bool norm     = mode == R_NORM;
+    for (size_t base = (size_t)blockIdx.x * BLOCK * U + threadIdx.x; base < nchunk; base += step) {
+        Cx<R> u[U][CPC], v[U][CPC];
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            const size_t c = base + (size_t)q * BLOCK;
+            if (c < nchunk) {
+                IO::load(x, c, u[q]);
+                if (!norm) IO::load(y, c, v[q]);
+            }
         }
-    } else {
-        for (size_t k = (size_t)blockIdx.x * BLOCK + threadIdx.x; k < n; k += step) {
-            vec2<R> u = x[k], v = y[k];
-            acc.x += (double)u.x * (double)v.x + (double)u.y * (double)v.y;
-            acc.y += (double)u.x * (double)v.y - (double)v.x * (double)u.y;
+#pragma unroll
+        for (int q = 0; q < U; ++q) {
+            if (base + (size_t)q * BLOCK >= nchunk) continue;
+#pragma unroll
+            for (int j = 0; j < CPC; ++j) {
+                if (norm) {
+                    acc.x += (double)u[q][j].re * (double)u[q][j].re + (double)u[q][j].im * (double)u[q][j].im;
+                } else {
+                    acc.x += (double)u[q][j].re * (double)v[q][j].re + (double)u[q][j].im * (double)v[q][j].im;
+                    acc.y += (double)u[q][j].re * (double)v[q][j].im - (double)v[q][j].re * (double)u[q][j].im;
+                }
+            }
         }
     }
     double2 s = block_sum2<BLOCK>(acc, lds);
@@ -133,16 +221,22 @@ __global__ __launch_bounds__(BLOCK) void k_reduce1(const vec2<R>* __restrict__ x
 hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t n, int prec, int mode,
                                 unsigned* nblocks)
 {
-    const unsigned nb = reduce_blocks(n);
-    *nblocks          = nb;
-    if (prec == CL2_F64)
-        hipLaunchKernelGGL((k_reduce1<double, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream,
-                           reinterpret_cast<const double2*>(x), reinterpret_cast<const double2*>(y), n, mode,
-                           c->partials);
-    else
-        hipLaunchKernelGGL((k_reduce1<float, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream,
-                           reinterpret_cast<const float2*>(x), reinterpret_cast<const float2*>(y), n, mode,
-                           c->partials);
+    if (prec == CL2_F64) {
+        const unsigned nb = chunk_blocks(n, BLAS_U_REDUCE);
+        *nblocks          = nb;
+        hipLaunchKernelGGL((k_reduce1<double, 1, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, x, y, n,
+                           mode, c->partials);
+    } else if (n % 2 == 0 && aligned16(x) && aligned16(y)) {
+        const unsigned nb = chunk_blocks(n / 2, BLAS_U_REDUCE);
+        *nblocks          = nb;
+        hipLaunchKernelGGL((k_reduce1<float, 2, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, x, y, n,
+                           mode, c->partials);
+    } else {
+        const unsigned nb = chunk_blocks(n, BLAS_U_REDUCE);
+        *nblocks          = nb;
+        hipLaunchKernelGGL((k_reduce1<float, 1, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, x, y, n,
+                           mode, c->partials);
+    }
     return hipGetLastError();
 }
 
@@ -181,7 +275,8 @@ __global__ __launch_bounds__(BLOCK) void k_reduce2(const double* __restrict__ pa
 hipError_t launch_reduce_stage2(cl2_ctx* c, unsigned nparts, double* res, int out_kind, const double* alpha_vec,
                                 int nvec)
 {
-    hipLaunchKernelGGL((k_reduce2<BLAS_BLOCK>), dim3(1), dim3(BLAS_BLOCK), 0, c->stream, c->partials, nparts, res,
+    // 1024 threads: the elementwise kernels leave up to BLAS_MAX_BLOCKS partials
+    hipLaunchKernelGGL((k_reduce2<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials, nparts, res,
                        out_kind, alpha_vec, nvec);
     return hipGetLastError();
 }
@@ -252,9 +347,16 @@ hipError_t launch_copy16(cl2_ctx* c, void* dst, const void* src, size_t n)
     return hipGetLastError();
 }
 
+/* one element per thread for the streaming kernels without partials (fill, conversion) */
+static unsigned stream_blocks(size_t n)
+{
+    const size_t nb = (n + BLAS_BLOCK - 1) / BLAS_BLOCK;
+    return nb == 0 ? 1u : (unsigned)std::min<size_t>(nb, (size_t)1 << 22);
+}
+
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im)
 {
-    const unsigned nb = reduce_blocks(n);
+    const unsigned nb = stream_blocks(n);
     if (prec == CL2_F64)
         hipLaunchKernelGGL((k_fill<double, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream,
                            reinterpret_cast<double2*>(out), n, re, im);
@@ -278,7 +380,7 @@ __global__ __launch_bounds__(BLOCK) void k_convert(vec2<RD>* __restrict__ out, c
 
 hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n)
 {
-    const unsigned nb = reduce_blocks(n);
+    const unsigned nb = stream_blocks(n);
     if (out_prec == CL2_F32 && in_prec == CL2_F64)
         hipLaunchKernelGGL((k_convert<float, double, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream,
                            reinterpret_cast<float2*>(out), reinterpret_cast<const double2*>(in), n);

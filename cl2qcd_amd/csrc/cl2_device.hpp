@@ -287,6 +287,11 @@ __device__ __forceinline__ double2 load_stream(const double2* p)
     nt_d2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_d2*>(p));
     return make_double2(v.x, v.y);
 }
+__device__ __forceinline__ float2 load_stream(const float2* p)
+{
+    nt_f2 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f2*>(p));
+    return make_float2(v.x, v.y);
+}
 __device__ __forceinline__ float4 load_stream(const float4* p)
 {
     nt_f4 v = __builtin_nontemporal_load(reinterpret_cast<const nt_f4*>(p));
@@ -301,6 +306,11 @@ __device__ __forceinline__ void store_stream(float2* p, float2 v)
 {
     nt_f2 w = {v.x, v.y};
     __builtin_nontemporal_store(w, reinterpret_cast<nt_f2*>(p));
+}
+__device__ __forceinline__ void store_stream(float4* p, float4 v)
+{
+    nt_f4 w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<nt_f4*>(p));
 }
 
 /* ---------------- gauge-link formats ----------------

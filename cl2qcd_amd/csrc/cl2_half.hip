@@ -38,13 +38,17 @@ __device__ __forceinline__ float* half_norms(void* p, uint32_t S)
     return reinterpret_cast<float*>(static_cast<char*>(p) + (size_t)48 * S);
 }
 
+/* STREAM: non-temporal loads (the BLAS kernels, which read each site once) */
+template<bool STREAM = false>
 __device__ __forceinline__ void load_site_h(const void* p, uint32_t S, uint32_t i, Cx<float> (&v)[12])
 {
     const h8* l   = reinterpret_cast<const h8*>(p);
-    const float n = half_norms(p, S)[i];
+    const float n = STREAM ? __builtin_nontemporal_load(half_norms(p, S) + i) : half_norms(p, S)[i];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const f8v f = __builtin_convertvector(l[(size_t)j * S + i], f8v);
+        const f8v f = __builtin_convertvector(STREAM ? __builtin_nontemporal_load(l + (size_t)j * S + i)
+                                                     : l[(size_t)j * S + i],
+                                              f8v);
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[4 * j + q] = Cx<float>{f[2 * q] * n, f[2 * q + 1] * n};
     }
@@ -255,7 +259,8 @@ __device__ __forceinline__ Cx<float> resolve_f(const cl2_scalar& s)
     }
 }
 
-/* the Wilson modes of k_blas, one thread per site; a.n is the site count */
+/* the Wilson modes of k_blas, one thread per site (a.n is the site count); the grid covers the field up to
+ * BLAS_MAX_BLOCKS blocks and loops beyond */
 template<int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_blas_h(BlasArgs a)
 {
@@ -265,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_h(BlasArgs a)
     double nrm = 0., ry = 0.;
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < S; i += gridDim.x * BLOCK) {
         Cx<float> x[12], r[12];
-        load_site_h(a.x, S, i, x);
+        load_site_h<true>(a.x, S, i, x);
         if (a.mode == B_AX) {
 #pragma unroll
             for (int k = 0; k < 12; ++k) r[k] = cmul(x[k], al);
@@ -274,12 +279,22 @@ __global__ __launch_bounds__(BLOCK) void k_blas_h(BlasArgs a)
             for (int k = 0; k < 12; ++k) r[k] = k < 6 ? x[k] : Cx<float>{-x[k].re, -x[k].im};
         } else {
             Cx<float> y[12];
-            load_site_h(a.y, S, i, y);
-            if (a.mode == B_AX_BY_Z) {
+            load_site_h<true>(a.y, S, i, y);
+            if (a.mode == B_AX_BY_Z || a.mode == B_CG_XP) {
                 Cx<float> z[12];
-                load_site_h(a.z, S, i, z);
+                load_site_h<true>(a.z, S, i, z);
+                if (a.mode == B_CG_XP) {
+                    Cx<float> r2[12];
 #pragma unroll
-                for (int k = 0; k < 12; ++k) r[k] = cmul(y[k], be) + cmul(x[k], al) + z[k];
+                    for (int k = 0; k < 12; ++k) {
+                        r[k]  = z[k] - cmul(x[k], al);
+                        r2[k] = y[k] - cmul(x[k], be);
+                    }
+                    store_site_h<true>(a.out2, S, i, r2);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 12; ++k) r[k] = cmul(y[k], be) + cmul(x[k], al) + z[k];
+                }
             } else {  // B_Y_MINUS_AX, B_G5_Y_MINUS_AX
                 const bool g5 = a.mode == B_G5_Y_MINUS_AX;
 #pragma unroll
@@ -290,7 +305,7 @@ __global__ __launch_bounds__(BLOCK) void k_blas_h(BlasArgs a)
                 }
             }
         }
-        store_site_h(a.out, S, i, r);
+        store_site_h<true>(a.out, S, i, r);
 #pragma unroll
         for (int k = 0; k < 12; ++k) nrm += (double)r[k].re * r[k].re + (double)r[k].im * r[k].im;
     }
@@ -302,13 +317,12 @@ __global__ __launch_bounds__(BLOCK) void k_blas_h(BlasArgs a)
 
 static unsigned site_blocks(uint32_t S)
 {
-    unsigned nb = blocks_for(S, BLAS_BLOCK);
-    return nb > (unsigned)REDUCE_MAX_BLOCKS ? (unsigned)REDUCE_MAX_BLOCKS : (nb == 0 ? 1u : nb);
+    const unsigned nb = blocks_for(S, BLAS_BLOCK);
+    return nb > BLAS_MAX_BLOCKS ? BLAS_MAX_BLOCKS : (nb == 0 ? 1u : nb);
 }
 
 hipError_t launch_blas_h(cl2_ctx* c, const BlasArgs& a, unsigned* nblocks_out)
 {
-    if (a.mode == B_AX_PLUS_Y || a.mode == B_AX_BY) return hipErrorInvalidValue;  // staggered modes
     const unsigned nb = site_blocks((uint32_t)a.n);
     if (nblocks_out) *nblocks_out = nb;
     hipLaunchKernelGGL((k_blas_h<BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, a);
@@ -323,13 +337,13 @@ __global__ __launch_bounds__(BLOCK) void k_reduce1_h(const void* x, const void* 
     double2 acc = make_double2(0., 0.);
     for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < S; i += gridDim.x * BLOCK) {
         Cx<float> u[12];
-        load_site_h(x, S, i, u);
+        load_site_h<true>(x, S, i, u);
         if (mode == R_NORM) {
 #pragma unroll
             for (int k = 0; k < 12; ++k) acc.x += (double)u[k].re * u[k].re + (double)u[k].im * u[k].im;
         } else {
             Cx<float> v[12];
-            load_site_h(y, S, i, v);
+            load_site_h<true>(y, S, i, v);
 #pragma unroll
             for (int k = 0; k < 12; ++k) {
                 acc.x += (double)u[k].re * v[k].re + (double)u[k].im * v[k].im;

@@ -50,6 +50,9 @@ namespace cl2 {
 constexpr int DSLASH_BLOCK = 256;
 constexpr int BLAS_BLOCK   = 256;
 constexpr int REDUCE_MAX_BLOCKS = 2048;
+/* most blocks (and partials) of one elementwise BLAS or reduction launch; each thread streams a few 16-byte
+ * chunks, and larger fields loop over the grid */
+constexpr unsigned BLAS_MAX_BLOCKS = 65536;
 
 inline unsigned blocks_for(size_t n, int block)
 {
@@ -86,11 +89,13 @@ enum BlasMode : int {
     B_AX_BY      = 4,  // a x + b y        (staggered saxpby)
     B_G5_Y_MINUS_AX = 5,  // gamma5 (y - a x) on a Wilson spinor
     B_NEG_TAIL   = 6,  // gamma5 in place: negate complex elements [tail, n)
+    B_CG_XP      = 7,  // out = z - a x and out2 = y - b x, both from the same x (the x and p updates of CG)
 };
 struct BlasArgs {
     size_t n;     // complex elements
     size_t tail;  // first element with a negative gamma5 sign (Wilson: 6 * Veo)
     void* out;
+    void* out2;  // B_CG_XP: second output
     const void* x;
     const void* y;
     const void* z;

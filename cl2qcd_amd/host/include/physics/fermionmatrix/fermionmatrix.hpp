@@ -81,6 +81,11 @@ class BuiltinEoOperator : public Fermionmatrix_eo {
     cl_ulong get_flops() const override;
     cl_ulong get_read_write_size() const override;
     int abi_operator() const noexcept override { return op; }
+    /* QplusQminus_eo only: out = A in and dot = <in, out> = |Qminus_eo in|^2 (a complex device scalar, imaginary
+     * part 0) from the kernel that writes Qminus_eo in; throws for the other operators */
+    void apply_and_dot(const lattices::Spinorfield_eo* out, const lattices::Gaugefield& gf,
+                       const lattices::Spinorfield_eo& in, const physics::AdditionalParameters& additionalParameters,
+                       const lattices::Scalar<hmc_complex>& dot) const;
 
   protected:
     BuiltinEoOperator(int op, bool herm, const hardware::System& system, const FermionEoParametersInterface& p);

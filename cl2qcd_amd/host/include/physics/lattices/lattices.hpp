@@ -203,6 +203,10 @@ void saxpy_AND_squarenorm(const Spinorfield_eo* out, const Scalar<hmc_complex>& 
  * reduced-precision CG */
 void saxpy_AND_cgnorm(const Spinorfield_eo* out, const Scalar<hmc_complex>& alpha, const Spinorfield_eo& x,
                       const Spinorfield_eo& y, const Vector<hmc_float>& res);
+/* the x and p updates of a CG step in one pass (Wilson saxpy convention): x <- x - malpha p, then
+ * p <- r - mbeta p, both from the same p (x, p, r distinct) */
+void cg_update_xp(const Spinorfield_eo* x, const Spinorfield_eo* p, const Spinorfield_eo& r,
+                  const Scalar<hmc_complex>& malpha, const Scalar<hmc_complex>& mbeta);
 void saxpy_AND_gamma5_eo(const Spinorfield_eo* out, const hmc_complex alpha, const Spinorfield_eo& x,
                          const Spinorfield_eo& y);
 void convert_to_eoprec(const Spinorfield_eo* even, const Spinorfield_eo* odd, const Spinorfield& in);
@@ -211,6 +215,9 @@ void log_squarenorm(const std::string& msg, const Spinorfield_eo& x);
 /* copy between precisions (or a plain copy when they agree) */
 void convert_precision(const Spinorfield_eo* to, const Spinorfield_eo& from);
 void convert_precision(const Staggeredfield_eo* to, const Staggeredfield_eo& from);
+/* staggered fields, same convention and updates as the Wilson overload */
+void cg_update_xp(const Staggeredfield_eo* x, const Staggeredfield_eo* p, const Staggeredfield_eo& r,
+                  const Scalar<hmc_float>& malpha, const Scalar<hmc_float>& mbeta);
 
 /* ---- staggered eo BLAS (physics/lattices/staggeredfield_eo.hpp:120-258) ---- */
 void copyData(const Staggeredfield_eo* to, const Staggeredfield_eo& from);

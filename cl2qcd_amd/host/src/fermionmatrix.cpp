@@ -116,23 +116,40 @@ BuiltinEoOperator::BuiltinEoOperator(int op, bool herm, const hardware::System& 
 {
 }
 
+namespace {
+int abi_action_of(const FermionmatrixParametersInterface& p)
+{
+    const common::action a = p.getFermionicActionType();
+    if (a == common::wilson) return CL2_ACTION_WILSON;
+    if (a == common::twistedmass) return CL2_ACTION_TWISTEDMASS;
+    throw std::invalid_argument("Unkown fermion action! wilson or twistedmass");
+}
+}  // namespace
+
 void BuiltinEoOperator::operator()(const Spinorfield_eo* out, const Gaugefield& gf, const Spinorfield_eo& in,
                                    const physics::AdditionalParameters& ap) const
 {
-    const common::action a = fermionmatrixParametersInterface.getFermionicActionType();
-    int abi_action;
-    if (a == common::wilson)
-        abi_action = CL2_ACTION_WILSON;
-    else if (a == common::twistedmass)
-        abi_action = CL2_ACTION_TWISTEDMASS;
-    else
-        throw std::invalid_argument("Unkown fermion action! wilson or twistedmass");
+    const int abi_action = abi_action_of(fermionmatrixParametersInterface);
     if (out->precision() != in.precision()) throw std::invalid_argument("fields of different precision");
     const hmc_float kappa = ap.getKappa();
     const hmc_float mubar = abi_action == CL2_ACTION_TWISTEDMASS ? ap.getMubar() : 0.;
     const Links l         = links_for(gf, in.precision());
     get_system().check(cl2_fermionmatrix_eo(get_system().ctx(), op, out->dev(), in.dev(), l.ptr, l.fmt, abi_action,
                                             kappa, mubar, tmp.ptr()));
+}
+
+void BuiltinEoOperator::apply_and_dot(const Spinorfield_eo* out, const Gaugefield& gf, const Spinorfield_eo& in,
+                                      const physics::AdditionalParameters& ap,
+                                      const lattices::Scalar<hmc_complex>& dot) const
+{
+    if (op != CL2_OP_QPLUSQMINUS_EO) throw std::invalid_argument("apply_and_dot: QplusQminus_eo only");
+    const int abi_action = abi_action_of(fermionmatrixParametersInterface);
+    if (out->precision() != in.precision()) throw std::invalid_argument("fields of different precision");
+    const hmc_float kappa = ap.getKappa();
+    const hmc_float mubar = abi_action == CL2_ACTION_TWISTEDMASS ? ap.getMubar() : 0.;
+    const Links l         = links_for(gf, in.precision());
+    get_system().check(cl2_fermionmatrix_eo_dot(get_system().ctx(), op, out->dev(), in.dev(), l.ptr, l.fmt,
+                                                abi_action, kappa, mubar, tmp.ptr(), dot.dev()));
 }
 
 cl_ulong BuiltinEoOperator::get_flops() const

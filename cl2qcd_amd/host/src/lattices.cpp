@@ -461,6 +461,24 @@ void saxpy_AND_cgnorm(const Spinorfield_eo* out, const Scalar<hmc_complex>& alph
     x.get_system().check(
         cl2_saxpy_AND_cgnorm_eo(ctx(x.get_system()), out->dev(), x.dev(), y.dev(), x.precision(), &a, res.dev()));
 }
+void cg_update_xp(const Spinorfield_eo* x, const Spinorfield_eo* p, const Spinorfield_eo& r,
+                  const Scalar<hmc_complex>& malpha, const Scalar<hmc_complex>& mbeta)
+{
+    if (x->precision() != p->precision() || p->precision() != r.precision())
+        throw std::invalid_argument("cg_update_xp: fields of different precision");
+    cl2_scalar a = dev(malpha), b = dev(mbeta);
+    r.get_system().check(cl2_cg_update_xp(ctx(r.get_system()), CL2_FIELD_SPINOR_EO, x->dev(), p->dev(), r.dev(),
+                                          r.precision(), &a, &b));
+}
+void cg_update_xp(const Staggeredfield_eo* x, const Staggeredfield_eo* p, const Staggeredfield_eo& r,
+                  const Scalar<hmc_float>& malpha, const Scalar<hmc_float>& mbeta)
+{
+    if (x->precision() != p->precision() || p->precision() != r.precision())
+        throw std::invalid_argument("cg_update_xp: fields of different precision");
+    cl2_scalar a = dev(malpha), b = dev(mbeta);
+    r.get_system().check(cl2_cg_update_xp(ctx(r.get_system()), CL2_FIELD_SU3VEC_EO, x->dev(), p->dev(), r.dev(),
+                                          r.precision(), &a, &b));
+}
 void saxpy_AND_gamma5_eo(const Spinorfield_eo* out, const hmc_complex alpha, const Spinorfield_eo& x,
                          const Spinorfield_eo& y)
 {

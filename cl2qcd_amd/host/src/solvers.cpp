@@ -362,17 +362,24 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
     x_lo.zero();
     r2c.store(hmc_complex{r2, 0.});  // |r|^2 of the current step as a complex device scalar
     A(&v, gf, p, ap);                // makes the reduced-precision link copy before anything is recorded
-    // one inner iteration, every scalar on the device: alpha = r2 / <p, A p>, x_lo += alpha p,
-    // r_lo -= alpha v with its norms, beta (Polak-Ribiere) and r2 <- |r_lo|^2, p = r_lo + beta p
+    // QplusQminus_eo = Qminus^dagger Qminus: <p, A p> = |Qminus p|^2 comes out of the operator's own kernels
+    const auto* builtin  = dynamic_cast<const fermionmatrix::BuiltinEoOperator*>(&A);
+    const bool fused_dot = builtin != nullptr && builtin->abi_operator() == CL2_OP_QPLUSQMINUS_EO;
+    // one inner iteration, every scalar on the device: v = A p with <p, v>, alpha = r2 / <p, v>,
+    // r_lo -= alpha v with its norms, beta (Polak-Ribiere) and r2 <- |r_lo|^2, then x_lo += alpha p and
+    // p = r_lo + beta p in one pass
     auto body = [&]() {
-        A(&v, gf, p, ap);
-        scalar_product(&pAp, p, v);
+        if (fused_dot) {
+            builtin->apply_and_dot(&v, gf, p, ap, pAp);
+        } else {
+            A(&v, gf, p, ap);
+            scalar_product(&pAp, p, v);
+        }
         divide(&alpha, r2c, pAp);
         multiply(&malpha, minus_one, alpha);
-        saxpy(&x_lo, malpha, p, x_lo);
         saxpy_AND_cgnorm(&r_lo, alpha, v, r_lo, norms);
         system.check(cl2_cg_beta(c, mbeta.dev(), r2c.dev(), norms.dev()));
-        saxpy(&p, mbeta, p, r_lo);
+        cg_update_xp(&x_lo, &p, r_lo, malpha, mbeta);
     };
     struct Graph {
         cl2_graph* g = nullptr;
@@ -659,9 +666,8 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
     const Staggeredfield_eo r(system, CL2_F64), t64(system, CL2_F64);
     const Staggeredfield_eo r_lo(system, lo), p(system, lo), v(system, lo), x_lo(system, lo);
     const Scalar<hmc_float> alpha(system), malpha(system), pAp(system), r2(system), rn_s(system), beta(system),
-        zero(system), one_s(system);
+        mbeta(system), zero(system);
     zero.store(0.);
-    one_s.store(1.);
     cl2_ctx* c  = system.ctx();
     auto apply = [&](const Staggeredfield_eo* out, const Staggeredfield_eo& in) {  // out = (A + sigma) in
         A(out, gf, in, &ap);
@@ -692,12 +698,12 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
         scalar_product_real_part(&pAp, p, v);
         divide(&alpha, r2, pAp);
         subtract(&malpha, zero, alpha);
-        saxpy(&x_lo, alpha, p, x_lo);   // x_lo += alpha p
         saxpy(&r_lo, malpha, v, r_lo);  // r_lo -= alpha v
         squarenorm(&rn_s, r_lo);
         divide(&beta, rn_s, r2);
-        add(&r2, rn_s, zero);           // r2 = |r_new|^2
-        saxpby(&p, one_s, r_lo, beta, p);  // p = r_lo + beta p
+        subtract(&mbeta, zero, beta);
+        add(&r2, rn_s, zero);                       // r2 = |r_new|^2
+        cg_update_xp(&x_lo, &p, r_lo, malpha, mbeta);  // x_lo += alpha p, p = r_lo + beta p
     };
     struct Graph {
         cl2_graph* g = nullptr;

@@ -187,6 +187,12 @@ int cl2_saxpy_AND_gamma5_eo(cl2_ctx* ctx, void* out, const void* x, const void* 
  * tmp: caller-provided scratch of 2 spinor fields of the operator's precision. */
 int cl2_fermionmatrix_eo(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge, int format, int action,
                          double kappa, double mubar, void* tmp);
+/* QplusQminus_eo (op must be CL2_OP_QPLUSQMINUS_EO) as cl2_fermionmatrix_eo, and dot (2 device doubles) =
+ * <in, out> = |Qminus_eo in|^2 (Qplus_eo = Qminus_eo^dagger), summed in FP64 by the kernel that writes
+ * Qminus_eo in.  Replaces the operator call and scalar_product(p, v) of cg (solvers/cg.cpp:327-330), one
+ * reading pass over in and out fewer. */
+int cl2_fermionmatrix_eo_dot(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge, int format,
+                             int action, double kappa, double mubar, void* tmp, double* dot);
 
 /* ==== Wilson eo BLAS (hardware/code/spinors.hpp:50-224) ==== */
 
@@ -210,6 +216,11 @@ int cl2_saxpy_AND_squarenorm_eo(cl2_ctx* ctx, void* out, const void* x, const vo
  * |r|^2 as a complex device scalar for the next step's alpha. */
 int cl2_saxpy_AND_cgnorm_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, const cl2_scalar* alpha,
                             double* res);
+/* The two search-space updates of a CG step in one pass (solvers/cg.cpp:334 and :149, Wilson saxpy
+ * convention y - a x): x <- x - alpha p, then p <- r - beta p, both from the same p.  kind:
+ * CL2_FIELD_SPINOR_EO or CL2_FIELD_SU3VEC_EO; x, p, r distinct fields of precision prec. */
+int cl2_cg_update_xp(cl2_ctx* ctx, int kind, void* x, void* p, const void* r, int prec, const cl2_scalar* alpha,
+                     const cl2_scalar* beta);
 /* set_zero_spinorfield_eoprec / set_eoprec_spinorfield_cold (spinors.cpp:448-460,805-817):
  * cold = every complex component 1/sqrt(12 V) */
 /* Spinorfield_eo::gaussian / Staggeredfield_eo::set_gaussian (physics/lattices/spinorfield_eo.hpp:93,

@@ -151,6 +151,27 @@ def test_dslash_gamma5_hermiticity():
     assert abs(lhs - rhs) < 1e-12 * abs(lhs)
 
 
+def test_qplus_is_qminus_adjoint():
+    # <phi, Qplus psi> = <Qminus phi, psi> (twisted mass and Wilson), so <p, QplusQminus p> = |Qminus p|^2:
+    # the identity behind the fused <p, A p> of the reliable-update CG
+    lat = O.lattice(4, 6, theta_s=0.3, theta_t=1.0)
+    g = random_su3_field(4 * O.volume(lat), 9)
+    rng = np.random.default_rng(3)
+    phi = rng.standard_normal((O.veo(lat), 24))
+    psi = rng.standard_normal((O.veo(lat), 24))
+    for action, mubar in ((O.TWISTEDMASS, 2 * 0.16 * 0.02), (O.WILSON, 0.0)):
+        lhs = O.dot(phi, O.apply_eo(lat, O.OP_QPLUS_EO, psi, g, action, 0.16, mubar))
+        rhs = O.dot(O.apply_eo(lat, O.OP_QMINUS_EO, phi, g, action, 0.16, mubar), psi)
+        assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+        pAp = O.dot(psi, O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, psi, g, action, 0.16, mubar))
+        qm = O.norm2(O.apply_eo(lat, O.OP_QMINUS_EO, psi, g, action, 0.16, mubar))
+        assert abs(pAp - qm) < 1e-12 * qm
+    # a wrong sign of mu in one factor breaks it
+    bad = O.dot(phi, O.apply_eo(lat, O.OP_QPLUS_EO, psi, g, O.TWISTEDMASS, 0.16, 0.0064))
+    good = O.dot(O.apply_eo(lat, O.OP_QPLUS_EO, phi, g, O.TWISTEDMASS, 0.16, 0.0064), psi)
+    assert abs(bad - good) > 1e-6 * abs(bad)
+
+
 def _gauge_transform(lat, g, gmat):
     """U_mu(x) -> G(x) U_mu(x) G(x+mu)^dagger on the AoS layout."""
     ns, nt = lat.ns, lat.nt
