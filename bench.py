@@ -112,6 +112,11 @@ def cg_leg(ns, nt, sweeps, overrelax, seed):
     b2 = H.squarenorm(b)
     qq, ap = H.QplusQminus_eo(l), l.wilson_parameters()
     r = H.Spinorfield_eo(l)
+    # launch shapes of the hopping kernel for the three link formats (FP64 operator of the plain CG and of the
+    # reliable updates, FP32 and FP16 inner loops), autotuned on this configuration before anything is timed
+    tuned_ms = {name: round(H.autotune_dslash(gf, l, prec, 1), 4)
+                for name, prec in (("f64", H.F64), ("f32", H.F32), ("f16", H.F16))}
+    log(f"autotuned hopping kernel (ms per application): {tuned_ms}")
 
     def true_rel_residual(x):
         qq(r, gf, x, ap)
@@ -121,7 +126,7 @@ def cg_leg(ns, nt, sweeps, overrelax, seed):
     out = {"lattice": f"{ns}^3x{nt}", "heatbath": f"beta 5.7, cold start, {sweeps} sweeps x (heatbath + "
                                                   f"{overrelax} overrelaxation), seed {seed}",
            "plaquette": round(plaq, 6), "operator": "QplusQminus_eo twisted mass kappa 0.16 mu 0.005",
-           "target_rel_residual": 1e-10}
+           "target_rel_residual": 1e-10, "autotuned_dslash_ms": tuned_ms}
     x64 = H.Spinorfield_eo(l)
     x64.zero()
     l.synchronize()

@@ -372,9 +372,9 @@ int cl2_ctx_create(cl2_ctx** out, int device, const cl2_lattice* lat)
     c->cp_fwd[1] = er * ci;
     c->cp_bwd[0] = mer * cr;
     c->cp_bwd[1] = mer * ci;
-    // one double2 per block: REDUCE_MAX_BLOCKS blocks for each of up to 16 right-hand sides, BLAS_MAX_BLOCKS
+    // one double2 per block: 16384 blocks for each of up to 16 right-hand sides (block kernels), BLAS_MAX_BLOCKS
     // for an elementwise kernel, or one per block of the smallest hopping-kernel block (128 threads)
-    c->npartials = std::max<size_t>({(size_t)REDUCE_MAX_BLOCKS * 16, (size_t)BLAS_MAX_BLOCKS, blocks_for(c->g.veo, 128)});
+    c->npartials = std::max<size_t>({(size_t)16384 * 16, (size_t)BLAS_MAX_BLOCKS, blocks_for(c->g.veo, 128)});
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, c->npartials * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
@@ -999,6 +999,17 @@ int cl2_block_axpy_eo(cl2_ctx* c, void* out, const void* x, const void* y, int p
     CL2_BLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, out && x && y && coef, "null pointer");
     CL2_HIP(c, launch_block_axpy(c, out, x, y, prec, nrhs, coef, sign, norms));
+    return CL2_OK;
+}
+
+int cl2_block_cg_update_xp_eo(cl2_ctx* c, void* x, void* p, const void* r, int prec, int nrhs, const double* alpha,
+                              const double* beta)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, x && p && r && alpha && beta, "null pointer");
+    CL2_REQUIRE(c, x != p && x != r && p != r, "x, p and r must be different blocks");
+    CL2_HIP(c, launch_block_xp(c, x, p, r, prec, nrhs, alpha, beta));
     return CL2_OK;
 }
 

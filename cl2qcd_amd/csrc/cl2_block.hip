@@ -7,30 +7,83 @@
 // one right-hand side (block sizes and grid strides are multiples of nrhs), a
 // block sums each right-hand side's column of its threads in a fixed order, and
 // stage 2 reduces each right-hand side's block partials with one block each.
+#include <initializer_list>
+
 #include "cl2_device.hpp"
 #include "cl2_internal.hpp"
 
 namespace cl2 {
 
+/* Elements are streamed in 16-byte chunks: one FP64 complex number, or two FP32 numbers (two adjacent
+ * right-hand sides) when nrhs is even or 1 and the operands are 16-byte aligned.  A chunk column is the
+ * chunk index modulo ncol = max(1, nrhs / CPC); a thread keeps one column, so slot j of its chunks always
+ * belongs to right-hand side (q CPC + j) mod nrhs. */
+template<typename R, int CPC>
+struct BChunk;
+template<>
+struct BChunk<double, 1> {
+    __device__ static void load(const void* p, size_t c, Cx<double> (&v)[1])
+    {
+        const double2 w = load_stream(reinterpret_cast<const double2*>(p) + c);
+        v[0]            = Cx<double>{w.x, w.y};
+    }
+    __device__ static void store(void* p, size_t c, const Cx<double> (&v)[1])
+    {
+        store_stream(reinterpret_cast<double2*>(p) + c, make_double2(v[0].re, v[0].im));
+    }
+};
+template<>
+struct BChunk<float, 2> {
+    __device__ static void load(const void* p, size_t c, Cx<float> (&v)[2])
+    {
+        const float4 w = load_stream(reinterpret_cast<const float4*>(p) + c);
+        v[0]           = Cx<float>{w.x, w.y};
+        v[1]           = Cx<float>{w.z, w.w};
+    }
+    __device__ static void store(void* p, size_t c, const Cx<float> (&v)[2])
+    {
+        store_stream(reinterpret_cast<float4*>(p) + c, make_float4(v[0].re, v[0].im, v[1].re, v[1].im));
+    }
+};
+template<>
+struct BChunk<float, 1> {
+    __device__ static void load(const void* p, size_t c, Cx<float> (&v)[1])
+    {
+        const float2 w = load_stream(reinterpret_cast<const float2*>(p) + c);
+        v[0]           = Cx<float>{w.x, w.y};
+    }
+    __device__ static void store(void* p, size_t c, const Cx<float> (&v)[1])
+    {
+        store_stream(reinterpret_cast<float2*>(p) + c, make_float2(v[0].re, v[0].im));
+    }
+};
+
 namespace {
 
-/* threads per block: the largest multiple of nrhs not above 256 */
-inline unsigned block_threads(int nrhs)
+/* per-right-hand-side column sums of a block's thread values (CPC slots per thread), valid in threads
+ * 0..nrhs-1 */
+template<int CPC>
+__device__ __forceinline__ double2 column_sum(const double2 (&v)[CPC], double2* lds, int nrhs, int ncol)
 {
-    return 256u - (256u % (unsigned)nrhs);
-}
-
-/* per-right-hand-side column sums of a block's thread values, valid in threads 0..nrhs-1 */
-__device__ __forceinline__ double2 column_sum(double2 v, double2* lds, int nrhs)
-{
-    lds[threadIdx.x] = v;
+#pragma unroll
+    for (int j = 0; j < CPC; ++j) lds[threadIdx.x * CPC + j] = v[j];
     __syncthreads();
-    double2 s = make_double2(0., 0.);
-    if ((int)threadIdx.x < nrhs)
-        for (unsigned k = threadIdx.x; k < blockDim.x; k += nrhs) {
-            s.x += lds[k].x;
-            s.y += lds[k].y;
+    double2 s   = make_double2(0., 0.);
+    const int t = (int)threadIdx.x;
+    if (t < nrhs) {
+        if (nrhs % CPC == 0) {
+            const int j = t % CPC;
+            for (unsigned k = (unsigned)(t / CPC); k < blockDim.x; k += (unsigned)ncol) {
+                s.x += lds[k * CPC + j].x;
+                s.y += lds[k * CPC + j].y;
+            }
+        } else {  // nrhs == 1 < CPC: every slot is right-hand side 0
+            for (unsigned k = 0; k < blockDim.x * CPC; ++k) {
+                s.x += lds[k].x;
+                s.y += lds[k].y;
+            }
         }
+    }
     __syncthreads();
     return s;
 }
@@ -38,63 +91,139 @@ __device__ __forceinline__ double2 column_sum(double2 v, double2* lds, int nrhs)
 }  // namespace
 
 /* out = y + sign * c_r x  (c: nrhs complex on the device); with norms: per right-hand side
- * (sum |out|^2, sum Re conj(out) y) into partials[r * nb + block] */
-template<typename R>
-__global__ __launch_bounds__(256) void k_block_axpy(vec2<R>* out, const vec2<R>* x, const vec2<R>* y, size_t n,
-                                                    int nrhs, const double* coef, double sign, double2* partials)
+ * (sum |out|^2, sum Re conj(out) y) into partials[r * nb + block].  U chunks per thread per step, their
+ * loads issued first. */
+template<typename R, int CPC, int U>
+__global__ __launch_bounds__(256) void k_block_axpy(void* out, const void* x, const void* y, size_t nchunk, int nrhs,
+                                                    int ncol, const double* coef, double sign, double2* partials)
 {
-    __shared__ double2 lds[256];
-    const int r   = (int)(threadIdx.x % (unsigned)nrhs);
-    const Cx<R> c = Cx<R>{R(sign * coef[2 * r]), R(sign * coef[2 * r + 1])};
-    double2 acc   = make_double2(0., 0.);
-    const size_t step = (size_t)gridDim.x * blockDim.x;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
-        const vec2<R> xv = x[e], yv = y[e];
-        const Cx<R> o    = Cx<R>{yv.x, yv.y} + cmul(c, Cx<R>{xv.x, xv.y});
-        out[e]           = vec2<R>{o.re, o.im};
-        acc.x += (double)o.re * (double)o.re + (double)o.im * (double)o.im;
-        acc.y += (double)o.re * (double)yv.x + (double)o.im * (double)yv.y;
+    using IO = BChunk<R, CPC>;
+    __shared__ double2 lds[256 * CPC];
+    const int q = (int)(threadIdx.x % (unsigned)ncol);
+    Cx<R> c[CPC];
+    double2 acc[CPC];
+#pragma unroll
+    for (int j = 0; j < CPC; ++j) {
+        const int r = (q * CPC + j) % nrhs;
+        c[j]        = Cx<R>{R(sign * coef[2 * r]), R(sign * coef[2 * r + 1])};
+        acc[j]      = make_double2(0., 0.);
+    }
+    const size_t bt   = blockDim.x;
+    const size_t step = (size_t)gridDim.x * bt * U;
+    for (size_t base = (size_t)blockIdx.x * bt * U + threadIdx.x; base < nchunk; base += step) {
+        Cx<R> xv[U][CPC], yv[U][CPC];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t e = base + u * bt;
+            if (e < nchunk) {
+                IO::load(x, e, xv[u]);
+                IO::load(y, e, yv[u]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t e = base + u * bt;
+            if (e >= nchunk) continue;
+            Cx<R> o[CPC];
+#pragma unroll
+            for (int j = 0; j < CPC; ++j) {
+                o[j] = yv[u][j] + cmul(c[j], xv[u][j]);
+                acc[j].x += (double)o[j].re * (double)o[j].re + (double)o[j].im * (double)o[j].im;
+                acc[j].y += (double)o[j].re * (double)yv[u][j].re + (double)o[j].im * (double)yv[u][j].im;
+            }
+            IO::store(out, e, o);
+        }
     }
     if (partials) {
-        const double2 s = column_sum(acc, lds, nrhs);
+        const double2 s = column_sum<CPC>(acc, lds, nrhs, ncol);
         if ((int)threadIdx.x < nrhs) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
     }
 }
 
-/* per right-hand side: sum conj(x) y (mode 0) or sum |x|^2 (mode 1) */
-template<typename R>
-__global__ __launch_bounds__(256) void k_block_dot(const vec2<R>* x, const vec2<R>* y, size_t n, int nrhs, int mode,
-                                                   double2* partials)
+/* the x and p updates of a lockstep CG step in one pass: x = x + alpha_r p, then p = r + beta_r p */
+template<typename R, int CPC>
+__global__ __launch_bounds__(256) void k_block_xp(void* xo, void* p, const void* rv, size_t nchunk, int nrhs, int ncol,
+                                                  const double* alpha, const double* beta)
 {
-    __shared__ double2 lds[256];
-    double2 acc       = make_double2(0., 0.);
+    using IO = BChunk<R, CPC>;
+    const int q = (int)(threadIdx.x % (unsigned)ncol);
+    Cx<R> ca[CPC], cb[CPC];
+#pragma unroll
+    for (int j = 0; j < CPC; ++j) {
+        const int r = (q * CPC + j) % nrhs;
+        ca[j]       = Cx<R>{R(alpha[2 * r]), R(alpha[2 * r + 1])};
+        cb[j]       = Cx<R>{R(beta[2 * r]), R(beta[2 * r + 1])};
+    }
     const size_t step = (size_t)gridDim.x * blockDim.x;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
-        const vec2<R> u = x[e];
-        if (mode == 1) {
-            acc.x += (double)u.x * (double)u.x + (double)u.y * (double)u.y;
-        } else {
-            const vec2<R> v = y[e];
-            acc.x += (double)u.x * (double)v.x + (double)u.y * (double)v.y;
-            acc.y += (double)u.x * (double)v.y - (double)v.x * (double)u.y;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < nchunk; e += step) {
+        Cx<R> xv[CPC], pv[CPC], rr[CPC], xn[CPC], pn[CPC];
+        IO::load(xo, e, xv);
+        IO::load(p, e, pv);
+        IO::load(rv, e, rr);
+#pragma unroll
+        for (int j = 0; j < CPC; ++j) {
+            xn[j] = xv[j] + cmul(ca[j], pv[j]);
+            pn[j] = rr[j] + cmul(cb[j], pv[j]);
+        }
+        IO::store(xo, e, xn);
+        IO::store(p, e, pn);
+    }
+}
+
+/* per right-hand side: sum conj(x) y (mode 0) or sum |x|^2 (mode 1) */
+template<typename R, int CPC, int U>
+__global__ __launch_bounds__(256) void k_block_dot(const void* x, const void* y, size_t nchunk, int nrhs, int ncol,
+                                                   int mode, double2* partials)
+{
+    using IO = BChunk<R, CPC>;
+    __shared__ double2 lds[256 * CPC];
+    double2 acc[CPC];
+#pragma unroll
+    for (int j = 0; j < CPC; ++j) acc[j] = make_double2(0., 0.);
+    const size_t bt   = blockDim.x;
+    const size_t step = (size_t)gridDim.x * bt * U;
+    for (size_t base = (size_t)blockIdx.x * bt * U + threadIdx.x; base < nchunk; base += step) {
+        Cx<R> u[U][CPC], v[U][CPC];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const size_t e = base + k * bt;
+            if (e < nchunk) {
+                IO::load(x, e, u[k]);
+                if (mode != 1) IO::load(y, e, v[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            if (base + k * bt >= nchunk) continue;
+#pragma unroll
+            for (int j = 0; j < CPC; ++j) {
+                if (mode == 1) {
+                    acc[j].x += (double)u[k][j].re * (double)u[k][j].re + (double)u[k][j].im * (double)u[k][j].im;
+                } else {
+                    acc[j].x += (double)u[k][j].re * (double)v[k][j].re + (double)u[k][j].im * (double)v[k][j].im;
+                    acc[j].y += (double)u[k][j].re * (double)v[k][j].im - (double)v[k][j].re * (double)u[k][j].im;
+                }
+            }
         }
     }
-    const double2 s = column_sum(acc, lds, nrhs);
+    const double2 s = column_sum<CPC>(acc, lds, nrhs, ncol);
     if ((int)threadIdx.x < nrhs) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
 }
 
 /* stage 2: block r sums right-hand side r's nb partials; out_kind 0: res[2r..2r+1] = complex sum,
  * 1: res[r] = real part */
-__global__ __launch_bounds__(256) void k_block_reduce2(const double2* partials, unsigned nb, double* res, int out_kind)
+template<int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_block_reduce2(const double2* partials, unsigned nb, double* res,
+                                                         int out_kind)
 {
-    __shared__ double lds[8];
+    __shared__ double lds[2 * BLOCK / 64];
     const double2* p = partials + (size_t)blockIdx.x * nb;
     double2 acc      = make_double2(0., 0.);
-    for (unsigned k = threadIdx.x; k < nb; k += 256) {
+    for (unsigned k = threadIdx.x; k < nb; k += BLOCK) {
         acc.x += p[k].x;
         acc.y += p[k].y;
     }
-    const double2 s = block_sum2<256>(acc, lds);
+    const double2 s = block_sum2<BLOCK>(acc, lds);
     if (threadIdx.x == 0) {
         if (out_kind == 0) {
             res[2 * blockIdx.x]     = s.x;
@@ -183,50 +312,113 @@ hipError_t launch_scalar_copy(cl2_ctx* c, double* dst, const double* src, int n)
     return hipGetLastError();
 }
 
-static unsigned grid_for(cl2_ctx* c, size_t n, int nrhs, bool partials)
+/* launch shape of a block kernel over n complex elements: CPC, chunk columns, threads per block (the largest
+ * multiple of the column count not above 256) and blocks (U chunks per thread; with per-right-hand-side
+ * partials at most what the partials buffer holds, beyond that the grid loops) */
+struct BlockShape {
+    int cpc, ncol;
+    unsigned bt, nb;
+    size_t nchunk;
+};
+
+static bool aligned16(const void* p)
 {
-    const unsigned bt = block_threads(nrhs);
-    size_t nb         = (n + bt - 1) / bt;
-    const size_t cap  = partials ? std::min<size_t>((size_t)REDUCE_MAX_BLOCKS, c->npartials / (size_t)nrhs)
-                                 : (size_t)65536;
+    return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+
+static BlockShape block_shape(cl2_ctx* c, size_t n, int prec, int nrhs, int u, bool partials,
+                              std::initializer_list<const void*> ptrs)
+{
+    BlockShape b;
+    bool pairs = prec != CL2_F64 && (nrhs % 2 == 0 || nrhs == 1) && n % 2 == 0;
+    for (const void* p : ptrs) pairs = pairs && aligned16(p);
+    b.cpc    = pairs ? 2 : 1;
+    b.ncol   = nrhs % b.cpc == 0 ? nrhs / b.cpc : 1;
+    b.bt     = 256u - (256u % (unsigned)b.ncol);
+    b.nchunk = n / (size_t)b.cpc;
+    size_t nb = (b.nchunk + (size_t)b.bt * u - 1) / ((size_t)b.bt * u);
+    const size_t cap = partials ? c->npartials / (size_t)nrhs : ((size_t)1 << 22);
     if (nb > cap) nb = cap;
-    return nb == 0 ? 1u : (unsigned)nb;
+    b.nb = nb == 0 ? 1u : (unsigned)nb;
+    return b;
+}
+
+constexpr int BLOCK_U_REDUCE = 4;
+
+static void block_reduce2(cl2_ctx* c, const double2* parts, unsigned nb, double* res, int out_kind, int nrhs)
+{
+    hipLaunchKernelGGL((k_block_reduce2<1024>), dim3(nrhs), dim3(1024), 0, c->stream, parts, nb, res, out_kind);
 }
 
 hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,
                              const double* coef, double sign, double* norms)
 {
-    const size_t n    = (size_t)12 * c->g.veo * nrhs;
-    const unsigned nb = grid_for(c, n, nrhs, norms != nullptr), bt = block_threads(nrhs);
-    double2* parts    = norms ? reinterpret_cast<double2*>(c->partials) : nullptr;
-    if (prec == CL2_F64)
-        hipLaunchKernelGGL(k_block_axpy<double>, dim3(nb), dim3(bt), 0, c->stream, reinterpret_cast<double2*>(out),
-                           reinterpret_cast<const double2*>(x), reinterpret_cast<const double2*>(y), n, nrhs, coef,
-                           sign, parts);
-    else
-        hipLaunchKernelGGL(k_block_axpy<float>, dim3(nb), dim3(bt), 0, c->stream, reinterpret_cast<float2*>(out),
-                           reinterpret_cast<const float2*>(x), reinterpret_cast<const float2*>(y), n, nrhs, coef, sign,
-                           parts);
+    const size_t n = (size_t)12 * c->g.veo * nrhs;
+    const int u    = norms ? BLOCK_U_REDUCE : 1;
+    const BlockShape b = block_shape(c, n, prec, nrhs, u, norms != nullptr, {out, x, y});
+    double2* parts     = norms ? reinterpret_cast<double2*>(c->partials) : nullptr;
+    if (prec == CL2_F64) {
+        if (norms)
+            hipLaunchKernelGGL((k_block_axpy<double, 1, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x,
+                               y, b.nchunk, nrhs, b.ncol, coef, sign, parts);
+        else
+            hipLaunchKernelGGL((k_block_axpy<double, 1, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x, y, b.nchunk,
+                               nrhs, b.ncol, coef, sign, parts);
+    } else if (b.cpc == 2) {
+        if (norms)
+            hipLaunchKernelGGL((k_block_axpy<float, 2, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x,
+                               y, b.nchunk, nrhs, b.ncol, coef, sign, parts);
+        else
+            hipLaunchKernelGGL((k_block_axpy<float, 2, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x, y, b.nchunk,
+                               nrhs, b.ncol, coef, sign, parts);
+    } else {
+        if (norms)
+            hipLaunchKernelGGL((k_block_axpy<float, 1, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x,
+                               y, b.nchunk, nrhs, b.ncol, coef, sign, parts);
+        else
+            hipLaunchKernelGGL((k_block_axpy<float, 1, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x, y, b.nchunk,
+                               nrhs, b.ncol, coef, sign, parts);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !norms) return e;
-    hipLaunchKernelGGL(k_block_reduce2, dim3(nrhs), dim3(256), 0, c->stream, parts, nb, norms, 0);
+    block_reduce2(c, parts, b.nb, norms, 0, nrhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_xp(cl2_ctx* c, void* x, void* p, const void* r, int prec, int nrhs, const double* alpha,
+                           const double* beta)
+{
+    const size_t n     = (size_t)12 * c->g.veo * nrhs;
+    const BlockShape b = block_shape(c, n, prec, nrhs, 1, false, {x, p, r});
+    if (prec == CL2_F64)
+        hipLaunchKernelGGL((k_block_xp<double, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, p, r, b.nchunk, nrhs,
+                           b.ncol, alpha, beta);
+    else if (b.cpc == 2)
+        hipLaunchKernelGGL((k_block_xp<float, 2>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, p, r, b.nchunk, nrhs,
+                           b.ncol, alpha, beta);
+    else
+        hipLaunchKernelGGL((k_block_xp<float, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, p, r, b.nchunk, nrhs,
+                           b.ncol, alpha, beta);
     return hipGetLastError();
 }
 
 hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode)
 {
-    const size_t n    = (size_t)12 * c->g.veo * nrhs;
-    const unsigned nb = grid_for(c, n, nrhs, true), bt = block_threads(nrhs);
-    double2* parts    = reinterpret_cast<double2*>(c->partials);
+    const size_t n     = (size_t)12 * c->g.veo * nrhs;
+    const BlockShape b = block_shape(c, n, prec, nrhs, BLOCK_U_REDUCE, true, {x, y});
+    double2* parts     = reinterpret_cast<double2*>(c->partials);
     if (prec == CL2_F64)
-        hipLaunchKernelGGL(k_block_dot<double>, dim3(nb), dim3(bt), 0, c->stream, reinterpret_cast<const double2*>(x),
-                           reinterpret_cast<const double2*>(y), n, nrhs, mode, parts);
+        hipLaunchKernelGGL((k_block_dot<double, 1, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, y,
+                           b.nchunk, nrhs, b.ncol, mode, parts);
+    else if (b.cpc == 2)
+        hipLaunchKernelGGL((k_block_dot<float, 2, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, y,
+                           b.nchunk, nrhs, b.ncol, mode, parts);
     else
-        hipLaunchKernelGGL(k_block_dot<float>, dim3(nb), dim3(bt), 0, c->stream, reinterpret_cast<const float2*>(x),
-                           reinterpret_cast<const float2*>(y), n, nrhs, mode, parts);
+        hipLaunchKernelGGL((k_block_dot<float, 1, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, y,
+                           b.nchunk, nrhs, b.ncol, mode, parts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_block_reduce2, dim3(nrhs), dim3(256), 0, c->stream, parts, nb, res, mode == 1 ? 1 : 0);
+    block_reduce2(c, parts, b.nb, res, mode == 1 ? 1 : 0, nrhs);
     return hipGetLastError();
 }
 

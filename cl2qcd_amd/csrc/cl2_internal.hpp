@@ -139,6 +139,8 @@ hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, i
 hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,
                              const double* coef, double sign, double* norms);
 hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode);
+hipError_t launch_block_xp(cl2_ctx* c, void* x, void* p, const void* r, int prec, int nrhs, const double* alpha,
+                           const double* beta);
 hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, void* single_out, int prec, int nrhs,
                              int r);
 hipError_t launch_scalar_copy(cl2_ctx* c, double* dst, const double* src, int n);

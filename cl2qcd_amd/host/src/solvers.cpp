@@ -525,10 +525,10 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
         apply(&V, P, tmplo);
         system.check(cl2_block_dot_eo(c, pAp.dev(), P.dev(), V.dev(), lo, n));
         system.check(cl2_block_cg_alpha(c, alpha.dev(), r2.dev(), pAp.dev(), active.dev(), n));
-        system.check(cl2_block_axpy_eo(c, Xlo.dev(), P.dev(), Xlo.dev(), lo, n, alpha.dev(), 1., nullptr));
         system.check(cl2_block_axpy_eo(c, Rlo.dev(), V.dev(), Rlo.dev(), lo, n, alpha.dev(), -1., norms.dev()));
         system.check(cl2_block_cg_beta(c, beta.dev(), r2.dev(), norms.dev(), active.dev(), n));
-        system.check(cl2_block_axpy_eo(c, P.dev(), P.dev(), Rlo.dev(), lo, n, beta.dev(), 1., nullptr));  // p = r + beta p
+        // x += alpha p and p = r + beta p in one pass
+        system.check(cl2_block_cg_update_xp_eo(c, Xlo.dev(), P.dev(), Rlo.dev(), lo, n, alpha.dev(), beta.dev()));
     };
     struct Graph {
         cl2_graph* g = nullptr;

@@ -267,6 +267,11 @@ int cl2_block_extract_eo(cl2_ctx* ctx, void* single, const void* block, int prec
  * per right-hand side (|out_r|^2, Re <out_r, y_r>) with y read before it is overwritten. */
 int cl2_block_axpy_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, int nrhs, const double* coef,
                       double sign, double* norms);
+/* No reference counterpart (the lockstep block CG's x and p updates, solvers/cg.cpp:334 and :149 for every
+ * right-hand side, in one pass): x_r = x_r + alpha_r p_r, then p_r = r_r + beta_r p_r (alpha, beta: nrhs
+ * complex on the device); x, p, r distinct blocks. */
+int cl2_block_cg_update_xp_eo(cl2_ctx* ctx, void* x, void* p, const void* r, int prec, int nrhs, const double* alpha,
+                              const double* beta);
 /* res[r] = <x_r, y_r> (complex, 2 doubles each) / |x_r|^2 (one double each) */
 int cl2_block_dot_eo(cl2_ctx* ctx, double* res, const void* x, const void* y, int prec, int nrhs);
 int cl2_block_squarenorm_eo(cl2_ctx* ctx, double* res, const void* x, int prec, int nrhs);

@@ -84,9 +84,9 @@ def test_fermionmatrix_mrhs(action):
             assert site_rel_err(block_slot(ctx, out, abi.F64, nrhs, r), ref) < 1e-12
 
 
+@pytest.mark.parametrize("nrhs", [1, 3, 12])  # FP32: two right-hand sides per 16-byte lane (1, 12) or one (3)
 @pytest.mark.parametrize("prec", [abi.F64, abi.F32])
-def test_block_blas(prec):
-    nrhs = 12
+def test_block_blas(prec, nrhs):
     with abi.Context(4, 8) as ctx:
         rng = np.random.default_rng(8)
         X = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
@@ -117,6 +117,32 @@ def test_block_blas(prec):
             assert site_rel_err(block_slot(ctx, by, prec, nrhs, r), new) < tol
             assert abs(nm[2 * r] - O.norm2(new)) < tol * O.norm2(new)
             assert abs(nm[2 * r + 1] - O.dot(new, Y[r]).real) < tol * O.norm2(new)
+
+
+@pytest.mark.parametrize("nrhs", [1, 3, 12])
+@pytest.mark.parametrize("prec", [abi.F64, abi.F32])
+def test_block_cg_update_xp(prec, nrhs):
+    """x += alpha_r p, p = r + beta_r p in one pass: the same bits as the two block axpys it replaces."""
+    with abi.Context(4, 8) as ctx:
+        rng = np.random.default_rng(9)
+        X, P, R = ([rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)] for _ in range(3))
+        x1, p1, r = make_block(ctx, X, prec), make_block(ctx, P, prec), make_block(ctx, R, prec)
+        x2, p2 = make_block(ctx, X, prec), make_block(ctx, P, prec)
+        al, be = ctx.scalars(2 * nrhs), ctx.scalars(2 * nrhs)
+        al.store(rng.standard_normal(2 * nrhs))
+        be.store(rng.standard_normal(2 * nrhs))
+        ctx.block_axpy_eo(x1.ptr, p1.ptr, x1.ptr, prec, nrhs, al.ptr, 1.0, None)
+        ctx.block_axpy_eo(p1.ptr, p1.ptr, r.ptr, prec, nrhs, be.ptr, 1.0, None)
+        ctx.block_cg_update_xp_eo(x2.ptr, p2.ptr, r.ptr, prec, nrhs, al.ptr, be.ptr)
+        for k in range(nrhs):
+            assert np.array_equal(block_slot(ctx, x1, prec, nrhs, k), block_slot(ctx, x2, prec, nrhs, k))
+            assert np.array_equal(block_slot(ctx, p1, prec, nrhs, k), block_slot(ctx, p2, prec, nrhs, k))
+        if prec == abi.F64:
+            a, b = al.get(), be.get()
+            for k in range(nrhs):
+                ca, cb = complex(a[2 * k], a[2 * k + 1]), complex(b[2 * k], b[2 * k + 1])
+                assert site_rel_err(block_slot(ctx, x2, prec, nrhs, k), O.saxpy(P[k], X[k], -ca)) < 1e-15
+                assert site_rel_err(block_slot(ctx, p2, prec, nrhs, k), O.saxpy(P[k], R[k], -cb)) < 1e-15
 
 
 def test_block_cg_matches_single_solves():
