@@ -172,14 +172,16 @@ struct RhsParams {
 
 int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void* gauge, int fmt, int evenodd,
                double kappa, int epilogue, const double f1[2], const double f2[2], double scale, int gamma5,
-               int nrhs = 1, const RhsParams* pr = nullptr, double* norm_res = nullptr)
+               int nrhs = 1, const RhsParams* pr = nullptr, double* norm_res = nullptr, void* cg_r = nullptr,
+               const double* cg_alpha = nullptr)
 {
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
     CL2_REQUIRE(c, supported_nrhs(nrhs) && (nrhs == 1 || fmt != CL2_GAUGE_F16_12),
                 "unsupported number of right-hand sides (1, 2, 4, 8, 12 or 16; FP16 fields: 1)");
     CL2_REQUIRE(c, evenodd == CL2_EVEN || evenodd == CL2_ODD, "evenodd must be CL2_EVEN or CL2_ODD");
-    CL2_REQUIRE(c, out && in && gauge, "null field pointer");
-    CL2_REQUIRE(c, out != in, "dslash cannot run in place: out aliases in");
+    CL2_REQUIRE(c, (out || cg_r) && in && gauge, "null field pointer");
+    CL2_REQUIRE(c, out != in && cg_r != in, "dslash cannot run in place: out aliases in");
+    CL2_REQUIRE(c, cg_r == nullptr || cg_r != x, "the CG residual must differ from the diagonal-term field");
     CL2_REQUIRE(c, epilogue != EPI_AEE || x != nullptr, "null diagonal-term field");
     CL2_REQUIRE(c, epilogue != EPI_AEE || x != in, "diagonal-term field must differ from the hopping input");
     DslashArgs a;
@@ -197,6 +199,8 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.f2[1]         = f2[1];
     a.scale         = scale;
     a.norm_partials = norm_res ? c->partials : nullptr;
+    a.cg_r          = cg_r;
+    a.cg_alpha      = cg_alpha;
     a.per_rhs       = (pr != nullptr && nrhs > 1) ? 1 : 0;
     a.w             = wilson_weights(c, a.per_rhs ? 1.0 : kappa);
     for (int r = 0; r < 16; ++r) {
@@ -216,10 +220,13 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.rb            = a.per_rhs ? 1 : t.rb;  // per-right-hand-side operators: one right-hand side per thread
     a.lds           = t.lds;
     if (nrhs > 1 && t.lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
-    CL2_REQUIRE(c, norm_res == nullptr || nrhs == 1, "output norms are for single fields");
+    CL2_REQUIRE(c, (norm_res == nullptr && cg_r == nullptr) || nrhs == 1, "output norms are for single fields");
+    CL2_REQUIRE(c, cg_r == nullptr || (cg_alpha != nullptr && norm_res != nullptr), "CG update without its scalars");
     CL2_HIP(c, launch_dslash(c, a, fmt));
-    // every single-field variant launches blocks_for(Veo, block) blocks, each writing one partial
-    if (norm_res) CL2_HIP(c, launch_reduce_stage2(c, blocks_for(a.g.veo, a.block), norm_res, 1, nullptr, 0));
+    // every single-field variant launches blocks_for(Veo, block) blocks, each writing one partial: |out|^2 as a
+    // complex (norm, 0), or with the CG update (|r_new|^2, Re <r_new, r_old>, |r_new|^2, 0)
+    if (norm_res)
+        CL2_HIP(c, launch_reduce_stage2(c, blocks_for(a.g.veo, a.block), norm_res, cg_r ? 3 : 1, nullptr, 0));
     return CL2_OK;
 }
 
@@ -255,7 +262,8 @@ void tm_factors(int variant, double mubar, double f1[2], double f2[2], double* s
  * out = M_site in - D tmp [gamma5]   (fermionmatrix.cpp:224-591). */
 int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge, int fmt, int action, double kappa,
             double mubar, bool minus, bool gamma5, int nrhs = 1, const double* kappas = nullptr,
-            const double* mubars = nullptr, double* norm_res = nullptr)
+            const double* mubars = nullptr, double* norm_res = nullptr, void* cg_r = nullptr,
+            const double* cg_alpha = nullptr)
 {
     double f1[2], f2[2], s;
     int rc;
@@ -288,14 +296,14 @@ int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge,
         rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_PLAIN, one, one, 1., 0, nrhs);
         if (rc) return rc;
         return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, one, one, 1., gamma5 ? 1 : 0, nrhs,
-                          nullptr, norm_res);
+                          nullptr, norm_res, cg_r, cg_alpha);
     }
     tm_factors(minus ? CL2_MTM_INVERSE_SITEDIAGONAL_MINUS : CL2_MTM_INVERSE_SITEDIAGONAL, mubar, f1, f2, &s);
     rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, kappa, EPI_MINV, f1, f2, s, 0, nrhs);
     if (rc) return rc;
     tm_factors(minus ? CL2_MTM_SITEDIAGONAL_MINUS : CL2_MTM_SITEDIAGONAL, mubar, f1, f2, &s);
     return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, kappa, EPI_AEE, f1, f2, 1., gamma5 ? 1 : 0, nrhs, nullptr,
-                      norm_res);
+                      norm_res, cg_r, cg_alpha);
 }
 
 /* the eo operators on nrhs right-hand sides ([12][Veo][nrhs] fields, tmp = 2 such fields) */
@@ -807,6 +815,29 @@ int cl2_fermionmatrix_eo_dot(cl2_ctx* c, int op, void* out, const void* in, cons
     CL2_REQUIRE(c, op == CL2_OP_QPLUSQMINUS_EO, "<in, A in> with the operator is for QplusQminus_eo");
     CL2_REQUIRE(c, dot != nullptr, "null result pointer");
     return run_fermionmatrix(c, op, out, in, gauge, fmt, action, kappa, mubar, tmp, 1, nullptr, nullptr, dot);
+}
+
+int cl2_fermionmatrix_eo_cg_step(cl2_ctx* c, int op, void* r, const void* p, const void* gauge, int fmt, int action,
+                                 double kappa, double mubar, void* tmp, const double* r2, double* pAp, double* alpha,
+                                 double* norms)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, op == CL2_OP_QPLUSQMINUS_EO, "the fused CG step is for QplusQminus_eo");
+    CL2_REQUIRE(c, action == CL2_ACTION_WILSON || action == CL2_ACTION_TWISTEDMASS,
+                "unknown fermion action: wilson or twistedmass");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, r && p && gauge && tmp && r2 && pAp && alpha && norms, "null pointer");
+    CL2_REQUIRE(c, r != p, "r and p must be different fields");
+    const size_t fb = spinor_bytes(c, fmt_prec(fmt));
+    void* t0        = tmp;
+    void* t1        = static_cast<char*>(tmp) + fb;
+    CL2_REQUIRE(c, r != t0 && r != t1 && p != t0 && p != t1, "tmp must not alias r or p");
+    // t1 = Qminus p with pAp = |t1|^2; alpha = r2 / pAp; then Qplus t1 is folded into r <- r - alpha Qplus t1
+    if (int rc = run_aee(c, t1, p, t0, gauge, fmt, action, kappa, mubar, true, true, 1, nullptr, nullptr, pAp))
+        return rc;
+    CL2_HIP(c, launch_scalar_op(c, CL2_SCALAR_OP_RATIO, 1, alpha, r2, pAp));
+    return run_aee(c, nullptr, t1, t0, gauge, fmt, action, kappa, mubar, false, true, 1, nullptr, nullptr, norms, r,
+                   alpha);
 }
 
 int cl2_fermionmatrix_eo_mrhs(cl2_ctx* c, int op, void* out, const void* in, const void* gauge, int fmt, int action,

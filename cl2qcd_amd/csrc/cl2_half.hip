@@ -189,7 +189,7 @@ void k_dslash_eo_h(DslashArgs a)
     __shared__ double lds[2 * BLOCK / 64];
     const uint32_t S = a.g.veo;
     uint32_t i       = 0;
-    double nrm       = 0.;
+    double nrm = 0., ry = 0.;
     if (map_site<BLOCK>(a.map, S, blockIdx.x, threadIdx.x, i)) {
         SiteNb nb;
         site_neighbours(a.g, i, a.out_parity, nb);
@@ -213,13 +213,27 @@ void k_dslash_eo_h(DslashArgs a)
                 if (a.gamma5 && k >= 6) acc[k] = Cx<float>{-acc[k].re, -acc[k].im};
             }
         }
-        store_site_h<true>(a.out, S, i, acc);
+        if (a.cg_r) {  // CG residual update in place of storing the result (see the FP64/FP32 kernel)
+            Cx<float> ro[12], rn[12];
+            load_site_h<true>(a.cg_r, S, i, ro);
+            const Cx<float> al{float(a.cg_alpha[0]), float(a.cg_alpha[1])};
 #pragma unroll
-        for (int k = 0; k < 12; ++k) nrm += (double)acc[k].re * acc[k].re + (double)acc[k].im * acc[k].im;
+            for (int k = 0; k < 12; ++k) {
+                rn[k] = ro[k] - cmul(acc[k], al);
+                ry += (double)rn[k].re * ro[k].re + (double)rn[k].im * ro[k].im;
+            }
+            store_site_h<true>(a.cg_r, S, i, rn);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) nrm += (double)rn[k].re * rn[k].re + (double)rn[k].im * rn[k].im;
+        } else {
+            store_site_h<true>(a.out, S, i, acc);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) nrm += (double)acc[k].re * acc[k].re + (double)acc[k].im * acc[k].im;
+        }
     }
     if (a.norm_partials) {
-        double s = block_sum<BLOCK>(nrm, lds);
-        if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s, a.norm_partials[2 * blockIdx.x + 1] = 0.;
+        const double2 s = block_sum2<BLOCK>(make_double2(nrm, ry), lds);
+        if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s.x, a.norm_partials[2 * blockIdx.x + 1] = s.y;
     }
 }
 

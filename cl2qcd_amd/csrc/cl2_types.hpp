@@ -91,7 +91,9 @@ struct DslashArgs {
     int gamma5;
     double f1[2], f2[2];  // site-diagonal factors
     double scale;
-    double* norm_partials;  // if non-null: per-block sum |out|^2
+    double* norm_partials;  // if non-null: per-block sum |out|^2 (with cg_r: |r_new|^2, Re <r_new, r_old>)
+    void* cg_r;             // single fields: if set, r <- r - (*cg_alpha) out in place of storing out
+    const double* cg_alpha; // complex device scalar
     HopWeights<double> w;
     WorkMap map;
     int nrhs;  // right-hand sides per launch (fields [12][Veo][nrhs]); 1 for single fields

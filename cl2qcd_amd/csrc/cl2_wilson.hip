@@ -143,9 +143,11 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
  * link addresses and each link leaves HBM once per launch.  NRHS = 1 is the single-field kernel. */
 /* Epilogue (per-right-hand-side kappa when the operator differs per right-hand side, site-diagonal factors,
  * Aee saxpy, gamma5) and streaming store of output element O (right-hand side rhs) of a field of component
- * stride SS, adding |out|^2 of the site to NRM.  A macro rather than a function: passing the
+ * stride SS, adding |out|^2 of the site to NRM.  With a.cg_r set the result v is not stored: the CG residual
+ * is updated in its place, r <- r - alpha v (alpha = a.cg_alpha, as the elementwise kernel computes it), and
+ * NRM, RY receive |r_new|^2 and Re <r_new, r_old>.  A macro rather than a function: passing the
  * accumulator array to a helper, even a force-inlined one, made the compiler keep it in scratch memory. */
-#define CL2_FINISH_SITE(O, NRM)                                                                                   \
+#define CL2_FINISH_SITE(O, NRM, RY)                                                                               \
     {                                                                                                             \
         const size_t o_ = (O);                                                                                    \
         Cx<R> f1{R(a.f1[0]), R(a.f1[1])}, f2{R(a.f2[0]), R(a.f2[1])};                                             \
@@ -167,10 +169,22 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
                 if (a.gamma5 && k >= 6) acc[k] = Cx<R>{-acc[k].re, -acc[k].im};                                   \
             }                                                                                                     \
         }                                                                                                         \
-        V* out_ = reinterpret_cast<V*>(a.out);                                                                    \
-        _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                          \
-            store_stream(out_ + (size_t)k * SS + o_, V{acc[k].re, acc[k].im});                                    \
-            NRM += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;                 \
+        if (a.cg_r) {                                                                                             \
+            V* r_ = reinterpret_cast<V*>(a.cg_r);                                                                 \
+            const Cx<R> al_{R(a.cg_alpha[0]), R(a.cg_alpha[1])};                                                  \
+            _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                      \
+                const V rv     = load_stream(r_ + (size_t)k * SS + o_);                                           \
+                const Cx<R> rn = Cx<R>{rv.x, rv.y} - cmul(acc[k], al_);                                           \
+                store_stream(r_ + (size_t)k * SS + o_, V{rn.re, rn.im});                                          \
+                NRM += (double)rn.re * (double)rn.re + (double)rn.im * (double)rn.im;                             \
+                RY += (double)rn.re * (double)rv.x + (double)rn.im * (double)rv.y;                                \
+            }                                                                                                     \
+        } else {                                                                                                  \
+            V* out_ = reinterpret_cast<V*>(a.out);                                                                \
+            _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                      \
+                store_stream(out_ + (size_t)k * SS + o_, V{acc[k].re, acc[k].im});                                \
+                NRM += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;             \
+            }                                                                                                     \
         }                                                                                                         \
     }
 
@@ -189,7 +203,7 @@ void k_dslash_eo(DslashArgs a)
     const uint32_t S = a.g.veo;
     const size_t SS  = (size_t)S * NRHS;
     uint32_t i = 0, rhs = 0;
-    double nrm = 0.;
+    double nrm = 0., ry = 0.;
     if (map_site_rhs<BLOCK, NRHS>(a.map, S, blockIdx.x, threadIdx.x, i, rhs)) {
         SiteNb nb;
         site_neighbours(a.g, i, a.out_parity, nb);
@@ -218,11 +232,11 @@ void k_dslash_eo(DslashArgs a)
         }
 #undef CL2_DIR
 #undef CL2_W
-        CL2_FINISH_SITE((size_t)i * NRHS + rhs, nrm)
+        CL2_FINISH_SITE((size_t)i * NRHS + rhs, nrm, ry)
     }
     if (a.norm_partials) {
-        double s = block_sum<BLOCK>(nrm, lds);
-        if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s, a.norm_partials[2 * blockIdx.x + 1] = 0.;
+        const double2 s = block_sum2<BLOCK>(make_double2(nrm, ry), lds);
+        if (threadIdx.x == 0) a.norm_partials[2 * blockIdx.x] = s.x, a.norm_partials[2 * blockIdx.x + 1] = s.y;
     }
 }
 
@@ -310,9 +324,10 @@ __global__ __launch_bounds__(MrhsLds<NRHS>::BLOCK) void k_dslash_eo_mrhs_lds(Dsl
     CL2_DIR(2)
     CL2_DIR(3)
 #undef CL2_DIR
-    double nrm = 0.;  // no per-block norms from this kernel
-    CL2_FINISH_SITE((size_t)i * NRHS + rhs, nrm)
+    double nrm = 0., ry = 0.;  // no per-block norms from this kernel
+    CL2_FINISH_SITE((size_t)i * NRHS + rhs, nrm, ry)
     (void)nrm;
+    (void)ry;
 }
 
 /* Register-blocked multi-RHS variant: one thread per (site, group of RB right-hand sides); each link is

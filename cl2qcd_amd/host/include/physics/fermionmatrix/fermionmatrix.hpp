@@ -86,6 +86,12 @@ class BuiltinEoOperator : public Fermionmatrix_eo {
     void apply_and_dot(const lattices::Spinorfield_eo* out, const lattices::Gaugefield& gf,
                        const lattices::Spinorfield_eo& in, const physics::AdditionalParameters& additionalParameters,
                        const lattices::Scalar<hmc_complex>& dot) const;
+    /* QplusQminus_eo only: the operator half of a CG step without storing A p: pAp = <p, A p>, alpha = r2 / pAp,
+     * r <- r - alpha A p and norms (4 values) = (|r_new|^2, Re <r_new, r_old>, |r_new|^2, 0) */
+    void cg_step(const lattices::Spinorfield_eo* r, const lattices::Gaugefield& gf, const lattices::Spinorfield_eo& p,
+                 const physics::AdditionalParameters& additionalParameters, const lattices::Scalar<hmc_complex>& r2,
+                 const lattices::Scalar<hmc_complex>& pAp, const lattices::Scalar<hmc_complex>& alpha,
+                 const lattices::Vector<hmc_float>& norms) const;
 
   protected:
     BuiltinEoOperator(int op, bool herm, const hardware::System& system, const FermionEoParametersInterface& p);

@@ -152,6 +152,23 @@ void BuiltinEoOperator::apply_and_dot(const Spinorfield_eo* out, const Gaugefiel
                                                 abi_action, kappa, mubar, tmp.ptr(), dot.dev()));
 }
 
+void BuiltinEoOperator::cg_step(const Spinorfield_eo* r, const Gaugefield& gf, const Spinorfield_eo& p,
+                                const physics::AdditionalParameters& ap, const lattices::Scalar<hmc_complex>& r2,
+                                const lattices::Scalar<hmc_complex>& pAp, const lattices::Scalar<hmc_complex>& alpha,
+                                const lattices::Vector<hmc_float>& norms) const
+{
+    if (op != CL2_OP_QPLUSQMINUS_EO) throw std::invalid_argument("cg_step: QplusQminus_eo only");
+    if (norms.get_vector_size() < 4) throw std::invalid_argument("cg_step needs a result vector of 4 elements");
+    const int abi_action = abi_action_of(fermionmatrixParametersInterface);
+    if (r->precision() != p.precision()) throw std::invalid_argument("fields of different precision");
+    const hmc_float kappa = ap.getKappa();
+    const hmc_float mubar = abi_action == CL2_ACTION_TWISTEDMASS ? ap.getMubar() : 0.;
+    const Links l         = links_for(gf, p.precision());
+    get_system().check(cl2_fermionmatrix_eo_cg_step(get_system().ctx(), op, r->dev(), p.dev(), l.ptr, l.fmt,
+                                                    abi_action, kappa, mubar, tmp.ptr(), r2.dev(), pAp.dev(),
+                                                    alpha.dev(), norms.dev()));
+}
+
 cl_ulong BuiltinEoOperator::get_flops() const
 {
     const cl_ulong S = get_system().eo_sites();

@@ -366,18 +366,18 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
     const auto* builtin  = dynamic_cast<const fermionmatrix::BuiltinEoOperator*>(&A);
     const bool fused_dot = builtin != nullptr && builtin->abi_operator() == CL2_OP_QPLUSQMINUS_EO;
     // one inner iteration, every scalar on the device: v = A p with <p, v>, alpha = r2 / <p, v>,
-    // r_lo -= alpha v with its norms, beta (Polak-Ribiere) and r2 <- |r_lo|^2, then x_lo += alpha p and
-    // p = r_lo + beta p in one pass
+    // r_lo -= alpha v with its norms (for QplusQminus_eo all of it inside the operator's kernels, v never
+    // stored), beta (Polak-Ribiere) and r2 <- |r_lo|^2, then x_lo += alpha p and p = r_lo + beta p in one pass
     auto body = [&]() {
         if (fused_dot) {
-            builtin->apply_and_dot(&v, gf, p, ap, pAp);
+            builtin->cg_step(&r_lo, gf, p, ap, r2c, pAp, alpha, norms);
         } else {
             A(&v, gf, p, ap);
             scalar_product(&pAp, p, v);
+            divide(&alpha, r2c, pAp);
+            saxpy_AND_cgnorm(&r_lo, alpha, v, r_lo, norms);
         }
-        divide(&alpha, r2c, pAp);
         multiply(&malpha, minus_one, alpha);
-        saxpy_AND_cgnorm(&r_lo, alpha, v, r_lo, norms);
         system.check(cl2_cg_beta(c, mbeta.dev(), r2c.dev(), norms.dev()));
         cg_update_xp(&x_lo, &p, r_lo, malpha, mbeta);
     };

@@ -193,6 +193,15 @@ int cl2_fermionmatrix_eo(cl2_ctx* ctx, int op, void* out, const void* in, const 
  * reading pass over in and out fewer. */
 int cl2_fermionmatrix_eo_dot(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge, int format,
                              int action, double kappa, double mubar, void* tmp, double* dot);
+/* The operator half of a CG step on QplusQminus_eo (op must be CL2_OP_QPLUSQMINUS_EO), replacing
+ * solvers/cg.cpp:327-341 (v = A p, rho = <p, v>, alpha = omega / rho, r -= alpha v with its norm):
+ * pAp (2 device doubles) = <p, A p> = |Qminus_eo p|^2, alpha (2) = r2 / pAp (r2: complex device scalar), and
+ * r <- r - alpha A p, with norms (4) = (|r_new|^2, Re <r_new, r_old>, |r_new|^2, 0) as
+ * cl2_saxpy_AND_cgnorm_eo gives them.  A p is never stored: the last hopping kernel updates r in its
+ * epilogue.  tmp: 2 spinor fields; r, p and tmp distinct. */
+int cl2_fermionmatrix_eo_cg_step(cl2_ctx* ctx, int op, void* r, const void* p, const void* gauge, int format,
+                                 int action, double kappa, double mubar, void* tmp, const double* r2, double* pAp,
+                                 double* alpha, double* norms);
 
 /* ==== Wilson eo BLAS (hardware/code/spinors.hpp:50-224) ==== */
 

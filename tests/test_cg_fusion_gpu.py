@@ -4,6 +4,8 @@
   |Qminus_eo in|^2 by the kernel that writes Qminus_eo in) agrees with scalar_product(in, out) to 1e-12 in
   FP64 and with the oracle's <in, QplusQminus_eo in> to 1e-12; FP32 to 1e-5; FP16 (the intermediate field is
   stored in half precision and re-read) to 1e-2.
+* cl2_fermionmatrix_eo_cg_step: the same bits as the unfused operator / ratio / saxpy_AND_cgnorm sequence
+  (FP64, FP32), FP16 to 5e-3, and the oracle's step in FP64 to 1e-12.
 * cl2_cg_update_xp: bit-identical to the two saxpy calls it replaces (Wilson FP64/FP32/FP16) and to the
   oracle's x - a p, r - b p (staggered); the FP32 kernel's unaligned path agrees bit for bit with the
   aligned one.
@@ -49,6 +51,56 @@ def test_fermionmatrix_eo_dot(fmt, prec, tol, action):
         with pytest.raises(abi.Cl2Error):
             ctx.fermionmatrix_eo_dot(abi.OP_QPLUS_EO, c.ptr, a.ptr, gf.ptr, fmt, action, KAPPA, mubar, tmp.ptr,
                                      res.at(0))
+
+
+@pytest.mark.parametrize("fmt,prec", [(abi.GAUGE_F64_18, abi.F64), (abi.GAUGE_F32_12, abi.F32),
+                                      (abi.GAUGE_F16_12, abi.F16)])
+def test_fermionmatrix_eo_cg_step(fmt, prec):
+    """pAp, alpha, r <- r - alpha A p and its norms from the fused step against the unfused sequence (operator
+    with <p, A p>, ratio, saxpy_AND_cgnorm): the same bits for FP64/FP32 fields and scalars (norms summed in
+    another grouping: 1e-12); FP16 stores A p in half precision in the unfused sequence only: 5e-3.  FP64
+    also against the oracle."""
+    lat = O.lattice(8, 8, theta_t=1.0)
+    with abi.Context(8, 8, theta_t=1.0) as ctx:
+        g = random_su3_field(4 * O.volume(lat), 41)
+        rng = np.random.default_rng(42)
+        P, R = rng.standard_normal((O.veo(lat), 24)), rng.standard_normal((O.veo(lat), 24))
+        g64 = ctx.gauge(g)
+        gf = g64 if fmt == abi.GAUGE_F64_18 else g64.converted(fmt)
+        p, r1, r2f = ctx.spinor(prec).upload(P), ctx.spinor(prec).upload(R), ctx.spinor(prec).upload(R)
+        v = ctx.spinor(prec)
+        tmp = ctx.buffer(2 * ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, prec))
+        s = ctx.scalars(24)  # r2 0..1, pAp 2..3, alpha 4..5, norms 6..9 | unfused: pAp 12..13, alpha 14..15, norms 16..19
+        r2 = O.norm2(R)
+        s.store([r2, 0.0] + [0.0] * 22)
+        ctx.fermionmatrix_eo_dot(abi.OP_QPLUSQMINUS_EO, v.ptr, p.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS, KAPPA, MUBAR,
+                                 tmp.ptr, s.at(12))
+        ctx.complex_op(abi.SCALAR_OP_RATIO, s.at(14), s.at(0), s.at(12))
+        ctx.saxpy_AND_cgnorm_eo(r1.ptr, v.ptr, r1.ptr, prec, abi.scalar(dev=s.at(14), kind=abi.SCALAR_DEV_COMPLEX),
+                                s.at(16))
+        ctx.fermionmatrix_eo_cg_step(abi.OP_QPLUSQMINUS_EO, r2f.ptr, p.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS, KAPPA,
+                                     MUBAR, tmp.ptr, s.at(0), s.at(2), s.at(4), s.at(6))
+        d = s.get()
+        assert np.array_equal(d[2:6], d[12:16])
+        got, ref = r2f.download(), r1.download()
+        if prec == abi.F16:
+            assert site_rel_err(got, ref) < 5e-3
+            assert abs(d[6] - d[16]) < 1e-2 * d[16] and abs(d[7] - d[17]) < 1e-2 * d[16]
+        else:
+            assert np.array_equal(got, ref)
+            assert abs(d[6] - d[16]) < 1e-12 * d[16] and abs(d[7] - d[17]) < 1e-12 * d[16]
+        assert d[8] == d[6] and d[9] == 0.0
+        if prec == abi.F64:
+            Ap = O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, P, g, O.TWISTEDMASS, KAPPA, MUBAR)
+            alpha = r2 / O.dot(P, Ap)
+            new = O.saxpy(Ap, R, alpha)
+            assert abs(complex(d[4], d[5]) - alpha) < 1e-12 * abs(alpha)
+            assert site_rel_err(got, new) < 1e-12
+            assert abs(d[6] - O.norm2(new)) < 1e-12 * O.norm2(new)
+            assert abs(d[7] - O.dot(new, R).real) < 1e-12 * O.norm2(new)
+        with pytest.raises(abi.Cl2Error):  # r aliasing p
+            ctx.fermionmatrix_eo_cg_step(abi.OP_QPLUSQMINUS_EO, p.ptr, p.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS,
+                                         KAPPA, MUBAR, tmp.ptr, s.at(0), s.at(2), s.at(4), s.at(6))
 
 
 @pytest.mark.parametrize("prec", [abi.F64, abi.F32, abi.F16])
