@@ -1414,7 +1414,8 @@ int cl2_tune_set(cl2_ctx* c, const char* key, long value)
         return CL2_OK;
     }
     if (k == "hop.occupancy") {
-        CL2_REQUIRE(c, value == -1 || value == 0 || value == 1 || value == 4, "hop.occupancy must be -1, 0, 1 or 4");
+        CL2_REQUIRE(c, value == -1 || value == 0 || value == 1 || value == 2 || value == 4,
+                    "hop.occupancy must be -1, 0, 1, 2 or 4");
         c->tune_occ    = (int)value;
         c->tune_manual = true;
         return CL2_OK;
@@ -1463,7 +1464,8 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
             if (nrhs % r == 0) rbs.push_back(r);
     // (block, occupancy hint) shapes of the single-field kernel
     std::vector<std::pair<int, int>> shapes = {{256, 0}};
-    if (nrhs == 1 && fmt == CL2_GAUGE_F32_12) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 4}, {128, 4}};
+    if (nrhs == 1 && fmt == CL2_GAUGE_F32_12)
+        shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 1}, {128, 1}, {256, 2}, {128, 2}, {256, 4}, {128, 4}};
     else if (nrhs == 1 && fmt == CL2_GAUGE_F16_12)
         shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 1}, {128, 1}, {256, 4}, {128, 4}};
     else if (nrhs == 1) shapes = {{256, 0}, {128, 0}, {512, 0}, {256, 1}, {128, 1}};

@@ -191,6 +191,7 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
 /* Register budget and hops in flight per thread:
  *   OCC = 0: at most 2 waves per SIMD (up to 256 VGPRs), the two hops of one direction in flight;
  *   OCC = 1: 1 wave per SIMD, the four hops of two directions in flight (blocks of at most 256);
+ *   OCC = 2: at most 2 waves per SIMD with the four hops of two directions in flight (FP32);
  *   OCC = 4: at least 4 waves per SIMD (at most 128 VGPRs), one hop at a time. */
 template<int FMT, int BLOCK, int NRHS, int OCC = 0>
 __global__ __launch_bounds__(BLOCK)
@@ -221,7 +222,7 @@ void k_dslash_eo(DslashArgs a)
     else                                                                                                            \
         hop_pair<m, FMT>(acc, in, SS, nb.up[m] * NRHS + rhs, nb.dn[m] * NRHS + rhs, a.gauge, S, P, i, nb.dn[m],     \
                          CL2_W(wf, m), CL2_W(wb, m));
-        if constexpr (OCC == 1) {
+        if constexpr (OCC == 1 || OCC == 2) {
             hop_quad<0, 1, FMT, NRHS>(acc, in, SS, nb, rhs, a.gauge, S, P, i, a.w);
             hop_quad<2, 3, FMT, NRHS>(acc, in, SS, nb, rhs, a.gauge, S, P, i, a.w);
         } else {
@@ -560,19 +561,25 @@ static hipError_t launch_dslash_n(cl2_ctx* c, const DslashArgs& a, int format)
             constexpr int F = decltype(fmt_tag)::value;
             const unsigned bs = (unsigned)a.block;
             const unsigned nb = blocks_for((size_t)a.g.veo, (int)bs);
-            // the 1-wave-per-SIMD shapes exist for the FP64 formats (blocks of 128 or 256), the 128-VGPR
-            // shapes for FP32 only (the FP64 kernels spill at that budget)
-            if constexpr (F != GAUGE_F32_12) {
-                if (a.occ == 1 && bs == 128) {
-                    hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 1>), dim3(nb), dim3(128), 0, c->stream, a);
-                    return hipGetLastError();
-                }
-                if (a.occ == 1 && bs != 512) {
-                    hipLaunchKernelGGL((k_dslash_eo<F, 256, 1, 1>), dim3(nb), dim3(256), 0, c->stream, a);
-                    return hipGetLastError();
-                }
+            // the 1-wave-per-SIMD shapes (blocks of 128 or 256), the 128-VGPR shapes and the four-hops-in-
+            // 256-VGPRs shapes for FP32 only (the FP64 kernels spill at those budgets)
+            if (a.occ == 1 && bs == 128) {
+                hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 1>), dim3(nb), dim3(128), 0, c->stream, a);
+                return hipGetLastError();
+            }
+            if (a.occ == 1 && bs != 512) {
+                hipLaunchKernelGGL((k_dslash_eo<F, 256, 1, 1>), dim3(nb), dim3(256), 0, c->stream, a);
+                return hipGetLastError();
             }
             if constexpr (F == GAUGE_F32_12) {
+                if (a.occ == 2 && bs == 128) {
+                    hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 2>), dim3(nb), dim3(128), 0, c->stream, a);
+                    return hipGetLastError();
+                }
+                if (a.occ == 2 && bs != 512) {
+                    hipLaunchKernelGGL((k_dslash_eo<F, 256, 1, 2>), dim3(nb), dim3(256), 0, c->stream, a);
+                    return hipGetLastError();
+                }
                 if (a.occ == 4 && bs == 128) {
                     hipLaunchKernelGGL((k_dslash_eo<F, 128, 1, 4>), dim3(nb), dim3(128), 0, c->stream, a);
                     return hipGetLastError();

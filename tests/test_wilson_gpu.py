@@ -276,7 +276,7 @@ def test_every_launch_variant_bit_identical(fmt, prec):
         tmp = ctx.buffer(2 * ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F64))
         results = {}
         for block in (128, 256, 512):
-            for occ in (0, 1, 4):
+            for occ in (0, 1, 2, 4):
                 for zp, xcd in ((-1, 1), (0, 0), (2, 1)):
                     for key, v in (("hop.block", block), ("hop.occupancy", occ), ("hop.zplanes", zp), ("hop.xcd", xcd)):
                         ctx.check(ctx.lib.cl2_tune_set(ctx.handle, key.encode(), v))
@@ -341,7 +341,7 @@ def test_staggered_launch_variants_bit_identical(fmt, prec):
         a, b = ctx.su3vec(prec).upload(src), ctx.su3vec(prec)
         results = {}
         for block in (128, 256, 512):
-            for occ in (0, 1, 4):
+            for occ in (0, 1, 2, 4):
                 for zp, xcd in ((-1, 1), (0, 0), (2, 1)):
                     for key, v in (("hop.block", block), ("hop.occupancy", occ), ("hop.zplanes", zp), ("hop.xcd", xcd)):
                         ctx.check(ctx.lib.cl2_tune_set(ctx.handle, key.encode(), v))
