@@ -95,6 +95,8 @@ SIGNATURES = {
     "cl2_fermionmatrix_eo": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _vp]),
     "cl2_fermionmatrix_eo_dot": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _dp]),
     "cl2_fermionmatrix_eo_cg_step": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, _vp, _dp, _dp, _dp, _dp]),
+    "cl2_fermionmatrix_eo_mrhs_cg_step": (_i, [_vp, _i, _vp, _vp, _vp, _i, _i, _d, _d, ctypes.POINTER(_d),
+                                               ctypes.POINTER(_d), _vp, _i, _dp, _dp, _dp, _dp, _dp]),
     "cl2_saxpy_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP]),
     "cl2_sax_eo": (_i, [_vp, _vp, _vp, _i, _SP]),
     "cl2_saxsbypz_eo": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _SP, _SP]),

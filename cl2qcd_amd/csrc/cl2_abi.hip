@@ -220,13 +220,25 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.rb            = a.per_rhs ? 1 : t.rb;  // per-right-hand-side operators: one right-hand side per thread
     a.lds           = t.lds;
     if (nrhs > 1 && t.lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
-    CL2_REQUIRE(c, (norm_res == nullptr && cg_r == nullptr) || nrhs == 1, "output norms are for single fields");
     CL2_REQUIRE(c, cg_r == nullptr || (cg_alpha != nullptr && norm_res != nullptr), "CG update without its scalars");
+    // per-right-hand-side norms come from the kernel with per-site staging (one right-hand side per thread)
+    const bool mrhs_norms = nrhs > 1 && norm_res != nullptr;
+    if (mrhs_norms) {
+        a.rb  = 1;
+        a.lds = 1;
+        a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
+    }
     CL2_HIP(c, launch_dslash(c, a, fmt));
     // every single-field variant launches blocks_for(Veo, block) blocks, each writing one partial: |out|^2 as a
-    // complex (norm, 0), or with the CG update (|r_new|^2, Re <r_new, r_old>, |r_new|^2, 0)
-    if (norm_res)
+    // complex (norm, 0), or with the CG update (|r_new|^2, Re <r_new, r_old>, |r_new|^2, 0); the staged
+    // multi-RHS kernel writes (|out|^2 or |r_new|^2, 0 or Re <r_new, r_old>) per right-hand side and block
+    if (norm_res && !mrhs_norms)
         CL2_HIP(c, launch_reduce_stage2(c, blocks_for(a.g.veo, a.block), norm_res, cg_r ? 3 : 1, nullptr, 0));
+    if (mrhs_norms) {
+        const int sb = mrhs_sites_per_block(nrhs);
+        CL2_HIP(c, launch_block_reduce2(c, reinterpret_cast<const double2*>(c->partials),
+                                        (unsigned)((a.g.veo + sb - 1) / sb), norm_res, 0, nrhs));
+    }
     return CL2_OK;
 }
 
@@ -289,7 +301,8 @@ int run_aee(cl2_ctx* c, void* out, const void* in, void* tmp, const void* gauge,
         }
         rc = run_dslash(c, tmp, in, nullptr, gauge, fmt, CL2_ODD, 1., EPI_MINV, one, one, 1., 0, nrhs, &p1);
         if (rc) return rc;
-        return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, 1., EPI_AEE, one, one, 1., gamma5 ? 1 : 0, nrhs, &p2);
+        return run_dslash(c, out, tmp, in, gauge, fmt, CL2_EVEN, 1., EPI_AEE, one, one, 1., gamma5 ? 1 : 0, nrhs, &p2,
+                          norm_res, cg_r, cg_alpha);
     }
     if (action == CL2_ACTION_WILSON) {
         const double one[2] = {1., 0.};
@@ -381,8 +394,10 @@ int cl2_ctx_create(cl2_ctx** out, int device, const cl2_lattice* lat)
     c->cp_bwd[0] = mer * cr;
     c->cp_bwd[1] = mer * ci;
     // one double2 per block: 16384 blocks for each of up to 16 right-hand sides (block kernels), BLAS_MAX_BLOCKS
-    // for an elementwise kernel, or one per block of the smallest hopping-kernel block (128 threads)
-    c->npartials = std::max<size_t>({(size_t)16384 * 16, (size_t)BLAS_MAX_BLOCKS, blocks_for(c->g.veo, 128)});
+    // for an elementwise kernel, one per block of the smallest hopping-kernel block (128 threads), or nrhs per
+    // block of the staged multi-RHS kernel (nrhs x ceil(Veo / sites per block) <= Veo + 16)
+    c->npartials = std::max<size_t>({(size_t)16384 * 16, (size_t)BLAS_MAX_BLOCKS, blocks_for(c->g.veo, 128),
+                                     (size_t)c->g.veo + 64});
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->partials, c->npartials * 2 * sizeof(double)) != hipSuccess ||
         hipMalloc(&c->dscratch, 64 * sizeof(double)) != hipSuccess ||
@@ -837,6 +852,32 @@ int cl2_fermionmatrix_eo_cg_step(cl2_ctx* c, int op, void* r, const void* p, con
         return rc;
     CL2_HIP(c, launch_scalar_op(c, CL2_SCALAR_OP_RATIO, 1, alpha, r2, pAp));
     return run_aee(c, nullptr, t1, t0, gauge, fmt, action, kappa, mubar, false, true, 1, nullptr, nullptr, norms, r,
+                   alpha);
+}
+
+int cl2_fermionmatrix_eo_mrhs_cg_step(cl2_ctx* c, int op, void* r, const void* p, const void* gauge, int fmt,
+                                      int action, double kappa, double mubar, const double* kappas,
+                                      const double* mubars, void* tmp, int nrhs, const double* r2,
+                                      const double* active, double* pAp, double* alpha, double* norms)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, op == CL2_OP_QPLUSQMINUS_EO, "the fused CG step is for QplusQminus_eo");
+    CL2_REQUIRE(c, action == CL2_ACTION_WILSON || action == CL2_ACTION_TWISTEDMASS,
+                "unknown fermion action: wilson or twistedmass");
+    CL2_REQUIRE(c, valid_fmt(fmt) && fmt != CL2_GAUGE_F16_12, "unknown gauge format (FP64/FP32 links)");
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1, "unsupported number of right-hand sides (2, 4, 8, 12 or 16)");
+    CL2_REQUIRE(c, r && p && gauge && tmp && r2 && active && pAp && alpha && norms, "null pointer");
+    CL2_REQUIRE(c, (kappas == nullptr) == (mubars == nullptr) || action == CL2_ACTION_WILSON,
+                "kappa and mubar per right-hand side come together");
+    CL2_REQUIRE(c, r != p, "r and p must be different blocks");
+    const size_t fb = spinor_bytes(c, fmt_prec(fmt)) * (size_t)nrhs;
+    void* t0        = tmp;
+    void* t1        = static_cast<char*>(tmp) + fb;
+    CL2_REQUIRE(c, r != t0 && r != t1 && p != t0 && p != t1, "tmp must not alias r or p");
+    if (int rc = run_aee(c, t1, p, t0, gauge, fmt, action, kappa, mubar, true, true, nrhs, kappas, mubars, pAp))
+        return rc;
+    CL2_HIP(c, launch_block_cg_scalars(c, 0, alpha, const_cast<double*>(r2), pAp, active, nrhs));
+    return run_aee(c, nullptr, t1, t0, gauge, fmt, action, kappa, mubar, false, true, nrhs, kappas, mubars, norms, r,
                    alpha);
 }
 

@@ -345,9 +345,10 @@ static BlockShape block_shape(cl2_ctx* c, size_t n, int prec, int nrhs, int u, b
 
 constexpr int BLOCK_U_REDUCE = 4;
 
-static void block_reduce2(cl2_ctx* c, const double2* parts, unsigned nb, double* res, int out_kind, int nrhs)
+hipError_t launch_block_reduce2(cl2_ctx* c, const double2* parts, unsigned nb, double* res, int out_kind, int nrhs)
 {
     hipLaunchKernelGGL((k_block_reduce2<1024>), dim3(nrhs), dim3(1024), 0, c->stream, parts, nb, res, out_kind);
+    return hipGetLastError();
 }
 
 hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,
@@ -381,8 +382,7 @@ hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !norms) return e;
-    block_reduce2(c, parts, b.nb, norms, 0, nrhs);
-    return hipGetLastError();
+    return launch_block_reduce2(c, parts, b.nb, norms, 0, nrhs);
 }
 
 hipError_t launch_block_xp(cl2_ctx* c, void* x, void* p, const void* r, int prec, int nrhs, const double* alpha,
@@ -418,8 +418,7 @@ hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* 
                            b.nchunk, nrhs, b.ncol, mode, parts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    block_reduce2(c, parts, b.nb, res, mode == 1 ? 1 : 0, nrhs);
-    return hipGetLastError();
+    return launch_block_reduce2(c, parts, b.nb, res, mode == 1 ? 1 : 0, nrhs);
 }
 
 hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, void* single_out, int prec, int nrhs,

@@ -139,6 +139,9 @@ hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, i
 hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,
                              const double* coef, double sign, double* norms);
 hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode);
+/* stage 2 of the per-right-hand-side reductions: block r sums partials[r * nb .. r * nb + nb) in order;
+ * out_kind 0: res[2r..2r+1] = complex sum, 1: res[r] = real part */
+hipError_t launch_block_reduce2(cl2_ctx* c, const double2* parts, unsigned nb, double* res, int out_kind, int nrhs);
 hipError_t launch_block_xp(cl2_ctx* c, void* x, void* p, const void* r, int prec, int nrhs, const double* alpha,
                            const double* beta);
 hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, void* single_out, int prec, int nrhs,

@@ -171,7 +171,7 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
         }                                                                                                         \
         if (a.cg_r) {                                                                                             \
             V* r_ = reinterpret_cast<V*>(a.cg_r);                                                                 \
-            const Cx<R> al_{R(a.cg_alpha[0]), R(a.cg_alpha[1])};                                                  \
+            const Cx<R> al_{R(a.cg_alpha[2 * rhs]), R(a.cg_alpha[2 * rhs + 1])};                                  \
             _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                      \
                 const V rv     = load_stream(r_ + (size_t)k * SS + o_);                                           \
                 const Cx<R> rn = Cx<R>{rv.x, rv.y} - cmul(acc[k], al_);                                           \
@@ -269,6 +269,7 @@ __global__ __launch_bounds__(MrhsLds<NRHS>::BLOCK) void k_dslash_eo_mrhs_lds(Dsl
     __shared__ Su3<R> lu[SB * 8];     // [site][hop], hop = 2 dir + backward
     __shared__ uint32_t lnb[SB * 8];  // neighbour eo index of each hop
     __shared__ uint32_t lsite[SB];    // eo index of the site (NONE past the end of the lattice)
+    __shared__ double2 lred[BLOCK];   // per-thread (|out|^2, Re <r_new, r_old>) for the per-RHS block sums
     const uint32_t S  = a.g.veo;
     const size_t SS   = (size_t)S * NRHS;
     const uint32_t Lb = logical_block(a.map, blockIdx.x);
@@ -303,7 +304,8 @@ __global__ __launch_bounds__(MrhsLds<NRHS>::BLOCK) void k_dslash_eo_mrhs_lds(Dsl
     const int s      = threadIdx.x / NRHS;
     const int rhs    = threadIdx.x - s * NRHS;
     const uint32_t i = lsite[s];
-    if (i == NONE) return;
+    double nrm = 0., ry = 0.;
+    if (i != NONE) {
     const V* in = reinterpret_cast<const V*>(a.in);
     Cx<R> acc[12];
 #pragma unroll
@@ -325,10 +327,22 @@ __global__ __launch_bounds__(MrhsLds<NRHS>::BLOCK) void k_dslash_eo_mrhs_lds(Dsl
     CL2_DIR(2)
     CL2_DIR(3)
 #undef CL2_DIR
-    double nrm = 0., ry = 0.;  // no per-block norms from this kernel
     CL2_FINISH_SITE((size_t)i * NRHS + rhs, nrm, ry)
-    (void)nrm;
-    (void)ry;
+    }
+    if (a.norm_partials) {
+        // per right-hand side: thread r < NRHS sums the SB site threads of its column in site order, into
+        // partials[r * gridDim.x + block] (double2)
+        lred[threadIdx.x] = make_double2(nrm, ry);
+        __syncthreads();
+        if ((int)threadIdx.x < NRHS) {
+            double2 t = make_double2(0., 0.);
+            for (int q = 0; q < SB; ++q) {
+                t.x += lred[q * NRHS + threadIdx.x].x;
+                t.y += lred[q * NRHS + threadIdx.x].y;
+            }
+            reinterpret_cast<double2*>(a.norm_partials)[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+        }
+    }
 }
 
 /* Register-blocked multi-RHS variant: one thread per (site, group of RB right-hand sides); each link is

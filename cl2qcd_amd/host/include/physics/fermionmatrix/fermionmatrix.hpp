@@ -133,6 +133,16 @@ void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block
 void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* out, const lattices::Gaugefield& gf,
                  const lattices::Spinorfield_eo_block& in, const std::vector<hmc_float>& kappa,
                  const std::vector<hmc_float>& mubar, const lattices::DeviceBuffer& tmp);
+/* The operator half of a lockstep CG step for A = QplusQminus_eo (false for any other operator, nothing
+ * done): pAp[r] = <p_r, A p_r>, alpha[r] = r2[r] / pAp[r] where active[r] != 0, r_r -= alpha[r] A p_r and
+ * norms[r] = (|r_new|^2, Re <r_new, r_old>), A p never stored.  kappa/mubar: one per right-hand side, or empty
+ * for ap's.  No counterpart in the reference. */
+bool cg_step_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* r, const lattices::Gaugefield& gf,
+                   const lattices::Spinorfield_eo_block& p, const physics::AdditionalParameters* ap,
+                   const std::vector<hmc_float>& kappa, const std::vector<hmc_float>& mubar,
+                   const lattices::DeviceBuffer& tmp, const lattices::Vector<hmc_float>& r2,
+                   const lattices::Vector<hmc_float>& active, const lattices::Vector<hmc_float>& pAp,
+                   const lattices::Vector<hmc_float>& alpha, const lattices::Vector<hmc_float>& norms);
 
 /* ---- staggered (fermionmatrix_stagg.hpp:49-152) ---- */
 class Fermionmatrix_stagg_eo : public Fermionmatrix_basic {

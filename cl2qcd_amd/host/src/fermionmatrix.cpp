@@ -270,6 +270,35 @@ void apply_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block
                                                            in.nrhs()));
 }
 
+bool cg_step_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_block* r, const Gaugefield& gf,
+                   const lattices::Spinorfield_eo_block& p, const physics::AdditionalParameters* ap,
+                   const std::vector<hmc_float>& kappa, const std::vector<hmc_float>& mubar,
+                   const lattices::DeviceBuffer& tmp, const lattices::Vector<hmc_float>& r2,
+                   const lattices::Vector<hmc_float>& active, const lattices::Vector<hmc_float>& pAp,
+                   const lattices::Vector<hmc_float>& alpha, const lattices::Vector<hmc_float>& norms)
+{
+    const int n = p.nrhs();
+    if (A.abi_operator() != CL2_OP_QPLUSQMINUS_EO || n < 2 || p.precision() == CL2_F16) return false;
+    if (r->precision() != p.precision() || r->nrhs() != n) throw std::invalid_argument("cg_step_block: blocks of different shape");
+    if (tmp.bytes() < 2 * p.bytes()) throw std::invalid_argument("cg_step_block: tmp must hold two blocks");
+    const bool per_rhs = !kappa.empty();
+    if (per_rhs && ((int)kappa.size() != n || (int)mubar.size() != n))
+        throw std::invalid_argument("cg_step_block: one kappa and one mubar per right-hand side");
+    if (!per_rhs && ap == nullptr) throw std::invalid_argument("cg_step_block: no operator parameters");
+    const common::action a = A.action();
+    if (a != common::wilson && a != common::twistedmass)
+        throw std::invalid_argument("Unkown fermion action! wilson or twistedmass");
+    const int abi_action  = a == common::twistedmass ? CL2_ACTION_TWISTEDMASS : CL2_ACTION_WILSON;
+    const hmc_float kap   = per_rhs ? 0. : ap->getKappa();
+    const hmc_float mubar0 = (!per_rhs && a == common::twistedmass) ? ap->getMubar() : 0.;
+    const Links l         = links_for(gf, p.precision());
+    p.get_system().check(cl2_fermionmatrix_eo_mrhs_cg_step(
+        p.get_system().ctx(), CL2_OP_QPLUSQMINUS_EO, r->dev(), p.dev(), l.ptr, l.fmt, abi_action, kap, mubar0,
+        per_rhs ? kappa.data() : nullptr, per_rhs ? mubar.data() : nullptr, tmp.ptr(), n, r2.dev(), active.dev(),
+        pAp.dev(), alpha.dev(), norms.dev()));
+    return true;
+}
+
 /* ---------------- staggered ---------------- */
 
 hmc_float Fermionmatrix_stagg_eo::getThresholdForMinimumEigenvalue(hmc_float) const

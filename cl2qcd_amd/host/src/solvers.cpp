@@ -447,9 +447,14 @@ namespace solvers {
 namespace {
 /* out = A in for a block (tmp: two blocks of in's shape) */
 using BlockApply = std::function<void(const Spinorfield_eo_block*, const Spinorfield_eo_block&, const DeviceBuffer&)>;
+/* the operator half of a lockstep step (r, p, tmp, r2, active, pAp, alpha, norms) where the operator has one
+ * (an empty function where not) */
+using BlockStep = std::function<bool(const Spinorfield_eo_block*, const Spinorfield_eo_block&, const DeviceBuffer&,
+                                     const Vector<hmc_float>&, const Vector<hmc_float>&, const Vector<hmc_float>&,
+                                     const Vector<hmc_float>&, const Vector<hmc_float>&)>;
 
 int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<const Spinorfield_eo*>& b,
-                const hardware::System& system, hmc_float rel_tol, const BlockApply& apply,
+                const hardware::System& system, hmc_float rel_tol, const BlockApply& apply, const BlockStep& step,
                 const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
 {
     const int n = (int)b.size();
@@ -520,12 +525,19 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
     int k           = 0;
     int reliable    = 0;
     apply(&V, P, tmplo);  // reduced-precision links made before recording
+    // whether the operator does the step's operator half itself (QplusQminus_eo: no A p stored, no dot pass)
+    const bool fused = static_cast<bool>(step) && n > 1;
     // one lockstep iteration, every scalar on the device
     auto body = [&]() {
-        apply(&V, P, tmplo);
-        system.check(cl2_block_dot_eo(c, pAp.dev(), P.dev(), V.dev(), lo, n));
-        system.check(cl2_block_cg_alpha(c, alpha.dev(), r2.dev(), pAp.dev(), active.dev(), n));
-        system.check(cl2_block_axpy_eo(c, Rlo.dev(), V.dev(), Rlo.dev(), lo, n, alpha.dev(), -1., norms.dev()));
+        if (!fused) {
+            apply(&V, P, tmplo);
+            system.check(cl2_block_dot_eo(c, pAp.dev(), P.dev(), V.dev(), lo, n));
+            system.check(cl2_block_cg_alpha(c, alpha.dev(), r2.dev(), pAp.dev(), active.dev(), n));
+            system.check(cl2_block_axpy_eo(c, Rlo.dev(), V.dev(), Rlo.dev(), lo, n, alpha.dev(), -1., norms.dev()));
+        } else {
+            if (!step(&Rlo, P, tmplo, r2, active, pAp, alpha, norms))
+                throw std::logic_error("cg_reliable_update_block: the operator has no fused CG step");
+        }
         system.check(cl2_block_cg_beta(c, beta.dev(), r2.dev(), norms.dev(), active.dev(), n));
         // x += alpha p and p = r + beta p in one pass
         system.check(cl2_block_cg_update_xp_eo(c, Xlo.dev(), P.dev(), Rlo.dev(), lo, n, alpha.dev(), beta.dev()));
@@ -612,12 +624,19 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
                              const hardware::System& system, hmc_float rel_tol, const physics::AdditionalParameters& ap,
                              const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
 {
+    BlockStep step;
+    if (A.abi_operator() == CL2_OP_QPLUSQMINUS_EO)
+        step = [&](const Spinorfield_eo_block* r, const Spinorfield_eo_block& p, const DeviceBuffer& tmp,
+                   const Vector<hmc_float>& r2, const Vector<hmc_float>& active, const Vector<hmc_float>& pAp,
+                   const Vector<hmc_float>& alpha, const Vector<hmc_float>& norms) {
+            return fermionmatrix::cg_step_block(A, r, gf, p, &ap, {}, {}, tmp, r2, active, pAp, alpha, norms);
+        };
     return block_solve(
         x, b, system, rel_tol,
         [&](const Spinorfield_eo_block* out, const Spinorfield_eo_block& in, const DeviceBuffer& tmp) {
             fermionmatrix::apply_block(A, out, gf, in, ap, tmp);
         },
-        opt, stats);
+        step, opt, stats);
 }
 
 int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const fermionmatrix::Fermionmatrix_eo& A,
@@ -637,12 +656,20 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
         uniform = uniform && kappa.back() == kappa[0] && mubar.back() == mubar[0];
     }
     if (uniform) return cg_reliable_update_block(x, A, gf, b, system, rel_tol, *aps[0], opt, stats);
+    BlockStep step;
+    if (A.abi_operator() == CL2_OP_QPLUSQMINUS_EO)
+        step = [&](const Spinorfield_eo_block* r, const Spinorfield_eo_block& p, const DeviceBuffer& tmp,
+                   const Vector<hmc_float>& r2, const Vector<hmc_float>& active, const Vector<hmc_float>& pAp,
+                   const Vector<hmc_float>& alpha, const Vector<hmc_float>& norms) {
+            return fermionmatrix::cg_step_block(A, r, gf, p, nullptr, kappa, mubar, tmp, r2, active, pAp, alpha,
+                                                norms);
+        };
     return block_solve(
         x, b, system, rel_tol,
         [&](const Spinorfield_eo_block* out, const Spinorfield_eo_block& in, const DeviceBuffer& tmp) {
             fermionmatrix::apply_block(A, out, gf, in, kappa, mubar, tmp);
         },
-        opt, stats);
+        step, opt, stats);
 }
 
 }  // namespace solvers

@@ -267,6 +267,18 @@ int cl2_fermionmatrix_eo_mrhs(cl2_ctx* ctx, int op, void* out, const void* in, c
 int cl2_fermionmatrix_eo_mrhs_params(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge,
                                      int format, int action, const double* kappa, const double* mubar, void* tmp,
                                      int nrhs);
+/* The operator half of a lockstep CG step on nrhs right-hand sides, QplusQminus_eo only (op must be
+ * CL2_OP_QPLUSQMINUS_EO), as cl2_fermionmatrix_eo_cg_step per right-hand side: pAp[r] (complex, 2 doubles
+ * each) = |Qminus_eo p_r|^2, alpha[r] = r2[r] / pAp[r] where active[r] != 0 (else 0, as cl2_block_cg_alpha),
+ * r_r <- r_r - alpha[r] A p_r, norms[r] (2 doubles) = (|r_new|^2, Re <r_new, r_old>) as cl2_block_axpy_eo
+ * gives them.  Replaces, for the block solver, the block operator call, cl2_block_dot_eo, cl2_block_cg_alpha
+ * and the residual cl2_block_axpy_eo; A p is never stored.  kappas/mubars: host arrays of nrhs values for an
+ * operator per right-hand side, or null for kappa/mubar.  FP64/FP32 links; r2, active, pAp, alpha, norms
+ * are device arrays; tmp: 2 blocks. */
+int cl2_fermionmatrix_eo_mrhs_cg_step(cl2_ctx* ctx, int op, void* r, const void* p, const void* gauge, int format,
+                                      int action, double kappa, double mubar, const double* kappas,
+                                      const double* mubars, void* tmp, int nrhs, const double* r2,
+                                      const double* active, double* pAp, double* alpha, double* norms);
 int cl2_dslash_eo_mrhs(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, int evenodd,
                        double kappa, int nrhs);
 /* copy a single field into / out of slot r of a block (nrhs in [1, 64]) */

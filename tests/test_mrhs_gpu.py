@@ -248,3 +248,65 @@ def test_block_cg_per_rhs_params():
     x1.zero()
     H.cg_reliable_update(x1, A, gf, bs[1], l, 1e-10, H.WilsonAdditionalParameters(kappas[1], mubars[1]))
     assert np.sqrt(((xs[1].get_host() - x1.get_host()) ** 2).sum() / (x1.get_host() ** 2).sum()) < 1e-6
+
+
+@pytest.mark.parametrize("per_rhs", [False, True])
+@pytest.mark.parametrize("nrhs", [4, 12])
+@pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12])
+def test_mrhs_cg_step(fmt, nrhs, per_rhs):
+    """The fused lockstep CG step (pAp per right-hand side from |Qminus p_r|^2, alpha, r_r -= alpha_r A p_r
+    with its norms, A p never stored) against the unfused block sequence (operator, block dot, alpha, block
+    axpy with norms): FP64 to 1e-12, FP32 to 1e-5 (pAp is summed from another field, so alpha and everything
+    after it differ by rounding); an inactive right-hand side keeps r and gets alpha 0; FP64 against the oracle."""
+    import ctypes
+    lat = O.lattice(8, 8, theta_t=1.0)
+    prec = abi.F32 if fmt == abi.GAUGE_F32_12 else abi.F64
+    tol = 1e-12 if prec == abi.F64 else 1e-5
+    rng = np.random.default_rng(nrhs)
+    kappas = list(0.15 + 0.01 * rng.random(nrhs)) if per_rhs else [0.16] * nrhs
+    mubars = [2 * k * m for k, m in zip(kappas, 0.003 + 0.017 * rng.random(nrhs))] if per_rhs else \
+        [2 * 0.16 * 0.005] * nrhs
+    with abi.Context(8, 8, theta_t=1.0) as ctx:
+        g = random_su3_field(4 * O.volume(lat), 14)
+        gf = ctx.gauge(g, fmt)
+        P = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
+        R = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
+        p, r1, r2b = make_block(ctx, P, prec), make_block(ctx, R, prec), make_block(ctx, R, prec)
+        v, tmp = ctx.buffer(p.nbytes), ctx.buffer(2 * p.nbytes)
+        n2 = 2 * nrhs
+        sc = ctx.scalars(14 * nrhs)  # r2 | active | fused pAp, alpha, norms | unfused pAp, alpha, norms
+        r2 = [O.norm2(x) for x in R]
+        active = [1.0] * nrhs
+        active[1] = 0.0
+        sc.store(r2 + active + [0.0] * (12 * nrhs))
+        at = lambda k: sc.at(2 * nrhs + k)
+        ka, ma = (ctypes.c_double * nrhs)(*kappas), (ctypes.c_double * nrhs)(*mubars)
+        if per_rhs:
+            ctx.fermionmatrix_eo_mrhs_params(abi.OP_QPLUSQMINUS_EO, v.ptr, p.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS,
+                                             ka, ma, tmp.ptr, nrhs)
+        else:
+            ctx.fermionmatrix_eo_mrhs(abi.OP_QPLUSQMINUS_EO, v.ptr, p.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS,
+                                      kappas[0], mubars[0], tmp.ptr, nrhs)
+        ctx.block_dot_eo(at(3 * n2), p.ptr, v.ptr, prec, nrhs)
+        ctx.block_cg_alpha(at(4 * n2), sc.at(0), at(3 * n2), sc.at(nrhs), nrhs)
+        ctx.block_axpy_eo(r1.ptr, v.ptr, r1.ptr, prec, nrhs, at(4 * n2), -1.0, at(5 * n2))
+        ctx.fermionmatrix_eo_mrhs_cg_step(abi.OP_QPLUSQMINUS_EO, r2b.ptr, p.ptr, gf.ptr, fmt, abi.ACTION_TWISTEDMASS,
+                                          kappas[0], mubars[0], ka if per_rhs else None, ma if per_rhs else None,
+                                          tmp.ptr, nrhs, sc.at(0), sc.at(nrhs), at(0), at(n2), at(2 * n2))
+        d = sc.get()[2 * nrhs:]
+        fp, fa, fn = d[0:n2], d[n2:2 * n2], d[2 * n2:3 * n2]
+        up, ua, un = d[3 * n2:4 * n2], d[4 * n2:5 * n2], d[5 * n2:6 * n2]
+        for k in range(nrhs):
+            assert abs(fp[2 * k] - up[2 * k]) < tol * up[2 * k] and fp[2 * k + 1] == 0.0
+            got, ref = block_slot(ctx, r2b, prec, nrhs, k), block_slot(ctx, r1, prec, nrhs, k)
+            if k == 1:
+                assert fa[2] == 0.0 and fa[3] == 0.0 and np.array_equal(got, ref)
+                continue
+            assert abs(complex(fa[2 * k], fa[2 * k + 1]) - complex(ua[2 * k], ua[2 * k + 1])) < tol * abs(ua[2 * k])
+            assert site_rel_err(got, ref) < 10 * tol
+            assert abs(fn[2 * k] - un[2 * k]) < 10 * tol * un[2 * k]
+            assert abs(fn[2 * k + 1] - un[2 * k + 1]) < 10 * tol * un[2 * k]
+            if prec == abi.F64:
+                Ap = O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, P[k], g, O.TWISTEDMASS, kappas[k], mubars[k])
+                alpha = r2[k] / O.dot(P[k], Ap)
+                assert site_rel_err(got, O.saxpy(Ap, R[k], alpha)) < 1e-12
