@@ -114,6 +114,16 @@ SIGNATURES = {
     "cl2_block_axpy_eo": (_i, [_vp, _vp, _vp, _vp, _i, _i, _dp, _d, _dp]),
     "cl2_block_dot_eo": (_i, [_vp, _dp, _vp, _vp, _i, _i]),
     "cl2_block_cg_update_xp_eo": (_i, [_vp, _vp, _vp, _vp, _i, _i, _dp, _dp]),
+    "cl2_D_KS_eo_mrhs": (_i, [_vp, _vp, _vp, _vp, _i, _i, _i]),
+    "cl2_MdagM_eo_mrhs": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, ctypes.POINTER(_d), _vp, _i]),
+    "cl2_block_insert_stagg_eo": (_i, [_vp, _vp, _vp, _i, _i, _i]),
+    "cl2_block_extract_stagg_eo": (_i, [_vp, _vp, _vp, _i, _i, _i]),
+    "cl2_block_axpy_stagg_eo": (_i, [_vp, _vp, _vp, _vp, _i, _i, _dp, _d, _dp]),
+    "cl2_block_dot_stagg_eo": (_i, [_vp, _dp, _vp, _vp, _i, _i]),
+    "cl2_block_squarenorm_stagg_eo": (_i, [_vp, _dp, _vp, _i, _i]),
+    "cl2_block_convert_stagg_eo": (_i, [_vp, _vp, _i, _vp, _i, _i]),
+    "cl2_block_set_zero_stagg_eo": (_i, [_vp, _vp, _i, _i]),
+    "cl2_block_cg_update_xp_stagg_eo": (_i, [_vp, _vp, _vp, _vp, _i, _i, _dp, _dp]),
     "cl2_block_squarenorm_eo": (_i, [_vp, _dp, _vp, _i, _i]),
     "cl2_block_convert_eo": (_i, [_vp, _vp, _i, _vp, _i, _i]),
     "cl2_block_set_zero_eo": (_i, [_vp, _vp, _i, _i]),

@@ -1126,6 +1126,116 @@ int cl2_block_set_zero_eo(cl2_ctx* c, void* out, int prec, int nrhs)
     return CL2_OK;
 }
 
+/* ---- blocks of staggered vectors [3][Veo][nrhs] ---- */
+int cl2_block_insert_stagg_eo(cl2_ctx* c, void* block, const void* single, int prec, int nrhs, int r)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, block && single && r >= 0 && r < nrhs, "bad argument");
+    CL2_HIP(c, launch_block_slot(c, block, single, nullptr, prec, nrhs, r, 3));
+    return CL2_OK;
+}
+
+int cl2_block_extract_stagg_eo(cl2_ctx* c, void* single, const void* block, int prec, int nrhs, int r)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, block && single && r >= 0 && r < nrhs, "bad argument");
+    CL2_HIP(c, launch_block_slot(c, const_cast<void*>(block), nullptr, single, prec, nrhs, r, 3));
+    return CL2_OK;
+}
+
+int cl2_block_axpy_stagg_eo(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,
+                            const double* coef, double sign, double* norms)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, out && x && y && coef, "null pointer");
+    CL2_HIP(c, launch_block_axpy(c, out, x, y, prec, nrhs, coef, sign, norms, 3));
+    return CL2_OK;
+}
+
+int cl2_block_dot_stagg_eo(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, res && x && y, "null pointer");
+    CL2_HIP(c, launch_block_dot(c, res, x, y, prec, nrhs, 0, 3));
+    return CL2_OK;
+}
+
+int cl2_block_squarenorm_stagg_eo(cl2_ctx* c, double* res, const void* x, int prec, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, res && x, "null pointer");
+    CL2_HIP(c, launch_block_dot(c, res, x, x, prec, nrhs, 1, 3));
+    return CL2_OK;
+}
+
+int cl2_block_convert_stagg_eo(cl2_ctx* c, void* dst, int dst_prec, const void* src, int src_prec, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(dst_prec, nrhs);
+    CL2_REQUIRE(c, valid_prec(src_prec), "block fields are FP64 or FP32");
+    CL2_REQUIRE(c, dst && src && dst != src, "bad pointers");
+    const size_t n = (size_t)3 * c->g.veo * nrhs;
+    if (dst_prec == src_prec)
+        CL2_HIP(c, hipMemcpyAsync(dst, src, n * (dst_prec == CL2_F64 ? 16 : 8), hipMemcpyDeviceToDevice, c->stream));
+    else
+        CL2_HIP(c, launch_convert(c, dst, dst_prec, src, src_prec, n));
+    return CL2_OK;
+}
+
+int cl2_block_set_zero_stagg_eo(cl2_ctx* c, void* out, int prec, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, out != nullptr, "null pointer");
+    CL2_HIP(c, launch_fill(c, out, (size_t)3 * c->g.veo * nrhs, prec, 0., 0.));
+    return CL2_OK;
+}
+
+int cl2_block_cg_update_xp_stagg_eo(cl2_ctx* c, void* x, void* p, const void* r, int prec, int nrhs,
+                                    const double* alpha, const double* beta)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_REQUIRE(c, x && p && r && alpha && beta, "null pointer");
+    CL2_REQUIRE(c, x != p && x != r && p != r, "x, p and r must be different blocks");
+    CL2_HIP(c, launch_block_xp(c, x, p, r, prec, nrhs, alpha, beta, 3));
+    return CL2_OK;
+}
+
+int cl2_D_KS_eo_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, int evenodd, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, valid_fmt(fmt) && fmt != CL2_GAUGE_F16_12, "unknown gauge format (FP64/FP32 links)");
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1, "unsupported number of right-hand sides (2, 4, 8, 12 or 16)");
+    CL2_REQUIRE(c, evenodd == CL2_EVEN || evenodd == CL2_ODD, "evenodd must be CL2_EVEN or CL2_ODD");
+    CL2_REQUIRE(c, out && in && gauge, "null field pointer");
+    CL2_REQUIRE(c, out != in, "D_KS_eo cannot run in place");
+    CL2_HIP(c, launch_dks_mrhs(c, out, in, gauge, fmt, evenodd, nrhs, nullptr, nullptr));
+    return CL2_OK;
+}
+
+int cl2_MdagM_eo_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, double mass, int upper_left,
+                      const double* sigma, void* tmp, int nrhs)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, valid_fmt(fmt) && fmt != CL2_GAUGE_F16_12, "unknown gauge format (FP64/FP32 links)");
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1, "unsupported number of right-hand sides (2, 4, 8, 12 or 16)");
+    CL2_REQUIRE(c, upper_left == CL2_EVEN || upper_left == CL2_ODD, "upper_left must be CL2_EVEN or CL2_ODD");
+    CL2_REQUIRE(c, out && in && gauge && tmp, "null field pointer");
+    CL2_REQUIRE(c, out != in && tmp != in && tmp != out, "out, in and tmp must be different blocks");
+    double coef[16];
+    for (int r = 0; r < nrhs; ++r) coef[r] = mass * mass + (sigma ? sigma[r] : 0.);
+    const int first = (upper_left == CL2_EVEN) ? CL2_ODD : CL2_EVEN;
+    CL2_HIP(c, launch_dks_mrhs(c, tmp, in, gauge, fmt, first, nrhs, nullptr, nullptr));
+    CL2_HIP(c, launch_dks_mrhs(c, out, tmp, gauge, fmt, upper_left, nrhs, in, coef));
+    return CL2_OK;
+}
+
 int cl2_block_cg_alpha(cl2_ctx* c, double* alpha, const double* r2, const double* pAp, const double* active,
                        int nrhs)
 {

@@ -352,9 +352,9 @@ hipError_t launch_block_reduce2(cl2_ctx* c, const double2* parts, unsigned nb, d
 }
 
 hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,
-                             const double* coef, double sign, double* norms)
+                             const double* coef, double sign, double* norms, int ncomp)
 {
-    const size_t n = (size_t)12 * c->g.veo * nrhs;
+    const size_t n = (size_t)ncomp * c->g.veo * nrhs;
     const int u    = norms ? BLOCK_U_REDUCE : 1;
     const BlockShape b = block_shape(c, n, prec, nrhs, u, norms != nullptr, {out, x, y});
     double2* parts     = norms ? reinterpret_cast<double2*>(c->partials) : nullptr;
@@ -386,9 +386,9 @@ hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y
 }
 
 hipError_t launch_block_xp(cl2_ctx* c, void* x, void* p, const void* r, int prec, int nrhs, const double* alpha,
-                           const double* beta)
+                           const double* beta, int ncomp)
 {
-    const size_t n     = (size_t)12 * c->g.veo * nrhs;
+    const size_t n     = (size_t)ncomp * c->g.veo * nrhs;
     const BlockShape b = block_shape(c, n, prec, nrhs, 1, false, {x, p, r});
     if (prec == CL2_F64)
         hipLaunchKernelGGL((k_block_xp<double, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, p, r, b.nchunk, nrhs,
@@ -402,9 +402,10 @@ hipError_t launch_block_xp(cl2_ctx* c, void* x, void* p, const void* r, int prec
     return hipGetLastError();
 }
 
-hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode)
+hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode,
+                            int ncomp)
 {
-    const size_t n     = (size_t)12 * c->g.veo * nrhs;
+    const size_t n     = (size_t)ncomp * c->g.veo * nrhs;
     const BlockShape b = block_shape(c, n, prec, nrhs, BLOCK_U_REDUCE, true, {x, y});
     double2* parts     = reinterpret_cast<double2*>(c->partials);
     if (prec == CL2_F64)
@@ -422,9 +423,9 @@ hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* 
 }
 
 hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, void* single_out, int prec, int nrhs,
-                             int r)
+                             int r, int ncomp)
 {
-    const size_t n    = (size_t)12 * c->g.veo;
+    const size_t n    = (size_t)ncomp * c->g.veo;
     const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
     if (prec == CL2_F64)
         hipLaunchKernelGGL(k_block_slot<double>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<double2*>(block),

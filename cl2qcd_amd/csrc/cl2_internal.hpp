@@ -79,6 +79,9 @@ hipError_t launch_dslash(cl2_ctx* c, const DslashArgs& a, int format);
 hipError_t launch_sitediag(cl2_ctx* c, void* out, const void* in, int prec, const double f1[2], const double f2[2],
                            double scale);
 hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd);
+/* nrhs = 2, 4, 8, 12, 16 staggered vectors [3][Veo][nrhs]; with x: out_r = coef[r] x_r - D in_r */
+hipError_t launch_dks_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd, int nrhs,
+                           const void* x, const double* coef);
 
 /* elementwise complex BLAS over n complex numbers */
 enum BlasMode : int {
@@ -135,17 +138,18 @@ hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im);
 hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec);
 hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n);
 
-/* blocks of nrhs Wilson spinors [12][Veo][nrhs] */
+/* blocks of nrhs eo fields [ncomp][Veo][nrhs]: Wilson spinors (ncomp 12) or staggered vectors (3) */
 hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,
-                             const double* coef, double sign, double* norms);
-hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode);
+                             const double* coef, double sign, double* norms, int ncomp = 12);
+hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs, int mode,
+                            int ncomp = 12);
 /* stage 2 of the per-right-hand-side reductions: block r sums partials[r * nb .. r * nb + nb) in order;
  * out_kind 0: res[2r..2r+1] = complex sum, 1: res[r] = real part */
 hipError_t launch_block_reduce2(cl2_ctx* c, const double2* parts, unsigned nb, double* res, int out_kind, int nrhs);
 hipError_t launch_block_xp(cl2_ctx* c, void* x, void* p, const void* r, int prec, int nrhs, const double* alpha,
-                           const double* beta);
+                           const double* beta, int ncomp = 12);
 hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, void* single_out, int prec, int nrhs,
-                             int r);
+                             int r, int ncomp = 12);
 hipError_t launch_scalar_copy(cl2_ctx* c, double* dst, const double* src, int n);
 hipError_t launch_cg_beta(cl2_ctx* c, double* mbeta, double* r2, const double* norms);
 hipError_t launch_block_cg_scalars(cl2_ctx* c, int which, double* out, double* r2, const double* in,

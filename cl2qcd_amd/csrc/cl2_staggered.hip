@@ -16,6 +16,10 @@ struct DksArgs {
     int out_parity;
     double sf[4][2];  // 0.5 * b_mu (* imaginary chemical potential on T)
     WorkMap map;
+    // multi-RHS launches ([3][Veo][nrhs] fields): with x set, out_r = coef[r] x_r - D in_r (the last step of
+    // MdagM_eo with a shift per right-hand side)
+    const void* x;
+    double coef[16];
 };
 
 /* forward and backward hop of one direction on neighbour vectors already in registers:
@@ -122,17 +126,98 @@ void k_dks_eo(DksArgs a)
     for (int c = 0; c < 3; ++c) store_stream(out + (size_t)c * S + i, V{acc[c].re, acc[c].im});
 }
 
-hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd)
+/* Multi-RHS D_KS_eo with per-site staging, as the Wilson multi-RHS kernel: a block owns SB consecutive sites of
+ * the work order and NRHS threads per site (right-hand side fastest); SB threads compute the sites' neighbour
+ * indices and SB x 8 threads load and reconstruct their 8 links once into LDS, then every right-hand-side
+ * thread gathers its own neighbour vectors.  Per right-hand side the arithmetic and order are the single-field
+ * kernel's, so slot r equals it bit for bit. */
+template<int NRHS>
+struct DksMrhs {
+    static constexpr int SB    = NRHS >= 8 ? 16 : 32;
+    static constexpr int BLOCK = SB * NRHS;
+};
+
+template<int FMT, int NRHS>
+__global__ __launch_bounds__(DksMrhs<NRHS>::BLOCK) void k_dks_eo_mrhs(DksArgs a)
 {
-    DksArgs a;
-    a.in         = in;
-    a.out        = out;
-    a.gauge      = gauge;
-    a.g          = c->g;
-    a.out_parity = evenodd;
-    const cl2_hop_tuning t = hop_tuning(c, format | DKS_TUNING_KEY, 1);
-    const int bs           = (t.block == 128 || t.block == 512) ? t.block : 256;
-    a.map                  = work_map(c, bs, 1, t);
+    using R = typename GaugeLoad<FMT>::R;
+    using V = vec2<R>;
+    constexpr int SB        = DksMrhs<NRHS>::SB;
+    constexpr int BLOCK     = DksMrhs<NRHS>::BLOCK;
+    constexpr uint32_t NONE = 0xffffffffu;
+    __shared__ Su3<R> lu[SB * 8];
+    __shared__ uint32_t lnb[SB * 8];
+    __shared__ uint32_t lsite[SB];
+    const uint32_t S  = a.g.veo;
+    const size_t SS   = (size_t)S * NRHS;
+    const uint32_t Lb = logical_block(a.map, blockIdx.x);
+    const int P       = a.out_parity;
+    if (threadIdx.x < SB) {
+        const uint32_t l = Lb * SB + threadIdx.x;
+        uint32_t i       = NONE;
+        if (l < S) {
+            i = logical_site(a.map, l);
+            SiteNb nb;
+            site_neighbours(a.g, i, P, nb);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                lnb[threadIdx.x * 8 + 2 * d]     = nb.up[d];
+                lnb[threadIdx.x * 8 + 2 * d + 1] = nb.dn[d];
+            }
+        }
+        lsite[threadIdx.x] = i;
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < SB * 8; w += BLOCK) {
+        const uint32_t i = lsite[w >> 3];
+        if (i != NONE) {
+            const int h = w & 7;
+            Su3<R> u;
+            GaugeLoad<FMT>::load(a.gauge, S, h >> 1, (h & 1) ? P ^ 1 : P, (h & 1) ? lnb[w] : i, u);
+            lu[w] = u;
+        }
+    }
+    __syncthreads();
+    const int s      = threadIdx.x / NRHS;
+    const int rhs    = threadIdx.x - s * NRHS;
+    const uint32_t i = lsite[s];
+    if (i == NONE) return;
+    uint32_t x, y, z, t;
+    site_coords(a.g, i, P, x, y, z, t);
+    const R eta[4] = {R(((x + y + z) & 1u) ? -1 : 1), R(1), R((x & 1u) ? -1 : 1), R(((x + y) & 1u) ? -1 : 1)};
+    const V* in    = reinterpret_cast<const V*>(a.in);
+    Cx<R> acc[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc[k] = Cx<R>{R(0), R(0)};
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        V vf[3], vb[3];
+        const size_t ef = (size_t)lnb[s * 8 + 2 * mu] * NRHS + rhs, eb = (size_t)lnb[s * 8 + 2 * mu + 1] * NRHS + rhs;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vf[c] = in[(size_t)c * SS + ef];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) vb[c] = in[(size_t)c * SS + eb];
+        dks_hops(acc, lu[s * 8 + 2 * mu], vf, lu[s * 8 + 2 * mu + 1], vb,
+                 Cx<R>{R(a.sf[mu][0]) * eta[mu], R(a.sf[mu][1]) * eta[mu]});
+    }
+    const size_t o = (size_t)i * NRHS + rhs;
+    if (a.x) {  // the MdagM_eo epilogue as the elementwise kernel computes it: coef x - out
+        const V* xv     = reinterpret_cast<const V*>(a.x);
+        const Cx<R> cf  = Cx<R>{R(a.coef[rhs]), R(0)};
+        const Cx<R> mone = Cx<R>{R(-1), R(0)};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const V v = xv[(size_t)c * SS + o];
+            acc[c]    = cmul(Cx<R>{v.x, v.y}, cf) + cmul(acc[c], mone);
+        }
+    }
+    V* out = reinterpret_cast<V*>(a.out);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) store_stream(out + (size_t)c * SS + o, V{acc[c].re, acc[c].im});
+}
+
+static void dks_weights(cl2_ctx* c, DksArgs& a)
+{
     for (int mu = 0; mu < 4; ++mu) {
         const double* b = (mu == 0) ? c->bt : c->bs;
         double re = 0.5 * b[0], im = 0.5 * b[1];
@@ -145,6 +230,60 @@ hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, 
         a.sf[mu][0] = re;
         a.sf[mu][1] = im;
     }
+}
+
+hipError_t launch_dks_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd, int nrhs,
+                           const void* x, const double* coef)
+{
+    DksArgs a;
+    a.in         = in;
+    a.out        = out;
+    a.gauge      = gauge;
+    a.g          = c->g;
+    a.out_parity = evenodd;
+    a.x          = x;
+    for (int r = 0; r < 16; ++r) a.coef[r] = (x && r < nrhs) ? coef[r] : 0.;
+    dks_weights(c, a);
+    const cl2_hop_tuning t = hop_tuning(c, format | DKS_TUNING_KEY, 1);
+    auto go = [&](auto fmt_tag, auto n_tag) -> hipError_t {
+        constexpr int F = decltype(fmt_tag)::value, N = decltype(n_tag)::value;
+        constexpr int SB = DksMrhs<N>::SB;
+        a.map = work_map(c, SB, 1, t);
+        hipLaunchKernelGGL((k_dks_eo_mrhs<F, N>), dim3((c->g.veo + SB - 1) / SB), dim3(DksMrhs<N>::BLOCK), 0, c->stream,
+                           a);
+        return hipGetLastError();
+    };
+    auto by_n = [&](auto fmt_tag) -> hipError_t {
+        switch (nrhs) {
+            case 2: return go(fmt_tag, std::integral_constant<int, 2>{});
+            case 4: return go(fmt_tag, std::integral_constant<int, 4>{});
+            case 8: return go(fmt_tag, std::integral_constant<int, 8>{});
+            case 12: return go(fmt_tag, std::integral_constant<int, 12>{});
+            case 16: return go(fmt_tag, std::integral_constant<int, 16>{});
+            default: return hipErrorInvalidValue;
+        }
+    };
+    switch (format) {
+        case GAUGE_F64_18: return by_n(std::integral_constant<int, GAUGE_F64_18>{});
+        case GAUGE_F64_12: return by_n(std::integral_constant<int, GAUGE_F64_12>{});
+        case GAUGE_F32_12: return by_n(std::integral_constant<int, GAUGE_F32_12>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd)
+{
+    DksArgs a;
+    a.in         = in;
+    a.out        = out;
+    a.gauge      = gauge;
+    a.g          = c->g;
+    a.out_parity = evenodd;
+    a.x          = nullptr;
+    const cl2_hop_tuning t = hop_tuning(c, format | DKS_TUNING_KEY, 1);
+    const int bs           = (t.block == 128 || t.block == 512) ? t.block : 256;
+    a.map                  = work_map(c, bs, 1, t);
+    dks_weights(c, a);
     const unsigned nb = blocks_for(c->g.veo, bs);
     auto go = [&](auto fmt_tag, auto block_tag) {
         constexpr int F = decltype(fmt_tag)::value, B = decltype(block_tag)::value;

@@ -94,6 +94,20 @@ int cg_reliable_update_stagg(const lattices::Staggeredfield_eo* x, const fermion
                              const physics::AdditionalParameters& additionalParameters,
                              const ReliableUpdateOptions& options, SolveStats* stats = nullptr);
 
+/* The shifted staggered systems (A + sigma[r]) x[r] = b of one right-hand side b, A = MdagM_eo, any number of
+ * shifts, solved in lockstep blocks with the reliable-update CG of cg_reliable_update_block: every inner
+ * iteration applies A to all shifts of a block (2, 4, 8, 12 or 16) with one pass over the links per D_KS_eo,
+ * each shift keeps its own alpha and beta and is frozen when |b - (A + sigma[r]) x[r]| <= rel_tol |b| in FP64;
+ * once at most half of a block is unconverged, those shifts continue (restarted from their solutions) in the
+ * next smaller block, a last single one with cg_reliable_update_stagg.  x[r] (FP64) are initial guesses and
+ * results.  Returns the lockstep iterations summed over the blocks. */
+int cg_reliable_update_stagg_block(const std::vector<const lattices::Staggeredfield_eo*>& x,
+                                   const fermionmatrix::MdagM_eo& A, const lattices::Gaugefield& gf,
+                                   const std::vector<hmc_float>& sigma, const lattices::Staggeredfield_eo& b,
+                                   const hardware::System& system, hmc_float rel_tol,
+                                   const physics::AdditionalParameters& additionalParameters,
+                                   const ReliableUpdateOptions& options, std::vector<SolveStats>* stats = nullptr);
+
 struct MultishiftOptions {
     int inner_precision = CL2_F32;  // precision of the multi-shift phase and of the refinement's inner loop
     double sloppy_tol   = 1e-7;     // relative residual of each shift at which the multi-shift phase stops
@@ -104,9 +118,11 @@ struct MultishiftOptions {
 
 /* Multi-shift CG for (A + sigma_k) x_k = b in two phases: the reference's CG-M recurrences
  * (solver_shifted.cpp:114-475) on inner-precision fields with all shifts' vector updates fused into one
- * pass per iteration, until every shift's residual zeta_k r is below sloppy_tol |b|; then each x_k is
- * refined with cg_reliable_update_stagg to |b - (A + sigma_k) x_k| <= rel_tol |b| in FP64.  Returns the
- * multi-shift iterations plus all refinement iterations; stats[k] holds shift k's refinement. */
+ * pass per iteration, until every shift's residual zeta_k r is below sloppy_tol |b|; then the x_k are
+ * refined to |b - (A + sigma_k) x_k| <= rel_tol |b| in FP64: for A = MdagM_eo in lockstep blocks of up to 16
+ * shifts (cg_reliable_update_stagg_block), else one by one with cg_reliable_update_stagg.  Returns the
+ * multi-shift iterations plus the refinement's operator applications (lockstep iterations of a block count
+ * once); stats[k] holds shift k's refinement. */
 int cg_m_mixed(const std::vector<std::shared_ptr<lattices::Staggeredfield_eo>>& x,
                const fermionmatrix::Fermionmatrix_stagg_eo& A, const lattices::Gaugefield& gf,
                const std::vector<hmc_float>& sigma, const lattices::Staggeredfield_eo& b, const hardware::System& system,

@@ -196,5 +196,11 @@ class MdagM_eo final : public Fermionmatrix_stagg_eo {
     mutable std::unique_ptr<lattices::Staggeredfield_eo> tmp_lo;  // for reduced-precision applications
 };
 
+/* (A + sigma[r]) on every right-hand side of a staggered block with one pass over the links per D_KS
+ * (nrhs 2, 4, 8, 12, 16; tmp holds one block of in's shape).  No counterpart in the reference. */
+void apply_block(const MdagM_eo& A, const lattices::Staggeredfield_eo_block* out, const lattices::Gaugefield& gf,
+                 const lattices::Staggeredfield_eo_block& in, const physics::AdditionalParameters& ap,
+                 const std::vector<hmc_float>& sigma, const lattices::DeviceBuffer& tmp);
+
 }  // namespace fermionmatrix
 }  // namespace physics

@@ -179,6 +179,25 @@ class Spinorfield_eo_block {
     DeviceBuffer buf;
 };
 
+/* nrhs staggered eo vectors in the multi-RHS layout [3][Veo][nrhs] (right-hand side fastest) */
+class Staggeredfield_eo_block {
+  public:
+    Staggeredfield_eo_block(const hardware::System& system, int nrhs, int prec = CL2_F64);
+    void insert(int r, const Staggeredfield_eo& f) const;
+    void extract(int r, const Staggeredfield_eo* f) const;
+    void zero() const;
+    void* dev() const noexcept { return buf.ptr(); }
+    int precision() const noexcept { return prec; }
+    int nrhs() const noexcept { return n; }
+    size_t bytes() const noexcept { return buf.bytes(); }
+    const hardware::System& get_system() const noexcept { return system; }
+
+  private:
+    const hardware::System& system;
+    int n, prec;
+    DeviceBuffer buf;
+};
+
 /* ---- Wilson eo BLAS (physics/lattices/spinorfield_eo.hpp:161-313) ---- */
 void copyData(const Spinorfield_eo* to, const Spinorfield_eo& from);
 void copyData(const Spinorfield_eo* to, const Spinorfield_eo* from);

@@ -452,6 +452,35 @@ PYBIND11_MODULE(_host, m)
         py::arg("rel_tol"), py::arg("ap"), py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1,
         py::arg("max_iterations") = 100000, py::arg("use_graph") = true);
     m.def(
+        "cg_reliable_update_stagg_block",
+        [](std::vector<Staggeredfield_eo*> xs, const fm::MdagM_eo& A, const Gaugefield& gf, std::vector<hmc_float> sigma,
+           const Staggeredfield_eo& b, Lattice& l, hmc_float rel_tol, const AdditionalParameters& ap, int inner_prec,
+           double delta, bool use_graph) {
+            sv::ReliableUpdateOptions o;
+            o.inner_precision = inner_prec;
+            o.delta           = delta;
+            o.use_graph       = use_graph;
+            std::vector<const Staggeredfield_eo*> x(xs.begin(), xs.end());
+            std::vector<sv::SolveStats> st;
+            int it;
+            {
+                py::gil_scoped_release rel;
+                it = sv::cg_reliable_update_stagg_block(x, A, gf, sigma, b, l.system, rel_tol, ap, o, &st);
+            }
+            py::list per;
+            for (const auto& s : st) {
+                py::dict d;
+                d["iterations"]             = s.iterations;
+                d["reliable_updates"]       = s.reliable_updates;
+                d["true_relative_residual"] = s.true_relative_residual;
+                per.append(d);
+            }
+            return py::make_tuple(it, per);
+        },
+        py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("sigma"), py::arg("b"), py::arg("lattice"),
+        py::arg("rel_tol"), py::arg("ap"), py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1,
+        py::arg("use_graph") = true);
+    m.def(
         "cg_m_mixed",
         [](std::vector<Staggeredfield_eo*> xs, const fm::Fermionmatrix_stagg_eo& A, const Gaugefield& gf,
            std::vector<hmc_float> sigma, const Staggeredfield_eo& b, Lattice& l, hmc_float rel_tol,

@@ -301,6 +301,21 @@ bool cg_step_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_blo
 
 /* ---------------- staggered ---------------- */
 
+void apply_block(const MdagM_eo& A, const lattices::Staggeredfield_eo_block* out, const Gaugefield& gf,
+                 const lattices::Staggeredfield_eo_block& in, const physics::AdditionalParameters& ap,
+                 const std::vector<hmc_float>& sigma, const lattices::DeviceBuffer& tmp)
+{
+    if (out->precision() != in.precision() || out->nrhs() != in.nrhs())
+        throw std::invalid_argument("apply_block: blocks of different shape");
+    if (tmp.bytes() < in.bytes()) throw std::invalid_argument("apply_block: tmp must hold one block");
+    if (!sigma.empty() && (int)sigma.size() != in.nrhs())
+        throw std::invalid_argument("apply_block: one shift per right-hand side");
+    const Links l = links_for(gf, in.precision());
+    in.get_system().check(cl2_MdagM_eo_mrhs(in.get_system().ctx(), out->dev(), in.dev(), l.ptr, l.fmt, ap.getMass(),
+                                            A.get_upper_left() ? 1 : 0, sigma.empty() ? nullptr : sigma.data(),
+                                            tmp.ptr(), in.nrhs()));
+}
+
 hmc_float Fermionmatrix_stagg_eo::getThresholdForMinimumEigenvalue(hmc_float) const
 {
     throw std::logic_error("Threshold for minimum eigenvalue not existing or not implemented!");

@@ -332,6 +332,28 @@ void Spinorfield_eo_block::zero() const
     system.check(cl2_block_set_zero_eo(system.ctx(), buf.ptr(), prec, n));
 }
 
+Staggeredfield_eo_block::Staggeredfield_eo_block(const hardware::System& system, int nrhs, int prec)
+    : system(system), n(nrhs), prec(prec),
+      buf(system, (nrhs >= 1 && nrhs <= 64 && (prec == CL2_F64 || prec == CL2_F32))
+                      ? (size_t)nrhs * cl2_field_bytes(system.ctx(), CL2_FIELD_SU3VEC_EO, prec)
+                      : 0)
+{
+}
+void Staggeredfield_eo_block::insert(int r, const Staggeredfield_eo& f) const
+{
+    if (f.precision() != prec) throw std::invalid_argument("block and field of different precision");
+    system.check(cl2_block_insert_stagg_eo(system.ctx(), buf.ptr(), f.dev(), prec, n, r));
+}
+void Staggeredfield_eo_block::extract(int r, const Staggeredfield_eo* f) const
+{
+    if (f->precision() != prec) throw std::invalid_argument("block and field of different precision");
+    system.check(cl2_block_extract_stagg_eo(system.ctx(), f->dev(), buf.ptr(), prec, n, r));
+}
+void Staggeredfield_eo_block::zero() const
+{
+    system.check(cl2_block_set_zero_stagg_eo(system.ctx(), buf.ptr(), prec, n));
+}
+
 Staggeredfield_eo::Staggeredfield_eo(const hardware::System& system, const StaggeredfieldEoParametersInterface&,
                                      int prec)
     : Staggeredfield_eo(system, prec)

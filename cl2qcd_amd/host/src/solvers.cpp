@@ -445,17 +445,75 @@ namespace algorithms {
 namespace solvers {
 
 namespace {
+/* the block operations of the lockstep solver for Wilson spinors and staggered vectors */
+struct WilsonBlockOps {
+    using Field = Spinorfield_eo;
+    using Block = Spinorfield_eo_block;
+    static int squarenorm(cl2_ctx* c, double* res, const void* x, int prec, int n)
+    {
+        return cl2_block_squarenorm_eo(c, res, x, prec, n);
+    }
+    static int dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int n)
+    {
+        return cl2_block_dot_eo(c, res, x, y, prec, n);
+    }
+    static int axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int n, const double* coef,
+                    double sign, double* norms)
+    {
+        return cl2_block_axpy_eo(c, out, x, y, prec, n, coef, sign, norms);
+    }
+    static int convert(cl2_ctx* c, void* dst, int dprec, const void* src, int sprec, int n)
+    {
+        return cl2_block_convert_eo(c, dst, dprec, src, sprec, n);
+    }
+    static int xp(cl2_ctx* c, void* x, void* p, const void* r, int prec, int n, const double* a, const double* b)
+    {
+        return cl2_block_cg_update_xp_eo(c, x, p, r, prec, n, a, b);
+    }
+};
+struct StaggBlockOps {
+    using Field = Staggeredfield_eo;
+    using Block = Staggeredfield_eo_block;
+    static int squarenorm(cl2_ctx* c, double* res, const void* x, int prec, int n)
+    {
+        return cl2_block_squarenorm_stagg_eo(c, res, x, prec, n);
+    }
+    static int dot(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int n)
+    {
+        return cl2_block_dot_stagg_eo(c, res, x, y, prec, n);
+    }
+    static int axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int n, const double* coef,
+                    double sign, double* norms)
+    {
+        return cl2_block_axpy_stagg_eo(c, out, x, y, prec, n, coef, sign, norms);
+    }
+    static int convert(cl2_ctx* c, void* dst, int dprec, const void* src, int sprec, int n)
+    {
+        return cl2_block_convert_stagg_eo(c, dst, dprec, src, sprec, n);
+    }
+    static int xp(cl2_ctx* c, void* x, void* p, const void* r, int prec, int n, const double* a, const double* b)
+    {
+        return cl2_block_cg_update_xp_stagg_eo(c, x, p, r, prec, n, a, b);
+    }
+};
+
 /* out = A in for a block (tmp: two blocks of in's shape) */
-using BlockApply = std::function<void(const Spinorfield_eo_block*, const Spinorfield_eo_block&, const DeviceBuffer&)>;
+template<class Ops>
+using BlockApplyT = std::function<void(const typename Ops::Block*, const typename Ops::Block&, const DeviceBuffer&)>;
+using BlockApply  = BlockApplyT<WilsonBlockOps>;
 /* the operator half of a lockstep step (r, p, tmp, r2, active, pAp, alpha, norms) where the operator has one
  * (an empty function where not) */
-using BlockStep = std::function<bool(const Spinorfield_eo_block*, const Spinorfield_eo_block&, const DeviceBuffer&,
-                                     const Vector<hmc_float>&, const Vector<hmc_float>&, const Vector<hmc_float>&,
-                                     const Vector<hmc_float>&, const Vector<hmc_float>&)>;
+template<class Ops>
+using BlockStepT = std::function<bool(const typename Ops::Block*, const typename Ops::Block&, const DeviceBuffer&,
+                                      const Vector<hmc_float>&, const Vector<hmc_float>&, const Vector<hmc_float>&,
+                                      const Vector<hmc_float>&, const Vector<hmc_float>&)>;
+using BlockStep = BlockStepT<WilsonBlockOps>;
 
-int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<const Spinorfield_eo*>& b,
-                const hardware::System& system, hmc_float rel_tol, const BlockApply& apply, const BlockStep& step,
-                const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
+template<class Ops>
+int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vector<const typename Ops::Field*>& b,
+                const hardware::System& system, hmc_float rel_tol, const BlockApplyT<Ops>& apply,
+                const BlockStepT<Ops>& step, const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats,
+                bool shrink = false)
 {
     const int n = (int)b.size();
     if (n < 1 || x.size() != b.size()) throw std::invalid_argument("cg_reliable_update_block: x and b differ in size");
@@ -469,9 +527,9 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
     if (lo != CL2_F32 && lo != CL2_F64) throw std::invalid_argument("cg_reliable_update_block: inner precision FP32 or FP64");
     cl2_ctx* c = system.ctx();
 
-    const Spinorfield_eo_block B(system, n, CL2_F64), X(system, n, CL2_F64), R(system, n, CL2_F64),
+    const typename Ops::Block B(system, n, CL2_F64), X(system, n, CL2_F64), R(system, n, CL2_F64),
         T(system, n, CL2_F64);
-    const Spinorfield_eo_block Rlo(system, n, lo), P(system, n, lo), V(system, n, lo), Xlo(system, n, lo);
+    const typename Ops::Block Rlo(system, n, lo), P(system, n, lo), V(system, n, lo), Xlo(system, n, lo);
     const DeviceBuffer tmp64(system, 2 * T.bytes()), tmplo(system, 2 * P.bytes());
     // per-right-hand-side device scalars
     const Vector<hmc_float> ones(2 * n, system), alpha(2 * n, system), beta(2 * n, system), pAp(2 * n, system),
@@ -483,15 +541,15 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
         B.insert(r, *b[r]);
         X.insert(r, *x[r]);
     }
-    auto block_norms = [&](const Spinorfield_eo_block& f) {
-        system.check(cl2_block_squarenorm_eo(c, res.dev(), f.dev(), f.precision(), n));
+    auto block_norms = [&](const typename Ops::Block& f) {
+        system.check(Ops::squarenorm(c, res.dev(), f.dev(), f.precision(), n));
         std::vector<hmc_float> v = res.get();
         v.resize(n);
         return v;
     };
     auto true_residual = [&]() {
         apply(&T, X, tmp64);
-        system.check(cl2_block_axpy_eo(c, R.dev(), T.dev(), B.dev(), CL2_F64, n, ones.dev(), -1., nullptr));
+        system.check(Ops::axpy(c, R.dev(), T.dev(), B.dev(), CL2_F64, n, ones.dev(), -1., nullptr));
         return block_norms(R);  // |b - A x|^2 per right-hand side
     };
     const std::vector<hmc_float> b2 = block_norms(B);
@@ -517,13 +575,14 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
         finish(0);
         return 0;
     }
-    system.check(cl2_block_convert_eo(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
+    system.check(Ops::convert(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
     system.check(cl2_copy(c, P.dev(), Rlo.dev(), P.bytes()));
     Xlo.zero();
     r2.store(r2h);
     active.store(act);
     int k           = 0;
     int reliable    = 0;
+    bool partial    = false;  // shrink: returned with some right-hand sides still active
     apply(&V, P, tmplo);  // reduced-precision links made before recording
     // whether the operator does the step's operator half itself (QplusQminus_eo: no A p stored, no dot pass)
     const bool fused = static_cast<bool>(step) && n > 1;
@@ -531,16 +590,16 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
     auto body = [&]() {
         if (!fused) {
             apply(&V, P, tmplo);
-            system.check(cl2_block_dot_eo(c, pAp.dev(), P.dev(), V.dev(), lo, n));
+            system.check(Ops::dot(c, pAp.dev(), P.dev(), V.dev(), lo, n));
             system.check(cl2_block_cg_alpha(c, alpha.dev(), r2.dev(), pAp.dev(), active.dev(), n));
-            system.check(cl2_block_axpy_eo(c, Rlo.dev(), V.dev(), Rlo.dev(), lo, n, alpha.dev(), -1., norms.dev()));
+            system.check(Ops::axpy(c, Rlo.dev(), V.dev(), Rlo.dev(), lo, n, alpha.dev(), -1., norms.dev()));
         } else {
             if (!step(&Rlo, P, tmplo, r2, active, pAp, alpha, norms))
                 throw std::logic_error("cg_reliable_update_block: the operator has no fused CG step");
         }
         system.check(cl2_block_cg_beta(c, beta.dev(), r2.dev(), norms.dev(), active.dev(), n));
         // x += alpha p and p = r + beta p in one pass
-        system.check(cl2_block_cg_update_xp_eo(c, Xlo.dev(), P.dev(), Rlo.dev(), lo, n, alpha.dev(), beta.dev()));
+        system.check(Ops::xp(c, Xlo.dev(), P.dev(), Rlo.dev(), lo, n, alpha.dev(), beta.dev()));
     };
     struct Graph {
         cl2_graph* g = nullptr;
@@ -573,8 +632,8 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
             rmax[r] = std::max(rmax[r], std::sqrt(q));
         }
         if (!update) continue;
-        system.check(cl2_block_convert_eo(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
-        system.check(cl2_block_axpy_eo(c, X.dev(), T.dev(), X.dev(), CL2_F64, n, ones.dev(), 1., nullptr));
+        system.check(Ops::convert(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
+        system.check(Ops::axpy(c, X.dev(), T.dev(), X.dev(), CL2_F64, n, ones.dev(), 1., nullptr));
         Xlo.zero();
         rn = true_residual();
         ++reliable;
@@ -591,11 +650,16 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
             ++k;
             break;
         }
+        if (shrink && 2 * n_active <= n) {  // at most half still active: return them for a smaller block
+            ++k;
+            partial = true;
+            break;
+        }
         // p = r_iter + beta p_old was formed with the iterated residuals: rebuild it on the true ones,
         // p = r_true + (beta p_old with its component along r_true removed)
-        system.check(cl2_block_axpy_eo(c, P.dev(), Rlo.dev(), P.dev(), lo, n, ones.dev(), -1., nullptr));
-        system.check(cl2_block_convert_eo(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
-        system.check(cl2_block_dot_eo(c, res.dev(), Rlo.dev(), P.dev(), lo, n));
+        system.check(Ops::axpy(c, P.dev(), Rlo.dev(), P.dev(), lo, n, ones.dev(), -1., nullptr));
+        system.check(Ops::convert(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
+        system.check(Ops::dot(c, res.dev(), Rlo.dev(), P.dev(), lo, n));
         const std::vector<hmc_float> rp = res.get();
         for (int r = 0; r < n; ++r) {
             if (act[r] == 0.) continue;
@@ -605,15 +669,15 @@ int block_solve(const std::vector<const Spinorfield_eo*>& x, const std::vector<c
             rmax[r]       = std::sqrt(rn[r]);
         }
         coef.store(oc);
-        system.check(cl2_block_axpy_eo(c, P.dev(), Rlo.dev(), P.dev(), lo, n, coef.dev(), -1., nullptr));
-        system.check(cl2_block_axpy_eo(c, P.dev(), Rlo.dev(), P.dev(), lo, n, ones.dev(), 1., nullptr));
+        system.check(Ops::axpy(c, P.dev(), Rlo.dev(), P.dev(), lo, n, coef.dev(), -1., nullptr));
+        system.check(Ops::axpy(c, P.dev(), Rlo.dev(), P.dev(), lo, n, ones.dev(), 1., nullptr));
         r2.store(r2h);
         active.store(act);
     }
     for (int r = 0; r < n; ++r) st[r].reliable_updates = reliable;
     const bool done = n_active == 0;
     finish(k);
-    if (!done) throw SolverDidNotSolve(k, "cg_reliable_update_block");
+    if (!done && !partial) throw SolverDidNotSolve(k, "cg_reliable_update_block");
     return k;
 }
 
@@ -631,7 +695,7 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
                    const Vector<hmc_float>& alpha, const Vector<hmc_float>& norms) {
             return fermionmatrix::cg_step_block(A, r, gf, p, &ap, {}, {}, tmp, r2, active, pAp, alpha, norms);
         };
-    return block_solve(
+    return block_solve<WilsonBlockOps>(
         x, b, system, rel_tol,
         [&](const Spinorfield_eo_block* out, const Spinorfield_eo_block& in, const DeviceBuffer& tmp) {
             fermionmatrix::apply_block(A, out, gf, in, ap, tmp);
@@ -664,7 +728,7 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
             return fermionmatrix::cg_step_block(A, r, gf, p, nullptr, kappa, mubar, tmp, r2, active, pAp, alpha,
                                                 norms);
         };
-    return block_solve(
+    return block_solve<WilsonBlockOps>(
         x, b, system, rel_tol,
         [&](const Spinorfield_eo_block* out, const Spinorfield_eo_block& in, const DeviceBuffer& tmp) {
             fermionmatrix::apply_block(A, out, gf, in, kappa, mubar, tmp);
@@ -679,6 +743,66 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
 namespace physics {
 namespace algorithms {
 namespace solvers {
+
+int cg_reliable_update_stagg_block(const std::vector<const Staggeredfield_eo*>& x, const fermionmatrix::MdagM_eo& A,
+                                   const Gaugefield& gf, const std::vector<hmc_float>& sigma, const Staggeredfield_eo& b,
+                                   const hardware::System& system, hmc_float rel_tol,
+                                   const physics::AdditionalParameters& ap, const ReliableUpdateOptions& opt,
+                                   std::vector<SolveStats>* stats)
+{
+    if (sigma.size() != x.size() || x.empty())
+        throw std::invalid_argument("cg_reliable_update_stagg_block: one shift per solution");
+    const int n = (int)x.size();
+    std::vector<SolveStats> st(n);
+    std::vector<int> todo(n);
+    for (int r = 0; r < n; ++r) todo[r] = r;
+    int total = 0;
+    while (!todo.empty()) {
+        // the largest supported block of pending shifts; a block returns once at most half of it is still
+        // unconverged, and those continue in a smaller block
+        int m = 1;
+        for (int s : {16, 12, 8, 4, 2})
+            if ((int)todo.size() >= s) {
+                m = s;
+                break;
+            }
+        if (m == 1) {
+            SolveStats one;
+            total += cg_reliable_update_stagg(x[todo[0]], A, gf, sigma[todo[0]], b, system, rel_tol, ap, opt, &one);
+            st[todo[0]].iterations += one.iterations;
+            st[todo[0]].reliable_updates += one.reliable_updates;
+            st[todo[0]].true_relative_residual = one.true_relative_residual;
+            todo.erase(todo.begin());
+            continue;
+        }
+        std::vector<const Staggeredfield_eo*> xs;
+        std::vector<hmc_float> sg;
+        for (int j = 0; j < m; ++j) {
+            xs.push_back(x[todo[j]]);
+            sg.push_back(sigma[todo[j]]);
+        }
+        const std::vector<const Staggeredfield_eo*> bs(m, &b);
+        std::vector<SolveStats> bst;
+        total += block_solve<StaggBlockOps>(
+            xs, bs, system, rel_tol,
+            [&](const Staggeredfield_eo_block* out, const Staggeredfield_eo_block& in, const DeviceBuffer& tmp) {
+                fermionmatrix::apply_block(A, out, gf, in, ap, sg, tmp);
+            },
+            BlockStepT<StaggBlockOps>(), opt, &bst, true);
+        std::vector<int> next;
+        for (int j = 0; j < m; ++j) {
+            SolveStats& s = st[todo[j]];
+            s.iterations += bst[j].iterations;
+            s.reliable_updates += bst[j].reliable_updates;
+            s.true_relative_residual = bst[j].true_relative_residual;
+            if (bst[j].true_relative_residual > rel_tol) next.push_back(todo[j]);
+        }
+        next.insert(next.end(), todo.begin() + m, todo.end());
+        todo.swap(next);
+    }
+    if (stats) *stats = st;
+    return total;
+}
 
 int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fermionmatrix_stagg_eo& A,
                              const Gaugefield& gf, hmc_float sigma, const Staggeredfield_eo& b,
@@ -911,8 +1035,15 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
     ro.max_iterations  = opt.max_iterations;
     ro.use_graph       = opt.use_graph;
     int total          = iter;
-    for (int k = 0; k < n; ++k)
-        total += cg_reliable_update_stagg(x[k].get(), A, gf, sigma[k], b, system, rel_tol, ap, ro, &st[k]);
+    const auto* mdagm  = dynamic_cast<const fermionmatrix::MdagM_eo*>(&A);
+    if (mdagm != nullptr) {
+        std::vector<const Staggeredfield_eo*> xs;
+        for (const auto& xi : x) xs.push_back(xi.get());
+        total += cg_reliable_update_stagg_block(xs, *mdagm, gf, sigma, b, system, rel_tol, ap, ro, &st);
+    } else {
+        for (int k = 0; k < n; ++k)
+            total += cg_reliable_update_stagg(x[k].get(), A, gf, sigma[k], b, system, rel_tol, ap, ro, &st[k]);
+    }
     if (stats) *stats = st;
     return total;
 }

@@ -298,6 +298,24 @@ int cl2_block_dot_eo(cl2_ctx* ctx, double* res, const void* x, const void* y, in
 int cl2_block_squarenorm_eo(cl2_ctx* ctx, double* res, const void* x, int prec, int nrhs);
 int cl2_block_convert_eo(cl2_ctx* ctx, void* dst, int dst_prec, const void* src, int src_prec, int nrhs);
 int cl2_block_set_zero_eo(cl2_ctx* ctx, void* out, int prec, int nrhs);
+/* Blocks of nrhs staggered eo vectors [3][Veo][nrhs] (no reference counterpart; the operations of the Wilson
+ * blocks above on staggered fields, and the lockstep refinement of the shifts of cg_m, solver_shifted.cpp):
+ * D_KS_eo and MdagM_eo on every right-hand side with one pass over the links (nrhs 2, 4, 8, 12, 16, FP64 or
+ * FP32 links), MdagM with a shift per right-hand side: out_r = (mass^2 + sigma[r]) in_r - D_KS D_KS in_r
+ * (sigma: host array of nrhs values, or null for none); slot r of D_KS_eo_mrhs equals cl2_D_KS_eo bit for bit. */
+int cl2_D_KS_eo_mrhs(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, int evenodd, int nrhs);
+int cl2_MdagM_eo_mrhs(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, double mass,
+                      int upper_left, const double* sigma, void* tmp, int nrhs);
+int cl2_block_insert_stagg_eo(cl2_ctx* ctx, void* block, const void* single, int prec, int nrhs, int r);
+int cl2_block_extract_stagg_eo(cl2_ctx* ctx, void* single, const void* block, int prec, int nrhs, int r);
+int cl2_block_axpy_stagg_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, int nrhs,
+                            const double* coef, double sign, double* norms);
+int cl2_block_dot_stagg_eo(cl2_ctx* ctx, double* res, const void* x, const void* y, int prec, int nrhs);
+int cl2_block_squarenorm_stagg_eo(cl2_ctx* ctx, double* res, const void* x, int prec, int nrhs);
+int cl2_block_convert_stagg_eo(cl2_ctx* ctx, void* dst, int dst_prec, const void* src, int src_prec, int nrhs);
+int cl2_block_set_zero_stagg_eo(cl2_ctx* ctx, void* out, int prec, int nrhs);
+int cl2_block_cg_update_xp_stagg_eo(cl2_ctx* ctx, void* x, void* p, const void* r, int prec, int nrhs,
+                                    const double* alpha, const double* beta);
 /* lockstep CG scalars: alpha_r = r2_r / pAp_r (complex) where active_r != 0, else 0;
  * beta_r = Polak-Ribiere (norms_r[0] - norms_r[1]) / r2_r (Fletcher-Reeves norms_r[0] / r2_r if negative)
  * where active_r != 0, else 0, and then r2_r = norms_r[0] */
