@@ -107,6 +107,14 @@ int cg_reliable_update_stagg_block(const std::vector<const lattices::Staggeredfi
                                    const hardware::System& system, hmc_float rel_tol,
                                    const physics::AdditionalParameters& additionalParameters,
                                    const ReliableUpdateOptions& options, std::vector<SolveStats>* stats = nullptr);
+/* the same with a right-hand side per system: (A + sigma[r]) x[r] = b[r] */
+int cg_reliable_update_stagg_block(const std::vector<const lattices::Staggeredfield_eo*>& x,
+                                   const fermionmatrix::MdagM_eo& A, const lattices::Gaugefield& gf,
+                                   const std::vector<hmc_float>& sigma,
+                                   const std::vector<const lattices::Staggeredfield_eo*>& b,
+                                   const hardware::System& system, hmc_float rel_tol,
+                                   const physics::AdditionalParameters& additionalParameters,
+                                   const ReliableUpdateOptions& options, std::vector<SolveStats>* stats = nullptr);
 
 struct MultishiftOptions {
     int inner_precision = CL2_F32;  // precision of the multi-shift phase and of the refinement's inner loop

@@ -168,7 +168,8 @@ size_t Engine::peak_work_bytes() const
 }
 
 /* device bytes a request's solve needs: the block solver's FP64 and inner-precision fields and
- * temporaries per (padded) right-hand side for Wilson, the reliable-update CG's fields for staggered */
+ * temporaries per (padded) right-hand side, plus for staggered each right-hand side's own source and
+ * solution */
 size_t Engine::work_bytes(const Pending& p) const
 {
     const size_t veo = (size_t)configs.at(p.req.config)->system->eo_sites();
@@ -176,14 +177,14 @@ size_t Engine::work_bytes(const Pending& p) const
         const int n = std::max(1, padded_rhs(p.nrhs()));
         return veo * (size_t)n * (6 * 192 + 6 * 96 + 2 * 192);
     }
-    return veo * (4 * 48 + 4 * 24 + 72 + 2 * 48 * (size_t)p.nrhs());
+    return veo * (size_t)p.nrhs() * (4 * 48 + 4 * 24 + 2 * 48 + 2 * 24 + 2 * 48);
 }
 
 bool Engine::compatible(const Pending& a, const Pending& b) const
 {
-    // any twisted-mass requests on one configuration: the block operator takes kappa and mu per right-hand side
-    return opt.mixed && a.req.action == Action::twistedmass && b.req.action == Action::twistedmass &&
-           a.req.config == b.req.config && a.req.rel_tol == b.req.rel_tol;
+    // requests of one action on one configuration: the twisted-mass block operator takes kappa and mu per
+    // right-hand side, the staggered one a shift per right-hand side (m_r^2 - m^2 on MdagM_eo of mass m)
+    return opt.mixed && a.req.action == b.req.action && a.req.config == b.req.config && a.req.rel_tol == b.req.rel_tol;
 }
 
 long Engine::submit(Request r)
@@ -355,6 +356,49 @@ void Engine::run_batch(Worker& wk, Batch& b)
                     const std::vector<spinor> h = xs[k]->get_host();
                     const double* d             = reinterpret_cast<const double*>(h.data());
                     sol.emplace_back(d, d + h.size() * 24);
+                } else {
+                    sol.emplace_back();
+                }
+            }
+        } else if (opt.mixed && total_rhs > 1) {
+            // every right-hand side of the batch in lockstep blocks over one pass of the links per D_KS_eo; a
+            // request of mass m_r is the head's MdagM_eo shifted by m_r^2 - m^2
+            const StaggeredAdditionalParameters ap(head.mass);
+            std::vector<std::unique_ptr<StaggeredAdditionalParameters>> own;
+            std::vector<const StaggeredAdditionalParameters*> aps;
+            std::vector<hmc_float> shift;
+            std::vector<std::unique_ptr<Staggeredfield_eo>> bs, xs;
+            for (const auto& it : b.items) {
+                own.emplace_back(new StaggeredAdditionalParameters(it->req.mass));
+                for (int j = 0; j < it->nrhs(); ++j) {
+                    aps.push_back(own.back().get());
+                    shift.push_back(it->req.mass * it->req.mass - head.mass * head.mass);
+                    bs.emplace_back(new Staggeredfield_eo(sys));
+                    xs.emplace_back(new Staggeredfield_eo(sys));
+                    if (it->req.sources.empty())
+                        bs.back()->gaussian_from_seed(it->req.gaussian_seeds[j]);
+                    else
+                        bs.back()->import(reinterpret_cast<const su3vec*>(it->req.sources[j].data()));
+                    xs.back()->set_zero();
+                }
+            }
+            std::vector<const Staggeredfield_eo*> xv, bv;
+            for (int k = 0; k < total_rhs; ++k) {
+                xv.push_back(xs[k].get());
+                bv.push_back(bs[k].get());
+            }
+            sv::ReliableUpdateOptions o;
+            o.inner_precision = CL2_F32;
+            iterations = sv::cg_reliable_update_stagg_block(xv, *L.mdagm, gf, shift, bv, sys, head.rel_tol, ap, o);
+            const Staggeredfield_eo t(sys);
+            for (int k = 0; k < total_rhs; ++k) {  // FP64 true residual with the request's own mass
+                (*L.mdagm)(&t, gf, *xs[k], aps[k]);
+                saxpby(&t, 1., *bs[k], -1., t);
+                resid.push_back(std::sqrt(squarenorm(t) / squarenorm(*bs[k])));
+                if (keep[k]) {
+                    const std::vector<su3vec> h = xs[k]->get_host();
+                    const double* d             = reinterpret_cast<const double*>(h.data());
+                    sol.emplace_back(d, d + h.size() * 6);
                 } else {
                     sol.emplace_back();
                 }

@@ -750,8 +750,18 @@ int cg_reliable_update_stagg_block(const std::vector<const Staggeredfield_eo*>& 
                                    const physics::AdditionalParameters& ap, const ReliableUpdateOptions& opt,
                                    std::vector<SolveStats>* stats)
 {
-    if (sigma.size() != x.size() || x.empty())
-        throw std::invalid_argument("cg_reliable_update_stagg_block: one shift per solution");
+    return cg_reliable_update_stagg_block(x, A, gf, sigma, std::vector<const Staggeredfield_eo*>(x.size(), &b), system,
+                                          rel_tol, ap, opt, stats);
+}
+
+int cg_reliable_update_stagg_block(const std::vector<const Staggeredfield_eo*>& x, const fermionmatrix::MdagM_eo& A,
+                                   const Gaugefield& gf, const std::vector<hmc_float>& sigma,
+                                   const std::vector<const Staggeredfield_eo*>& b, const hardware::System& system,
+                                   hmc_float rel_tol, const physics::AdditionalParameters& ap,
+                                   const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats)
+{
+    if (sigma.size() != x.size() || b.size() != x.size() || x.empty())
+        throw std::invalid_argument("cg_reliable_update_stagg_block: one shift and one right-hand side per solution");
     const int n = (int)x.size();
     std::vector<SolveStats> st(n);
     std::vector<int> todo(n);
@@ -768,20 +778,21 @@ int cg_reliable_update_stagg_block(const std::vector<const Staggeredfield_eo*>& 
             }
         if (m == 1) {
             SolveStats one;
-            total += cg_reliable_update_stagg(x[todo[0]], A, gf, sigma[todo[0]], b, system, rel_tol, ap, opt, &one);
+            total += cg_reliable_update_stagg(x[todo[0]], A, gf, sigma[todo[0]], *b[todo[0]], system, rel_tol, ap, opt,
+                                              &one);
             st[todo[0]].iterations += one.iterations;
             st[todo[0]].reliable_updates += one.reliable_updates;
             st[todo[0]].true_relative_residual = one.true_relative_residual;
             todo.erase(todo.begin());
             continue;
         }
-        std::vector<const Staggeredfield_eo*> xs;
+        std::vector<const Staggeredfield_eo*> xs, bs;
         std::vector<hmc_float> sg;
         for (int j = 0; j < m; ++j) {
             xs.push_back(x[todo[j]]);
+            bs.push_back(b[todo[j]]);
             sg.push_back(sigma[todo[j]]);
         }
-        const std::vector<const Staggeredfield_eo*> bs(m, &b);
         std::vector<SolveStats> bst;
         total += block_solve<StaggBlockOps>(
             xs, bs, system, rel_tol,

@@ -6,7 +6,8 @@
   another request's multi-RHS solve, overlapping other solves on other streams) to
   1e-6 relative -- both answers meet the 1e-10 residual, and 1e-6 bounds cond(A) x 1e-10
   at these lattices;
-* compatible requests are merged, incompatible ones overlap in time, a tiny HBM budget
+* compatible requests (one action on one configuration, any masses) are merged, incompatible ones overlap in
+  time, a tiny HBM budget
   serialises the solves, and the FP64 one-at-a-time mode agrees with the mixed mode.
 """
 import numpy as np
@@ -49,10 +50,12 @@ def requests(e, c0, c1):
     rng = np.random.default_rng(5)
     out = []
     # with two workers the first two run at once and the rest queue, so the three twisted-mass requests on
-    # config c0 are merged when a worker frees up, one of them with its own kappa and mu
+    # config c0 are merged when a worker frees up, one of them with its own kappa and mu, and so are the two
+    # staggered requests on config c1, of different masses
     for cfg, action, kappa, mu, mass, n in ((c1, "twistedmass", 0.155, 0.01, 0, 1), (c0, "staggered", 0, 0, 0.05, 2),
                                              (c0, "twistedmass", 0.15, 0.02, 0, 2), (c1, "staggered", 0, 0, 0.1, 1),
-                                             (c0, "twistedmass", 0.155, 0.005, 0, 3), (c0, "twistedmass", 0.15, 0.02, 0, 1)):
+                                             (c0, "twistedmass", 0.155, 0.005, 0, 3), (c0, "twistedmass", 0.15, 0.02, 0, 1),
+                                             (c1, "staggered", 0, 0, 0.07, 2)):
         w = 24 if action == "twistedmass" else 6
         srcs = [rng.standard_normal((e.eo_sites(cfg), w)) for _ in range(n)]
         out.append(dict(config=cfg, action=action, kappa=kappa, mu=mu, mass=mass, sources=srcs))
@@ -78,6 +81,8 @@ def test_same_alone_and_in_mix():
             assert rel(xm, xa) < 1e-6
     # the twisted-mass requests on config 0 were merged into one solve despite their different masses
     merged = [m["batch_requests"] for r, m in zip(reqs, mix) if r["config"] == c0 and r["action"] == "twistedmass"]
+    assert max(merged) >= 2
+    merged = [m["batch_requests"] for r, m in zip(reqs, mix) if r["config"] == c1 and r["action"] == "staggered"]
     assert max(merged) >= 2
     # solves overlapped in time
     spans = sorted(set((m["admitted"], m["finished"]) for m in mix))  # one span per solve
