@@ -219,14 +219,14 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.nrhs          = nrhs;
     a.rb            = a.per_rhs ? 1 : t.rb;  // per-right-hand-side operators: one right-hand side per thread
     a.lds           = t.lds;
-    if (nrhs > 1 && t.lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
+    if (nrhs > 1 && t.lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs, t.lds), 1, t);
     CL2_REQUIRE(c, cg_r == nullptr || (cg_alpha != nullptr && norm_res != nullptr), "CG update without its scalars");
     // per-right-hand-side norms come from the kernel with per-site staging (one right-hand side per thread)
     const bool mrhs_norms = nrhs > 1 && norm_res != nullptr;
     if (mrhs_norms) {
         a.rb  = 1;
-        a.lds = 1;
-        a.map = work_map(c, mrhs_sites_per_block(nrhs), 1, t);
+        a.lds = t.lds == 2 ? 2 : 1;
+        a.map = work_map(c, mrhs_sites_per_block(nrhs, a.lds), 1, t);
     }
     CL2_HIP(c, launch_dslash(c, a, fmt));
     // every single-field variant launches blocks_for(Veo, block) blocks, each writing one partial: |out|^2 as a
@@ -235,7 +235,7 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     if (norm_res && !mrhs_norms)
         CL2_HIP(c, launch_reduce_stage2(c, blocks_for(a.g.veo, a.block), norm_res, cg_r ? 3 : 1, nullptr, 0));
     if (mrhs_norms) {
-        const int sb = mrhs_sites_per_block(nrhs);
+        const int sb = mrhs_sites_per_block(nrhs, a.lds);
         CL2_HIP(c, launch_block_reduce2(c, reinterpret_cast<const double2*>(c->partials),
                                         (unsigned)((a.g.veo + sb - 1) / sb), norm_res, 0, nrhs));
     }
@@ -1554,7 +1554,8 @@ int cl2_tune_set(cl2_ctx* c, const char* key, long value)
         return CL2_OK;
     }
     if (k == "mrhs.lds") {
-        c->tune_mrhs_lds = value ? 1 : 0;
+        CL2_REQUIRE(c, value == 0 || value == 1 || value == 2, "mrhs.lds must be 0, 1 or 2");
+        c->tune_mrhs_lds = (int)value;
         c->tune_manual   = true;
         return CL2_OK;
     }
@@ -1630,7 +1631,10 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     std::vector<std::pair<int, int>> rls;
     for (int r : rbs) {
         rls.push_back({r, 0});
-        if (nrhs > 1 && r == 1) rls.push_back({r, 1});
+        if (nrhs > 1 && r == 1) {
+            rls.push_back({r, 1});
+            rls.push_back({r, 2});
+        }
     }
     for (int zp : zps)
         for (int x : {1, 0})
@@ -1665,7 +1669,7 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     if (best_ms) *best_ms = best;
     if (zplanes) *zplanes = bt.zplanes;
     if (xcd) *xcd = bt.xcd;
-    if (rb) *rb = (nrhs > 1 && bt.rb <= 1 && bt.lds) ? -1 : bt.rb;
+    if (rb) *rb = (nrhs > 1 && bt.rb <= 1 && bt.lds) ? -bt.lds : bt.rb;
     if (block) *block = bt.block;
     if (occupancy) *occupancy = bt.occ;
     return CL2_OK;

@@ -67,7 +67,7 @@ WorkMap work_map(const cl2_ctx* c, int block, int nrhs, const cl2_hop_tuning& t)
 /* tuning-table key of the staggered hopping kernel: its link format | DKS_TUNING_KEY */
 constexpr int DKS_TUNING_KEY = 0x100;
 /* sites per block of the multi-RHS kernel that stages links in LDS (blocks of sites x nrhs threads) */
-int mrhs_sites_per_block(int nrhs);
+int mrhs_sites_per_block(int nrhs, int lds);
 /* right-hand-side counts with a compiled multi-RHS hopping kernel */
 bool supported_nrhs(int n);
 

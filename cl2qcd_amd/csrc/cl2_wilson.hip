@@ -241,14 +241,15 @@ void k_dslash_eo(DslashArgs a)
     }
 }
 
-int mrhs_sites_per_block(int nrhs)
+int mrhs_sites_per_block(int nrhs, int lds)
 {
-    return nrhs >= 8 ? 16 : 32;
+    return (nrhs >= 8 ? 16 : 32) * (lds == 2 ? 2 : 1);
 }
 
-template<int NRHS>
+/* sites per block of the staged kernel: 16 (8 or more right-hand sides) or 32, times SBX (1 or 2) */
+template<int NRHS, int SBX = 1>
 struct MrhsLds {
-    static constexpr int SB    = NRHS >= 8 ? 16 : 32;  // sites per block (mrhs_sites_per_block)
+    static constexpr int SB    = (NRHS >= 8 ? 16 : 32) * SBX;  // sites per block (mrhs_sites_per_block)
     static constexpr int BLOCK = SB * NRHS;
 };
 
@@ -258,13 +259,13 @@ struct MrhsLds {
  * them from LDS and gathers only its own neighbour spinors, both hops of a direction in flight together.
  * This removes the per-lane index arithmetic, link loads and 12-real reconstruction that otherwise repeat
  * NRHS times per site.  Per right-hand side the arithmetic is the single-field kernel's, in the same order. */
-template<int FMT, int NRHS>
-__global__ __launch_bounds__(MrhsLds<NRHS>::BLOCK) void k_dslash_eo_mrhs_lds(DslashArgs a)
+template<int FMT, int NRHS, int SBX>
+__global__ __launch_bounds__((MrhsLds<NRHS, SBX>::BLOCK)) void k_dslash_eo_mrhs_lds(DslashArgs a)
 {
     using R = typename GaugeLoad<FMT>::R;
     using V = vec2<R>;
-    constexpr int SB    = MrhsLds<NRHS>::SB;
-    constexpr int BLOCK = MrhsLds<NRHS>::BLOCK;
+    constexpr int SB    = MrhsLds<NRHS, SBX>::SB;
+    constexpr int BLOCK = MrhsLds<NRHS, SBX>::BLOCK;
     constexpr uint32_t NONE = 0xffffffffu;
     __shared__ Su3<R> lu[SB * 8];     // [site][hop], hop = 2 dir + backward
     __shared__ uint32_t lnb[SB * 8];  // neighbour eo index of each hop
@@ -540,9 +541,13 @@ static hipError_t launch_rb(cl2_ctx* c, const DslashArgs& a)
     // one right-hand side per thread: per-site LDS staging of links and neighbour indices, or the single-field
     // kernel over the [12][Veo][NRHS] block; both keep the hops of a direction in flight together
     if constexpr (RB == 1) {
-        if (a.lds) {
+        if (a.lds == 2) {
+            constexpr int SB = MrhsLds<NRHS, 2>::SB;
+            hipLaunchKernelGGL((k_dslash_eo_mrhs_lds<FMT, NRHS, 2>), dim3((a.g.veo + SB - 1) / SB),
+                               dim3(MrhsLds<NRHS, 2>::BLOCK), 0, c->stream, a);
+        } else if (a.lds) {
             constexpr int SB = MrhsLds<NRHS>::SB;
-            hipLaunchKernelGGL((k_dslash_eo_mrhs_lds<FMT, NRHS>), dim3((a.g.veo + SB - 1) / SB),
+            hipLaunchKernelGGL((k_dslash_eo_mrhs_lds<FMT, NRHS, 1>), dim3((a.g.veo + SB - 1) / SB),
                                dim3(MrhsLds<NRHS>::BLOCK), 0, c->stream, a);
         } else {
             hipLaunchKernelGGL((k_dslash_eo<FMT, DSLASH_BLOCK, NRHS, 0>), dim3(nb), dim3(DSLASH_BLOCK), 0, c->stream,

@@ -420,7 +420,9 @@ int cl2_cg_beta(cl2_ctx* ctx, double* mbeta, double* r2, const double* norms);
  *       FP16);
  *       "mrhs.rb" -- right-hand sides per thread of the multi-right-hand-side kernel (0: by format, 1, 2, 4);
  *       "mrhs.lds" -- with one right-hand side per thread, 1 (default): a block computes each site's neighbour
- *       indices and loads its links once into LDS for all its right-hand-side threads, 0: every thread does.
+ *       indices and loads its links once into LDS for all its right-hand-side threads (16 sites per block for
+ *       8 or more right-hand sides, else 32), 2: the same with twice the sites per block, 0: every thread
+ *       does.
  * Every setting changes only which thread computes which site and when: results are bit identical. */
 int cl2_tune_set(cl2_ctx* ctx, const char* key, long value);
 long cl2_tune_get(const cl2_ctx* ctx, const char* key);
@@ -430,8 +432,8 @@ long cl2_tune_get(const cl2_ctx* ctx, const char* key);
  * mapping, right-hand sides per thread, and for single fields the block size (128, 256, 512) and register
  * budget -- by timing every candidate on scratch fields with the given gauge field; the fastest is kept for
  * later launches (unless cl2_tune_set fixed the shape) and returned.  occupancy: the register budget, as
- * the "hop.occupancy" key of cl2_tune_set; rhs_per_thread: 1, 2 or 4, or -1 for one right-hand side per
- * thread with the links staged in LDS ("mrhs.lds").  Output pointers may be null. */
+ * the "hop.occupancy" key of cl2_tune_set; rhs_per_thread: 1, 2 or 4, or -1 / -2 for one right-hand side per
+ * thread with the links staged in LDS ("mrhs.lds" 1 / 2).  Output pointers may be null. */
 int cl2_autotune_dslash(cl2_ctx* ctx, const void* gauge, int format, int nrhs, double* best_ms, int* zplanes,
                         int* xcd, int* rhs_per_thread, int* block, int* occupancy);
 

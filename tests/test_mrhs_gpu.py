@@ -34,12 +34,13 @@ def block_slot(ctx, blk, prec, n, r):
     return out
 
 
-@pytest.mark.parametrize("lds", [1, 0])
+@pytest.mark.parametrize("lds", [1, 2, 0])
 @pytest.mark.parametrize("nrhs", [2, 12, 16])
 @pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F64_12, abi.GAUGE_F32_12])
 def test_dslash_mrhs_matches_single(nrhs, fmt, lds):
     """Slot r of the multi-RHS result is the single-field result for source r bit for bit, with the links
-    staged in LDS per site (lds=1, the default) or loaded by every right-hand-side thread (lds=0)."""
+    staged in LDS per site (lds=1, the default; lds=2 with twice the sites per block) or loaded by every
+    right-hand-side thread (lds=0)."""
     lat = O.lattice(8, 8, theta_t=1.0)
     prec = abi.F32 if fmt == abi.GAUGE_F32_12 else abi.F64
     with abi.Context(8, 8, theta_t=1.0) as ctx:
