@@ -9,6 +9,7 @@
 #include <cstring>
 
 #include "../include/physics/algorithms/inversion.hpp"
+#include "../include/physics/algorithms/rational_approximation.hpp"
 #include "../include/physics/algorithms/solvers/solvers.hpp"
 #include "../include/physics/algorithms/su3heatbath.hpp"
 #include "../include/physics/observables/gaugeObservables.hpp"
@@ -451,6 +452,36 @@ PYBIND11_MODULE(_host, m)
         py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("sigma"), py::arg("b"), py::arg("lattice"),
         py::arg("rel_tol"), py::arg("ap"), py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1,
         py::arg("max_iterations") = 100000, py::arg("use_graph") = true);
+    {
+        namespace pa = physics::algorithms;
+        auto coeffs = [](const pa::Rational_Coefficients& c) {
+            py::dict d;
+            d["order"] = c.Get_order();
+            d["a0"]    = c.Get_a0();
+            d["a"]     = c.Get_a();
+            d["b"]     = c.Get_b();
+            return d;
+        };
+        py::class_<pa::Rational_Approximation>(m, "Rational_Approximation")
+            .def(py::init([](int d, int y, int z, hmc_float low, hmc_float high, bool inv) {
+                     py::gil_scoped_release rel;
+                     return new pa::Rational_Approximation(d, y, z, low, high, inv);
+                 }),
+                 py::arg("d"), py::arg("y"), py::arg("z"), py::arg("low"), py::arg("high"), py::arg("inv") = true)
+            .def(py::init<const std::string&>(), py::arg("filename"))
+            .def("order", &pa::Rational_Approximation::Get_order)
+            .def("a0", &pa::Rational_Approximation::Get_a0)
+            .def("a", &pa::Rational_Approximation::Get_a)
+            .def("b", &pa::Rational_Approximation::Get_b)
+            .def("error", &pa::Rational_Approximation::Get_error)
+            .def("exponent", &pa::Rational_Approximation::Get_exponent)
+            .def("iterations", &pa::Rational_Approximation::iterations)
+            .def("evaluate", &pa::Rational_Approximation::evaluate)
+            .def("save", &pa::Rational_Approximation::Save_rational_approximation)
+            .def("rescale", [coeffs](const pa::Rational_Approximation& r, hmc_float mn, hmc_float mx) {
+                return coeffs(r.Rescale_Coefficients(mn, mx));
+            });
+    }
     m.def(
         "cg_reliable_update_stagg_block",
         [](std::vector<Staggeredfield_eo*> xs, const fm::MdagM_eo& A, const Gaugefield& gf, std::vector<hmc_float> sigma,
