@@ -53,15 +53,16 @@ def test_fermionmatrix_eo_dot(fmt, prec, tol, action):
                                      res.at(0))
 
 
+@pytest.mark.parametrize("ns,nt", [(8, 8), (6, 10)])  # 6^3x10: 1080 eo sites, partial blocks and link tiles
 @pytest.mark.parametrize("fmt,prec", [(abi.GAUGE_F64_18, abi.F64), (abi.GAUGE_F32_12, abi.F32),
                                       (abi.GAUGE_F16_12, abi.F16)])
-def test_fermionmatrix_eo_cg_step(fmt, prec):
+def test_fermionmatrix_eo_cg_step(fmt, prec, ns, nt):
     """pAp, alpha, r <- r - alpha A p and its norms from the fused step against the unfused sequence (operator
     with <p, A p>, ratio, saxpy_AND_cgnorm): the same bits for FP64/FP32 fields and scalars (norms summed in
     another grouping: 1e-12); FP16 stores A p in half precision in the unfused sequence only: 5e-3.  FP64
     also against the oracle."""
-    lat = O.lattice(8, 8, theta_t=1.0)
-    with abi.Context(8, 8, theta_t=1.0) as ctx:
+    lat = O.lattice(ns, nt, theta_t=1.0)
+    with abi.Context(ns, nt, theta_t=1.0) as ctx:
         g = random_su3_field(4 * O.volume(lat), 41)
         rng = np.random.default_rng(42)
         P, R = rng.standard_normal((O.veo(lat), 24)), rng.standard_normal((O.veo(lat), 24))

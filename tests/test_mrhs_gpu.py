@@ -251,23 +251,24 @@ def test_block_cg_per_rhs_params():
     assert np.sqrt(((xs[1].get_host() - x1.get_host()) ** 2).sum() / (x1.get_host() ** 2).sum()) < 1e-6
 
 
+@pytest.mark.parametrize("ns,nt", [(8, 8), (6, 10)])  # 6^3x10: a partial last block of staged sites
 @pytest.mark.parametrize("per_rhs", [False, True])
 @pytest.mark.parametrize("nrhs", [4, 12])
 @pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12])
-def test_mrhs_cg_step(fmt, nrhs, per_rhs):
+def test_mrhs_cg_step(fmt, nrhs, per_rhs, ns, nt):
     """The fused lockstep CG step (pAp per right-hand side from |Qminus p_r|^2, alpha, r_r -= alpha_r A p_r
     with its norms, A p never stored) against the unfused block sequence (operator, block dot, alpha, block
     axpy with norms): FP64 to 1e-12, FP32 to 1e-5 (pAp is summed from another field, so alpha and everything
     after it differ by rounding); an inactive right-hand side keeps r and gets alpha 0; FP64 against the oracle."""
     import ctypes
-    lat = O.lattice(8, 8, theta_t=1.0)
+    lat = O.lattice(ns, nt, theta_t=1.0)
     prec = abi.F32 if fmt == abi.GAUGE_F32_12 else abi.F64
     tol = 1e-12 if prec == abi.F64 else 1e-5
     rng = np.random.default_rng(nrhs)
     kappas = list(0.15 + 0.01 * rng.random(nrhs)) if per_rhs else [0.16] * nrhs
     mubars = [2 * k * m for k, m in zip(kappas, 0.003 + 0.017 * rng.random(nrhs))] if per_rhs else \
         [2 * 0.16 * 0.005] * nrhs
-    with abi.Context(8, 8, theta_t=1.0) as ctx:
+    with abi.Context(ns, nt, theta_t=1.0) as ctx:
         g = random_su3_field(4 * O.volume(lat), 14)
         gf = ctx.gauge(g, fmt)
         P = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
