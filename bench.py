@@ -467,7 +467,7 @@ def main():
     USE_GRAPH = not args.no_graph
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
-              "described in DESIGN.md but not built)", file=sys.stderr)
+              "is designed but not built)", file=sys.stderr)
         sys.exit(2)
 
     from cl2qcd_amd import abi
