@@ -218,14 +218,20 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     a.map           = work_map(c, a.block, nrhs, t);
     a.nrhs          = nrhs;
     a.rb            = a.per_rhs ? 1 : t.rb;  // per-right-hand-side operators: one right-hand side per thread
-    a.lds           = t.lds;
-    if (nrhs > 1 && t.lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs, t.lds), 1, t);
+    // twice the sites per staged block only where the FP64 gathers can use 32-bit offsets
+    const int lds =
+        (t.lds == 2 && fmt != GAUGE_F32_12 &&
+         !(c->tune_mrhs_off32 && mrhs_offsets_narrow(a.g.veo, nrhs, 2 * sizeof(double))))
+            ? 1
+            : t.lds;
+    a.lds = lds;
+    if (nrhs > 1 && lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs, lds), 1, t);
     CL2_REQUIRE(c, cg_r == nullptr || (cg_alpha != nullptr && norm_res != nullptr), "CG update without its scalars");
     // per-right-hand-side norms come from the kernel with per-site staging (one right-hand side per thread)
     const bool mrhs_norms = nrhs > 1 && norm_res != nullptr;
     if (mrhs_norms) {
         a.rb  = 1;
-        a.lds = t.lds == 2 ? 2 : 1;
+        a.lds = lds == 2 ? 2 : 1;
         a.map = work_map(c, mrhs_sites_per_block(nrhs, a.lds), 1, t);
     }
     CL2_HIP(c, launch_dslash(c, a, fmt));
@@ -1559,6 +1565,10 @@ int cl2_tune_set(cl2_ctx* c, const char* key, long value)
         c->tune_manual   = true;
         return CL2_OK;
     }
+    if (k == "mrhs.off32") {
+        c->tune_mrhs_off32 = value ? 1 : 0;
+        return CL2_OK;
+    }
     if (k == "hop.block") {
         CL2_REQUIRE(c, value == 128 || value == 256 || value == 512, "hop.block must be 128, 256 or 512");
         c->tune_block  = (int)value;
@@ -1584,6 +1594,7 @@ long cl2_tune_get(const cl2_ctx* c, const char* key)
     if (k == "mrhs.rb") return c->tune_mrhs_rb;
     if (k == "hop.block") return c->tune_block;
     if (k == "mrhs.lds") return c->tune_mrhs_lds;
+    if (k == "mrhs.off32") return c->tune_mrhs_off32;
     if (k == "hop.occupancy") return c->tune_occ;
     return -2;
 }

@@ -219,6 +219,26 @@ __device__ __forceinline__ bool map_site(const WorkMap& m, uint32_t veo, uint32_
     return true;
 }
 
+/* a wave-uniform pointer pinned to scalar registers, so that base + 32-bit lane offset addressing is
+ * selected for loads from it instead of per-lane 64-bit address arithmetic */
+typedef const __attribute__((address_space(1))) char* gchar_ptr;
+__device__ __forceinline__ gchar_ptr uniform_ptr(const char* p)
+{
+    const uint64_t v  = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (gchar_ptr)(((uint64_t)hi << 32) | lo);
+}
+
+/* the complex number at byte offset off of a uniform base */
+template<typename R>
+__device__ __forceinline__ vec2<R> load_at(gchar_ptr base, uint32_t off)
+{
+    typedef R N __attribute__((ext_vector_type(2)));
+    const N v = *(const __attribute__((address_space(1))) N*)(base + off);
+    return vec2<R>{v.x, v.y};
+}
+
 /* eo index of logical site l of the work order (l < veo) */
 __device__ __forceinline__ uint32_t logical_site(const WorkMap& m, uint32_t l)
 {

@@ -40,6 +40,7 @@ struct cl2_ctx {
     int tune_block   = 256; // threads per block of the single-field hopping kernel
     int tune_occ     = -1;  // register budget of the single-field hopping kernel (-1: the format's default)
     int tune_mrhs_lds = 1;  // multi-RHS hopping kernel: links and neighbour indices staged in LDS per site
+    int tune_mrhs_off32 = 1;  // staged multi-RHS kernel: FP64 gathers with 32-bit offsets where they fit
     bool tune_manual = false;  // set by cl2_tune_set: the values above override autotuned ones
     std::map<std::pair<int, int>, cl2_hop_tuning> tuned;  // (link format, right-hand sides) -> autotuned shape
     std::string err;
@@ -68,6 +69,9 @@ WorkMap work_map(const cl2_ctx* c, int block, int nrhs, const cl2_hop_tuning& t)
 constexpr int DKS_TUNING_KEY = 0x100;
 /* sites per block of the multi-RHS kernel that stages links in LDS (blocks of sites x nrhs threads) */
 int mrhs_sites_per_block(int nrhs, int lds);
+/* the staged multi-RHS kernel's FP64 gathers use 32-bit byte offsets within a component: veo * nrhs complex
+ * numbers of complex_bytes each must stay below 4 GiB, else it takes 64-bit offsets (and lds 1 only) */
+bool mrhs_offsets_narrow(uint32_t veo, int nrhs, size_t complex_bytes);
 /* right-hand-side counts with a compiled multi-RHS hopping kernel */
 bool supported_nrhs(int n);
 

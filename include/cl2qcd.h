@@ -422,7 +422,10 @@ int cl2_cg_beta(cl2_ctx* ctx, double* mbeta, double* r2, const double* norms);
  *       "mrhs.lds" -- with one right-hand side per thread, 1 (default): a block computes each site's neighbour
  *       indices and loads its links once into LDS for all its right-hand-side threads (16 sites per block for
  *       8 or more right-hand sides, else 32), 2: the same with twice the sites per block, 0: every thread
- *       does.
+ *       does;
+ *       "mrhs.off32" -- 1 (default): the staged kernel's FP64 neighbour gathers use a scalar base per spinor
+ *       component and 32-bit lane offsets while Veo * nrhs * 16 bytes < 4 GiB (beyond that, and with 0,
+ *       64-bit offsets and one-size staged blocks).
  * Every setting changes only which thread computes which site and when: results are bit identical. */
 int cl2_tune_set(cl2_ctx* ctx, const char* key, long value);
 long cl2_tune_get(const cl2_ctx* ctx, const char* key);

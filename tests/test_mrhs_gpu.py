@@ -34,17 +34,19 @@ def block_slot(ctx, blk, prec, n, r):
     return out
 
 
-@pytest.mark.parametrize("lds", [1, 2, 0])
+@pytest.mark.parametrize("lds", [1, 2, 0, -1, -2])
 @pytest.mark.parametrize("nrhs", [2, 12, 16])
 @pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F64_12, abi.GAUGE_F32_12])
 def test_dslash_mrhs_matches_single(nrhs, fmt, lds):
     """Slot r of the multi-RHS result is the single-field result for source r bit for bit, with the links
     staged in LDS per site (lds=1, the default; lds=2 with twice the sites per block) or loaded by every
-    right-hand-side thread (lds=0)."""
+    right-hand-side thread (lds=0); -1 / -2: lds 1 / 2 with the 64-bit-offset gathers ("mrhs.off32" 0) that
+    FP64 fields above 4 GiB per component take."""
     lat = O.lattice(8, 8, theta_t=1.0)
     prec = abi.F32 if fmt == abi.GAUGE_F32_12 else abi.F64
     with abi.Context(8, 8, theta_t=1.0) as ctx:
-        ctx.check(ctx.lib.cl2_tune_set(ctx.handle, b"mrhs.lds", lds))
+        ctx.check(ctx.lib.cl2_tune_set(ctx.handle, b"mrhs.lds", abs(lds)))
+        ctx.check(ctx.lib.cl2_tune_set(ctx.handle, b"mrhs.off32", 0 if lds < 0 else 1))
         g = random_su3_field(4 * O.volume(lat), 5)
         gf = ctx.gauge(g, fmt)
         rng = np.random.default_rng(nrhs)
@@ -254,13 +256,15 @@ def test_block_cg_per_rhs_params():
 @pytest.mark.parametrize("ns,nt", [(8, 8), (6, 10)])  # 6^3x10: a partial last block of staged sites
 @pytest.mark.parametrize("per_rhs", [False, True])
 @pytest.mark.parametrize("nrhs", [4, 12])
-@pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12])
+@pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12, "F64_18 64-bit offsets"])
 def test_mrhs_cg_step(fmt, nrhs, per_rhs, ns, nt):
     """The fused lockstep CG step (pAp per right-hand side from |Qminus p_r|^2, alpha, r_r -= alpha_r A p_r
     with its norms, A p never stored) against the unfused block sequence (operator, block dot, alpha, block
     axpy with norms): FP64 to 1e-12, FP32 to 1e-5 (pAp is summed from another field, so alpha and everything
     after it differ by rounding); an inactive right-hand side keeps r and gets alpha 0; FP64 against the oracle."""
     import ctypes
+    off32 = 0 if isinstance(fmt, str) else 1
+    fmt = abi.GAUGE_F64_18 if isinstance(fmt, str) else fmt
     lat = O.lattice(ns, nt, theta_t=1.0)
     prec = abi.F32 if fmt == abi.GAUGE_F32_12 else abi.F64
     tol = 1e-12 if prec == abi.F64 else 1e-5
@@ -269,6 +273,7 @@ def test_mrhs_cg_step(fmt, nrhs, per_rhs, ns, nt):
     mubars = [2 * k * m for k, m in zip(kappas, 0.003 + 0.017 * rng.random(nrhs))] if per_rhs else \
         [2 * 0.16 * 0.005] * nrhs
     with abi.Context(ns, nt, theta_t=1.0) as ctx:
+        ctx.check(ctx.lib.cl2_tune_set(ctx.handle, b"mrhs.off32", off32))
         g = random_su3_field(4 * O.volume(lat), 14)
         gf = ctx.gauge(g, fmt)
         P = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
