@@ -57,16 +57,33 @@ __device__ __forceinline__ void load_site_h(const void* p, uint32_t S, uint32_t 
 /* the same without the scale: the site's 24 stored values (converted to float) and its norm, for the
  * hopping kernels, which fold the norm into the hop weight (the hop is linear in the spinor) and so save
  * 24 multiplications per neighbour and let the half-to-float conversions fuse into the arithmetic */
-__device__ __forceinline__ float load_site_h_raw(const void* p, uint32_t S, uint32_t i, Cx<float> (&v)[12])
+struct Ch {
+    _Float16 re, im;
+};
+
+__device__ __forceinline__ float load_site_h_raw(const void* p, uint32_t S, uint32_t i, Ch (&v)[12])
 {
     const h8* l = reinterpret_cast<const h8*>(p);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        const f8v f = __builtin_convertvector(l[(size_t)j * S + i], f8v);
+        const h8 f = l[(size_t)j * S + i];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[4 * j + q] = Cx<float>{f[2 * q], f[2 * q + 1]};
+        for (int q = 0; q < 4; ++q) v[4 * j + q] = Ch{f[2 * q], f[2 * q + 1]};
     }
     return half_norms(p, S)[i];
+}
+
+/* a + U*b in float from half inputs.  Each component is one fma(x, +-1, y) -- the float add or subtract
+ * of the exactly converted halves, bit for bit -- with a 1 the compiler cannot fold away, so that the
+ * conversions fuse into the operation (v_fma_mix_f32) instead of costing an instruction each. */
+template<int U>
+__device__ __forceinline__ Cx<float> add_unit_h(Ch a, Ch b, float one)
+{
+    const float ar = (float)a.re, ai = (float)a.im, br = (float)b.re, bi = (float)b.im;
+    if constexpr (U == P1) return Cx<float>{__builtin_fmaf(br, one, ar), __builtin_fmaf(bi, one, ai)};
+    if constexpr (U == M1) return Cx<float>{__builtin_fmaf(br, -one, ar), __builtin_fmaf(bi, -one, ai)};
+    if constexpr (U == PI) return Cx<float>{__builtin_fmaf(bi, -one, ar), __builtin_fmaf(br, one, ai)};
+    return Cx<float>{__builtin_fmaf(bi, one, ar), __builtin_fmaf(br, -one, ai)};
 }
 
 /* hop weight times a site's norm */
@@ -124,12 +141,12 @@ __device__ __forceinline__ void load_link_h(const void* g, uint32_t S, int dir, 
 /* one projector row pair of a hop on a neighbour spinor already in registers
  * (same algebra as the FP64/FP32 kernel: h = psi_A + C psi_B, r = w U h) */
 template<int A, int B, int CU, bool FWD>
-__device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const Cx<float> (&psi)[12], const Su3<float>& u,
-                                           Cx<float> w)
+__device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const Ch (&psi)[12], const Su3<float>& u,
+                                           Cx<float> w, float one)
 {
     Cx<float> h[3], r[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) h[c] = add_unit<CU>(psi[3 * A + c], psi[3 * B + c]);
+    for (int c = 0; c < 3; ++c) h[c] = add_unit_h<CU>(psi[3 * A + c], psi[3 * B + c], one);
     if constexpr (FWD)
         su3_times(u, h, r);
     else
@@ -143,21 +160,23 @@ __device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const Cx<float>
 }
 
 template<int DIR, bool FWD>
-__device__ __forceinline__ void apply_hop_h(Cx<float> (&acc)[12], const Cx<float> (&psi)[12], const Su3<float>& u,
+__device__ __forceinline__ void apply_hop_h(Cx<float> (&acc)[12], const Ch (&psi)[12], const Su3<float>& u,
                                             Cx<float> w)
 {
+    float one;  // 1.0 hidden from constant folding (see add_unit_h)
+    asm("v_mov_b32 %0, 1.0" : "=v"(one));
     if constexpr (DIR == 0) {
-        half_hop_r<0, 2, FWD ? P1 : M1, FWD>(acc, psi, u, w);
-        half_hop_r<1, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w);
+        half_hop_r<0, 2, FWD ? P1 : M1, FWD>(acc, psi, u, w, one);
+        half_hop_r<1, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w, one);
     } else if constexpr (DIR == 1) {
-        half_hop_r<0, 3, FWD ? PI : MI, FWD>(acc, psi, u, w);
-        half_hop_r<1, 2, FWD ? PI : MI, FWD>(acc, psi, u, w);
+        half_hop_r<0, 3, FWD ? PI : MI, FWD>(acc, psi, u, w, one);
+        half_hop_r<1, 2, FWD ? PI : MI, FWD>(acc, psi, u, w, one);
     } else if constexpr (DIR == 2) {
-        half_hop_r<0, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w);
-        half_hop_r<1, 2, FWD ? M1 : P1, FWD>(acc, psi, u, w);
+        half_hop_r<0, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w, one);
+        half_hop_r<1, 2, FWD ? M1 : P1, FWD>(acc, psi, u, w, one);
     } else {
-        half_hop_r<0, 2, FWD ? PI : MI, FWD>(acc, psi, u, w);
-        half_hop_r<1, 3, FWD ? MI : PI, FWD>(acc, psi, u, w);
+        half_hop_r<0, 2, FWD ? PI : MI, FWD>(acc, psi, u, w, one);
+        half_hop_r<1, 3, FWD ? MI : PI, FWD>(acc, psi, u, w, one);
     }
 }
 
@@ -170,7 +189,7 @@ __device__ __forceinline__ void hops_h(Cx<float> (&acc)[12], const void* in, uin
     if constexpr (ND > 0) {
         constexpr int G = H < 2 ? 1 : H / 2;  // directions per group
         Su3<float> u[2 * G];
-        Cx<float> psi[2 * G][12];
+        Ch psi[2 * G][12];
         if constexpr (H == 1) {
             load_link_h(gauge, S, D0, P, i, u[0]);
             float n = load_site_h_raw(in, S, nb.up[D0], psi[0]);
