@@ -1774,8 +1774,10 @@ uint64_t cl2_reference_read_write_size(const cl2_ctx* c, const char* k)
         n == "dslash_AND_M_tm_inverse_sitediagonal_minus_eo")
         return (C * 12 * (2 * 4 + 1) + C * 2 * 4 * R) * D * Seo;  // fermions.cpp:860-871
     if (n == "gamma5_eo" || n.rfind("M_tm_", 0) == 0) return 48 * D * Seo;  // fermions.cpp:834-857,872-877
-    if (n == "saxpy_AND_gamma5_eo") return C * D * Seo * (12 * (2 + 1) + 1);  // fermions.cpp:878-880
+    if (n == "saxpy_AND_gamma5_eo") return C * D * Seo * (12 * (2 + 1) + 1);  // fermions.cpp:887-891
+    if (n == "saxpy_eoprec") return C * D * Seo * (12 * (2 + 1) + 1);         // spinors.cpp:969-972
     if (n == "D_KS_eo") return (C * 3 * (2 * 4 + 1) + C * 2 * 4 * R) * D * Seo;  // fermions_staggered.cpp:189-193
+    if (n == "saxpby_cplx_staggered_eoprec") return C * D * Seo * (3 * (2 + 1) + 2);  // spinors_staggered.cpp:1712-1715
     return 0;
 }
 
@@ -1792,7 +1794,12 @@ uint64_t cl2_reference_flop_size(const cl2_ctx* c, const char* k)
     if (n == "M_tm_sitediagonal" || n == "M_tm_sitediagonal_minus") return Seo * diag;
     if (n == "M_tm_inverse_sitediagonal" || n == "M_tm_inverse_sitediagonal_minus") return Seo * (diag + 24);
     if (n == "gamma5_eo") return Seo * 12;
-    if (n == "D_KS_eo") return Seo * 570;  // fermions_staggered.cpp:259-263
+    if (n == "M_tm_sitediagonal_AND_gamma5_eo" || n == "M_tm_sitediagonal_minus_AND_gamma5_eo")
+        return Seo * diag + Seo * 12;                                // fermions.cpp:961-968
+    if (n == "saxpy_AND_gamma5_eo") return Seo * 4 * 3 * (1 + 6 + 2);  // fermions.cpp:969-973
+    if (n == "saxpy_eoprec") return Seo * 4 * 3 * (6 + 2);             // spinors.cpp:1085-1088
+    if (n == "D_KS_eo") return Seo * 570;                              // fermions_staggered.cpp:259-263
+    if (n == "saxpby_cplx_staggered_eoprec") return Seo * 3 * (2 * 6 + 2);  // spinors_staggered.cpp:1853-1856
     return 0;
 }
 

@@ -37,4 +37,7 @@ namespace common {
 enum action { wilson = 0, clover, twistedmass, tlsym, iwasaki, dbw2, rooted_stagg };
 enum solver { cg = 0, bicgstab, bicgstab_save };
 enum startcondition { cold_start = 0, hot_start, start_from_source };
+/* inverter sources (common_header_files/types.hpp:203-204) */
+enum sourcetypes { point = 1, volume, timeslice, zslice };
+enum sourcecontents { one = 1, z4, gaussian, z2 };
 }  // namespace common

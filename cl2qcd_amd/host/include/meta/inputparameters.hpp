@@ -28,6 +28,8 @@ class Inputparameters {
     int get_source_z() const noexcept { return source_z; }
     int get_source_t() const noexcept { return source_t; }
     int get_corr_dir() const noexcept { return corr_dir; }
+    common::sourcetypes get_sourcetype() const noexcept { return sourcetype; }
+    common::sourcecontents get_sourcecontent() const noexcept { return sourcecontent; }
     uint32_t get_host_seed() const noexcept { return host_seed; }
 
     common::action get_fermact() const noexcept { return fermact; }
@@ -62,6 +64,8 @@ class Inputparameters {
     std::string initial_conf;
     int num_sources = 12, source_x = 0, source_y = 0, source_z = 0, source_t = 0;
     int corr_dir = 3;  // meta/parametersObs.cpp:72: z by default
+    common::sourcetypes sourcetype       = common::point;  // meta/parametersSources.cpp:67-79
+    common::sourcecontents sourcecontent = common::one;
     uint32_t host_seed = 4815;
     common::action fermact = common::wilson;
     double beta = 4.0;

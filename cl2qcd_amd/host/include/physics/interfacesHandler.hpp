@@ -84,6 +84,23 @@ class D_KS_eo;
 class MdagM_eo;
 }  // namespace fermionmatrix
 
+/* physics/sourcesInterface.hpp:26-37: where and what the inverter's sources are.  The extents return int
+ * (as the gauge-field interface's, which the same object implements); the source content, a compile-time
+ * kernel parameter in the reference (SOURCE_CONTENT, hardware/code/opencl_module.cpp), is read here too. */
+class SourcesParametersInterface {
+  public:
+    virtual ~SourcesParametersInterface() = default;
+    virtual bool placeSourcesOnHost() const                 = 0;
+    virtual common::sourcetypes getSourceType() const       = 0;
+    virtual common::sourcecontents getSourceContent() const = 0;
+    virtual unsigned getSourceT() const                     = 0;
+    virtual unsigned getSourceX() const                     = 0;
+    virtual unsigned getSourceY() const                     = 0;
+    virtual unsigned getSourceZ() const                     = 0;
+    virtual int getNt() const                               = 0;
+    virtual int getNs() const                               = 0;
+};
+
 /* one object that implements every interface from the run parameters */
 class ParametersImplementation : public lattices::GaugefieldParametersInterface,
                                  public lattices::SpinorfieldParametersInterface,
@@ -91,7 +108,8 @@ class ParametersImplementation : public lattices::GaugefieldParametersInterface,
                                  public lattices::StaggeredfieldEoParametersInterface,
                                  public fermionmatrix::FermionEoParametersInterface,
                                  public fermionmatrix::FermionStaggeredEoParametersInterface,
-                                 public algorithms::SolversParametersInterface {
+                                 public algorithms::SolversParametersInterface,
+                                 public SourcesParametersInterface {
   public:
     explicit ParametersImplementation(const meta::Inputparameters& p) : p(p) {}
     int getNs() const override { return p.get_nspace(); }
@@ -110,6 +128,13 @@ class ParametersImplementation : public lattices::GaugefieldParametersInterface,
     int getCgMinimumIterationCount() const override { return p.get_cg_minimum_iteration_count(); }
     bool getCgUseAsyncCopy() const override { return p.get_cg_use_async_copy(); }
     bool getUseMergeKernelsSpinor() const override { return p.get_use_merge_kernels_spinor(); }
+    bool placeSourcesOnHost() const override { return true; }  // full-lattice fields are host arrays here
+    common::sourcetypes getSourceType() const override { return p.get_sourcetype(); }
+    common::sourcecontents getSourceContent() const override { return p.get_sourcecontent(); }
+    unsigned getSourceT() const override { return (unsigned)p.get_source_t(); }
+    unsigned getSourceX() const override { return (unsigned)p.get_source_x(); }
+    unsigned getSourceY() const override { return (unsigned)p.get_source_y(); }
+    unsigned getSourceZ() const override { return (unsigned)p.get_source_z(); }
 
   private:
     const meta::Inputparameters& p;
@@ -153,6 +178,7 @@ class InterfacesHandler {
   public:
     virtual ~InterfacesHandler() = default;
     virtual const algorithms::SolversParametersInterface& getSolversParametersInterface() = 0;
+    virtual const SourcesParametersInterface& getSourcesParametersInterface() = 0;
     template<class T>
     const typename InterfaceType<T>::value& getInterface()
     {
@@ -190,6 +216,7 @@ class InterfacesHandlerImplementation final : public InterfacesHandler {
     {
     }
     const algorithms::SolversParametersInterface& getSolversParametersInterface() override { return impl; }
+    const SourcesParametersInterface& getSourcesParametersInterface() override { return impl; }
 
   protected:
     const ParametersImplementation& parameters() const override { return impl; }

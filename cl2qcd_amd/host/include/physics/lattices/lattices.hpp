@@ -105,6 +105,7 @@ class Spinorfield {
     unsigned get_elements() const noexcept { return (unsigned)host.size(); }
     void import(const spinor* h) const;
     std::vector<spinor> get_host() const { return host; }
+    const hardware::System& get_system() const noexcept { return system; }
 
   private:
     friend void convert_to_eoprec(const class Spinorfield_eo*, const class Spinorfield_eo*, const Spinorfield&);

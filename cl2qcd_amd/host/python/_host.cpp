@@ -400,18 +400,58 @@ PYBIND11_MODULE(_host, m)
         py::arg("x"), py::arg("A"), py::arg("gf"), py::arg("b"), py::arg("lattice"), py::arg("rel_tol"),
         py::arg("kappa"), py::arg("mubar"), py::arg("inner_prec") = CL2_F32, py::arg("delta") = 0.1,
         py::arg("max_iterations") = 100000, py::arg("use_graph") = true);
-    m.def("point_source_ps_correlator", [](const Gaugefield& gf, Lattice& l) {
-        py::gil_scoped_release rel;
-        return physics::algorithms::point_source_ps_correlator(gf, l.system, l.handler, l.handler.getAdditionalParameters<Spinorfield_eo>());
-    });
+    auto block_opts = [](int inner_prec, double rel_tol, int block) {
+        physics::algorithms::BlockInversionOptions o;
+        o.solver.inner_precision = inner_prec;
+        o.rel_tol                = rel_tol;
+        o.block                  = block;
+        return o;
+    };
+    m.def(
+        "point_source_ps_correlator",
+        [block_opts](const Gaugefield& gf, Lattice& l, bool block, int inner_prec, double rel_tol, int block_size) {
+            const auto o = block_opts(inner_prec, rel_tol, block_size);
+            py::gil_scoped_release rel;
+            return physics::algorithms::point_source_ps_correlator(
+                gf, l.system, l.handler, l.handler.getAdditionalParameters<Spinorfield_eo>(), block ? &o : nullptr);
+        },
+        py::arg("gf"), py::arg("lattice"), py::arg("block") = false, py::arg("inner_prec") = CL2_F32,
+        py::arg("rel_tol") = 1e-10, py::arg("block_size") = 12);
+    m.def(
+        "ps_correlator",
+        [block_opts](const Gaugefield& gf, Lattice& l, bool block, int inner_prec, double rel_tol, int block_size) {
+            const auto o = block_opts(inner_prec, rel_tol, block_size);
+            py::gil_scoped_release rel;
+            return physics::algorithms::ps_correlator(gf, l.system, l.handler,
+                                                      l.handler.getAdditionalParameters<Spinorfield_eo>(), l.prng,
+                                                      block ? &o : nullptr);
+        },
+        py::arg("gf"), py::arg("lattice"), py::arg("block") = false, py::arg("inner_prec") = CL2_F32,
+        py::arg("rel_tol") = 1e-10, py::arg("block_size") = 12);
     m.def(
         "invert_eo",
         [](const Spinorfield_eo& pe, const Spinorfield_eo& po, const Gaugefield& gf, const Spinorfield_eo& ee,
            const Spinorfield_eo& eo, Lattice& l, hmc_float prec) {
-            py::gil_scoped_release rel;
-            return physics::algorithms::invert_eo(&pe, &po, gf, ee, eo, l.system, l.handler, prec,
-                                                  l.handler.getAdditionalParameters<Spinorfield_eo>());
+            physics::algorithms::InversionStats st;
+            {
+                py::gil_scoped_release rel;
+                physics::algorithms::invert_eo(&pe, &po, gf, ee, eo, l.system, l.handler, prec,
+                                               l.handler.getAdditionalParameters<Spinorfield_eo>(), &st);
+            }
+            py::dict d;
+            d["iterations"]          = st.iterations;
+            d["bicgstab_iterations"] = st.bicgstab_iterations;
+            d["cg_fallback"]         = st.cg_fallback;
+            return d;
         });
+    m.def("create_sources", [](Lattice& l, size_t n) {
+        py::list out;
+        for (auto* f : physics::create_sources(l.system, l.prng, n, l.handler)) {
+            out.append(to_numpy(f->get_host()));
+            delete f;
+        }
+        return out;
+    });
     m.def(
         "autotune_dslash",
         [](const Gaugefield& gf, Lattice& l, int prec, int nrhs) {

@@ -37,12 +37,16 @@ void sitediag(const Spinorfield_eo* out, const Spinorfield_eo& in, hmc_float mub
         cl2_M_tm_sitediagonal_eo(in.get_system().ctx(), out->dev(), in.dev(), in.precision(), mubar, variant));
 }
 
-/* per-application models of the reference (hardware/code/fermions.cpp:811-975,
- * spinors.cpp, fermions_staggered.cpp:175-278), FP64 with 18-real links */
+/* per-site, per-application models of the reference, FP64 with 18-real links:
+ *   dslash_eo 1632 flop / 2880 B, M_tm_sitediagonal 72 / 384, M_tm_inverse_sitediagonal 96 / 384,
+ *   gamma5_eo 12 / 384, saxpy_AND_gamma5_eo 108 / 592 (hardware/code/fermions.cpp:811-975);
+ *   saxpy_eoprec 96 / 592 (hardware/code/spinors.cpp:969-972,1085-1088);
+ *   D_KS_eo 570 / 1584 (hardware/code/fermions_staggered.cpp:189-193,259-263);
+ *   saxpby_cplx_staggered_eoprec 42 / 176 (hardware/code/spinors_staggered.cpp:1712-1715,1853-1856) */
 constexpr cl_ulong DSLASH_FLOPS = 1632, DSLASH_BYTES = 2880, DIAG_FLOPS = 72, DIAG_INV_FLOPS = 96,
-                   DIAG_BYTES = 384, SAXPY_FLOPS = 96, SAXPY_BYTES = 608, G5SAXPY_FLOPS = 108,
+                   DIAG_BYTES = 384, SAXPY_FLOPS = 96, SAXPY_BYTES = 592, G5SAXPY_FLOPS = 108,
                    G5SAXPY_BYTES = 592, G5_FLOPS = 12, G5_BYTES = 384, DKS_FLOPS = 570, DKS_BYTES = 1584,
-                   SAXPBY_STAGG_FLOPS = 42, SAXPBY_STAGG_BYTES = 144;
+                   SAXPBY_STAGG_FLOPS = 42, SAXPBY_STAGG_BYTES = 176;
 
 }  // namespace
 
@@ -173,13 +177,14 @@ cl_ulong BuiltinEoOperator::get_flops() const
 {
     const cl_ulong S = get_system().eo_sites();
     const bool tm    = fermionmatrixParametersInterface.getFermionicActionType() == common::twistedmass;
-    const bool merged = fermionmatrixParametersInterface.useMergedFermionicKernels();
     auto aee = [&](bool g5) {
         cl_ulong f = 2 * DSLASH_FLOPS + (g5 ? G5SAXPY_FLOPS : SAXPY_FLOPS);
         if (tm) f += DIAG_FLOPS + DIAG_INV_FLOPS;
         return f;
     };
-    auto q = [&]() { return merged ? aee(true) : aee(false) + G5_FLOPS; };
+    // Qplus_eo / Qminus_eo count Aee (or Aee_minus) plus gamma5_eo whether or not the merged kernel runs
+    // (physics/fermionmatrix/fermionmatrix.cpp:603-613,625-635)
+    auto q = [&]() { return aee(false) + G5_FLOPS; };
     switch (op) {
         case CL2_OP_AEE:
         case CL2_OP_AEE_MINUS: return S * aee(false);
@@ -195,13 +200,12 @@ cl_ulong BuiltinEoOperator::get_read_write_size() const
 {
     const cl_ulong S = get_system().eo_sites();
     const bool tm    = fermionmatrixParametersInterface.getFermionicActionType() == common::twistedmass;
-    const bool merged = fermionmatrixParametersInterface.useMergedFermionicKernels();
     auto aee = [&](bool g5) {
         cl_ulong b = 2 * DSLASH_BYTES + (g5 ? G5SAXPY_BYTES : SAXPY_BYTES);
         if (tm) b += 2 * DIAG_BYTES;
         return b;
     };
-    auto q = [&]() { return merged ? aee(true) : aee(false) + G5_BYTES; };
+    auto q = [&]() { return aee(false) + G5_BYTES; };  // fermionmatrix.cpp:636-657
     switch (op) {
         case CL2_OP_AEE:
         case CL2_OP_AEE_MINUS: return S * aee(false);

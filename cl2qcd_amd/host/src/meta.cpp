@@ -3,6 +3,7 @@
 // boost::program_options in the reference.
 #include "../include/meta/inputparameters.hpp"
 
+#include <cctype>
 #include <cstdlib>
 #include <functional>
 #include <map>
@@ -22,6 +23,11 @@ double parse_double(const std::string& v)
     double d   = std::stod(v, &pos);
     if (pos != v.size()) throw std::invalid_argument("not a number: " + v);
     return d;
+}
+std::string lower(std::string v)
+{
+    for (auto& ch : v) ch = (char)std::tolower((unsigned char)ch);
+    return v;
 }
 int parse_int(const std::string& v)
 {
@@ -87,11 +93,21 @@ void Inputparameters::parse(const std::vector<std::string>& args)
         {"sourceY", [&](const std::string& v) { source_y = parse_int(v); }},
         {"sourceZ", [&](const std::string& v) { source_z = parse_int(v); }},
         {"sourceT", [&](const std::string& v) { source_t = parse_int(v); }},
-        {"sourceType", [&](const std::string& v) {
-             if (v != "point") throw std::invalid_argument("only point sources are supported: " + v);
+        {"sourceType", [&](const std::string& v) {  // meta/parametersSources.cpp:93-108
+             const std::string l = lower(v);
+             if (l == "point") sourcetype = common::point;
+             else if (l == "volume") sourcetype = common::volume;
+             else if (l == "timeslice") sourcetype = common::timeslice;
+             else if (l == "zslice") sourcetype = common::zslice;
+             else throw std::invalid_argument("sourceType: point, volume, timeslice or zslice: " + v);
          }},
-        {"sourceContent", [&](const std::string& v) {
-             if (v != "one") throw std::invalid_argument("only the source content 'one' is supported: " + v);
+        {"sourceContent", [&](const std::string& v) {  // meta/parametersSources.cpp:110-125
+             const std::string l = lower(v);
+             if (l == "one") sourcecontent = common::one;
+             else if (l == "z4") sourcecontent = common::z4;
+             else if (l == "gaussian") sourcecontent = common::gaussian;
+             else if (l == "z2") sourcecontent = common::z2;
+             else throw std::invalid_argument("sourceContent: one, z4, gaussian or z2: " + v);
          }},
         {"measureCorrelators", [&](const std::string& v) { parse_bool(v); }},
         {"correlatorDirection", [&](const std::string& v) {

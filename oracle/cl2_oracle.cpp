@@ -767,6 +767,19 @@ void orc_dslash_eo(const orc_lattice* lat, double* out, const double* in, const 
     });
 }
 
+/* orc_dslash_eo restricted to a list of output eo indices (the same site formula,
+ * fermionmatrix_eo_dslash.cl:50-58): out[k] is the output spinor at eo index eo[k].
+ * Sampled checks of lattices too large for a full CPU pass use it. */
+void orc_dslash_eo_sites(const orc_lattice* lat, double* out, const double* in, const double* gauge, int evenodd,
+                         double kappa, const long* eo, long n)
+{
+    Geo g(lat);
+    site_loop(n, [&](long k) {
+        long pos = (evenodd == ODD) ? g.even_site(eo[k]) : g.odd_site(eo[k]);
+        store_sp(out, k, dslash_site(g, in, gauge, pos, kappa));
+    });
+}
+
 /* fermionmatrix_eo_m.cl:22-109 and fermionmatrix_eo_m_merged.cl:22-64 */
 void orc_m_tm(const orc_lattice* lat, double* out, const double* in, double mubar, int variant)
 {

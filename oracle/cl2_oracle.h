@@ -81,6 +81,9 @@ void orc_convert_from_eo(const orc_lattice* lat, const double* even, const doubl
 /* Wilson / twisted-mass kernels */
 void orc_dslash_eo(const orc_lattice* lat, double* out, const double* in, const double* gauge, int evenodd,
                    double kappa);
+/* dslash_eo at the n output eo indices eo[0..n-1] only; out holds n spinors */
+void orc_dslash_eo_sites(const orc_lattice* lat, double* out, const double* in, const double* gauge, int evenodd,
+                         double kappa, const long* eo, long n);
 void orc_m_tm(const orc_lattice* lat, double* out, const double* in, double mubar, int variant);
 void orc_dslash_minv_eo(const orc_lattice* lat, double* out, const double* in, const double* gauge, int evenodd,
                         double kappa, double mubar, int minus);

@@ -68,6 +68,7 @@ def lib():
             "orc_convert_to_eo": (None, [L, _P, _P, _P]),
             "orc_convert_from_eo": (None, [L, _P, _P, _P]),
             "orc_dslash_eo": (None, [L, _P, _P, _P, i, d]),
+            "orc_dslash_eo_sites": (None, [L, _P, _P, _P, i, d, ctypes.POINTER(l), l]),
             "orc_m_tm": (None, [L, _P, _P, d, i]),
             "orc_dslash_minv_eo": (None, [L, _P, _P, _P, i, d, d, i]),
             "orc_gamma5_eo": (None, [L, _P]),
@@ -170,6 +171,15 @@ def convert_to_eo(lat, full):
 def dslash_eo(lat, inp, g, evenodd, kappa):
     out = np.empty_like(inp)
     lib().orc_dslash_eo(ctypes.byref(lat), _p(out), _p(inp), _p(g), evenodd, kappa)
+    return out
+
+
+def dslash_eo_sites(lat, inp, g, evenodd, kappa, eo_idx):
+    """dslash_eo at the given output eo indices only: returns (len(eo_idx), 24)."""
+    idx = np.ascontiguousarray(eo_idx, dtype=np.int64)
+    out = np.empty((idx.size, 24))
+    lib().orc_dslash_eo_sites(ctypes.byref(lat), _p(out), _p(inp), _p(g), evenodd, kappa,
+                              idx.ctypes.data_as(ctypes.POINTER(ctypes.c_long)), idx.size)
     return out
 
 

@@ -15,27 +15,13 @@ import pytest
 
 import oracle as O
 from cl2qcd_amd import abi
-from fields import random_su3_field, site_rel_err
+from fields import half_links, random_su3_field, site_rel_err
 
 pytestmark = pytest.mark.gpu
 
 
 def field_rel(a, b):
     return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
-
-
-def half_links(g):
-    """links as the F16_12 format stores them: rows 0, 1 rounded to FP16, row 2 = conj(row0 x row1)"""
-    u = g.reshape(-1, 9, 2).copy()
-    u[:, :6] = u[:, :6].astype(np.float16).astype(np.float64)
-    c = u[:, :, 0] + 1j * u[:, :, 1]
-    r0, r1 = c[:, 0:3], c[:, 3:6]
-    r2 = np.conj(np.cross(r0, r1))
-    c[:, 6:9] = r2
-    out = np.empty_like(g)
-    out[:, 0::2] = c.real
-    out[:, 1::2] = c.imag
-    return out
 
 
 def test_storage_roundtrip():

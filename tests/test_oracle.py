@@ -113,6 +113,33 @@ def test_m_tm_inverse_identity(cold16):
         assert np.abs(a - b).max() < 1e-15
 
 
+def test_dslash_minv_is_m_inverse_after_dslash():
+    """The fused dslash_AND_M_tm_inverse_sitediagonal{,_minus}_eo restatement (its own factor table) equals
+    M_tm_inverse_sitediagonal{,_minus} applied to dslash_eo, both pinned by the reference's goldens above
+    (fermionmatrix_eo_dslash_AND_M_tm_inverse_sitediagonal{,_minus}.cl:30-62)."""
+    lat = O.lattice(4, 6, theta_t=1.0, cp_re=0.05)
+    g = random_su3_field(4 * O.volume(lat), 17)
+    src = np.random.default_rng(18).standard_normal((O.veo(lat), 24))
+    for evenodd in (O.EVEN, O.ODD):
+        d = O.dslash_eo(lat, src, g, evenodd, 0.16)
+        for minus, variant in ((False, O.MTM_INVERSE), (True, O.MTM_INVERSE_MINUS)):
+            fused = O.dslash_minv_eo(lat, src, g, evenodd, 0.16, 0.0016, minus)
+            assert np.abs(fused - O.m_tm(lat, d, 0.0016, variant)).max() <= 1e-15 * np.abs(d).max()
+            # and it is not the non-inverse or the other sign
+            other = O.MTM_INVERSE_MINUS if not minus else O.MTM_INVERSE
+            assert np.abs(fused - O.m_tm(lat, d, 0.0016, other)).max() > 1e-6 * np.abs(d).max()
+
+
+def test_dslash_eo_sites_matches_full():
+    lat = O.lattice(6, 4, theta_t=1.0, theta_s=0.3)
+    g = random_su3_field(4 * O.volume(lat), 19)
+    src = np.random.default_rng(20).standard_normal((O.veo(lat), 24))
+    sites = np.random.default_rng(21).choice(O.veo(lat), 50, replace=False)
+    for evenodd in (O.EVEN, O.ODD):
+        full = O.dslash_eo(lat, src, g, evenodd, 0.13)
+        assert np.array_equal(O.dslash_eo_sites(lat, src, g, evenodd, 0.13, sites), full[sites])
+
+
 # physics/fermionmatrix/fermionmatrix_test.cpp:68-133 (Wilson action default)
 @pytest.mark.parametrize("op,seed,refs", [
     (O.OP_AEE, 5, (1111.6772283004893, 1290.5580533222192, 1114.183875802898, 1304.0219632505139)),
