@@ -149,14 +149,17 @@ int Engine::load_config(int ns, int nt, double beta, int sweeps, uint64_t seed)
 
 double Engine::plaquette(int config) const
 {
+    std::lock_guard<std::mutex> lk(mu);
     return configs.at(config)->plaquette;
 }
 long Engine::eo_sites(int config) const
 {
+    std::lock_guard<std::mutex> lk(mu);
     return configs.at(config)->system->eo_sites();
 }
 size_t Engine::resident_bytes() const
 {
+    std::lock_guard<std::mutex> lk(mu);
     size_t s = 0;
     for (const auto& c : configs) s += c->bytes;
     return s;
@@ -169,7 +172,7 @@ size_t Engine::peak_work_bytes() const
 
 /* device bytes a request's solve needs: the block solver's FP64 and inner-precision fields and
  * temporaries per (padded) right-hand side, plus for staggered each right-hand side's own source and
- * solution */
+ * solution.  Called with mu held. */
 size_t Engine::work_bytes(const Pending& p) const
 {
     const size_t veo = (size_t)configs.at(p.req.config)->system->eo_sites();
@@ -189,18 +192,18 @@ bool Engine::compatible(const Pending& a, const Pending& b) const
 
 long Engine::submit(Request r)
 {
-    if (r.config < 0 || r.config >= (int)configs.size()) throw std::invalid_argument("serving: unknown configuration");
     if (r.nrhs() < 1 || r.nrhs() > 16) throw std::invalid_argument("serving: 1 to 16 right-hand sides");
     if (!r.sources.empty() && !r.gaussian_seeds.empty())
         throw std::invalid_argument("serving: give sources or Gaussian seeds, not both");
+    std::unique_ptr<Pending> p(new Pending);
+    p->arrival = now();
+    std::lock_guard<std::mutex> lk(mu);
+    if (r.config < 0 || r.config >= (int)configs.size()) throw std::invalid_argument("serving: unknown configuration");
     const size_t veo = (size_t)configs[r.config]->system->eo_sites();
     const size_t per = veo * (r.action == Action::twistedmass ? 24 : 6);
     for (const auto& s : r.sources)
         if (s.size() != per) throw std::invalid_argument("serving: source of the wrong size for its lattice");
-    std::unique_ptr<Pending> p(new Pending);
-    p->req     = std::move(r);
-    p->arrival = now();
-    std::lock_guard<std::mutex> lk(mu);
+    p->req = std::move(r);
     p->id = next_id++;
     const long id = p->id;
     queue.push_back(std::move(p));
@@ -211,7 +214,8 @@ long Engine::submit(Request r)
 
 /* FIFO admission: the head of the queue starts when a worker is idle and its work fields fit the HBM
  * budget (or nothing is running, so an oversized request cannot wait forever); queued requests that
- * share its operator are merged into the same solve while right-hand sides and memory allow. */
+ * share its operator are merged into the same solve while right-hand sides allow and the merged solve's
+ * work fields, next to those of the running solves, fit the budget.  Called with mu held. */
 void Engine::schedule_locked()
 {
     while (!queue.empty()) {
@@ -231,21 +235,28 @@ void Engine::schedule_locked()
         b->items.push_back(std::move(queue.front()));
         queue.pop_front();
         int rhs = b->items[0]->nrhs();
+        // memory of a merged solve of k right-hand sides (its padded count)
+        auto merged_bytes = [&](int k) {
+            Pending m;
+            m.req.config = b->items[0]->req.config;
+            m.req.action = b->items[0]->req.action;
+            m.req.gaussian_seeds.resize(k);
+            return work_bytes(m);
+        };
         for (auto it = queue.begin(); it != queue.end();) {
-            if (compatible(*b->items[0], **it) && rhs + (*it)->nrhs() <= opt.max_batch_rhs) {
-                rhs += (*it)->nrhs();
+            const int k = rhs + (*it)->nrhs();
+            // a candidate joins only while the merged solve still fits the budget next to the running ones
+            // (only a lone head may exceed it, and only with nothing running)
+            if (compatible(*b->items[0], **it) && k <= opt.max_batch_rhs &&
+                used_bytes + merged_bytes(k) <= opt.hbm_budget_bytes) {
+                rhs = k;
                 b->items.push_back(std::move(*it));
                 it = queue.erase(it);
             } else {
                 ++it;
             }
         }
-        // memory of the merged solve (its padded right-hand-side count)
-        Pending merged;
-        merged.req.config = b->items[0]->req.config;
-        merged.req.action = b->items[0]->req.action;
-        merged.req.gaussian_seeds.resize(rhs);
-        b->bytes    = work_bytes(merged);
+        b->bytes    = merged_bytes(rhs);
         b->admitted = now();
         used_bytes += b->bytes;
         peak_bytes = std::max(peak_bytes, used_bytes);
@@ -281,7 +292,13 @@ void Engine::worker_loop(int w)
 void Engine::run_batch(Worker& wk, Batch& b)
 {
     const Request& head = b.items[0]->req;
-    Config& cfg         = *configs[head.config];
+    Config* cfgp        = nullptr;
+    {
+        // configurations are owned by unique_ptr: the object stays put when the vector grows
+        std::lock_guard<std::mutex> lk(mu);
+        cfgp = configs[head.config].get();
+    }
+    Config& cfg = *cfgp;
     std::vector<Result> res(b.items.size());
     int total_rhs = 0;
     for (const auto& it : b.items) total_rhs += it->nrhs();

@@ -131,3 +131,24 @@ def test_gaussian_requests_and_errors():
     with pytest.raises(Exception):
         e.submit(config=c0, action="twistedmass", sources=[np.zeros((5, 24))])
     e.drain()
+
+
+def test_merge_respects_budget_next_to_running_solve():
+    """A budget that fits one running 1-RHS solve and a 2-RHS merged solve: queued compatible 1-RHS requests
+    behind a 1-RHS head join it only while the merged solve still fits next to the running one."""
+    from cl2qcd_amd import _host as H
+    veo = 8 * 8 * 8 * 8 // 2
+    per_rhs = veo * (6 * 192 + 6 * 96 + 2 * 192)  # the engine's twisted-mass work fields per padded RHS
+    budget = 3 * per_rhs
+    e = H.Engine(workers=2, hbm_budget_bytes=budget)
+    c0 = e.load_config(8, 8, 5.7, 5, 11)
+    e.warmup()
+    assert e.eo_sites(c0) == veo
+    first = e.submit(config=c0, action="twistedmass", kappa=0.16, mu=0.003, gaussian_seeds=[1])
+    rest = [e.submit(config=c0, action="twistedmass", kappa=0.15, mu=0.01, gaussian_seeds=[10 + k],
+                     return_solutions=False) for k in range(8)]
+    res = [e.wait(i) for i in [first] + rest]
+    assert all(r["ok"] and max(r["true_rel_residual"]) <= 1e-10 for r in res)
+    assert e.peak_work_bytes() <= budget
+    # the merged solves stayed within the 2 RHS that fit (12 would have fitted the RHS limit)
+    assert max(r["batch_rhs"] for r in res) <= 2
