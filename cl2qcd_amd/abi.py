@@ -101,6 +101,7 @@ SIGNATURES = {
     "cl2_sax_eo": (_i, [_vp, _vp, _vp, _i, _SP]),
     "cl2_saxsbypz_eo": (_i, [_vp, _vp, _vp, _vp, _vp, _i, _SP, _SP]),
     "cl2_scalar_product_eo": (_i, [_vp, _dp, _vp, _vp, _i]),
+    "cl2_scalar_product_eo_noalloc": (_i, [_vp, _dp, _vp, _vp, _i]),
     "cl2_squarenorm_eo": (_i, [_vp, _dp, _vp, _i]),
     "cl2_saxpy_AND_squarenorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),
     "cl2_saxpy_AND_cgnorm_eo": (_i, [_vp, _vp, _vp, _vp, _i, _SP, _dp]),

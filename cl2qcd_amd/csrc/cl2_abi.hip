@@ -20,6 +20,35 @@ namespace {
 std::string g_create_error;
 std::mutex g_create_mutex;
 
+/* Reduction scratch of cl2_scalar_product_eo_noalloc: one double2 partial sum per stage-1 block, at most
+ * BLAS_MAX_BLOCKS of them (1 MiB).  Every device the library runs on holds its own copy until the process
+ * ends; host code reaches it only through the per-device address that hipGetSymbolAddress returns. */
+constexpr unsigned NOALLOC_PARTIALS = BLAS_MAX_BLOCKS;
+__attribute__((used)) static __device__ __align__(256) double scratch[2 * NOALLOC_PARTIALS];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+std::mutex g_scratch_mutex;
+
+/* the device address of scratch on the current device (which is `device`), looked up once per device */
+hipError_t scratch_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    double* ptr     = nullptr;
+    const hipError_t e = hipGetSymbolAddress(reinterpret_cast<void**>(&ptr), HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (ptr == nullptr) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+double* scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mutex);
+    return (device >= 0 && device < NOALLOC_MAX_DEVICES) ? g_scratch_ptr[device] : nullptr;
+}
+
 int fail(cl2_ctx* c, int code, const std::string& msg)
 {
     if (c) c->err = msg;
@@ -381,6 +410,8 @@ int cl2_ctx_create(cl2_ctx** out, int device, const cl2_lattice* lat)
     if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)
         return create_fail(CL2_E_ARG, "no such HIP device");
     if (hipSetDevice(device) != hipSuccess) return create_fail(CL2_E_HIP, "hipSetDevice failed");
+    if (const hipError_t e = scratch_setup(device); e != hipSuccess)
+        return create_fail(CL2_E_HIP, std::string("hipGetSymbolAddress of the reduction scratch: ") + hipGetErrorString(e));
     cl2_ctx* c = new cl2_ctx();
     c->device  = device;
     c->lat     = *lat;
@@ -945,6 +976,22 @@ int cl2_scalar_product_eo(cl2_ctx* c, double* res, const void* x, const void* y,
 {
     if (!c) return CL2_E_ARG;
     return run_reduce(c, res, x, y, (size_t)12 * c->g.veo, prec, R_DOT, 1);
+}
+
+int cl2_scalar_product_eo_noalloc(cl2_ctx* c, double* res, const void* x, const void* y, int prec)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, valid_prec(prec), "cl2_scalar_product_eo_noalloc: FP64 or FP32 fields");
+    CL2_REQUIRE(c, res && x && y, "null pointer");
+    double* parts = scratch_ptr(c->device);
+    CL2_REQUIRE(c, parts != nullptr, "no reduction scratch on this device");
+    const size_t n    = (size_t)12 * c->g.veo;
+    const unsigned nb = reduce_stage1_blocks(n, prec, x, y);
+    CL2_REQUIRE(c, nb <= NOALLOC_PARTIALS, "cl2_scalar_product_eo_noalloc: more partial sums than the scratch holds");
+    unsigned written = 0;
+    CL2_HIP(c, launch_reduce_stage1(c, x, y, n, prec, R_DOT, &written, parts));
+    CL2_HIP(c, launch_reduce_stage2(c, written, res, 1, nullptr, 0, parts));
+    return CL2_OK;
 }
 
 int cl2_squarenorm_eo(cl2_ctx* c, double* res, const void* x, int prec)

@@ -218,25 +218,31 @@ __global__ __launch_bounds__(BLOCK) void k_reduce1(const void* x, const void* y,
     }
 }
 
-hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t n, int prec, int mode,
-                                unsigned* nblocks)
+static bool reduce_pairs(size_t n, int prec, const void* x, const void* y)
 {
-    if (prec == CL2_F64) {
-        const unsigned nb = chunk_blocks(n, BLAS_U_REDUCE);
-        *nblocks          = nb;
+    return prec != CL2_F64 && n % 2 == 0 && aligned16(x) && aligned16(y);
+}
+
+unsigned reduce_stage1_blocks(size_t n, int prec, const void* x, const void* y)
+{
+    return chunk_blocks(reduce_pairs(n, prec, x, y) ? n / 2 : n, BLAS_U_REDUCE);
+}
+
+hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t n, int prec, int mode,
+                                unsigned* nblocks, double* parts)
+{
+    if (!parts) parts = c->partials;
+    const unsigned nb = reduce_stage1_blocks(n, prec, x, y);
+    *nblocks          = nb;
+    if (prec == CL2_F64)
         hipLaunchKernelGGL((k_reduce1<double, 1, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, x, y, n,
-                           mode, c->partials);
-    } else if (n % 2 == 0 && aligned16(x) && aligned16(y)) {
-        const unsigned nb = chunk_blocks(n / 2, BLAS_U_REDUCE);
-        *nblocks          = nb;
+                           mode, parts);
+    else if (reduce_pairs(n, prec, x, y))
         hipLaunchKernelGGL((k_reduce1<float, 2, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, x, y, n,
-                           mode, c->partials);
-    } else {
-        const unsigned nb = chunk_blocks(n, BLAS_U_REDUCE);
-        *nblocks          = nb;
+                           mode, parts);
+    else
         hipLaunchKernelGGL((k_reduce1<float, 1, BLAS_BLOCK>), dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, x, y, n,
-                           mode, c->partials);
-    }
+                           mode, parts);
     return hipGetLastError();
 }
 
@@ -273,10 +279,10 @@ __global__ __launch_bounds__(BLOCK) void k_reduce2(const double* __restrict__ pa
 }
 
 hipError_t launch_reduce_stage2(cl2_ctx* c, unsigned nparts, double* res, int out_kind, const double* alpha_vec,
-                                int nvec)
+                                int nvec, const double* parts)
 {
     // 1024 threads: the elementwise kernels leave up to BLAS_MAX_BLOCKS partials
-    hipLaunchKernelGGL((k_reduce2<1024>), dim3(1), dim3(1024), 0, c->stream, c->partials, nparts, res,
+    hipLaunchKernelGGL((k_reduce2<1024>), dim3(1), dim3(1024), 0, c->stream, parts ? parts : c->partials, nparts, res,
                        out_kind, alpha_vec, nvec);
     return hipGetLastError();
 }

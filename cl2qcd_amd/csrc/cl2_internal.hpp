@@ -114,14 +114,17 @@ struct BlasArgs {
 hipError_t launch_blas(cl2_ctx* c, const BlasArgs& a, int prec, unsigned* nblocks_out);
 
 enum ReduceMode : int { R_DOT = 0, R_DOT_RE = 1, R_NORM = 2 };
-/* stage 1 of a two-stage reduction; returns the number of partials written */
+/* stage 1 of a two-stage reduction; returns the number of partials written (double2 each) to parts, or to
+ * the context's partials when parts is null */
 hipError_t launch_reduce_stage1(cl2_ctx* c, const void* x, const void* y, size_t n, int prec, int mode,
-                                unsigned* nblocks);
+                                unsigned* nblocks, double* parts = nullptr);
+/* partials launch_reduce_stage1 writes for these operands (at most BLAS_MAX_BLOCKS) */
+unsigned reduce_stage1_blocks(size_t n, int prec, const void* x, const void* y);
 /* stage 2: sum nparts double2 partials in fixed order.  out_kind 0: res[0] = real part;
  * 1: res[0..1] = complex sum;  2: res[k] = alpha_vec[k]^2 * real part for k < nvec;
  * 3: res[0..3] = (re, im, re, 0) */
 hipError_t launch_reduce_stage2(cl2_ctx* c, unsigned nparts, double* res, int out_kind, const double* alpha_vec,
-                                int nvec);
+                                int nvec, const double* parts = nullptr);
 
 hipError_t launch_gauge_update(cl2_ctx* c, void* gauge, double beta, int mode, uint64_t key);
 hipError_t launch_plaquette(cl2_ctx* c, const void* gauge, double* res);

@@ -214,6 +214,13 @@ int cl2_saxsbypz_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, const
                     const cl2_scalar* alpha, const cl2_scalar* beta);
 /* scalar_product_eoprec + reduction (spinors.cpp:670-713): res = sum conj(x) y, complex, on device */
 int cl2_scalar_product_eo(cl2_ctx* ctx, double* res, const void* x, const void* y, int prec);
+/* No counterpart in the reference: cl2_scalar_product_eo for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same result, bit
+ * for bit; FP64 or FP32 fields.  Its partial sums go to a library-owned 1 MiB device array (65536 partials,
+ * the most one reduction launch writes, whatever the field size) whose address is looked up once per device
+ * by cl2_ctx_create; the call itself allocates nothing.  Every call on one device shares that array, so
+ * calls that overlap on one device (different streams or contexts, concurrently) race. */
+int cl2_scalar_product_eo_noalloc(cl2_ctx* ctx, double* res, const void* x, const void* y, int prec);
 /* global_squarenorm_eoprec + reduction (spinors.cpp:715-789): res = sum |x|^2, real, on device */
 int cl2_squarenorm_eo(cl2_ctx* ctx, double* res, const void* x, int prec);
 /* saxpy_AND_squarenorm_eo (spinors.cpp:858-916): out = y - alpha x; res = |out|^2 (complex (norm, 0)) */
