@@ -61,21 +61,30 @@ def time_steps(ctx, fn, steps):
     return ms.value, (time.perf_counter() - t0) * 1e3
 
 
-def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src, autotune=True):
-    g = ctx.gauge(fmt=fmt, seed=20261202, epsilon=0.35)
+def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src, gauge18_ptr, autotune=True):
+    """FP64 / FP32 / FP16 hopping applications on the heatbath configuration whose 18-real links sit at
+    gauge18_ptr (the other link formats are converted from it on the device)"""
+    g = None
+    if fmt == abi.GAUGE_F64_18:
+        gptr = gauge18_ptr
+    else:
+        g = ctx.buffer(ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_GAUGE, fmt))
+        ctx.check(ctx.lib.cl2_gauge_convert(ctx.handle, g.ptr, fmt, ctypes.c_void_p(gauge18_ptr)))
+        gptr = g.ptr
     a, b = ctx.spinor(prec).upload(src), ctx.spinor(prec)
     kappa = 0.16
     # launch shape from the autotuner (z-planes per slab, XCD mapping, block size), timed on this field;
     # without it the library's default shape
     tuned = [ctypes.c_double(), ctypes.c_int(-1), ctypes.c_int(1), ctypes.c_int(1), ctypes.c_int(256), ctypes.c_int(0)]
     if autotune:
-        ctx.check(ctx.lib.cl2_autotune_dslash(ctx.handle, g.ptr, fmt, 1, *[ctypes.byref(t) for t in tuned]))
+        ctx.check(ctx.lib.cl2_autotune_dslash(ctx.handle, ctypes.c_void_p(gptr), fmt, 1,
+                                              *[ctypes.byref(t) for t in tuned]))
 
     def step(k):
         if k % 2 == 0:
-            ctx.dslash_eo(b.ptr, a.ptr, g.ptr, fmt, abi.EVEN, kappa)
+            ctx.dslash_eo(b.ptr, a.ptr, gptr, fmt, abi.EVEN, kappa)
         else:
-            ctx.dslash_eo(a.ptr, b.ptr, g.ptr, fmt, abi.ODD, kappa)
+            ctx.dslash_eo(a.ptr, b.ptr, gptr, fmt, abi.ODD, kappa)
 
     for k in range(warmup):
         step(k)
@@ -84,29 +93,47 @@ def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src, autotune=True):
     finite = bool(np.isfinite(out).all())
     a.close()
     b.close()
-    g.close()
+    if g is not None:
+        g.close()
     alg = ctx.lib.cl2_algorithmic_bytes(ctx.handle, b"dslash_eo", fmt)
     return dict(ms=ev_ms / steps, wall_ms=wall_ms / steps, alg_bytes=alg, GBps=alg * steps / (ev_ms * 1e6),
                 finite=finite, tuned={"zplanes": tuned[1].value, "xcd": tuned[2].value, "block": tuned[4].value,
                                       "occupancy": tuned[5].value, "autotuned": autotune})
 
 
-def cg_leg(ns, nt, sweeps, overrelax, seed):
-    """FP64 eo-CG (reference algorithm) vs FP32-inner reliable-update CG on QplusQminus_eo, both to
-    |r|/|b| <= 1e-10, through the C++ drop-in.  Twisted mass kappa 0.16, mu 0.005, antiperiodic in t, on a
-    heatbath beta 5.7 configuration made on the device from a cold start (seeded), Gaussian source."""
+def wilson_lattice(ns, nt, seed, extra=()):
     from cl2qcd_amd import _host as H
 
-    l = H.Lattice([f"--nSpace={ns}", f"--nTime={nt}", "--beta=5.7", "--fermionAction=twistedmass",
-                   "--kappa=0.16", "--mu=0.005", "--thetaFermionTemporal=1", "--solverMaxIterations=200000",
-                   f"--hostSeed={seed}"])
+    return H.Lattice([f"--nSpace={ns}", f"--nTime={nt}", "--beta=5.7", "--fermionAction=twistedmass",
+                      "--kappa=0.16", "--mu=0.005", "--thetaFermionTemporal=1", "--solverMaxIterations=200000",
+                      f"--hostSeed={seed}", *extra])
+
+
+def make_config(ns, nt, sweeps, overrelax, seed):
+    """The heatbath beta 5.7 configuration of the benchmark lattice, made on the device by the drop-in's
+    su3heatbath from a cold start (seeded): the links of the dslash legs and of the solver legs."""
+    from cl2qcd_amd import _host as H
+
+    l = wilson_lattice(ns, nt, seed)
     gf = H.Gaugefield(l, False)
     t0 = time.perf_counter()
     for _ in range(sweeps):
         H.su3heatbath(gf, l, overrelax)
     plaq = H.measurePlaquette(gf)
+    l.synchronize()
     hb_s = time.perf_counter() - t0
     log(f"heatbath: {sweeps} x (1 + {overrelax} OR) sweeps in {hb_s:.1f} s, plaquette {plaq:.6f}")
+    desc = f"beta 5.7, cold start, {sweeps} sweeps x (heatbath + {overrelax} overrelaxation), seed {seed}"
+    return l, gf, plaq, desc
+
+
+def cg_leg(ns, nt, l, gf, plaq, desc, seed):
+    """FP64 eo-CG (reference algorithm) vs FP32-inner reliable-update CG on QplusQminus_eo, both to
+    |r|/|b| <= 1e-10, through the C++ drop-in.  Twisted mass kappa 0.16, mu 0.005, antiperiodic in t, on a
+    heatbath beta 5.7 configuration made on the device from a cold start (seeded), Gaussian source.  The
+    plain FP64 CG runs with the reference's 18-real links and with 12-real links (useReconstruct12)."""
+    from cl2qcd_amd import _host as H
+
     b = H.Spinorfield_eo(l)
     b.import_(np.random.default_rng(seed).standard_normal((l.eo_sites, 24)))
     b2 = H.squarenorm(b)
@@ -123,9 +150,7 @@ def cg_leg(ns, nt, sweeps, overrelax, seed):
         H.saxpy(r, 1.0, r, b)  # r = b - A x
         return float(np.sqrt(H.squarenorm(r) / b2))
 
-    out = {"lattice": f"{ns}^3x{nt}", "heatbath": f"beta 5.7, cold start, {sweeps} sweeps x (heatbath + "
-                                                  f"{overrelax} overrelaxation), seed {seed}",
-           "plaquette": round(plaq, 6), "operator": "QplusQminus_eo twisted mass kappa 0.16 mu 0.005",
+    out = {"lattice": f"{ns}^3x{nt}", "heatbath": desc, "plaquette": round(plaq, 6), "operator": "QplusQminus_eo twisted mass kappa 0.16 mu 0.005",
            "target_rel_residual": 1e-10, "autotuned_dslash_ms": tuned_ms}
     x64 = H.Spinorfield_eo(l)
     x64.zero()
@@ -139,6 +164,31 @@ def cg_leg(ns, nt, sweeps, overrelax, seed):
     log(f"FP64 CG: {it} iterations, {dt:.2f} s, true |r|/|b| {out['fp64']['true_rel_residual']:.3e}")
     x64n = H.squarenorm(x64)
     d = H.Spinorfield_eo(l)
+    # the same plain FP64 CG with 12-real links (row 2 rebuilt in the kernel): a second FP64 baseline
+    l12 = wilson_lattice(ns, nt, seed, ("--useReconstruct12=true",))
+    gf12 = H.Gaugefield(l12, False)
+    gf12.convert_from(gf)
+    H.autotune_dslash(gf12, l12, H.F64, 1)
+    b12, x12 = H.Spinorfield_eo(l12), H.Spinorfield_eo(l12)
+    b12.import_(b.get_host())
+    x12.zero()
+    l12.synchronize()
+    t0 = time.perf_counter()
+    it12 = H.cg(x12, H.QplusQminus_eo(l12), gf12, b12, l12, 1e-20 * b2, l12.wilson_parameters())
+    l12.synchronize()
+    dt12 = time.perf_counter() - t0
+    x12h = x12.get_host()
+    del x12, b12, gf12, l12
+    x12d = H.Spinorfield_eo(l)
+    x12d.import_(x12h)
+    del x12h
+    H.saxpy(d, 1.0, x64, x12d)
+    out["fp64_12real_links"] = dict(iterations=it12, seconds=round(dt12, 3),
+                                    ms_per_iteration=round(1e3 * dt12 / max(it12, 1), 3),
+                                    true_rel_residual=true_rel_residual(x12d),
+                                    rel_diff_to_fp64_solution=float(np.sqrt(H.squarenorm(d) / x64n)))
+    del x12d
+    log(f"FP64 CG, 12-real links: {it12} iterations, {dt12:.2f} s")
     for name, prec in (("fp32", H.F32), ("fp16", H.F16)):
         x = H.Spinorfield_eo(l)
         x.zero()
@@ -152,7 +202,8 @@ def cg_leg(ns, nt, sweeps, overrelax, seed):
                          seconds=round(dt, 3), ms_per_iteration=round(1e3 * dt / max(st["iterations"], 1), 3),
                          true_rel_residual=true_rel_residual(x),
                          rel_diff_to_fp64_solution=float(np.sqrt(H.squarenorm(d) / x64n)),
-                         speedup_vs_fp64=round(out["fp64"]["seconds"] / dt, 3))
+                         speedup_vs_fp64=round(out["fp64"]["seconds"] / dt, 3),
+                         speedup_vs_fp64_12real_links=round(out["fp64_12real_links"]["seconds"] / dt, 3))
         log(f"{name}-inner CG: {st['iterations']} iterations ({st['reliable_updates']} reliable updates), "
             f"{dt:.2f} s, true |r|/|b| {out[name]['true_rel_residual']:.3e}, "
             f"speedup {out[name]['speedup_vs_fp64']:.2f}x")
@@ -413,6 +464,79 @@ def read_counters():
         return None
 
 
+def host_cpu():
+    """core count and model of the host this runs on"""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_count": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "model": model,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def config1_leg(seed=20261201, sweeps=50):
+    """The CPU-runnable workload exactly as written: 8^4, beta 5.7 heatbath (50 sweeps from a cold start,
+    the drop-in's su3heatbath, seeded), twisted mass kappa 0.15, mu 0.01, antiperiodic in t, FP64 eo CG on
+    QplusQminus_eo to |r|^2/|b|^2 <= 1e-20 from a seeded unit-variance Gaussian source.  Timed three ways on
+    the same inputs: the drop-in on the GPU, and the oracle (the reference's CG restated in C++ with
+    OpenMP, the reference's CPU execution mode) on all host cores and on one core."""
+    from cl2qcd_amd import _host as H
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    l = H.Lattice(["--nSpace=8", "--nTime=8", "--beta=5.7", f"--hostSeed={seed}", "--fermionAction=twistedmass",
+                   "--kappa=0.15", "--mu=0.01", "--thetaFermionTemporal=1", "--solverMaxIterations=5000"])
+    gf = H.Gaugefield(l, False)
+    for _ in range(sweeps):
+        H.su3heatbath(gf, l, 0)
+    g = np.ascontiguousarray(gf.get_host().reshape(-1, 18))
+    bh = np.random.default_rng(seed).standard_normal((l.eo_sites, 24))
+    b, x = H.Spinorfield_eo(l), H.Spinorfield_eo(l)
+    b.import_(bh)
+    x.cold()
+    x0 = x.get_host()
+    prec = 1e-20 * float((bh ** 2).sum())
+    qq, ap = H.QplusQminus_eo(l), l.wilson_parameters()
+    H.cg(x, qq, gf, b, l, prec, ap)  # warm: kernels loaded, launch shapes chosen
+    x.import_(x0)
+    l.synchronize()
+    t0 = time.perf_counter()
+    it = H.cg(x, qq, gf, b, l, prec, ap)
+    l.synchronize()
+    dt = time.perf_counter() - t0
+    lat = O.lattice(8, 8, theta_t=1.0)
+    mubar = l.mubar
+
+    def true_rel(sol):
+        r = O.saxpy(O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, sol, g, O.TWISTEDMASS, 0.15, mubar), bh, 1.0)
+        return float(np.sqrt(O.norm2(r) / O.norm2(bh)))
+
+    out = {"workload": "8^4, beta 5.7 heatbath (50 sweeps, cold start, seeded), twisted mass kappa 0.15 mu 0.01, "
+                       "antiperiodic t, FP64 eo CG on QplusQminus_eo to |r|^2/|b|^2 <= 1e-20, Gaussian source",
+           "plaquette": round(float(H.measurePlaquette(gf)), 6),
+           "gpu_dropin": {"iterations": it, "seconds": round(dt, 4), "ms_per_iteration": round(1e3 * dt / it, 4),
+                          "true_rel_residual": true_rel(x.get_host())}}
+    all_cores = O.threads()
+    for key, n in (("oracle_all_cores", all_cores), ("oracle_one_core", 1)):
+        O.set_threads(n)
+        O.cg_eo(lat, O.OP_QPLUSQMINUS_EO, g, bh, x0, O.TWISTEDMASS, 0.15, mubar, prec=prec, cgmax=5000)  # warm
+        t0 = time.perf_counter()
+        rc, it_o, xo = O.cg_eo(lat, O.OP_QPLUSQMINUS_EO, g, bh, x0, O.TWISTEDMASS, 0.15, mubar, prec=prec,
+                               cgmax=5000)
+        dto = time.perf_counter() - t0
+        out[key] = {"threads": n, "iterations": it_o, "converged": rc == O.OK, "seconds": round(dto, 4),
+                    "ms_per_iteration": round(1e3 * dto / max(it_o, 1), 4), "true_rel_residual": true_rel(xo)}
+    O.set_threads(all_cores)
+    out["gpu_speedup_vs_oracle_all_cores"] = round(out["oracle_all_cores"]["seconds"] / dt, 2)
+    log(f"config 1 (8^4): GPU drop-in {it} iterations {dt * 1e3:.1f} ms; oracle {out['oracle_all_cores']['seconds']:.3f} s "
+        f"on {all_cores} threads, {out['oracle_one_core']['seconds']:.3f} s on 1")
+    return out
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle's dslash_eo on host cores over a bounded sample (24^3x48, FP64)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -447,6 +571,8 @@ def main():
     ap.add_argument("--no-cg", action="store_true", help="skip the FP64 vs mixed-precision CG comparison")
     ap.add_argument("--hb-sweeps", type=int, default=50, help="heatbath sweeps for the CG configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-source block solve at 32^3x64")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the 8^4 CPU-reference workload (GPU drop-in and oracle CG)")
     ap.add_argument("--no-multishift", action="store_true", help="skip the 16-shift staggered solve at 48^3x12")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
     ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run")
@@ -472,6 +598,9 @@ def main():
 
     from cl2qcd_amd import abi
 
+    # the benchmark configuration: its 18-real links are the dslash legs' input and the solver legs' gauge field
+    cfg_seed = 20261202
+    l_cfg, gf_cfg, plaq_cfg, desc_cfg = make_config(args.ns, args.nt, args.hb_sweeps, args.hb_overrelax, cfg_seed)
     ctx = abi.Context(args.ns, args.nt, theta_t=1.0)
     src = np.random.default_rng(1).standard_normal((ctx.veo, 24))
     # counter calibration for rocprofv3 runs of this script: one pure read and one
@@ -493,13 +622,15 @@ def main():
     res.close()
     log(f"copy kernel: {copy_GBps:.0f} GB/s (read + write of one FP64 field)")
     log(f"lattice {args.ns}^3x{args.nt}, {ctx.veo} eo sites; warmup {args.warmup}, steps {args.steps}")
-    f64 = dslash_leg(ctx, abi, abi.GAUGE_F64_18, abi.F64, args.steps, args.warmup, src, not args.no_autotune)
+    f64 = dslash_leg(ctx, abi, abi.GAUGE_F64_18, abi.F64, args.steps, args.warmup, src, gf_cfg.device_ptr,
+                     not args.no_autotune)
     log(f"dslash F64/18: {f64['ms']:.4f} ms/app, {f64['GBps']:.0f} GB/s")
     extra = {}
     if not args.only_f64:
         for name, fmt, prec in (("f64_12", abi.GAUGE_F64_12, abi.F64), ("f32_12", abi.GAUGE_F32_12, abi.F32),
                                 ("f16_12", abi.GAUGE_F16_12, abi.F16)):
-            r = dslash_leg(ctx, abi, fmt, prec, max(args.steps // 4, 2), args.warmup, src, not args.no_autotune)
+            r = dslash_leg(ctx, abi, fmt, prec, max(args.steps // 4, 2), args.warmup, src, gf_cfg.device_ptr,
+                           not args.no_autotune)
             log(f"dslash {name}: {r['ms']:.4f} ms/app, {r['GBps']:.0f} GB/s")
             extra[f"dslash_{name}"] = dict(ms_per_app=round(r["ms"], 5), GBps=round(r["GBps"], 1),
                                            pct_hbm_peak=round(100 * r["GBps"] / HBM_PEAK_GBPS, 1),
@@ -510,7 +641,12 @@ def main():
 
     cg = None
     if not args.no_cg:
-        cg = cg_leg(args.ns, args.nt, args.hb_sweeps, args.hb_overrelax, 20261202)
+        cg = cg_leg(args.ns, args.nt, l_cfg, gf_cfg, plaq_cfg, desc_cfg, cfg_seed)
+    del gf_cfg, l_cfg
+
+    config1 = None
+    if not args.no_config1:
+        config1 = config1_leg()
 
     batch = None
     if not args.no_batch:
@@ -539,7 +675,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic: device-generated random SU(3) links (epsilon 0.35, seeded), seeded Gaussian spinor",
+        "data": f"synthetic: heatbath configuration made on the device ({desc_cfg}, plaquette {plaq_cfg:.6f}), "
+                "seeded Gaussian spinor",
         "config": {"workload": f"dslash_eo FP64 18-real links, {args.ns}^3x{args.nt}, kappa 0.16, antiperiodic t",
                    "lattice": f"{args.ns}^3x{args.nt}", "eo_sites": veo,
                    "bytes_per_site": 1536},
@@ -554,6 +691,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic},
         **extra,
         "cg": cg,
+        "config1": config1,
         "batch": batch,
         "multishift": multishift,
         "serving": serving,
@@ -561,6 +699,9 @@ def main():
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle on host cores) ...")
         line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"]["host"] = host_cpu()
+        if config1 is not None:
+            line["cpu_baseline"]["cg_config1"] = {k: config1[k] for k in ("oracle_all_cores", "oracle_one_core")}
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

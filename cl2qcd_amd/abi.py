@@ -138,6 +138,7 @@ SIGNATURES = {
     "cl2_graph_end": (_i, [_vp, ctypes.POINTER(_vp)]),
     "cl2_graph_launch": (_i, [_vp, _vp, _i]),
     "cl2_graph_destroy": (_i, [_vp]),
+    "cl2_graph_node_count": (_i, [_vp, _vp, _vp]),
     "cl2_cg_beta": (_i, [_vp, _dp, _dp, _dp]),
     "cl2_scalar_copy": (_i, [_vp, _dp, _dp, _i]),
     "cl2_slice_squarenorm_eo": (_i, [_vp, _dp, _vp, _i, _i, _i]),

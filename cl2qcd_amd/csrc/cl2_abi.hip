@@ -1555,6 +1555,24 @@ int cl2_graph_launch(cl2_ctx* c, cl2_graph* g, int times)
     return CL2_OK;
 }
 
+int cl2_graph_node_count(const cl2_graph* g, int* kernel_nodes, int* all_nodes)
+{
+    if (!g || !g->graph || !kernel_nodes || !all_nodes) return CL2_E_ARG;
+    size_t n = 0;
+    if (hipGraphGetNodes(g->graph, nullptr, &n) != hipSuccess) return CL2_E_HIP;
+    std::vector<hipGraphNode_t> nodes(n);
+    if (n && hipGraphGetNodes(g->graph, nodes.data(), &n) != hipSuccess) return CL2_E_HIP;
+    int k = 0;
+    for (size_t i = 0; i < n; ++i) {
+        hipGraphNodeType t;
+        if (hipGraphNodeGetType(nodes[i], &t) != hipSuccess) return CL2_E_HIP;
+        k += t == hipGraphNodeTypeKernel;
+    }
+    *kernel_nodes = k;
+    *all_nodes    = (int)n;
+    return CL2_OK;
+}
+
 int cl2_graph_destroy(cl2_graph* g)
 {
     if (!g) return CL2_OK;

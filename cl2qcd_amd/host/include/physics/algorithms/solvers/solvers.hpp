@@ -72,6 +72,8 @@ struct SolveStats {
     int iterations       = 0;
     int reliable_updates = 0;
     double true_relative_residual = 0.;  // |b - A x| / |b| recomputed in FP64 at exit
+    int graph_nodes        = 0;  // nodes of the captured inner iteration (0: not captured)
+    int graph_kernel_nodes = 0;  // of which kernel launches
 };
 
 /* CG with an FP64 reliable-update outer loop (solution accumulation, true residual,

@@ -72,6 +72,9 @@ class Gaugefield {
                std::string ildgfile);
     /* links from a host AoS array (link index mu + 4*site, row-major 3x3) */
     void import(const Matrixsu3* host) const;
+    /* ILDG/LIME file of the links with trajectory number `number` in its xlf-info record
+     * (physics/lattices/gaugefield.cpp:113-127; precision 64, or 32 if the run parameters say so) */
+    void save(const std::string& outputfile, int number) const;
     std::vector<Matrixsu3> get_host() const;
     /* device pointer and link format of the resident FP64 field */
     const void* dev() const noexcept { return buf->ptr(); }

@@ -70,6 +70,11 @@ class Engine {
     /* heatbath configuration of ns^3 x nt at beta (cold start, sweeps x (heatbath + overrelaxation),
      * antiperiodic fermion boundary in time), resident until the engine is destroyed; returns its id */
     int load_config(int ns, int nt, double beta, int sweeps, uint64_t seed);
+    /* the configuration of an ILDG/LIME file of an ns^3 x nt lattice, resident until the engine is
+     * destroyed; returns its id */
+    int load_config_file(int ns, int nt, const std::string& path);
+    /* checkpoint: resident configuration `config` to an ILDG/LIME file (with its plaquette and checksum) */
+    void save_config(int config, const std::string& path) const;
     double plaquette(int config) const;
     long eo_sites(int config) const;
     size_t resident_bytes() const;
@@ -92,6 +97,7 @@ class Engine {
     struct Pending;
     struct Batch;
     struct Worker;
+    int add_config(std::unique_ptr<Config> c);
     size_t work_bytes(const Pending& p) const;
     bool compatible(const Pending& a, const Pending& b) const;
     void schedule_locked();

@@ -8,6 +8,7 @@
 
 #include <cstring>
 
+#include "../include/ildg_io/ildgIo.hpp"
 #include "../include/physics/algorithms/inversion.hpp"
 #include "../include/physics/algorithms/rational_approximation.hpp"
 #include "../include/physics/algorithms/solvers/solvers.hpp"
@@ -125,7 +126,60 @@ PYBIND11_MODULE(_host, m)
         .def("import_", [](const Gaugefield& g, py::array_t<double> a) {
             auto v = from_numpy<Matrixsu3>(a, 8 * g.get_system().eo_sites());
             g.import(v.data());
+        })
+        .def("save", &Gaugefield::save, py::arg("outputfile"), py::arg("number"))
+        // plumbing for benchmarks: the resident field's device pointer and link format
+        .def_property_readonly("device_ptr", [](const Gaugefield& g) { return reinterpret_cast<uintptr_t>(g.dev()); })
+        .def_property_readonly("format", &Gaugefield::format)
+        // this field's links := src's (an 18-real field of the same lattice size on the same device), converted to
+        // this field's format on the device
+        .def("convert_from", [](Gaugefield& g, const Gaugefield& src) {
+            if (src.format() != CL2_GAUGE_F64_18) throw std::invalid_argument("convert_from: src must be 18-real");
+            if (src.get_system().eo_sites() != g.get_system().eo_sites())
+                throw std::invalid_argument("convert_from: lattices differ in size");
+            src.get_system().synchronize();
+            const hardware::System& sys = g.get_system();
+            sys.check(cl2_gauge_convert(sys.ctx(), g.dev_mutable(), g.format(), src.dev()));
+            sys.synchronize();
         });
+
+    // ILDG/LIME files on the host (no device needed)
+    m.def("ildg_checksum", [](py::bytes data, size_t nt, size_t ns) {
+        const std::string d = data;
+        const ildgIo::Checksum c = ildgIo::calculate_ildg_checksum(d.data(), d.size(), nt, ns);
+        return py::make_tuple(c.suma, c.sumb);
+    });
+    m.def(
+        "ildg_read",
+        [](const std::string& path, int ns, int nt, bool ignore_checksum) {
+            ildgIo::Sourcefileparameters info;
+            std::vector<Matrixsu3> links = ildgIo::readGaugefieldFromSourcefile(path, ns, nt, &info, ignore_checksum);
+            py::dict d;
+            d["precision"]    = info.precision;
+            d["has_checksum"] = info.has_checksum;
+            d["suma"]         = info.checksum.suma;
+            d["sumb"]         = info.checksum.sumb;
+            d["xlf_info"]     = info.xlf_info;
+            return py::make_tuple(to_numpy(links), d);
+        },
+        py::arg("path"), py::arg("ns"), py::arg("nt"), py::arg("ignore_checksum") = false);
+    m.def(
+        "ildg_write",
+        [](const std::string& path, py::array_t<double> links, int ns, int nt, int trajectory, double plaquette,
+           double beta, double kappa, double mu, double mubar, int precision) {
+            const size_t vol = (size_t)ns * ns * ns * nt;
+            ildgIo::XlfInfo info;
+            info.trajectory = trajectory;
+            info.plaquette  = plaquette;
+            info.beta       = beta;
+            info.kappa      = kappa;
+            info.mu         = mu;
+            info.mubar      = mubar;
+            ildgIo::writeGaugefieldToFile(path, from_numpy<Matrixsu3>(links, 4 * vol), ns, nt, info, precision);
+        },
+        py::arg("path"), py::arg("links"), py::arg("ns"), py::arg("nt"), py::arg("trajectory") = 0,
+        py::arg("plaquette") = 0., py::arg("beta") = 0., py::arg("kappa") = 0., py::arg("mu") = 0.,
+        py::arg("mubar") = 0., py::arg("precision") = 64);
 
     m.def(
         "su3heatbath", [](Gaugefield& gf, Lattice& l, int overrelax) { physics::algorithms::su3heatbath(gf, l.prng, overrelax); },
@@ -326,6 +380,8 @@ PYBIND11_MODULE(_host, m)
             py::dict d;
             d["iterations"]             = st.iterations;
             d["reliable_updates"]       = st.reliable_updates;
+            d["graph_nodes"]            = st.graph_nodes;
+            d["graph_kernel_nodes"]     = st.graph_kernel_nodes;
             d["true_relative_residual"] = st.true_relative_residual;
             return d;
         },
@@ -354,6 +410,8 @@ PYBIND11_MODULE(_host, m)
                 py::dict d;
                 d["iterations"]             = s.iterations;
                 d["reliable_updates"]       = s.reliable_updates;
+                d["graph_nodes"]            = s.graph_nodes;
+                d["graph_kernel_nodes"]     = s.graph_kernel_nodes;
                 d["true_relative_residual"] = s.true_relative_residual;
                 per.append(d);
             }
@@ -392,6 +450,8 @@ PYBIND11_MODULE(_host, m)
                 py::dict d;
                 d["iterations"]             = s.iterations;
                 d["reliable_updates"]       = s.reliable_updates;
+                d["graph_nodes"]            = s.graph_nodes;
+                d["graph_kernel_nodes"]     = s.graph_kernel_nodes;
                 d["true_relative_residual"] = s.true_relative_residual;
                 per.append(d);
             }
@@ -486,6 +546,8 @@ PYBIND11_MODULE(_host, m)
             py::dict d;
             d["iterations"]             = st.iterations;
             d["reliable_updates"]       = st.reliable_updates;
+            d["graph_nodes"]            = st.graph_nodes;
+            d["graph_kernel_nodes"]     = st.graph_kernel_nodes;
             d["true_relative_residual"] = st.true_relative_residual;
             return d;
         },
@@ -543,6 +605,8 @@ PYBIND11_MODULE(_host, m)
                 py::dict d;
                 d["iterations"]             = s.iterations;
                 d["reliable_updates"]       = s.reliable_updates;
+                d["graph_nodes"]            = s.graph_nodes;
+                d["graph_kernel_nodes"]     = s.graph_kernel_nodes;
                 d["true_relative_residual"] = s.true_relative_residual;
                 per.append(d);
             }
@@ -576,6 +640,8 @@ PYBIND11_MODULE(_host, m)
                 py::dict d;
                 d["iterations"]             = s.iterations;
                 d["reliable_updates"]       = s.reliable_updates;
+                d["graph_nodes"]            = s.graph_nodes;
+                d["graph_kernel_nodes"]     = s.graph_kernel_nodes;
                 d["true_relative_residual"] = s.true_relative_residual;
                 per.append(d);
             }
@@ -612,6 +678,20 @@ PYBIND11_MODULE(_host, m)
                 return e.load_config(ns, nt, beta, sweeps, seed);
             },
             py::arg("ns"), py::arg("nt"), py::arg("beta"), py::arg("sweeps"), py::arg("seed"))
+        .def(
+            "load_config_file",
+            [](serving::Engine& e, int ns, int nt, const std::string& path) {
+                py::gil_scoped_release rel;
+                return e.load_config_file(ns, nt, path);
+            },
+            py::arg("ns"), py::arg("nt"), py::arg("path"))
+        .def(
+            "save_config",
+            [](const serving::Engine& e, int config, const std::string& path) {
+                py::gil_scoped_release rel;
+                e.save_config(config, path);
+            },
+            py::arg("config"), py::arg("path"))
         .def("plaquette", &serving::Engine::plaquette)
         .def("eo_sites", &serving::Engine::eo_sites)
         .def("resident_bytes", &serving::Engine::resident_bytes)

@@ -7,6 +7,7 @@
 #include <iterator>
 #include <stdexcept>
 
+#include "../include/ildg_io/ildgIo.hpp"
 #include "../include/physics/lattices/lattices.hpp"
 
 namespace physics {
@@ -194,65 +195,30 @@ const void* Gaugefield::dev_f16() const
     return reduced_copy(buf_f16, CL2_GAUGE_F16_12);
 }
 
-/* ILDG/LIME reader (ildg_io/ildgIo_gaugefield.cpp:142-221, ildg_io/lime/limeFileReader.cpp:56-130):
- * records of a 144-byte header (magic 0x456789ab, version, flags, 64-bit length, type) and
- * payload padded to 8 bytes; the ildg-binary-data payload holds [t][z][y][x][mu_ildg][3][3]
- * big-endian complex numbers with mu_ildg = x,y,z,t, internal mu = (mu_ildg + 1) % 4. */
+/* links of an ILDG/LIME file (ildg_io/ildgIo_gaugefield.cpp:44-67, 142-221), checksum verified */
 void Gaugefield::initializeFromILDGSourcefile(const std::string& file)
 {
-    std::ifstream in(file, std::ios::binary);
-    if (!in) throw std::invalid_argument("cannot open gauge configuration " + file);
-    std::vector<unsigned char> d((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
-    auto be = [&](size_t off, int n) {
-        uint64_t v = 0;
-        for (int i = 0; i < n; ++i) v = (v << 8) | d[off + i];
-        return v;
-    };
-    const int ns = system.get_inputparameters().get_nspace(), nt = system.get_inputparameters().get_ntime();
-    size_t off = 0, data = 0, len_data = 0;
-    int prec   = 64;
-    while (off + 144 <= d.size()) {
-        if (be(off, 4) != 0x456789abu) throw std::runtime_error(file + ": not a LIME file");
-        const uint64_t len = be(off + 8, 8);
-        const std::string type(reinterpret_cast<const char*>(&d[off + 16]),
-                               strnlen(reinterpret_cast<const char*>(&d[off + 16]), 128));
-        const size_t payload = off + 144;
-        if (payload + len > d.size()) throw std::runtime_error(file + ": truncated LIME record");
-        if (type == "ildg-format") {
-            const std::string xml(reinterpret_cast<const char*>(&d[payload]), len);
-            auto field = [&](const std::string& tag) {
-                size_t q = xml.find("<" + tag + ">");
-                return q == std::string::npos ? -1 : std::atoi(xml.c_str() + q + tag.size() + 2);
-            };
-            prec = field("precision");
-            if (field("lx") != ns || field("ly") != ns || field("lz") != ns || field("lt") != nt)
-                throw std::invalid_argument(file + ": lattice extents differ from the run parameters");
-        } else if (type == "ildg-binary-data") {
-            data     = payload;
-            len_data = len;
-        }
-        off = payload + ((len + 7) / 8) * 8;
+    const meta::Inputparameters& p = system.get_inputparameters();
+    const std::vector<Matrixsu3> host = ildgIo::readGaugefieldFromSourcefile(file, p.get_nspace(), p.get_ntime());
+    system.check(cl2_gauge_import(system.ctx(), buf->ptr(), reinterpret_cast<const double*>(host.data()), fmt));
+}
+
+void Gaugefield::save(const std::string& outputfile, int number) const
+{
+    const meta::Inputparameters& p = system.get_inputparameters();
+    ildgIo::XlfInfo info;
+    info.trajectory = number;
+    info.beta       = p.get_beta();
+    info.kappa      = p.get_kappa();
+    info.mu         = p.get_mu();
+    info.mubar      = meta::get_mubar(p);
+    std::vector<Matrixsu3> host = get_host();
+    if (fmt == CL2_GAUGE_F64_18) {
+        double plaq = 0.;
+        system.check(cl2_gauge_plaquette(system.ctx(), buf->ptr(), &plaq));
+        info.plaquette = plaq;
     }
-    const size_t vol = (size_t)ns * ns * ns * nt, esz = prec == 32 ? 4 : 8;
-    if (len_data != vol * 4 * 18 * esz) throw std::runtime_error(file + ": ildg-binary-data has the wrong size");
-    std::vector<double> host(vol * 4 * 18);
-    for (size_t s = 0; s < vol; ++s)
-        for (int l = 0; l < 4; ++l)
-            for (int k = 0; k < 18; ++k) {
-                const size_t src = data + ((s * 4 + l) * 18 + k) * esz;
-                double v;
-                if (esz == 8) {
-                    uint64_t b = be(src, 8);
-                    std::memcpy(&v, &b, 8);
-                } else {
-                    uint32_t b = (uint32_t)be(src, 4);
-                    float f;
-                    std::memcpy(&f, &b, 4);
-                    v = f;
-                }
-                host[((l + 1) % 4 + 4 * s) * 18 + k] = v;
-            }
-    system.check(cl2_gauge_import(system.ctx(), buf->ptr(), host.data(), fmt));
+    ildgIo::writeGaugefieldToFile(outputfile, host, p.get_nspace(), p.get_ntime(), info, p.get_precision());
 }
 
 /* ---------------- spinor fields ---------------- */

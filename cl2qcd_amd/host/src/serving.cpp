@@ -135,8 +135,28 @@ int Engine::load_config(int ns, int nt, double beta, int sweeps, uint64_t seed)
     c->handler.reset(new InterfacesHandlerImplementation(*c->params));
     c->gf.reset(new Gaugefield(*c->system, &c->handler->getInterface<Gaugefield>(), *c->prng, false));
     for (int s = 0; s < sweeps; ++s) physics::algorithms::su3heatbath(*c->gf, *c->prng, 1);
+    return add_config(std::move(c));
+}
+
+int Engine::load_config_file(int ns, int nt, const std::string& path)
+{
+    std::unique_ptr<Config> c(new Config);
+    c->ns = ns;
+    c->nt = nt;
+    c->params.reset(new meta::Inputparameters(lattice_args(ns, nt, "twistedmass")));
+    c->system.reset(new hardware::System(*c->params));
+    c->prng_params.reset(new PrngParametersImplementation(*c->params));
+    c->prng.reset(new PRNG(*c->system, c->prng_params.get()));
+    c->handler.reset(new InterfacesHandlerImplementation(*c->params));
+    c->gf.reset(new Gaugefield(*c->system, &c->handler->getInterface<Gaugefield>(), *c->prng, path));
+    return add_config(std::move(c));
+}
+
+/* measures the plaquette, makes the reduced-precision link copies (so workers only read them) and makes
+ * the configuration resident */
+int Engine::add_config(std::unique_ptr<Config> c)
+{
     c->plaquette = physics::observables::measurePlaquette(c->gf.get());
-    // reduced-precision link copies are made now, so workers only read them
     c->gf->dev_f32();
     if (opt.inner_precision == CL2_F16) c->gf->dev_f16();
     c->system->synchronize();
@@ -145,6 +165,17 @@ int Engine::load_config(int ns, int nt, double beta, int sweeps, uint64_t seed)
     std::lock_guard<std::mutex> lk(mu);
     configs.push_back(std::move(c));
     return (int)configs.size() - 1;
+}
+
+void Engine::save_config(int config, const std::string& path) const
+{
+    const Config* c = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        c = configs.at(config).get();
+    }
+    // workers only read a resident configuration, so it can be written while they run
+    c->gf->save(path, 0);
 }
 
 double Engine::plaquette(int config) const

@@ -381,6 +381,7 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
         system.check(cl2_cg_beta(c, mbeta.dev(), r2c.dev(), norms.dev()));
         cg_update_xp(&x_lo, &p, r_lo, malpha, mbeta);
     };
+    int graph_nodes = 0, graph_kernel_nodes = 0;  // census of the captured iteration
     struct Graph {
         cl2_graph* g = nullptr;
         ~Graph() { cl2_graph_destroy(g); }
@@ -396,14 +397,21 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
             throw;
         }
         system.check(cl2_graph_end(c, &graph.g));
+        system.check(cl2_graph_node_count(graph.g, &graph_kernel_nodes, &graph_nodes));
+        // replays are checked against eager execution for kernel launches only (memset nodes have replayed
+        // wrongly on this hardware): a capture that recorded anything else is a bug here
+        if (graph_nodes != graph_kernel_nodes) throw std::logic_error("captured solver iteration holds non-kernel nodes");
     }
     hmc_float rmax = std::sqrt(r2);
-    int k          = 0;
+    hmc_float rtrue = r2;  // |b - A x|^2 of the FP64 solution
+    bool pending    = false;  // x_lo holds a correction not yet added to x
+    int k           = 0;
     for (k = 0; k < opt.max_iterations; ++k) {
         if (graph.g)
             system.check(cl2_graph_launch(c, graph.g, 1));
         else
             body();
+        pending                         = true;
         const std::vector<hmc_float> nr = norms.get();
         hmc_float rn                    = nr[0];
         if (rn != rn) throw SolverStuck(k, "cg_reliable_update: NAN occured");
@@ -411,7 +419,9 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
             convert_precision(&t64, x_lo);
             saxpy(x, hmc_complex{-1., 0.}, t64, *x);  // x += x_lo
             x_lo.zero();
-            rn = true_residual();
+            pending = false;
+            rn      = true_residual();
+            rtrue   = rn;
             ++st.reliable_updates;
             if (rn <= target) {
                 ++k;
@@ -429,8 +439,15 @@ int cg_reliable_update(const Spinorfield_eo* x, const fermionmatrix::Fermionmatr
         }
         rmax = std::max(rmax, std::sqrt(rn));
     }
-    st.iterations = k;
-    st.true_relative_residual = std::sqrt(squarenorm(r) / b2);
+    if (pending) {  // stopped by the iteration limit: keep the correction gathered since the last update
+        convert_precision(&t64, x_lo);
+        saxpy(x, hmc_complex{-1., 0.}, t64, *x);
+        rtrue = true_residual();
+    }
+    st.iterations             = k;
+    st.true_relative_residual = std::sqrt(rtrue / b2);
+    st.graph_nodes            = graph_nodes;
+    st.graph_kernel_nodes     = graph_kernel_nodes;
     if (stats) *stats = st;
     if (st.true_relative_residual > rel_tol) throw SolverDidNotSolve(k, "cg_reliable_update");
     return k;
@@ -601,6 +618,7 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         // x += alpha p and p = r + beta p in one pass
         system.check(Ops::xp(c, Xlo.dev(), P.dev(), Rlo.dev(), lo, n, alpha.dev(), beta.dev()));
     };
+    int graph_nodes = 0, graph_kernel_nodes = 0;  // census of the captured iteration
     struct Graph {
         cl2_graph* g = nullptr;
         ~Graph() { cl2_graph_destroy(g); }
@@ -616,12 +634,18 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
             throw;
         }
         system.check(cl2_graph_end(c, &graph.g));
+        system.check(cl2_graph_node_count(graph.g, &graph_kernel_nodes, &graph_nodes));
+        // replays are checked against eager execution for kernel launches only (memset nodes have replayed
+        // wrongly on this hardware): a capture that recorded anything else is a bug here
+        if (graph_nodes != graph_kernel_nodes) throw std::logic_error("captured solver iteration holds non-kernel nodes");
     }
+    bool pending = false;  // Xlo holds corrections not yet added to X
     for (k = 0; k < opt.max_iterations && n_active > 0; ++k) {
         if (graph.g)
             system.check(cl2_graph_launch(c, graph.g, 1));
         else
             body();
+        pending                         = true;
         const std::vector<hmc_float> nr = norms.get();
         bool update                     = false;
         for (int r = 0; r < n; ++r) {
@@ -635,7 +659,8 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         system.check(Ops::convert(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
         system.check(Ops::axpy(c, X.dev(), T.dev(), X.dev(), CL2_F64, n, ones.dev(), 1., nullptr));
         Xlo.zero();
-        rn = true_residual();
+        pending = false;
+        rn      = true_residual();
         ++reliable;
         std::vector<hmc_float> oc(2 * n, 0.);
         for (int r = 0; r < n; ++r) {
@@ -674,7 +699,16 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         r2.store(r2h);
         active.store(act);
     }
-    for (int r = 0; r < n; ++r) st[r].reliable_updates = reliable;
+    if (pending) {  // stopped by the iteration limit: keep the corrections gathered since the last update
+        system.check(Ops::convert(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
+        system.check(Ops::axpy(c, X.dev(), T.dev(), X.dev(), CL2_F64, n, ones.dev(), 1., nullptr));
+        rn = true_residual();
+    }
+    for (int r = 0; r < n; ++r) {
+        st[r].reliable_updates   = reliable;
+        st[r].graph_nodes        = graph_nodes;
+        st[r].graph_kernel_nodes = graph_kernel_nodes;
+    }
     const bool done = n_active == 0;
     finish(k);
     if (!done && !partial) throw SolverDidNotSolve(k, "cg_reliable_update_block");
@@ -867,6 +901,7 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
         add(&r2, rn_s, zero);                       // r2 = |r_new|^2
         cg_update_xp(&x_lo, &p, r_lo, malpha, mbeta);  // x_lo += alpha p, p = r_lo + beta p
     };
+    int graph_nodes = 0, graph_kernel_nodes = 0;  // census of the captured iteration
     struct Graph {
         cl2_graph* g = nullptr;
         ~Graph() { cl2_graph_destroy(g); }
@@ -882,21 +917,30 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
             throw;
         }
         system.check(cl2_graph_end(c, &graph.g));
+        system.check(cl2_graph_node_count(graph.g, &graph_kernel_nodes, &graph_nodes));
+        // replays are checked against eager execution for kernel launches only (memset nodes have replayed
+        // wrongly on this hardware): a capture that recorded anything else is a bug here
+        if (graph_nodes != graph_kernel_nodes) throw std::logic_error("captured solver iteration holds non-kernel nodes");
     }
-    hmc_float rmax = std::sqrt(rn);
-    int k          = 0;
+    hmc_float rmax  = std::sqrt(rn);
+    hmc_float rtrue = rn;     // |b - (A + sigma) x|^2 of the FP64 solution
+    bool pending    = false;  // x_lo holds a correction not yet added to x
+    int k           = 0;
     for (k = 0; k < opt.max_iterations; ++k) {
         if (graph.g)
             system.check(cl2_graph_launch(c, graph.g, 1));
         else
             body();
-        rn = rn_s.get();
+        pending = true;
+        rn      = rn_s.get();
         if (rn != rn) throw SolverStuck(k, "cg_reliable_update_stagg: NAN occured");
         if (std::sqrt(rn) < opt.delta * rmax || rn <= target) {
             convert_precision(&t64, x_lo);
             saxpy(x, 1., t64, *x);  // x += x_lo
             x_lo.set_zero();
-            rn = true_residual();
+            pending = false;
+            rn      = true_residual();
+            rtrue   = rn;
             ++st.reliable_updates;
             if (rn <= target) {
                 ++k;
@@ -914,8 +958,15 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
         }
         rmax = std::max(rmax, std::sqrt(rn));
     }
+    if (pending) {  // stopped by the iteration limit: keep the correction gathered since the last update
+        convert_precision(&t64, x_lo);
+        saxpy(x, 1., t64, *x);
+        rtrue = true_residual();
+    }
     st.iterations             = k;
-    st.true_relative_residual = std::sqrt(squarenorm(r) / b2);
+    st.true_relative_residual = std::sqrt(rtrue / b2);
+    st.graph_nodes            = graph_nodes;
+    st.graph_kernel_nodes     = graph_kernel_nodes;
     if (stats) *stats = st;
     if (st.true_relative_residual > rel_tol) throw SolverDidNotSolve(k, "cg_reliable_update_stagg");
     return k;
@@ -1001,6 +1052,7 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
         copyData(&zeta, zeta_foll);
         copyData(&tmp1, tmp2);
     };
+    int graph_nodes = 0, graph_kernel_nodes = 0;  // census of the captured iteration
     struct Graph {
         cl2_graph* g = nullptr;
         ~Graph() { cl2_graph_destroy(g); }
@@ -1016,6 +1068,10 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
             throw;
         }
         system.check(cl2_graph_end(c, &graph.g));
+        system.check(cl2_graph_node_count(graph.g, &graph_kernel_nodes, &graph_nodes));
+        // replays are checked against eager execution for kernel launches only (memset nodes have replayed
+        // wrongly on this hardware): a capture that recorded anything else is a bug here
+        if (graph_nodes != graph_kernel_nodes) throw std::logic_error("captured solver iteration holds non-kernel nodes");
     }
     int n_active = n, iter = 0;
     while (iter < opt.max_iterations && n_active > 0) {

@@ -405,6 +405,10 @@ int cl2_graph_begin(cl2_ctx* ctx);
 int cl2_graph_end(cl2_ctx* ctx, cl2_graph** graph);
 int cl2_graph_launch(cl2_ctx* ctx, cl2_graph* graph, int times);
 int cl2_graph_destroy(cl2_graph* graph);
+/* No counterpart in the reference: node census of a captured sequence -- its kernel nodes and all its
+ * nodes -- so a caller can check that a capture recorded kernel launches only (no memset, copy or host
+ * nodes, which replay differently). */
+int cl2_graph_node_count(const cl2_graph* graph, int* kernel_nodes, int* all_nodes);
 /* copyData between device scalars / vectors (physics/lattices/scalar.hpp, vector.hpp): n doubles, done by a
  * kernel so that it can be part of a recorded sequence */
 int cl2_scalar_copy(cl2_ctx* ctx, double* dst, const double* src, int n);

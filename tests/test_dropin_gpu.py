@@ -295,6 +295,54 @@ def test_cg_reliable_update(action, op):
     assert np.sqrt(((xh - x64h) ** 2).sum() / (x64h ** 2).sum()) < 1e-6
 
 
+@pytest.mark.parametrize("solver", ["single", "block", "stagg"])
+def test_reliable_update_iteration_limit_keeps_the_correction(solver):
+    """Stopped by the iteration limit before any reliable update (delta 1e-6, 20 iterations), the
+    reduced-precision solvers still add the correction gathered so far to the FP64 solution: x (zero at the
+    start) reduces the true residual, recomputed by the oracle's FP64 operator, and SolverDidNotSolve is
+    raised."""
+    l = lattice("--nSpace=8", "--nTime=8", "--fermionAction=twistedmass", "--kappa=0.15", "--mu=0.02",
+                "--beta=5.7")
+    gf = H.Gaugefield(l, False)
+    for _ in range(3):
+        H.su3heatbath(gf, l, 1)
+    g = gf.get_host()
+    lat = O.lattice(8, 8)
+    opts = dict(delta=1e-6, max_iterations=20)
+    if solver == "stagg":
+        ls = lattice("--nSpace=8", "--nTime=8", "--fermionAction=rooted_stagg", "--mass=0.1")
+        gs = H.Gaugefield(ls, False)
+        gs.import_(g)
+        bh = np.random.default_rng(53).standard_normal((ls.eo_sites, 6))
+        b, x = H.Staggeredfield_eo(ls), H.Staggeredfield_eo(ls)
+        b.import_(bh)
+        x.set_zero()
+        with pytest.raises(H.SolverDidNotSolve):
+            H.cg_reliable_update_stagg(x, H.MdagM_eo(ls), gs, 0.0, b, ls, 1e-10, ls.staggered_parameters(), **opts)
+        xs = [x.get_host()]
+        res = [O.norm2(O.stagg_saxpby(O.mdagm_eo(lat, xs[0], g, 0.1), bh, -1.0, 1.0)) / O.norm2(bh)]
+    else:
+        A = H.QplusQminus_eo(l)
+        bs = [eo_pair(l, 61 + r)[0] for r in range(2 if solver == "block" else 1)]
+        xs_f = [H.Spinorfield_eo(l) for _ in bs]
+        for x in xs_f:
+            x.zero()
+        with pytest.raises(H.SolverDidNotSolve):
+            if solver == "block":
+                H.cg_reliable_update_block(xs_f, A, gf, bs, l, 1e-10, l.wilson_parameters(), **opts)
+            else:
+                H.cg_reliable_update(xs_f[0], A, gf, bs[0], l, 1e-10, l.wilson_parameters(), **opts)
+        xs, res = [], []
+        for x, b in zip(xs_f, bs):
+            xh, bh = x.get_host(), b.get_host()
+            xs.append(xh)
+            r = O.saxpy(O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, xh, g, O.TWISTEDMASS, 0.15, l.mubar), bh, 1.0)
+            res.append(O.norm2(r) / O.norm2(bh))
+    for xh, rr in zip(xs, res):
+        assert np.abs(xh).max() > 0, "the correction of the last iterations was dropped"
+        assert rr < 0.5, rr
+
+
 def test_cg_reliable_update_half():
     """FP16 (per-site scale) inner CG with FP64 reliable updates reaches |b - A x| / |b| <= 1e-10 in FP64."""
     l = lattice("--nSpace=8", "--nTime=8", "--fermionAction=twistedmass", "--kappa=0.15", "--mu=0.02", "--beta=5.7")
@@ -329,6 +377,8 @@ def test_reliable_update_graph_replays_match_eager(prec):
         st = H.cg_reliable_update(x, A, gf, b, l, 1e-10, l.wilson_parameters(), getattr(H, prec),
                                   use_graph=use_graph)
         assert st["true_relative_residual"] <= 1e-10
+        # the captured iteration is kernel launches only
+        assert st["graph_nodes"] == st["graph_kernel_nodes"] and (st["graph_nodes"] > 0) == use_graph
         out.append((st["iterations"], x.get_host()))
     for it, xh in out[1:]:
         assert it == out[0][0] and np.array_equal(xh, out[0][1])
@@ -353,6 +403,7 @@ def test_block_and_staggered_graph_replays_match_eager():
             x.zero()
         it, per = H.cg_reliable_update_block(xs, A, gf, bs, l, 1e-10, l.wilson_parameters(), use_graph=use_graph)
         assert max(p["true_relative_residual"] for p in per) <= 1e-10
+        assert all(p["graph_nodes"] == p["graph_kernel_nodes"] and (p["graph_nodes"] > 0) == use_graph for p in per)
         res.append((it, [x.get_host() for x in xs]))
     assert res[0][0] == res[1][0] and all(np.array_equal(a, b) for a, b in zip(res[0][1], res[1][1]))
     ls = lattice("--nTime=4", "--fermionAction=rooted_stagg", "--mass=0.05")
@@ -367,5 +418,6 @@ def test_block_and_staggered_graph_replays_match_eager():
         st = H.cg_reliable_update_stagg(x, M, g2, 0.01, bst, ls, 1e-10, ls.staggered_parameters(),
                                         use_graph=use_graph)
         assert st["true_relative_residual"] <= 1e-10
+        assert st["graph_nodes"] == st["graph_kernel_nodes"] and (st["graph_nodes"] > 0) == use_graph
         sres.append((st["iterations"], x.get_host()))
     assert sres[0][0] == sres[1][0] and np.array_equal(sres[0][1], sres[1][1])

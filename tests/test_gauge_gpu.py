@@ -99,3 +99,32 @@ def test_dropin_heatbath():
     l12 = H.Lattice(["--nTime=4", "--useReconstruct12=true"])
     with pytest.raises(Exception):
         H.su3heatbath(H.Gaugefield(l12, False), l12)
+
+
+def test_save_and_reload_ildg(tmp_path):
+    """Gaugefield.save (physics/lattices/gaugefield.cpp:113-127) of a heatbath configuration, read back by the
+    ILDG constructor: bit-identical links and the same plaquette; the serving engine checkpoints a resident
+    configuration the same way and loads it back as a new resident one."""
+    from cl2qcd_amd import _host as H
+    l = H.Lattice(["--nSpace=8", "--nTime=8", "--beta=5.7", "--hostSeed=77"])
+    gf = H.Gaugefield(l, False)
+    for _ in range(5):
+        H.su3heatbath(gf, l, 1)
+    path = str(tmp_path / "conf.hb")
+    gf.save(path, 5)
+    back = H.Gaugefield(l, path)
+    assert np.array_equal(back.get_host(), gf.get_host())
+    assert H.measurePlaquette(back) == H.measurePlaquette(gf)
+    links, info = H.ildg_read(path, 8, 8)
+    assert "trajectory nr = 5" in info["xlf_info"] and info["has_checksum"]
+    lat = O.lattice(8, 8)
+    assert abs(O.plaquette(lat, np.ascontiguousarray(links.reshape(-1, 18))) - H.measurePlaquette(gf)) < 1e-12
+    e = H.Engine(workers=1)
+    c = e.load_config(8, 8, 5.7, 3, 11)
+    ck = str(tmp_path / "conf.engine")
+    e.save_config(c, ck)
+    c2 = e.load_config_file(8, 8, ck)
+    assert e.plaquette(c2) == e.plaquette(c)
+    a = e.wait(e.submit(config=c, action="twistedmass", kappa=0.15, mu=0.01, gaussian_seeds=[3]))
+    b = e.wait(e.submit(config=c2, action="twistedmass", kappa=0.15, mu=0.01, gaussian_seeds=[3]))
+    assert a["ok"] and b["ok"] and np.array_equal(a["solutions"][0], b["solutions"][0])
