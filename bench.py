@@ -381,6 +381,8 @@ def serve_requests(n, seed, cfgs):
 
 def serve_run(engine, reqs, times=None):
     """Submit reqs at the given arrival times (seconds from now; None: all at once) and wait for all."""
+    engine.reset_peaks()
+    s0 = engine.stats()
     start = engine.now()
     ids = []
     for k, req in enumerate(reqs):
@@ -395,20 +397,29 @@ def serve_run(engine, reqs, times=None):
     ok = all(r["ok"] for r in res)
     worst = max(max(r["true_rel_residual"]) for r in res) if ok else None
     rhs = sum(len(r["true_rel_residual"]) for r in res)
+    s1 = engine.stats()
+    per_solve = {}
+    for r in res:
+        per_solve[r["solve"]] = per_solve.get(r["solve"], 0) + 1
     return {"requests": len(res), "right_hand_sides": rhs, "all_ok": ok, "worst_true_rel_residual": worst,
+            "solves": len(per_solve), "solves_carrying_several_requests": sum(v > 1 for v in per_solve.values()),
+            "requests_merged_at_admission": s1["requests_merged"] - s0["requests_merged"],
+            "requests_joined_running_solves": s1["requests_joined"] - s0["requests_joined"],
+            "max_queue_depth": s1["max_queue_depth"],
             "throughput_req_per_s": round(len(res) / (end - start), 3),
             "throughput_rhs_per_s": round(rhs / (end - start), 3),
             "latency_p50_s": round(float(np.percentile(lat, 50)), 4),
             "latency_p99_s": round(float(np.percentile(lat, 99)), 4),
-            "merged_solves": len({(r["admitted"], r["finished"]) for r in res if r["batch_requests"] > 1}),
             "makespan_s": round(end - start, 3)}
 
 
-def serving_leg(n, load, seed, n_baseline, rate=None, inner="f32"):
+def serving_leg(n, load, seed, n_baseline, rate=None, inner="f32", n_capacity=60, n_burst=100, burst_rate=20.0):
     """Continuous-batching engine (4 workers / HIP streams, FP32-inner mixed solvers, HBM budget 160 GB):
-    its capacity with all n requests queued at once, then latency under a Poisson stream of the same
-    requests offered at `load` times that capacity (or at `rate`); against the plain FP64 solvers one
-    request at a time on the first n_baseline requests (and the engine on those same requests)."""
+    its capacity with the first n_capacity requests queued at once; latency under a Poisson stream of n
+    requests offered at `load` times that capacity (or at `rate`); a bounded run of n_burst requests arriving
+    as a Poisson stream at the workload's own burst_rate (20/s) with the queue depth, admissions and joins it
+    causes; and the plain FP64 solvers one request at a time on the first n_baseline requests against the
+    engine on those same requests."""
     from cl2qcd_amd import _host as H
 
     def engine(**kw):
@@ -423,27 +434,31 @@ def serving_leg(n, load, seed, n_baseline, rate=None, inner="f32"):
         return e, cfgs
 
     e, cfgs = engine(workers=4, mixed=True, inner_precision=H.F16 if inner == "f16" else H.F32)
-    reqs = serve_requests(n, seed, cfgs)
-    log(f"serving: {n} requests queued at once ...")
-    closed = serve_run(e, reqs)
+    reqs = serve_requests(max(n, n_capacity, n_burst, n_baseline), seed, cfgs)
+    log(f"serving: {n_capacity} requests queued at once ...")
+    closed = serve_run(e, reqs[:n_capacity])
     log(f"  capacity: {closed}")
     offered = rate if rate else load * closed["throughput_req_per_s"]
     times = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / offered, size=n))
-    log(f"serving: the same {n} requests as a Poisson stream at {offered:.3f}/s ...")
-    stream = serve_run(e, reqs, times)
+    log(f"serving: {n} requests as a Poisson stream at {offered:.3f}/s ...")
+    stream = serve_run(e, reqs[:n], times)
     log(f"  stream: {stream}")
+    times = np.cumsum(np.random.default_rng(seed + 2).exponential(1.0 / burst_rate, size=n_burst))
+    log(f"serving: {n_burst} requests as a Poisson stream at {burst_rate:.1f}/s ...")
+    burst = serve_run(e, reqs[:n_burst], times)
+    log(f"  burst: {burst}")
     first = serve_run(e, reqs[:n_baseline])
     del e
     f, _ = engine(workers=1, mixed=False)
     base = serve_run(f, reqs[:n_baseline])
     del f
     log(f"  first {n_baseline}, queued at once: engine {first}; FP64 one at a time {base}")
-    return {"requests": f"{n} requests, seed {seed}; sizes 16^3x32 / 24^3x48 / 32^3x64, twisted mass or "
-                        "staggered, 1-12 Gaussian RHS, |r|/|b| <= 1e-10; 8 resident heatbath beta 5.7 "
-                        "configurations",
+    return {"requests": f"seed {seed}; sizes 16^3x32 / 24^3x48 / 32^3x64, twisted mass or staggered, 1-12 "
+                        "Gaussian RHS, |r|/|b| <= 1e-10; 8 resident heatbath beta 5.7 configurations",
             "capacity_all_queued": closed,
             "poisson_stream": {"offered_req_per_s": round(offered, 3),
                                "load": round(offered / closed["throughput_req_per_s"], 3), **stream},
+            "burst_at_workload_rate": {"offered_req_per_s": burst_rate, **burst},
             "first_requests_all_queued": {"n": n_baseline, "engine": first, "fp64_one_at_a_time": base,
                                           "throughput_ratio": round(first["throughput_req_per_s"] /
                                                                     base["throughput_req_per_s"], 3)}}
@@ -575,8 +590,8 @@ def main():
                     help="skip the 8^4 CPU-reference workload (GPU drop-in and oracle CG)")
     ap.add_argument("--no-multishift", action="store_true", help="skip the 16-shift staggered solve at 48^3x12")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving run")
-    ap.add_argument("--serve-requests", type=int, default=24, help="requests of the serving run")
-    ap.add_argument("--serve-load", type=float, default=0.7,
+    ap.add_argument("--serve-requests", type=int, default=300, help="requests of the serving Poisson stream")
+    ap.add_argument("--serve-load", type=float, default=0.8,
                     help="Poisson arrival rate as a fraction of the engine's measured capacity")
     ap.add_argument("--serve-rate", type=float, default=None, help="fixed arrival rate (requests per second)")
     ap.add_argument("--serve-baseline", type=int, default=10, help="requests also run by the FP64 baseline")

@@ -159,6 +159,29 @@ int cg_reliable_update_block(const std::vector<const lattices::Spinorfield_eo*>&
                              hmc_float rel_tol, const std::vector<const physics::AdditionalParameters*>& parameters,
                              const ReliableUpdateOptions& options, std::vector<SolveStats>* stats = nullptr);
 
+/* Continuous batching of a lockstep block (no counterpart in the reference): at every reliable update the
+ * block solver reports the right-hand sides that have converged since the last one and offers its free
+ * slots (converged, or zero from the start) to new problems, which start from x = 0 in those slots while
+ * the others continue. */
+struct BlockRefill {
+    virtual ~BlockRefill() = default;
+    /* right-hand side r has converged: x[r] holds its final FP64 solution; st counts from the slot's start */
+    virtual void converged(int r, const SolveStats& st) = 0;
+    /* Offers the free slots.  For each slot it takes, the callee writes the new right-hand side to b[r],
+     * zeroes x[r] and sets kappa[r], mubar[r]; returns the slots taken (empty: none). */
+    virtual std::vector<int> refill(const std::vector<int>& free_slots, std::vector<hmc_float>& kappa,
+                                    std::vector<hmc_float>& mubar) = 0;
+};
+/* cg_reliable_update_block with kappa[r], mubar[r] per right-hand side (twisted mass or Wilson) and slot
+ * refills through `refill`; runs until every right-hand side, the refilled ones included, has converged.
+ * Returns the lockstep iterations. */
+int cg_reliable_update_block(const std::vector<const lattices::Spinorfield_eo*>& x,
+                             const fermionmatrix::Fermionmatrix_eo& A, const lattices::Gaugefield& gf,
+                             const std::vector<const lattices::Spinorfield_eo*>& b, const hardware::System& system,
+                             hmc_float rel_tol, std::vector<hmc_float> kappa, std::vector<hmc_float> mubar,
+                             const ReliableUpdateOptions& options, BlockRefill& refill,
+                             std::vector<SolveStats>* stats = nullptr);
+
 }  // namespace solvers
 }  // namespace algorithms
 }  // namespace physics

@@ -49,7 +49,18 @@ struct Result {
     int iterations = 0;           // inner iterations of the solve that carried this request
     int batch_requests = 0;       // requests merged into that solve
     int batch_rhs      = 0;       // right-hand sides in that solve
+    bool joined_running = false;  // taken into free slots of a lockstep solve that was already running
+    long solve = -1;              // sequence number of the solve that carried this request
     double arrival = 0., admitted = 0., finished = 0.;  // seconds on the engine clock
+};
+
+/* scheduler counters since the engine started */
+struct EngineStats {
+    long solves_started      = 0;  // solves a worker started (each may carry several requests)
+    long requests_merged     = 0;  // requests merged into another request's solve when it started
+    long requests_joined     = 0;  // requests taken into free slots of a running lockstep solve
+    long max_queue_depth     = 0;  // most requests waiting at once
+    long queue_depth         = 0;  // requests waiting now
 };
 
 struct EngineOptions {
@@ -58,6 +69,9 @@ struct EngineOptions {
     int inner_precision     = CL2_F32;           // reduced-precision inner loop of the mixed solvers
     int max_batch_rhs       = 12;                // right-hand sides merged into one Wilson solve
     bool mixed              = true;              // false: the reference's FP64 solvers, one request at a time
+    // twisted-mass lockstep solves take queued compatible requests into slots that are free (padding, or
+    // converged) at each reliable update; such requests wait for those slots instead of starting a new solve
+    bool join_running = true;
 };
 
 class Engine {
@@ -91,13 +105,20 @@ class Engine {
     void warmup();
     /* largest HBM use of work fields seen so far */
     size_t peak_work_bytes() const;
+    EngineStats stats() const;
+    /* restart max_queue_depth from the current depth (for measurements over a window) */
+    void reset_peaks();
 
   private:
     struct Config;
     struct Pending;
     struct Batch;
     struct Worker;
+    struct OpenBlock;
     int add_config(std::unique_ptr<Config> c);
+    bool claimable_locked(const Pending& p, std::map<const OpenBlock*, int>& claimed) const;
+    void publish_locked(Pending& p, Result r);
+    void run_tm_block(Worker& wk, Batch& b, Config& cfg);
     size_t work_bytes(const Pending& p) const;
     bool compatible(const Pending& a, const Pending& b) const;
     void schedule_locked();
@@ -111,6 +132,8 @@ class Engine {
     std::deque<std::unique_ptr<Pending>> queue;
     std::map<long, Result> done;
     std::vector<std::unique_ptr<Worker>> workers;
+    std::vector<OpenBlock*> open_blocks;  // running lockstep solves that take refills
+    EngineStats counters;
     std::vector<std::thread> threads;
     long next_id       = 0;
     long outstanding   = 0;

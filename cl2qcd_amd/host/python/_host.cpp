@@ -660,17 +660,20 @@ PYBIND11_MODULE(_host, m)
     });
 
     py::class_<serving::Engine>(m, "Engine")
-        .def(py::init([](size_t hbm_budget_bytes, int workers, int inner_precision, int max_batch_rhs, bool mixed) {
+        .def(py::init([](size_t hbm_budget_bytes, int workers, int inner_precision, int max_batch_rhs, bool mixed,
+                         bool join_running) {
                  serving::EngineOptions o;
                  o.hbm_budget_bytes = hbm_budget_bytes;
                  o.workers          = workers;
                  o.inner_precision  = inner_precision;
                  o.max_batch_rhs    = max_batch_rhs;
                  o.mixed            = mixed;
+                 o.join_running     = join_running;
                  return new serving::Engine(o);
              }),
              py::arg("hbm_budget_bytes") = (size_t)64 << 30, py::arg("workers") = 4,
-             py::arg("inner_precision") = CL2_F32, py::arg("max_batch_rhs") = 12, py::arg("mixed") = true)
+             py::arg("inner_precision") = CL2_F32, py::arg("max_batch_rhs") = 12, py::arg("mixed") = true,
+             py::arg("join_running") = true)
         .def(
             "load_config",
             [](serving::Engine& e, int ns, int nt, double beta, int sweeps, uint64_t seed) {
@@ -696,6 +699,18 @@ PYBIND11_MODULE(_host, m)
         .def("eo_sites", &serving::Engine::eo_sites)
         .def("resident_bytes", &serving::Engine::resident_bytes)
         .def("peak_work_bytes", &serving::Engine::peak_work_bytes)
+        .def("reset_peaks", &serving::Engine::reset_peaks)
+        .def("stats",
+             [](const serving::Engine& e) {
+                 const serving::EngineStats s = e.stats();
+                 py::dict d;
+                 d["solves_started"]  = s.solves_started;
+                 d["requests_merged"] = s.requests_merged;
+                 d["requests_joined"] = s.requests_joined;
+                 d["max_queue_depth"] = s.max_queue_depth;
+                 d["queue_depth"]     = s.queue_depth;
+                 return d;
+             })
         .def("now", &serving::Engine::now)
         .def(
             "submit",
@@ -748,6 +763,8 @@ PYBIND11_MODULE(_host, m)
                  d["iterations"]        = r.iterations;
                  d["batch_requests"]    = r.batch_requests;
                  d["batch_rhs"]         = r.batch_rhs;
+                 d["joined_running"]    = r.joined_running;
+                 d["solve"]             = r.solve;
                  d["arrival"]           = r.arrival;
                  d["admitted"]          = r.admitted;
                  d["finished"]          = r.finished;

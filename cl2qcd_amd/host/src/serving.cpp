@@ -53,13 +53,25 @@ struct Engine::Pending {
     long id;
     Request req;
     double arrival;
+    double admitted  = 0.;     // when a solve took it
+    long solve       = -1;     // which solve
+    bool joined      = false;  // taken into a running lockstep solve
+    bool published   = false;  // its result is in `done`
     int nrhs() const { return req.nrhs(); }
+};
+
+/* a running twisted-mass lockstep solve that takes refills: its configuration, tolerance and free slots */
+struct Engine::OpenBlock {
+    int config;
+    double rel_tol;
+    int free;
 };
 
 struct Engine::Batch {
     std::vector<std::unique_ptr<Pending>> items;
     size_t bytes   = 0;
     double admitted = 0.;
+    long solve      = -1;  // sequence number
 };
 
 /* a worker's device context and operators for one lattice size */
@@ -200,6 +212,46 @@ size_t Engine::peak_work_bytes() const
     std::lock_guard<std::mutex> lk(mu);
     return peak_bytes;
 }
+EngineStats Engine::stats() const
+{
+    std::lock_guard<std::mutex> lk(mu);
+    EngineStats s = counters;
+    s.queue_depth = (long)queue.size();
+    return s;
+}
+
+void Engine::reset_peaks()
+{
+    std::lock_guard<std::mutex> lk(mu);
+    counters.max_queue_depth = (long)queue.size();
+}
+
+/* whether an open lockstep solve will take p at its next reliable update (free slots not yet claimed by
+ * requests ahead of p in this scheduling pass) */
+bool Engine::claimable_locked(const Pending& p, std::map<const OpenBlock*, int>& claimed) const
+{
+    if (!opt.join_running || !opt.mixed || p.req.action != Action::twistedmass) return false;
+    for (const OpenBlock* o : open_blocks) {
+        if (o->config != p.req.config || o->rel_tol != p.req.rel_tol) continue;
+        if (o->free - claimed[o] >= p.nrhs()) {
+            claimed[o] += p.nrhs();
+            return true;
+        }
+    }
+    return false;
+}
+
+void Engine::publish_locked(Pending& p, Result r)
+{
+    r.id           = p.id;
+    r.arrival      = p.arrival;
+    r.admitted     = p.admitted;
+    r.joined_running = p.joined;
+    r.solve          = p.solve;
+    done[p.id]     = std::move(r);
+    p.published    = true;
+    --outstanding;
+}
 
 /* device bytes a request's solve needs: the block solver's FP64 and inner-precision fields and
  * temporaries per (padded) right-hand side, plus for staggered each right-hand side's own source and
@@ -239,6 +291,7 @@ long Engine::submit(Request r)
     const long id = p->id;
     queue.push_back(std::move(p));
     ++outstanding;
+    counters.max_queue_depth = std::max(counters.max_queue_depth, (long)queue.size());
     schedule_locked();
     return id;
 }
@@ -260,11 +313,16 @@ void Engine::schedule_locked()
         }
         if (free_w < 0) return;
         if (!opt.mixed && any_busy) return;  // the FP64 baseline runs one request at a time
-        const size_t need = work_bytes(*queue.front());
+        // the first queued request that no running lockstep solve will take into its free slots
+        std::map<const OpenBlock*, int> claimed;
+        auto head = queue.begin();
+        while (head != queue.end() && claimable_locked(**head, claimed)) ++head;
+        if (head == queue.end()) return;
+        const size_t need = work_bytes(**head);
         if (used_bytes + need > opt.hbm_budget_bytes && any_busy) return;
         std::unique_ptr<Batch> b(new Batch);
-        b->items.push_back(std::move(queue.front()));
-        queue.pop_front();
+        b->items.push_back(std::move(*head));
+        queue.erase(head);
         int rhs = b->items[0]->nrhs();
         // memory of a merged solve of k right-hand sides (its padded count)
         auto merged_bytes = [&](int k) {
@@ -274,21 +332,29 @@ void Engine::schedule_locked()
             m.req.gaussian_seeds.resize(k);
             return work_bytes(m);
         };
+        std::map<const OpenBlock*, int> claimed2;
         for (auto it = queue.begin(); it != queue.end();) {
             const int k = rhs + (*it)->nrhs();
             // a candidate joins only while the merged solve still fits the budget next to the running ones
-            // (only a lone head may exceed it, and only with nothing running)
+            // (only a lone head may exceed it, and only with nothing running); requests a running lockstep
+            // solve will take stay for it
             if (compatible(*b->items[0], **it) && k <= opt.max_batch_rhs &&
-                used_bytes + merged_bytes(k) <= opt.hbm_budget_bytes) {
+                used_bytes + merged_bytes(k) <= opt.hbm_budget_bytes && !claimable_locked(**it, claimed2)) {
                 rhs = k;
                 b->items.push_back(std::move(*it));
                 it = queue.erase(it);
+                ++counters.requests_merged;
             } else {
                 ++it;
             }
         }
         b->bytes    = merged_bytes(rhs);
         b->admitted = now();
+        b->solve    = counters.solves_started++;
+        for (auto& it : b->items) {
+            it->admitted = b->admitted;
+            it->solve    = b->solve;
+        }
         used_bytes += b->bytes;
         peak_bytes = std::max(peak_bytes, used_bytes);
         workers[free_w]->busy  = true;
@@ -313,7 +379,6 @@ void Engine::worker_loop(int w)
             std::lock_guard<std::mutex> lk(mu);
             used_bytes -= b->bytes;
             wk.busy = false;
-            outstanding -= (long)b->items.size();
             schedule_locked();
         }
         cv_done.notify_all();
@@ -330,9 +395,13 @@ void Engine::run_batch(Worker& wk, Batch& b)
         cfgp = configs[head.config].get();
     }
     Config& cfg = *cfgp;
-    std::vector<Result> res(b.items.size());
     int total_rhs = 0;
     for (const auto& it : b.items) total_rhs += it->nrhs();
+    if (opt.mixed && head.action == Action::twistedmass && padded_rhs(total_rhs) >= 2) {
+        run_tm_block(wk, b, cfg);
+        return;
+    }
+    std::vector<Result> res(b.items.size());
     try {
         WorkerLattice& L              = wk.lattice(cfg.ns, cfg.nt);
         const hardware::System& sys   = *L.system;
@@ -380,19 +449,10 @@ void Engine::run_batch(Worker& wk, Batch& b)
                         if (squarenorm(t) <= head.rel_tol * head.rel_tol * b2) break;
                     }
                 }
-            } else if (n == 1) {
+            } else {  // a single right-hand side (blocks of 2 and more run in run_tm_block)
                 sv::ReliableUpdateOptions o;
                 o.inner_precision = opt.inner_precision;
                 iterations = sv::cg_reliable_update(xs[0].get(), *L.qq, gf, *bs[0], sys, head.rel_tol, *aps[0], o);
-            } else {
-                sv::ReliableUpdateOptions o;
-                o.inner_precision = CL2_F32;
-                std::vector<const Spinorfield_eo*> xv, bv;
-                for (int k = 0; k < n; ++k) {
-                    xv.push_back(xs[k].get());
-                    bv.push_back(bs[k].get());
-                }
-                iterations = sv::cg_reliable_update_block(xv, *L.qq, gf, bv, sys, head.rel_tol, aps, o);
             }
             // FP64 true residual of every answer
             const Spinorfield_eo t(sys);
@@ -520,12 +580,207 @@ void Engine::run_batch(Worker& wk, Batch& b)
         }
     }
     std::lock_guard<std::mutex> lk(mu);
-    for (size_t i = 0; i < b.items.size(); ++i) {
-        res[i].id       = b.items[i]->id;
-        res[i].arrival  = b.items[i]->arrival;
-        res[i].admitted = b.admitted;
-        done[res[i].id] = std::move(res[i]);
+    for (size_t i = 0; i < b.items.size(); ++i) publish_locked(*b.items[i], std::move(res[i]));
+}
+
+/* A twisted-mass lockstep solve (FP32 inner, FP64 reliable updates) of 2 to 16 slots whose right-hand sides
+ * each carry their request's kappa and mu.  Each request's result is published as soon as its own right-hand
+ * sides have converged; while join_running is set, the slots that are free at a reliable update (padding, or
+ * converged) take queued compatible requests, which start there from x = 0 while the others continue. */
+void Engine::run_tm_block(Worker& wk, Batch& b, Config& cfg)
+{
+    const double rel_tol = b.items[0]->req.rel_tol;
+    const int config     = b.items[0]->req.config;
+    int total_rhs        = 0;
+    for (const auto& it : b.items) total_rhs += it->nrhs();
+    const int n = padded_rhs(total_rhs);
+    OpenBlock open{config, rel_tol, n - total_rhs};
+    struct Refill : sv::BlockRefill {
+        Engine& e;
+        Batch& b;
+        OpenBlock& open;
+        WorkerLattice* L = nullptr;
+        const Gaugefield* gf = nullptr;
+        std::vector<std::unique_ptr<Spinorfield_eo>> bs, xs;
+        std::vector<int> item, rhs;  // per slot: request (index into b.items, -1 free) and its right-hand side
+        std::vector<int> left;       // per request: right-hand sides not yet converged
+        std::vector<std::vector<double>> resid;
+        std::vector<std::vector<std::vector<double>>> sols;
+        std::vector<int> iters;
+        std::vector<hmc_float> kappa_of, mubar_of;  // per slot, for the FP64 residual of its answer
+        Refill(Engine& e, Batch& b, OpenBlock& o) : e(e), b(b), open(o) {}
+
+        /* request i into `slots`: sources, x = 0, parameters */
+        void place(size_t i, const std::vector<int>& slots, std::vector<hmc_float>& kappa,
+                   std::vector<hmc_float>& mubar)
+        {
+            const Request& q = b.items[i]->req;
+            const hmc_float mb = 2. * q.kappa * q.mu;
+            for (int j = 0; j < q.nrhs(); ++j) {
+                const int r = slots[j];
+                if (q.sources.empty())
+                    bs[r]->gaussian_from_seed(q.gaussian_seeds[j]);
+                else
+                    bs[r]->import(reinterpret_cast<const spinor*>(q.sources[j].data()));
+                xs[r]->zero();
+                item[r] = (int)i;
+                rhs[r]  = j;
+                kappa[r] = kappa_of[r] = q.kappa;
+                mubar[r] = mubar_of[r] = mb;
+            }
+            if (left.size() <= i) {
+                left.resize(i + 1);
+                resid.resize(i + 1);
+                sols.resize(i + 1);
+                iters.resize(i + 1);
+            }
+            left[i] = q.nrhs();
+            resid[i].assign(q.nrhs(), 0.);
+            sols[i].assign(q.nrhs(), std::vector<double>());
+            iters[i] = 0;
+        }
+
+        void converged(int r, const sv::SolveStats& st) override
+        {
+            const int i = item[r];
+            if (i < 0) return;  // padding
+            const hardware::System& sys = *L->system;
+            const WilsonAdditionalParameters ap(kappa_of[r], mubar_of[r]);
+            const Spinorfield_eo t(sys);
+            (*L->qq)(&t, *gf, *xs[r], ap);
+            saxpy(&t, hmc_complex{1., 0.}, t, *bs[r]);  // b - A x in FP64 with the request's own operator
+            const hmc_float b2 = squarenorm(*bs[r]);
+            resid[i][rhs[r]]   = b2 > 0. ? std::sqrt(squarenorm(t) / b2) : 0.;
+            if (b.items[i]->req.return_solutions) {
+                const std::vector<spinor> h = xs[r]->get_host();
+                const double* d             = reinterpret_cast<const double*>(h.data());
+                sols[i][rhs[r]].assign(d, d + h.size() * 24);
+            }
+            iters[i] = std::max(iters[i], st.iterations);
+            item[r]  = -1;
+            if (--left[i] > 0) {
+                std::lock_guard<std::mutex> lk(e.mu);
+                ++open.free;
+                return;
+            }
+            Result o;
+            o.ok             = true;
+            o.iterations     = iters[i];
+            o.batch_rhs      = (int)item.size();
+            o.width          = 24;
+            o.true_rel_residual = resid[i];
+            if (b.items[i]->req.return_solutions) o.solutions = std::move(sols[i]);
+            o.finished = e.now();
+            {
+                std::lock_guard<std::mutex> lk(e.mu);
+                o.batch_requests = (int)b.items.size();
+                ++open.free;
+                e.publish_locked(*b.items[i], std::move(o));
+            }
+            e.cv_done.notify_all();
+        }
+
+        std::vector<int> refill(const std::vector<int>& free_slots, std::vector<hmc_float>& kappa,
+                                std::vector<hmc_float>& mubar) override
+        {
+            std::vector<size_t> joined;
+            {
+                std::lock_guard<std::mutex> lk(e.mu);
+                if (!e.opt.join_running) return {};
+                int room = (int)free_slots.size();
+                const double t = e.now();
+                for (auto it = e.queue.begin(); it != e.queue.end() && room > 0;) {
+                    const Request& q = (*it)->req;
+                    if (q.action == Action::twistedmass && q.config == open.config && q.rel_tol == open.rel_tol &&
+                        q.nrhs() <= room) {
+                        room -= q.nrhs();
+                        (*it)->admitted = t;
+                        (*it)->joined   = true;
+                        (*it)->solve    = b.solve;
+                        joined.push_back(b.items.size());
+                        b.items.push_back(std::move(*it));
+                        it = e.queue.erase(it);
+                        ++e.counters.requests_joined;
+                    } else {
+                        ++it;
+                    }
+                }
+                open.free = room;
+            }
+            std::vector<int> taken;
+            size_t next = 0;
+            for (size_t i : joined) {
+                std::vector<int> slots(free_slots.begin() + next, free_slots.begin() + next + b.items[i]->nrhs());
+                next += slots.size();
+                place(i, slots, kappa, mubar);
+                taken.insert(taken.end(), slots.begin(), slots.end());
+            }
+            return taken;
+        }
+    } rf(*this, b, open);
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        open_blocks.push_back(&open);
     }
+    try {
+        WorkerLattice& L = wk.lattice(cfg.ns, cfg.nt);
+        rf.L             = &L;
+        rf.gf            = cfg.gf.get();
+        const hardware::System& sys = *L.system;
+        std::vector<hmc_float> kappa(n), mubar(n);
+        rf.item.assign(n, -1);
+        rf.rhs.assign(n, 0);
+        rf.kappa_of.assign(n, 0.);
+        rf.mubar_of.assign(n, 0.);
+        for (int r = 0; r < n; ++r) {
+            rf.bs.emplace_back(new Spinorfield_eo(sys));
+            rf.xs.emplace_back(new Spinorfield_eo(sys));
+            rf.xs.back()->zero();
+        }
+        int next = 0;
+        for (size_t i = 0; i < b.items.size(); ++i) {
+            std::vector<int> slots;
+            for (int j = 0; j < b.items[i]->nrhs(); ++j) slots.push_back(next++);
+            rf.place(i, slots, kappa, mubar);
+        }
+        for (int r = next; r < n; ++r) {  // padding right-hand sides are zero: free from the start
+            rf.bs[r]->zero();
+            kappa[r] = kappa[0];
+            mubar[r] = mubar[0];
+        }
+        std::vector<const Spinorfield_eo*> xv, bv;
+        for (int r = 0; r < n; ++r) {
+            xv.push_back(rf.xs[r].get());
+            bv.push_back(rf.bs[r].get());
+        }
+        sv::ReliableUpdateOptions o;
+        o.inner_precision = CL2_F32;
+        sv::cg_reliable_update_block(xv, *L.qq, *cfg.gf, bv, sys, rel_tol, kappa, mubar, o, rf);
+    } catch (const std::exception& ex) {
+        const double fin = now();
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto& it : b.items)
+            if (!it->published) {
+                Result r;
+                r.ok       = false;
+                r.error    = ex.what();
+                r.finished = fin;
+                publish_locked(*it, std::move(r));
+            }
+    }
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        open_blocks.erase(std::find(open_blocks.begin(), open_blocks.end(), &open));
+        for (auto& it : b.items)  // every right-hand side converged and was published by the solver's callback
+            if (!it->published) {
+                Result r;
+                r.ok       = false;
+                r.error    = "serving: lockstep solve ended before this request converged";
+                r.finished = now();
+                publish_locked(*it, std::move(r));
+            }
+    }
+    cv_done.notify_all();
 }
 
 void Engine::warmup()

@@ -526,11 +526,19 @@ using BlockStepT = std::function<bool(const typename Ops::Block*, const typename
                                       const Vector<hmc_float>&, const Vector<hmc_float>&)>;
 using BlockStep = BlockStepT<WilsonBlockOps>;
 
+/* slot refills of a running block (continuous batching): at a reliable update, `converged` hears of every
+ * right-hand side frozen there and `refill` is offered the free slots; it returns the slots it gave new
+ * problems (new b[r], x[r] = 0, new operator parameters) */
+struct BlockHooks {
+    std::function<void(int, const SolveStats&)> converged;
+    std::function<std::vector<int>(const std::vector<int>&)> refill;
+};
+
 template<class Ops>
 int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vector<const typename Ops::Field*>& b,
                 const hardware::System& system, hmc_float rel_tol, const BlockApplyT<Ops>& apply,
                 const BlockStepT<Ops>& step, const ReliableUpdateOptions& opt, std::vector<SolveStats>* stats,
-                bool shrink = false)
+                bool shrink = false, const BlockHooks* hooks = nullptr)
 {
     const int n = (int)b.size();
     if (n < 1 || x.size() != b.size()) throw std::invalid_argument("cg_reliable_update_block: x and b differ in size");
@@ -569,9 +577,10 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         system.check(Ops::axpy(c, R.dev(), T.dev(), B.dev(), CL2_F64, n, ones.dev(), -1., nullptr));
         return block_norms(R);  // |b - A x|^2 per right-hand side
     };
-    const std::vector<hmc_float> b2 = block_norms(B);
+    std::vector<hmc_float> b2 = block_norms(B);
     std::vector<hmc_float> target(n), rn = true_residual(), act(n), r2h(n), rmax(n);
     std::vector<SolveStats> st(n);
+    std::vector<int> start_k(n, 0), start_rel(n, 0);  // iteration and reliable update at which a slot started
     int n_active = 0;
     for (int r = 0; r < n; ++r) {
         target[r] = rel_tol * rel_tol * b2[r];
@@ -584,10 +593,17 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         for (int r = 0; r < n; ++r) {
             X.extract(r, x[r]);
             st[r].true_relative_residual = b2[r] > 0. ? std::sqrt(rn[r] / b2[r]) : 0.;
-            if (act[r] != 0.) st[r].iterations = k;
+            if (act[r] != 0.) st[r].iterations = k - start_k[r];
         }
         if (stats) *stats = st;
     };
+    if (hooks)  // right-hand sides that need no iteration (zero, or already solved) are answered now
+        for (int r = 0; r < n; ++r)
+            if (act[r] == 0.) {
+                X.extract(r, x[r]);
+                st[r].true_relative_residual = b2[r] > 0. ? std::sqrt(rn[r] / b2[r]) : 0.;
+                hooks->converged(r, st[r]);
+            }
     if (n_active == 0) {
         finish(0);
         return 0;
@@ -623,7 +639,11 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         cl2_graph* g = nullptr;
         ~Graph() { cl2_graph_destroy(g); }
     } graph;
-    if (opt.use_graph) {
+    // (re)record the iteration: the operator's per-right-hand-side parameters are kernel arguments, so a refill
+    // that changes them records again
+    auto capture = [&]() {
+        cl2_graph_destroy(graph.g);
+        graph.g = nullptr;
         system.check(cl2_graph_begin(c));
         try {
             body();
@@ -638,7 +658,8 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         // replays are checked against eager execution for kernel launches only (memset nodes have replayed
         // wrongly on this hardware): a capture that recorded anything else is a bug here
         if (graph_nodes != graph_kernel_nodes) throw std::logic_error("captured solver iteration holds non-kernel nodes");
-    }
+    };
+    if (opt.use_graph) capture();
     bool pending = false;  // Xlo holds corrections not yet added to X
     for (k = 0; k < opt.max_iterations && n_active > 0; ++k) {
         if (graph.g)
@@ -663,12 +684,48 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         rn      = true_residual();
         ++reliable;
         std::vector<hmc_float> oc(2 * n, 0.);
+        std::vector<int> frozen;
         for (int r = 0; r < n; ++r) {
             if (act[r] == 0.) continue;
             if (rn[r] <= target[r]) {  // converged: freeze this right-hand side
                 act[r]           = 0.;
-                st[r].iterations = k + 1;
+                st[r].iterations = k + 1 - start_k[r];
                 --n_active;
+                frozen.push_back(r);
+            }
+        }
+        std::vector<int> taken;  // slots refilled with new problems at this update
+        if (hooks) {
+            for (int r : frozen) {
+                X.extract(r, x[r]);
+                st[r].true_relative_residual = std::sqrt(rn[r] / b2[r]);
+                st[r].reliable_updates       = reliable - start_rel[r];
+                hooks->converged(r, st[r]);
+            }
+            std::vector<int> free_slots;
+            for (int r = 0; r < n; ++r)
+                if (act[r] == 0.) free_slots.push_back(r);
+            if (!free_slots.empty()) taken = hooks->refill(free_slots);
+            if (!taken.empty()) {
+                for (int r : taken) {
+                    B.insert(r, *b[r]);
+                    X.insert(r, *x[r]);  // zero
+                }
+                b2 = block_norms(B);
+                rn = true_residual();  // r = b in the new slots
+                for (int r : taken) {
+                    if (act[r] != 0.) throw std::logic_error("cg_reliable_update_block: a refill took an active slot");
+                    target[r]    = rel_tol * rel_tol * b2[r];
+                    st[r]        = SolveStats();
+                    start_k[r]   = k + 1;
+                    start_rel[r] = reliable;
+                    if (b2[r] > 0.) {
+                        act[r] = 1.;
+                        ++n_active;
+                    } else {  // a zero right-hand side: x = 0 is its answer
+                        hooks->converged(r, st[r]);
+                    }
+                }
             }
         }
         if (n_active == 0) {
@@ -683,6 +740,12 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         // p = r_iter + beta p_old was formed with the iterated residuals: rebuild it on the true ones,
         // p = r_true + (beta p_old with its component along r_true removed)
         system.check(Ops::axpy(c, P.dev(), Rlo.dev(), P.dev(), lo, n, ones.dev(), -1., nullptr));
+        if (!taken.empty()) {  // refilled slots start afresh: their p_old is zero, so p = r there
+            std::vector<hmc_float> z(2 * n, 0.);
+            for (int r : taken) z[2 * r] = 1.;
+            coef.store(z);
+            system.check(Ops::axpy(c, P.dev(), P.dev(), P.dev(), lo, n, coef.dev(), -1., nullptr));  // p_r -= p_r
+        }
         system.check(Ops::convert(c, Rlo.dev(), lo, R.dev(), CL2_F64, n));
         system.check(Ops::dot(c, res.dev(), Rlo.dev(), P.dev(), lo, n));
         const std::vector<hmc_float> rp = res.get();
@@ -698,6 +761,7 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         system.check(Ops::axpy(c, P.dev(), Rlo.dev(), P.dev(), lo, n, ones.dev(), 1., nullptr));
         r2.store(r2h);
         active.store(act);
+        if (!taken.empty() && graph.g) capture();
     }
     if (pending) {  // stopped by the iteration limit: keep the corrections gathered since the last update
         system.check(Ops::convert(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
@@ -768,6 +832,45 @@ int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const 
             fermionmatrix::apply_block(A, out, gf, in, kappa, mubar, tmp);
         },
         step, opt, stats);
+}
+
+int cg_reliable_update_block(const std::vector<const Spinorfield_eo*>& x, const fermionmatrix::Fermionmatrix_eo& A,
+                             const Gaugefield& gf, const std::vector<const Spinorfield_eo*>& b,
+                             const hardware::System& system, hmc_float rel_tol, std::vector<hmc_float> kappa,
+                             std::vector<hmc_float> mubar, const ReliableUpdateOptions& opt, BlockRefill& refill,
+                             std::vector<SolveStats>* stats)
+{
+    if (kappa.size() != b.size() || mubar.size() != b.size())
+        throw std::invalid_argument("cg_reliable_update_block: one kappa and one mubar per right-hand side");
+    if (b.size() < 2) throw std::invalid_argument("cg_reliable_update_block: slot refills need a block of 2 or more");
+    if (A.action() != common::twistedmass)
+        for (hmc_float m : mubar)
+            if (m != 0.) throw std::invalid_argument("cg_reliable_update_block: mubar is for twisted mass");
+    // the operator reads kappa and mubar at every call (and at every capture), so refills that change them
+    // take effect from the next iteration
+    BlockStep step;
+    if (A.abi_operator() == CL2_OP_QPLUSQMINUS_EO)
+        step = [&](const Spinorfield_eo_block* r, const Spinorfield_eo_block& p, const DeviceBuffer& tmp,
+                   const Vector<hmc_float>& r2, const Vector<hmc_float>& active, const Vector<hmc_float>& pAp,
+                   const Vector<hmc_float>& alpha, const Vector<hmc_float>& norms) {
+            return fermionmatrix::cg_step_block(A, r, gf, p, nullptr, kappa, mubar, tmp, r2, active, pAp, alpha,
+                                                norms);
+        };
+    BlockHooks hooks;
+    hooks.converged = [&](int r, const SolveStats& st) { refill.converged(r, st); };
+    hooks.refill    = [&](const std::vector<int>& free_slots) {
+        std::vector<int> taken = refill.refill(free_slots, kappa, mubar);
+        for (int r : taken)
+            if (std::find(free_slots.begin(), free_slots.end(), r) == free_slots.end())
+                throw std::logic_error("cg_reliable_update_block: a refill took a slot that was not free");
+        return taken;
+    };
+    return block_solve<WilsonBlockOps>(
+        x, b, system, rel_tol,
+        [&](const Spinorfield_eo_block* out, const Spinorfield_eo_block& in, const DeviceBuffer& tmp) {
+            fermionmatrix::apply_block(A, out, gf, in, kappa, mubar, tmp);
+        },
+        step, opt, stats, false, &hooks);
 }
 
 }  // namespace solvers

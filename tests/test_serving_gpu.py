@@ -38,6 +38,16 @@ def test_gaussian_source():
         assert abs(y.mean()) < 0.03 and abs(y.var() - 1.0) < 0.03
 
 
+def solve_spans(results):
+    """(start, end) of every solve, in start order: a solve runs from its first admission to its last answer
+    (requests joined into a running lockstep solve share its span)"""
+    spans = {}
+    for m in results:
+        a, f = spans.get(m["solve"], (m["admitted"], m["finished"]))
+        spans[m["solve"]] = (min(a, m["admitted"]), max(f, m["finished"]))
+    return sorted(spans.values())
+
+
 def make_engine(H, **kw):
     e = H.Engine(**kw)
     c0 = e.load_config(8, 8, 5.7, 5, 11)
@@ -85,7 +95,7 @@ def test_same_alone_and_in_mix():
     merged = [m["batch_requests"] for r, m in zip(reqs, mix) if r["config"] == c1 and r["action"] == "staggered"]
     assert max(merged) >= 2
     # solves overlapped in time
-    spans = sorted(set((m["admitted"], m["finished"]) for m in mix))  # one span per solve
+    spans = solve_spans(mix)
     assert any(spans[k + 1][0] < spans[k][1] for k in range(len(spans) - 1))
     # the oracle agrees that the first answer solves its system
     lat = O.lattice(8, 8, theta_t=1.0)
@@ -106,7 +116,7 @@ def test_budget_serialises_and_fp64_mode_agrees():
     reqs = requests(e, c0, c1)
     ids = [e.submit(**r) for r in reqs]
     res = [e.wait(i) for i in ids]
-    spans = sorted(set((m["admitted"], m["finished"]) for m in res))  # one span per solve
+    spans = solve_spans(res)
     assert all(spans[k + 1][0] >= spans[k][1] for k in range(len(spans) - 1))
     f, d0, d1 = make_engine(H, workers=2, mixed=False)
     ids = [f.submit(**r) for r in reqs]
@@ -152,3 +162,51 @@ def test_merge_respects_budget_next_to_running_solve():
     assert e.peak_work_bytes() <= budget
     # the merged solves stayed within the 2 RHS that fit (12 would have fitted the RHS limit)
     assert max(r["batch_rhs"] for r in res) <= 2
+
+
+def test_running_block_takes_queued_requests_into_free_slots():
+    """Continuous batching: with one worker busy on a 3-RHS twisted-mass solve (padded to 4 slots), a 1-RHS
+    request of another quark mass on the same configuration joins the running solve's free slot at its next
+    reliable update instead of waiting for the worker; two more join slots that converged.  Every answer meets
+    1e-10 (the oracle recomputes one) and equals the same request solved alone to 1e-6; the first request's
+    answer is published before the solve ends."""
+    from cl2qcd_amd import _host as H
+    e = H.Engine(workers=1)
+    c0 = e.load_config(8, 8, 5.7, 5, 11)
+    e.warmup()
+    rng = np.random.default_rng(9)
+    mk = lambda kappa, mu, n: dict(config=c0, action="twistedmass", kappa=kappa, mu=mu,
+                                   sources=[rng.standard_normal((e.eo_sites(c0), 24)) for _ in range(n)])
+    reqs = [mk(0.15, 0.02, 3), mk(0.155, 0.008, 1), mk(0.152, 0.012, 2), mk(0.15, 0.003, 1)]
+    alone = []
+    for r in reqs:
+        a = e.wait(e.submit(**r))
+        assert a["ok"] and not a["joined_running"]
+        alone.append(a)
+    s0 = e.stats()
+    ids = [e.submit(**r) for r in reqs]
+    res = [e.wait(i) for i in ids]
+    s1 = e.stats()
+    assert all(r["ok"] for r in res), [r["error"] for r in res]
+    assert all(max(r["true_rel_residual"]) <= 1e-10 for r in res)
+    joined = [r for r in res[1:] if r["joined_running"]]
+    assert joined, "no request joined the running solve"
+    assert all(r["solve"] == res[0]["solve"] and r["admitted"] > res[0]["admitted"] for r in joined)
+    assert s1["requests_joined"] - s0["requests_joined"] == len(joined)
+    assert s1["solves_started"] - s0["solves_started"] == 1 + (len(reqs) - 1 - len(joined))
+    for a, m in zip(alone, res):
+        for xa, xm in zip(a["solutions"], m["solutions"]):
+            assert rel(xm, xa) < 1e-6
+    # the first request was answered before its solve had finished the joined ones
+    assert res[0]["finished"] < max(r["finished"] for r in joined)
+    lat = O.lattice(8, 8, theta_t=1.0)
+    l = H.Lattice(["--nSpace=8", "--nTime=8", "--beta=5.7", "--hostSeed=11", "--thetaFermionTemporal=1"])
+    gf = H.Gaugefield(l, False)
+    for _ in range(5):
+        H.su3heatbath(gf, l, 1)
+    g = gf.get_host()
+    k = res.index(joined[0])
+    x, b = res[k]["solutions"][0], reqs[k]["sources"][0]
+    kap, mu = reqs[k]["kappa"], reqs[k]["mu"]
+    r = O.saxpy(O.apply_eo(lat, O.OP_QPLUSQMINUS_EO, x, g, O.TWISTEDMASS, kap, 2 * kap * mu), b, 1.0)
+    assert np.sqrt(O.norm2(r) / O.norm2(b)) <= 1.01e-10
