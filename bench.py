@@ -75,7 +75,8 @@ def dslash_leg(ctx, abi, fmt, prec, steps, warmup, src, gauge18_ptr, autotune=Tr
     kappa = 0.16
     # launch shape from the autotuner (z-planes per slab, XCD mapping, block size), timed on this field;
     # without it the library's default shape
-    tuned = [ctypes.c_double(), ctypes.c_int(-1), ctypes.c_int(1), ctypes.c_int(1), ctypes.c_int(256), ctypes.c_int(0)]
+    # without the autotuner: the library default (-1 = the format's register budget)
+    tuned = [ctypes.c_double(), ctypes.c_int(-1), ctypes.c_int(1), ctypes.c_int(1), ctypes.c_int(256), ctypes.c_int(-1)]
     if autotune:
         ctx.check(ctx.lib.cl2_autotune_dslash(ctx.handle, ctypes.c_void_p(gptr), fmt, 1,
                                               *[ctypes.byref(t) for t in tuned]))
@@ -292,6 +293,37 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
             "dslash_f32_ms": {"single": round(t1, 5), f"mrhs{nrhs}": round(tn, 5),
                               f"mrhs{nrhs}_per_rhs": round(tn / nrhs, 5),
                               f"mrhs{nrhs}_alg_GBps": round(alg_n / (tn * 1e6), 1)}}
+
+
+def mrhs_shapes_leg(ns=32, nt=64, nrhs=12, reps=30):
+    """The FP32 12-RHS hopping kernel alone at 32^3x64 in every work order (z-planes per slab x XCD mapping):
+    time per launch from HIP events, after one copy of an FP64 field for counter calibration.  Run under
+    rocprofv3 --pmc, the dispatches come in the printed order (one warmup and `reps` timed per shape)."""
+    from cl2qcd_amd import abi
+
+    ctx = abi.Context(ns, nt, theta_t=1.0)
+    g = ctx.gauge(fmt=abi.GAUGE_F32_12, seed=3, epsilon=0.35)
+    one = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F32)
+    an, bn = ctx.buffer(nrhs * one), ctx.buffer(nrhs * one)
+    ctx.block_set_zero_eo(an.ptr, abi.F32, nrhs)
+    f64, dst = ctx.spinor(), ctx.spinor()
+    nb = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F64)
+    ctx.check(ctx.lib.cl2_copy(ctx.handle, dst.ptr, f64.ptr, nb))
+    out = []
+    for zp in (1, 2, 4, 8, 16, 32, 0):
+        for xcd in (1, 0):
+            ctx.check(ctx.lib.cl2_tune_set(ctx.handle, b"hop.zplanes", zp))
+            ctx.check(ctx.lib.cl2_tune_set(ctx.handle, b"hop.xcd", xcd))
+
+            def f():
+                ctx.dslash_eo_mrhs(bn.ptr, an.ptr, g.ptr, abi.GAUGE_F32_12, abi.EVEN, 0.16, nrhs)
+
+            f()
+            ms, _ = time_steps(ctx, lambda k: f(), reps)
+            out.append({"zplanes": zp, "xcd": xcd, "ms": round(ms / reps, 5)})
+            log(f"12-RHS FP32 hopping kernel, zplanes {zp:2d} xcd {xcd}: {ms / reps:.4f} ms")
+    ctx.close()
+    return out
 
 
 def multishift_leg(ns, nt, sweeps, seed, nshift=16):
@@ -586,6 +618,9 @@ def main():
     ap.add_argument("--no-cg", action="store_true", help="skip the FP64 vs mixed-precision CG comparison")
     ap.add_argument("--hb-sweeps", type=int, default=50, help="heatbath sweeps for the CG configuration")
     ap.add_argument("--no-batch", action="store_true", help="skip the 12-source block solve at 32^3x64")
+    ap.add_argument("--mrhs-shapes", action="store_true",
+                    help="only time the 12-RHS FP32 hopping kernel at 32^3x64 in every work order (--steps launches "
+                         "each), for profiles")
     ap.add_argument("--no-config1", action="store_true",
                     help="skip the 8^4 CPU-reference workload (GPU drop-in and oracle CG)")
     ap.add_argument("--no-multishift", action="store_true", help="skip the 16-shift staggered solve at 48^3x12")
@@ -604,6 +639,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="launch the solvers' inner iterations kernel by kernel instead of replaying a hipGraph")
     args = ap.parse_args()
+    if args.mrhs_shapes:
+        print(json.dumps({"mrhs_shapes_f32_12rhs_32x64": mrhs_shapes_leg(reps=args.steps)}), flush=True)
+        return
     global USE_GRAPH
     USE_GRAPH = not args.no_graph
     if args.gpus != 1:
