@@ -9,6 +9,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <initializer_list>
 #include <vector>
 
 #include "cl2_internal.hpp"
@@ -97,6 +98,20 @@ bool valid_wprec(int p)
 {
     return valid_prec(p) || p == CL2_F16;
 }
+/* FP32 Wilson spinors are read and written as 16-byte component pairs */
+static bool aligned16(const void* p)
+{
+    return (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
+}
+static bool all_aligned16(std::initializer_list<const void*> ps)
+{
+    for (const void* p : ps)
+        if (!aligned16(p)) return false;
+    return true;
+}
+#define CL2_PAIRS_ALIGNED(c, prec, ...) \
+    CL2_REQUIRE(c, (prec) != CL2_F32 || all_aligned16({__VA_ARGS__}), "FP32 Wilson spinor fields must be 16-byte aligned")
+
 bool valid_fmt(int f)
 {
     return f == CL2_GAUGE_F64_18 || f == CL2_GAUGE_F64_12 || f == CL2_GAUGE_F32_12 || f == CL2_GAUGE_F16_12;
@@ -213,6 +228,11 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
     CL2_REQUIRE(c, cg_r == nullptr || cg_r != x, "the CG residual must differ from the diagonal-term field");
     CL2_REQUIRE(c, epilogue != EPI_AEE || x != nullptr, "null diagonal-term field");
     CL2_REQUIRE(c, epilogue != EPI_AEE || x != in, "diagonal-term field must differ from the hopping input");
+    if (fmt == CL2_GAUGE_F32_12) {
+        CL2_PAIRS_ALIGNED(c, CL2_F32, in);
+        CL2_REQUIRE(c, aligned16(out) && aligned16(x) && aligned16(cg_r),
+                    "FP32 Wilson spinor fields must be 16-byte aligned");
+    }
     DslashArgs a;
     a.in            = in;
     a.out           = out;
@@ -530,6 +550,14 @@ int cl2_copy(cl2_ctx* c, void* dst, const void* src, size_t bytes)
     return CL2_OK;
 }
 
+/* complex index of component k of site i in a device field of S sites: component-major, FP32 Wilson spinors
+ * (ncomp 12) in component pairs (the device layout of load_spinor) */
+static size_t host_cx(int prec, int ncomp, int k, size_t S, size_t i)
+{
+    if (prec == CL2_F32 && ncomp == 12) return (size_t)(k >> 1) * 2 * S + 2 * i + (size_t)(k & 1);
+    return (size_t)k * S + i;
+}
+
 static int complex_field_import(cl2_ctx* c, void* dev, const double* host, int prec, int ncomp)
 {
     CL2_REQUIRE(c, dev && host, "null pointer");
@@ -545,11 +573,13 @@ static int complex_field_import(cl2_ctx* c, void* dev, const double* host, int p
         CL2_HIP(c, hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
         CL2_HIP(c, hipStreamSynchronize(c->stream));
     } else {
+        CL2_PAIRS_ALIGNED(c, ncomp == 12 ? prec : CL2_F64, dev);
         std::vector<float> stage(2 * ncomp * S);
         for (size_t i = 0; i < S; ++i)
             for (int k = 0; k < ncomp; ++k) {
-                stage[2 * (k * S + i)]     = (float)host[2 * ncomp * i + 2 * k];
-                stage[2 * (k * S + i) + 1] = (float)host[2 * ncomp * i + 2 * k + 1];
+                const size_t e   = host_cx(prec, ncomp, k, S, i);
+                stage[2 * e]     = (float)host[2 * ncomp * i + 2 * k];
+                stage[2 * e + 1] = (float)host[2 * ncomp * i + 2 * k + 1];
             }
         CL2_HIP(c, hipMemcpyAsync(dev, stage.data(), stage.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
         CL2_HIP(c, hipStreamSynchronize(c->stream));
@@ -577,8 +607,9 @@ static int complex_field_export(cl2_ctx* c, double* host, const void* dev, int p
         CL2_HIP(c, hipStreamSynchronize(c->stream));
         for (size_t i = 0; i < S; ++i)
             for (int k = 0; k < ncomp; ++k) {
-                host[2 * ncomp * i + 2 * k]     = stage[2 * (k * S + i)];
-                host[2 * ncomp * i + 2 * k + 1] = stage[2 * (k * S + i) + 1];
+                const size_t e                  = host_cx(prec, ncomp, k, S, i);
+                host[2 * ncomp * i + 2 * k]     = stage[2 * e];
+                host[2 * ncomp * i + 2 * k + 1] = stage[2 * e + 1];
             }
     }
     return CL2_OK;
@@ -828,6 +859,7 @@ int cl2_M_tm_sitediagonal_eo(cl2_ctx* c, void* out, const void* in, int prec, do
     CL2_REQUIRE(c, valid_wprec(prec), "unknown precision");
     CL2_REQUIRE(c, variant >= CL2_MTM_SITEDIAGONAL && variant <= CL2_MTM_SITEDIAGONAL_MINUS_AND_GAMMA5,
                 "unknown site-diagonal variant");
+    CL2_PAIRS_ALIGNED(c, prec, out, in);
     double f1[2], f2[2], s;
     tm_factors(variant, mubar, f1, f2, &s);
     if (prec == CL2_F16)
@@ -1037,7 +1069,9 @@ int cl2_gaussian_eo(cl2_ctx* c, int kind, void* out, int prec, uint64_t seed)
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, kind == CL2_FIELD_SPINOR_EO || kind == CL2_FIELD_SU3VEC_EO, "kind must be a spinor or su3vec field");
     CL2_REQUIRE(c, out && valid_prec(prec), "bad argument");
-    CL2_HIP(c, launch_gaussian(c, out, prec, (size_t)(kind == CL2_FIELD_SPINOR_EO ? 12 : 3) * c->g.veo, seed));
+    if (kind == CL2_FIELD_SPINOR_EO) CL2_PAIRS_ALIGNED(c, prec, out);
+    CL2_HIP(c, launch_gaussian(c, out, prec, (size_t)(kind == CL2_FIELD_SPINOR_EO ? 12 : 3) * c->g.veo, seed,
+                               kind == CL2_FIELD_SPINOR_EO));
     return CL2_OK;
 }
 
@@ -1083,11 +1117,17 @@ int cl2_convert_precision_eo(cl2_ctx* c, int kind, void* dst, int dst_prec, cons
     else
         CL2_REQUIRE(c, valid_prec(dst_prec) && valid_prec(src_prec), "unknown precision (FP16 is for Wilson spinors)");
     const size_t n = (size_t)(kind == CL2_FIELD_SPINOR_EO ? 12 : 3) * c->g.veo;
+    if (kind == CL2_FIELD_SPINOR_EO) {
+        CL2_PAIRS_ALIGNED(c, dst_prec, dst);
+        CL2_PAIRS_ALIGNED(c, src_prec, src);
+    }
     if (dst_prec == src_prec)
         CL2_HIP(c, hipMemcpyAsync(dst, src, kind == CL2_FIELD_SPINOR_EO ? spinor_bytes(c, dst_prec) : su3vec_bytes(c, dst_prec),
                                   hipMemcpyDeviceToDevice, c->stream));
     else if (dst_prec == CL2_F16 || src_prec == CL2_F16)
         CL2_HIP(c, launch_convert_h(c, dst, dst_prec, src, src_prec));
+    else if (kind == CL2_FIELD_SPINOR_EO)
+        CL2_HIP(c, launch_convert_spinor(c, dst, dst_prec, src, src_prec, c->g.veo));
     else
         CL2_HIP(c, launch_convert(c, dst, dst_prec, src, src_prec, n));
     return CL2_OK;
@@ -1104,6 +1144,7 @@ int cl2_block_insert_eo(cl2_ctx* c, void* block, const void* single, int prec, i
     if (!c) return CL2_E_ARG;
     CL2_BLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, block && single && r >= 0 && r < nrhs, "bad argument");
+    CL2_PAIRS_ALIGNED(c, prec, block, single);
     CL2_HIP(c, launch_block_slot(c, block, single, nullptr, prec, nrhs, r));
     return CL2_OK;
 }
@@ -1113,6 +1154,7 @@ int cl2_block_extract_eo(cl2_ctx* c, void* single, const void* block, int prec, 
     if (!c) return CL2_E_ARG;
     CL2_BLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, block && single && r >= 0 && r < nrhs, "bad argument");
+    CL2_PAIRS_ALIGNED(c, prec, block, single);
     CL2_HIP(c, launch_block_slot(c, const_cast<void*>(block), nullptr, single, prec, nrhs, r));
     return CL2_OK;
 }
@@ -1123,6 +1165,7 @@ int cl2_block_axpy_eo(cl2_ctx* c, void* out, const void* x, const void* y, int p
     if (!c) return CL2_E_ARG;
     CL2_BLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, out && x && y && coef, "null pointer");
+    CL2_PAIRS_ALIGNED(c, prec, out, x, y);
     CL2_HIP(c, launch_block_axpy(c, out, x, y, prec, nrhs, coef, sign, norms));
     return CL2_OK;
 }
@@ -1134,6 +1177,7 @@ int cl2_block_cg_update_xp_eo(cl2_ctx* c, void* x, void* p, const void* r, int p
     CL2_BLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, x && p && r && alpha && beta, "null pointer");
     CL2_REQUIRE(c, x != p && x != r && p != r, "x, p and r must be different blocks");
+    CL2_PAIRS_ALIGNED(c, prec, x, p, r);
     CL2_HIP(c, launch_block_xp(c, x, p, r, prec, nrhs, alpha, beta));
     return CL2_OK;
 }
@@ -1143,6 +1187,7 @@ int cl2_block_dot_eo(cl2_ctx* c, double* res, const void* x, const void* y, int 
     if (!c) return CL2_E_ARG;
     CL2_BLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, res && x && y, "null pointer");
+    CL2_PAIRS_ALIGNED(c, prec, x, y);
     CL2_HIP(c, launch_block_dot(c, res, x, y, prec, nrhs, 0));
     return CL2_OK;
 }
@@ -1152,6 +1197,7 @@ int cl2_block_squarenorm_eo(cl2_ctx* c, double* res, const void* x, int prec, in
     if (!c) return CL2_E_ARG;
     CL2_BLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, res && x, "null pointer");
+    CL2_PAIRS_ALIGNED(c, prec, x);
     CL2_HIP(c, launch_block_dot(c, res, x, x, prec, nrhs, 1));
     return CL2_OK;
 }
@@ -1163,10 +1209,12 @@ int cl2_block_convert_eo(cl2_ctx* c, void* dst, int dst_prec, const void* src, i
     CL2_REQUIRE(c, valid_prec(src_prec), "block fields are FP64 or FP32");
     CL2_REQUIRE(c, dst && src && dst != src, "bad pointers");
     const size_t n = (size_t)12 * c->g.veo * nrhs;
+    CL2_PAIRS_ALIGNED(c, dst_prec, dst);
+    CL2_PAIRS_ALIGNED(c, src_prec, src);
     if (dst_prec == src_prec)
         CL2_HIP(c, hipMemcpyAsync(dst, src, n * (dst_prec == CL2_F64 ? 16 : 8), hipMemcpyDeviceToDevice, c->stream));
     else
-        CL2_HIP(c, launch_convert(c, dst, dst_prec, src, src_prec, n));
+        CL2_HIP(c, launch_convert_spinor(c, dst, dst_prec, src, src_prec, (size_t)c->g.veo * nrhs));
     return CL2_OK;
 }
 

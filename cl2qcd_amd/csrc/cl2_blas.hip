@@ -309,11 +309,10 @@ __global__ __launch_bounds__(256) void k_slice_norm(double* res, const vec2<R>* 
     double acc       = 0.;
     for (uint32_t i = threadIdx.x; i < nchunk * chunk; i += 256) {
         const uint32_t e = s * slice_off + (i / chunk) * chunk_stride + i % chunk;
+        vec2<R> v[12];
+        load_spinor<R>(v, x, S, e);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) {
-            const vec2<R> v = x[(size_t)k * S + e];
-            acc += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
-        }
+        for (int k = 0; k < 12; ++k) acc += (double)v[k].x * (double)v[k].x + (double)v[k].y * (double)v[k].y;
     }
     const double sum = block_sum<256>(acc, lds);
     if (threadIdx.x == 0) res[s] = accumulate ? res[s] + sum : sum;
@@ -372,7 +371,7 @@ hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, dou
     return hipGetLastError();
 }
 
-/* precision conversion between complex fields (FP64 <-> FP32) */
+/* precision conversion between complex fields of the same layout (FP64 <-> FP32 staggered fields) */
 template<typename RD, typename RS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_convert(vec2<RD>* __restrict__ out, const vec2<RS>* __restrict__ in,
                                                    size_t n)
@@ -382,6 +381,47 @@ __global__ __launch_bounds__(BLOCK) void k_convert(vec2<RD>* __restrict__ out, c
         vec2<RS> v = in[k];
         out[k]     = vec2<RD>{RD(v.x), RD(v.y)};
     }
+}
+
+/* FP64 <-> FP32 conversion of Wilson spinors (cs = Veo) or blocks of them (cs = Veo nrhs): FP64 [12][cs]
+ * complex numbers <-> FP32 [6][cs] component pairs; thread (o, q) converts components 2q and 2q+1 of
+ * site offset o */
+template<typename RD, typename RS>
+__global__ __launch_bounds__(256) void k_convert_spinor(vec2<RD>* __restrict__ out, const vec2<RS>* __restrict__ in,
+                                                        size_t cs)
+{
+    const size_t o = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t q = blockIdx.y;
+    if (o >= cs) return;
+    vec2<RS> a, b;
+    if constexpr (sizeof(RS) == 8) {
+        a = load_stream(in + 2 * q * cs + o);
+        b = load_stream(in + (2 * q + 1) * cs + o);
+    } else {
+        const float4 v = load_stream(reinterpret_cast<const float4*>(in) + q * cs + o);
+        a              = vec2<RS>{v.x, v.y};
+        b              = vec2<RS>{v.z, v.w};
+    }
+    if constexpr (sizeof(RD) == 8) {
+        store_stream(out + 2 * q * cs + o, vec2<RD>{RD(a.x), RD(a.y)});
+        store_stream(out + (2 * q + 1) * cs + o, vec2<RD>{RD(b.x), RD(b.y)});
+    } else {
+        store_stream(reinterpret_cast<float4*>(out) + q * cs + o, make_float4(float(a.x), float(a.y), float(b.x), float(b.y)));
+    }
+}
+
+hipError_t launch_convert_spinor(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t cs)
+{
+    const dim3 grid((unsigned)((cs + 255) / 256), 6);
+    if (out_prec == CL2_F32 && in_prec == CL2_F64)
+        hipLaunchKernelGGL((k_convert_spinor<float, double>), grid, dim3(256), 0, c->stream,
+                           reinterpret_cast<float2*>(out), reinterpret_cast<const double2*>(in), cs);
+    else if (out_prec == CL2_F64 && in_prec == CL2_F32)
+        hipLaunchKernelGGL((k_convert_spinor<double, float>), grid, dim3(256), 0, c->stream,
+                           reinterpret_cast<double2*>(out), reinterpret_cast<const float2*>(in), cs);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n)

@@ -1,7 +1,8 @@
 // Level-1 operations on blocks of nrhs Wilson eo spinors stored [12][Veo][nrhs]
-// (right-hand side fastest, the layout of the multi-RHS hopping kernel), with
-// one scalar per right-hand side, and the per-right-hand-side scalar
-// recurrences of a lockstep CG over the block.
+// (FP64) or [6][Veo][nrhs] of component pairs (FP32), right-hand side fastest,
+// the layout of the multi-RHS hopping kernel, and on blocks of staggered
+// fields [3][Veo][nrhs], with one scalar per right-hand side, and the
+// per-right-hand-side scalar recurrences of a lockstep CG over the block.
 //
 // Reductions are per right-hand side and deterministic: a thread always works on
 // one right-hand side (block sizes and grid strides are multiples of nrhs), a
@@ -14,10 +15,11 @@
 
 namespace cl2 {
 
-/* Elements are streamed in 16-byte chunks: one FP64 complex number, or two FP32 numbers (two adjacent
- * right-hand sides) when nrhs is even or 1 and the operands are 16-byte aligned.  A chunk column is the
- * chunk index modulo ncol = max(1, nrhs / CPC); a thread keeps one column, so slot j of its chunks always
- * belongs to right-hand side (q CPC + j) mod nrhs. */
+/* Elements are streamed in 16-byte chunks: one FP64 complex number, or two FP32 numbers.  For FP32 Wilson
+ * blocks a chunk is one component pair of one right-hand side ("pairs": both slots belong to right-hand
+ * side q, the chunk index modulo ncol = nrhs); for FP32 staggered blocks it holds two adjacent right-hand
+ * sides when nrhs is even or 1 and the operands are 16-byte aligned (slot j of chunk column q belongs to
+ * right-hand side (q CPC + j) mod nrhs, ncol = max(1, nrhs / CPC)).  A thread keeps one column. */
 template<typename R, int CPC>
 struct BChunk;
 template<>
@@ -60,10 +62,17 @@ struct BChunk<float, 1> {
 
 namespace {
 
+/* right-hand side of slot j of chunk column q */
+template<int CPC>
+__device__ __forceinline__ int slot_rhs(int q, int j, int nrhs, bool pairs)
+{
+    return pairs ? q : (q * CPC + j) % nrhs;
+}
+
 /* per-right-hand-side column sums of a block's thread values (CPC slots per thread), valid in threads
  * 0..nrhs-1 */
 template<int CPC>
-__device__ __forceinline__ double2 column_sum(const double2 (&v)[CPC], double2* lds, int nrhs, int ncol)
+__device__ __forceinline__ double2 column_sum(const double2 (&v)[CPC], double2* lds, int nrhs, int ncol, bool pairs)
 {
 #pragma unroll
     for (int j = 0; j < CPC; ++j) lds[threadIdx.x * CPC + j] = v[j];
@@ -71,7 +80,13 @@ __device__ __forceinline__ double2 column_sum(const double2 (&v)[CPC], double2* 
     double2 s   = make_double2(0., 0.);
     const int t = (int)threadIdx.x;
     if (t < nrhs) {
-        if (nrhs % CPC == 0) {
+        if (pairs) {  // column t is right-hand side t, both slots
+            for (unsigned k = (unsigned)t; k < blockDim.x; k += (unsigned)ncol)
+                for (int j = 0; j < CPC; ++j) {
+                    s.x += lds[k * CPC + j].x;
+                    s.y += lds[k * CPC + j].y;
+                }
+        } else if (nrhs % CPC == 0) {
             const int j = t % CPC;
             for (unsigned k = (unsigned)(t / CPC); k < blockDim.x; k += (unsigned)ncol) {
                 s.x += lds[k * CPC + j].x;
@@ -95,7 +110,8 @@ __device__ __forceinline__ double2 column_sum(const double2 (&v)[CPC], double2* 
  * loads issued first. */
 template<typename R, int CPC, int U>
 __global__ __launch_bounds__(256) void k_block_axpy(void* out, const void* x, const void* y, size_t nchunk, int nrhs,
-                                                    int ncol, const double* coef, double sign, double2* partials)
+                                                    int ncol, bool pairs, const double* coef, double sign,
+                                                    double2* partials)
 {
     using IO = BChunk<R, CPC>;
     __shared__ double2 lds[256 * CPC];
@@ -104,7 +120,7 @@ __global__ __launch_bounds__(256) void k_block_axpy(void* out, const void* x, co
     double2 acc[CPC];
 #pragma unroll
     for (int j = 0; j < CPC; ++j) {
-        const int r = (q * CPC + j) % nrhs;
+        const int r = slot_rhs<CPC>(q, j, nrhs, pairs);
         c[j]        = Cx<R>{R(sign * coef[2 * r]), R(sign * coef[2 * r + 1])};
         acc[j]      = make_double2(0., 0.);
     }
@@ -135,7 +151,7 @@ __global__ __launch_bounds__(256) void k_block_axpy(void* out, const void* x, co
         }
     }
     if (partials) {
-        const double2 s = column_sum<CPC>(acc, lds, nrhs, ncol);
+        const double2 s = column_sum<CPC>(acc, lds, nrhs, ncol, pairs);
         if ((int)threadIdx.x < nrhs) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
     }
 }
@@ -143,14 +159,14 @@ __global__ __launch_bounds__(256) void k_block_axpy(void* out, const void* x, co
 /* the x and p updates of a lockstep CG step in one pass: x = x + alpha_r p, then p = r + beta_r p */
 template<typename R, int CPC>
 __global__ __launch_bounds__(256) void k_block_xp(void* xo, void* p, const void* rv, size_t nchunk, int nrhs, int ncol,
-                                                  const double* alpha, const double* beta)
+                                                  bool pairs, const double* alpha, const double* beta)
 {
     using IO = BChunk<R, CPC>;
     const int q = (int)(threadIdx.x % (unsigned)ncol);
     Cx<R> ca[CPC], cb[CPC];
 #pragma unroll
     for (int j = 0; j < CPC; ++j) {
-        const int r = (q * CPC + j) % nrhs;
+        const int r = slot_rhs<CPC>(q, j, nrhs, pairs);
         ca[j]       = Cx<R>{R(alpha[2 * r]), R(alpha[2 * r + 1])};
         cb[j]       = Cx<R>{R(beta[2 * r]), R(beta[2 * r + 1])};
     }
@@ -173,7 +189,7 @@ __global__ __launch_bounds__(256) void k_block_xp(void* xo, void* p, const void*
 /* per right-hand side: sum conj(x) y (mode 0) or sum |x|^2 (mode 1) */
 template<typename R, int CPC, int U>
 __global__ __launch_bounds__(256) void k_block_dot(const void* x, const void* y, size_t nchunk, int nrhs, int ncol,
-                                                   int mode, double2* partials)
+                                                   bool pairs, int mode, double2* partials)
 {
     using IO = BChunk<R, CPC>;
     __shared__ double2 lds[256 * CPC];
@@ -206,7 +222,7 @@ __global__ __launch_bounds__(256) void k_block_dot(const void* x, const void* y,
             }
         }
     }
-    const double2 s = column_sum<CPC>(acc, lds, nrhs, ncol);
+    const double2 s = column_sum<CPC>(acc, lds, nrhs, ncol, pairs);
     if ((int)threadIdx.x < nrhs) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
 }
 
@@ -234,10 +250,10 @@ __global__ __launch_bounds__(BLOCK) void k_block_reduce2(const double2* partials
     }
 }
 
-/* copy single field <-> slot r of a block */
-template<typename R>
-__global__ __launch_bounds__(256) void k_block_slot(vec2<R>* block, const vec2<R>* single, vec2<R>* single_out,
-                                                    size_t n_single, int nrhs, int r)
+/* copy single field <-> slot r of a block, in units of T (one complex number, or one FP32 component pair) */
+template<typename T>
+__global__ __launch_bounds__(256) void k_block_slot(T* block, const T* single, T* single_out, size_t n_single, int nrhs,
+                                                    int r)
 {
     const size_t step = (size_t)gridDim.x * 256;
     for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < n_single; e += step) {
@@ -317,6 +333,7 @@ hipError_t launch_scalar_copy(cl2_ctx* c, double* dst, const double* src, int n)
  * partials at most what the partials buffer holds, beyond that the grid loops) */
 struct BlockShape {
     int cpc, ncol;
+    bool pairs;
     unsigned bt, nb;
     size_t nchunk;
 };
@@ -326,14 +343,17 @@ static bool aligned16(const void* p)
     return p == nullptr || (reinterpret_cast<uintptr_t>(p) & 15u) == 0;
 }
 
+/* FP32 Wilson blocks (ncomp 12) are arrays of component pairs: always 16-byte chunks, one right-hand side each
+ * (the caller has checked the 16-byte alignment) */
 static BlockShape block_shape(cl2_ctx* c, size_t n, int prec, int nrhs, int u, bool partials,
-                              std::initializer_list<const void*> ptrs)
+                              std::initializer_list<const void*> ptrs, int ncomp)
 {
     BlockShape b;
-    bool pairs = prec != CL2_F64 && (nrhs % 2 == 0 || nrhs == 1) && n % 2 == 0;
-    for (const void* p : ptrs) pairs = pairs && aligned16(p);
-    b.cpc    = pairs ? 2 : 1;
-    b.ncol   = nrhs % b.cpc == 0 ? nrhs / b.cpc : 1;
+    b.pairs   = prec != CL2_F64 && ncomp == 12;
+    bool wide = prec != CL2_F64 && (b.pairs || ((nrhs % 2 == 0 || nrhs == 1) && n % 2 == 0));
+    for (const void* p : ptrs) wide = wide && (b.pairs || aligned16(p));
+    b.cpc    = wide ? 2 : 1;
+    b.ncol   = b.pairs ? nrhs : (nrhs % b.cpc == 0 ? nrhs / b.cpc : 1);
     b.bt     = 256u - (256u % (unsigned)b.ncol);
     b.nchunk = n / (size_t)b.cpc;
     size_t nb = (b.nchunk + (size_t)b.bt * u - 1) / ((size_t)b.bt * u);
@@ -356,29 +376,29 @@ hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y
 {
     const size_t n = (size_t)ncomp * c->g.veo * nrhs;
     const int u    = norms ? BLOCK_U_REDUCE : 1;
-    const BlockShape b = block_shape(c, n, prec, nrhs, u, norms != nullptr, {out, x, y});
+    const BlockShape b = block_shape(c, n, prec, nrhs, u, norms != nullptr, {out, x, y}, ncomp);
     double2* parts     = norms ? reinterpret_cast<double2*>(c->partials) : nullptr;
     if (prec == CL2_F64) {
         if (norms)
             hipLaunchKernelGGL((k_block_axpy<double, 1, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x,
-                               y, b.nchunk, nrhs, b.ncol, coef, sign, parts);
+                               y, b.nchunk, nrhs, b.ncol, b.pairs, coef, sign, parts);
         else
             hipLaunchKernelGGL((k_block_axpy<double, 1, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x, y, b.nchunk,
-                               nrhs, b.ncol, coef, sign, parts);
+                               nrhs, b.ncol, b.pairs, coef, sign, parts);
     } else if (b.cpc == 2) {
         if (norms)
             hipLaunchKernelGGL((k_block_axpy<float, 2, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x,
-                               y, b.nchunk, nrhs, b.ncol, coef, sign, parts);
+                               y, b.nchunk, nrhs, b.ncol, b.pairs, coef, sign, parts);
         else
             hipLaunchKernelGGL((k_block_axpy<float, 2, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x, y, b.nchunk,
-                               nrhs, b.ncol, coef, sign, parts);
+                               nrhs, b.ncol, b.pairs, coef, sign, parts);
     } else {
         if (norms)
             hipLaunchKernelGGL((k_block_axpy<float, 1, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x,
-                               y, b.nchunk, nrhs, b.ncol, coef, sign, parts);
+                               y, b.nchunk, nrhs, b.ncol, b.pairs, coef, sign, parts);
         else
             hipLaunchKernelGGL((k_block_axpy<float, 1, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, out, x, y, b.nchunk,
-                               nrhs, b.ncol, coef, sign, parts);
+                               nrhs, b.ncol, b.pairs, coef, sign, parts);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !norms) return e;
@@ -389,16 +409,16 @@ hipError_t launch_block_xp(cl2_ctx* c, void* x, void* p, const void* r, int prec
                            const double* beta, int ncomp)
 {
     const size_t n     = (size_t)ncomp * c->g.veo * nrhs;
-    const BlockShape b = block_shape(c, n, prec, nrhs, 1, false, {x, p, r});
+    const BlockShape b = block_shape(c, n, prec, nrhs, 1, false, {x, p, r}, ncomp);
     if (prec == CL2_F64)
         hipLaunchKernelGGL((k_block_xp<double, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, p, r, b.nchunk, nrhs,
-                           b.ncol, alpha, beta);
+                           b.ncol, b.pairs, alpha, beta);
     else if (b.cpc == 2)
         hipLaunchKernelGGL((k_block_xp<float, 2>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, p, r, b.nchunk, nrhs,
-                           b.ncol, alpha, beta);
+                           b.ncol, b.pairs, alpha, beta);
     else
         hipLaunchKernelGGL((k_block_xp<float, 1>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, p, r, b.nchunk, nrhs,
-                           b.ncol, alpha, beta);
+                           b.ncol, b.pairs, alpha, beta);
     return hipGetLastError();
 }
 
@@ -406,17 +426,17 @@ hipError_t launch_block_dot(cl2_ctx* c, double* res, const void* x, const void* 
                             int ncomp)
 {
     const size_t n     = (size_t)ncomp * c->g.veo * nrhs;
-    const BlockShape b = block_shape(c, n, prec, nrhs, BLOCK_U_REDUCE, true, {x, y});
+    const BlockShape b = block_shape(c, n, prec, nrhs, BLOCK_U_REDUCE, true, {x, y}, ncomp);
     double2* parts     = reinterpret_cast<double2*>(c->partials);
     if (prec == CL2_F64)
         hipLaunchKernelGGL((k_block_dot<double, 1, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, y,
-                           b.nchunk, nrhs, b.ncol, mode, parts);
+                           b.nchunk, nrhs, b.ncol, b.pairs, mode, parts);
     else if (b.cpc == 2)
         hipLaunchKernelGGL((k_block_dot<float, 2, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, y,
-                           b.nchunk, nrhs, b.ncol, mode, parts);
+                           b.nchunk, nrhs, b.ncol, b.pairs, mode, parts);
     else
         hipLaunchKernelGGL((k_block_dot<float, 1, BLOCK_U_REDUCE>), dim3(b.nb), dim3(b.bt), 0, c->stream, x, y,
-                           b.nchunk, nrhs, b.ncol, mode, parts);
+                           b.nchunk, nrhs, b.ncol, b.pairs, mode, parts);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_block_reduce2(c, parts, b.nb, res, mode == 1 ? 1 : 0, nrhs);
@@ -428,11 +448,15 @@ hipError_t launch_block_slot(cl2_ctx* c, void* block, const void* single_in, voi
     const size_t n    = (size_t)ncomp * c->g.veo;
     const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
     if (prec == CL2_F64)
-        hipLaunchKernelGGL(k_block_slot<double>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<double2*>(block),
+        hipLaunchKernelGGL(k_block_slot<double2>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<double2*>(block),
                            reinterpret_cast<const double2*>(single_in), reinterpret_cast<double2*>(single_out), n, nrhs,
                            r);
+    else if (ncomp == 12)  // FP32 Wilson: component pairs
+        hipLaunchKernelGGL(k_block_slot<float4>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<float4*>(block),
+                           reinterpret_cast<const float4*>(single_in), reinterpret_cast<float4*>(single_out), n / 2,
+                           nrhs, r);
     else
-        hipLaunchKernelGGL(k_block_slot<float>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<float2*>(block),
+        hipLaunchKernelGGL(k_block_slot<float2>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<float2*>(block),
                            reinterpret_cast<const float2*>(single_in), reinterpret_cast<float2*>(single_out), n, nrhs,
                            r);
     return hipGetLastError();

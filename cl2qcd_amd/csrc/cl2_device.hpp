@@ -333,6 +333,86 @@ __device__ __forceinline__ void store_stream(float4* p, float4 v)
     __builtin_nontemporal_store(w, reinterpret_cast<nt_f4*>(p));
 }
 
+/* ---------------- Wilson spinor storage ----------------
+ * An eo Wilson spinor field, or a block of N of them, is component-major.  FP64 fields hold component k
+ * (k = 3 spin + colour) of site offset o at complex index k CS + o: [12][V] (CS = V, o = site) or
+ * [12][V][N] (CS = V N, o = site N + right-hand side).  FP32 fields hold the components in pairs:
+ * components 2q and 2q+1 of site offset o share the 16-byte lane q CS + o, [6][V] or [6][V][N] of float4,
+ * so a neighbour gather moves 16 bytes per lane in either precision (for FP32 twice the bytes per load
+ * instruction of one complex number per lane, half the instructions).  In both layouts the upper spin
+ * components 0-5 fill the first half of the array, which is all the elementwise kernels (gamma5, the
+ * site-diagonal factors) rely on. */
+template<typename R>
+__device__ __forceinline__ size_t spinor_cx(int k, size_t CS, size_t o)
+{
+    if constexpr (sizeof(R) == 8)
+        return (size_t)k * CS + o;
+    else
+        return (size_t)(k >> 1) * 2 * CS + 2 * o + (size_t)(k & 1);
+}
+/* the 12 components of site offset o (cached loads: neighbour spinors are reused through the L2) */
+template<typename R>
+__device__ __forceinline__ void load_spinor(vec2<R> (&s)[12], const vec2<R>* __restrict__ in, size_t CS, size_t o)
+{
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s[k] = in[(size_t)k * CS + o];
+    } else {
+        const float4* p = reinterpret_cast<const float4*>(in);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const float4 v = p[(size_t)q * CS + o];
+            s[2 * q]       = vec2<R>{v.x, v.y};
+            s[2 * q + 1]   = vec2<R>{v.z, v.w};
+        }
+    }
+}
+/* the same with streaming loads (data read once) */
+template<typename R>
+__device__ __forceinline__ void load_spinor_stream(vec2<R> (&s)[12], const vec2<R>* in, size_t CS, size_t o)
+{
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) s[k] = load_stream(in + (size_t)k * CS + o);
+    } else {
+        const float4* p = reinterpret_cast<const float4*>(in);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) {
+            const float4 v = load_stream(p + (size_t)q * CS + o);
+            s[2 * q]       = vec2<R>{v.x, v.y};
+            s[2 * q + 1]   = vec2<R>{v.z, v.w};
+        }
+    }
+}
+/* streaming store of 12 components to site offset o */
+template<typename R>
+__device__ __forceinline__ void store_spinor_stream(vec2<R>* out, size_t CS, size_t o, const vec2<R> (&s)[12])
+{
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) store_stream(out + (size_t)k * CS + o, s[k]);
+    } else {
+        float4* p = reinterpret_cast<float4*>(out);
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            store_stream(p + (size_t)q * CS + o, make_float4(s[2 * q].x, s[2 * q].y, s[2 * q + 1].x, s[2 * q + 1].y));
+    }
+}
+/* plain store of 12 components to site offset o */
+template<typename R>
+__device__ __forceinline__ void store_spinor(vec2<R>* out, size_t CS, size_t o, const vec2<R> (&s)[12])
+{
+    if constexpr (sizeof(R) == 8) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) out[(size_t)k * CS + o] = s[k];
+    } else {
+        float4* p = reinterpret_cast<float4*>(out);
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+            p[(size_t)q * CS + o] = make_float4(s[2 * q].x, s[2 * q].y, s[2 * q + 1].x, s[2 * q + 1].y);
+    }
+}
+
 /* ---------------- gauge-link formats ----------------
  * Device layout: link_elem (tiles of 64 eo sites per (dir, parity), lane-major inside a tile), where a
  * lane is one complex entry (FP64) or a pair of complex entries packed into a 16-byte vector (FP32);

@@ -112,26 +112,32 @@ __global__ void k_gauge_convert(void* dst, int dfmt, const double2* __restrict__
     }
 }
 
-/* complex Gaussian field, N(0, 1) per real component, from the counter hash of (seed, element) (Box-Muller) */
+/* complex Gaussian field, N(0, 1) per real component, from the counter hash of (seed, element) (Box-Muller).
+ * e is the element's index in the component-major order k cs + o; with pair_cs != 0 (FP32 Wilson spinors,
+ * cs = pair_cs) it is stored where the component-pair layout keeps it, so a seed gives the same field in
+ * either precision. */
 template<typename R>
-__global__ void k_gaussian(vec2<R>* out, size_t n, uint64_t seed)
+__global__ void k_gaussian(vec2<R>* out, size_t n, uint64_t seed, size_t pair_cs)
 {
     const size_t step = (size_t)gridDim.x * blockDim.x;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += step) {
         const uint64_t ctr = (seed * 0x100000001b3ull) ^ (e * 2ull);
         const double r1 = u01(mix64(ctr)), r2 = u01(mix64(ctr + 1));
         const double rad = sqrt(-2.0 * log(r1)), ph = 6.283185307179586 * r2;
-        out[e]           = vec2<R>{R(rad * cos(ph)), R(rad * sin(ph))};
+        const size_t at  = pair_cs ? spinor_cx<float>((int)(e / pair_cs), pair_cs, e % pair_cs) : e;
+        out[at]          = vec2<R>{R(rad * cos(ph)), R(rad * sin(ph))};
     }
 }
 
-hipError_t launch_gaussian(cl2_ctx* c, void* dev, int prec, size_t n, uint64_t seed)
+hipError_t launch_gaussian(cl2_ctx* c, void* dev, int prec, size_t n, uint64_t seed, bool wilson)
 {
     const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 16384);
     if (prec == CL2_F64)
-        hipLaunchKernelGGL(k_gaussian<double>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<double2*>(dev), n, seed);
+        hipLaunchKernelGGL(k_gaussian<double>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<double2*>(dev), n, seed,
+                           (size_t)0);
     else
-        hipLaunchKernelGGL(k_gaussian<float>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<float2*>(dev), n, seed);
+        hipLaunchKernelGGL(k_gaussian<float>, dim3(nb), dim3(256), 0, c->stream, reinterpret_cast<float2*>(dev), n, seed,
+                           wilson ? n / 12 : (size_t)0);
     return hipGetLastError();
 }
 

@@ -459,18 +459,17 @@ hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im)
     return hipGetLastError();
 }
 
-/* FP64 / FP32 component-major spinor <-> FP16 with per-site norm */
+/* FP64 / FP32 spinor (the layouts of load_spinor) <-> FP16 with per-site norm */
 template<typename R>
 __global__ __launch_bounds__(BLAS_BLOCK) void k_to_half(void* out, const vec2<R>* __restrict__ in, uint32_t S)
 {
     const uint32_t i = blockIdx.x * BLAS_BLOCK + threadIdx.x;
     if (i >= S) return;
+    vec2<R> q[12];
+    load_spinor<R>(q, in, S, i);
     Cx<float> v[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) {
-        vec2<R> q = in[(size_t)k * S + i];
-        v[k]      = Cx<float>{float(q.x), float(q.y)};
-    }
+    for (int k = 0; k < 12; ++k) v[k] = Cx<float>{float(q[k].x), float(q[k].y)};
     store_site_h(out, S, i, v);
 }
 template<typename R>
@@ -480,8 +479,10 @@ __global__ __launch_bounds__(BLAS_BLOCK) void k_from_half(vec2<R>* __restrict__ 
     if (i >= S) return;
     Cx<float> v[12];
     load_site_h(in, S, i, v);
+    vec2<R> q[12];
 #pragma unroll
-    for (int k = 0; k < 12; ++k) out[(size_t)k * S + i] = vec2<R>{R(v[k].re), R(v[k].im)};
+    for (int k = 0; k < 12; ++k) q[k] = vec2<R>{R(v[k].re), R(v[k].im)};
+    store_spinor<R>(out, S, i, q);
 }
 
 hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec)

@@ -130,7 +130,7 @@ hipError_t launch_gauge_update(cl2_ctx* c, void* gauge, double beta, int mode, u
 hipError_t launch_plaquette(cl2_ctx* c, const void* gauge, double* res);
 hipError_t launch_gauge_random(cl2_ctx* c, void* dev, int fmt, uint64_t seed, double eps);
 hipError_t launch_gauge_convert(cl2_ctx* c, void* dst, int dfmt, const void* src);
-hipError_t launch_gaussian(cl2_ctx* c, void* dev, int prec, size_t n, uint64_t seed);
+hipError_t launch_gaussian(cl2_ctx* c, void* dev, int prec, size_t n, uint64_t seed, bool wilson);
 
 hipError_t launch_fill(cl2_ctx* c, void* out, size_t n, int prec, double re, double im);
 hipError_t launch_copy16(cl2_ctx* c, void* dst, const void* src, size_t n);
@@ -144,6 +144,7 @@ hipError_t launch_sitediag_h(cl2_ctx* c, void* out, const void* in, const double
 hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im);
 hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec);
 hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n);
+hipError_t launch_convert_spinor(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t cs);
 
 /* blocks of nrhs eo fields [ncomp][Veo][nrhs]: Wilson spinors (ncomp 12) or staggered vectors (3) */
 hipError_t launch_block_axpy(cl2_ctx* c, void* out, const void* x, const void* y, int prec, int nrhs,

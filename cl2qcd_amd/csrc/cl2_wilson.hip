@@ -82,14 +82,10 @@ __device__ __forceinline__ void hop_quad(Cx<R> (&acc)[12], const vec2<R>* __rest
     GaugeLoad<FMT>::load(gauge, S, D0, P ^ 1, nb.dn[D0], u0b);
     GaugeLoad<FMT>::load(gauge, S, D1, P, i, u1f);
     GaugeLoad<FMT>::load(gauge, S, D1, P ^ 1, nb.dn[D1], u1b);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) s0f[k] = in[(size_t)k * SS + nb.up[D0] * NRHS + rhs];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) s0b[k] = in[(size_t)k * SS + nb.dn[D0] * NRHS + rhs];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) s1f[k] = in[(size_t)k * SS + nb.up[D1] * NRHS + rhs];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) s1b[k] = in[(size_t)k * SS + nb.dn[D1] * NRHS + rhs];
+    load_spinor<R>(s0f, in, SS, (size_t)nb.up[D0] * NRHS + rhs);
+    load_spinor<R>(s0b, in, SS, (size_t)nb.dn[D0] * NRHS + rhs);
+    load_spinor<R>(s1f, in, SS, (size_t)nb.up[D1] * NRHS + rhs);
+    load_spinor<R>(s1b, in, SS, (size_t)nb.dn[D1] * NRHS + rhs);
     apply_hop<D0, true>(acc, s0f, u0f, Cx<R>{R(w.wf[D0][0]), R(w.wf[D0][1])});
     apply_hop<D0, false>(acc, s0b, u0b, Cx<R>{R(w.wb[D0][0]), R(w.wb[D0][1])});
     apply_hop<D1, true>(acc, s1f, u1f, Cx<R>{R(w.wf[D1][0]), R(w.wf[D1][1])});
@@ -107,16 +103,14 @@ __device__ __forceinline__ void hop_seq(Cx<R> (&acc)[12], const vec2<R>* __restr
         Su3<R> u;
         vec2<R> sp[12];
         GaugeLoad<FMT>::load(gauge, S, DIR, P, i, u);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) sp[k] = in[(size_t)k * SS + nf];
+        load_spinor<R>(sp, in, SS, nf);
         apply_hop<DIR, true>(acc, sp, u, wf);
     }
     {
         Su3<R> u;
         vec2<R> sp[12];
         GaugeLoad<FMT>::load(gauge, S, DIR, P ^ 1, linkbk, u);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) sp[k] = in[(size_t)k * SS + nbk];
+        load_spinor<R>(sp, in, SS, nbk);
         apply_hop<DIR, false>(acc, sp, u, wb);
     }
 }
@@ -130,10 +124,8 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
     vec2<R> sf[12], sb[12];
     GaugeLoad<FMT>::load(gauge, S, DIR, P, i, uf);
     GaugeLoad<FMT>::load(gauge, S, DIR, P ^ 1, linkbk, ub);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) sf[k] = in[(size_t)k * SS + nf];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) sb[k] = in[(size_t)k * SS + nbk];
+    load_spinor<R>(sf, in, SS, nf);
+    load_spinor<R>(sb, in, SS, nbk);
     apply_hop<DIR, true>(acc, sf, uf, wf);
     apply_hop<DIR, false>(acc, sb, ub, wb);
 }
@@ -162,29 +154,32 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
         if (a.epilogue == EPI_MINV) {                                                                             \
             _Pragma("unroll") for (int k = 0; k < 12; ++k) acc[k] = scale(cmul(acc[k], k < 6 ? f1 : f2), s_);    \
         } else if (a.epilogue == EPI_AEE) {                                                                       \
-            const V* x_ = reinterpret_cast<const V*>(a.x);                                                        \
+            V xs_[12];                                                                                            \
+            load_spinor<R>(xs_, reinterpret_cast<const V*>(a.x), SS, o_);                                         \
             _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                      \
-                V xv   = x_[(size_t)k * SS + o_];                                                                 \
-                acc[k] = cmul(Cx<R>{xv.x, xv.y}, k < 6 ? f1 : f2) - acc[k];                                       \
+                acc[k] = cmul(Cx<R>{xs_[k].x, xs_[k].y}, k < 6 ? f1 : f2) - acc[k];                               \
                 if (a.gamma5 && k >= 6) acc[k] = Cx<R>{-acc[k].re, -acc[k].im};                                   \
             }                                                                                                     \
         }                                                                                                         \
+        V ov_[12];                                                                                                \
         if (a.cg_r) {                                                                                             \
             V* r_ = reinterpret_cast<V*>(a.cg_r);                                                                 \
             const Cx<R> al_{R(a.cg_alpha[2 * rhs]), R(a.cg_alpha[2 * rhs + 1])};                                  \
+            V rv_[12];                                                                                            \
+            load_spinor_stream<R>(rv_, r_, SS, o_);                                                               \
             _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                      \
-                const V rv     = load_stream(r_ + (size_t)k * SS + o_);                                           \
-                const Cx<R> rn = Cx<R>{rv.x, rv.y} - cmul(acc[k], al_);                                           \
-                store_stream(r_ + (size_t)k * SS + o_, V{rn.re, rn.im});                                          \
+                const Cx<R> rn = Cx<R>{rv_[k].x, rv_[k].y} - cmul(acc[k], al_);                                   \
+                ov_[k]         = V{rn.re, rn.im};                                                                 \
                 NRM += (double)rn.re * (double)rn.re + (double)rn.im * (double)rn.im;                             \
-                RY += (double)rn.re * (double)rv.x + (double)rn.im * (double)rv.y;                                \
+                RY += (double)rn.re * (double)rv_[k].x + (double)rn.im * (double)rv_[k].y;                        \
             }                                                                                                     \
+            store_spinor_stream<R>(r_, SS, o_, ov_);                                                              \
         } else {                                                                                                  \
-            V* out_ = reinterpret_cast<V*>(a.out);                                                                \
             _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                      \
-                store_stream(out_ + (size_t)k * SS + o_, V{acc[k].re, acc[k].im});                                \
+                ov_[k] = V{acc[k].re, acc[k].im};                                                                 \
                 NRM += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;             \
             }                                                                                                     \
+            store_spinor_stream<R>(reinterpret_cast<V*>(a.out), SS, o_, ov_);                                     \
         }                                                                                                         \
     }
 
@@ -237,10 +232,10 @@ void k_dslash_eo(DslashArgs a)
 #undef CL2_DIR
 #undef CL2_W
         if constexpr (PLAIN) {
-            V* out_        = reinterpret_cast<V*>(a.out);
-            const size_t o = (size_t)i * NRHS + rhs;
+            V ov[12];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) store_stream(out_ + (size_t)k * SS + o, V{acc[k].re, acc[k].im});
+            for (int k = 0; k < 12; ++k) ov[k] = V{acc[k].re, acc[k].im};
+            store_spinor_stream<R>(reinterpret_cast<V*>(a.out), SS, (size_t)i * NRHS + rhs, ov);
         } else {
             CL2_FINISH_SITE((size_t)i * NRHS + rhs, nrm, ry)
         }
@@ -350,8 +345,8 @@ __global__ __launch_bounds__((MrhsLds<NRHS, SBX>::BLOCK)) void k_dslash_eo_mrhs_
         } else {                                                                                                \
             const size_t ef = (size_t)lnb[s * 8 + 2 * (m)] * NRHS + rhs;                                        \
             const size_t eb = (size_t)lnb[s * 8 + 2 * (m) + 1] * NRHS + rhs;                                    \
-            _Pragma("unroll") for (int k = 0; k < 12; ++k) sf[k] = in[(size_t)k * SS + ef];                     \
-            _Pragma("unroll") for (int k = 0; k < 12; ++k) sb[k] = in[(size_t)k * SS + eb];                     \
+            load_spinor<R>(sf, in, SS, ef);                                                                     \
+            load_spinor<R>(sb, in, SS, eb);                                                                     \
         }                                                                                                       \
         const Su3<R> uf = lu[s * 8 + 2 * (m)], ub = lu[s * 8 + 2 * (m) + 1];                                    \
         apply_hop<m, true>(acc, sf, uf, Cx<R>{R(a.w.wf[m][0]), R(a.w.wf[m][1])});                             \
@@ -393,8 +388,8 @@ __device__ __forceinline__ void half_hop_rb(Cx<R> (&acc)[RB][12], const vec2<R>*
         Cx<R> h[3], r[3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            vec2<R> a = in[(size_t)(3 * A + c) * SS + nb + q];
-            vec2<R> b = in[(size_t)(3 * B + c) * SS + nb + q];
+            vec2<R> a = in[spinor_cx<R>(3 * A + c, SS, (size_t)nb + q)];
+            vec2<R> b = in[spinor_cx<R>(3 * B + c, SS, (size_t)nb + q)];
             h[c]      = add_unit<CU>(Cx<R>{a.x, a.y}, Cx<R>{b.x, b.y});
         }
         if constexpr (FWD)
@@ -477,13 +472,13 @@ __global__ __launch_bounds__(BLOCK) void k_dslash_eo_rb(DslashArgs a)
         } else if (a.epilogue == EPI_AEE) {
 #pragma unroll
             for (int k = 0; k < 12; ++k) {
-                V xv      = x[(size_t)k * SS + o + q];
+                V xv      = x[spinor_cx<R>(k, SS, (size_t)o + q)];
                 acc[q][k] = cmul(Cx<R>{xv.x, xv.y}, k < 6 ? f1 : f2) - acc[q][k];
                 if (a.gamma5 && k >= 6) acc[q][k] = Cx<R>{-acc[q][k].re, -acc[q][k].im};
             }
         }
 #pragma unroll
-        for (int k = 0; k < 12; ++k) out[(size_t)k * SS + o + q] = V{acc[q][k].re, acc[q][k].im};
+        for (int k = 0; k < 12; ++k) out[spinor_cx<R>(k, SS, (size_t)o + q)] = V{acc[q][k].re, acc[q][k].im};
     }
 }
 
@@ -496,12 +491,14 @@ __global__ __launch_bounds__(BLOCK) void k_sitediag(vec2<R>* __restrict__ out, c
 {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i >= S) return;
+    vec2<R> v[12];
+    load_spinor<R>(v, in, S, i);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-        vec2<R> v = in[(size_t)k * S + i];
-        Cx<R> r   = scale(cmul(Cx<R>{v.x, v.y}, k < 6 ? f1 : f2), s);
-        out[(size_t)k * S + i] = vec2<R>{r.re, r.im};
+        Cx<R> r = scale(cmul(Cx<R>{v[k].x, v[k].y}, k < 6 ? f1 : f2), s);
+        v[k]    = vec2<R>{r.re, r.im};
     }
+    store_spinor<R>(out, S, i, v);
 }
 
 cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs)

@@ -10,7 +10,7 @@
  *
  * Device layouts (spinor fields are dense, stride = number of eo sites Veo):
  *   Wilson eo spinor  F64: double2[12][Veo]  component c = 3*spin + colour
- *                     F32: float2 [12][Veo]
+ *                     F32: float4 [6][Veo]   components 2q, 2q+1 as (re, im, re, im), 16-byte aligned
  *                     F16: half[3][Veo][8] (components 4j..4j+3 as re,im pairs) then float norm[Veo];
  *                          component = half * norm (norm = largest |re|,|im| on the site)
  *   staggered eo vec  F64: double2[3][Veo],  F32: float2[3][Veo]
@@ -261,9 +261,9 @@ int cl2_convert_precision_eo(cl2_ctx* ctx, int kind, void* dst, int dst_prec, co
 /* ==== blocks of right-hand sides ====
  * No counterpart in the reference, which inverts the 12 spin-colour sources of a propagator one after
  * the other (physics/algorithms/inversion.cpp:78-242).  A block holds nrhs Wilson eo spinors as one
- * [12][Veo][nrhs] complex array (right-hand side fastest), so one launch of an operator reuses every
- * loaded link for all right-hand sides.  Block fields are FP64 or FP32; per-right-hand-side scalars are
- * device arrays (complex: 2 doubles each). */
+ * double2[12][Veo][nrhs] (FP64) or float4[6][Veo][nrhs] (FP32, component pairs as in a single field)
+ * array, right-hand side fastest, so one launch of an operator reuses every loaded link for all
+ * right-hand sides.  Per-right-hand-side scalars are device arrays (complex: 2 doubles each). */
 /* the eo operators of cl2_fermionmatrix_eo / cl2_dslash_eo on nrhs = 1, 2, 4, 8, 12 or 16 right-hand
  * sides; tmp = 2 block fields */
 int cl2_fermionmatrix_eo_mrhs(cl2_ctx* ctx, int op, void* out, const void* in, const void* gauge, int format,

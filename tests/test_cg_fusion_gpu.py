@@ -7,8 +7,8 @@
 * cl2_fermionmatrix_eo_cg_step: the same bits as the unfused operator / ratio / saxpy_AND_cgnorm sequence
   (FP64, FP32), FP16 to 5e-3, and the oracle's step in FP64 to 1e-12.
 * cl2_cg_update_xp: bit-identical to the two saxpy calls it replaces (Wilson FP64/FP32/FP16) and to the
-  oracle's x - a p, r - b p (staggered); the FP32 kernel's unaligned path agrees bit for bit with the
-  aligned one.
+  oracle's x - a p, r - b p (staggered); the FP32 BLAS kernels' unaligned path (staggered fields) agrees bit
+  for bit with the aligned one, and FP32 Wilson spinors off a 16-byte boundary are refused.
 """
 import numpy as np
 import pytest
@@ -145,25 +145,34 @@ def test_cg_update_xp_staggered(prec):
 
 
 def test_blas_fp32_unaligned_matches_aligned():
-    """FP32 operands 8 bytes off a 16-byte boundary take the one-complex-per-lane path: same bits."""
+    """FP32 staggered operands 8 bytes off a 16-byte boundary take the one-complex-per-lane path: same bits.
+    FP32 Wilson spinors are stored as 16-byte component pairs: 8 bytes off, the entry points refuse them."""
     with abi.Context(8, 4) as ctx:
         rng = np.random.default_rng(53)
-        X, Y = rng.standard_normal((ctx.veo, 24)), rng.standard_normal((ctx.veo, 24))
-        nb = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F32)
+        X, Y = rng.standard_normal((ctx.veo, 6)), rng.standard_normal((ctx.veo, 6))
+        nb = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SU3VEC_EO, abi.F32)
         bx, by, bo = ctx.buffer(nb + 16), ctx.buffer(nb + 16), ctx.buffer(nb + 16)
         ux, uy, uo = bx.ptr + 8, by.ptr + 8, bo.ptr + 8
         for ptr, v in ((ux, X), (uy, Y)):
             h = np.ascontiguousarray(v, dtype=np.float64)
-            ctx.spinor_import(ptr, h.ctypes.data, abi.F32)
-        fx, fy, fo = ctx.spinor(abi.F32).upload(X), ctx.spinor(abi.F32).upload(Y), ctx.spinor(abi.F32)
+            ctx.su3vec_import(ptr, h.ctypes.data, abi.F32)
+        fx, fy, fo = ctx.su3vec(abi.F32).upload(X), ctx.su3vec(abi.F32).upload(Y), ctx.su3vec(abi.F32)
         a = abi.scalar(0.3 - 0.7j)
-        ctx.saxpy_eo(fo.ptr, fx.ptr, fy.ptr, abi.F32, a)
-        ctx.saxpy_eo(uo, ux, uy, abi.F32, a)
-        out = np.empty((ctx.veo, 24))
-        ctx.spinor_export(out.ctypes.data, uo, abi.F32)
+        ctx.saxpy_stagg_eo(fo.ptr, fx.ptr, fy.ptr, abi.F32, a)
+        ctx.saxpy_stagg_eo(uo, ux, uy, abi.F32, a)
+        out = np.empty((ctx.veo, 6))
+        ctx.su3vec_export(out.ctypes.data, uo, abi.F32)
         assert np.array_equal(out, fo.download())
         res = ctx.scalars(4)
-        ctx.scalar_product_eo(res.at(0), fx.ptr, fy.ptr, abi.F32)
-        ctx.scalar_product_eo(res.at(2), ux, uy, abi.F32)
+        ctx.scalar_product_stagg_eo(res.at(0), fx.ptr, fy.ptr, abi.F32)
+        ctx.scalar_product_stagg_eo(res.at(2), ux, uy, abi.F32)
         d = res.get()
         assert abs(d[0] - d[2]) < 1e-12 * abs(d[0]) and abs(d[1] - d[3]) < 1e-12 * abs(d[0])
+        W = np.ascontiguousarray(rng.standard_normal((ctx.veo, 24)))
+        bw = ctx.buffer(ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F32) + 16)
+        with pytest.raises(abi.Cl2Error, match="16-byte aligned"):
+            ctx.spinor_import(bw.ptr + 8, W.ctypes.data, abi.F32)
+        g = ctx.gauge(fmt=abi.GAUGE_F32_12, seed=1, epsilon=0.3)
+        fw = ctx.spinor(abi.F32).upload(W)
+        with pytest.raises(abi.Cl2Error, match="16-byte aligned"):
+            ctx.dslash_eo(bw.ptr + 8, fw.ptr, g.ptr, abi.GAUGE_F32_12, abi.EVEN, 0.125)
