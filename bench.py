@@ -214,8 +214,9 @@ def cg_leg(ns, nt, l, gf, plaq, desc, seed):
 def batch_leg(ns, nt, sweeps, seed, nrhs=12):
     """12 point sources (every spin x colour at the origin) on QplusQminus_eo, twisted mass kappa 0.16,
     mu 0.005, heatbath beta 5.7: one lockstep block solve with the fused multi-RHS operator against 12
-    single-source solves, both FP32 inner / FP64 reliable updates to |r|/|b| <= 1e-10.  Also the per-RHS
-    cost of one FP32 hopping application with 12 right-hand sides against one with a single field."""
+    single-source solves, both with FP64 reliable updates to |r|/|b| <= 1e-10, for an FP32 and for an FP16
+    (per-site norm) inner loop.  Also the per-RHS cost of one hopping application with 12 right-hand sides
+    against one with a single field, FP32 and FP16."""
     from cl2qcd_amd import _host as H, abi
 
     l = H.Lattice([f"--nSpace={ns}", f"--nTime={nt}", "--beta=5.7", "--fermionAction=twistedmass",
@@ -225,9 +226,11 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
     for _ in range(sweeps):
         H.su3heatbath(gf, l, 1)
     qq, ap = H.QplusQminus_eo(l), l.wilson_parameters()
-    # launch shapes of the FP32 hopping kernel for one and for nrhs right-hand sides, autotuned on this lattice
+    # launch shapes of the FP32 hopping kernel for one and for nrhs right-hand sides, and of the single-field
+    # FP16 kernel, autotuned on this lattice
     H.autotune_dslash(gf, l, H.F32, 1)
     H.autotune_dslash(gf, l, H.F32, nrhs)
+    H.autotune_dslash(gf, l, H.F16, 1)
     veo = l.eo_sites
     bs, xs = [], []
     for r in range(nrhs):
@@ -239,32 +242,41 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
         x = H.Spinorfield_eo(l)
         x.zero()
         xs.append(x)
-    l.synchronize()
-    t0 = time.perf_counter()
-    seq_it = []
-    for r in range(nrhs):
-        st = H.cg_reliable_update(xs[r], qq, gf, bs[r], l, 1e-10, ap, H.F32, 0.1, use_graph=USE_GRAPH)
-        seq_it.append(st["iterations"])
-    l.synchronize()
-    t_seq = time.perf_counter() - t0
-    for x in xs:
-        x.zero()
-    l.synchronize()
-    t0 = time.perf_counter()
-    it, per = H.cg_reliable_update_block(xs, qq, gf, bs, l, 1e-10, ap, H.F32, 0.1, use_graph=USE_GRAPH)
-    l.synchronize()
-    t_blk = time.perf_counter() - t0
-    worst = max(p["true_relative_residual"] for p in per)
-    log(f"12 point sources {ns}^3x{nt}: sequential {t_seq:.2f} s ({sum(seq_it)} iterations), block {t_blk:.2f} s "
-        f"({it} lockstep iterations), worst true |r|/|b| {worst:.2e}")
-    # hopping term alone, FP32 12-real links: 12 right-hand sides in one launch vs one field
+
+    def solves(prec):
+        for x in xs:
+            x.zero()
+        l.synchronize()
+        t0 = time.perf_counter()
+        seq_it = []
+        for r in range(nrhs):
+            st = H.cg_reliable_update(xs[r], qq, gf, bs[r], l, 1e-10, ap, prec, 0.1, use_graph=USE_GRAPH)
+            seq_it.append(st["iterations"])
+        l.synchronize()
+        t_seq = time.perf_counter() - t0
+        for x in xs:
+            x.zero()
+        l.synchronize()
+        t0 = time.perf_counter()
+        it, per = H.cg_reliable_update_block(xs, qq, gf, bs, l, 1e-10, ap, prec, 0.1, use_graph=USE_GRAPH)
+        l.synchronize()
+        t_blk = time.perf_counter() - t0
+        worst = max(p["true_relative_residual"] for p in per)
+        name = "FP16" if prec == H.F16 else "FP32"
+        log(f"12 point sources {ns}^3x{nt}, {name} inner: sequential {t_seq:.2f} s ({sum(seq_it)} iterations), "
+            f"block {t_blk:.2f} s ({it} lockstep iterations), worst true |r|/|b| {worst:.2e}")
+        return t_seq, t_blk, {"sequential": {"seconds": round(t_seq, 3), "per_rhs_seconds": round(t_seq / nrhs, 4),
+                                             "iterations": seq_it},
+                              "block": {"seconds": round(t_blk, 3), "per_rhs_seconds": round(t_blk / nrhs, 4),
+                                        "lockstep_iterations": it, "iterations": [p["iterations"] for p in per],
+                                        "worst_true_rel_residual": worst},
+                              "speedup_per_rhs": round(t_seq / t_blk, 3)}
+
+    t_seq32, t_blk32, f32 = solves(H.F32)
+    t_seq16, t_blk16, f16 = solves(H.F16)
+    f16["block_speedup_vs_fp32_sequential"] = round(t_seq32 / t_blk16, 3)
+    # hopping term alone: nrhs right-hand sides in one launch vs one field, FP32 and FP16 (random links)
     ctx = abi.Context(ns, nt, theta_t=1.0)
-    g = ctx.gauge(fmt=abi.GAUGE_F32_12, seed=3, epsilon=0.35)
-    one_b = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, abi.F32)
-    a1, b1 = ctx.buffer(one_b), ctx.buffer(one_b)
-    an, bn = ctx.buffer(nrhs * one_b), ctx.buffer(nrhs * one_b)
-    ctx.set_cold_eo(a1.ptr, abi.F32)
-    ctx.block_set_zero_eo(an.ptr, abi.F32, nrhs)
 
     def rate(fn, reps=50):
         fn()
@@ -276,23 +288,32 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
         ctx.check(ctx.lib.cl2_timer_stop(ctx.handle, ctypes.byref(ms)))
         return ms.value / reps
 
-    t1 = rate(lambda: ctx.dslash_eo(b1.ptr, a1.ptr, g.ptr, abi.GAUGE_F32_12, abi.EVEN, 0.16))
-    tn = rate(lambda: ctx.dslash_eo_mrhs(bn.ptr, an.ptr, g.ptr, abi.GAUGE_F32_12, abi.EVEN, 0.16, nrhs))
-    alg_n = veo * (8 * 48 + 2 * nrhs * 96)
+    def hop_rates(fmt, prec, link_b, site_b):
+        g = ctx.gauge(fmt=fmt, seed=3, epsilon=0.35)
+        one_b = ctx.lib.cl2_field_bytes(ctx.handle, abi.FIELD_SPINOR_EO, prec)
+        a1, b1 = ctx.buffer(one_b), ctx.buffer(one_b)
+        an, bn = ctx.buffer(nrhs * one_b), ctx.buffer(nrhs * one_b)
+        ctx.set_cold_eo(a1.ptr, prec)
+        ctx.block_set_zero_eo(an.ptr, prec, nrhs)
+        t1 = rate(lambda: ctx.dslash_eo(b1.ptr, a1.ptr, g.ptr, fmt, abi.EVEN, 0.16))
+        tn = rate(lambda: ctx.dslash_eo_mrhs(bn.ptr, an.ptr, g.ptr, fmt, abi.EVEN, 0.16, nrhs))
+        alg_n = veo * (8 * link_b + 2 * nrhs * site_b)
+        for buf in (a1, b1, an, bn, g):
+            buf.close()
+        name = "FP16" if prec == abi.F16 else "FP32"
+        log(f"{name} hopping term: 1 field {t1:.4f} ms, {nrhs} fields {tn:.4f} ms ({tn / nrhs:.4f} ms per field, "
+            f"{alg_n / (tn * 1e6):.0f} GB/s algorithmic)")
+        return {"single": round(t1, 5), f"mrhs{nrhs}": round(tn, 5), f"mrhs{nrhs}_per_rhs": round(tn / nrhs, 5),
+                f"mrhs{nrhs}_alg_GBps": round(alg_n / (tn * 1e6), 1)}
+
+    d32 = hop_rates(abi.GAUGE_F32_12, abi.F32, 48, 96)
+    d16 = hop_rates(abi.GAUGE_F16_12, abi.F16, 24, 52)
     ctx.close()
-    log(f"FP32 hopping term: 1 field {t1:.4f} ms, {nrhs} fields {tn:.4f} ms ({tn / nrhs:.4f} ms per field, "
-        f"{alg_n / (tn * 1e6):.0f} GB/s algorithmic)")
     return {"lattice": f"{ns}^3x{nt}", "sources": f"{nrhs} point sources (spin x colour) at the origin",
             "operator": "QplusQminus_eo twisted mass kappa 0.16 mu 0.005, FP32 inner, FP64 reliable updates",
-            "sequential": {"seconds": round(t_seq, 3), "per_rhs_seconds": round(t_seq / nrhs, 4),
-                           "iterations": seq_it},
-            "block": {"seconds": round(t_blk, 3), "per_rhs_seconds": round(t_blk / nrhs, 4),
-                      "lockstep_iterations": it, "iterations": [p["iterations"] for p in per],
-                      "worst_true_rel_residual": worst},
-            "speedup_per_rhs": round(t_seq / t_blk, 3),
-            "dslash_f32_ms": {"single": round(t1, 5), f"mrhs{nrhs}": round(tn, 5),
-                              f"mrhs{nrhs}_per_rhs": round(tn / nrhs, 5),
-                              f"mrhs{nrhs}_alg_GBps": round(alg_n / (tn * 1e6), 1)}}
+            **f32, "dslash_f32_ms": d32,
+            "fp16_inner": {"operator": "the same with an FP16 (per-site norm) inner loop", **f16},
+            "dslash_f16_ms": d16}
 
 
 def mrhs_shapes_leg(ns=32, nt=64, nrhs=12, reps=30):

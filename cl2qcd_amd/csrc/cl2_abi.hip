@@ -109,8 +109,9 @@ static bool all_aligned16(std::initializer_list<const void*> ps)
         if (!aligned16(p)) return false;
     return true;
 }
-#define CL2_PAIRS_ALIGNED(c, prec, ...) \
-    CL2_REQUIRE(c, (prec) != CL2_F32 || all_aligned16({__VA_ARGS__}), "FP32 Wilson spinor fields must be 16-byte aligned")
+#define CL2_PAIRS_ALIGNED(c, prec, ...)                                                                  \
+    CL2_REQUIRE(c, (prec) == CL2_F64 || all_aligned16({__VA_ARGS__}),                                     \
+                "FP32 and FP16 Wilson spinor fields must be 16-byte aligned")
 
 bool valid_fmt(int f)
 {
@@ -220,8 +221,7 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
                const double* cg_alpha = nullptr)
 {
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
-    CL2_REQUIRE(c, supported_nrhs(nrhs) && (nrhs == 1 || fmt != CL2_GAUGE_F16_12),
-                "unsupported number of right-hand sides (1, 2, 4, 8, 12 or 16; FP16 fields: 1)");
+    CL2_REQUIRE(c, supported_nrhs(nrhs), "unsupported number of right-hand sides (1, 2, 4, 8, 12 or 16)");
     CL2_REQUIRE(c, evenodd == CL2_EVEN || evenodd == CL2_ODD, "evenodd must be CL2_EVEN or CL2_ODD");
     CL2_REQUIRE(c, (out || cg_r) && in && gauge, "null field pointer");
     CL2_REQUIRE(c, out != in && cg_r != in, "dslash cannot run in place: out aliases in");
@@ -274,13 +274,17 @@ int run_dslash(cl2_ctx* c, void* out, const void* in, const void* x, const void*
             ? 1
             : t.lds;
     a.lds = lds;
-    if (nrhs > 1 && lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs, lds), 1, t);
+    if (nrhs > 1 && fmt == CL2_GAUGE_F16_12) {  // FP16 blocks: the staged kernel, one right-hand side per thread
+        a.rb  = 1;
+        a.lds = 1;
+    }
+    if (nrhs > 1 && a.lds && a.rb <= 1) a.map = work_map(c, mrhs_sites_per_block(nrhs, a.lds), 1, t);
     CL2_REQUIRE(c, cg_r == nullptr || (cg_alpha != nullptr && norm_res != nullptr), "CG update without its scalars");
     // per-right-hand-side norms come from the kernel with per-site staging (one right-hand side per thread)
     const bool mrhs_norms = nrhs > 1 && norm_res != nullptr;
     if (mrhs_norms) {
         a.rb  = 1;
-        a.lds = lds == 2 ? 2 : 1;
+        a.lds = (lds == 2 && fmt != CL2_GAUGE_F16_12) ? 2 : 1;
         a.map = work_map(c, mrhs_sites_per_block(nrhs, a.lds), 1, t);
     }
     CL2_HIP(c, launch_dslash(c, a, fmt));
@@ -933,7 +937,7 @@ int cl2_fermionmatrix_eo_mrhs_cg_step(cl2_ctx* c, int op, void* r, const void* p
     CL2_REQUIRE(c, op == CL2_OP_QPLUSQMINUS_EO, "the fused CG step is for QplusQminus_eo");
     CL2_REQUIRE(c, action == CL2_ACTION_WILSON || action == CL2_ACTION_TWISTEDMASS,
                 "unknown fermion action: wilson or twistedmass");
-    CL2_REQUIRE(c, valid_fmt(fmt) && fmt != CL2_GAUGE_F16_12, "unknown gauge format (FP64/FP32 links)");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
     CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1, "unsupported number of right-hand sides (2, 4, 8, 12 or 16)");
     CL2_REQUIRE(c, r && p && gauge && tmp && r2 && active && pAp && alpha && norms, "null pointer");
     CL2_REQUIRE(c, (kappas == nullptr) == (mubars == nullptr) || action == CL2_ACTION_WILSON,
@@ -954,8 +958,7 @@ int cl2_fermionmatrix_eo_mrhs(cl2_ctx* c, int op, void* out, const void* in, con
                               double kappa, double mubar, void* tmp, int nrhs)
 {
     if (!c) return CL2_E_ARG;
-    CL2_REQUIRE(c, supported_nrhs(nrhs) && fmt != CL2_GAUGE_F16_12,
-                "unsupported number of right-hand sides (1, 2, 4, 8, 12 or 16; FP64/FP32 fields)");
+    CL2_REQUIRE(c, supported_nrhs(nrhs), "unsupported number of right-hand sides (1, 2, 4, 8, 12 or 16)");
     return run_fermionmatrix(c, op, out, in, gauge, fmt, action, kappa, mubar, tmp, nrhs);
 }
 
@@ -963,8 +966,7 @@ int cl2_fermionmatrix_eo_mrhs_params(cl2_ctx* c, int op, void* out, const void* 
                                      int action, const double* kappa, const double* mubar, void* tmp, int nrhs)
 {
     if (!c) return CL2_E_ARG;
-    CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1 && fmt != CL2_GAUGE_F16_12,
-                "unsupported number of right-hand sides (2, 4, 8, 12 or 16; FP64/FP32 fields)");
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1, "unsupported number of right-hand sides (2, 4, 8, 12 or 16)");
     CL2_REQUIRE(c, kappa != nullptr && (action == CL2_ACTION_WILSON || mubar != nullptr), "null parameter array");
     return run_fermionmatrix(c, op, out, in, gauge, fmt, action, 0., 0., tmp, nrhs, kappa, mubar);
 }
@@ -1138,24 +1140,34 @@ int cl2_convert_precision_eo(cl2_ctx* c, int kind, void* dst, int dst_prec, cons
 #define CL2_BLOCK_ARGS(prec, nrhs)                                                                        \
     CL2_REQUIRE(c, valid_prec(prec), "block fields are FP64 or FP32");                                    \
     CL2_REQUIRE(c, (nrhs) >= 1 && (nrhs) <= 64, "number of right-hand sides must be in [1, 64]")
+/* Wilson blocks may also be FP16 (one FP16 field of Veo nrhs sites) */
+#define CL2_WBLOCK_ARGS(prec, nrhs)                                                                       \
+    CL2_REQUIRE(c, valid_wprec(prec), "Wilson block fields are FP64, FP32 or FP16");                      \
+    CL2_REQUIRE(c, (nrhs) >= 1 && (nrhs) <= 64, "number of right-hand sides must be in [1, 64]")
 
 int cl2_block_insert_eo(cl2_ctx* c, void* block, const void* single, int prec, int nrhs, int r)
 {
     if (!c) return CL2_E_ARG;
-    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_WBLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, block && single && r >= 0 && r < nrhs, "bad argument");
     CL2_PAIRS_ALIGNED(c, prec, block, single);
-    CL2_HIP(c, launch_block_slot(c, block, single, nullptr, prec, nrhs, r));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_block_slot_h(c, block, single, nullptr, nrhs, r));
+    else
+        CL2_HIP(c, launch_block_slot(c, block, single, nullptr, prec, nrhs, r));
     return CL2_OK;
 }
 
 int cl2_block_extract_eo(cl2_ctx* c, void* single, const void* block, int prec, int nrhs, int r)
 {
     if (!c) return CL2_E_ARG;
-    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_WBLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, block && single && r >= 0 && r < nrhs, "bad argument");
     CL2_PAIRS_ALIGNED(c, prec, block, single);
-    CL2_HIP(c, launch_block_slot(c, const_cast<void*>(block), nullptr, single, prec, nrhs, r));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_block_slot_h(c, const_cast<void*>(block), nullptr, single, nrhs, r));
+    else
+        CL2_HIP(c, launch_block_slot(c, const_cast<void*>(block), nullptr, single, prec, nrhs, r));
     return CL2_OK;
 }
 
@@ -1163,10 +1175,13 @@ int cl2_block_axpy_eo(cl2_ctx* c, void* out, const void* x, const void* y, int p
                       double sign, double* norms)
 {
     if (!c) return CL2_E_ARG;
-    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_WBLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, out && x && y && coef, "null pointer");
     CL2_PAIRS_ALIGNED(c, prec, out, x, y);
-    CL2_HIP(c, launch_block_axpy(c, out, x, y, prec, nrhs, coef, sign, norms));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_block_axpy_h(c, out, x, y, nrhs, coef, sign, norms));
+    else
+        CL2_HIP(c, launch_block_axpy(c, out, x, y, prec, nrhs, coef, sign, norms));
     return CL2_OK;
 }
 
@@ -1174,56 +1189,71 @@ int cl2_block_cg_update_xp_eo(cl2_ctx* c, void* x, void* p, const void* r, int p
                               const double* beta)
 {
     if (!c) return CL2_E_ARG;
-    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_WBLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, x && p && r && alpha && beta, "null pointer");
     CL2_REQUIRE(c, x != p && x != r && p != r, "x, p and r must be different blocks");
     CL2_PAIRS_ALIGNED(c, prec, x, p, r);
-    CL2_HIP(c, launch_block_xp(c, x, p, r, prec, nrhs, alpha, beta));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_block_xp_h(c, x, p, r, nrhs, alpha, beta));
+    else
+        CL2_HIP(c, launch_block_xp(c, x, p, r, prec, nrhs, alpha, beta));
     return CL2_OK;
 }
 
 int cl2_block_dot_eo(cl2_ctx* c, double* res, const void* x, const void* y, int prec, int nrhs)
 {
     if (!c) return CL2_E_ARG;
-    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_WBLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, res && x && y, "null pointer");
     CL2_PAIRS_ALIGNED(c, prec, x, y);
-    CL2_HIP(c, launch_block_dot(c, res, x, y, prec, nrhs, 0));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_block_dot_h(c, res, x, y, nrhs, 0));
+    else
+        CL2_HIP(c, launch_block_dot(c, res, x, y, prec, nrhs, 0));
     return CL2_OK;
 }
 
 int cl2_block_squarenorm_eo(cl2_ctx* c, double* res, const void* x, int prec, int nrhs)
 {
     if (!c) return CL2_E_ARG;
-    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_WBLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, res && x, "null pointer");
     CL2_PAIRS_ALIGNED(c, prec, x);
-    CL2_HIP(c, launch_block_dot(c, res, x, x, prec, nrhs, 1));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_block_dot_h(c, res, x, x, nrhs, 1));
+    else
+        CL2_HIP(c, launch_block_dot(c, res, x, x, prec, nrhs, 1));
     return CL2_OK;
 }
 
 int cl2_block_convert_eo(cl2_ctx* c, void* dst, int dst_prec, const void* src, int src_prec, int nrhs)
 {
     if (!c) return CL2_E_ARG;
-    CL2_BLOCK_ARGS(dst_prec, nrhs);
-    CL2_REQUIRE(c, valid_prec(src_prec), "block fields are FP64 or FP32");
+    CL2_WBLOCK_ARGS(dst_prec, nrhs);
+    CL2_REQUIRE(c, valid_wprec(src_prec), "Wilson block fields are FP64, FP32 or FP16");
     CL2_REQUIRE(c, dst && src && dst != src, "bad pointers");
-    const size_t n = (size_t)12 * c->g.veo * nrhs;
     CL2_PAIRS_ALIGNED(c, dst_prec, dst);
     CL2_PAIRS_ALIGNED(c, src_prec, src);
+    const uint32_t sites = c->g.veo * (uint32_t)nrhs;
     if (dst_prec == src_prec)
-        CL2_HIP(c, hipMemcpyAsync(dst, src, n * (dst_prec == CL2_F64 ? 16 : 8), hipMemcpyDeviceToDevice, c->stream));
+        CL2_HIP(c, hipMemcpyAsync(dst, src, spinor_bytes(c, dst_prec) * (size_t)nrhs, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    else if (dst_prec == CL2_F16 || src_prec == CL2_F16)
+        CL2_HIP(c, launch_convert_h(c, dst, dst_prec, src, src_prec, sites));
     else
-        CL2_HIP(c, launch_convert_spinor(c, dst, dst_prec, src, src_prec, (size_t)c->g.veo * nrhs));
+        CL2_HIP(c, launch_convert_spinor(c, dst, dst_prec, src, src_prec, (size_t)sites));
     return CL2_OK;
 }
 
 int cl2_block_set_zero_eo(cl2_ctx* c, void* out, int prec, int nrhs)
 {
     if (!c) return CL2_E_ARG;
-    CL2_BLOCK_ARGS(prec, nrhs);
+    CL2_WBLOCK_ARGS(prec, nrhs);
     CL2_REQUIRE(c, out != nullptr, "null pointer");
-    CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo * nrhs, prec, 0., 0.));
+    if (prec == CL2_F16)
+        CL2_HIP(c, launch_fill_h(c, out, 0., 0., c->g.veo * (uint32_t)nrhs));
+    else
+        CL2_HIP(c, launch_fill(c, out, (size_t)12 * c->g.veo * nrhs, prec, 0., 0.));
     return CL2_OK;
 }
 
@@ -1311,7 +1341,7 @@ int cl2_block_cg_update_xp_stagg_eo(cl2_ctx* c, void* x, void* p, const void* r,
 int cl2_D_KS_eo_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int fmt, int evenodd, int nrhs)
 {
     if (!c) return CL2_E_ARG;
-    CL2_REQUIRE(c, valid_fmt(fmt) && fmt != CL2_GAUGE_F16_12, "unknown gauge format (FP64/FP32 links)");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
     CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1, "unsupported number of right-hand sides (2, 4, 8, 12 or 16)");
     CL2_REQUIRE(c, evenodd == CL2_EVEN || evenodd == CL2_ODD, "evenodd must be CL2_EVEN or CL2_ODD");
     CL2_REQUIRE(c, out && in && gauge, "null field pointer");
@@ -1324,7 +1354,7 @@ int cl2_MdagM_eo_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, 
                       const double* sigma, void* tmp, int nrhs)
 {
     if (!c) return CL2_E_ARG;
-    CL2_REQUIRE(c, valid_fmt(fmt) && fmt != CL2_GAUGE_F16_12, "unknown gauge format (FP64/FP32 links)");
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
     CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1, "unsupported number of right-hand sides (2, 4, 8, 12 or 16)");
     CL2_REQUIRE(c, upper_left == CL2_EVEN || upper_left == CL2_ODD, "upper_left must be CL2_EVEN or CL2_ODD");
     CL2_REQUIRE(c, out && in && gauge && tmp, "null field pointer");

@@ -17,6 +17,7 @@
 // with their links and neighbour sites loaded before any is applied, under a
 // register budget that leaves room for them; every variant computes the same
 // bits.
+#include <algorithm>
 #include <type_traits>
 
 #include "cl2_device.hpp"
@@ -452,14 +453,16 @@ __global__ __launch_bounds__(BLAS_BLOCK) void k_fill_h(void* out, uint32_t S, fl
     store_site_h(out, S, i, v);
 }
 
-hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im)
+hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im, uint32_t sites)
 {
-    hipLaunchKernelGGL(k_fill_h, dim3(blocks_for(c->g.veo, BLAS_BLOCK)), dim3(BLAS_BLOCK), 0, c->stream, out,
-                       c->g.veo, float(re), float(im));
+    const uint32_t S = sites ? sites : c->g.veo;
+    hipLaunchKernelGGL(k_fill_h, dim3(blocks_for(S, BLAS_BLOCK)), dim3(BLAS_BLOCK), 0, c->stream, out, S, float(re),
+                       float(im));
     return hipGetLastError();
 }
 
-/* FP64 / FP32 spinor (the layouts of load_spinor) <-> FP16 with per-site norm */
+/* FP64 / FP32 spinor (the layouts of load_spinor) <-> FP16 with per-site norm; with S = Veo nrhs the same
+ * kernels convert blocks, whose site offset i nrhs + r is the same in every precision */
 template<typename R>
 __global__ __launch_bounds__(BLAS_BLOCK) void k_to_half(void* out, const vec2<R>* __restrict__ in, uint32_t S)
 {
@@ -485,9 +488,9 @@ __global__ __launch_bounds__(BLAS_BLOCK) void k_from_half(vec2<R>* __restrict__ 
     store_spinor<R>(out, S, i, q);
 }
 
-hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec)
+hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, uint32_t sites)
 {
-    const uint32_t S  = c->g.veo;
+    const uint32_t S  = sites ? sites : c->g.veo;
     const unsigned nb = blocks_for(S, BLAS_BLOCK);
     if (out_prec == CL2_F16 && in_prec == CL2_F64)
         hipLaunchKernelGGL(k_to_half<double>, dim3(nb), dim3(BLAS_BLOCK), 0, c->stream, out,
@@ -503,6 +506,325 @@ hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in,
                            reinterpret_cast<float2*>(out), in, S);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+
+/* ---------------- blocks of FP16 spinors ----------------
+ * A block of nrhs FP16 spinors is one FP16 field of Veo nrhs sites: right-hand side r of eo site i is site
+ * offset i nrhs + r (three 16-byte lanes [3][Veo nrhs], then float norm[Veo nrhs]), so the nrhs lanes of a
+ * site are adjacent as in the FP64 and FP32 blocks, and every single-field routine that takes the site
+ * count applies to a block unchanged. */
+
+/* Multi-RHS hopping kernel on FP16 blocks, the staging of k_dslash_eo_mrhs_lds: a block owns SB consecutive
+ * sites of the work order and NRHS threads per site; the sites' neighbour indices and their 8 links
+ * (reconstructed to FP32) are put in LDS once, then every right-hand-side thread gathers its own FP16
+ * neighbour sites.  Per right-hand side the arithmetic is k_dslash_eo_h's (same hop order, norms folded into
+ * the hop weights), so slot r equals the single-field kernel bit for bit. */
+template<int NRHS>
+struct MrhsH {
+    static constexpr int SB    = NRHS >= 8 ? 16 : 32;
+    static constexpr int BLOCK = SB * NRHS;
+};
+
+template<int NRHS>
+__global__ __launch_bounds__(MrhsH<NRHS>::BLOCK) void k_dslash_eo_mrhs_h(DslashArgs a)
+{
+    constexpr int SB        = MrhsH<NRHS>::SB;
+    constexpr int BLOCK     = MrhsH<NRHS>::BLOCK;
+    constexpr uint32_t NONE = 0xffffffffu;
+    __shared__ Su3<float> lu[SB * 8];
+    __shared__ uint32_t lnb[SB * 8];
+    __shared__ uint32_t lsite[SB];
+    __shared__ double2 lred[BLOCK];
+    const uint32_t S  = a.g.veo;
+    const uint32_t SN = S * NRHS;
+    const uint32_t Lb = logical_block(a.map, blockIdx.x);
+    const int P       = a.out_parity;
+    if (threadIdx.x < SB) {
+        const uint32_t l = Lb * SB + threadIdx.x;
+        uint32_t i       = NONE;
+        if (l < S) {
+            i = logical_site(a.map, l);
+            SiteNb nb;
+            site_neighbours(a.g, i, P, nb);
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                lnb[threadIdx.x * 8 + 2 * d]     = nb.up[d];
+                lnb[threadIdx.x * 8 + 2 * d + 1] = nb.dn[d];
+            }
+        }
+        lsite[threadIdx.x] = i;
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < SB * 8; w += BLOCK) {
+        const uint32_t i = lsite[w >> 3];
+        if (i != NONE) {
+            const int h = w & 7;
+            Su3<float> u;
+            // forward link: the site's own; backward link: the neighbour's, of the other parity
+            load_link_h(a.gauge, S, h >> 1, (h & 1) ? P ^ 1 : P, (h & 1) ? lnb[w] : i, u);
+            lu[w] = u;
+        }
+    }
+    __syncthreads();
+    const int s      = threadIdx.x / NRHS;
+    const int rhs    = threadIdx.x - s * NRHS;
+    const uint32_t i = lsite[s];
+    double nrm = 0., ry = 0.;
+    if (i != NONE) {
+        Cx<float> acc[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[k] = Cx<float>{0.f, 0.f};
+        // order T, X, Y, Z, forward then backward, as fermionmatrix_eo_dslash.cl:38-45
+#define CL2_DIR_H(m)                                                                                   \
+    {                                                                                                  \
+        Ch pf[12], pb[12];                                                                             \
+        const float nf = load_site_h_raw(a.in, SN, lnb[s * 8 + 2 * (m)] * NRHS + rhs, pf);             \
+        const float nk = load_site_h_raw(a.in, SN, lnb[s * 8 + 2 * (m) + 1] * NRHS + rhs, pb);         \
+        const Su3<float> uf = lu[s * 8 + 2 * (m)], ub = lu[s * 8 + 2 * (m) + 1];                       \
+        apply_hop_h<m, true>(acc, pf, uf, weight_n(a.w.wf[m], nf));                                    \
+        apply_hop_h<m, false>(acc, pb, ub, weight_n(a.w.wb[m], nk));                                   \
+    }
+        CL2_DIR_H(0)
+        CL2_DIR_H(1)
+        CL2_DIR_H(2)
+        CL2_DIR_H(3)
+#undef CL2_DIR_H
+        const uint32_t o = i * NRHS + rhs;
+        Cx<float> f1{float(a.f1[0]), float(a.f1[1])}, f2{float(a.f2[0]), float(a.f2[1])};
+        float sc = float(a.scale);
+        if (a.per_rhs) {  // an operator per right-hand side: the hop ran with unit weights
+            const float kr = float(a.rhs_kappa[rhs]);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) acc[k] = scale(acc[k], kr);
+            f1 = Cx<float>{float(a.rhs_f1[rhs][0]), float(a.rhs_f1[rhs][1])};
+            f2 = Cx<float>{float(a.rhs_f2[rhs][0]), float(a.rhs_f2[rhs][1])};
+            sc = float(a.rhs_scale[rhs]);
+        }
+        if (a.epilogue == EPI_MINV) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) acc[k] = scale(cmul(acc[k], k < 6 ? f1 : f2), sc);
+        } else if (a.epilogue == EPI_AEE) {
+            Cx<float> xv[12];
+            load_site_h(a.x, SN, o, xv);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                acc[k] = cmul(xv[k], k < 6 ? f1 : f2) - acc[k];
+                if (a.gamma5 && k >= 6) acc[k] = Cx<float>{-acc[k].re, -acc[k].im};
+            }
+        }
+        if (a.cg_r) {  // CG residual update in place of storing the result
+            Cx<float> ro[12], rn[12];
+            load_site_h<true>(a.cg_r, SN, o, ro);
+            const Cx<float> al{float(a.cg_alpha[2 * rhs]), float(a.cg_alpha[2 * rhs + 1])};
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                rn[k] = ro[k] - cmul(acc[k], al);
+                ry += (double)rn[k].re * ro[k].re + (double)rn[k].im * ro[k].im;
+            }
+            store_site_h<true>(a.cg_r, SN, o, rn);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) nrm += (double)rn[k].re * rn[k].re + (double)rn[k].im * rn[k].im;
+        } else {
+            store_site_h<true>(a.out, SN, o, acc);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) nrm += (double)acc[k].re * acc[k].re + (double)acc[k].im * acc[k].im;
+        }
+    }
+    if (a.norm_partials) {
+        // per right-hand side: thread r < NRHS sums the SB site threads of its column in site order
+        lred[threadIdx.x] = make_double2(nrm, ry);
+        __syncthreads();
+        if ((int)threadIdx.x < NRHS) {
+            double2 t = make_double2(0., 0.);
+            for (int q = 0; q < SB; ++q) {
+                t.x += lred[q * NRHS + threadIdx.x].x;
+                t.y += lred[q * NRHS + threadIdx.x].y;
+            }
+            reinterpret_cast<double2*>(a.norm_partials)[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = t;
+        }
+    }
+}
+
+hipError_t launch_dslash_mrhs_h(cl2_ctx* c, const DslashArgs& a)
+{
+    auto go = [&](auto n_tag) -> hipError_t {
+        constexpr int N = decltype(n_tag)::value;
+        hipLaunchKernelGGL((k_dslash_eo_mrhs_h<N>), dim3((a.g.veo + MrhsH<N>::SB - 1) / MrhsH<N>::SB),
+                           dim3(MrhsH<N>::BLOCK), 0, c->stream, a);
+        return hipGetLastError();
+    };
+    switch (a.nrhs) {
+        case 2: return go(std::integral_constant<int, 2>{});
+        case 4: return go(std::integral_constant<int, 4>{});
+        case 8: return go(std::integral_constant<int, 8>{});
+        case 12: return go(std::integral_constant<int, 12>{});
+        case 16: return go(std::integral_constant<int, 16>{});
+        default: return hipErrorInvalidValue;
+    }
+}
+
+/* per-right-hand-side sums of a block's thread values, in threads 0..nrhs-1: thread t's column is right-hand
+ * side t mod nrhs (block size and grid stride are multiples of nrhs), summed in thread order */
+__device__ __forceinline__ double2 rhs_column_sum(double2 v, double2* lds, int nrhs)
+{
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    double2 s = make_double2(0., 0.);
+    if ((int)threadIdx.x < nrhs)
+        for (unsigned k = threadIdx.x; k < blockDim.x; k += (unsigned)nrhs) {
+            s.x += lds[k].x;
+            s.y += lds[k].y;
+        }
+    __syncthreads();
+    return s;
+}
+
+/* out_r = y_r + sign c_r x_r; with partials: per right-hand side (|out_r|^2, Re <out_r, y_r>) into
+ * partials[r * nb + block].  out may alias y (each site is read before it is written). */
+__global__ __launch_bounds__(256) void k_block_axpy_h(void* out, const void* x, const void* y, uint32_t SN, int nrhs,
+                                                      const double* coef, double sign, double2* partials)
+{
+    __shared__ double2 lds[256];
+    const int r = (int)(threadIdx.x % (unsigned)nrhs);
+    const Cx<float> cr{float(sign * coef[2 * r]), float(sign * coef[2 * r + 1])};
+    double2 acc = make_double2(0., 0.);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < SN; e += gridDim.x * blockDim.x) {
+        Cx<float> xv[12], yv[12], o[12];
+        load_site_h<true>(x, SN, e, xv);
+        load_site_h<true>(y, SN, e, yv);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            o[k] = yv[k] + cmul(cr, xv[k]);
+            acc.x += (double)o[k].re * o[k].re + (double)o[k].im * o[k].im;
+            acc.y += (double)o[k].re * yv[k].re + (double)o[k].im * yv[k].im;
+        }
+        store_site_h<true>(out, SN, e, o);
+    }
+    if (partials) {
+        const double2 s = rhs_column_sum(acc, lds, nrhs);
+        if ((int)threadIdx.x < nrhs) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+    }
+}
+
+/* the x and p updates of a lockstep CG step: x_r = x_r + alpha_r p_r, then p_r = r_r + beta_r p_r */
+__global__ __launch_bounds__(256) void k_block_xp_h(void* xo, void* p, const void* rv, uint32_t SN, int nrhs,
+                                                    const double* alpha, const double* beta)
+{
+    const int r = (int)(threadIdx.x % (unsigned)nrhs);
+    const Cx<float> ca{float(alpha[2 * r]), float(alpha[2 * r + 1])}, cb{float(beta[2 * r]), float(beta[2 * r + 1])};
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < SN; e += gridDim.x * blockDim.x) {
+        Cx<float> xv[12], pv[12], rr[12];
+        load_site_h<true>(xo, SN, e, xv);
+        load_site_h<true>(p, SN, e, pv);
+        load_site_h<true>(rv, SN, e, rr);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            xv[k] = xv[k] + cmul(ca, pv[k]);
+            pv[k] = rr[k] + cmul(cb, pv[k]);
+        }
+        store_site_h<true>(xo, SN, e, xv);
+        store_site_h<true>(p, SN, e, pv);
+    }
+}
+
+/* per right-hand side: sum conj(x) y (mode 0) or sum |x|^2 (mode 1) */
+__global__ __launch_bounds__(256) void k_block_dot_h(const void* x, const void* y, uint32_t SN, int nrhs, int mode,
+                                                     double2* partials)
+{
+    __shared__ double2 lds[256];
+    double2 acc = make_double2(0., 0.);
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < SN; e += gridDim.x * blockDim.x) {
+        Cx<float> u[12];
+        load_site_h<true>(x, SN, e, u);
+        if (mode == 1) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) acc.x += (double)u[k].re * u[k].re + (double)u[k].im * u[k].im;
+        } else {
+            Cx<float> v[12];
+            load_site_h<true>(y, SN, e, v);
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                acc.x += (double)u[k].re * v[k].re + (double)u[k].im * v[k].im;
+                acc.y += (double)u[k].re * v[k].im - (double)v[k].re * u[k].im;
+            }
+        }
+    }
+    const double2 s = rhs_column_sum(acc, lds, nrhs);
+    if ((int)threadIdx.x < nrhs) partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
+}
+
+/* single FP16 field <-> slot r of an FP16 block: the stored halves and norm are copied as they are */
+__global__ __launch_bounds__(256) void k_block_slot_h(void* block, const void* single, void* single_out, uint32_t S,
+                                                      int nrhs, int r)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= S) return;
+    const uint32_t SN = S * (uint32_t)nrhs, e = i * (uint32_t)nrhs + (uint32_t)r;
+    h8* bl            = reinterpret_cast<h8*>(block);
+    if (single) {
+        const h8* sl = reinterpret_cast<const h8*>(single);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) bl[(size_t)j * SN + e] = sl[(size_t)j * S + i];
+        half_norms(block, SN)[e] = half_norms(single, S)[i];
+    } else {
+        h8* sl = reinterpret_cast<h8*>(single_out);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sl[(size_t)j * S + i] = bl[(size_t)j * SN + e];
+        half_norms(single_out, S)[i] = half_norms(static_cast<const void*>(block), SN)[e];
+    }
+}
+
+/* grid of an FP16 block kernel: blocks of the largest multiple of nrhs not above 256 threads, at most what the
+ * per-right-hand-side partials buffer holds (the grid loops beyond) */
+static void block_shape_h(cl2_ctx* c, uint32_t SN, int nrhs, unsigned& bt, unsigned& nb)
+{
+    bt               = 256u - (256u % (unsigned)nrhs);
+    size_t n         = (SN + bt - 1) / bt;
+    const size_t cap = c->npartials / (size_t)nrhs;
+    nb               = (unsigned)std::max<size_t>(1, std::min(n, std::min(cap, (size_t)BLAS_MAX_BLOCKS)));
+}
+
+hipError_t launch_block_axpy_h(cl2_ctx* c, void* out, const void* x, const void* y, int nrhs, const double* coef,
+                               double sign, double* norms)
+{
+    const uint32_t SN = c->g.veo * (uint32_t)nrhs;
+    unsigned bt, nb;
+    block_shape_h(c, SN, nrhs, bt, nb);
+    double2* parts = norms ? reinterpret_cast<double2*>(c->partials) : nullptr;
+    hipLaunchKernelGGL(k_block_axpy_h, dim3(nb), dim3(bt), 0, c->stream, out, x, y, SN, nrhs, coef, sign, parts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !norms) return e;
+    return launch_block_reduce2(c, parts, nb, norms, 0, nrhs);
+}
+
+hipError_t launch_block_xp_h(cl2_ctx* c, void* x, void* p, const void* r, int nrhs, const double* alpha,
+                             const double* beta)
+{
+    const uint32_t SN = c->g.veo * (uint32_t)nrhs;
+    unsigned bt, nb;
+    block_shape_h(c, SN, nrhs, bt, nb);
+    hipLaunchKernelGGL(k_block_xp_h, dim3(nb), dim3(bt), 0, c->stream, x, p, r, SN, nrhs, alpha, beta);
+    return hipGetLastError();
+}
+
+hipError_t launch_block_dot_h(cl2_ctx* c, double* res, const void* x, const void* y, int nrhs, int mode)
+{
+    const uint32_t SN = c->g.veo * (uint32_t)nrhs;
+    unsigned bt, nb;
+    block_shape_h(c, SN, nrhs, bt, nb);
+    double2* parts = reinterpret_cast<double2*>(c->partials);
+    hipLaunchKernelGGL(k_block_dot_h, dim3(nb), dim3(bt), 0, c->stream, x, y, SN, nrhs, mode, parts);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_block_reduce2(c, parts, nb, res, mode == 1 ? 1 : 0, nrhs);
+}
+
+hipError_t launch_block_slot_h(cl2_ctx* c, void* block, const void* single_in, void* single_out, int nrhs, int r)
+{
+    hipLaunchKernelGGL(k_block_slot_h, dim3(blocks_for(c->g.veo, 256)), dim3(256), 0, c->stream, block, single_in,
+                       single_out, c->g.veo, nrhs, r);
     return hipGetLastError();
 }
 

@@ -141,8 +141,17 @@ hipError_t launch_dslash_h(cl2_ctx* c, const DslashArgs& a);
 hipError_t launch_blas_h(cl2_ctx* c, const BlasArgs& a, unsigned* nblocks_out);  // a.n = sites
 hipError_t launch_reduce_stage1_h(cl2_ctx* c, const void* x, const void* y, int mode, unsigned* nblocks);
 hipError_t launch_sitediag_h(cl2_ctx* c, void* out, const void* in, const double f1[2], const double f2[2], double s);
-hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im);
-hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec);
+/* sites: the site count of the field (0: Veo; Veo nrhs for a block of FP16 spinors) */
+hipError_t launch_fill_h(cl2_ctx* c, void* out, double re, double im, uint32_t sites = 0);
+hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, uint32_t sites = 0);
+/* blocks of FP16 spinors: the multi-RHS hopping kernel and the block BLAS */
+hipError_t launch_dslash_mrhs_h(cl2_ctx* c, const DslashArgs& a);
+hipError_t launch_block_axpy_h(cl2_ctx* c, void* out, const void* x, const void* y, int nrhs, const double* coef,
+                               double sign, double* norms);
+hipError_t launch_block_xp_h(cl2_ctx* c, void* x, void* p, const void* r, int nrhs, const double* alpha,
+                             const double* beta);
+hipError_t launch_block_dot_h(cl2_ctx* c, double* res, const void* x, const void* y, int nrhs, int mode);
+hipError_t launch_block_slot_h(cl2_ctx* c, void* block, const void* single_in, void* single_out, int nrhs, int r);
 hipError_t launch_convert(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t n);
 hipError_t launch_convert_spinor(cl2_ctx* c, void* out, int out_prec, const void* in, int in_prec, size_t cs);
 

@@ -686,6 +686,7 @@ static hipError_t launch_dslash_n(cl2_ctx* c, const DslashArgs& a, int format)
             case GAUGE_F64_18: return by_rb(std::integral_constant<int, GAUGE_F64_18>{});
             case GAUGE_F64_12: return by_rb(std::integral_constant<int, GAUGE_F64_12>{});
             case GAUGE_F32_12: return by_rb(std::integral_constant<int, GAUGE_F32_12>{});
+            case GAUGE_F16_12: return launch_dslash_mrhs_h(c, a);
             default: return hipErrorInvalidValue;
         }
     }

@@ -143,7 +143,7 @@ int cg_m_mixed(const std::vector<std::shared_ptr<lattices::Staggeredfield_eo>>& 
  * 4, 8, 12 or 16), solved in lockstep: every inner iteration applies A once to the whole block with the
  * multi-right-hand-side operator, each right-hand side keeps its own alpha and beta (device arrays), a
  * reliable update is made for all right-hand sides when one of them has dropped by delta, and a
- * right-hand side whose FP64 true residual meets rel_tol is frozen.  Inner precision FP32 or FP64.
+ * right-hand side whose FP64 true residual meets rel_tol is frozen.  Inner precision FP32, FP16 (per-site norm) or FP64.
  * Returns the number of lockstep iterations; stats (if given) receives one entry per right-hand side. */
 int cg_reliable_update_block(const std::vector<const lattices::Spinorfield_eo*>& x,
                              const fermionmatrix::Fermionmatrix_eo& A, const lattices::Gaugefield& gf,

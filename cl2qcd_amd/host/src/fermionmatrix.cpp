@@ -282,7 +282,7 @@ bool cg_step_block(const Fermionmatrix_eo& A, const lattices::Spinorfield_eo_blo
                    const lattices::Vector<hmc_float>& alpha, const lattices::Vector<hmc_float>& norms)
 {
     const int n = p.nrhs();
-    if (A.abi_operator() != CL2_OP_QPLUSQMINUS_EO || n < 2 || p.precision() == CL2_F16) return false;
+    if (A.abi_operator() != CL2_OP_QPLUSQMINUS_EO || n < 2) return false;
     if (r->precision() != p.precision() || r->nrhs() != n) throw std::invalid_argument("cg_step_block: blocks of different shape");
     if (tmp.bytes() < 2 * p.bytes()) throw std::invalid_argument("cg_step_block: tmp must hold two blocks");
     const bool per_rhs = !kappa.empty();

@@ -278,7 +278,7 @@ std::vector<spinor> Spinorfield_eo::get_host() const
 
 Spinorfield_eo_block::Spinorfield_eo_block(const hardware::System& system, int nrhs, int prec)
     : system(system), n(nrhs), prec(prec),
-      buf(system, (nrhs >= 1 && nrhs <= 64 && (prec == CL2_F64 || prec == CL2_F32))
+      buf(system, (nrhs >= 1 && nrhs <= 64 && (prec == CL2_F64 || prec == CL2_F32 || prec == CL2_F16))
                       ? (size_t)nrhs * cl2_field_bytes(system.ctx(), CL2_FIELD_SPINOR_EO, prec)
                       : 0)
 {

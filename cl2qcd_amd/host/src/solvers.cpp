@@ -466,6 +466,7 @@ namespace {
 struct WilsonBlockOps {
     using Field = Spinorfield_eo;
     using Block = Spinorfield_eo_block;
+    static constexpr bool half = true;  // FP16 blocks (per-site norm) for the inner loop
     static int squarenorm(cl2_ctx* c, double* res, const void* x, int prec, int n)
     {
         return cl2_block_squarenorm_eo(c, res, x, prec, n);
@@ -491,6 +492,7 @@ struct WilsonBlockOps {
 struct StaggBlockOps {
     using Field = Staggeredfield_eo;
     using Block = Staggeredfield_eo_block;
+    static constexpr bool half = false;
     static int squarenorm(cl2_ctx* c, double* res, const void* x, int prec, int n)
     {
         return cl2_block_squarenorm_stagg_eo(c, res, x, prec, n);
@@ -549,7 +551,9 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
             throw std::invalid_argument("cg_reliable_update_block: x and b are FP64 fields");
     if (opt.delta <= 0. || opt.delta >= 1.) throw std::invalid_argument("cg_reliable_update_block: delta must be in (0,1)");
     const int lo = opt.inner_precision;
-    if (lo != CL2_F32 && lo != CL2_F64) throw std::invalid_argument("cg_reliable_update_block: inner precision FP32 or FP64");
+    if (lo != CL2_F32 && lo != CL2_F64 && !(lo == CL2_F16 && Ops::half))
+        throw std::invalid_argument(Ops::half ? "cg_reliable_update_block: inner precision FP64, FP32 or FP16"
+                                              : "cg_reliable_update_block: inner precision FP32 or FP64");
     cl2_ctx* c = system.ctx();
 
     const typename Ops::Block B(system, n, CL2_F64), X(system, n, CL2_F64), R(system, n, CL2_F64),

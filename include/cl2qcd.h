@@ -262,7 +262,8 @@ int cl2_convert_precision_eo(cl2_ctx* ctx, int kind, void* dst, int dst_prec, co
  * No counterpart in the reference, which inverts the 12 spin-colour sources of a propagator one after
  * the other (physics/algorithms/inversion.cpp:78-242).  A block holds nrhs Wilson eo spinors as one
  * double2[12][Veo][nrhs] (FP64) or float4[6][Veo][nrhs] (FP32, component pairs as in a single field)
- * array, right-hand side fastest, so one launch of an operator reuses every loaded link for all
+ * array, right-hand side fastest, or (FP16, F16_12 links) as one FP16 field of Veo*nrhs sites, right-hand
+ * side r of eo site i at site i*nrhs + r, so one launch of an operator reuses every loaded link for all
  * right-hand sides.  Per-right-hand-side scalars are device arrays (complex: 2 doubles each). */
 /* the eo operators of cl2_fermionmatrix_eo / cl2_dslash_eo on nrhs = 1, 2, 4, 8, 12 or 16 right-hand
  * sides; tmp = 2 block fields */
@@ -280,15 +281,16 @@ int cl2_fermionmatrix_eo_mrhs_params(cl2_ctx* ctx, int op, void* out, const void
  * r_r <- r_r - alpha[r] A p_r, norms[r] (2 doubles) = (|r_new|^2, Re <r_new, r_old>) as cl2_block_axpy_eo
  * gives them.  Replaces, for the block solver, the block operator call, cl2_block_dot_eo, cl2_block_cg_alpha
  * and the residual cl2_block_axpy_eo; A p is never stored.  kappas/mubars: host arrays of nrhs values for an
- * operator per right-hand side, or null for kappa/mubar.  FP64/FP32 links; r2, active, pAp, alpha, norms
- * are device arrays; tmp: 2 blocks. */
+ * operator per right-hand side, or null for kappa/mubar.  Any link format (FP16 blocks with F16_12 links);
+ * r2, active, pAp, alpha, norms are device arrays; tmp: 2 blocks. */
 int cl2_fermionmatrix_eo_mrhs_cg_step(cl2_ctx* ctx, int op, void* r, const void* p, const void* gauge, int format,
                                       int action, double kappa, double mubar, const double* kappas,
                                       const double* mubars, void* tmp, int nrhs, const double* r2,
                                       const double* active, double* pAp, double* alpha, double* norms);
 int cl2_dslash_eo_mrhs(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, int evenodd,
                        double kappa, int nrhs);
-/* copy a single field into / out of slot r of a block (nrhs in [1, 64]) */
+/* copy a single field into / out of slot r of a block (nrhs in [1, 64]; FP64, FP32 or FP16 Wilson blocks, and
+ * likewise for the block BLAS below) */
 int cl2_block_insert_eo(cl2_ctx* ctx, void* block, const void* single, int prec, int nrhs, int r);
 int cl2_block_extract_eo(cl2_ctx* ctx, void* single, const void* block, int prec, int nrhs, int r);
 /* out_r = y_r + sign * coef_r x_r (coef: nrhs complex); out may alias y.  If norms is not null it receives

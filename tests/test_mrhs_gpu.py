@@ -66,6 +66,27 @@ def test_dslash_mrhs_matches_single(nrhs, fmt, lds):
                 assert site_rel_err(block_slot(ctx, out, prec, nrhs, nrhs - 1), ref) < 1e-12
 
 
+@pytest.mark.parametrize("nrhs", [2, 12, 16])
+def test_dslash_mrhs_f16_matches_single(nrhs):
+    """FP16 blocks (one FP16 field of Veo nrhs sites with per-site norms) and FP16 links: slot r of the
+    multi-RHS result equals the single-field FP16 kernel's result for source r bit for bit, both parities."""
+    lat = O.lattice(8, 8, theta_t=1.0)
+    with abi.Context(8, 8, theta_t=1.0) as ctx:
+        g = random_su3_field(4 * O.volume(lat), 5)
+        gf = ctx.gauge(g, abi.GAUGE_F16_12)
+        rng = np.random.default_rng(nrhs)
+        srcs = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
+        blk = make_block(ctx, srcs, abi.F16)
+        out = ctx.buffer(blk.nbytes)
+        a, b = ctx.spinor(abi.F16), ctx.spinor(abi.F16)
+        for evenodd in (abi.EVEN, abi.ODD):
+            ctx.dslash_eo_mrhs(out.ptr, blk.ptr, gf.ptr, abi.GAUGE_F16_12, evenodd, 0.16, nrhs)
+            for r in range(nrhs):
+                a.upload(srcs[r])
+                ctx.dslash_eo(b.ptr, a.ptr, gf.ptr, abi.GAUGE_F16_12, evenodd, 0.16)
+                assert np.array_equal(b.download(), block_slot(ctx, out, abi.F16, nrhs, r))
+
+
 @pytest.mark.parametrize("action", [abi.ACTION_WILSON, abi.ACTION_TWISTEDMASS])
 def test_fermionmatrix_mrhs(action):
     lat = O.lattice(8, 8, theta_t=1.0)
@@ -87,21 +108,25 @@ def test_fermionmatrix_mrhs(action):
             assert site_rel_err(block_slot(ctx, out, abi.F64, nrhs, r), ref) < 1e-12
 
 
-@pytest.mark.parametrize("nrhs", [1, 3, 12])  # FP32: two right-hand sides per 16-byte lane (1, 12) or one (3)
-@pytest.mark.parametrize("prec", [abi.F64, abi.F32])
+@pytest.mark.parametrize("nrhs", [1, 3, 12])
+@pytest.mark.parametrize("prec", [abi.F64, abi.F32, abi.F16])
 def test_block_blas(prec, nrhs):
+    """Block dot, norm and axpy (with its norms) per right-hand side against numpy on the stored values:
+    FP64 to 1e-13, FP32 to 1e-6; FP16 outputs are requantised per site (1e-3), their norms are summed before
+    that (1e-5)."""
     with abi.Context(4, 8) as ctx:
         rng = np.random.default_rng(8)
         X = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
         Y = [rng.standard_normal((ctx.veo, 24)) for _ in range(nrhs)]
         bx, by = make_block(ctx, X, prec), make_block(ctx, Y, prec)
-        if prec == abi.F32:
-            X = [x.astype(np.float32).astype(np.float64) for x in X]
-            Y = [y.astype(np.float32).astype(np.float64) for y in Y]
+        if prec != abi.F64:  # the values the block holds
+            X = [block_slot(ctx, bx, prec, nrhs, r) for r in range(nrhs)]
+            Y = [block_slot(ctx, by, prec, nrhs, r) for r in range(nrhs)]
         coef = rng.standard_normal(2 * nrhs)
         cdev, norms, res = ctx.scalars(2 * nrhs), ctx.scalars(2 * nrhs), ctx.scalars(2 * nrhs)
         cdev.store(coef)
-        tol = 1e-13 if prec == abi.F64 else 1e-6
+        tol = {abi.F64: 1e-13, abi.F32: 1e-6, abi.F16: 1e-3}[prec]
+        ntol = 1e-5 if prec == abi.F16 else tol
         ctx.block_dot_eo(res.ptr, bx.ptr, by.ptr, prec, nrhs)
         d = res.get()
         for r in range(nrhs):
@@ -118,12 +143,12 @@ def test_block_blas(prec, nrhs):
             c = complex(coef[2 * r], coef[2 * r + 1])
             new = O.saxpy(X[r], Y[r], c)
             assert site_rel_err(block_slot(ctx, by, prec, nrhs, r), new) < tol
-            assert abs(nm[2 * r] - O.norm2(new)) < tol * O.norm2(new)
-            assert abs(nm[2 * r + 1] - O.dot(new, Y[r]).real) < tol * O.norm2(new)
+            assert abs(nm[2 * r] - O.norm2(new)) < ntol * O.norm2(new)
+            assert abs(nm[2 * r + 1] - O.dot(new, Y[r]).real) < ntol * O.norm2(new)
 
 
 @pytest.mark.parametrize("nrhs", [1, 3, 12])
-@pytest.mark.parametrize("prec", [abi.F64, abi.F32])
+@pytest.mark.parametrize("prec", [abi.F64, abi.F32, abi.F16])
 def test_block_cg_update_xp(prec, nrhs):
     """x += alpha_r p, p = r + beta_r p in one pass: the same bits as the two block axpys it replaces."""
     with abi.Context(4, 8) as ctx:
@@ -148,8 +173,12 @@ def test_block_cg_update_xp(prec, nrhs):
                 assert site_rel_err(block_slot(ctx, p2, prec, nrhs, k), O.saxpy(P[k], R[k], -cb)) < 1e-15
 
 
-def test_block_cg_matches_single_solves():
+@pytest.mark.parametrize("inner", ["f32", "f16"])
+def test_block_cg_matches_single_solves(inner):
+    """The lockstep block solve with an FP32 or FP16 (per-site norm) inner loop: every right-hand side meets
+    1e-10 in the oracle's FP64 operator and agrees with its own single solve (same inner precision) to 1e-6."""
     from cl2qcd_amd import _host as H
+    prec = H.F16 if inner == "f16" else H.F32
     l = H.Lattice(["--nSpace=8", "--nTime=8", "--fermionAction=twistedmass", "--kappa=0.15", "--mu=0.02",
                    "--beta=5.7", "--thetaFermionTemporal=1"])
     gf = H.Gaugefield(l, False)
@@ -166,7 +195,7 @@ def test_block_cg_matches_single_solves():
         x.zero()
         xs.append(x)
     A = H.QplusQminus_eo(l)
-    it, per = H.cg_reliable_update_block(xs, A, gf, bs, l, 1e-10, l.wilson_parameters())
+    it, per = H.cg_reliable_update_block(xs, A, gf, bs, l, 1e-10, l.wilson_parameters(), prec)
     lat = O.lattice(8, 8, theta_t=1.0)
     g = gf.get_host()
     for r in range(nrhs):
@@ -176,7 +205,7 @@ def test_block_cg_matches_single_solves():
         assert np.sqrt(O.norm2(res) / O.norm2(bh)) <= 1.01e-10
         xs1 = H.Spinorfield_eo(l)
         xs1.zero()
-        H.cg_reliable_update(xs1, A, gf, bs[r], l, 1e-10, l.wilson_parameters())
+        H.cg_reliable_update(xs1, A, gf, bs[r], l, 1e-10, l.wilson_parameters(), prec)
         x1 = xs1.get_host()
         assert np.sqrt(((xh - x1) ** 2).sum() / (x1 ** 2).sum()) < 1e-6
     assert it <= 1.5 * max(p["iterations"] for p in per)
@@ -256,18 +285,19 @@ def test_block_cg_per_rhs_params():
 @pytest.mark.parametrize("ns,nt", [(8, 8), (6, 10)])  # 6^3x10: a partial last block of staged sites
 @pytest.mark.parametrize("per_rhs", [False, True])
 @pytest.mark.parametrize("nrhs", [4, 12])
-@pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12, "F64_18 64-bit offsets"])
+@pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12, abi.GAUGE_F16_12, "F64_18 64-bit offsets"])
 def test_mrhs_cg_step(fmt, nrhs, per_rhs, ns, nt):
     """The fused lockstep CG step (pAp per right-hand side from |Qminus p_r|^2, alpha, r_r -= alpha_r A p_r
     with its norms, A p never stored) against the unfused block sequence (operator, block dot, alpha, block
     axpy with norms): FP64 to 1e-12, FP32 to 1e-5 (pAp is summed from another field, so alpha and everything
-    after it differ by rounding); an inactive right-hand side keeps r and gets alpha 0; FP64 against the oracle."""
+    after it differ by rounding), FP16 to 5e-3 (the unfused sequence also stores A p in half precision); an
+    inactive right-hand side keeps r and gets alpha 0; FP64 against the oracle."""
     import ctypes
     off32 = 0 if isinstance(fmt, str) else 1
     fmt = abi.GAUGE_F64_18 if isinstance(fmt, str) else fmt
     lat = O.lattice(ns, nt, theta_t=1.0)
-    prec = abi.F32 if fmt == abi.GAUGE_F32_12 else abi.F64
-    tol = 1e-12 if prec == abi.F64 else 1e-5
+    prec = {abi.GAUGE_F32_12: abi.F32, abi.GAUGE_F16_12: abi.F16}.get(fmt, abi.F64)
+    tol = {abi.F64: 1e-12, abi.F32: 1e-5, abi.F16: 5e-3}[prec]
     rng = np.random.default_rng(nrhs)
     kappas = list(0.15 + 0.01 * rng.random(nrhs)) if per_rhs else [0.16] * nrhs
     mubars = [2 * k * m for k, m in zip(kappas, 0.003 + 0.017 * rng.random(nrhs))] if per_rhs else \
