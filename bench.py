@@ -466,8 +466,9 @@ def serve_run(engine, reqs, times=None):
             "makespan_s": round(end - start, 3)}
 
 
-def serving_leg(n, load, seed, n_baseline, rate=None, inner="f32", n_capacity=60, n_burst=100, burst_rate=20.0):
-    """Continuous-batching engine (4 workers / HIP streams, FP32-inner mixed solvers, HBM budget 160 GB):
+def serving_leg(n, load, seed, n_baseline, rate=None, inner="f16", n_capacity=60, n_burst=100, burst_rate=20.0):
+    """Continuous-batching engine (4 workers / HIP streams, mixed solvers with an FP16 (or FP32) inner loop for
+    twisted mass and FP32 for staggered, HBM budget 160 GB):
     its capacity with the first n_capacity requests queued at once; latency under a Poisson stream of n
     requests offered at `load` times that capacity (or at `rate`); a bounded run of n_burst requests arriving
     as a Poisson stream at the workload's own burst_rate (20/s) with the queue depth, admissions and joins it
@@ -508,6 +509,7 @@ def serving_leg(n, load, seed, n_baseline, rate=None, inner="f32", n_capacity=60
     log(f"  first {n_baseline}, queued at once: engine {first}; FP64 one at a time {base}")
     return {"requests": f"seed {seed}; sizes 16^3x32 / 24^3x48 / 32^3x64, twisted mass or staggered, 1-12 "
                         "Gaussian RHS, |r|/|b| <= 1e-10; 8 resident heatbath beta 5.7 configurations",
+            "inner_precision": {"twisted_mass": inner, "staggered": "f32"},
             "capacity_all_queued": closed,
             "poisson_stream": {"offered_req_per_s": round(offered, 3),
                                "load": round(offered / closed["throughput_req_per_s"], 3), **stream},
@@ -651,8 +653,9 @@ def main():
                     help="Poisson arrival rate as a fraction of the engine's measured capacity")
     ap.add_argument("--serve-rate", type=float, default=None, help="fixed arrival rate (requests per second)")
     ap.add_argument("--serve-baseline", type=int, default=10, help="requests also run by the FP64 baseline")
-    ap.add_argument("--serve-inner", choices=("f32", "f16"), default="f32",
-                    help="inner precision of the engine's single-right-hand-side twisted-mass solves")
+    ap.add_argument("--serve-inner", choices=("f32", "f16"), default="f16",
+                    help="inner precision of the engine's twisted-mass solves (single and lockstep blocks; "
+                         "staggered solves run FP32 inner)")
     ap.add_argument("--hb-overrelax", type=int, default=1, help="overrelaxation sweeps per heatbath sweep")
     ap.add_argument("--no-autotune", action="store_true",
                     help="time the hopping kernels in the library's default launch shape (for profiles whose "

@@ -66,7 +66,8 @@ struct EngineStats {
 struct EngineOptions {
     size_t hbm_budget_bytes = (size_t)64 << 30;  // device memory for solver work fields
     int workers             = 4;                 // concurrent solves, one HIP stream each
-    int inner_precision     = CL2_F32;           // reduced-precision inner loop of the mixed solvers
+    int inner_precision     = CL2_F32;           // inner loop of the mixed Wilson solvers (CL2_F32 or CL2_F16;
+                                                 // staggered solves run FP32 inner)
     int max_batch_rhs       = 12;                // right-hand sides merged into one Wilson solve
     bool mixed              = true;              // false: the reference's FP64 solvers, one request at a time
     // twisted-mass lockstep solves take queued compatible requests into slots that are free (padding, or

@@ -754,7 +754,7 @@ void Engine::run_tm_block(Worker& wk, Batch& b, Config& cfg)
             bv.push_back(rf.bs[r].get());
         }
         sv::ReliableUpdateOptions o;
-        o.inner_precision = CL2_F32;
+        o.inner_precision = opt.inner_precision;  // FP32, or FP16 blocks with per-site norms
         sv::cg_reliable_update_block(xv, *L.qq, *cfg.gf, bv, sys, rel_tol, kappa, mubar, o, rf);
     } catch (const std::exception& ex) {
         const double fin = now();

@@ -164,14 +164,15 @@ def test_merge_respects_budget_next_to_running_solve():
     assert max(r["batch_rhs"] for r in res) <= 2
 
 
-def test_running_block_takes_queued_requests_into_free_slots():
+@pytest.mark.parametrize("inner", ["f32", "f16"])
+def test_running_block_takes_queued_requests_into_free_slots(inner):
     """Continuous batching: with one worker busy on a 3-RHS twisted-mass solve (padded to 4 slots), a 1-RHS
     request of another quark mass on the same configuration joins the running solve's free slot at its next
     reliable update instead of waiting for the worker; two more join slots that converged.  Every answer meets
     1e-10 (the oracle recomputes one) and equals the same request solved alone to 1e-6; the first request's
-    answer is published before the solve ends."""
+    answer is published before the solve ends.  With an FP32 and with an FP16 (per-site norm) inner loop."""
     from cl2qcd_amd import _host as H
-    e = H.Engine(workers=1)
+    e = H.Engine(workers=1, inner_precision=H.F16 if inner == "f16" else H.F32)
     c0 = e.load_config(8, 8, 5.7, 5, 11)
     e.warmup()
     rng = np.random.default_rng(9)
