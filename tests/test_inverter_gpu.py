@@ -6,7 +6,7 @@ z direction and inverter_test_ref3 for --correlatorDirection=0; the Wilson actio
 inverter_test_ref5) at the reference's 0.1 % tolerance (testInverter.py) and at 1e-8,
 for its default BiCGStab on Aee, bicgstab_save and solver=cg (BiCGStab first, CG on
 QplusQminus_eo only if BiCGStab throws), and for the 12 sources solved as one lockstep
-block (FP32 inner, FP64 reliable updates).  On the cold 4^4 configuration with periodic
+block (FP32 or FP16 inner, FP64 reliable updates).  On the cold 4^4 configuration with periodic
 boundaries t and z are equivalent, so inverter_test_ref holds for both.  The eo solution
 is checked against the oracle's unpreconditioned operator, M phi = eta.
 
@@ -53,12 +53,14 @@ def test_ps_correlator_goldens(solver, args, conf, direction, ref):
         assert abs(got - want) <= 1e-8 * want  # what FP64 actually reaches
 
 
+@pytest.mark.parametrize("inner", ["f32", "f16"])
 @pytest.mark.parametrize("args,conf,direction,ref", GOLDEN_CASES, ids=GOLDEN_IDS)
-def test_ps_correlator_goldens_block(args, conf, direction, ref):
-    """the 12 point sources as one 12-RHS lockstep block (FP32 inner loop, each source to 1e-10 in FP64)"""
+def test_ps_correlator_goldens_block(args, conf, direction, ref, inner):
+    """the 12 point sources as one 12-RHS lockstep block (FP32 or FP16 inner loop, each source to 1e-10 in
+    FP64)"""
     l = H.Lattice(args + [f"--correlatorDirection={direction}"])
     gf = H.Gaugefield(l, CONF_00200) if conf else H.Gaugefield(l, False)
-    c = H.point_source_ps_correlator(gf, l, block=True)
+    c = H.point_source_ps_correlator(gf, l, block=True, inner_prec=H.F16 if inner == "f16" else H.F32)
     for got, want in zip(c, ref):
         assert abs(got - want) <= 1e-8 * want
 
