@@ -264,11 +264,13 @@ def batch_leg(ns, nt, sweeps, seed, nrhs=12):
         worst = max(p["true_relative_residual"] for p in per)
         name = "FP16" if prec == H.F16 else "FP32"
         log(f"12 point sources {ns}^3x{nt}, {name} inner: sequential {t_seq:.2f} s ({sum(seq_it)} iterations), "
-            f"block {t_blk:.2f} s ({it} lockstep iterations), worst true |r|/|b| {worst:.2e}")
+            f"block {t_blk:.2f} s ({it} lockstep iterations, {per[0]['reliable_updates']} reliable updates), "
+            f"worst true |r|/|b| {worst:.2e}")
         return t_seq, t_blk, {"sequential": {"seconds": round(t_seq, 3), "per_rhs_seconds": round(t_seq / nrhs, 4),
                                              "iterations": seq_it},
                               "block": {"seconds": round(t_blk, 3), "per_rhs_seconds": round(t_blk / nrhs, 4),
                                         "lockstep_iterations": it, "iterations": [p["iterations"] for p in per],
+                                        "reliable_updates": per[0]["reliable_updates"],
                                         "worst_true_rel_residual": worst},
                               "speedup_per_rhs": round(t_seq / t_blk, 3)}
 
