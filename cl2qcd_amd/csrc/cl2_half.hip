@@ -140,8 +140,10 @@ __device__ __forceinline__ void load_link_h(const void* g, uint32_t S, int dir, 
 }
 
 /* one projector row pair of a hop on a neighbour spinor already in registers
- * (same algebra as the FP64/FP32 kernel: h = psi_A + C psi_B, r = w U h) */
-template<int A, int B, int CU, bool FWD>
+ * (same algebra as the FP64/FP32 kernel: h = psi_A + C psi_B, r = w U h).  RW: the weight is real (a spatial
+ * hop without a boundary phase), so w r is two multiplications instead of a complex product -- the same bits,
+ * since the complex product with a zero imaginary part adds only exact zeros. */
+template<int A, int B, int CU, bool FWD, bool RW = false>
 __device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const Ch (&psi)[12], const Su3<float>& u,
                                            Cx<float> w, float one)
 {
@@ -154,36 +156,40 @@ __device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const Ch (&psi)
         su3dag_times(u, h, r);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        Cx<float> t    = cmul(r[c], w);
+        Cx<float> t;
+        if constexpr (RW)
+            t = scale(r[c], w.re);
+        else
+            t = cmul(r[c], w);
         acc[3 * A + c] = acc[3 * A + c] + t;
         acc[3 * B + c] = add_unit<UnitConj<CU>::v>(acc[3 * B + c], t);
     }
 }
 
-template<int DIR, bool FWD>
+template<int DIR, bool FWD, bool RW = false>
 __device__ __forceinline__ void apply_hop_h(Cx<float> (&acc)[12], const Ch (&psi)[12], const Su3<float>& u,
                                             Cx<float> w)
 {
     float one;  // 1.0 hidden from constant folding (see add_unit_h)
     asm("v_mov_b32 %0, 1.0" : "=v"(one));
     if constexpr (DIR == 0) {
-        half_hop_r<0, 2, FWD ? P1 : M1, FWD>(acc, psi, u, w, one);
-        half_hop_r<1, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w, one);
+        half_hop_r<0, 2, FWD ? P1 : M1, FWD, RW>(acc, psi, u, w, one);
+        half_hop_r<1, 3, FWD ? P1 : M1, FWD, RW>(acc, psi, u, w, one);
     } else if constexpr (DIR == 1) {
-        half_hop_r<0, 3, FWD ? PI : MI, FWD>(acc, psi, u, w, one);
-        half_hop_r<1, 2, FWD ? PI : MI, FWD>(acc, psi, u, w, one);
+        half_hop_r<0, 3, FWD ? PI : MI, FWD, RW>(acc, psi, u, w, one);
+        half_hop_r<1, 2, FWD ? PI : MI, FWD, RW>(acc, psi, u, w, one);
     } else if constexpr (DIR == 2) {
-        half_hop_r<0, 3, FWD ? P1 : M1, FWD>(acc, psi, u, w, one);
-        half_hop_r<1, 2, FWD ? M1 : P1, FWD>(acc, psi, u, w, one);
+        half_hop_r<0, 3, FWD ? P1 : M1, FWD, RW>(acc, psi, u, w, one);
+        half_hop_r<1, 2, FWD ? M1 : P1, FWD, RW>(acc, psi, u, w, one);
     } else {
-        half_hop_r<0, 2, FWD ? PI : MI, FWD>(acc, psi, u, w, one);
-        half_hop_r<1, 3, FWD ? MI : PI, FWD>(acc, psi, u, w, one);
+        half_hop_r<0, 2, FWD ? PI : MI, FWD, RW>(acc, psi, u, w, one);
+        half_hop_r<1, 3, FWD ? MI : PI, FWD, RW>(acc, psi, u, w, one);
     }
 }
 
 /* hops of directions D0 .. D0 + ND - 1 (forward then backward each), with the links and neighbour sites of
  * H of them loaded before they are applied (H = 1, 2 or 4, ND * 2 a multiple of H) */
-template<int D0, int ND, int H>
+template<int D0, int ND, int H, bool RS = false>
 __device__ __forceinline__ void hops_h(Cx<float> (&acc)[12], const void* in, uint32_t S, const SiteNb& nb,
                                        const void* gauge, int P, uint32_t i, const HopWeights<double>& w)
 {
@@ -194,11 +200,11 @@ __device__ __forceinline__ void hops_h(Cx<float> (&acc)[12], const void* in, uin
         if constexpr (H == 1) {
             load_link_h(gauge, S, D0, P, i, u[0]);
             float n = load_site_h_raw(in, S, nb.up[D0], psi[0]);
-            apply_hop_h<D0, true>(acc, psi[0], u[0], weight_n(w.wf[D0], n));
+            apply_hop_h<D0, true, RS && D0 != 0>(acc, psi[0], u[0], weight_n(w.wf[D0], n));
             load_link_h(gauge, S, D0, P ^ 1, nb.dn[D0], u[0]);
             n = load_site_h_raw(in, S, nb.dn[D0], psi[0]);
-            apply_hop_h<D0, false>(acc, psi[0], u[0], weight_n(w.wb[D0], n));
-            hops_h<D0 + 1, ND - 1, H>(acc, in, S, nb, gauge, P, i, w);
+            apply_hop_h<D0, false, RS && D0 != 0>(acc, psi[0], u[0], weight_n(w.wb[D0], n));
+            hops_h<D0 + 1, ND - 1, H, RS>(acc, in, S, nb, gauge, P, i, w);
         } else {
             float n[2 * G];
 #pragma unroll
@@ -211,20 +217,20 @@ __device__ __forceinline__ void hops_h(Cx<float> (&acc)[12], const void* in, uin
                 n[2 * g]     = load_site_h_raw(in, S, nb.up[D0 + g], psi[2 * g]);
                 n[2 * g + 1] = load_site_h_raw(in, S, nb.dn[D0 + g], psi[2 * g + 1]);
             }
-            apply_hop_h<D0, true>(acc, psi[0], u[0], weight_n(w.wf[D0], n[0]));
-            apply_hop_h<D0, false>(acc, psi[1], u[1], weight_n(w.wb[D0], n[1]));
+            apply_hop_h<D0, true, RS && D0 != 0>(acc, psi[0], u[0], weight_n(w.wf[D0], n[0]));
+            apply_hop_h<D0, false, RS && D0 != 0>(acc, psi[1], u[1], weight_n(w.wb[D0], n[1]));
             if constexpr (G == 2) {
-                apply_hop_h<D0 + 1, true>(acc, psi[2], u[2], weight_n(w.wf[D0 + 1], n[2]));
-                apply_hop_h<D0 + 1, false>(acc, psi[3], u[3], weight_n(w.wb[D0 + 1], n[3]));
+                apply_hop_h<D0 + 1, true, RS && D0 + 1 != 0>(acc, psi[2], u[2], weight_n(w.wf[D0 + 1], n[2]));
+                apply_hop_h<D0 + 1, false, RS && D0 + 1 != 0>(acc, psi[3], u[3], weight_n(w.wb[D0 + 1], n[3]));
             }
-            hops_h<D0 + G, ND - G, H>(acc, in, S, nb, gauge, P, i, w);
+            hops_h<D0 + G, ND - G, H, RS>(acc, in, S, nb, gauge, P, i, w);
         }
     }
 }
 
 /* H hops in flight per thread; the register budget grows with H (H = 4: up to 2 waves per SIMD, H = 2: 4,
  * H = 1: the compiler's choice) */
-template<int BLOCK, int H>
+template<int BLOCK, int H, bool RS>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, H == 4 ? 2 : (H == 2 ? 4 : 8))))
 void k_dslash_eo_h(DslashArgs a)
 {
@@ -239,7 +245,7 @@ void k_dslash_eo_h(DslashArgs a)
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] = Cx<float>{0.f, 0.f};
         // order T, X, Y, Z, forward then backward, as fermionmatrix_eo_dslash.cl:38-45
-        hops_h<0, 4, H>(acc, a.in, S, nb, a.gauge, a.out_parity, i, a.w);
+        hops_h<0, 4, H, RS>(acc, a.in, S, nb, a.gauge, a.out_parity, i, a.w);
 
         const Cx<float> f1{float(a.f1[0]), float(a.f1[1])}, f2{float(a.f2[0]), float(a.f2[1])};
         if (a.epilogue == EPI_MINV) {
@@ -279,20 +285,31 @@ void k_dslash_eo_h(DslashArgs a)
     }
 }
 
+/* whether the X, Y, Z hop weights have no imaginary part (no spatial boundary phase): the kernels then
+ * multiply by them as reals */
+static bool spatial_weights_real(const HopWeights<double>& w)
+{
+    for (int mu = 1; mu < 4; ++mu)
+        if (w.wf[mu][1] != 0. || w.wb[mu][1] != 0.) return false;
+    return true;
+}
+
 /* launch shape: block size a.block (128, 256, 512), hops in flight by the register budget a.occ (4: one
  * hop at a time, 0: two, 1: four) */
 hipError_t launch_dslash_h(cl2_ctx* c, const DslashArgs& a)
 {
     const int bs      = (a.block == 128 || a.block == 512) ? a.block : 256;
     const unsigned nb = blocks_for(a.g.veo, bs);
+    const bool rs = spatial_weights_real(a.w);
     auto go = [&](auto block_tag) {
         constexpr int B = decltype(block_tag)::value;
+        auto launch = [&](auto kernel) { hipLaunchKernelGGL(kernel, dim3(nb), dim3(B), 0, c->stream, a); };
         if (a.occ == 4)
-            hipLaunchKernelGGL((k_dslash_eo_h<B, 1>), dim3(nb), dim3(B), 0, c->stream, a);
+            rs ? launch(k_dslash_eo_h<B, 1, true>) : launch(k_dslash_eo_h<B, 1, false>);
         else if (a.occ == 1 && B != 512)
-            hipLaunchKernelGGL((k_dslash_eo_h<B, 4>), dim3(nb), dim3(B), 0, c->stream, a);
+            rs ? launch(k_dslash_eo_h<B, 4, true>) : launch(k_dslash_eo_h<B, 4, false>);
         else
-            hipLaunchKernelGGL((k_dslash_eo_h<B, 2>), dim3(nb), dim3(B), 0, c->stream, a);
+            rs ? launch(k_dslash_eo_h<B, 2, true>) : launch(k_dslash_eo_h<B, 2, false>);
     };
     if (bs == 128)
         go(std::integral_constant<int, 128>{});
@@ -527,7 +544,7 @@ struct MrhsH {
     static constexpr int BLOCK = SB * NRHS;
 };
 
-template<int NRHS>
+template<int NRHS, bool RS>
 __global__ __launch_bounds__(MrhsH<NRHS>::BLOCK) void k_dslash_eo_mrhs_h(DslashArgs a)
 {
     constexpr int SB        = MrhsH<NRHS>::SB;
@@ -583,8 +600,8 @@ __global__ __launch_bounds__(MrhsH<NRHS>::BLOCK) void k_dslash_eo_mrhs_h(DslashA
         const float nf = load_site_h_raw(a.in, SN, lnb[s * 8 + 2 * (m)] * NRHS + rhs, pf);             \
         const float nk = load_site_h_raw(a.in, SN, lnb[s * 8 + 2 * (m) + 1] * NRHS + rhs, pb);         \
         const Su3<float> uf = lu[s * 8 + 2 * (m)], ub = lu[s * 8 + 2 * (m) + 1];                       \
-        apply_hop_h<m, true>(acc, pf, uf, weight_n(a.w.wf[m], nf));                                    \
-        apply_hop_h<m, false>(acc, pb, ub, weight_n(a.w.wb[m], nk));                                   \
+        apply_hop_h<m, true, RS && (m) != 0>(acc, pf, uf, weight_n(a.w.wf[m], nf));                  \
+        apply_hop_h<m, false, RS && (m) != 0>(acc, pb, ub, weight_n(a.w.wb[m], nk));                 \
     }
         CL2_DIR_H(0)
         CL2_DIR_H(1)
@@ -649,10 +666,14 @@ __global__ __launch_bounds__(MrhsH<NRHS>::BLOCK) void k_dslash_eo_mrhs_h(DslashA
 
 hipError_t launch_dslash_mrhs_h(cl2_ctx* c, const DslashArgs& a)
 {
+    const bool rs = spatial_weights_real(a.w);
     auto go = [&](auto n_tag) -> hipError_t {
         constexpr int N = decltype(n_tag)::value;
-        hipLaunchKernelGGL((k_dslash_eo_mrhs_h<N>), dim3((a.g.veo + MrhsH<N>::SB - 1) / MrhsH<N>::SB),
-                           dim3(MrhsH<N>::BLOCK), 0, c->stream, a);
+        const dim3 grid((a.g.veo + MrhsH<N>::SB - 1) / MrhsH<N>::SB), block(MrhsH<N>::BLOCK);
+        if (rs)
+            hipLaunchKernelGGL((k_dslash_eo_mrhs_h<N, true>), grid, block, 0, c->stream, a);
+        else
+            hipLaunchKernelGGL((k_dslash_eo_mrhs_h<N, false>), grid, block, 0, c->stream, a);
         return hipGetLastError();
     };
     switch (a.nrhs) {
