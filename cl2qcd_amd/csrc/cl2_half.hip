@@ -1,6 +1,8 @@
 // FP16 storage for the reduced-precision inner solves: Wilson eo spinors as
 // half precision with one FP32 scale per site, and 12-real links in half
-// precision.  All arithmetic is FP32; reductions accumulate in FP64.
+// precision.  BLAS arithmetic is FP32 and reductions accumulate in FP64; the
+// hopping kernels form the spin projection in half precision and multiply it by
+// the link with half-precision dot products that accumulate in FP32.
 //
 // Spinor layout (52 bytes per eo site): 3 lanes of 8 halves (16 bytes), lane j
 // holding complex components 4j..4j+3 (component = 3*spin + colour) as re, im
@@ -25,6 +27,7 @@
 
 namespace cl2 {
 
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 typedef _Float16 h4 __attribute__((ext_vector_type(4)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -55,36 +58,39 @@ __device__ __forceinline__ void load_site_h(const void* p, uint32_t S, uint32_t 
     }
 }
 
-/* the same without the scale: the site's 24 stored values (converted to float) and its norm, for the
- * hopping kernels, which fold the norm into the hop weight (the hop is linear in the spinor) and so save
- * 24 multiplications per neighbour and let the half-to-float conversions fuse into the arithmetic */
-struct Ch {
-    _Float16 re, im;
-};
-
-__device__ __forceinline__ float load_site_h_raw(const void* p, uint32_t S, uint32_t i, Ch (&v)[12])
+/* the same without the scale: the site's 12 stored (re, im) half pairs and its norm, for the hopping kernels,
+ * which fold the norm into the hop weight (the hop is linear in the spinor) and so work on the stored halves */
+__device__ __forceinline__ float load_site_h_raw(const void* p, uint32_t S, uint32_t i, h2 (&v)[12])
 {
     const h8* l = reinterpret_cast<const h8*>(p);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const h8 f = l[(size_t)j * S + i];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) v[4 * j + q] = Ch{f[2 * q], f[2 * q + 1]};
+        for (int q = 0; q < 4; ++q) v[4 * j + q] = h2{f[2 * q], f[2 * q + 1]};
     }
     return half_norms(p, S)[i];
 }
 
-/* a + U*b in float from half inputs.  Each component is one fma(x, +-1, y) -- the float add or subtract
- * of the exactly converted halves, bit for bit -- with a 1 the compiler cannot fold away, so that the
- * conversions fuse into the operation (v_fma_mix_f32) instead of costing an instruction each. */
-template<int U>
-__device__ __forceinline__ Cx<float> add_unit_h(Ch a, Ch b, float one)
+/* the two operand forms of one component of h = a + C b (C in {+1, -1, +i, -i}, a and b stored halves) for
+ * the dot products of the link product: (h_re, -h_im) and (h_im, h_re) for U h (FWD), (h_re, h_im) and
+ * (h_im, -h_re) for U^dagger h.  Each form is one packed half add (signs and the swap are operand modifiers),
+ * so h is rounded to half once and both forms carry that value. */
+template<int C, bool FWD>
+__device__ __forceinline__ void proj_h(h2 a, h2 b, h2& f1, h2& f2)
 {
-    const float ar = (float)a.re, ai = (float)a.im, br = (float)b.re, bi = (float)b.im;
-    if constexpr (U == P1) return Cx<float>{__builtin_fmaf(br, one, ar), __builtin_fmaf(bi, one, ai)};
-    if constexpr (U == M1) return Cx<float>{__builtin_fmaf(br, -one, ar), __builtin_fmaf(bi, -one, ai)};
-    if constexpr (U == PI) return Cx<float>{__builtin_fmaf(bi, -one, ar), __builtin_fmaf(br, one, ai)};
-    return Cx<float>{__builtin_fmaf(bi, one, ar), __builtin_fmaf(br, -one, ai)};
+    h2 cb;  // C b
+    if constexpr (C == P1) cb = b;
+    else if constexpr (C == M1) cb = -b;
+    else if constexpr (C == PI) cb = h2{-b.y, b.x};
+    else cb = h2{b.y, -b.x};
+    if constexpr (FWD) {
+        f1 = h2{a.x, -a.y} + h2{cb.x, -cb.y};
+        f2 = h2{a.y, a.x} + h2{cb.y, cb.x};
+    } else {
+        f1 = a + cb;
+        f2 = h2{a.y, -a.x} + h2{cb.y, -cb.x};
+    }
 }
 
 /* hop weight times a site's norm */
@@ -122,68 +128,117 @@ __device__ __forceinline__ void store_site_h(void* p, uint32_t S, uint32_t i, co
         half_norms(p, S)[i] = m;
 }
 
-__device__ __forceinline__ void load_link_h(const void* g, uint32_t S, int dir, int par, uint32_t eo, Su3<float>& u)
+/* an FP16 link as the hopping kernels use it: rows 0 and 1 as stored, row 2 = conj(row0 x row1) formed in
+ * float from them and rounded to half; entries are (re, im) half pairs */
+struct Su3h {
+    h2 e[3][3];
+};
+
+__device__ __forceinline__ void load_link_h(const void* g, uint32_t S, int dir, int par, uint32_t eo, Su3h& u)
 {
     const char* base = static_cast<const char*>(g);
     const uint32_t dp = (uint32_t)(dir * 2 + par);
-    const f8v a       = __builtin_convertvector(
-        __builtin_nontemporal_load(reinterpret_cast<const h8*>(base + link_h16_offset(dp, eo, S))), f8v);
-    const f4v b = __builtin_convertvector(
-        __builtin_nontemporal_load(reinterpret_cast<const h4*>(base + link_h8_offset(dp, eo, S))), f4v);
-    u.e[0][0] = Cx<float>{a[0], a[1]};
-    u.e[0][1] = Cx<float>{a[2], a[3]};
-    u.e[0][2] = Cx<float>{a[4], a[5]};
-    u.e[1][0] = Cx<float>{a[6], a[7]};
-    u.e[1][1] = Cx<float>{b.x, b.y};
-    u.e[1][2] = Cx<float>{b.z, b.w};
-    su3_reconstruct_row2(u);
+    const h8 a = __builtin_nontemporal_load(reinterpret_cast<const h8*>(base + link_h16_offset(dp, eo, S)));
+    const h4 b = __builtin_nontemporal_load(reinterpret_cast<const h4*>(base + link_h8_offset(dp, eo, S)));
+    u.e[0][0] = h2{a[0], a[1]};
+    u.e[0][1] = h2{a[2], a[3]};
+    u.e[0][2] = h2{a[4], a[5]};
+    u.e[1][0] = h2{a[6], a[7]};
+    u.e[1][1] = h2{b[0], b[1]};
+    u.e[1][2] = h2{b[2], b[3]};
+    Su3<float> f;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) f.e[r][c] = Cx<float>{(float)u.e[r][c].x, (float)u.e[r][c].y};
+    su3_reconstruct_row2(f);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u.e[2][c] = h2{(_Float16)f.e[2][c].re, (_Float16)f.e[2][c].im};
 }
 
-/* one projector row pair of a hop on a neighbour spinor already in registers
- * (same algebra as the FP64/FP32 kernel: h = psi_A + C psi_B, r = w U h).  RW: the weight is real (a spatial
- * hop without a boundary phase), so w r is two multiplications instead of a complex product -- the same bits,
- * since the complex product with a zero imaginary part adds only exact zeros. */
-template<int A, int B, int CU, bool FWD, bool RW = false>
-__device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const Ch (&psi)[12], const Su3<float>& u,
-                                           Cx<float> w, float one)
+/* x.lo y.lo + x.hi y.hi in float: the first dot product of a chain.  ASM: the same instruction with the zero
+ * accumulator as an inline constant (the builtin's accumulating form first copies a zero into the result
+ * register; the multi-RHS kernel saves those copies, while in the single-field kernels the asm form makes the
+ * scheduler hold far more registers) */
+template<bool ASM>
+__device__ __forceinline__ float dot2_first(h2 x, h2 y)
 {
-    Cx<float> h[3], r[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) h[c] = add_unit_h<CU>(psi[3 * A + c], psi[3 * B + c], one);
-    if constexpr (FWD)
-        su3_times(u, h, r);
-    else
-        su3dag_times(u, h, r);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        Cx<float> t;
-        if constexpr (RW)
-            t = scale(r[c], w.re);
-        else
-            t = cmul(r[c], w);
-        acc[3 * A + c] = acc[3 * A + c] + t;
-        acc[3 * B + c] = add_unit<UnitConj<CU>::v>(acc[3 * B + c], t);
+    if constexpr (ASM) {
+        float r;
+        asm("v_dot2_f32_f16 %0, %1, %2, 0" : "=v"(r) : "v"(x), "v"(y));
+        return r;
+    } else {
+        return __builtin_amdgcn_fdot2(x, y, 0.f, false);
     }
 }
 
-template<int DIR, bool FWD, bool RW = false>
-__device__ __forceinline__ void apply_hop_h(Cx<float> (&acc)[12], const Ch (&psi)[12], const Su3<float>& u,
-                                            Cx<float> w)
+/* r = U h (FWD) or U^dagger h from the two forms of each component of h: Re r_a = sum_b dot(U_ab, f1_b),
+ * Im r_a = sum_b dot(U_ab, f2_b) (U_ba for U^dagger).  Each dot takes two exact half products and accumulates
+ * in float, b = 0, 1, 2 in turn. */
+template<bool FWD, bool ASM>
+__device__ __forceinline__ void su3_dot_h(const Su3h& u, const h2 (&f1)[3], const h2 (&f2)[3], Cx<float> (&r)[3])
 {
-    float one;  // 1.0 hidden from constant folding (see add_unit_h)
-    asm("v_mov_b32 %0, 1.0" : "=v"(one));
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const h2 e0 = FWD ? u.e[a][0] : u.e[0][a];
+        float re    = dot2_first<ASM>(e0, f1[0]), im = dot2_first<ASM>(e0, f2[0]);
+#pragma unroll
+        for (int b = 1; b < 3; ++b) {
+            const h2 e = FWD ? u.e[a][b] : u.e[b][a];
+            re         = __builtin_amdgcn_fdot2(e, f1[b], re, false);
+            im         = __builtin_amdgcn_fdot2(e, f2[b], im, false);
+        }
+        r[a] = Cx<float>{re, im};
+    }
+}
+
+/* one projector row pair of a hop on a neighbour spinor already in registers
+ * (same algebra as the FP64/FP32 kernel: h = psi_A + C psi_B, r = w U h, with h in half precision and U h
+ * accumulated in float).  RW: the weight is real (a spatial hop without a boundary phase), so w r enters both
+ * accumulators by fused multiply-adds instead of a complex product and two additions. */
+template<int A, int B, int CU, bool FWD, bool RW, bool ASM>
+__device__ __forceinline__ void half_hop_r(Cx<float> (&acc)[12], const h2 (&psi)[12], const Su3h& u, Cx<float> w)
+{
+    h2 f1[3], f2[3];
+    Cx<float> r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) proj_h<CU, FWD>(psi[3 * A + c], psi[3 * B + c], f1[c], f2[c]);
+    su3_dot_h<FWD, ASM>(u, f1, f2, r);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        if constexpr (RW) {  // acc_A += w r, acc_B += conj(C) w r as fused multiply-adds
+            constexpr int CB = UnitConj<CU>::v;
+            Cx<float>& x     = acc[3 * A + c];
+            Cx<float>& y     = acc[3 * B + c];
+            const float wr   = w.re;
+            x = Cx<float>{__builtin_fmaf(r[c].re, wr, x.re), __builtin_fmaf(r[c].im, wr, x.im)};
+            if constexpr (CB == P1) y = Cx<float>{__builtin_fmaf(r[c].re, wr, y.re), __builtin_fmaf(r[c].im, wr, y.im)};
+            if constexpr (CB == M1) y = Cx<float>{__builtin_fmaf(r[c].re, -wr, y.re), __builtin_fmaf(r[c].im, -wr, y.im)};
+            if constexpr (CB == PI) y = Cx<float>{__builtin_fmaf(r[c].im, -wr, y.re), __builtin_fmaf(r[c].re, wr, y.im)};
+            if constexpr (CB == MI) y = Cx<float>{__builtin_fmaf(r[c].im, wr, y.re), __builtin_fmaf(r[c].re, -wr, y.im)};
+        } else {
+            const Cx<float> t = cmul(r[c], w);
+            acc[3 * A + c]    = acc[3 * A + c] + t;
+            acc[3 * B + c]    = add_unit<UnitConj<CU>::v>(acc[3 * B + c], t);
+        }
+    }
+}
+
+template<int DIR, bool FWD, bool RW = false, bool ASM = false>
+__device__ __forceinline__ void apply_hop_h(Cx<float> (&acc)[12], const h2 (&psi)[12], const Su3h& u, Cx<float> w)
+{
     if constexpr (DIR == 0) {
-        half_hop_r<0, 2, FWD ? P1 : M1, FWD, RW>(acc, psi, u, w, one);
-        half_hop_r<1, 3, FWD ? P1 : M1, FWD, RW>(acc, psi, u, w, one);
+        half_hop_r<0, 2, FWD ? P1 : M1, FWD, RW, ASM>(acc, psi, u, w);
+        half_hop_r<1, 3, FWD ? P1 : M1, FWD, RW, ASM>(acc, psi, u, w);
     } else if constexpr (DIR == 1) {
-        half_hop_r<0, 3, FWD ? PI : MI, FWD, RW>(acc, psi, u, w, one);
-        half_hop_r<1, 2, FWD ? PI : MI, FWD, RW>(acc, psi, u, w, one);
+        half_hop_r<0, 3, FWD ? PI : MI, FWD, RW, ASM>(acc, psi, u, w);
+        half_hop_r<1, 2, FWD ? PI : MI, FWD, RW, ASM>(acc, psi, u, w);
     } else if constexpr (DIR == 2) {
-        half_hop_r<0, 3, FWD ? P1 : M1, FWD, RW>(acc, psi, u, w, one);
-        half_hop_r<1, 2, FWD ? M1 : P1, FWD, RW>(acc, psi, u, w, one);
+        half_hop_r<0, 3, FWD ? P1 : M1, FWD, RW, ASM>(acc, psi, u, w);
+        half_hop_r<1, 2, FWD ? M1 : P1, FWD, RW, ASM>(acc, psi, u, w);
     } else {
-        half_hop_r<0, 2, FWD ? PI : MI, FWD, RW>(acc, psi, u, w, one);
-        half_hop_r<1, 3, FWD ? MI : PI, FWD, RW>(acc, psi, u, w, one);
+        half_hop_r<0, 2, FWD ? PI : MI, FWD, RW, ASM>(acc, psi, u, w);
+        half_hop_r<1, 3, FWD ? MI : PI, FWD, RW, ASM>(acc, psi, u, w);
     }
 }
 
@@ -195,8 +250,8 @@ __device__ __forceinline__ void hops_h(Cx<float> (&acc)[12], const void* in, uin
 {
     if constexpr (ND > 0) {
         constexpr int G = H < 2 ? 1 : H / 2;  // directions per group
-        Su3<float> u[2 * G];
-        Ch psi[2 * G][12];
+        Su3h u[2 * G];
+        h2 psi[2 * G][12];
         if constexpr (H == 1) {
             load_link_h(gauge, S, D0, P, i, u[0]);
             float n = load_site_h_raw(in, S, nb.up[D0], psi[0]);
@@ -534,8 +589,8 @@ hipError_t launch_convert_h(cl2_ctx* c, void* out, int out_prec, const void* in,
  * count applies to a block unchanged. */
 
 /* Multi-RHS hopping kernel on FP16 blocks, the staging of k_dslash_eo_mrhs_lds: a block owns SB consecutive
- * sites of the work order and NRHS threads per site; the sites' neighbour indices and their 8 links
- * (reconstructed to FP32) are put in LDS once, then every right-hand-side thread gathers its own FP16
+ * sites of the work order and NRHS threads per site; the sites' neighbour indices and their 8 links (row 2
+ * reconstructed, as half pairs) are put in LDS once, then every right-hand-side thread gathers its own FP16
  * neighbour sites.  Per right-hand side the arithmetic is k_dslash_eo_h's (same hop order, norms folded into
  * the hop weights), so slot r equals the single-field kernel bit for bit. */
 template<int NRHS>
@@ -550,7 +605,7 @@ __global__ __launch_bounds__(MrhsH<NRHS>::BLOCK) void k_dslash_eo_mrhs_h(DslashA
     constexpr int SB        = MrhsH<NRHS>::SB;
     constexpr int BLOCK     = MrhsH<NRHS>::BLOCK;
     constexpr uint32_t NONE = 0xffffffffu;
-    __shared__ Su3<float> lu[SB * 8];
+    __shared__ Su3h lu[SB * 8];
     __shared__ uint32_t lnb[SB * 8];
     __shared__ uint32_t lsite[SB];
     __shared__ double2 lred[BLOCK];
@@ -578,7 +633,7 @@ __global__ __launch_bounds__(MrhsH<NRHS>::BLOCK) void k_dslash_eo_mrhs_h(DslashA
         const uint32_t i = lsite[w >> 3];
         if (i != NONE) {
             const int h = w & 7;
-            Su3<float> u;
+            Su3h u;
             // forward link: the site's own; backward link: the neighbour's, of the other parity
             load_link_h(a.gauge, S, h >> 1, (h & 1) ? P ^ 1 : P, (h & 1) ? lnb[w] : i, u);
             lu[w] = u;
@@ -596,12 +651,12 @@ __global__ __launch_bounds__(MrhsH<NRHS>::BLOCK) void k_dslash_eo_mrhs_h(DslashA
         // order T, X, Y, Z, forward then backward, as fermionmatrix_eo_dslash.cl:38-45
 #define CL2_DIR_H(m)                                                                                   \
     {                                                                                                  \
-        Ch pf[12], pb[12];                                                                             \
+        h2 pf[12], pb[12];                                                                             \
         const float nf = load_site_h_raw(a.in, SN, lnb[s * 8 + 2 * (m)] * NRHS + rhs, pf);             \
         const float nk = load_site_h_raw(a.in, SN, lnb[s * 8 + 2 * (m) + 1] * NRHS + rhs, pb);         \
-        const Su3<float> uf = lu[s * 8 + 2 * (m)], ub = lu[s * 8 + 2 * (m) + 1];                       \
-        apply_hop_h<m, true, RS && (m) != 0>(acc, pf, uf, weight_n(a.w.wf[m], nf));                  \
-        apply_hop_h<m, false, RS && (m) != 0>(acc, pb, ub, weight_n(a.w.wb[m], nk));                 \
+        const Su3h uf = lu[s * 8 + 2 * (m)], ub = lu[s * 8 + 2 * (m) + 1];                             \
+        apply_hop_h<m, true, RS && (m) != 0, true>(acc, pf, uf, weight_n(a.w.wf[m], nf));            \
+        apply_hop_h<m, false, RS && (m) != 0, true>(acc, pb, ub, weight_n(a.w.wb[m], nk));           \
     }
         CL2_DIR_H(0)
         CL2_DIR_H(1)

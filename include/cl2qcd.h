@@ -49,7 +49,8 @@ extern "C" {
 /* ---- precisions and gauge formats ---- */
 #define CL2_F64 0
 #define CL2_F32 1
-#define CL2_F16 2 /* Wilson eo spinors only: half storage with one float scale per site, FP32 arithmetic */
+#define CL2_F16 2 /* Wilson eo spinors only: half storage with one float scale per site; BLAS in FP32, the
+                     hopping term with half spin projections and half dot products accumulated in FP32 */
 #define CL2_GAUGE_F64_18 0
 #define CL2_GAUGE_F64_12 1
 #define CL2_GAUGE_F32_12 2

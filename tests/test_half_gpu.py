@@ -5,7 +5,8 @@ Bars:
   the site's largest entry (2^-12 * max, plus FP32 rounding) of the FP64 value;
 * arithmetic on already-quantised inputs (the dequantised field, links rounded
   to FP16 with row 2 rebuilt in FP64) matches the FP64 oracle to 1e-3 relative
-  per site: FP32 arithmetic plus one output quantisation;
+  per site: the hopping term rounds each spin projection and link row 2 to FP16
+  and accumulates its half products in FP32, plus one output quantisation;
 * against the exact FP64 inputs, the whole-field relative error stays below 3e-3;
 * reductions of FP16 fields accumulate in FP64 from the dequantised values, so
   they match numpy on those values to 1e-12.
