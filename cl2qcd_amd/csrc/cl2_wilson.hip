@@ -507,11 +507,11 @@ cl2_hop_tuning hop_tuning(const cl2_ctx* c, int fmt, int nrhs)
         auto it = c->tuned.find({fmt, nrhs});
         if (it != c->tuned.end()) return it->second;
     }
-    // default register budget (the autotuner's choices at 48^3x96): four hops in flight for the FP64 links,
-    // hop pairs for FP32, one hop at a time for FP16
+    // default register budget (the autotuner's choices at 48^3x96): four hops in flight for the FP64 links (one
+    // wave per SIMD) and for FP32 (two waves per SIMD), hop pairs for FP16
     const int occ = c->tune_occ >= 0
                         ? c->tune_occ
-                        : ((fmt == GAUGE_F64_18 || fmt == GAUGE_F64_12) ? 1 : (fmt == GAUGE_F16_12 ? 4 : 0));
+                        : ((fmt == GAUGE_F64_18 || fmt == GAUGE_F64_12) ? 1 : (fmt == GAUGE_F32_12 ? 2 : 0));
     return cl2_hop_tuning{c->tune_zplanes, c->tune_xcd, c->tune_mrhs_rb, c->tune_mrhs_lds, c->tune_block, occ};
 }
 

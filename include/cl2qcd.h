@@ -430,7 +430,7 @@ int cl2_cg_beta(cl2_ctx* ctx, double* mbeta, double* r2, const double* norms);
  *       "hop.occupancy" -- its register budget: 0 (<= 2 waves per SIMD, the two hops of a direction in
  *       flight), 1 (one wave per SIMD, four hops in flight), 2 (<= 2 waves per SIMD with four hops in
  *       flight; FP32 links, else as 0), 4 (one hop at a time: at least 4 waves per SIMD for FP32 links, the
- *       compiler's choice for FP16, else as 0), -1 the format's default (1 for FP64 links, 0 for FP32, 4 for
+ *       compiler's choice for FP16, else as 0), -1 the format's default (1 for FP64 links, 2 for FP32, 0 for
  *       FP16);
  *       "mrhs.rb" -- right-hand sides per thread of the multi-right-hand-side kernel (0: by format, 1, 2, 4);
  *       "mrhs.lds" -- with one right-hand side per thread, 1 (default): a block computes each site's neighbour
