@@ -99,6 +99,17 @@ __device__ __forceinline__ Cx<float> weight_n(const double (&w)[2], float n)
     return Cx<float>{float(w[0]) * n, float(w[1]) * n};
 }
 
+/* Re <a, b> over one site's 12 components, in float by fused multiply-adds in component order: the per-site
+ * terms of the hopping kernels' norm partials, which are summed over sites in FP64 (the float sum of 24 terms
+ * is exact to a few float ulps, far below the FP16 values' own rounding) */
+__device__ __forceinline__ float site_redot(const Cx<float> (&a)[12], const Cx<float> (&b)[12])
+{
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s = __builtin_fmaf(a[k].im, b[k].im, __builtin_fmaf(a[k].re, b[k].re, s));
+    return s;
+}
+
 /* quantise one site; NaN and Inf stay visible as NaN after the round trip.  STREAM: non-temporal stores
  * (results of a hopping kernel, which no later work of the launch reads) */
 template<bool STREAM = false>
@@ -321,17 +332,13 @@ void k_dslash_eo_h(DslashArgs a)
             load_site_h<true>(a.cg_r, S, i, ro);
             const Cx<float> al{float(a.cg_alpha[0]), float(a.cg_alpha[1])};
 #pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                rn[k] = ro[k] - cmul(acc[k], al);
-                ry += (double)rn[k].re * ro[k].re + (double)rn[k].im * ro[k].im;
-            }
+            for (int k = 0; k < 12; ++k) rn[k] = ro[k] - cmul(acc[k], al);
             store_site_h<true>(a.cg_r, S, i, rn);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) nrm += (double)rn[k].re * rn[k].re + (double)rn[k].im * rn[k].im;
+            ry  = (double)site_redot(rn, ro);
+            nrm = (double)site_redot(rn, rn);
         } else {
             store_site_h<true>(a.out, S, i, acc);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) nrm += (double)acc[k].re * acc[k].re + (double)acc[k].im * acc[k].im;
+            if (a.norm_partials) nrm = (double)site_redot(acc, acc);
         }
     }
     if (a.norm_partials) {
@@ -691,17 +698,13 @@ __global__ __launch_bounds__(MrhsH<NRHS>::BLOCK) void k_dslash_eo_mrhs_h(DslashA
             load_site_h<true>(a.cg_r, SN, o, ro);
             const Cx<float> al{float(a.cg_alpha[2 * rhs]), float(a.cg_alpha[2 * rhs + 1])};
 #pragma unroll
-            for (int k = 0; k < 12; ++k) {
-                rn[k] = ro[k] - cmul(acc[k], al);
-                ry += (double)rn[k].re * ro[k].re + (double)rn[k].im * ro[k].im;
-            }
+            for (int k = 0; k < 12; ++k) rn[k] = ro[k] - cmul(acc[k], al);
             store_site_h<true>(a.cg_r, SN, o, rn);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) nrm += (double)rn[k].re * rn[k].re + (double)rn[k].im * rn[k].im;
+            ry  = (double)site_redot(rn, ro);
+            nrm = (double)site_redot(rn, rn);
         } else {
             store_site_h<true>(a.out, SN, o, acc);
-#pragma unroll
-            for (int k = 0; k < 12; ++k) nrm += (double)acc[k].re * acc[k].re + (double)acc[k].im * acc[k].im;
+            if (a.norm_partials) nrm = (double)site_redot(acc, acc);
         }
     }
     if (a.norm_partials) {

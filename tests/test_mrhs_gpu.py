@@ -11,7 +11,7 @@ import pytest
 
 import oracle as O
 from cl2qcd_amd import abi
-from fields import random_su3_field, site_rel_err
+from fields import half_links, random_su3_field, site_rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -66,12 +66,16 @@ def test_dslash_mrhs_matches_single(nrhs, fmt, lds):
                 assert site_rel_err(block_slot(ctx, out, prec, nrhs, nrhs - 1), ref) < 1e-12
 
 
+@pytest.mark.parametrize("phases", [dict(theta_t=1.0), dict(theta_t=1.0, theta_s=0.3, cp_re=0.11)],
+                         ids=["real-spatial-weights", "complex-weights"])
 @pytest.mark.parametrize("nrhs", [2, 12, 16])
-def test_dslash_mrhs_f16_matches_single(nrhs):
+def test_dslash_mrhs_f16_matches_single(nrhs, phases):
     """FP16 blocks (one FP16 field of Veo nrhs sites with per-site norms) and FP16 links: slot r of the
-    multi-RHS result equals the single-field FP16 kernel's result for source r bit for bit, both parities."""
-    lat = O.lattice(8, 8, theta_t=1.0)
-    with abi.Context(8, 8, theta_t=1.0) as ctx:
+    multi-RHS result equals the single-field FP16 kernel's result for source r bit for bit, both parities,
+    with real spatial hop weights and with a spatial boundary phase and chemical potential (complex weights
+    in every direction)."""
+    lat = O.lattice(8, 8, **phases)
+    with abi.Context(8, 8, **phases) as ctx:
         g = random_su3_field(4 * O.volume(lat), 5)
         gf = ctx.gauge(g, abi.GAUGE_F16_12)
         rng = np.random.default_rng(nrhs)
@@ -85,6 +89,9 @@ def test_dslash_mrhs_f16_matches_single(nrhs):
                 a.upload(srcs[r])
                 ctx.dslash_eo(b.ptr, a.ptr, gf.ptr, abi.GAUGE_F16_12, evenodd, 0.16)
                 assert np.array_equal(b.download(), block_slot(ctx, out, abi.F16, nrhs, r))
+            # and the last slot against exact arithmetic on the stored values and links
+            ref = O.dslash_eo(lat, a.download(), half_links(g), evenodd, 0.16)
+            assert site_rel_err(block_slot(ctx, out, abi.F16, nrhs, nrhs - 1), ref) < 1e-3
 
 
 @pytest.mark.parametrize("action", [abi.ACTION_WILSON, abi.ACTION_TWISTEDMASS])
