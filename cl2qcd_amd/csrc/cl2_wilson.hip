@@ -175,10 +175,10 @@ __device__ __forceinline__ void hop_pair(Cx<R> (&acc)[12], const vec2<R>* __rest
             }                                                                                                     \
             store_spinor_stream<R>(r_, SS, o_, ov_);                                                              \
         } else {                                                                                                  \
-            _Pragma("unroll") for (int k = 0; k < 12; ++k) {                                                      \
-                ov_[k] = V{acc[k].re, acc[k].im};                                                                 \
-                NRM += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;             \
-            }                                                                                                     \
+            _Pragma("unroll") for (int k = 0; k < 12; ++k) ov_[k] = V{acc[k].re, acc[k].im};                      \
+            if (a.norm_partials)                                                                                  \
+                _Pragma("unroll") for (int k = 0; k < 12; ++k)                                                    \
+                    NRM += (double)acc[k].re * (double)acc[k].re + (double)acc[k].im * (double)acc[k].im;         \
             store_spinor_stream<R>(reinterpret_cast<V*>(a.out), SS, o_, ov_);                                     \
         }                                                                                                         \
     }
