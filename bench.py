@@ -671,8 +671,8 @@ def main():
     global USE_GRAPH
     USE_GRAPH = not args.no_graph
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU design is "
-              "is designed but not built)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the multi-GPU path is "
+              "designed but not built)", file=sys.stderr)
         sys.exit(2)
 
     from cl2qcd_amd import abi
