@@ -1589,6 +1589,28 @@ int cl2_update_alpha_cgm(cl2_ctx* c, double* out, const double* salpha_pres, con
     return CL2_OK;
 }
 
+int cl2_cgm_beta_step(cl2_ctx* c, double* beta, double* beta_prev, const double* rr, const double* pap)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, beta && beta_prev && rr && pap, "bad argument");
+    CL2_HIP(c, launch_cgm_beta_step(c, beta, beta_prev, rr, pap));
+    return CL2_OK;
+}
+
+int cl2_cgm_alpha_step(cl2_ctx* c, int numeq, double* alpha, double* alpha_prev, const double* rr_new,
+                       double* rr_old, const double* beta, const double* beta_prev, const double* sigma, double* zeta,
+                       double* zeta_prev, double* zeta_foll, double* beta_vec, double* alpha_vec)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, numeq > 0 && numeq <= 64, "cl2_cgm_alpha_step: 1 to 64 shifts");
+    CL2_REQUIRE(c, alpha && alpha_prev && rr_new && rr_old && beta && beta_prev && sigma && zeta && zeta_prev &&
+                       zeta_foll && beta_vec && alpha_vec,
+                "bad argument");
+    CL2_HIP(c, launch_cgm_alpha_step(c, numeq, alpha, alpha_prev, rr_new, rr_old, beta, beta_prev, sigma, zeta,
+                                     zeta_prev, zeta_foll, beta_vec, alpha_vec));
+    return CL2_OK;
+}
+
 /* ---------------- captured launch sequences ---------------- */
 
 }  // extern "C"

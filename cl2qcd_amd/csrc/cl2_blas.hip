@@ -497,24 +497,99 @@ hipError_t launch_vec_element(cl2_ctx* c, double* out, const double* in, int out
     return hipGetLastError();
 }
 
-/* Multi-shift CG recurrences, one thread per shift (ocl_kernel/operations_real.cl:20-39):
+/* Multi-shift CG recurrences (ocl_kernel/operations_real.cl:20-39), without contraction so that the separate
+ * launches below and the fused CG-M step compute the same bits:
  *   zeta:  (a b c) / (d e (b - a) + b c (1 - f d))
  *   beta:  -a b / c
- *   alpha: -a b c / (d e)
- * Pointer roles follow the launcher argument lists of real.cpp:285-400. */
+ *   alpha: -a b c / (d e) */
+__device__ __forceinline__ double cgm_zeta(double A, double B, double C, double D, double E, double F)
+{
+#pragma clang fp contract(off)
+    return (A * B * C) / (D * E * (B - A) + B * C * (1 - F * D));
+}
+__device__ __forceinline__ double cgm_beta(double a, double b, double c)
+{
+#pragma clang fp contract(off)
+    return -a * b / c;
+}
+__device__ __forceinline__ double cgm_alpha(double a, double b, double c, double d, double e)
+{
+#pragma clang fp contract(off)
+    return -a * b * c / (d * e);
+}
+
+/* One thread per shift; pointer roles follow the launcher argument lists of real.cpp:285-400. */
 __global__ void k_cgm(int which, double* out, const double* a, const double* b, const double* d, const double* e,
                       const double* f, const double* g, int numeq)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= numeq) return;
-    if (which == 0) {  // zeta(zeta_prev, zeta_prev_prev, *sbeta_prev, *sbeta_pres, *salpha_prev, sigma)
-        double A = a[k], B = b[k], C = d[0], D = e[0], E = f[0], F = g[k];
-        out[k]   = (A * B * C) / (D * E * (B - A) + B * C * (1 - F * D));
-    } else if (which == 1) {  // beta(*sbeta_pres, zeta_pres, zeta_prev)
-        out[k] = -a[0] * b[k] / d[k];
-    } else {  // alpha(*salpha_pres, zeta_pres, beta_pres, zeta_prev, *sbeta_pres)
-        out[k] = -a[0] * b[k] * d[k] / (e[k] * f[0]);
+    if (which == 0)  // zeta(zeta_prev, zeta_prev_prev, *sbeta_prev, *sbeta_pres, *salpha_prev, sigma)
+        out[k] = cgm_zeta(a[k], b[k], d[0], e[0], f[0], g[k]);
+    else if (which == 1)  // beta(*sbeta_pres, zeta_pres, zeta_prev)
+        out[k] = cgm_beta(a[0], b[k], d[k]);
+    else  // alpha(*salpha_pres, zeta_pres, beta_pres, zeta_prev, *sbeta_pres)
+        out[k] = cgm_alpha(a[0], b[k], d[k], e[k], f[0]);
+}
+
+/* The scalar work of one CG-M iteration (solver_shifted.cpp:136-198) in two launches instead of thirteen:
+ * beta step:  beta_prev = beta; beta = 0 - rr / pap;
+ * alpha step: alpha_prev = alpha; alpha = rr_new / rr_old; per shift zeta_foll, beta_vec, alpha_vec from
+ *             (zeta, zeta_prev, beta_prev, beta, alpha_prev, sigma); then zeta_prev = zeta, zeta = zeta_foll and
+ *             rr_old = rr_new.  Every value is the one the separate launches give (same operations, same
+ *             order).  One block of 64 threads (numeq <= 64): all reads happen before the barrier, all
+ *             writes of shared scalars after it. */
+__global__ void k_cgm_beta_step(double* beta, double* beta_prev, const double* rr, const double* pap)
+{
+    if (threadIdx.x != 0) return;
+    const double b = beta[0];
+    beta_prev[0]   = b;
+    beta[0]        = 0.0 - rr[0] / pap[0];
+}
+
+__global__ __launch_bounds__(64) void k_cgm_alpha_step(int numeq, double* alpha, double* alpha_prev,
+                                                       const double* rr_new, double* rr_old, const double* beta,
+                                                       const double* beta_prev, const double* sigma, double* zeta,
+                                                       double* zeta_prev, double* zeta_foll, double* beta_vec,
+                                                       double* alpha_vec)
+{
+    const int k        = threadIdx.x;
+    const double a_old = alpha[0], rn = rr_new[0];
+    const double a_new = rn / rr_old[0];
+    const double b = beta[0], bp = beta_prev[0];
+    double z = 0., zp = 0., s = 0.;
+    if (k < numeq) z = zeta[k], zp = zeta_prev[k], s = sigma[k];
+    __syncthreads();
+    if (k < numeq) {
+        const double zf = cgm_zeta(z, zp, bp, b, a_old, s);
+        const double bv = cgm_beta(b, zf, z);
+        zeta_foll[k]    = zf;
+        beta_vec[k]     = bv;
+        alpha_vec[k]    = cgm_alpha(a_new, zf, bv, z, b);
+        zeta_prev[k]    = z;
+        zeta[k]         = zf;
     }
+    if (k == 0) {
+        alpha_prev[0] = a_old;
+        alpha[0]      = a_new;
+        rr_old[0]     = rn;
+    }
+}
+
+hipError_t launch_cgm_beta_step(cl2_ctx* c, double* beta, double* beta_prev, const double* rr, const double* pap)
+{
+    hipLaunchKernelGGL(k_cgm_beta_step, dim3(1), dim3(64), 0, c->stream, beta, beta_prev, rr, pap);
+    return hipGetLastError();
+}
+
+hipError_t launch_cgm_alpha_step(cl2_ctx* c, int numeq, double* alpha, double* alpha_prev, const double* rr_new,
+                                 double* rr_old, const double* beta, const double* beta_prev, const double* sigma,
+                                 double* zeta, double* zeta_prev, double* zeta_foll, double* beta_vec,
+                                 double* alpha_vec)
+{
+    hipLaunchKernelGGL(k_cgm_alpha_step, dim3(1), dim3(64), 0, c->stream, numeq, alpha, alpha_prev, rr_new, rr_old,
+                       beta, beta_prev, sigma, zeta, zeta_prev, zeta_foll, beta_vec, alpha_vec);
+    return hipGetLastError();
 }
 
 hipError_t launch_cgm_update(cl2_ctx* c, int which, double* out, const double* a, const double* b, const double* d,

@@ -181,5 +181,10 @@ hipError_t launch_scalar_op(cl2_ctx* c, int op, int is_complex, double* out, con
 hipError_t launch_vec_element(cl2_ctx* c, double* out, const double* in, int out_index, int in_index);
 hipError_t launch_cgm_update(cl2_ctx* c, int which, double* out, const double* a, const double* b, const double* d,
                              const double* e, const double* f, const double* g, int numeq);
+hipError_t launch_cgm_beta_step(cl2_ctx* c, double* beta, double* beta_prev, const double* rr, const double* pap);
+hipError_t launch_cgm_alpha_step(cl2_ctx* c, int numeq, double* alpha, double* alpha_prev, const double* rr_new,
+                                 double* rr_old, const double* beta, const double* beta_prev, const double* sigma,
+                                 double* zeta, double* zeta_prev, double* zeta_foll, double* beta_vec,
+                                 double* alpha_vec);
 
 }  // namespace cl2

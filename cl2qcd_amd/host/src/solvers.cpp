@@ -1110,8 +1110,7 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
     const Vector<hmc_float> zeta_prev(n, system), zeta(n, system), zeta_foll(n, system), alpha_vec(n, system),
         beta_vec(n, system), shift(n, system), active(n, system);
     const Scalar<hmc_float> alpha_scalar_prev(system), alpha_scalar(system), beta_scalar_prev(system),
-        beta_scalar(system), zero(system), tmp1(system), tmp2(system), tmp3(system);
-    zero.store(0.0);
+        beta_scalar(system), tmp1(system), tmp2(system), tmp3(system);
     zeta_prev.store(std::vector<hmc_float>(n, 1.));
     zeta.store(std::vector<hmc_float>(n, 1.));
     alpha_vec.store(std::vector<hmc_float>(n, 0.));
@@ -1138,26 +1137,22 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
         A(&v, gf, p, &ap);  // reduced-precision links and MdagM temporary
         system.synchronize();
     }
-    // one CG-M iteration (solver_shifted.cpp:136-198), every scalar on the device
+    // one CG-M iteration (solver_shifted.cpp:136-198), every scalar on the device: the reference's copies,
+    // ratio, subtraction and zeta/beta/alpha updates run as two fused launches with the same values
+    // (beta_prev = beta, beta = -tmp1/tmp3; then alpha_prev = alpha, alpha = tmp2/tmp1, the per-shift
+    // recurrences, zeta_prev = zeta, zeta = zeta_foll, tmp1 = tmp2)
     auto body = [&]() {
-        copyData(&beta_scalar_prev, beta_scalar);
         A(&v, gf, p, &ap);
         scalar_product_real_part(&tmp3, p, v);
-        divide(&beta_scalar, tmp1, tmp3);
-        subtract(&beta_scalar, zero, beta_scalar);
+        system.check(cl2_cgm_beta_step(c, beta_scalar.dev(), beta_scalar_prev.dev(), tmp1.dev(), tmp3.dev()));
         saxpy(&r, beta_scalar, v, r);  // r = beta v + r
         scalar_product_real_part(&tmp2, r, r);
-        copyData(&alpha_scalar_prev, alpha_scalar);
-        divide(&alpha_scalar, tmp2, tmp1);
+        system.check(cl2_cgm_alpha_step(c, n, alpha_scalar.dev(), alpha_scalar_prev.dev(), tmp2.dev(), tmp1.dev(),
+                                        beta_scalar.dev(), beta_scalar_prev.dev(), shift.dev(), zeta.dev(),
+                                        zeta_prev.dev(), zeta_foll.dev(), beta_vec.dev(), alpha_vec.dev()));
         saxpy(&p, alpha_scalar, p, r);  // p = alpha p + r
-        update_zeta_cgm(&zeta_foll, zeta, zeta_prev, beta_scalar_prev, beta_scalar, alpha_scalar_prev, shift, n);
-        update_beta_cgm(&beta_vec, beta_scalar, zeta_foll, zeta, n);
-        update_alpha_cgm(&alpha_vec, alpha_scalar, zeta_foll, beta_vec, zeta, beta_scalar, n);
         system.check(cl2_multishift_update_stagg_eo(c, xp.data(), CL2_F64, pp.data(), r.dev(), lo, n, beta_vec.dev(),
                                                     zeta_foll.dev(), alpha_vec.dev(), active.dev()));
-        copyData(&zeta_prev, zeta);
-        copyData(&zeta, zeta_foll);
-        copyData(&tmp1, tmp2);
     };
     int graph_nodes = 0, graph_kernel_nodes = 0;  // census of the captured iteration
     struct Graph {

@@ -396,6 +396,16 @@ int cl2_update_beta_cgm(cl2_ctx* ctx, double* out, const double* sbeta_pres, con
                         const double* zeta_prev, int numeq);
 int cl2_update_alpha_cgm(cl2_ctx* ctx, double* out, const double* salpha_pres, const double* zeta_pres,
                          const double* beta_pres, const double* zeta_prev, const double* sbeta_pres, int numeq);
+/* The scalar work of one CG-M iteration (solver_shifted.cpp:136-198) fused into two launches; no counterpart in
+ * the reference, which makes thirteen (copies, ratio, subtraction and the three updates above).  Every value
+ * equals what that sequence gives.  All arguments are device scalars or device vectors of numeq (1..64).
+ * beta step, after <p, A p>:  beta_prev = beta; beta = 0 - rr / pap.
+ * alpha step, after |r|^2:   alpha_prev = alpha; alpha = rr_new / rr_old; zeta_foll, beta_vec, alpha_vec as
+ *   update_{zeta,beta,alpha}_cgm; then zeta_prev = zeta, zeta = zeta_foll, rr_old = rr_new. */
+int cl2_cgm_beta_step(cl2_ctx* ctx, double* beta, double* beta_prev, const double* rr, const double* pap);
+int cl2_cgm_alpha_step(cl2_ctx* ctx, int numeq, double* alpha, double* alpha_prev, const double* rr_new,
+                       double* rr_old, const double* beta, const double* beta_prev, const double* sigma, double* zeta,
+                       double* zeta_prev, double* zeta_foll, double* beta_vec, double* alpha_vec);
 
 /* ==== captured launch sequences ====
  * No counterpart in the reference.  Everything enqueued on the context's stream between cl2_graph_begin and

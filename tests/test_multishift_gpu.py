@@ -118,3 +118,66 @@ def test_cg_m_mixed_graph_replays_match_eager():
         out.append((total, ms, [x.get_host() for x in xs]))
     for total, ms, xs in out[1:]:
         assert (total, ms) == out[0][:2] and all(np.array_equal(a, b) for a, b in zip(xs, out[0][2]))
+
+
+@pytest.mark.parametrize("n", [1, 5, 16, 64])
+def test_fused_cgm_scalar_steps_equal_reference_sequence(n):
+    """cl2_cgm_beta_step and cl2_cgm_alpha_step give, bit for bit, the values of the reference's per-iteration
+    scalar sequence (solver_shifted.cpp:136-198): copy, ratio and subtraction for beta; copy and ratio for
+    alpha; update_{zeta,beta,alpha}_cgm; and the copies zeta_prev = zeta, zeta = zeta_foll, rr_old = rr_new.
+    Three iterations in a row, so every updated value feeds the next step."""
+    rng = np.random.default_rng(n)
+    # layout: 6 vectors of n (zeta, zeta_prev, zeta_foll, beta_vec, alpha_vec, sigma), then scalars
+    V = ["zeta", "zeta_prev", "zeta_foll", "beta_vec", "alpha_vec", "sigma"]
+    S = ["beta", "beta_prev", "alpha", "alpha_prev", "rr_old", "rr_new", "pap", "zero"]
+    init = np.concatenate([rng.uniform(0.5, 1.0, n), rng.uniform(0.5, 1.0, n), np.zeros(3 * n),
+                           np.sort(rng.uniform(1e-4, 1.0, n)), [-0.9, 0.0, 0.3, 0.0, 2.0, 0.0, 0.0, 0.0]])
+    with abi.Context(4, 4) as ctx:
+        fused, ref = ctx.scalars(len(init)), ctx.scalars(len(init))
+        fused.store(init)
+        ref.store(init)
+
+        def at(d, name):
+            return d.at(V.index(name) * n) if name in V else d.at(6 * n + S.index(name))
+
+        for it in range(3):
+            rr_new, pap = 1.1 * 0.8 ** it, 1.7 + 0.1 * it
+            for d in (fused, ref):  # the two reductions of the iteration write their results
+                vals = d.get()
+                vals[6 * n + S.index("rr_new")] = rr_new
+                vals[6 * n + S.index("pap")] = pap
+                d.store(vals)
+            ctx.cgm_beta_step(at(fused, "beta"), at(fused, "beta_prev"), at(fused, "rr_old"), at(fused, "pap"))
+            ctx.cgm_alpha_step(n, at(fused, "alpha"), at(fused, "alpha_prev"), at(fused, "rr_new"),
+                               at(fused, "rr_old"), at(fused, "beta"), at(fused, "beta_prev"), at(fused, "sigma"),
+                               at(fused, "zeta"), at(fused, "zeta_prev"), at(fused, "zeta_foll"),
+                               at(fused, "beta_vec"), at(fused, "alpha_vec"))
+            r = lambda name: at(ref, name)  # noqa: E731
+            ctx.scalar_copy(r("beta_prev"), r("beta"), 1)
+            ctx.real_op(abi.SCALAR_OP_RATIO, r("beta"), r("rr_old"), r("pap"))
+            ctx.real_op(abi.SCALAR_OP_SUBTRACTION, r("beta"), r("zero"), r("beta"))
+            ctx.scalar_copy(r("alpha_prev"), r("alpha"), 1)
+            ctx.real_op(abi.SCALAR_OP_RATIO, r("alpha"), r("rr_new"), r("rr_old"))
+            ctx.update_zeta_cgm(r("zeta_foll"), r("zeta"), r("zeta_prev"), r("beta_prev"), r("beta"),
+                                r("alpha_prev"), r("sigma"), n)
+            ctx.update_beta_cgm(r("beta_vec"), r("beta"), r("zeta_foll"), r("zeta"), n)
+            ctx.update_alpha_cgm(r("alpha_vec"), r("alpha"), r("zeta_foll"), r("beta_vec"), r("zeta"), r("beta"), n)
+            ctx.scalar_copy(r("zeta_prev"), r("zeta"), n)
+            ctx.scalar_copy(r("zeta"), r("zeta_foll"), n)
+            ctx.scalar_copy(r("rr_old"), r("rr_new"), 1)
+            a, b = fused.get(), ref.get()
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), (it, np.flatnonzero(a != b))
+        # the same three steps in numpy from the closed forms (operations_real.cl:20-39)
+        z, zp, sig = init[:n].copy(), init[n:2 * n].copy(), init[5 * n:6 * n]
+        beta, alpha, rr_old = -0.9, 0.3, 2.0
+        for it in range(3):
+            rr_new, pap = 1.1 * 0.8 ** it, 1.7 + 0.1 * it
+            beta_prev, beta = beta, -(rr_old / pap)
+            alpha_prev, alpha = alpha, rr_new / rr_old
+            zf = (z * zp * beta_prev) / (beta * alpha_prev * (zp - z) + zp * beta_prev * (1 - sig * beta))
+            bv = -beta * zf / z
+            av = -alpha * zf * bv / (z * beta)
+            zp, z, rr_old = z, zf, rr_new
+        got = a[:6 * n].reshape(6, n)
+        for want, row in ((z, 0), (zp, 1), (zf, 2), (bv, 3), (av, 4)):
+            assert np.abs(got[row] - want).max() <= 1e-13 * np.abs(want).max()
