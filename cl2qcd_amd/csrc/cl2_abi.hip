@@ -1367,6 +1367,30 @@ int cl2_MdagM_eo_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, 
     return CL2_OK;
 }
 
+int cl2_MdagM_eo_mrhs_cg_step(cl2_ctx* c, void* r, const void* p, const void* gauge, int fmt, double mass,
+                              int upper_left, const double* sigma, void* tmp, int nrhs, const double* r2,
+                              const double* active, double* pAp, double* alpha, double* norms)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, supported_nrhs(nrhs) && nrhs > 1, "unsupported number of right-hand sides (2, 4, 8, 12 or 16)");
+    CL2_REQUIRE(c, upper_left == CL2_EVEN || upper_left == CL2_ODD, "upper_left must be CL2_EVEN or CL2_ODD");
+    CL2_REQUIRE(c, r && p && gauge && tmp && r2 && active && pAp && alpha && norms, "null pointer");
+    CL2_REQUIRE(c, r != p && tmp != p && tmp != r, "r, p and tmp must be different blocks");
+    double coef[16];
+    for (int k = 0; k < nrhs; ++k) coef[k] = mass * mass + (sigma ? sigma[k] : 0.);
+    const int first = (upper_left == CL2_EVEN) ? CL2_ODD : CL2_EVEN;
+    double2* parts  = reinterpret_cast<double2*>(c->partials);
+    unsigned nb     = 0;
+    // w = D_KS p with per-right-hand-side (|w|^2, |p|^2); pAp and alpha on the device; then r -= alpha (coef p - D w)
+    CL2_HIP(c, launch_dks_mrhs(c, tmp, p, gauge, fmt, first, nrhs, nullptr, nullptr, parts, p, nullptr, nullptr, &nb));
+    CL2_HIP(c, launch_block_reduce2(c, parts, nb, pAp, 0, nrhs));
+    CL2_HIP(c, launch_block_cg_alpha_mdagm(c, alpha, r2, pAp, coef, active, nrhs));
+    CL2_HIP(c, launch_dks_mrhs(c, nullptr, tmp, gauge, fmt, upper_left, nrhs, p, coef, parts, nullptr, r, alpha, &nb));
+    CL2_HIP(c, launch_block_reduce2(c, parts, nb, norms, 0, nrhs));
+    return CL2_OK;
+}
+
 int cl2_block_cg_alpha(cl2_ctx* c, double* alpha, const double* r2, const double* pAp, const double* active,
                        int nrhs)
 {

@@ -279,6 +279,35 @@ __global__ void k_block_cg_alpha(double* alpha, const double* r2, const double* 
     alpha[2 * r + 1] = ai;
 }
 
+/* the staggered fused CG step: pAp holds (|D p_r|^2, |p_r|^2) per right-hand side on entry and the complex
+ * <p_r, (m^2 + sigma_r - D D) p_r> = coef_r |p_r|^2 + |D p_r|^2 (D_KS anti-hermitian) on exit; then alpha as
+ * k_block_cg_alpha */
+struct Coef16 {
+    double v[16];
+};
+__global__ void k_block_cg_alpha_mdagm(double* alpha, const double* r2, double* pAp, Coef16 coef, const double* active,
+                                       int nrhs)
+{
+    const int r = threadIdx.x;
+    if (r >= nrhs) return;
+    const double a = coef.v[r] * pAp[2 * r + 1] + pAp[2 * r];
+    pAp[2 * r]     = a;
+    pAp[2 * r + 1] = 0.;
+    double ar = 0.;
+    if (active[r] != 0.) ar = r2[r] * a / (a * a);
+    alpha[2 * r]     = ar;
+    alpha[2 * r + 1] = 0.;
+}
+
+hipError_t launch_block_cg_alpha_mdagm(cl2_ctx* c, double* alpha, const double* r2, double* pAp, const double* coef,
+                                       const double* active, int nrhs)
+{
+    Coef16 k{};
+    for (int r = 0; r < nrhs && r < 16; ++r) k.v[r] = coef[r];
+    hipLaunchKernelGGL(k_block_cg_alpha_mdagm, dim3(1), dim3(64), 0, c->stream, alpha, r2, pAp, k, active, nrhs);
+    return hipGetLastError();
+}
+
 /* beta_r = Polak-Ribiere <r_new, r_new - r_old> / |r_old|^2 (Fletcher-Reeves when that is negative) for
  * active right-hand sides, else 0; then r2_r = |r_new|^2.  norms = (|r_new|^2, Re <r_new, r_old>) per r. */
 __global__ void k_block_cg_beta(double* beta, double* r2, const double* norms, const double* active, int nrhs)

@@ -85,7 +85,9 @@ hipError_t launch_sitediag(cl2_ctx* c, void* out, const void* in, int prec, cons
 hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd);
 /* nrhs = 2, 4, 8, 12, 16 staggered vectors [3][Veo][nrhs]; with x: out_r = coef[r] x_r - D in_r */
 hipError_t launch_dks_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd, int nrhs,
-                           const void* x, const double* coef);
+                           const void* x, const double* coef, double2* norm_partials = nullptr,
+                           const void* own = nullptr, void* cg_r = nullptr, const double* cg_alpha = nullptr,
+                           unsigned* nblocks = nullptr);
 
 /* elementwise complex BLAS over n complex numbers */
 enum BlasMode : int {
@@ -171,6 +173,8 @@ hipError_t launch_scalar_copy(cl2_ctx* c, double* dst, const double* src, int n)
 hipError_t launch_cg_beta(cl2_ctx* c, double* mbeta, double* r2, const double* norms);
 hipError_t launch_block_cg_scalars(cl2_ctx* c, int which, double* out, double* r2, const double* in,
                                    const double* active, int nrhs);
+hipError_t launch_block_cg_alpha_mdagm(cl2_ctx* c, double* alpha, const double* r2, double* pAp, const double* coef,
+                                       const double* active, int nrhs);
 
 /* fused multi-shift CG vector update over n complex elements */
 hipError_t launch_multishift(cl2_ctx* c, void* const* xs_dev, int x_prec, void* const* ps_dev, const void* r, int prec,

@@ -20,6 +20,14 @@ struct DksArgs {
     // MdagM_eo with a shift per right-hand side)
     const void* x;
     double coef[16];
+    // the fused CG step of the lockstep solver (multi-RHS only): norm_partials[r * gridDim.x + block] gets per
+    // right-hand side (|out_r|^2, |own_r|^2) — own is summed at the thread's eo index, so every site of its
+    // field once — or, with cg_r set, r_r <- r_r - cg_alpha[r] out_r is stored instead of out (never written)
+    // and the partials are (|r_new|^2, Re <r_new, r_old>)
+    double2* norm_partials = nullptr;
+    const void* own        = nullptr;
+    void* cg_r             = nullptr;
+    const double* cg_alpha = nullptr;
 };
 
 /* forward and backward hop of one direction on neighbour vectors already in registers:
@@ -148,6 +156,7 @@ __global__ __launch_bounds__(DksMrhs<NRHS>::BLOCK) void k_dks_eo_mrhs(DksArgs a)
     __shared__ Su3<R> lu[SB * 8];
     __shared__ uint32_t lnb[SB * 8];
     __shared__ uint32_t lsite[SB];
+    __shared__ double2 lred[BLOCK];
     const uint32_t S  = a.g.veo;
     const size_t SS   = (size_t)S * NRHS;
     const uint32_t Lb = logical_block(a.map, blockIdx.x);
@@ -181,39 +190,81 @@ __global__ __launch_bounds__(DksMrhs<NRHS>::BLOCK) void k_dks_eo_mrhs(DksArgs a)
     const int s      = threadIdx.x / NRHS;
     const int rhs    = threadIdx.x - s * NRHS;
     const uint32_t i = lsite[s];
-    if (i == NONE) return;
-    uint32_t x, y, z, t;
-    site_coords(a.g, i, P, x, y, z, t);
-    const R eta[4] = {R(((x + y + z) & 1u) ? -1 : 1), R(1), R((x & 1u) ? -1 : 1), R(((x + y) & 1u) ? -1 : 1)};
-    const V* in    = reinterpret_cast<const V*>(a.in);
-    Cx<R> acc[3];
+    double nrm = 0., ry = 0.;
+    if (i != NONE) {
+        uint32_t x, y, z, t;
+        site_coords(a.g, i, P, x, y, z, t);
+        const R eta[4] = {R(((x + y + z) & 1u) ? -1 : 1), R(1), R((x & 1u) ? -1 : 1), R(((x + y) & 1u) ? -1 : 1)};
+        const V* in    = reinterpret_cast<const V*>(a.in);
+        Cx<R> acc[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) acc[k] = Cx<R>{R(0), R(0)};
+        for (int k = 0; k < 3; ++k) acc[k] = Cx<R>{R(0), R(0)};
 #pragma unroll
-    for (int mu = 0; mu < 4; ++mu) {
-        V vf[3], vb[3];
-        const size_t ef = (size_t)lnb[s * 8 + 2 * mu] * NRHS + rhs, eb = (size_t)lnb[s * 8 + 2 * mu + 1] * NRHS + rhs;
+        for (int mu = 0; mu < 4; ++mu) {
+            V vf[3], vb[3];
+            const size_t ef = (size_t)lnb[s * 8 + 2 * mu] * NRHS + rhs;
+            const size_t eb = (size_t)lnb[s * 8 + 2 * mu + 1] * NRHS + rhs;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) vf[c] = in[(size_t)c * SS + ef];
+            for (int c = 0; c < 3; ++c) vf[c] = in[(size_t)c * SS + ef];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) vb[c] = in[(size_t)c * SS + eb];
-        dks_hops(acc, lu[s * 8 + 2 * mu], vf, lu[s * 8 + 2 * mu + 1], vb,
-                 Cx<R>{R(a.sf[mu][0]) * eta[mu], R(a.sf[mu][1]) * eta[mu]});
-    }
-    const size_t o = (size_t)i * NRHS + rhs;
-    if (a.x) {  // the MdagM_eo epilogue as the elementwise kernel computes it: coef x - out
-        const V* xv     = reinterpret_cast<const V*>(a.x);
-        const Cx<R> cf  = Cx<R>{R(a.coef[rhs]), R(0)};
-        const Cx<R> mone = Cx<R>{R(-1), R(0)};
+            for (int c = 0; c < 3; ++c) vb[c] = in[(size_t)c * SS + eb];
+            dks_hops(acc, lu[s * 8 + 2 * mu], vf, lu[s * 8 + 2 * mu + 1], vb,
+                     Cx<R>{R(a.sf[mu][0]) * eta[mu], R(a.sf[mu][1]) * eta[mu]});
+        }
+        const size_t o = (size_t)i * NRHS + rhs;
+        if (a.x) {  // the MdagM_eo epilogue as the elementwise kernel computes it: coef x - out
+            const V* xv      = reinterpret_cast<const V*>(a.x);
+            const Cx<R> cf   = Cx<R>{R(a.coef[rhs]), R(0)};
+            const Cx<R> mone = Cx<R>{R(-1), R(0)};
 #pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const V v = xv[(size_t)c * SS + o];
-            acc[c]    = cmul(Cx<R>{v.x, v.y}, cf) + cmul(acc[c], mone);
+            for (int c = 0; c < 3; ++c) {
+                const V v = xv[(size_t)c * SS + o];
+                acc[c]    = cmul(Cx<R>{v.x, v.y}, cf) + cmul(acc[c], mone);
+            }
+        }
+        if (a.cg_r) {  // r <- r - alpha A p, A p never stored
+            V* rp            = reinterpret_cast<V*>(a.cg_r);
+            const Cx<R> al   = Cx<R>{R(a.cg_alpha[2 * rhs]), R(a.cg_alpha[2 * rhs + 1])};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const V ro     = rp[(size_t)c * SS + o];
+                const Cx<R> rn = Cx<R>{ro.x, ro.y} - cmul(acc[c], al);
+                store_stream(rp + (size_t)c * SS + o, V{rn.re, rn.im});
+                nrm += (double)rn.re * (double)rn.re + (double)rn.im * (double)rn.im;
+                ry += (double)rn.re * (double)ro.x + (double)rn.im * (double)ro.y;
+            }
+        } else {
+            V* out = reinterpret_cast<V*>(a.out);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) store_stream(out + (size_t)c * SS + o, V{acc[c].re, acc[c].im});
+            if (a.norm_partials) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    nrm += (double)acc[c].re * (double)acc[c].re + (double)acc[c].im * (double)acc[c].im;
+                if (a.own) {
+                    const V* ov = reinterpret_cast<const V*>(a.own);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const V v = ov[(size_t)c * SS + o];
+                        ry += (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+                    }
+                }
+            }
         }
     }
-    V* out = reinterpret_cast<V*>(a.out);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) store_stream(out + (size_t)c * SS + o, V{acc[c].re, acc[c].im});
+    if (a.norm_partials) {
+        // per right-hand side: thread r < NRHS sums the SB site threads of its column in site order
+        lred[threadIdx.x] = make_double2(nrm, ry);
+        __syncthreads();
+        if ((int)threadIdx.x < NRHS) {
+            double2 tsum = make_double2(0., 0.);
+            for (int q = 0; q < SB; ++q) {
+                tsum.x += lred[q * NRHS + threadIdx.x].x;
+                tsum.y += lred[q * NRHS + threadIdx.x].y;
+            }
+            a.norm_partials[(size_t)threadIdx.x * gridDim.x + blockIdx.x] = tsum;
+        }
+    }
 }
 
 static void dks_weights(cl2_ctx* c, DksArgs& a)
@@ -233,15 +284,20 @@ static void dks_weights(cl2_ctx* c, DksArgs& a)
 }
 
 hipError_t launch_dks_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd, int nrhs,
-                           const void* x, const double* coef)
+                           const void* x, const double* coef, double2* norm_partials, const void* own, void* cg_r,
+                           const double* cg_alpha, unsigned* nblocks)
 {
     DksArgs a;
-    a.in         = in;
-    a.out        = out;
-    a.gauge      = gauge;
-    a.g          = c->g;
-    a.out_parity = evenodd;
-    a.x          = x;
+    a.in            = in;
+    a.out           = out;
+    a.gauge         = gauge;
+    a.g             = c->g;
+    a.out_parity    = evenodd;
+    a.x             = x;
+    a.norm_partials = norm_partials;
+    a.own           = own;
+    a.cg_r          = cg_r;
+    a.cg_alpha      = cg_alpha;
     for (int r = 0; r < 16; ++r) a.coef[r] = (x && r < nrhs) ? coef[r] : 0.;
     dks_weights(c, a);
     const cl2_hop_tuning t = hop_tuning(c, format | DKS_TUNING_KEY, 1);
@@ -249,6 +305,7 @@ hipError_t launch_dks_mrhs(cl2_ctx* c, void* out, const void* in, const void* ga
         constexpr int F = decltype(fmt_tag)::value, N = decltype(n_tag)::value;
         constexpr int SB = DksMrhs<N>::SB;
         a.map = work_map(c, SB, 1, t);
+        if (nblocks) *nblocks = (c->g.veo + SB - 1) / SB;
         hipLaunchKernelGGL((k_dks_eo_mrhs<F, N>), dim3((c->g.veo + SB - 1) / SB), dim3(DksMrhs<N>::BLOCK), 0, c->stream,
                            a);
         return hipGetLastError();

@@ -201,6 +201,15 @@ class MdagM_eo final : public Fermionmatrix_stagg_eo {
 void apply_block(const MdagM_eo& A, const lattices::Staggeredfield_eo_block* out, const lattices::Gaugefield& gf,
                  const lattices::Staggeredfield_eo_block& in, const physics::AdditionalParameters& ap,
                  const std::vector<hmc_float>& sigma, const lattices::DeviceBuffer& tmp);
+/* The operator half of a lockstep CG step on (A + sigma[r]) (cl2_MdagM_eo_mrhs_cg_step): pAp, alpha,
+ * r_r -= alpha_r (A + sigma_r) p_r and the residual norms, without storing A p.  No counterpart in the
+ * reference; returns true (the staggered operator always has it). */
+bool cg_step_block(const MdagM_eo& A, const lattices::Staggeredfield_eo_block* r, const lattices::Gaugefield& gf,
+                   const lattices::Staggeredfield_eo_block& p, const physics::AdditionalParameters& ap,
+                   const std::vector<hmc_float>& sigma, const lattices::DeviceBuffer& tmp,
+                   const lattices::Vector<hmc_float>& r2, const lattices::Vector<hmc_float>& active,
+                   const lattices::Vector<hmc_float>& pAp, const lattices::Vector<hmc_float>& alpha,
+                   const lattices::Vector<hmc_float>& norms);
 
 }  // namespace fermionmatrix
 }  // namespace physics

@@ -320,6 +320,26 @@ void apply_block(const MdagM_eo& A, const lattices::Staggeredfield_eo_block* out
                                             tmp.ptr(), in.nrhs()));
 }
 
+bool cg_step_block(const MdagM_eo& A, const lattices::Staggeredfield_eo_block* r, const Gaugefield& gf,
+                   const lattices::Staggeredfield_eo_block& p, const physics::AdditionalParameters& ap,
+                   const std::vector<hmc_float>& sigma, const lattices::DeviceBuffer& tmp,
+                   const lattices::Vector<hmc_float>& r2, const lattices::Vector<hmc_float>& active,
+                   const lattices::Vector<hmc_float>& pAp, const lattices::Vector<hmc_float>& alpha,
+                   const lattices::Vector<hmc_float>& norms)
+{
+    if (r->precision() != p.precision() || r->nrhs() != p.nrhs())
+        throw std::invalid_argument("cg_step_block: blocks of different shape");
+    if (tmp.bytes() < p.bytes()) throw std::invalid_argument("cg_step_block: tmp must hold one block");
+    if (!sigma.empty() && (int)sigma.size() != p.nrhs())
+        throw std::invalid_argument("cg_step_block: one shift per right-hand side");
+    const Links l = links_for(gf, p.precision());
+    p.get_system().check(cl2_MdagM_eo_mrhs_cg_step(
+        p.get_system().ctx(), r->dev(), p.dev(), l.ptr, l.fmt, ap.getMass(), A.get_upper_left() ? 1 : 0,
+        sigma.empty() ? nullptr : sigma.data(), tmp.ptr(), p.nrhs(), r2.dev(), active.dev(), pAp.dev(), alpha.dev(),
+        norms.dev()));
+    return true;
+}
+
 hmc_float Fermionmatrix_stagg_eo::getThresholdForMinimumEigenvalue(hmc_float) const
 {
     throw std::logic_error("Threshold for minimum eigenvalue not existing or not implemented!");

@@ -940,7 +940,12 @@ int cg_reliable_update_stagg_block(const std::vector<const Staggeredfield_eo*>& 
             [&](const Staggeredfield_eo_block* out, const Staggeredfield_eo_block& in, const DeviceBuffer& tmp) {
                 fermionmatrix::apply_block(A, out, gf, in, ap, sg, tmp);
             },
-            BlockStepT<StaggBlockOps>(), opt, &bst, true);
+            [&](const Staggeredfield_eo_block* r, const Staggeredfield_eo_block& p, const DeviceBuffer& tmp,
+                const Vector<hmc_float>& r2, const Vector<hmc_float>& active, const Vector<hmc_float>& pAp,
+                const Vector<hmc_float>& alpha, const Vector<hmc_float>& norms) {
+                return fermionmatrix::cg_step_block(A, r, gf, p, ap, sg, tmp, r2, active, pAp, alpha, norms);
+            },
+            opt, &bst, true);
         std::vector<int> next;
         for (int j = 0; j < m; ++j) {
             SolveStats& s = st[todo[j]];

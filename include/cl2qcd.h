@@ -316,6 +316,17 @@ int cl2_block_set_zero_eo(cl2_ctx* ctx, void* out, int prec, int nrhs);
 int cl2_D_KS_eo_mrhs(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, int evenodd, int nrhs);
 int cl2_MdagM_eo_mrhs(cl2_ctx* ctx, void* out, const void* in, const void* gauge, int format, double mass,
                       int upper_left, const double* sigma, void* tmp, int nrhs);
+/* The operator half of a lockstep CG step for MdagM_eo + sigma_r on nrhs right-hand sides, as
+ * cl2_fermionmatrix_eo_mrhs_cg_step is for QplusQminus_eo; no counterpart in the reference.  The first D_KS_eo
+ * kernel (w = D p into tmp) also sums |w_r|^2 and |p_r|^2, so pAp[r] (complex, 2 doubles) = (m^2 + sigma[r])
+ * |p_r|^2 + |w_r|^2 (D_KS_eo is anti-hermitian) and alpha[r] = r2[r] / pAp[r] where active[r] != 0 (else 0) are
+ * formed before the second kernel, whose epilogue updates r_r <- r_r - alpha[r] A p_r and sums norms[r] =
+ * (|r_new|^2, Re <r_new, r_old>); A p is never stored.  Replaces, for the block solver, cl2_MdagM_eo_mrhs,
+ * cl2_block_dot_stagg_eo, cl2_block_cg_alpha and the residual cl2_block_axpy_stagg_eo (values agree to rounding).
+ * sigma: host array of nrhs shifts or null; r2, active, pAp, alpha, norms: device arrays; tmp: one block. */
+int cl2_MdagM_eo_mrhs_cg_step(cl2_ctx* ctx, void* r, const void* p, const void* gauge, int format, double mass,
+                              int upper_left, const double* sigma, void* tmp, int nrhs, const double* r2,
+                              const double* active, double* pAp, double* alpha, double* norms);
 int cl2_block_insert_stagg_eo(cl2_ctx* ctx, void* block, const void* single, int prec, int nrhs, int r);
 int cl2_block_extract_stagg_eo(cl2_ctx* ctx, void* single, const void* block, int prec, int nrhs, int r);
 int cl2_block_axpy_stagg_eo(cl2_ctx* ctx, void* out, const void* x, const void* y, int prec, int nrhs,

@@ -155,3 +155,62 @@ def test_stagg_block_solver(inner):
         H.cg_reliable_update_stagg(one, A, gf, s, b, l, 1e-10, l.staggered_parameters(), prec)
         oh = one.get_host()
         assert np.sqrt(((oh - xh) ** 2).sum() / (oh ** 2).sum()) < 1e-6
+
+
+@pytest.mark.parametrize("nrhs", [2, 8, 16])
+@pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12])
+def test_mdagm_mrhs_cg_step_matches_unfused_sequence(fmt, nrhs):
+    """cl2_MdagM_eo_mrhs_cg_step against the sequence it replaces (cl2_MdagM_eo_mrhs, cl2_block_dot_stagg_eo,
+    cl2_block_cg_alpha, the residual cl2_block_axpy_stagg_eo with norms), with a shift per right-hand side,
+    boundary phases and an imaginary chemical potential: pAp from (m^2 + sigma) |p|^2 + |D p|^2 equals <p, A p>
+    to rounding (D_KS_eo anti-hermitian), and so do alpha, the updated residual and its norms; inactive
+    right-hand sides get alpha 0 and keep their residual."""
+    kw = dict(theta_s=0.3, theta_t=1.0, cp_im=0.2)
+    lat = O.lattice(8, 4, **kw)
+    prec = abi.F32 if fmt == abi.GAUGE_F32_12 else abi.F64
+    tol = 1e-12 if prec == abi.F64 else 2e-5
+    mass = 0.05
+    with abi.Context(8, 4, **kw) as ctx:
+        g = random_su3_field(4 * O.volume(lat), 67)
+        g64 = ctx.gauge(g)
+        gf = g64 if fmt == abi.GAUGE_F64_18 else g64.converted(fmt)
+        rng = np.random.default_rng(100 + nrhs)
+        P = make_block(ctx, [rng.standard_normal((ctx.veo, 6)) for _ in range(nrhs)], prec)
+        R0 = [rng.standard_normal((ctx.veo, 6)) for _ in range(nrhs)]
+        Ra, Rb = make_block(ctx, R0, prec), make_block(ctx, R0, prec)
+        V, tmp = ctx.buffer(P.nbytes), ctx.buffer(2 * P.nbytes)
+        sigma = np.sort(rng.uniform(1e-3, 1.0, nrhs))
+        sig = sigma.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        r2v = rng.uniform(0.5, 2.0, nrhs)
+        act = np.ones(nrhs)
+        act[nrhs - 1] = 0.
+        sc = ctx.scalars(2 * nrhs + 2 * nrhs + 2 * nrhs + nrhs + nrhs)
+        sd = ctx.scalars(2 * nrhs + 2 * nrhs + 2 * nrhs + nrhs + nrhs)
+        for s in (sc, sd):
+            s.store(np.concatenate([np.zeros(6 * nrhs), r2v, act]))
+        pap, alpha, norms, r2, active = (lambda s: (s.at(0), s.at(2 * nrhs), s.at(4 * nrhs), s.at(6 * nrhs),
+                                                    s.at(7 * nrhs)))(sc)
+        ctx.MdagM_eo_mrhs_cg_step(Ra.ptr, P.ptr, gf.ptr, fmt, mass, abi.EVEN, sig, tmp.ptr, nrhs, r2, active, pap,
+                                  alpha, norms)
+        pap2, alpha2, norms2, r22, active2 = (lambda s: (s.at(0), s.at(2 * nrhs), s.at(4 * nrhs), s.at(6 * nrhs),
+                                                         s.at(7 * nrhs)))(sd)
+        ctx.MdagM_eo_mrhs(V.ptr, P.ptr, gf.ptr, fmt, mass, abi.EVEN, sig, tmp.ptr, nrhs)
+        ctx.block_dot_stagg_eo(pap2, P.ptr, V.ptr, prec, nrhs)
+        ctx.block_cg_alpha(alpha2, r22, pap2, active2, nrhs)
+        ctx.block_axpy_stagg_eo(Rb.ptr, V.ptr, Rb.ptr, prec, nrhs, alpha2, -1.0, norms2)
+        a, b = sc.get(), sd.get()
+        for k in range(nrhs):
+            assert abs(a[2 * k] - b[2 * k]) <= tol * abs(b[2 * k]), ("pAp", k)
+            assert a[2 * k + 1] == 0.0
+            assert abs(a[2 * nrhs + 2 * k] - b[2 * nrhs + 2 * k]) <= tol * abs(b[2 * nrhs + 2 * k]) + 1e-300, ("alpha", k)
+            assert abs(a[4 * nrhs + 2 * k] - b[4 * nrhs + 2 * k]) <= tol * abs(b[4 * nrhs + 2 * k]), ("|r|^2", k)
+            assert abs(a[4 * nrhs + 2 * k + 1] - b[4 * nrhs + 2 * k + 1]) <= tol * abs(b[4 * nrhs + 2 * k]), ("Re<>", k)
+            ra, rb = slot(ctx, Ra, prec, nrhs, k), slot(ctx, Rb, prec, nrhs, k)
+            assert site_rel_err(ra, rb) <= (1e-12 if prec == abi.F64 else 1e-5), ("r", k)
+        # the inactive right-hand side: alpha 0, residual unchanged
+        assert a[2 * nrhs + 2 * (nrhs - 1)] == 0.0
+        assert np.array_equal(slot(ctx, Ra, prec, nrhs, nrhs - 1), slot(ctx, Rb, prec, nrhs, nrhs - 1))
+        # and pAp against the FP64 oracle's <p, (m^2 + sigma - D D) p> for the first right-hand side
+        p0 = slot(ctx, P, prec, nrhs, 0)
+        ref = O.dot(p0, O.mdagm_eo(lat, p0, g, mass) + sigma[0] * p0).real
+        assert abs(a[0] - ref) <= (1e-12 if prec == abi.F64 else 1e-5) * ref
