@@ -1803,10 +1803,17 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     TempDev in, out;
     CL2_HIP(c, hipMalloc(&in.p, fb));
     CL2_HIP(c, hipMalloc(&out.p, fb));
-    if (prec == CL2_F16)
-        CL2_HIP(c, launch_fill_h(c, in.p, 0.1, 0.));
-    else
-        CL2_HIP(c, launch_fill(c, in.p, (size_t)12 * c->g.veo * nrhs, prec, 0.1, 0.));
+    // Gaussian spinors (FP16: rounded from FP32 ones), as a solver's fields: a constant field runs the FP64
+    // kernel several per cent faster than random data and would bias the choice
+    if (prec == CL2_F16) {
+        TempDev g32;
+        CL2_HIP(c, hipMalloc(&g32.p, spinor_bytes(c, CL2_F32)));
+        CL2_HIP(c, launch_gaussian(c, g32.p, CL2_F32, (size_t)12 * c->g.veo, 0x5eed, true));
+        CL2_HIP(c, launch_convert_h(c, in.p, CL2_F16, g32.p, CL2_F32));
+        CL2_HIP(c, hipStreamSynchronize(c->stream));
+    } else {
+        CL2_HIP(c, launch_gaussian(c, in.p, prec, (size_t)12 * c->g.veo * nrhs, 0x5eed, nrhs == 1));
+    }
     std::vector<int> zps = {-1, 0};
     for (uint32_t d = 1; d <= c->g.ns; ++d)
         if (c->g.ns % d == 0 && (size_t)d * c->g.ns * c->g.ns / 2 * nrhs <= 262144) zps.push_back((int)d);
