@@ -1621,6 +1621,27 @@ int cl2_cgm_beta_step(cl2_ctx* c, double* beta, double* beta_prev, const double*
     return CL2_OK;
 }
 
+int cl2_MdagM_eo_cgm_step(cl2_ctx* c, void* r, const void* p, const void* gauge, int fmt, double mass, int upper_left,
+                          void* tmp, const double* rr_old, double* pap, double* beta, double* beta_prev,
+                          double* rr_new)
+{
+    if (!c) return CL2_E_ARG;
+    CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
+    CL2_REQUIRE(c, upper_left == CL2_EVEN || upper_left == CL2_ODD, "upper_left must be CL2_EVEN or CL2_ODD");
+    CL2_REQUIRE(c, r && p && gauge && tmp && rr_old && pap && beta && beta_prev && rr_new, "null pointer");
+    CL2_REQUIRE(c, r != p && tmp != p && tmp != r, "r, p and tmp must be different fields");
+    const int first = (upper_left == CL2_EVEN) ? CL2_ODD : CL2_EVEN;
+    const double m2 = mass * mass;
+    unsigned nb     = 0;
+    double* wp      = c->dscratch + 2;  // (|D p|^2, |p|^2)
+    CL2_HIP(c, launch_dks_step(c, tmp, p, gauge, fmt, first, p, nullptr, 0., nullptr, nullptr, &nb));
+    CL2_HIP(c, launch_reduce_stage2(c, nb, wp, 1, nullptr, 0));
+    CL2_HIP(c, launch_cgm_beta_step_mdagm(c, beta, beta_prev, rr_old, pap, wp, m2));
+    CL2_HIP(c, launch_dks_step(c, nullptr, tmp, gauge, fmt, upper_left, nullptr, p, m2, r, beta, &nb));
+    CL2_HIP(c, launch_reduce_stage2(c, nb, rr_new, 0, nullptr, 0));
+    return CL2_OK;
+}
+
 int cl2_cgm_alpha_step(cl2_ctx* c, int numeq, double* alpha, double* alpha_prev, const double* rr_new,
                        double* rr_old, const double* beta, const double* beta_prev, const double* sigma, double* zeta,
                        double* zeta_prev, double* zeta_foll, double* beta_vec, double* alpha_vec)

@@ -576,6 +576,25 @@ __global__ __launch_bounds__(64) void k_cgm_alpha_step(int numeq, double* alpha,
     }
 }
 
+/* the beta step from the fused MdagM step's sums: pap = m2 |p|^2 + |D p|^2 from wp = (|D p|^2, |p|^2) */
+__global__ void k_cgm_beta_step_mdagm(double* beta, double* beta_prev, const double* rr, double* pap, const double* wp,
+                                      double m2)
+{
+    if (threadIdx.x != 0) return;
+    const double a = m2 * wp[1] + wp[0];
+    pap[0]         = a;
+    const double b = beta[0];
+    beta_prev[0]   = b;
+    beta[0]        = 0.0 - rr[0] / a;
+}
+
+hipError_t launch_cgm_beta_step_mdagm(cl2_ctx* c, double* beta, double* beta_prev, const double* rr, double* pap,
+                                      const double* wp, double m2)
+{
+    hipLaunchKernelGGL(k_cgm_beta_step_mdagm, dim3(1), dim3(64), 0, c->stream, beta, beta_prev, rr, pap, wp, m2);
+    return hipGetLastError();
+}
+
 hipError_t launch_cgm_beta_step(cl2_ctx* c, double* beta, double* beta_prev, const double* rr, const double* pap)
 {
     hipLaunchKernelGGL(k_cgm_beta_step, dim3(1), dim3(64), 0, c->stream, beta, beta_prev, rr, pap);

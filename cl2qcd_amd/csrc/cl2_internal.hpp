@@ -84,6 +84,9 @@ hipError_t launch_sitediag(cl2_ctx* c, void* out, const void* in, int prec, cons
                            double scale);
 hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd);
 /* nrhs = 2, 4, 8, 12, 16 staggered vectors [3][Veo][nrhs]; with x: out_r = coef[r] x_r - D in_r */
+hipError_t launch_dks_step(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd,
+                           const void* own, const void* x, double coef, void* cg_r, const double* beta,
+                           unsigned* nblocks);
 hipError_t launch_dks_mrhs(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd, int nrhs,
                            const void* x, const double* coef, double2* norm_partials = nullptr,
                            const void* own = nullptr, void* cg_r = nullptr, const double* cg_alpha = nullptr,
@@ -186,6 +189,8 @@ hipError_t launch_vec_element(cl2_ctx* c, double* out, const double* in, int out
 hipError_t launch_cgm_update(cl2_ctx* c, int which, double* out, const double* a, const double* b, const double* d,
                              const double* e, const double* f, const double* g, int numeq);
 hipError_t launch_cgm_beta_step(cl2_ctx* c, double* beta, double* beta_prev, const double* rr, const double* pap);
+hipError_t launch_cgm_beta_step_mdagm(cl2_ctx* c, double* beta, double* beta_prev, const double* rr, double* pap,
+                                      const double* wp, double m2);
 hipError_t launch_cgm_alpha_step(cl2_ctx* c, int numeq, double* alpha, double* alpha_prev, const double* rr_new,
                                  double* rr_old, const double* beta, const double* beta_prev, const double* sigma,
                                  double* zeta, double* zeta_prev, double* zeta_foll, double* beta_vec,

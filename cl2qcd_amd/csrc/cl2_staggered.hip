@@ -328,6 +328,109 @@ hipError_t launch_dks_mrhs(cl2_ctx* c, void* out, const void* in, const void* ga
     }
 }
 
+/* The two D_KS_eo kernels of a fused CG-M step on MdagM_eo (single field, the hop order of k_dks_eo): without
+ * cg_r, out = D in is stored and each block sums (|out|^2, |own|^2), own read at the thread's eo index (every
+ * site of that field once); with cg_r, v = coef[0] x - D in is formed in registers, r <- r + beta v (beta a
+ * device scalar) is stored and each block sums (|r_new|^2, 0).  One double2 per block into norm_partials. */
+template<int FMT, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_dks_eo_step(DksArgs a, const double* beta)
+{
+    using R = typename GaugeLoad<FMT>::R;
+    using V = vec2<R>;
+    __shared__ double lds[2 * BLOCK / 64];
+    const uint32_t S = a.g.veo;
+    uint32_t i       = 0;
+    double nrm = 0., ry = 0.;
+    if (map_site<BLOCK>(a.map, S, blockIdx.x, threadIdx.x, i)) {
+        uint32_t x, y, z, t;
+        site_coords(a.g, i, a.out_parity, x, y, z, t);
+        SiteNb nb;
+        site_neighbours(a.g, i, a.out_parity, nb);
+        const V* in = reinterpret_cast<const V*>(a.in);
+        const int P = a.out_parity;
+        Cx<R> acc[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) acc[k] = Cx<R>{R(0), R(0)};
+        const R eta[4] = {R(((x + y + z) & 1u) ? -1 : 1), R(1), R((x & 1u) ? -1 : 1), R(((x + y) & 1u) ? -1 : 1)};
+#pragma unroll
+        for (int mu = 0; mu < 4; ++mu) {
+            Su3<R> uf, ub;
+            V vf[3], vb[3];
+            GaugeLoad<FMT>::load(a.gauge, S, mu, P, i, uf);
+            GaugeLoad<FMT>::load(a.gauge, S, mu, P ^ 1, nb.dn[mu], ub);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vf[c] = in[(size_t)c * S + nb.up[mu]];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vb[c] = in[(size_t)c * S + nb.dn[mu]];
+            dks_hops(acc, uf, vf, ub, vb, Cx<R>{R(a.sf[mu][0]) * eta[mu], R(a.sf[mu][1]) * eta[mu]});
+        }
+        if (a.cg_r) {
+            const V* xv      = reinterpret_cast<const V*>(a.x);
+            V* rp            = reinterpret_cast<V*>(a.cg_r);
+            const Cx<R> cf   = Cx<R>{R(a.coef[0]), R(0)};
+            const Cx<R> mone = Cx<R>{R(-1), R(0)};
+            const Cx<R> b    = Cx<R>{R(beta[0]), R(0)};
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const V xs     = xv[(size_t)c * S + i];
+                const Cx<R> v  = cmul(Cx<R>{xs.x, xs.y}, cf) + cmul(acc[c], mone);
+                const V ro     = rp[(size_t)c * S + i];
+                const Cx<R> rn = cmul(b, v) + Cx<R>{ro.x, ro.y};
+                store_stream(rp + (size_t)c * S + i, V{rn.re, rn.im});
+                nrm += (double)rn.re * (double)rn.re + (double)rn.im * (double)rn.im;
+            }
+        } else {
+            V* out = reinterpret_cast<V*>(a.out);
+            const V* ov = reinterpret_cast<const V*>(a.own);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                store_stream(out + (size_t)c * S + i, V{acc[c].re, acc[c].im});
+                nrm += (double)acc[c].re * (double)acc[c].re + (double)acc[c].im * (double)acc[c].im;
+                const V o = ov[(size_t)c * S + i];
+                ry += (double)o.x * (double)o.x + (double)o.y * (double)o.y;
+            }
+        }
+    }
+    const double2 s2 = block_sum2<BLOCK>(make_double2(nrm, ry), lds);
+    if (threadIdx.x == 0) a.norm_partials[blockIdx.x] = s2;
+}
+
+hipError_t launch_dks_step(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd,
+                           const void* own, const void* x, double coef, void* cg_r, const double* beta,
+                           unsigned* nblocks)
+{
+    constexpr int B = 256;
+    DksArgs a;
+    a.in            = in;
+    a.out           = out;
+    a.gauge         = gauge;
+    a.g             = c->g;
+    a.out_parity    = evenodd;
+    a.x             = x;
+    a.coef[0]       = coef;
+    a.own           = own;
+    a.cg_r          = cg_r;
+    a.norm_partials = reinterpret_cast<double2*>(c->partials);
+    const cl2_hop_tuning t = hop_tuning(c, format | DKS_TUNING_KEY, 1);
+    a.map                  = work_map(c, B, 1, t);
+    dks_weights(c, a);
+    const unsigned nb = blocks_for(c->g.veo, B);
+    *nblocks          = nb;
+    switch (format) {
+        case GAUGE_F64_18:
+            hipLaunchKernelGGL((k_dks_eo_step<GAUGE_F64_18, B>), dim3(nb), dim3(B), 0, c->stream, a, beta);
+            break;
+        case GAUGE_F64_12:
+            hipLaunchKernelGGL((k_dks_eo_step<GAUGE_F64_12, B>), dim3(nb), dim3(B), 0, c->stream, a, beta);
+            break;
+        case GAUGE_F32_12:
+            hipLaunchKernelGGL((k_dks_eo_step<GAUGE_F32_12, B>), dim3(nb), dim3(B), 0, c->stream, a, beta);
+            break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_dks(cl2_ctx* c, void* out, const void* in, const void* gauge, int format, int evenodd)
 {
     DksArgs a;

@@ -1146,12 +1146,17 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
     // ratio, subtraction and zeta/beta/alpha updates run as two fused launches with the same values
     // (beta_prev = beta, beta = -tmp1/tmp3; then alpha_prev = alpha, alpha = tmp2/tmp1, the per-shift
     // recurrences, zeta_prev = zeta, zeta = zeta_foll, tmp1 = tmp2)
-    auto body = [&]() {
-        A(&v, gf, p, &ap);
-        scalar_product_real_part(&tmp3, p, v);
-        system.check(cl2_cgm_beta_step(c, beta_scalar.dev(), beta_scalar_prev.dev(), tmp1.dev(), tmp3.dev()));
-        saxpy(&r, beta_scalar, v, r);  // r = beta v + r
-        scalar_product_real_part(&tmp2, r, r);
+    const auto* mdagm = dynamic_cast<const fermionmatrix::MdagM_eo*>(&A);
+    auto body         = [&]() {
+        if (mdagm != nullptr) {  // the same from two D_KS_eo kernels: A p never stored, no reduction passes
+            fermionmatrix::cgm_step(*mdagm, &r, gf, p, ap, v, tmp1, tmp3, beta_scalar, beta_scalar_prev, tmp2);
+        } else {
+            A(&v, gf, p, &ap);
+            scalar_product_real_part(&tmp3, p, v);
+            system.check(cl2_cgm_beta_step(c, beta_scalar.dev(), beta_scalar_prev.dev(), tmp1.dev(), tmp3.dev()));
+            saxpy(&r, beta_scalar, v, r);  // r = beta v + r
+            scalar_product_real_part(&tmp2, r, r);
+        }
         system.check(cl2_cgm_alpha_step(c, n, alpha_scalar.dev(), alpha_scalar_prev.dev(), tmp2.dev(), tmp1.dev(),
                                         beta_scalar.dev(), beta_scalar_prev.dev(), shift.dev(), zeta.dev(),
                                         zeta_prev.dev(), zeta_foll.dev(), beta_vec.dev(), alpha_vec.dev()));
@@ -1209,7 +1214,6 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
     ro.max_iterations  = opt.max_iterations;
     ro.use_graph       = opt.use_graph;
     int total          = iter;
-    const auto* mdagm  = dynamic_cast<const fermionmatrix::MdagM_eo*>(&A);
     if (mdagm != nullptr) {
         std::vector<const Staggeredfield_eo*> xs;
         for (const auto& xi : x) xs.push_back(xi.get());
