@@ -168,7 +168,7 @@ SIGNATURES = {
     "cl2_update_beta_cgm": (_i, [_vp, _dp, _dp, _dp, _dp, _i]),
     "cl2_update_alpha_cgm": (_i, [_vp, _dp, _dp, _dp, _dp, _dp, _dp, _i]),
     "cl2_cgm_beta_step": (_i, [_vp, _dp, _dp, _dp, _dp]),
-    "cl2_MdagM_eo_cgm_step": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, _vp, _dp, _dp, _dp, _dp, _dp]),
+    "cl2_MdagM_eo_cgm_step": (_i, [_vp, _vp, _vp, _vp, _i, _d, _d, _i, _vp, _dp, _dp, _dp, _dp, _dp]),
     "cl2_MdagM_eo_mrhs_cg_step": (_i, [_vp, _vp, _vp, _vp, _i, _d, _i, ctypes.POINTER(_d), _vp, _i, _dp, _dp, _dp,
                                        _dp, _dp]),
     "cl2_cgm_alpha_step": (_i, [_vp, _i, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),

@@ -1621,9 +1621,9 @@ int cl2_cgm_beta_step(cl2_ctx* c, double* beta, double* beta_prev, const double*
     return CL2_OK;
 }
 
-int cl2_MdagM_eo_cgm_step(cl2_ctx* c, void* r, const void* p, const void* gauge, int fmt, double mass, int upper_left,
-                          void* tmp, const double* rr_old, double* pap, double* beta, double* beta_prev,
-                          double* rr_new)
+int cl2_MdagM_eo_cgm_step(cl2_ctx* c, void* r, const void* p, const void* gauge, int fmt, double mass, double sigma,
+                          int upper_left, void* tmp, const double* rr_old, double* pap, double* beta,
+                          double* beta_prev, double* rr_new)
 {
     if (!c) return CL2_E_ARG;
     CL2_REQUIRE(c, valid_fmt(fmt), "unknown gauge format");
@@ -1631,7 +1631,7 @@ int cl2_MdagM_eo_cgm_step(cl2_ctx* c, void* r, const void* p, const void* gauge,
     CL2_REQUIRE(c, r && p && gauge && tmp && rr_old && pap && beta && beta_prev && rr_new, "null pointer");
     CL2_REQUIRE(c, r != p && tmp != p && tmp != r, "r, p and tmp must be different fields");
     const int first = (upper_left == CL2_EVEN) ? CL2_ODD : CL2_EVEN;
-    const double m2 = mass * mass;
+    const double m2 = mass * mass + sigma;
     unsigned nb     = 0;
     double* wp      = c->dscratch + 2;  // (|D p|^2, |p|^2)
     CL2_HIP(c, launch_dks_step(c, tmp, p, gauge, fmt, first, p, nullptr, 0., nullptr, nullptr, &nb));

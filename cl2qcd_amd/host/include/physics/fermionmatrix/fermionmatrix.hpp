@@ -201,11 +201,11 @@ class MdagM_eo final : public Fermionmatrix_stagg_eo {
 void apply_block(const MdagM_eo& A, const lattices::Staggeredfield_eo_block* out, const lattices::Gaugefield& gf,
                  const lattices::Staggeredfield_eo_block& in, const physics::AdditionalParameters& ap,
                  const std::vector<hmc_float>& sigma, const lattices::DeviceBuffer& tmp);
-/* The operator half of a CG-M iteration on A (cl2_MdagM_eo_cgm_step): beta_prev = beta, beta = -rr_old / <p, A p>,
- * r += beta A p (A p never stored; tmp is a field of p's precision) and rr_new = |r|^2, all device scalars.  No
- * counterpart in the reference. */
+/* The operator half of a CG-M iteration on A + sigma (cl2_MdagM_eo_cgm_step): beta_prev = beta,
+ * beta = -rr_old / <p, (A + sigma) p>, r += beta (A + sigma) p (never stored; tmp is a field of p's precision) and
+ * rr_new = |r|^2, all device scalars.  No counterpart in the reference. */
 void cgm_step(const MdagM_eo& A, const lattices::Staggeredfield_eo* r, const lattices::Gaugefield& gf,
-              const lattices::Staggeredfield_eo& p, const physics::AdditionalParameters& ap,
+              const lattices::Staggeredfield_eo& p, const physics::AdditionalParameters& ap, hmc_float sigma,
               const lattices::Staggeredfield_eo& tmp, const lattices::Scalar<hmc_float>& rr_old,
               const lattices::Scalar<hmc_float>& pap, const lattices::Scalar<hmc_float>& beta,
               const lattices::Scalar<hmc_float>& beta_prev, const lattices::Scalar<hmc_float>& rr_new);

@@ -321,7 +321,7 @@ void apply_block(const MdagM_eo& A, const lattices::Staggeredfield_eo_block* out
 }
 
 void cgm_step(const MdagM_eo& A, const lattices::Staggeredfield_eo* r, const Gaugefield& gf,
-              const lattices::Staggeredfield_eo& p, const physics::AdditionalParameters& ap,
+              const lattices::Staggeredfield_eo& p, const physics::AdditionalParameters& ap, hmc_float sigma,
               const lattices::Staggeredfield_eo& tmp, const lattices::Scalar<hmc_float>& rr_old,
               const lattices::Scalar<hmc_float>& pap, const lattices::Scalar<hmc_float>& beta,
               const lattices::Scalar<hmc_float>& beta_prev, const lattices::Scalar<hmc_float>& rr_new)
@@ -330,7 +330,7 @@ void cgm_step(const MdagM_eo& A, const lattices::Staggeredfield_eo* r, const Gau
         throw std::invalid_argument("cgm_step: fields of different precision");
     const Links l = links_for(gf, p.precision());
     p.get_system().check(cl2_MdagM_eo_cgm_step(p.get_system().ctx(), r->dev(), p.dev(), l.ptr, l.fmt, ap.getMass(),
-                                               A.get_upper_left() ? 1 : 0, tmp.dev(), rr_old.dev(), pap.dev(),
+                                               sigma, A.get_upper_left() ? 1 : 0, tmp.dev(), rr_old.dev(), pap.dev(),
                                                beta.dev(), beta_prev.dev(), rr_new.dev()));
 }
 

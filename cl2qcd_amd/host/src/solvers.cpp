@@ -1000,14 +1000,20 @@ int cg_reliable_update_stagg(const Staggeredfield_eo* x, const fermionmatrix::Fe
     x_lo.set_zero();
     r2.store(rn);
     apply(&v, p);  // makes the reduced-precision link copy and MdagM temporary before anything is recorded
-    // one inner iteration with every scalar on the device (beta = |r_new|^2 / |r_old|^2)
-    auto body = [&]() {
-        apply(&v, p);
-        scalar_product_real_part(&pAp, p, v);
-        divide(&alpha, r2, pAp);
-        subtract(&malpha, zero, alpha);
-        saxpy(&r_lo, malpha, v, r_lo);  // r_lo -= alpha v
-        squarenorm(&rn_s, r_lo);
+    // one inner iteration with every scalar on the device (beta = |r_new|^2 / |r_old|^2); on MdagM_eo the
+    // operator half is the fused CG-M step (malpha = -r2 / pAp, r_lo += malpha (A + sigma) p, rn_s = |r_lo|^2)
+    const auto* mdagm = dynamic_cast<const fermionmatrix::MdagM_eo*>(&A);
+    auto body         = [&]() {
+        if (mdagm != nullptr) {
+            fermionmatrix::cgm_step(*mdagm, &r_lo, gf, p, ap, sigma, v, r2, pAp, malpha, alpha, rn_s);
+        } else {
+            apply(&v, p);
+            scalar_product_real_part(&pAp, p, v);
+            divide(&alpha, r2, pAp);
+            subtract(&malpha, zero, alpha);
+            saxpy(&r_lo, malpha, v, r_lo);  // r_lo -= alpha v
+            squarenorm(&rn_s, r_lo);
+        }
         divide(&beta, rn_s, r2);
         subtract(&mbeta, zero, beta);
         add(&r2, rn_s, zero);                       // r2 = |r_new|^2
@@ -1149,7 +1155,7 @@ int cg_m_mixed(const std::vector<std::shared_ptr<Staggeredfield_eo>>& x, const f
     const auto* mdagm = dynamic_cast<const fermionmatrix::MdagM_eo*>(&A);
     auto body         = [&]() {
         if (mdagm != nullptr) {  // the same from two D_KS_eo kernels: A p never stored, no reduction passes
-            fermionmatrix::cgm_step(*mdagm, &r, gf, p, ap, v, tmp1, tmp3, beta_scalar, beta_scalar_prev, tmp2);
+            fermionmatrix::cgm_step(*mdagm, &r, gf, p, ap, 0., v, tmp1, tmp3, beta_scalar, beta_scalar_prev, tmp2);
         } else {
             A(&v, gf, p, &ap);
             scalar_product_real_part(&tmp3, p, v);
