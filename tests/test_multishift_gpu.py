@@ -183,9 +183,10 @@ def test_fused_cgm_scalar_steps_equal_reference_sequence(n):
             assert np.abs(got[row] - want).max() <= 1e-13 * np.abs(want).max()
 
 
+@pytest.mark.parametrize("upper_left", [abi.EVEN, abi.ODD])
 @pytest.mark.parametrize("sigma", [0.0, 0.3])
 @pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12])
-def test_mdagm_cgm_step_matches_unfused_sequence(fmt, sigma):
+def test_mdagm_cgm_step_matches_unfused_sequence(fmt, sigma, upper_left):
     """cl2_MdagM_eo_cgm_step against the sequence it replaces in a CG-M iteration (solver_shifted.cpp:136-198):
     MdagM_eo (+ sigma, as the reduced-precision CG on a shifted system uses it), <p, A p>, cl2_cgm_beta_step,
     r = beta A p + r and |r|^2, with boundary phases and an imaginary chemical potential; pAp =
@@ -210,9 +211,9 @@ def test_mdagm_cgm_step_matches_unfused_sequence(fmt, sigma):
         sa, sb = ctx.scalars(5), ctx.scalars(5)
         for s in (sa, sb):
             s.store([1.7, 0.0, -0.4, 0.0, 0.0])
-        ctx.MdagM_eo_cgm_step(ra.ptr, p.ptr, gf.ptr, fmt, mass, sigma, abi.EVEN, t1.ptr, sa.at(0), sa.at(1),
+        ctx.MdagM_eo_cgm_step(ra.ptr, p.ptr, gf.ptr, fmt, mass, sigma, upper_left, t1.ptr, sa.at(0), sa.at(1),
                               sa.at(2), sa.at(3), sa.at(4))
-        ctx.MdagM_eo(v.ptr, p.ptr, gf.ptr, fmt, mass, abi.EVEN, t2.ptr)
+        ctx.MdagM_eo(v.ptr, p.ptr, gf.ptr, fmt, mass, upper_left, t2.ptr)
         if sigma:
             ctx.saxpy_stagg_eo(v.ptr, p.ptr, v.ptr, prec, abi.scalar(sigma))
         ctx.scalar_product_real_part_stagg_eo(sb.at(1), p.ptr, v.ptr, prec)
@@ -226,5 +227,5 @@ def test_mdagm_cgm_step_matches_unfused_sequence(fmt, sigma):
         assert site_rel_err(ra.download(), rb.download()) <= (1e-12 if prec == abi.F64 else 1e-5)
         # pap against the oracle's <p, MdagM p> on the stored p
         ps = p.download()
-        ref = O.dot(ps, O.mdagm_eo(lat, ps, g, mass) + sigma * ps).real
+        ref = O.dot(ps, O.mdagm_eo(lat, ps, g, mass, upper_left) + sigma * ps).real
         assert abs(a[1] - ref) <= (1e-12 if prec == abi.F64 else 1e-5) * ref

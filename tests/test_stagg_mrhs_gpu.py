@@ -157,9 +157,10 @@ def test_stagg_block_solver(inner):
         assert np.sqrt(((oh - xh) ** 2).sum() / (oh ** 2).sum()) < 1e-6
 
 
+@pytest.mark.parametrize("upper_left", [abi.EVEN, abi.ODD])
 @pytest.mark.parametrize("nrhs", [2, 8, 16])
 @pytest.mark.parametrize("fmt", [abi.GAUGE_F64_18, abi.GAUGE_F32_12])
-def test_mdagm_mrhs_cg_step_matches_unfused_sequence(fmt, nrhs):
+def test_mdagm_mrhs_cg_step_matches_unfused_sequence(fmt, nrhs, upper_left):
     """cl2_MdagM_eo_mrhs_cg_step against the sequence it replaces (cl2_MdagM_eo_mrhs, cl2_block_dot_stagg_eo,
     cl2_block_cg_alpha, the residual cl2_block_axpy_stagg_eo with norms), with a shift per right-hand side,
     boundary phases and an imaginary chemical potential: pAp from (m^2 + sigma) |p|^2 + |D p|^2 equals <p, A p>
@@ -190,11 +191,11 @@ def test_mdagm_mrhs_cg_step_matches_unfused_sequence(fmt, nrhs):
             s.store(np.concatenate([np.zeros(6 * nrhs), r2v, act]))
         pap, alpha, norms, r2, active = (lambda s: (s.at(0), s.at(2 * nrhs), s.at(4 * nrhs), s.at(6 * nrhs),
                                                     s.at(7 * nrhs)))(sc)
-        ctx.MdagM_eo_mrhs_cg_step(Ra.ptr, P.ptr, gf.ptr, fmt, mass, abi.EVEN, sig, tmp.ptr, nrhs, r2, active, pap,
+        ctx.MdagM_eo_mrhs_cg_step(Ra.ptr, P.ptr, gf.ptr, fmt, mass, upper_left, sig, tmp.ptr, nrhs, r2, active, pap,
                                   alpha, norms)
         pap2, alpha2, norms2, r22, active2 = (lambda s: (s.at(0), s.at(2 * nrhs), s.at(4 * nrhs), s.at(6 * nrhs),
                                                          s.at(7 * nrhs)))(sd)
-        ctx.MdagM_eo_mrhs(V.ptr, P.ptr, gf.ptr, fmt, mass, abi.EVEN, sig, tmp.ptr, nrhs)
+        ctx.MdagM_eo_mrhs(V.ptr, P.ptr, gf.ptr, fmt, mass, upper_left, sig, tmp.ptr, nrhs)
         ctx.block_dot_stagg_eo(pap2, P.ptr, V.ptr, prec, nrhs)
         ctx.block_cg_alpha(alpha2, r22, pap2, active2, nrhs)
         ctx.block_axpy_stagg_eo(Rb.ptr, V.ptr, Rb.ptr, prec, nrhs, alpha2, -1.0, norms2)
@@ -212,5 +213,5 @@ def test_mdagm_mrhs_cg_step_matches_unfused_sequence(fmt, nrhs):
         assert np.array_equal(slot(ctx, Ra, prec, nrhs, nrhs - 1), slot(ctx, Rb, prec, nrhs, nrhs - 1))
         # and pAp against the FP64 oracle's <p, (m^2 + sigma - D D) p> for the first right-hand side
         p0 = slot(ctx, P, prec, nrhs, 0)
-        ref = O.dot(p0, O.mdagm_eo(lat, p0, g, mass) + sigma[0] * p0).real
+        ref = O.dot(p0, O.mdagm_eo(lat, p0, g, mass, upper_left) + sigma[0] * p0).real
         assert abs(a[0] - ref) <= (1e-12 if prec == abi.F64 else 1e-5) * ref
