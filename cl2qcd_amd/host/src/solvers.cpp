@@ -673,13 +673,26 @@ int block_solve(const std::vector<const typename Ops::Field*>& x, const std::vec
         pending                         = true;
         const std::vector<hmc_float> nr = norms.get();
         bool update                     = false;
+        // a right-hand side whose iterated residual has reached its target waits, still iterating in lockstep,
+        // for the next update to verify it in FP64: that update comes when another one has dropped by delta,
+        // when every active one has reached its target, or (a shrinking block) when half of them have.  A
+        // right-hand side verified one at a time would cost one FP64 update each as the sources converge a few
+        // iterations apart.  With hooks (serving) each is verified at once, so that its answer leaves early.
+        int reached = 0, open = 0;
         for (int r = 0; r < n; ++r) {
             if (act[r] == 0.) continue;
             const hmc_float q = nr[2 * r];
             if (q != q) throw SolverStuck(k, "cg_reliable_update_block: NAN occured");
-            if (std::sqrt(q) < opt.delta * rmax[r] || q <= target[r]) update = true;
+            if (q <= target[r]) {
+                ++reached;
+                if (hooks) update = true;
+            } else {
+                ++open;
+                if (std::sqrt(q) < opt.delta * rmax[r]) update = true;
+            }
             rmax[r] = std::max(rmax[r], std::sqrt(q));
         }
+        if (open == 0 || (shrink && 2 * reached >= n_active)) update = true;
         if (!update) continue;
         system.check(Ops::convert(c, T.dev(), CL2_F64, Xlo.dev(), lo, n));
         system.check(Ops::axpy(c, X.dev(), T.dev(), X.dev(), CL2_F64, n, ones.dev(), 1., nullptr));
