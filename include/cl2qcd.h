@@ -414,6 +414,9 @@ int cl2_update_alpha_cgm(cl2_ctx* ctx, double* out, const double* salpha_pres, c
  * alpha step, after |r|^2:   alpha_prev = alpha; alpha = rr_new / rr_old; zeta_foll, beta_vec, alpha_vec as
  *   update_{zeta,beta,alpha}_cgm; then zeta_prev = zeta, zeta = zeta_foll, rr_old = rr_new. */
 int cl2_cgm_beta_step(cl2_ctx* ctx, double* beta, double* beta_prev, const double* rr, const double* pap);
+int cl2_cgm_alpha_step(cl2_ctx* ctx, int numeq, double* alpha, double* alpha_prev, const double* rr_new,
+                       double* rr_old, const double* beta, const double* beta_prev, const double* sigma, double* zeta,
+                       double* zeta_prev, double* zeta_foll, double* beta_vec, double* alpha_vec);
 /* The operator half of a CG-M iteration on A = MdagM_eo + sigma (solver_shifted.cpp:136-198, from A p to |r|^2;
  * sigma 0 there) in two D_KS_eo kernels; no counterpart in the reference.  w = D_KS_eo p into tmp (one field of
  * p's precision) with (|w|^2, |p|^2) summed; pap = (m^2 + sigma) |p|^2 + |w|^2 (= <p, A p>, D_KS_eo
@@ -424,9 +427,6 @@ int cl2_cgm_beta_step(cl2_ctx* ctx, double* beta, double* beta_prev, const doubl
 int cl2_MdagM_eo_cgm_step(cl2_ctx* ctx, void* r, const void* p, const void* gauge, int format, double mass,
                           double sigma, int upper_left, void* tmp, const double* rr_old, double* pap, double* beta,
                           double* beta_prev, double* rr_new);
-int cl2_cgm_alpha_step(cl2_ctx* ctx, int numeq, double* alpha, double* alpha_prev, const double* rr_new,
-                       double* rr_old, const double* beta, const double* beta_prev, const double* sigma, double* zeta,
-                       double* zeta_prev, double* zeta_foll, double* beta_vec, double* alpha_vec);
 
 /* ==== captured launch sequences ====
  * No counterpart in the reference.  Everything enqueued on the context's stream between cl2_graph_begin and
