@@ -787,24 +787,37 @@ __global__ __launch_bounds__(256) void k_block_axpy_h(void* out, const void* x, 
     }
 }
 
-/* the x and p updates of a lockstep CG step: x_r = x_r + alpha_r p_r, then p_r = r_r + beta_r p_r */
+/* the x and p updates of a lockstep CG step: x_r = x_r + alpha_r p_r, then p_r = r_r + beta_r p_r; U (site,
+ * right-hand side) elements per thread, a block size apart, with all their loads issued first (two per thread
+ * measured 7 % faster than one) */
+template<int U>
 __global__ __launch_bounds__(256) void k_block_xp_h(void* xo, void* p, const void* rv, uint32_t SN, int nrhs,
                                                     const double* alpha, const double* beta)
 {
-    const int r = (int)(threadIdx.x % (unsigned)nrhs);
+    const int r = (int)(threadIdx.x % (unsigned)nrhs);  // the block size is a multiple of nrhs
     const Cx<float> ca{float(alpha[2 * r]), float(alpha[2 * r + 1])}, cb{float(beta[2 * r]), float(beta[2 * r + 1])};
-    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < SN; e += gridDim.x * blockDim.x) {
-        Cx<float> xv[12], pv[12], rr[12];
-        load_site_h<true>(xo, SN, e, xv);
-        load_site_h<true>(p, SN, e, pv);
-        load_site_h<true>(rv, SN, e, rr);
+    const uint32_t base = blockIdx.x * blockDim.x * U + threadIdx.x;
+    Cx<float> xv[U][12], pv[U][12], rr[U][12];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t e = base + u * blockDim.x;
+        if (e < SN) {
+            load_site_h<true>(xo, SN, e, xv[u]);
+            load_site_h<true>(p, SN, e, pv[u]);
+            load_site_h<true>(rv, SN, e, rr[u]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        const uint32_t e = base + u * blockDim.x;
+        if (e >= SN) continue;
 #pragma unroll
         for (int k = 0; k < 12; ++k) {
-            xv[k] = xv[k] + cmul(ca, pv[k]);
-            pv[k] = rr[k] + cmul(cb, pv[k]);
+            xv[u][k] = xv[u][k] + cmul(ca, pv[u][k]);
+            pv[u][k] = rr[u][k] + cmul(cb, pv[u][k]);
         }
-        store_site_h<true>(xo, SN, e, xv);
-        store_site_h<true>(p, SN, e, pv);
+        store_site_h<true>(xo, SN, e, xv[u]);
+        store_site_h<true>(p, SN, e, pv[u]);
     }
 }
 
@@ -881,10 +894,10 @@ hipError_t launch_block_axpy_h(cl2_ctx* c, void* out, const void* x, const void*
 hipError_t launch_block_xp_h(cl2_ctx* c, void* x, void* p, const void* r, int nrhs, const double* alpha,
                              const double* beta)
 {
-    // no partials: one (site, right-hand side) per thread over the full grid (2 % faster than the capped grid)
+    // no partials: two (site, right-hand side) elements per thread over the full grid
     const uint32_t SN = c->g.veo * (uint32_t)nrhs;
-    const unsigned bt = 256u - (256u % (unsigned)nrhs), nb = (SN + bt - 1) / bt;
-    hipLaunchKernelGGL(k_block_xp_h, dim3(nb), dim3(bt), 0, c->stream, x, p, r, SN, nrhs, alpha, beta);
+    const unsigned bt = 256u - (256u % (unsigned)nrhs), nb = (SN + 2 * bt - 1) / (2 * bt);
+    hipLaunchKernelGGL(k_block_xp_h<2>, dim3(nb), dim3(bt), 0, c->stream, x, p, r, SN, nrhs, alpha, beta);
     return hipGetLastError();
 }
 
